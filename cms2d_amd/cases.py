@@ -1,0 +1,141 @@
+"""Seeded synthetic inputs for the CMS-Flow implicit step workloads.
+
+Every floating-point input is rounded to float32 before use: the reference reads
+grid depths, Manning fields and cell sizes through single-precision buffers
+(source/input/in_lib.f90:86, source/geospatial/geo_def.f90:118), so an input
+that is not float32-representable would differ between the reference and any
+FP64 build of it.
+
+Workloads (all on Cartesian grids, cells numbered row-major from the south-west
+corner, as source/geospatial/geospatial.F90:2709-2828 numbers them):
+
+* ``basin``   -- 200 x 100 cells of 10 m, linear offshore slope 1 m -> 10 m
+  towards the east plus seeded Gaussian bumps (sigma 0.2 m), M2 harmonic water
+  level on the east side (0.5 m, 12.42 h), walls elsewhere, Manning 0.025,
+  dt = 60 s, flow only.
+* ``tidal``   -- 2000 x 1500 cells of 20 m (3.0 M cells), beach at the west
+  (bed from +1 m to -15 m), wetting and drying with hdry = 0.05 m, lognormal
+  Manning (median 0.025, sigma_log 0.2), smooth seeded radiation-stress field
+  up to 0.05 N/m^2, M2 tide on the east side, dt = 30 s.
+* ``square``  -- 1000 x 1000 cells of 20 m, the same physics as ``tidal``
+  without the radiation stress.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+RHOW = 1025.0
+M2_SPEED_DEG_HR = 360.0 / 12.42        # harmonic speed of a 12.42 h period [deg/hr]
+
+
+def f32(a):
+    """Round to float32 and return float64."""
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    zb: np.ndarray            # bed elevation [ny*nx] (row-major, j*nx+i), m (negative = below datum)
+    mann: np.ndarray          # Manning n [ny*nx]
+    wavestrx: np.ndarray | None
+    wavestry: np.ndarray | None
+    th_side: int              # 0=N,1=E,2=S,3=W, -1 none
+    amp: list
+    speed_deg_hr: list
+    phase_deg: list
+    dt: float
+    nsteps: int
+    rampdur: float = 0.0      # hours
+    hdry: float = 0.05
+    noptset: int = 2          # >=3: wave branches (radiation-stress forcing) active
+    wallfric: int = 1
+    ndsch: int = 4            # exponential scheme
+
+    @property
+    def ncells(self) -> int:
+        return self.nx * self.ny
+
+    @property
+    def speed_rad_hr(self):
+        return [s * math.pi / 180.0 for s in self.speed_deg_hr]
+
+    @property
+    def phase_rad(self):
+        return [p * math.pi / 180.0 for p in self.phase_deg]
+
+
+def _bumps(rng, nx, ny, nbumps, sigma_amp, rmin, rmax):
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    z = np.zeros((ny, nx))
+    for _ in range(nbumps):
+        ci, cj = rng.uniform(0, nx), rng.uniform(0, ny)
+        r = rng.uniform(rmin, rmax)
+        a = rng.normal(0.0, sigma_amp)
+        z += a * np.exp(-((ii - ci) ** 2 + (jj - cj) ** 2) / (2 * r * r))
+    return z
+
+
+def _smooth_field(rng, nx, ny, nmodes, peak):
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    x, y = ii / nx, jj / ny
+    f = np.zeros((ny, nx))
+    for _ in range(nmodes):
+        kx, ky = rng.integers(1, 4), rng.integers(1, 4)
+        ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+        f += rng.normal() * np.sin(np.pi * kx * x + ph1) * np.sin(np.pi * ky * y + ph2)
+    return f * (peak / max(np.abs(f).max(), 1e-30))
+
+
+def basin(nx=200, ny=100, dx=10.0, seed=0, nsteps=1440, dt=60.0) -> Case:
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    depth = 1.0 + 9.0 * x[None, :] * np.ones((ny, 1))
+    depth = depth + _bumps(rng, nx, ny, 12, 0.2, max(2, nx / 40), max(3, nx / 12))
+    zb = f32(-depth).ravel()
+    mann = f32(np.full(nx * ny, 0.025))
+    return Case("basin", nx, ny, dx, dx, zb, mann, None, None, 1, [0.5], [M2_SPEED_DEG_HR], [90.0],
+                dt, nsteps)
+
+
+def tidal(nx=2000, ny=1500, dx=20.0, seed=1, nsteps=2880, dt=30.0, radstress=True) -> Case:
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    depth = -1.0 + 16.0 * x[None, :] * np.ones((ny, 1))
+    depth = depth + _bumps(rng, nx, ny, 40, 0.2, max(2, nx / 80), max(3, nx / 20))
+    zb = f32(-depth).ravel()
+    mann = f32(0.025 * np.exp(0.2 * rng.standard_normal(nx * ny)))
+    wx = wy = None
+    noptset = 2
+    if radstress:
+        tx = _smooth_field(rng, nx, ny, 6, 0.05)
+        ty = _smooth_field(rng, nx, ny, 6, 0.05)
+        mag = np.sqrt(tx * tx + ty * ty)
+        scale = np.where(mag > 0.05, 0.05 / np.maximum(mag, 1e-30), 1.0)
+        wx = f32((tx * scale / RHOW)).ravel()
+        wy = f32((ty * scale / RHOW)).ravel()
+        noptset = 3
+    return Case("tidal", nx, ny, dx, dx, zb, mann, wx, wy, 1, [0.5], [M2_SPEED_DEG_HR], [90.0],
+                dt, nsteps, noptset=noptset)
+
+
+def square(nx=1000, ny=1000, dx=20.0, seed=3, nsteps=720, dt=60.0) -> Case:
+    c = tidal(nx, ny, dx, seed=seed, nsteps=nsteps, dt=dt, radstress=False)
+    c.name = "square"
+    return c
+
+
+def lake_at_rest(nx=40, ny=30, dx=10.0, seed=5) -> Case:
+    """Closed basin with a still, flat water surface over a bumpy bed."""
+    rng = np.random.default_rng(seed)
+    depth = 3.0 + _bumps(rng, nx, ny, 6, 0.5, 2, 6)
+    zb = f32(-depth).ravel()
+    return Case("lake", nx, ny, dx, dx, zb, f32(np.full(nx * ny, 0.03)), None, None, 1,
+                [0.0], [M2_SPEED_DEG_HR], [0.0], 60.0, 10)

@@ -1,0 +1,1340 @@
+/*
+ * cms_oracle.c -- FP64 CPU restatement of CMS-Flow's implicit hydrodynamic step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path in cms2d_amd/.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Every routine cites the reference routine it restates (paths relative to the
+ * erdc/cms2d tree).  Constants that the reference writes as default-real
+ * literals (e.g. 0.8, 1.0e-6, 0.3333) keep the value of that single-precision
+ * literal, F4(x), exactly as an ikind=8 rebuild of the reference evaluates them;
+ * constants written with the _ikind suffix (grav=9.81_ikind) are exact doubles.
+ *
+ * Loops whose OpenMP form in the reference reads values that other iterations
+ * of the same loop write (the dry-to-wet initialisation, flow.F90:1477-1491;
+ * the eddy-viscosity fix-ups at dry boundaries and corners,
+ * flow_eddyvis.f90:168-212) have two modes: serial (the reference run on one
+ * thread) and race-free (every read sees the values from before the loop, the
+ * semantics of the HIP path).  cms_case.serial selects.
+ */
+#include "cms_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#define F4(x) ((double)(float)(x))
+#define KN 0
+#define KE 1
+#define KS 2
+#define KW 3
+
+static const int kopp[4] = {KS, KW, KN, KE};      /* kkface, geospatial.F90:58-61 */
+
+/* reference constants (definitions.F90:88-92, flow.F90:21-169, friction.F90:18-65) */
+static const double GRAV = 9.81;                  /* 9.81_ikind */
+#define SMALL   F4(1.0e-15)                       /* const_def small */
+#define RHOW    1025.0
+#define CVISCON F4(1.0e-6)
+#define CVISMAX 20.0
+#define CVISBOT F4(0.0667)
+#define CVISHOR F4(0.2)
+#define CVISWAV 0.5
+#define CVISWAVBRK F4(0.1)
+#define RELAX_UV F4(0.8)
+#define RELAX_PP 1.0
+#define FACPP   F4(0.2)
+#define CFRICWAV F4(0.65)
+#define HMAXFRIC 10.0
+/* check_conv thresholds, flow.F90:135-150 */
+#define RMOMMAXP    F4(1.0e-3)
+#define RMOMTARGETP F4(1.0e-4)
+#define RMOMMINP    F4(1.0e-8)
+#define RMOMRATIOP  F4(0.5)
+#define RMOMMAXUV   F4(1.0e-2)
+#define RMOMTARGETUV F4(1.0e-3)
+#define RMOMMINUV   F4(1.0e-7)
+#define RMOMRATIOUV 1.0
+
+typedef struct {
+    int ncx, ncy;
+    int icx[5], icy[5];
+    double wcx[5], wcy[5];
+} gradop;
+
+struct cms_oracle {
+    cms_case c;
+    int nx, ny, ncells, ncellsD;
+    /* connectivity / geometry (geo_def) */
+    int *c2c, *llec, *nxy, *kxy;
+    double *dx, *dy, *x, *y, *area;
+    double *ds, *dn, *fnx, *fny, *rx, *ry, *dsx, *dsy, *dsxy, *fintp;
+    double areaavg;
+    /* boundary strings (bnd_def) */
+    int nW, *Wc, *Wk;
+    int nTH, *THc, *THk;
+    double *wsebnd, *wseadj;
+    int wseadjust;
+    /* state (flow_def, fric_def, wave_flowgrid_def) */
+    double *u, *v, *p, *h, *pp, *dppx, *dppy, *dux, *duy, *dvx, *dvy, *dpx, *dpy;
+    double *vis, *zb, *eta, *u1, *v1, *p1, *h1;
+    int *iwet, *iwet1, *icorner;
+    double *flux, *flux1, *hk, *pk, *visk, *zbk, *acoef;
+    double *su, *sv, *sp, *spu, *spv, *ssu0, *ssv0, *spuv0, *sppp0, *sspp0;
+    double *apuareap, *sumu, *Hu, *Hv, *rsu, *rsv, *rsp;
+    double *coefman, *cfrict, *z0, *uv, *uelwc, *bsxy, *bsvel, *cmb, *cbcfuwcap;
+    double *wavestrx, *wavestry, *fc;
+    gradop *gow;
+    /* control (comvarbl) */
+    double dtime, timesecs, ctime, timehrs, ramp, rampdur, hmin, hdry, hdry1, hdry2;
+    int maxit, niter, iconv, ntime;
+    double rmom[3], rmom0[3], rmom1[3];
+    int kry[3];
+    double bndvol;
+    /* solver workspace */
+    double *wk;
+    int *ia, *ja, *diagpos;
+    double *lu;
+};
+
+static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!p) abort(); return p; }
+#define DALLOC(n) ((double *)xcalloc((size_t)(n), sizeof(double)))
+#define IALLOC(n) ((int *)xcalloc((size_t)(n), sizeof(int)))
+
+/* ---------------------------------------------------------------------------
+ * Closed-form helpers
+ * ------------------------------------------------------------------------- */
+
+/* comp_lib.f90:54-109 upwindcoef / hybridcoef / powerlawcoef / exponentialcoef */
+double cms_oracle_schmcoef(int ndsch, double dk, double fk)
+{
+    double absfk = fabs(fk), peck;
+    switch (ndsch) {
+    case 2:
+        peck = absfk / fmax(dk, F4(1.0e-6));
+        return dk * fmax(0.0, 1.0 - 0.5 * peck) + fmax(0.0, -fk);
+    case 3:
+        peck = absfk / fmax(fmax(dk, absfk), F4(1.0e-6));
+        return dk * (1.0 - pow(peck, 5)) + fmax(0.0, -fk);
+    case 4:
+        peck = absfk / fmax(fmax(dk, F4(1.e-6)), absfk);
+        return dk * peck / fmax(exp(peck) - 1.0, F4(1.0e-6)) + fmax(0.0, -fk);
+    default:
+        return dk + fmax(0.0, -fk);
+    }
+}
+
+/* fric_lib.f90:162-194 wall_coef */
+double cms_oracle_wall_coef(double yp, double y0)
+{
+    double y0lim = fmin(fmax(y0, F4(1.0e-6)), yp * F4(0.9999));
+    double t = F4(0.4) / log(yp / y0lim);
+    return t * t;
+}
+
+/* fric_lib.f90:944-969 fric_conv_man2drag */
+double cms_oracle_man2drag(double g, double cman, double h, double hmax)
+{
+    double hh = fmax(fmin(hmax, h), F4(0.1) * cman);
+    return g * cman * cman / pow(hh, F4(0.3333));
+}
+
+/* fric_lib.f90:972-996 fric_conv_drag2length */
+double cms_oracle_drag2length(double cd, double h)
+{
+    double z0 = h / exp(F4(0.4) / sqrt(cd) + 1.0);
+    return fmax(z0, F4(1.0e-6));
+}
+
+/* time_lib.F90:234-256 ramp_func */
+static double ramp_func(double timehrs, double rampdur)
+{
+    return 0.5 - 0.5 * cos(M_PI * fmin(timehrs, rampdur) / rampdur);
+}
+
+/* ---------------------------------------------------------------------------
+ * Grid: geo_cart (geospatial.F90:2709-2828), geo_init_cart (:1400-1611),
+ * interp_coef_cell2face (interp_lib.f90:2669-2724)
+ * ------------------------------------------------------------------------- */
+static void build_grid(cms_oracle *o)
+{
+    const int nx = o->nx, ny = o->ny, nc = nx * ny;
+    int nd = nc;
+    /* count ghost cells: every missing neighbour of an active cell gets one */
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+            nd += (j == ny - 1) + (i == nx - 1) + (j == 0) + (i == 0);
+    o->ncells = nc;
+    o->ncellsD = nd;
+    o->c2c = IALLOC(4 * nd);
+    o->llec = IALLOC(4 * nd);
+    for (int f = 0; f < 4 * nd; f++) { o->c2c[f] = -1; o->llec[f] = -1; }
+    o->dx = DALLOC(nd); o->dy = DALLOC(nd); o->x = DALLOC(nd); o->y = DALLOC(nd);
+    o->zb = DALLOC(nd);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            int c = j * nx + i;
+            o->dx[c] = o->c.dx; o->dy[c] = o->c.dy;
+            o->x[c] = (i + 1) * o->c.dx - 0.5 * o->c.dx;
+            o->y[c] = (j + 1) * o->c.dy - 0.5 * o->c.dy;
+            o->zb[c] = o->c.zb[c];
+            if (j < ny - 1) o->c2c[4 * c + KN] = c + nx;
+            if (i < nx - 1) o->c2c[4 * c + KE] = c + 1;
+            if (j > 0) o->c2c[4 * c + KS] = c - nx;
+            if (i > 0) o->c2c[4 * c + KW] = c - 1;
+        }
+    /* dummy cells, in the reference's order (cell by cell, N,E,S,W) */
+    int g = nc;
+    for (int c = 0; c < nc; c++) {
+        for (int k = 0; k < 4; k++) {
+            if (o->c2c[4 * c + k] >= 0) continue;
+            o->c2c[4 * c + k] = g;
+            o->c2c[4 * g + kopp[k]] = c;
+            o->dx[g] = o->dx[c]; o->dy[g] = o->dy[c];
+            o->x[g] = o->x[c] + (k == KE ? o->dx[c] : k == KW ? -o->dx[c] : 0.0);
+            o->y[g] = o->y[c] + (k == KN ? o->dy[c] : k == KS ? -o->dy[c] : 0.0);
+            o->zb[g] = o->zb[c];
+            g++;
+        }
+    }
+    o->area = DALLOC(nd);
+    double asum = 0.0;
+    for (int i = 0; i < nd; i++) o->area[i] = o->dx[i] * o->dy[i];
+    for (int i = 0; i < nc; i++) asum += o->area[i];
+    o->areaavg = asum / (double)(float)nc;           /* sum(areap)/real(ncells) */
+    /* backward connectivity */
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            if (nck < nc) {
+                for (int kk = 0; kk < 4; kk++)
+                    if (o->c2c[4 * nck + kk] == i) o->llec[4 * i + k] = kk;
+            } else {
+                o->llec[4 * i + k] = kopp[k];
+            }
+        }
+    int F = 4 * nd;
+    o->ds = DALLOC(F); o->dn = DALLOC(F); o->fnx = DALLOC(F); o->fny = DALLOC(F);
+    o->rx = DALLOC(F); o->ry = DALLOC(F); o->dsx = DALLOC(F); o->dsy = DALLOC(F);
+    o->dsxy = DALLOC(F); o->fintp = DALLOC(F);
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 4; k++) {
+            int f = 4 * i + k, nck = o->c2c[f];
+            if (k == KN || k == KS) o->ds[f] = fmin(o->dx[i], o->dx[nck]);
+            else o->ds[f] = fmin(o->dy[i], o->dy[nck]);
+            switch (k) {
+            case KN: o->fnx[f] = 0; o->fny[f] = 1; o->dn[f] = fabs(o->y[nck] - o->y[i]);
+                o->rx[f] = o->x[nck] - o->x[i]; o->ry[f] = o->dy[i] / 2.0; break;
+            case KE: o->fnx[f] = 1; o->fny[f] = 0; o->dn[f] = fabs(o->x[nck] - o->x[i]);
+                o->rx[f] = o->dx[i] / 2.0; o->ry[f] = o->y[nck] - o->y[i]; break;
+            case KS: o->fnx[f] = 0; o->fny[f] = -1; o->dn[f] = fabs(o->y[nck] - o->y[i]);
+                o->rx[f] = o->x[nck] - o->x[i]; o->ry[f] = -o->dy[i] / 2.0; break;
+            default: o->fnx[f] = -1; o->fny[f] = 0; o->dn[f] = fabs(o->x[nck] - o->x[i]);
+                o->rx[f] = -o->dx[i] / 2.0; o->ry[f] = o->y[nck] - o->y[i]; break;
+            }
+            o->dsx[f] = o->fnx[f] * o->ds[f];
+            o->dsy[f] = o->fny[f] * o->ds[f];
+            o->dsxy[f] = o->ds[f] / o->dn[f];
+            /* interp_coef_cell2face, simple cells */
+            if (k == KN || k == KS) o->fintp[f] = o->dy[i] / (o->dy[i] + o->dy[nck]);
+            else o->fintp[f] = o->dx[i] / (o->dx[i] + o->dx[nck]);
+        }
+    /* no-repeat face lists kxyface (geospatial.F90:1544-1568) */
+    o->nxy = IALLOC(nd);
+    o->kxy = IALLOC(4 * nd);
+    char *ind = (char *)xcalloc((size_t)F, 1);
+    memset(ind, 1, (size_t)F);
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 4; k++)
+            if (ind[4 * i + k]) {
+                int nck = o->c2c[4 * i + k];
+                ind[4 * i + k] = 0;
+                ind[4 * nck + o->llec[4 * i + k]] = 0;
+                o->kxy[4 * i + o->nxy[i]++] = k;
+            }
+    free(ind);
+}
+
+/* Boundary strings: harmonic WSE string on one side, walls elsewhere
+ * (boundaries.F90:2380-2435 wall-string construction). */
+static void build_bnd(cms_oracle *o)
+{
+    const int nx = o->nx, ny = o->ny, nc = o->ncells;
+    int side = o->c.th_side;
+    o->nTH = 0;
+    if (side == KE || side == KW) o->nTH = ny;
+    else if (side == KN || side == KS) o->nTH = nx;
+    o->THc = IALLOC(o->nTH); o->THk = IALLOC(o->nTH);
+    for (int j = 0; j < o->nTH; j++) {
+        int c;
+        if (side == KE) c = j * nx + (nx - 1);
+        else if (side == KW) c = j * nx;
+        else if (side == KN) c = (ny - 1) * nx + j;
+        else c = j;
+        o->THc[j] = c; o->THk[j] = side;
+    }
+    o->wsebnd = DALLOC(o->nTH > 0 ? o->nTH : 1);
+    o->wseadj = DALLOC(o->nTH > 0 ? o->nTH : 1);
+    o->wseadjust = o->nTH >= 3;          /* boundaries.F90:1370,1808 */
+    char *claimed = (char *)xcalloc((size_t)o->ncellsD, 1);
+    for (int j = 0; j < o->nTH; j++) claimed[o->c2c[4 * o->THc[j] + o->THk[j]]] = 1;
+    int n = 0;
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            if (nck >= nc && !claimed[nck]) n++;
+        }
+    o->nW = n; o->Wc = IALLOC(n); o->Wk = IALLOC(n);
+    n = 0;
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            if (nck >= nc && !claimed[nck]) { o->Wc[n] = i; o->Wk[n] = k; n++; }
+        }
+    free(claimed);
+}
+
+/* ---------------------------------------------------------------------------
+ * Gradient operator: der_grad_cbgg (der_lib.f90:272-378) + screen_weights
+ * (:681-715), gow%ibc=1 (wet cells only) or goa%ibc=0.
+ * ------------------------------------------------------------------------- */
+static void der_grad_cbgg(cms_oracle *o, gradop *go, int ibc)
+{
+    for (int i = 0; i < o->ncells; i++) {
+        gradop *G = &go[i];
+        if (o->iwet[i] == 0 && ibc == 1) { G->ncx = 0; G->ncy = 0; continue; }
+        double wx[5] = {0}, wy[5] = {0};
+        int ix[5], iy[5];
+        ix[0] = iy[0] = i;
+        for (int k = 0; k < 4; k++) {
+            int f = 4 * i + k, nck = o->c2c[f];
+            ix[k + 1] = iy[k + 1] = nck;
+            if (o->iwet[nck] == 1 || ibc == 0) {
+                wx[0] += o->dsx[f] * (1.0 - o->fintp[f]);
+                wx[k + 1] = o->dsx[f] * o->fintp[f];
+                wy[0] += o->dsy[f] * (1.0 - o->fintp[f]);
+                wy[k + 1] = o->dsy[f] * o->fintp[f];
+            } else {
+                wx[0] += o->dsx[f];
+                wy[0] += o->dsy[f];
+            }
+        }
+        for (int k = 0; k < 5; k++) { wx[k] /= o->area[i]; wy[k] /= o->area[i]; }
+        const double tolw = F4(1.0e-8);
+        G->ncx = G->ncy = 0;
+        for (int k = 0; k < 5; k++) {
+            if (fabs(wx[k]) > tolw) { G->icx[G->ncx] = ix[k]; G->wcx[G->ncx] = wx[k]; G->ncx++; }
+            if (fabs(wy[k]) > tolw) { G->icy[G->ncy] = iy[k]; G->wcy[G->ncy] = wy[k]; G->ncy++; }
+        }
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < G->ncx; k++) sx += G->wcx[k];
+        for (int k = 0; k < G->ncy; k++) sy += G->wcy[k];
+        for (int k = 0; k < G->ncx; k++) G->wcx[k] -= sx / (double)G->ncx;
+        for (int k = 0; k < G->ncy; k++) G->wcy[k] -= sy / (double)G->ncy;
+    }
+}
+
+/* der_grad_eval, der_lib.f90:726-839 (first order, nder=1 on Cartesian grids) */
+static void der_grad_eval(cms_oracle *o, const gradop *go, const double *var, double *gx, double *gy)
+{
+    for (int i = 0; i < o->ncells; i++) {
+        const gradop *G = &go[i];
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < G->ncx; k++) sx += G->wcx[k] * var[G->icx[k]];
+        for (int k = 0; k < G->ncy; k++) sy += G->wcy[k] * var[G->icy[k]];
+        gx[i] = sx; gy[i] = sy;
+    }
+}
+
+/* interp_scal_cell2face, interp_lib.f90:2727-2906, ibc = 0, 1, 3 (no skew terms
+ * on Cartesian grids). */
+static void interp_cell2face(cms_oracle *o, const double *phi, int ibc, double *phik)
+{
+    for (int i = 0; i < o->ncells; i++)
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k;
+            int nck = o->c2c[f], jcn = o->llec[f];
+            double fk = o->fintp[f], val;
+            int wi = o->iwet[i], wn = o->iwet[nck];
+            if (ibc == 0) {
+                val = fk * phi[nck] + (1.0 - fk) * phi[i];
+            } else if (ibc == 1) {
+                if (wi * wn == 1) val = fk * phi[nck] + (1.0 - fk) * phi[i];
+                else if (wi == 1) val = phi[i];
+                else if (wn == 1) val = phi[nck];
+                else val = 0.5 * (phi[i] + phi[nck]);
+            } else { /* 3 */
+                if (wi * wn == 1) val = fk * phi[nck] + (1.0 - fk) * phi[i];
+                else val = 0.0;
+            }
+            phik[f] = val;
+            phik[4 * nck + jcn] = val;
+        }
+}
+
+/* ---------------------------------------------------------------------------
+ * Wetting and drying: flow_wetdry(0) (flow.F90:1231-1539), flow_pond (:1807-1876)
+ * ------------------------------------------------------------------------- */
+static void flow_pond(cms_oracle *o)
+{
+    const int nc = o->ncells, nd = o->ncellsD;
+    int *ipond = IALLOC(nd);
+    for (int j = 0; j < o->nTH; j++) {
+        int i = o->THc[j], nck = o->c2c[4 * i + o->THk[j]];
+        ipond[i] = o->iwet[i];
+        ipond[nck] = o->iwet[nck];
+    }
+    int npond = 10;
+    while (npond > 0) {
+        npond = 0;
+        for (int i = 0; i < nc; i++)
+            if (ipond[i] == 1)
+                for (int k = 0; k < 4; k++) {
+                    int nck = o->c2c[4 * i + k];
+                    int d = o->iwet[nck] - ipond[nck];
+                    npond += d > 0 ? d : 0;
+                    ipond[nck] = o->iwet[nck];
+                }
+        for (int i = nc - 1; i >= 0; i--)
+            if (ipond[i] == 1)
+                for (int k = 0; k < 4; k++) {
+                    int nck = o->c2c[4 * i + k];
+                    int d = o->iwet[nck] - ipond[nck];
+                    npond += d > 0 ? d : 0;
+                    ipond[nck] = o->iwet[nck];
+                }
+    }
+    for (int i = 0; i < nd; i++)
+        if (ipond[i] == 0 && o->iwet[i] == 1) o->iwet[i] = 0;
+    free(ipond);
+}
+
+static void flow_wetdry(cms_oracle *o)
+{
+    const int nc = o->ncells, nd = o->ncellsD;
+    for (int i = 0; i < nd; i++) {
+        o->h[i] = o->p[i] * (1.0 / GRAV) - o->zb[i];   /* p*gravinv, gravinv = 1/grav */
+        if (o->h[i] < o->hdry) { o->h[i] = fmax(o->h[i], o->hmin); o->iwet[i] = 0; }
+        else o->iwet[i] = 1;
+    }
+    for (int j = 0; j < o->nW; j++) o->iwet[o->c2c[4 * o->Wc[j] + o->Wk[j]]] = 0;
+    /* narrow-channel removal (Cartesian branch, :1342-1455), sequential in place */
+    int nsw = 1;
+    while (nsw > 0) {
+        nsw = 0;
+        for (int i = 0; i < nc; i++) {
+            if (o->iwet[i] == 0) continue;
+            const int *nb = &o->c2c[4 * i];
+            int ksum = o->iwet[nb[0]] + o->iwet[nb[1]] + o->iwet[nb[2]] + o->iwet[nb[3]];
+            if (ksum <= 1) { o->iwet[i] = 0; nsw++; }
+            int kns = o->iwet[nb[KN]] + o->iwet[nb[KS]];
+            int kew = o->iwet[nb[KE]] + o->iwet[nb[KW]];
+            if ((kns == 0 || kew == 0) && o->h[i] < o->hdry1) { o->iwet[i] = 0; nsw++; }
+            /* second neighbour in a direction exists only through an active cell */
+#define NB2(c, k) ((c) < nc ? o->iwet[o->c2c[4 * (c) + (k)]] : 0)
+            int knss = o->iwet[nb[KN]] + NB2(nb[KS], KS);
+            if (knss == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
+            /* the reference tests knss a second time here (flow.F90:1403) */
+            if (knss == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
+            int keww = o->iwet[nb[KE]] + NB2(nb[KW], KW);
+            if (keww == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
+            int keew = o->iwet[nb[KW]] + NB2(nb[KE], KE);
+            if (keew == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
+#undef NB2
+        }
+    }
+    flow_pond(o);
+    for (int i = 0; i < nc; i++) {
+        o->icorner[i] = 0;
+        if (o->iwet[i] == 1) {
+            int nds = 0;
+            for (int k = 0; k < 4; k++) nds += o->iwet[o->c2c[4 * i + k]] == 0;
+            if (nds > 1) o->icorner[i] = 1;
+        }
+    }
+    /* dry-to-wet initialisation, 5 sweeps (:1477-1491) */
+    double *tu = NULL, *tv = NULL, *tvis = NULL;
+    if (!o->c.serial) { tu = DALLOC(nd); tv = DALLOC(nd); tvis = DALLOC(nd); }
+    for (int sweep = 0; sweep < 5; sweep++) {
+        const double *ru = o->u, *rv = o->v, *rvis = o->vis;
+        if (!o->c.serial) {
+            memcpy(tu, o->u, sizeof(double) * nd); memcpy(tv, o->v, sizeof(double) * nd);
+            memcpy(tvis, o->vis, sizeof(double) * nd);
+            ru = tu; rv = tv; rvis = tvis;
+        }
+        for (int i = 0; i < nc; i++)
+            if (o->iwet[i] == 1 && o->iwet1[i] == 0) {
+                const int *nb = &o->c2c[4 * i];
+                double us = 0, vs = 0, viss = 0; int ns = 0;
+                for (int k = 0; k < 4; k++) { us += ru[nb[k]]; vs += rv[nb[k]]; viss += rvis[nb[k]]; ns += o->iwet[nb[k]]; }
+                o->u[i] = us / ((double)(float)ns + SMALL) / F4(1.25);
+                o->v[i] = vs / ((double)(float)ns + SMALL) / F4(1.25);
+                o->vis[i] = viss / ((double)(float)ns + SMALL);
+            }
+    }
+    free(tu); free(tv); free(tvis);
+    /* zero fluxes and viscosities on faces that are not wet on both sides */
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f], jcn = o->llec[f];
+            if (o->iwet[i] * o->iwet[nck] == 0) {
+                o->flux[f] = 0; o->visk[f] = 0;
+                o->flux[4 * nck + jcn] = 0; o->visk[4 * nck + jcn] = 0;
+            }
+        }
+}
+
+/* ---------------------------------------------------------------------------
+ * Boundary values: bnd_eval harmonic branch (boundaries.F90:4098-4124)
+ * ------------------------------------------------------------------------- */
+static void bnd_eval(cms_oracle *o)
+{
+    for (int j = 0; j < o->nTH; j++) {
+        double w = 0.0;
+        for (int k = 0; k < o->c.ntc; k++)
+            w += o->c.amp[k] * cos(o->c.speed[k] * o->timehrs - o->c.phase[k]);
+        o->wsebnd[j] = (1.0 - o->ramp) * 0.0 + o->ramp * (w + 0.0);
+    }
+}
+
+/* flow_srcsnk_static, flow_imp.F90:316-406 (ntsch=1) */
+static void flow_srcsnk_static(cms_oracle *o)
+{
+    double gdi = 1.0 / (GRAV * o->dtime), dti = 1.0 / o->dtime;
+    for (int i = 0; i < o->ncells; i++) {
+        double fac1 = o->area[i] * o->h1[i] * dti;
+        o->spuv0[i] = -fac1;
+        o->ssu0[i] = fac1 * o->u1[i];
+        o->ssv0[i] = fac1 * o->v1[i];
+        o->sppp0[i] = -o->area[i] * gdi;
+        o->sspp0[i] = -o->p1[i] * o->sppp0[i];
+    }
+    if (o->c.noptset >= 3)
+        for (int i = 0; i < o->ncells; i++) {
+            o->ssu0[i] += o->wavestrx[i] * o->area[i];
+            o->ssv0[i] += o->wavestry[i] * o->area[i];
+        }
+}
+
+/* coeffsourcesink_uv, flow_imp.F90:409-574 */
+static void coeffsourcesink_uv(cms_oracle *o)
+{
+    for (int i = 0; i < o->ncells; i++) {
+        o->su[i] = o->ssu0[i] + (o->fc[i] * o->v[i] - o->dpx[i]) * o->h[i] * o->area[i];
+        o->sv[i] = o->ssv0[i] + (-o->fc[i] * o->u[i] - o->dpy[i]) * o->h[i] * o->area[i];
+        o->cbcfuwcap[i] = o->cmb[i] * o->cfrict[i] * o->uelwc[i] * o->area[i];
+        o->sp[i] = o->spuv0[i] - o->cbcfuwcap[i];
+        for (int k = 0; k < 4; k++) {
+            int f = 4 * i + k;
+            double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
+            o->acoef[f] = cms_oracle_schmcoef(o->c.ndsch, ddk, o->flux[f]);
+        }
+    }
+}
+
+/* bound_uv, boundaries.F90:3310-3490, with bnd_vel_openwse (bnd_vel.f90:35-95) */
+static void bound_uv(cms_oracle *o)
+{
+    for (int j = 0; j < o->nTH; j++) {
+        int i = o->THc[j], k = o->THk[j], f = 4 * i + k, nck = o->c2c[f];
+        double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
+        o->acoef[f] = cms_oracle_schmcoef(1, ddk, o->flux[f]);
+        if (fabs(o->fny[f]) > F4(0.9999)) {
+            o->u[nck] = o->iwet[i] * o->u[i] * o->h[i] / o->h[nck];
+            o->v[nck] = o->fny[f] * o->flux[f] / (o->ds[f] * o->hk[f]);
+            o->sv[i] += o->acoef[f] * o->v[nck];
+        } else {
+            o->u[nck] = o->fnx[f] * o->flux[f] / (o->ds[f] * o->hk[f]);
+            o->v[nck] = o->iwet[i] * o->v[i] * o->h[i] / o->h[nck];
+            o->su[i] += o->acoef[f] * o->u[nck];
+        }
+        o->sp[i] -= o->acoef[f];
+        o->acoef[f] = 0.0;
+    }
+    if (o->c.wallfric) {
+        for (int i = 0; i < o->ncells; i++) {
+            if (o->iwet[i] == 1) {
+                o->spu[i] = o->sp[i];
+                o->spv[i] = o->sp[i];
+                for (int k = 0; k < 4; k++) {
+                    int f = 4 * i + k, nck = o->c2c[f];
+                    if (o->iwet[nck] == 0) {
+                        double z0w = o->z0[i] * 1.0;   /* wallfac = 1.0 */
+                        if (fabs(o->fny[f]) > F4(0.999)) {
+                            double delta = fabs(o->ry[f]);
+                            o->spu[i] -= o->ds[f] * o->hk[f] * cms_oracle_wall_coef(delta, z0w) * fabs(o->u[i]);
+                        } else {
+                            double delta = fabs(o->rx[f]);
+                            o->spv[i] -= o->ds[f] * o->hk[f] * cms_oracle_wall_coef(delta, z0w) * fabs(o->v[i]);
+                        }
+                    }
+                }
+            } else {
+                o->u[i] = 0.0; o->v[i] = 0.0;
+                o->spu[i] = -1.0; o->spv[i] = -1.0; o->sp[i] = -1.0;
+                o->su[i] = 0.0; o->sv[i] = 0.0;
+            }
+        }
+    } else {
+        for (int i = 0; i < o->ncells; i++) {
+            int w = o->iwet[i];
+            o->u[i] = w * o->u[i];
+            o->v[i] = w * o->v[i];
+            o->sp[i] = w * o->sp[i] + (double)(w - 1);
+            o->spu[i] = o->sp[i];
+            o->spv[i] = o->spu[i];
+            o->su[i] = w * o->su[i];
+            o->sv[i] = w * o->sv[i];
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Linear solver: solve (solvers.f90:205-304) + right-preconditioned BiCGStab
+ * (pbicgstab.f90:2-98).  The preconditioner is ILU(0) on the natural ordering;
+ * the reference factors with ILUTP (ilutp.f90), which only changes the
+ * iterates, not the converged solution.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int n, nd;
+    const double *acoef;
+    const int *c2c;
+    const double *ap;
+    int *ia, *ja, *dpos;
+    double *lu;
+} sysmat;
+
+static void ilu0_build(sysmat *S)
+{
+    int n = S->n, nz = 0;
+    for (int i = 0; i < n; i++) {
+        S->ia[i] = nz;
+        int cols[5]; double vals[5]; int m = 0;
+        cols[m] = i; vals[m] = S->ap[i]; m++;
+        for (int k = 0; k < 4; k++) {
+            int nck = S->c2c[4 * i + k];
+            if (nck >= 0 && nck < n) { cols[m] = nck; vals[m] = -S->acoef[4 * i + k]; m++; }
+        }
+        for (int a = 1; a < m; a++) {          /* insertion sort by column */
+            int cc = cols[a]; double vv = vals[a]; int b = a - 1;
+            while (b >= 0 && cols[b] > cc) { cols[b + 1] = cols[b]; vals[b + 1] = vals[b]; b--; }
+            cols[b + 1] = cc; vals[b + 1] = vv;
+        }
+        for (int a = 0; a < m; a++) {
+            S->ja[nz] = cols[a]; S->lu[nz] = vals[a];
+            if (cols[a] == i) S->dpos[i] = nz;
+            nz++;
+        }
+    }
+    S->ia[n] = nz;
+    /* IKJ ILU(0) */
+    for (int i = 0; i < n; i++) {
+        for (int pk = S->ia[i]; pk < S->dpos[i]; pk++) {
+            int kc = S->ja[pk];
+            double lik = S->lu[pk] / S->lu[S->dpos[kc]];
+            S->lu[pk] = lik;
+            int pj = S->dpos[kc] + 1, pi = pk + 1;
+            while (pj < S->ia[kc + 1] && pi < S->ia[i + 1]) {
+                if (S->ja[pj] == S->ja[pi]) { S->lu[pi] -= lik * S->lu[pj]; pj++; pi++; }
+                else if (S->ja[pj] < S->ja[pi]) pj++;
+                else pi++;
+            }
+        }
+    }
+}
+
+static void ilu0_apply(const sysmat *S, const double *r, double *z)
+{
+    int n = S->n;
+    for (int i = 0; i < n; i++) {
+        double s = r[i];
+        for (int p = S->ia[i]; p < S->dpos[i]; p++) s -= S->lu[p] * z[S->ja[p]];
+        z[i] = s;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = z[i];
+        for (int p = S->dpos[i] + 1; p < S->ia[i + 1]; p++) s -= S->lu[p] * z[S->ja[p]];
+        z[i] = s / S->lu[S->dpos[i]];
+    }
+}
+
+/* y = A z with z = 0 on ghost cells (pbicgstab.f90:55,76) */
+static void spmv(const sysmat *S, const double *z, double *y)
+{
+    for (int i = 0; i < S->n; i++) {
+        double s = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int nck = S->c2c[4 * i + k];
+            if (nck >= 0 && nck < S->n) s += S->acoef[4 * i + k] * z[nck];
+        }
+        y[i] = S->ap[i] * z[i] - s;
+    }
+}
+
+/* Returns Krylov iterations. mode 0: tight; mode 1: reference stopping rule
+ * (L1 ratio < 1e-3 or nmax iterations). */
+static int bicgstab(sysmat *S, const double *su, double *phi, int mode, int nmax,
+                    double rtol, double atol, double *wk)
+{
+    int n = S->n;
+    double *pk = wk, *r = wk + n, *uk = wk + 2 * n, *vk = wk + 3 * n, *r0 = wk + 4 * n, *zz = wk + 5 * n;
+    double res0 = 0.0, n20 = 0.0;
+    for (int i = 0; i < n; i++) {
+        double term = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int nck = S->c2c[4 * i + k];
+            if (nck >= 0) term += S->acoef[4 * i + k] * phi[nck];
+        }
+        term = term + su[i] - S->ap[i] * phi[i];
+        r[i] = term + SMALL;
+        r0[i] = r[i];
+        res0 += fabs(r0[i]);
+        n20 += r0[i] * r0[i];
+        pk[i] = uk[i] = vk[i] = 0.0;
+    }
+    double thr = fmax(rtol * sqrt(n20), atol * sqrt((double)n));
+    if (mode == 0 && sqrt(n20) <= thr) return 0;
+    ilu0_build(S);
+    double alf = 1.0, bet0 = 1.0, gam = 1.0, nr0 = sqrt(n20), nr = nr0;
+    int it;
+    for (it = 1; it <= nmax; it++) {
+        double bet = 0.0;
+        for (int i = 0; i < n; i++) bet += r[i] * r0[i];
+        double omg;
+        if (mode == 1) {
+            omg = bet * gam / (alf * bet0 + SMALL);        /* pbicgstab.f90:43 */
+        } else if (fabs(bet) <= 1e-12 * nr * nr0 || it == 1) {
+            /* start, or rho breakdown: restart with the shadow residual r0 = r */
+            if (it > 1) {
+                for (int i = 0; i < n; i++) r0[i] = r[i];
+                bet = nr * nr; nr0 = nr;
+            }
+            omg = 0.0;
+        } else {
+            omg = (bet / bet0) * (gam / alf);
+        }
+        bet0 = bet;
+        for (int i = 0; i < n; i++) pk[i] = r[i] + omg * (pk[i] - alf * uk[i]);
+        ilu0_apply(S, pk, zz);
+        spmv(S, zz, uk);
+        double ukr0 = 0.0;
+        for (int i = 0; i < n; i++) ukr0 += uk[i] * r0[i];
+        if (ukr0 == 0.0) break;
+        gam = bet / ukr0;
+        for (int i = 0; i < n; i++) { phi[i] += gam * zz[i]; r[i] -= gam * uk[i]; }
+        ilu0_apply(S, r, zz);
+        spmv(S, zz, vk);
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < n; i++) { s1 += vk[i] * r[i]; s2 += vk[i] * vk[i]; }
+        alf = mode == 1 ? s1 / fmax(s2, SMALL) : (s2 > 0.0 ? s1 / s2 : 0.0);
+        double res = 0.0, n2 = 0.0;
+        for (int i = 0; i < n; i++) {
+            phi[i] += alf * zz[i];
+            r[i] -= alf * vk[i];
+            res += fabs(r[i]);
+            n2 += r[i] * r[i];
+        }
+        nr = sqrt(n2);
+        if (mode == 0 && alf == 0.0) break;
+        if (mode == 1) { if (res / res0 < F4(0.001)) break; }
+        else if (sqrt(n2) <= thr) break;
+    }
+    return it > nmax ? nmax : it;
+}
+
+/* solve(acoef,ss,sp,res,phi,n), solvers.f90:205-304 */
+static int solve_sys(cms_oracle *o, double *ss, const double *sp, double *res, double *phi,
+                     int n, double *rmom_out)
+{
+    const int nc = o->ncells;
+    double relax = (n == 1) ? RELAX_PP : RELAX_UV;
+    double rinv = 1.0 / relax, relaxm = 1.0 - relax;
+    double *ap = o->wk + 6 * nc;
+    double adak = 0.0;
+    for (int i = 0; i < nc; i++) {
+        double asum = 0.0, anb = 0.0;
+        for (int k = 0; k < 4; k++) {
+            asum += o->acoef[4 * i + k];
+            anb += o->acoef[4 * i + k] * phi[o->c2c[4 * i + k]];
+        }
+        ap[i] = (asum - sp[i]) * rinv;
+        ss[i] = ss[i] + relaxm * phi[i] * ap[i];
+        res[i] = o->iwet[i] * (anb + ss[i] - ap[i] * phi[i]) / ap[i];
+        adak += res[i] * res[i];
+    }
+    *rmom_out = sqrt(adak / (double)(float)nc);
+    sysmat S = {nc, o->ncellsD, o->acoef, o->c2c, ap, o->ia, o->ja, o->diagpos, o->lu};
+    int nswp = (n == 1) ? 15 : 5;                       /* flow.F90:779-783 */
+    int nmax = o->c.solver_mode == 1 ? nswp : o->c.kmax;
+    return bicgstab(&S, ss, phi, o->c.solver_mode, nmax, o->c.rtol, o->c.atol, o->wk);
+}
+
+/* ---------------------------------------------------------------------------
+ * Pressure correction: coefsourcesink_pp (flow_imp.F90:577-794) with
+ * flow_imp_flux_noskew (flow_imp_flux.f90:361-409), bound_pp (boundaries.F90:
+ * 3493-3646), bnd_wse_adjust (:4620-4686), bnd_pp (:4767-4795)
+ * ------------------------------------------------------------------------- */
+static void coefsourcesink_pp(cms_oracle *o)
+{
+    const int nc = o->ncells;
+    for (int i = 0; i < nc; i++) {
+        double asum = 0.0;
+        for (int k = 0; k < 4; k++) asum += o->acoef[4 * i + k];
+        double apu = o->iwet[i] * RELAX_UV / fmax(asum - o->sp[i], F4(1.0e-6));
+        o->apuareap[i] = apu * o->area[i];
+        o->sumu[i] = asum * apu;
+        o->Hu[i] = o->iwet[i] * (o->u[i] + o->apuareap[i] * o->dpx[i] * o->h[i]);
+        o->Hv[i] = o->iwet[i] * (o->v[i] + o->apuareap[i] * o->dpy[i] * o->h[i]);
+    }
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            double a;
+            if (o->iwet[i] * o->iwet[nck] == 0) a = 0.0;
+            else if (nck >= nc)
+                a = o->hk[f] * o->hk[f] * o->ds[f] * o->apuareap[i] / (1.0 - o->sumu[i]) / o->dn[f];
+            else {
+                double fk = o->fintp[f], fp = 1.0 - fk;
+                double apf = fk * o->apuareap[nck] + fp * o->apuareap[i];
+                double suf = fk * o->sumu[nck] + fp * o->sumu[i];
+                a = o->hk[f] * o->hk[f] * o->ds[f] * apf / (1.0 - suf) / o->dn[f];
+            }
+            o->acoef[f] = a;
+            o->acoef[4 * nck + o->llec[f]] = a;
+        }
+    /* flow_imp_flux_noskew */
+    for (int i = 0; i < nc; i++)
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            double fl;
+            if (o->iwet[i] * o->iwet[nck] == 0) fl = 0.0;
+            else if (nck >= nc)
+                fl = o->hk[f] * o->ds[f] * ((o->fnx[f] * o->Hu[i] + o->fny[f] * o->Hv[i])
+                     - o->apuareap[i] * o->hk[f] * (o->p[nck] - o->p[i]) / o->dn[f]);
+            else {
+                double fk = o->fintp[f], fp = 1.0 - fk;
+                double apf = fk * o->apuareap[nck] + fp * o->apuareap[i];
+                double Huf = fk * o->Hu[nck] + fp * o->Hu[i];
+                double Hvf = fk * o->Hv[nck] + fp * o->Hv[i];
+                fl = o->hk[f] * o->ds[f] * ((o->fnx[f] * Huf + o->fny[f] * Hvf)
+                     - apf * o->hk[f] * (o->p[nck] - o->p[i]) / o->dn[f]);
+            }
+            o->flux[f] = fl;
+            o->flux[4 * nck + o->llec[f]] = -fl;
+        }
+    for (int i = 0; i < nc; i++) {
+        o->sp[i] = o->sppp0[i];
+        double fs = 0.0;
+        for (int k = 0; k < 4; k++) fs += o->flux[4 * i + k];
+        o->su[i] = o->sspp0[i] - fs + o->p[i] * o->sp[i];
+    }
+}
+
+static void bound_pp(cms_oracle *o)
+{
+    const int nc = o->ncells;
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] != 0) continue;
+        int sumwet = 0;
+        for (int k = 0; k < 4; k++) { int nck = o->c2c[4 * i + k]; if (nck < nc) sumwet += o->iwet[nck]; }
+        double aploc = 0.0;
+        o->su[i] = 0.0;
+        if (sumwet >= 1) {
+            for (int k = 0; k < 4; k++) {
+                int f = 4 * i + k, nck = o->c2c[f];
+                o->acoef[f] = o->iwet[nck];
+                aploc += o->acoef[f];
+                o->su[i] += o->acoef[f] * o->p[nck];
+            }
+        } else {
+            for (int k = 0; k < 4; k++) {
+                int f = 4 * i + k, nck = o->c2c[f];
+                if (nck >= nc) o->acoef[f] = 0.0;
+                else { o->acoef[f] = o->dsxy[f]; aploc += o->acoef[f]; o->su[i] += o->acoef[f] * o->p[nck]; }
+            }
+        }
+        o->su[i] -= aploc * o->p[i];
+        o->sp[i] = 0.0;
+    }
+    if (o->nTH > 0) {
+        const double *wse = o->wsebnd;
+        if (o->wseadjust) {
+            /* bnd_wse_adjust: setup integrated from the centre cell outward */
+            int n = o->nTH, im0 = n / 2 - 1;           /* 1-based nbndcells/2 */
+            double *ws = DALLOC(n);
+            ws[im0] = 0.0;
+            for (int im = im0 + 1; im < n; im++) {
+                int i = o->THc[im], i1 = o->THc[im - 1], f = 4 * i + o->THk[im];
+                double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
+                double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
+                double sx = (o->x[i] - o->x[i1]) * (fx + fx1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
+                double sy = (o->y[i] - o->y[i1]) * (fy + fy1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
+                if (fabs(o->ry[f]) >= F4(0.999)) ws[im] = ws[im - 1] + sx;
+                else if (fabs(o->rx[f]) >= F4(0.999)) ws[im] = ws[im - 1] + sy;
+                else ws[im] = ws[im - 1] + sx + sy;
+            }
+            for (int im = im0 - 1; im >= 0; im--) {
+                int i = o->THc[im], i1 = o->THc[im + 1], f = 4 * i + o->THk[im];
+                double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
+                double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
+                double sx = (o->x[i] - o->x[i1]) * (fx + fx1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
+                double sy = (o->y[i] - o->y[i1]) * (fy + fy1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
+                if (fabs(o->ry[f]) >= F4(0.999)) ws[im] = ws[im + 1] + sx;
+                else if (fabs(o->rx[f]) >= F4(0.999)) ws[im] = ws[im + 1] + sy;
+                else ws[im] = ws[im + 1] + sx + sy;
+            }
+            for (int im = 0; im < n; im++) o->wseadj[im] = o->wsebnd[im] + ws[im];
+            free(ws);
+            wse = o->wseadj;
+        }
+        for (int j = 0; j < o->nTH; j++) {
+            int i = o->THc[j], f = 4 * i + o->THk[j], nck = o->c2c[f];
+            o->pp[nck] = wse[j] * GRAV - o->p[nck];
+            o->su[i] += o->acoef[f] * o->pp[nck];
+            o->sp[i] -= o->acoef[f];
+            o->acoef[f] = 0.0;
+        }
+    }
+}
+
+/* pvfc, flow_imp.F90:797-1072 (no skewness terms on Cartesian grids) */
+static void pvfc(cms_oracle *o)
+{
+    const int nc = o->ncells, nd = o->ncellsD;
+    for (int j = 0; j < o->nW; j++) {
+        int i = o->Wc[j];
+        o->pp[o->c2c[4 * i + o->Wk[j]]] = o->pp[i];
+    }
+    der_grad_eval(o, o->gow, o->pp, o->dppx, o->dppy);
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] == 0) continue;
+        double val = o->h[i] * o->apuareap[i] / (1.0 - o->sumu[i] + SMALL);
+        o->u[i] -= val * o->dppx[i];
+        o->v[i] -= val * o->dppy[i];
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            if (o->iwet[nck] == 0) continue;
+            o->flux[f] -= o->acoef[f] * (o->pp[nck] - o->pp[i]);
+            o->flux[4 * nck + o->llec[f]] = -o->flux[f];
+        }
+    }
+    for (int i = 0; i < nd; i++) {
+        o->p[i] += RELAX_PP * o->pp[i];
+        o->h[i] = fmax(o->hmin, o->p[i] * (1.0 / GRAV) - o->zb[i]);
+        o->pp[i] = FACPP * o->pp[i];
+    }
+    for (int j = 0; j < o->nW; j++) {
+        int i = o->Wc[j];
+        o->p[o->c2c[4 * i + o->Wk[j]]] = o->p[i];
+    }
+}
+
+/* flow_grad_interp, flow_imp.F90:1075-1130 */
+static void flow_grad_interp(cms_oracle *o)
+{
+    der_grad_eval(o, o->gow, o->u, o->dux, o->duy);
+    der_grad_eval(o, o->gow, o->v, o->dvx, o->dvy);
+    interp_cell2face(o, o->p, 1, o->pk);
+    for (int i = 0; i < o->ncells; i++) {
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int f = 4 * i + k;
+            sx += o->dsx[f] * o->pk[f];
+            sy += o->dsy[f] * o->pk[f];
+            o->hk[f] = fmax(o->hmin, o->pk[f] * (1.0 / GRAV) - o->zbk[f]);
+        }
+        o->dpx[i] = sx / o->area[i];
+        o->dpy[i] = sy / o->area[i];
+    }
+}
+
+/* fric_eval, friction.F90:615-797 (mbedfric=2 Manning; mwavcurint=1 or no waves) */
+static void fric_eval(cms_oracle *o)
+{
+    for (int i = 0; i < o->ncells; i++) {
+        o->cfrict[i] = cms_oracle_man2drag(GRAV, o->coefman[i], o->h[i], HMAXFRIC);
+        if (o->cfrict[i] != 0.0) o->z0[i] = cms_oracle_drag2length(o->cfrict[i], o->h[i]);
+    }
+    for (int i = 0; i < o->ncells; i++) {
+        double Um, Uwc, taum;
+        if (o->c.noptset >= 3) {
+            /* fric_wavcurmean_quad with us=vs=Worb=0 (fric_lib.f90:840-874) */
+            double Um2 = o->u[i] * o->u[i] + o->v[i] * o->v[i];
+            Um = sqrt(Um2);
+            Uwc = sqrt(Um2 + CFRICWAV * 0.0 * 0.0);
+            taum = RHOW * o->cfrict[i] * Um * Uwc;
+        } else {
+            Um = sqrt(o->u[i] * o->u[i] + o->v[i] * o->v[i]);
+            Uwc = Um;
+            taum = RHOW * o->cfrict[i] * Um * Um;
+        }
+        o->uv[i] = Um; o->uelwc[i] = Uwc; o->bsxy[i] = taum;
+        o->bsvel[i] = sqrt(taum / RHOW + F4(1.0e-15));
+    }
+}
+
+/* flow_eddyvis, flow_eddyvis.f90:2-225 (mturbul=5, Cartesian mesh) */
+static void flow_eddyvis(cms_oracle *o)
+{
+    const int nc = o->ncells, nd = o->ncellsD;
+    const double c2a = CVISHOR * CVISHOR * o->areaavg;
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] == 1) {
+            double stens = sqrt(2.0 * (o->dux[i] * o->dux[i] + o->dvy[i] * o->dvy[i])
+                                + (o->duy[i] + o->dvx[i]) * (o->duy[i] + o->dvx[i]));
+            double vv = CVISCON + CVISBOT * o->bsvel[i] * o->h[i] + c2a * stens;
+            if (o->c.noptset >= 3) {
+                vv += CVISWAV * 0.0 * 0.0;
+                vv += CVISWAVBRK * o->h[i] * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
+            }
+            o->vis[i] = fmin(vv, CVISMAX);
+        } else {
+            o->vis[i] = F4(1.0e-6);
+        }
+    }
+    /* bndcopy2ghost: forcing strings */
+    for (int j = 0; j < o->nTH; j++) {
+        int i = o->THc[j];
+        o->vis[o->c2c[4 * i + o->THk[j]]] = o->vis[i];
+    }
+    /* boundaries near dry cells (Cartesian branch) */
+    double *rv = o->vis;
+    double *old = NULL;
+    if (!o->c.serial) { old = DALLOC(nd); memcpy(old, o->vis, sizeof(double) * nd); rv = old; }
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] != 1) continue;
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            if (o->iwet[nck] == 0 || nck >= nc) {
+                int opp = o->c2c[4 * i + kopp[k]];
+                double vissum = rv[opp] - CVISCON;
+                o->vis[i] = CVISCON + vissum / (1.0 + SMALL);
+                o->vis[nck] = o->vis[i];
+            }
+        }
+    }
+    /* corner wet nodes */
+    if (!o->c.serial) memcpy(old, o->vis, sizeof(double) * nd);
+    for (int i = 0; i < nc; i++) {
+        if (o->icorner[i] != 1) continue;
+        double cid = 0.0, vs = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            cid += o->iwet[nck];
+            vs += (rv[nck] - CVISCON) * o->iwet[nck];
+        }
+        o->vis[i] = CVISCON + vs / (cid + SMALL);
+    }
+    free(old);
+    interp_cell2face(o, o->vis, 3, o->visk);
+}
+
+/* check_conv, flow_imp.F90:1576-1747 */
+static void check_conv(cms_oracle *o)
+{
+    double *rmom = o->rmom, *r0 = o->rmom0, *r1 = o->rmom1;
+    int niter = o->niter;
+    o->iconv = 1;
+    if (niter <= 2) {
+        for (int n = 0; n < 3; n++) r0[n] = fmax(rmom[n], F4(1e-15));
+        for (int n = 0; n < 3; n++) r1[n] = r0[n];
+    }
+    double rmax = fmax(rmom[0], fmax(rmom[1], rmom[2]));
+    if (rmax > 5.0) { o->iconv = 0; return; }
+    if (niter <= 5) return;
+    int nhalf = o->maxit / 2;
+    double rp = rmom[0] / r0[0];
+    double ruv = fmax(rmom[1] / r0[1], rmom[2] / r0[2]);
+    if ((rp >= RMOMRATIOP * 10.0 || ruv >= RMOMRATIOUV * 10.0) && rmom[0] > RMOMTARGETP) { o->iconv = 0; return; }
+    if (niter == o->maxit && (rp >= RMOMRATIOP || ruv >= RMOMRATIOUV) && rmom[0] > RMOMMAXP) { o->iconv = 0; return; }
+    if (niter == o->maxit && (rp >= RMOMRATIOP / 5.0 || ruv >= RMOMRATIOUV / 5.0) && rmom[0] > RMOMTARGETP) { o->iconv = 0; return; }
+    double rmomuv = fmax(rmom[1], rmom[2]);
+    if (rmom[0] <= RMOMMINP && rmomuv <= RMOMMINUV) { o->iconv = 2; return; }
+    if (niter == o->maxit && (rmom[0] > RMOMMAXP || rmomuv > RMOMMAXUV)) { o->iconv = 0; return; }
+    if (niter == o->maxit && (rmom[0] > RMOMTARGETP || rmomuv > RMOMTARGETUV)) { o->iconv = 0; return; }
+    if (niter <= nhalf || niter % 2 != 0) return;
+    double rchp = r1[0] - rmom[0];
+    double rchuv = fmax(r1[1], r1[2]) - rmomuv;
+    if (rchp < 0.0 || rchuv < 0.0) return;
+    if (rchp <= RMOMMINP && rchuv <= RMOMMINUV) { o->iconv = 2; return; }
+    for (int n = 0; n < 3; n++) r1[n] = rmom[n];
+}
+
+/* velface_eta, flow_imp.F90:1245-1573 (Cartesian branch) */
+static void velface_eta(cms_oracle *o)
+{
+    for (int i = 0; i < o->ncells; i++) {
+        if (o->h[i] > o->hmin && o->iwet[i] == 1) {
+            double umin = 1.0e6, umax = -1.0e6, vmin = 1.0e6, vmax = -1.0e6;
+            for (int k = 0; k < 4; k++) {
+                int f = 4 * i + k;
+                double vel = o->flux[f] / (o->hk[f] * o->ds[f]);
+                double uf = o->fnx[f] * vel, vf = o->fny[f] * vel;
+                umin = fmin(umin, uf); umax = fmax(umax, uf);
+                vmin = fmin(vmin, vf); vmax = fmax(vmax, vf);
+            }
+            o->u[i] = fmin(fmax(o->u[i], umin), umax);
+            o->v[i] = fmin(fmax(o->v[i], vmin), vmax);
+            o->eta[i] = o->p[i] * (1.0 / GRAV);
+            o->uv[i] = sqrt(o->u[i] * o->u[i] + o->v[i] * o->v[i]);
+        } else {
+            o->h[i] = fmax(o->hmin, o->h[i]);
+            o->u[i] = 0.0; o->v[i] = 0.0; o->uv[i] = 0.0;
+            o->eta[i] = -999.0;
+        }
+    }
+}
+
+/* flow_imp, flow_imp.F90:29-312, fixed time step (VARIABLE_TIMESTEP OFF) */
+static int flow_imp(cms_oracle *o)
+{
+    bnd_eval(o);
+    flow_wetdry(o);
+    der_grad_cbgg(o, o->gow, 1);                 /* der_update: weights follow iwet */
+    flow_srcsnk_static(o);
+    o->kry[0] = o->kry[1] = o->kry[2] = 0;
+    for (o->niter = 1; o->niter <= o->maxit; o->niter++) {
+        coeffsourcesink_uv(o);
+        bound_uv(o);
+        o->kry[1] += solve_sys(o, o->su, o->spu, o->rsu, o->u, 2, &o->rmom[1]);
+        o->kry[2] += solve_sys(o, o->sv, o->spv, o->rsv, o->v, 3, &o->rmom[2]);
+        coefsourcesink_pp(o);
+        bound_pp(o);
+        o->kry[0] += solve_sys(o, o->su, o->sp, o->rsp, o->pp, 1, &o->rmom[0]);
+        pvfc(o);
+        flow_grad_interp(o);
+        fric_eval(o);
+        flow_eddyvis(o);
+        check_conv(o);
+        if (o->iconv == 0) return -1;
+        if (o->iconv == 2) break;
+    }
+    if (o->niter > o->maxit) o->niter = o->maxit;
+    velface_eta(o);
+    fric_eval(o);
+    return 0;
+}
+
+/* flow_update, CMS_Flow.f90:243-411 */
+static void flow_update(cms_oracle *o)
+{
+    const int nd = o->ncellsD;
+    for (int i = 0; i < nd; i++) {
+        o->iwet1[i] = o->iwet[i];
+        o->u1[i] = o->u[i]; o->v1[i] = o->v[i]; o->p1[i] = o->p[i]; o->h1[i] = o->h[i];
+        o->pp[i] = 0.0; o->dppx[i] = 0.0; o->dppy[i] = 0.0;
+    }
+    memcpy(o->flux1, o->flux, sizeof(double) * 4 * nd);
+}
+
+/* boundary inflow volume over one step from the final boundary fluxes */
+static double bnd_inflow(const cms_oracle *o)
+{
+    double q = 0.0;
+    for (int j = 0; j < o->nTH; j++) q -= o->flux[4 * o->THc[j] + o->THk[j]];
+    return q * o->dtime;
+}
+
+/* flow_convflux, flow.F90:1879-1909 */
+static void flow_convflux(cms_oracle *o)
+{
+    for (int i = 0; i < o->ncells; i++)
+        for (int j = 0; j < o->nxy[i]; j++) {
+            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            double uni = o->fnx[f] * o->u[i] + o->fny[f] * o->v[i];
+            double unk = o->fnx[f] * o->u[nck] + o->fny[f] * o->v[nck];
+            o->flux[f] = o->iwet[i] * o->iwet[nck] * o->ds[f] * o->hk[f]
+                         * (o->fintp[f] * unk + (1.0 - o->fintp[f]) * uni);
+            o->flux[4 * nck + o->llec[f]] = -o->flux[f];
+        }
+}
+
+cms_oracle *cms_oracle_create(const cms_case *c)
+{
+    cms_oracle *o = (cms_oracle *)xcalloc(1, sizeof(cms_oracle));
+    o->c = *c;
+    o->nx = c->nx; o->ny = c->ny;
+    build_grid(o);
+    const int nd = o->ncellsD, nc = o->ncells, F = 4 * nd;
+#define A(f) o->f = DALLOC(nd)
+    A(u); A(v); A(p); A(h); A(pp); A(dppx); A(dppy); A(dux); A(duy); A(dvx); A(dvy); A(dpx); A(dpy);
+    A(vis); A(eta); A(u1); A(v1); A(p1); A(h1);
+    A(su); A(sv); A(sp); A(spu); A(spv); A(ssu0); A(ssv0); A(spuv0); A(sppp0); A(sspp0);
+    A(apuareap); A(sumu); A(Hu); A(Hv); A(rsu); A(rsv); A(rsp);
+    A(coefman); A(cfrict); A(z0); A(uv); A(uelwc); A(bsxy); A(bsvel); A(cmb); A(cbcfuwcap);
+    A(wavestrx); A(wavestry); A(fc);
+#undef A
+    o->iwet = IALLOC(nd); o->iwet1 = IALLOC(nd); o->icorner = IALLOC(nd);
+    o->flux = DALLOC(F); o->flux1 = DALLOC(F); o->hk = DALLOC(F); o->pk = DALLOC(F);
+    o->visk = DALLOC(F); o->zbk = DALLOC(F); o->acoef = DALLOC(F);
+    o->gow = (gradop *)xcalloc((size_t)nc, sizeof(gradop));
+    o->wk = DALLOC(7 * (size_t)nc);
+    o->ia = IALLOC(nc + 1); o->ja = IALLOC(5 * nc); o->diagpos = IALLOC(nc); o->lu = DALLOC(5 * nc);
+    for (int i = 0; i < nc; i++) {
+        o->coefman[i] = c->mann[i];
+        o->cmb[i] = 1.0;
+        o->wavestrx[i] = c->wavestrx ? c->wavestrx[i] : 0.0;
+        o->wavestry[i] = c->wavestry ? c->wavestry[i] : 0.0;
+    }
+    build_bnd(o);
+    /* flow_init (flow.F90:592-835) */
+    o->dtime = c->dt;
+    o->hdry = c->hdry > 0 ? c->hdry : F4(0.05);
+    o->hmin = o->hdry;
+    o->hdry1 = F4(5.0) * o->hdry;
+    o->hdry2 = F4(2.0) * o->hdry;
+    o->rampdur = fmax(c->rampdur, o->dtime / 3600.0);
+    o->maxit = c->maxit > 0 ? c->maxit : 20 + nc / 10000;
+    o->timesecs = 0.0; o->ctime = 0.0; o->timehrs = 0.0;
+    for (int n = 0; n < 3; n++) { o->rmom0[n] = 1.e10; o->rmom[n] = 0.0; }
+    if (o->c.rtol <= 0) o->c.rtol = 1e-10;
+    if (o->c.kmax <= 0) o->c.kmax = 2000;
+    flow_wetdry(o);
+    memcpy(o->u1, o->u, sizeof(double) * nd); memcpy(o->v1, o->v, sizeof(double) * nd);
+    memcpy(o->p1, o->p, sizeof(double) * nd); memcpy(o->h1, o->h, sizeof(double) * nd);
+    der_grad_cbgg(o, o->gow, 1);                 /* der_init */
+    /* prestart initial state (prestart.f90:309-329) */
+    flow_wetdry(o);
+    memcpy(o->iwet1, o->iwet, sizeof(int) * nd);
+    for (int i = 0; i < nd; i++) o->eta[i] = o->iwet[i] * o->p[i] * (1.0 / GRAV) - 999.0 * (1 - o->iwet[i]);
+    /* zbk from interp_scal_cell2face(zb,0,...) covers every face of every active cell */
+    interp_cell2face(o, o->zb, 0, o->zbk);
+    der_grad_cbgg(o, o->gow, 1);
+    flow_grad_interp(o);
+    flow_convflux(o);
+    fric_eval(o);
+    flow_eddyvis(o);
+    return o;
+}
+
+void cms_oracle_destroy(cms_oracle *o)
+{
+    if (!o) return;
+    void *ptrs[] = {o->c2c, o->llec, o->nxy, o->kxy, o->dx, o->dy, o->x, o->y, o->area, o->ds, o->dn,
+                    o->fnx, o->fny, o->rx, o->ry, o->dsx, o->dsy, o->dsxy, o->fintp, o->Wc, o->Wk, o->THc,
+                    o->THk, o->wsebnd, o->wseadj, o->u, o->v, o->p, o->h, o->pp, o->dppx, o->dppy, o->dux,
+                    o->duy, o->dvx, o->dvy, o->dpx, o->dpy, o->vis, o->zb, o->eta, o->u1, o->v1, o->p1, o->h1,
+                    o->iwet, o->iwet1, o->icorner, o->flux, o->flux1, o->hk, o->pk, o->visk, o->zbk, o->acoef,
+                    o->su, o->sv, o->sp, o->spu, o->spv, o->ssu0, o->ssv0, o->spuv0, o->sppp0, o->sspp0,
+                    o->apuareap, o->sumu, o->Hu, o->Hv, o->rsu, o->rsv, o->rsp, o->coefman, o->cfrict, o->z0,
+                    o->uv, o->uelwc, o->bsxy, o->bsvel, o->cmb, o->cbcfuwcap, o->wavestrx, o->wavestry, o->fc,
+                    o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
+    for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
+    free(o);
+}
+
+int cms_oracle_ncells(const cms_oracle *o) { return o->ncells; }
+int cms_oracle_ncellsD(const cms_oracle *o) { return o->ncellsD; }
+double cms_oracle_time(const cms_oracle *o) { return o->timesecs; }
+
+int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info)
+{
+    for (int s = 0; s < nsteps; s++) {
+        /* timing, CMS_Flow.f90:109-240 (fixed dt) */
+        o->ntime++;
+        o->timesecs += o->dtime;
+        o->ctime = o->timesecs;
+        o->timehrs = o->ctime / 3600.0;
+        o->ramp = ramp_func(o->timehrs, o->rampdur);
+        int rc = flow_imp(o);
+        if (info) {
+            info->niter = o->niter; info->iconv = o->iconv;
+            for (int n = 0; n < 3; n++) { info->rmom[n] = o->rmom[n]; info->kry_iters[n] = o->kry[n]; }
+            int nw = 0;
+            for (int i = 0; i < o->ncells; i++) nw += o->iwet[i];
+            info->nwet = nw;
+        }
+        if (rc != 0) return rc;
+        o->bndvol += bnd_inflow(o);
+        flow_update(o);
+    }
+    return 0;
+}
+
+static const double *field(const cms_oracle *o, const char *name)
+{
+    if (!strcmp(name, "eta")) return o->eta;
+    if (!strcmp(name, "u")) return o->u;
+    if (!strcmp(name, "v")) return o->v;
+    if (!strcmp(name, "p")) return o->p;
+    if (!strcmp(name, "h")) return o->h;
+    if (!strcmp(name, "vis")) return o->vis;
+    if (!strcmp(name, "pp")) return o->pp;
+    if (!strcmp(name, "dpx")) return o->dpx;
+    if (!strcmp(name, "dpy")) return o->dpy;
+    if (!strcmp(name, "cfrict")) return o->cfrict;
+    if (!strcmp(name, "uv")) return o->uv;
+    if (!strcmp(name, "bsvel")) return o->bsvel;
+    if (!strcmp(name, "zb")) return o->zb;
+    if (!strcmp(name, "z0")) return o->z0;
+    return NULL;
+}
+
+int cms_oracle_get(const cms_oracle *o, const char *name, double *out)
+{
+    if (!strcmp(name, "iwet")) {
+        for (int i = 0; i < o->ncells; i++) out[i] = o->iwet[i];
+        return 0;
+    }
+    const double *f = field(o, name);
+    if (!f) return -1;
+    memcpy(out, f, sizeof(double) * o->ncells);
+    return 0;
+}
+
+int cms_oracle_set(cms_oracle *o, const char *name, const double *in)
+{
+    double *f = (double *)field(o, name);
+    if (!f) return -1;
+    memcpy(f, in, sizeof(double) * o->ncells);
+    return 0;
+}
+
+int cms_oracle_get_face(const cms_oracle *o, const char *name, double *out)
+{
+    const double *f = NULL;
+    if (!strcmp(name, "flux")) f = o->flux;
+    else if (!strcmp(name, "hk")) f = o->hk;
+    else if (!strcmp(name, "visk")) f = o->visk;
+    else if (!strcmp(name, "pk")) f = o->pk;
+    if (!f) return -1;
+    memcpy(out, f, sizeof(double) * 4 * o->ncells);
+    return 0;
+}
+
+void cms_oracle_volume(const cms_oracle *o, double *vol, double *bndvol)
+{
+    double s = 0.0;
+    for (int i = 0; i < o->ncells; i++) s += o->area[i] * o->h[i];
+    *vol = s;
+    *bndvol = o->bndvol;
+}
+
+int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2c,
+                     const int *iwet, double *ss, const double *sp, double *res,
+                     double *phi, double relax, double rtol, double atol, int kmax,
+                     double *rmom)
+{
+    double rinv = 1.0 / relax, relaxm = 1.0 - relax, adak = 0.0;
+    double *ap = DALLOC(ncells), *wk = DALLOC(6 * (size_t)ncells);
+    for (int i = 0; i < ncells; i++) {
+        double asum = 0.0, anb = 0.0;
+        for (int k = 0; k < 4; k++) {
+            int nck = c2c[4 * i + k];
+            if (nck < 0) continue;
+            asum += acoef[4 * i + k];
+            anb += acoef[4 * i + k] * phi[nck];
+        }
+        ap[i] = (asum - sp[i]) * rinv;
+        ss[i] = ss[i] + relaxm * phi[i] * ap[i];
+        res[i] = iwet[i] * (anb + ss[i] - ap[i] * phi[i]) / ap[i];
+        adak += res[i] * res[i];
+    }
+    *rmom = sqrt(adak / (double)(float)ncells);
+    int *ia = IALLOC(ncells + 1), *ja = IALLOC(5 * ncells), *dp = IALLOC(ncells);
+    double *lu = DALLOC(5 * ncells);
+    sysmat S = {ncells, ncellsD, acoef, c2c, ap, ia, ja, dp, lu};
+    int it = bicgstab(&S, ss, phi, 0, kmax, rtol, atol, wk);
+    free(ap); free(wk); free(ia); free(ja); free(dp); free(lu);
+    return it;
+}

@@ -1,0 +1,96 @@
+/*
+ * cms_oracle.h -- FP64 CPU restatement of CMS-Flow's implicit hydrodynamic
+ * time step (reference: erdc/cms2d 5.4, source/flow/flow_imp.F90:29-312).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in cms2d_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * Data model: the reference's own general cell list (active cells numbered
+ * row-major, ghost cells appended, directed faces k = N,E,S,W per cell, no-repeat
+ * face lists), restated in C with 0-based indices.  Face arrays are stored
+ * cell-major, a[4*i+k], like the reference's column-major a(k,i).
+ */
+#ifndef CMS_ORACLE_H
+#define CMS_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cms_oracle cms_oracle;
+
+/* Case description for a rectangular, fully active Cartesian grid. */
+typedef struct {
+    int nx, ny;               /* active cells in x (columns) and y (rows)        */
+    double dx, dy;            /* uniform cell sizes [m]                          */
+    const double *zb;         /* bed elevation [nx*ny], row-major (j*nx+i) [m]   */
+    const double *mann;       /* Manning n per cell [nx*ny]                      */
+    const double *wavestrx;   /* kinematic radiation-stress force [nx*ny] or NULL */
+    const double *wavestry;
+    int th_side;              /* side carrying the harmonic WSE string: 0=N,1=E,2=S,3=W, -1 none */
+    int ntc;                  /* number of harmonic constituents (<=8)           */
+    double amp[8];            /* [m]                                             */
+    double speed[8];          /* [rad/hr]                                        */
+    double phase[8];          /* [rad]                                           */
+    double dt;                /* time step [s]                                   */
+    double rampdur;           /* ramp duration [hr] (0 -> one time step)         */
+    double hdry;              /* drying depth [m] (reference default 0.05)       */
+    int noptset;              /* >=3 selects the wave branches (radiation stress) */
+    int wallfric;             /* wall friction on/off                            */
+    int ndsch;                /* advection scheme: 1 upwind, 2 hybrid, 3 power-law, 4 exponential */
+    int maxit;                /* outer iterations cap (0 -> 20+ncells/10000)     */
+    int solver_mode;          /* 0 = tight (rtol), 1 = reference stopping rule   */
+    double rtol;              /* tight mode: ||r||2 <= rtol*||r0||2 ...          */
+    double atol;              /* ... or ||r||2 <= atol*sqrt(ncells)              */
+    int kmax;                 /* tight mode iteration cap per solve              */
+    int serial;               /* 1: reference serial loop order for the three loops whose
+                                 OpenMP form races; 0: race-free (all reads from the values
+                                 before the loop), the semantics of the HIP path      */
+} cms_case;
+
+typedef struct {
+    int niter;                /* outer iterations taken in the last step         */
+    int iconv;                /* 0 divergent, 1 continue, 2 converged, 3 reduce dt */
+    double rmom[3];           /* pp, u, v normalized residual RMS (last outer iter) */
+    int kry_iters[3];         /* Krylov iterations summed over the last step (pp,u,v) */
+    int nwet;
+} cms_step_info;
+
+cms_oracle *cms_oracle_create(const cms_case *c);
+void cms_oracle_destroy(cms_oracle *o);
+int cms_oracle_ncells(const cms_oracle *o);
+int cms_oracle_ncellsD(const cms_oracle *o);
+/* Runs nsteps time steps (timing + flow_imp + flow_update).  Returns 0 on
+ * success, -1 if the step diverged (iconv==0 with fixed dt). */
+int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info);
+/* Copies an active-cell field (row-major nx*ny) out.  name: "eta","u","v","p",
+ * "h","iwet","vis","pp","dpx","dpy","cfrict","uv","bsvel","zb". */
+int cms_oracle_get(const cms_oracle *o, const char *name, double *out);
+/* Unique-face fields: "fluxE","fluxN" [(nx+1)*ny] / [nx*(ny+1)] style is not used;
+ * instead the directed face of every active cell: out[4*c+k], k=N,E,S,W. */
+int cms_oracle_get_face(const cms_oracle *o, const char *name, double *out);
+int cms_oracle_set(cms_oracle *o, const char *name, const double *in);
+double cms_oracle_time(const cms_oracle *o);
+/* Total water volume sum(area*h) over wet active cells and the net boundary
+ * inflow volume accumulated over the steps run so far. */
+void cms_oracle_volume(const cms_oracle *o, double *vol, double *bndvol);
+
+/* --- Unit entry points used by the tests (closed forms) --------------------- */
+double cms_oracle_schmcoef(int ndsch, double dk, double fk);
+double cms_oracle_wall_coef(double yp, double y0);
+double cms_oracle_man2drag(double g, double cman, double h, double hmax);
+double cms_oracle_drag2length(double cd, double h);
+
+/* Stand-alone linear solve on a general cell list, the reference's
+ * solve(acoef,ss,sp,res,phi,n) (solvers.f90:205-304) with the tight stopping rule.
+ * acoef[4*ncellsD], c2c[4*ncellsD] (0-based, -1 none), iwet[ncellsD].
+ * Returns iterations used, rmom via *rmom. */
+int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2c,
+                     const int *iwet, double *ss, const double *sp, double *res,
+                     double *phi, double relax, double rtol, double atol, int kmax,
+                     double *rmom);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

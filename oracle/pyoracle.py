@@ -1,0 +1,142 @@
+"""ctypes front end of the FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, never by cms2d_amd/.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+
+class CmsCase(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+        ("zb", D), ("mann", D), ("wavestrx", D), ("wavestry", D),
+        ("th_side", C.c_int), ("ntc", C.c_int),
+        ("amp", C.c_double * 8), ("speed", C.c_double * 8), ("phase", C.c_double * 8),
+        ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
+        ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
+        ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
+        ("serial", C.c_int),
+    ]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("niter", C.c_int), ("iconv", C.c_int), ("rmom", C.c_double * 3),
+                ("kry_iters", C.c_int * 3), ("nwet", C.c_int)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
+        L.cms_oracle_create.restype = C.c_void_p
+        L.cms_oracle_create.argtypes = [C.POINTER(CmsCase)]
+        L.cms_oracle_destroy.argtypes = [C.c_void_p]
+        L.cms_oracle_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepInfo)]
+        L.cms_oracle_get.argtypes = [C.c_void_p, C.c_char_p, D]
+        L.cms_oracle_set.argtypes = [C.c_void_p, C.c_char_p, D]
+        L.cms_oracle_get_face.argtypes = [C.c_void_p, C.c_char_p, D]
+        L.cms_oracle_ncells.argtypes = [C.c_void_p]
+        L.cms_oracle_ncellsD.argtypes = [C.c_void_p]
+        L.cms_oracle_time.restype = C.c_double
+        L.cms_oracle_time.argtypes = [C.c_void_p]
+        L.cms_oracle_volume.argtypes = [C.c_void_p, D, D]
+        for fn, na in (("cms_oracle_schmcoef", 3), ("cms_oracle_wall_coef", 2),
+                       ("cms_oracle_man2drag", 4), ("cms_oracle_drag2length", 2)):
+            f = getattr(L, fn)
+            f.restype = C.c_double
+            f.argtypes = ([C.c_int] if fn == "cms_oracle_schmcoef" else []) + \
+                [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
+        L.cms_oracle_solve.restype = C.c_int
+        L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, D]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(D)
+
+
+class Oracle:
+    """FP64 restatement run on a Case (see cms2d_amd.cases)."""
+
+    def __init__(self, case, *, serial=0, solver_mode=0, rtol=1e-10, atol=1e-12, kmax=4000,
+                 maxit=0):
+        L = lib()
+        self.case = case
+        self._keep = []
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return _dp(a)
+
+        c = CmsCase()
+        c.nx, c.ny, c.dx, c.dy = case.nx, case.ny, case.dx, case.dy
+        c.zb, c.mann = arr(case.zb), arr(case.mann)
+        c.wavestrx = arr(case.wavestrx) if case.wavestrx is not None else D()
+        c.wavestry = arr(case.wavestry) if case.wavestry is not None else D()
+        c.th_side = case.th_side
+        c.ntc = len(case.amp)
+        for k in range(c.ntc):
+            c.amp[k], c.speed[k], c.phase[k] = case.amp[k], case.speed_rad_hr[k], case.phase_rad[k]
+        c.dt, c.rampdur, c.hdry = case.dt, case.rampdur, float(np.float32(case.hdry))
+        c.noptset, c.wallfric, c.ndsch = case.noptset, case.wallfric, case.ndsch
+        c.maxit, c.solver_mode, c.rtol, c.atol, c.kmax = maxit, solver_mode, rtol, atol, kmax
+        c.serial = serial
+        self._c = c
+        self.h = L.cms_oracle_create(C.byref(c))
+        self.ncells = L.cms_oracle_ncells(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().cms_oracle_destroy(self.h)
+            self.h = None
+
+    def step(self, n=1):
+        info = StepInfo()
+        rc = lib().cms_oracle_step(self.h, n, C.byref(info))
+        return rc, {"niter": info.niter, "iconv": info.iconv, "rmom": list(info.rmom),
+                    "kry": list(info.kry_iters), "nwet": info.nwet}
+
+    def get(self, name):
+        out = np.empty(self.ncells)
+        if lib().cms_oracle_get(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if lib().cms_oracle_set(self.h, name.encode(), _dp(a)) != 0:
+            raise KeyError(name)
+
+    def get_face(self, name):
+        out = np.empty(4 * self.ncells)
+        if lib().cms_oracle_get_face(self.h, name.encode(), _dp(out)) != 0:
+            raise KeyError(name)
+        return out.reshape(self.ncells, 4)
+
+    def volume(self):
+        v, b = C.c_double(), C.c_double()
+        lib().cms_oracle_volume(self.h, C.byref(v), C.byref(b))
+        return v.value, b.value
+
+    @property
+    def time(self):
+        return lib().cms_oracle_time(self.h)
