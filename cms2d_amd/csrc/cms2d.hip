@@ -1,0 +1,651 @@
+// cms2d.hip -- host driver and C ABI of the MI355X implicit CMS-Flow step.
+//
+// One HIP stream per flow object.  The host keeps the reference's scalar
+// control (timing, the SIMPLEC outer loop, check_conv) and everything else
+// stays on the device for the whole run; only scalars cross per outer
+// iteration (Krylov flags and residual norms, wet/dry change flags).
+#include "flow_kernels.hip"
+#include "krylov_kernels.hip"
+#include "../../include/cms2d_amd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace cms;
+
+static thread_local std::string g_err;
+static int set_err(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return set_err(CMS2D_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct cms2d_flow {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    Geo g{};
+    Par P{};
+    St s{};
+    KrySys<2> Kuv{};
+    KrySys<1> Kpp{};
+    KryScal *d_sc = nullptr;   // [3]: pp, u, v
+    KryScal *h_sc = nullptr;   // pinned mirror
+    int *h_flags = nullptr;    // pinned
+    std::vector<void *> allocs;
+    int npad = 0, nblk_act = 0, nblk_pad = 0;
+    int ntc = 0;
+    double amp[8]{}, speed[8]{}, phase[8]{};
+    double timesecs = 0.0, rampdur = 0.0;
+    int ntime = 0, maxit = 0, niter = 0, iconv = 1;
+    double rmom[3]{}, rmom0[3]{}, rmom1[3]{};
+    int kry[3]{}, prev_iters[3]{4, 2, 2};
+    // timing of the pp-system SpMV launches (events bracket each launch)
+    bool stats = true;
+    std::vector<hipEvent_t> ev;
+    double spmv_ms = 0.0;
+    long spmv_n = 0;
+    hipEvent_t step_ev[2]{};
+};
+
+extern "C" const char *cms2d_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cms2d_device_count(int *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return CMS2D_OK;
+}
+
+template <typename T>
+static int dalloc(cms2d_flow *F, T **p, size_t n) {
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), F->st));
+    F->allocs.push_back(q);
+    *p = (T *)q;
+    return 0;
+}
+
+#define LAUNCH(kern, nb, ...) hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, F->st, __VA_ARGS__)
+
+static int sync_flags(cms2d_flow *F) {
+    HIPCHK(hipMemcpyAsync(F->h_flags, F->s.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+// flow_wetdry(0), flow.F90:1231-1539
+static int wetdry(cms2d_flow *F) {
+    Geo &g = F->g;
+    St &s = F->s;
+    LAUNCH(k_wetdry_h, F->nblk_pad, g, s, F->P);
+    for (int it = 0;; it++) {
+        LAUNCH(k_set_int, 1, s.flags, 3, 0);
+        LAUNCH(k_narrow, F->nblk_act, g, s, F->P);
+        if (int rc = sync_flags(F)) return rc;
+        if (F->h_flags[0] == 0) break;
+        if (it > 100000) return set_err(CMS2D_ERR_HIP, "narrow-channel pass did not settle");
+    }
+    LAUNCH(k_pond_seed, F->nblk_pad, g, s);
+    dim3 pg((g.nxp + PT_X - 1) / PT_X, (g.nyp + PT_Y - 1) / PT_Y);
+    for (int it = 0;; it++) {
+        LAUNCH(k_set_int, 1, s.flags, 3, 0);
+        hipLaunchKernelGGL(k_pond_tile, pg, dim3(256), 0, F->st, g, s);
+        if (int rc = sync_flags(F)) return rc;
+        if (F->h_flags[1] == 0) break;
+        if (it > 1000000) return set_err(CMS2D_ERR_HIP, "ponding pass did not settle");
+    }
+    LAUNCH(k_pond_final, F->nblk_pad, g, s);
+    LAUNCH(k_set_int, 1, s.flags, 3, 0);
+    LAUNCH(k_wetdry_post, F->nblk_act, g, s);
+    if (int rc = sync_flags(F)) return rc;
+    if (F->h_flags[2] > 0) {
+        for (int sweep = 0; sweep < 5; sweep++) {
+            LAUNCH(k_d2w_sweep, F->nblk_act, g, s);
+            LAUNCH(k_d2w_commit, F->nblk_act, g, s);
+        }
+    }
+    return 0;
+}
+
+// Krylov launch context: stream, reduction scratch and scalar mirrors.
+struct KryCtx {
+    hipStream_t st;
+    int nblk;
+    int kmax;
+    double *partials;
+    unsigned *counter;
+    KryScal *d_sc, *h_sc;
+    int *prev_iters;
+    std::vector<hipEvent_t> *ev;   // event pairs bracketing timed SpMV launches (or null)
+    double *ev_ms;
+    long *ev_n;
+};
+
+// Runs BiCGStab iterations in batches until every system has stopped.  Each
+// batch is enqueued without host synchronisation; kernels launched after a
+// system has converged return at once (they read the device-side done flag).
+template <class Op, int NS>
+static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
+    int batch = std::max(1, X.prev_iters[sc0] - 1);
+    HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
+    HIPCHK(hipStreamSynchronize(X.st));
+    int done_all = 1;
+    for (int q = 0; q < NS; q++) done_all &= X.h_sc[sc0 + q].done;
+    int launched = 0;
+    while (!done_all && launched < X.kmax) {
+        int nb = std::min(batch, X.kmax - launched);
+        int it0 = X.h_sc[sc0].iters;
+        int nev = 0;
+        for (int b = 0; b < nb; b++) {
+            hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K);
+            bool rec = X.ev && nev < (int)X.ev->size() / 2;
+            if (rec) HIPCHK(hipEventRecord((*X.ev)[2 * nev], X.st));
+            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials,
+                               X.counter);
+            if (rec) { HIPCHK(hipEventRecord((*X.ev)[2 * nev + 1], X.st)); nev++; }
+            hipLaunchKernelGGL((k_bicg_s<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K);
+            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials,
+                               X.counter);
+            hipLaunchKernelGGL((k_bicg_r<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials, X.counter,
+                               X.kmax);
+        }
+        HIPCHK(hipGetLastError());
+        launched += nb;
+        HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
+        HIPCHK(hipStreamSynchronize(X.st));
+        done_all = 1;
+        for (int q = 0; q < NS; q++) done_all &= X.h_sc[sc0 + q].done;
+        if (nev > 0) {
+            int real = std::min(nev, X.h_sc[sc0].iters - it0);
+            for (int e = 0; e < real; e++) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, (*X.ev)[2 * e], (*X.ev)[2 * e + 1]));
+                *X.ev_ms += ms;
+                *X.ev_n += 1;
+            }
+        }
+        batch = std::min(2 * std::max(batch, 2), 64);
+    }
+    for (int q = 0; q < NS; q++) X.prev_iters[sc0 + q] = std::max(1, X.h_sc[sc0 + q].iters);
+    return 0;
+}
+
+static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
+    KryCtx X;
+    X.st = F->st; X.nblk = F->nblk_act; X.kmax = F->P.kmax;
+    X.partials = F->s.partials; X.counter = F->s.counter;
+    X.d_sc = F->d_sc; X.h_sc = F->h_sc; X.prev_iters = F->prev_iters;
+    X.ev = (timed && F->stats) ? &F->ev : nullptr;
+    X.ev_ms = &F->spmv_ms; X.ev_n = &F->spmv_n;
+    return X;
+}
+
+static double ramp_func(double timehrs, double rampdur) {
+    return 0.5 - 0.5 * std::cos(M_PI * std::min(timehrs, rampdur) / rampdur);
+}
+
+// check_conv, flow_imp.F90:1576-1747
+static void check_conv(cms2d_flow *F) {
+    double *rmom = F->rmom, *r0 = F->rmom0, *r1 = F->rmom1;
+    const int niter = F->niter, maxit = F->maxit;
+    F->iconv = 1;
+    if (niter <= 2) {
+        for (int n = 0; n < 3; n++) r0[n] = std::max(rmom[n], F4(1e-15));
+        for (int n = 0; n < 3; n++) r1[n] = r0[n];
+    }
+    double rmax = std::max(rmom[0], std::max(rmom[1], rmom[2]));
+    if (rmax > 5.0) { F->iconv = 0; return; }
+    if (niter <= 5) return;
+    int nhalf = maxit / 2;
+    double rp = rmom[0] / r0[0];
+    double ruv = std::max(rmom[1] / r0[1], rmom[2] / r0[2]);
+    const double RP = F4(0.5), RUV = 1.0;
+    if ((rp >= RP * 10.0 || ruv >= RUV * 10.0) && rmom[0] > F4(1.0e-4)) { F->iconv = 0; return; }
+    if (niter == maxit && (rp >= RP || ruv >= RUV) && rmom[0] > F4(1.0e-3)) { F->iconv = 0; return; }
+    if (niter == maxit && (rp >= RP / 5.0 || ruv >= RUV / 5.0) && rmom[0] > F4(1.0e-4)) { F->iconv = 0; return; }
+    double rmomuv = std::max(rmom[1], rmom[2]);
+    if (rmom[0] <= F4(1.0e-8) && rmomuv <= F4(1.0e-7)) { F->iconv = 2; return; }
+    if (niter == maxit && (rmom[0] > F4(1.0e-3) || rmomuv > F4(1.0e-2))) { F->iconv = 0; return; }
+    if (niter == maxit && (rmom[0] > F4(1.0e-4) || rmomuv > F4(1.0e-3))) { F->iconv = 0; return; }
+    if (niter <= nhalf || niter % 2 != 0) return;
+    double rchp = r1[0] - rmom[0];
+    double rchuv = std::max(r1[1], r1[2]) - rmomuv;
+    if (rchp < 0.0 || rchuv < 0.0) return;
+    if (rchp <= F4(1.0e-8) && rchuv <= F4(1.0e-7)) { F->iconv = 2; return; }
+    for (int n = 0; n < 3; n++) r1[n] = rmom[n];
+}
+
+static int eddyvis_tail(cms2d_flow *F) {
+    LAUNCH(k_vis_fix, F->nblk_pad, F->g, F->s);
+    LAUNCH(k_vis_corner, F->nblk_act, F->g, F->s);
+    LAUNCH(k_visk, F->nblk_act, F->g, F->s);
+    return 0;
+}
+
+// flow_imp, flow_imp.F90:29-312 (fixed time step)
+static int flow_imp(cms2d_flow *F) {
+    Geo &g = F->g;
+    St &s = F->s;
+    const double timehrs = F->timesecs / 3600.0;
+    const double ramp = ramp_func(timehrs, F->rampdur);
+    double w = 0.0;
+    for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
+    const double wsebnd = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);     // bnd_eval, boundaries.F90:4109-4122
+    if (int rc = wetdry(F)) return rc;
+    LAUNCH(k_srcsnk, F->nblk_act, g, s, F->P);
+    StencilOp A{g, s.aN, s.aE, s.aS, s.aW};
+    for (int n = 0; n < 3; n++) F->kry[n] = 0;
+    for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
+        LAUNCH(k_mom, F->nblk_act, g, s, F->P);
+        LAUNCH(k_uv_prologue, F->nblk_act, g, s, F->P, F->Kuv);
+        {
+            KryCtx X = flow_ctx(F, false);
+            if (int rc = run_bicg(X, F->Kuv, A, 1)) return rc;
+        }
+        LAUNCH(k_pp_prep, F->nblk_act, g, s);
+        LAUNCH(k_pp_coef_flux, F->nblk_act, g, s);
+        for (int sd = 0; sd < 4; sd++) {
+            if (g.side[sd] != 1) continue;
+            int n = (sd == KE || sd == KW) ? g.ny : g.nx;
+            hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd, wsebnd);
+        }
+        LAUNCH(k_pp_src, F->nblk_act, g, s, F->P, F->Kpp);
+        {
+            KryCtx X = flow_ctx(F, true);
+            if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
+        }
+        LAUNCH(k_pvfc1, F->nblk_act, g, s);
+        LAUNCH(k_pvfc2, F->nblk_pad, g, s, F->P);
+        LAUNCH(k_ghost_p, (2 * g.nx + 2 * g.ny + BLOCK - 1) / BLOCK, g, s);
+        LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, F->P);
+        if (int rc = eddyvis_tail(F)) return rc;
+        for (int n = 0; n < 3; n++) {
+            F->rmom[n] = F->h_sc[n].rmom;
+            F->kry[n] += F->h_sc[n].iters;
+        }
+        check_conv(F);
+        if (F->iconv == 0) return set_err(CMS2D_ERR_DIVERGED, "Model is divergent (check_conv iconv=0)");
+        if (F->iconv == 2) break;
+    }
+    if (F->niter > F->maxit) F->niter = F->maxit;
+    LAUNCH(k_velface, F->nblk_act, g, s, F->P);
+    return 0;
+}
+
+static void fill_geo(Geo &g, int nx, int ny, double dx, double dy, const int *side) {
+    g.nx = nx; g.ny = ny; g.nxp = nx + 2; g.nyp = ny + 2; g.nact = nx * ny;
+    g.dx = dx; g.dy = dy; g.area = dx * dy;
+    for (int k = 0; k < 4; k++) {
+        bool ns = (k == KN || k == KS);
+        g.ds[k] = ns ? dx : dy;                       // min(dx,dx_nb) on a uniform grid
+        g.dn[k] = ns ? dy : dx;                       // |y_nb - y|, |x_nb - x|
+        g.dsxy[k] = g.ds[k] / g.dn[k];
+        g.fnx[k] = k == KE ? 1.0 : k == KW ? -1.0 : 0.0;
+        g.fny[k] = k == KN ? 1.0 : k == KS ? -1.0 : 0.0;
+        g.fintp[k] = ns ? dy / (dy + dy) : dx / (dx + dx);
+        g.rx[k] = k == KE ? dx / 2.0 : k == KW ? -dx / 2.0 : 0.0;
+        g.ry[k] = k == KN ? dy / 2.0 : k == KS ? -dy / 2.0 : 0.0;
+        g.dsx[k] = g.fnx[k] * g.ds[k];
+        g.dsy[k] = g.fny[k] * g.ds[k];
+        g.side[k] = side[k];
+    }
+}
+
+static int upload_padded(cms2d_flow *F, double *dst, const double *src, bool ghosts_copy) {
+    const Geo &g = F->g;
+    std::vector<double> h((size_t)g.nxp * g.nyp, 0.0);
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) h[(size_t)(j + 1) * g.nxp + (i + 1)] = src ? src[(size_t)j * g.nx + i] : 0.0;
+    if (ghosts_copy) {
+        for (int i = 0; i < g.nx; i++) {
+            h[(size_t)0 * g.nxp + i + 1] = h[(size_t)1 * g.nxp + i + 1];
+            h[(size_t)(g.ny + 1) * g.nxp + i + 1] = h[(size_t)g.ny * g.nxp + i + 1];
+        }
+        for (int j = 0; j < g.ny; j++) {
+            h[(size_t)(j + 1) * g.nxp + 0] = h[(size_t)(j + 1) * g.nxp + 1];
+            h[(size_t)(j + 1) * g.nxp + g.nx + 1] = h[(size_t)(j + 1) * g.nxp + g.nx];
+        }
+    }
+    HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
+    if (!F) return CMS2D_OK;
+    (void)hipSetDevice(F->dev);
+    if (F->st) (void)hipStreamSynchronize(F->st);
+    for (void *p : F->allocs) (void)hipFree(p);
+    if (F->h_sc) (void)hipHostFree(F->h_sc);
+    if (F->h_flags) (void)hipHostFree(F->h_flags);
+    for (auto e : F->ev) (void)hipEventDestroy(e);
+    for (auto e : F->step_ev) if (e) (void)hipEventDestroy(e);
+    if (F->st) (void)hipStreamDestroy(F->st);
+    delete F;
+    return CMS2D_OK;
+}
+
+static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
+    if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
+        d->ntc > 8 || !(d->dt > 0))
+        return set_err(CMS2D_ERR_ARG, "invalid flow description");
+    if ((long)(d->nx + 2) * (d->ny + 2) > (1L << 30)) return set_err(CMS2D_ERR_ARG, "grid too large");
+    for (int k = 0; k < 4; k++)
+        if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC)
+            return set_err(CMS2D_ERR_ARG, "unknown boundary type");
+    if (d->precision != CMS2D_PREC_FP64) return set_err(CMS2D_ERR_ARG, "precision form not available");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device)
+        return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
+    F->dev = d->device;
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamCreate(&F->st));
+    fill_geo(F->g, d->nx, d->ny, d->dx, d->dy, d->side_type);
+    Geo &g = F->g;
+    F->npad = g.nxp * g.nyp;
+    F->nblk_act = (g.nact + BLOCK - 1) / BLOCK;
+    F->nblk_pad = (F->npad + BLOCK - 1) / BLOCK;
+    Par &P = F->P;
+    P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
+    P.hmin = P.hdry;                                  // flow.F90:690-694
+    P.hdry1 = F4(5.0) * P.hdry;
+    P.hdry2 = F4(2.0) * P.hdry;
+    P.dtime = d->dt;
+    double asum = 0.0;                                // areaavg, geospatial.F90:1441
+    for (int c = 0; c < g.nact; c++) asum += g.area;
+    double areaavg = asum / (double)(float)g.nact;
+    P.c2a = F4(0.2) * F4(0.2) * areaavg;              // cvishor2areaavg, flow.F90:749
+    P.rtol = d->rtol > 0 ? d->rtol : 1e-10;
+    P.atol = d->atol >= 0 ? d->atol : 1e-12;
+    P.kmax = d->kmax > 0 ? d->kmax : 4000;
+    P.noptset = d->noptset;
+    P.wallfric = d->wallfric;
+    P.ndsch = d->ndsch;
+    P.lmax = std::max(g.nx, g.ny);
+    F->maxit = d->maxit > 0 ? d->maxit : 20 + g.nact / 10000;   // flow.F90:784
+    F->rampdur = std::max(d->rampdur, d->dt / 3600.0);
+    F->ntc = d->ntc;
+    for (int k = 0; k < d->ntc; k++) { F->amp[k] = d->amp[k]; F->speed[k] = d->speed[k]; F->phase[k] = d->phase[k]; }
+    for (int n = 0; n < 3; n++) { F->rmom0[n] = 1e10; F->rmom[n] = 0.0; }
+
+    St &s = F->s;
+    const size_t N = F->npad;
+    double **dfields[] = {&s.u, &s.v, &s.p, &s.h, &s.pp, &s.zb, &s.vis, &s.visb, &s.visc, &s.u1, &s.v1, &s.p1,
+                          &s.h1, &s.eta, &s.dppx, &s.dppy, &s.dux, &s.duy, &s.dvx, &s.dvy, &s.dpx, &s.dpy,
+                          &s.coefman, &s.cfrict, &s.z0, &s.uv, &s.uelwc, &s.bsvel, &s.su, &s.sv, &s.sp, &s.spu,
+                          &s.spv, &s.ssu0, &s.ssv0, &s.spuv0, &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu,
+                          &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
+                          &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
+                          &s.tu, &s.tv, &s.tvis};
+    for (double **f : dfields)
+        if (int rc = dalloc(F, f, N)) return rc;
+    if (int rc = dalloc(F, &s.wseadj, 4 * (size_t)P.lmax)) return rc;
+    uint8_t **bfields[] = {&s.iwet, &s.iwet1, &s.icorner, &s.ipond};
+    for (uint8_t **f : bfields)
+        if (int rc = dalloc(F, f, N)) return rc;
+    if (int rc = dalloc(F, &s.flags, 4)) return rc;
+    if (int rc = dalloc(F, &s.partials, (size_t)F->nblk_act * 8 + 64)) return rc;
+    if (int rc = dalloc(F, &s.counter, 1)) return rc;
+    if (int rc = dalloc(F, &F->d_sc, 3)) return rc;
+    // Krylov vectors
+    {
+        double *kv[3][6];
+        for (int q = 0; q < 3; q++)
+            for (int m = 0; m < 6; m++)
+                if (int rc = dalloc(F, &kv[q][m], N)) return rc;
+        F->Kpp.x[0] = s.pp;
+        F->Kpp.r[0] = kv[0][0]; F->Kpp.r0[0] = kv[0][1]; F->Kpp.p[0] = kv[0][2];
+        F->Kpp.v[0] = kv[0][3]; F->Kpp.t[0] = kv[0][4]; F->Kpp.ap_w[0] = kv[0][5];
+        F->Kpp.sc = F->d_sc;
+        for (int q = 0; q < 2; q++) {
+            F->Kuv.x[q] = q ? s.v : s.u;
+            F->Kuv.r[q] = kv[q + 1][0]; F->Kuv.r0[q] = kv[q + 1][1]; F->Kuv.p[q] = kv[q + 1][2];
+            F->Kuv.v[q] = kv[q + 1][3]; F->Kuv.t[q] = kv[q + 1][4]; F->Kuv.ap_w[q] = kv[q + 1][5];
+        }
+        F->Kuv.sc = F->d_sc + 1;
+    }
+    HIPCHK(hipHostMalloc((void **)&F->h_sc, 3 * sizeof(KryScal), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
+    memset(F->h_sc, 0, 3 * sizeof(KryScal));
+    F->ev.resize(2 * 64);
+    for (auto &e : F->ev) HIPCHK(hipEventCreate(&e));
+    for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
+    // inputs
+    if (int rc = upload_padded(F, s.zb, d->zb, true)) return rc;
+    if (int rc = upload_padded(F, s.coefman, d->mann, false)) return rc;
+    if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
+    if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
+    // flow_init (flow.F90:688-707) and prestart (prestart.f90:309-329)
+    if (int rc = wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.u1, s.u, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.v1, s.v, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.p1, s.p, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.h1, s.h, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    if (int rc = wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.iwet1, s.iwet, N, hipMemcpyDeviceToDevice, F->st));
+    LAUNCH(k_zbk, F->nblk_act, g, s);
+    LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, P);
+    // flow_convflux with u = v = 0 leaves every flux at zero
+    if (int rc = eddyvis_tail(F)) return rc;
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+extern "C" int cms2d_flow_create(const cms2d_flow_desc *d, cms2d_flow **out) {
+    if (!d || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    *out = nullptr;
+    cms2d_flow *F = new (std::nothrow) cms2d_flow();
+    if (!F) return set_err(CMS2D_ERR_ARG, "out of host memory");
+    int rc = create_impl(d, F);
+    if (rc != 0) {
+        std::string keep = g_err;
+        cms2d_flow_destroy(F);
+        g_err = keep;
+        return rc;
+    }
+    *out = F;
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {
+    if (!F || nsteps < 0) return set_err(CMS2D_ERR_ARG, "invalid argument");
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipEventRecord(F->step_ev[0], F->st));
+    for (int n = 0; n < nsteps; n++) {
+        // timing, CMS_Flow.f90:109-240 (fixed dt)
+        F->ntime++;
+        F->timesecs += F->P.dtime;
+        if (int rc = flow_imp(F)) return rc;
+        LAUNCH(k_update, F->nblk_pad, F->g, F->s);
+    }
+    HIPCHK(hipEventRecord(F->step_ev[1], F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    if (info) {
+        info->niter = F->niter;
+        info->iconv = F->iconv;
+        for (int n = 0; n < 3; n++) { info->rmom[n] = F->rmom[n]; info->kry_iters[n] = F->kry[n]; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, F->step_ev[0], F->step_ev[1]);
+        info->kernel_ms = ms;
+        std::vector<uint8_t> w(F->npad);
+        HIPCHK(hipMemcpy(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost));
+        int nw = 0;
+        for (int j = 0; j < F->g.ny; j++)
+            for (int i = 0; i < F->g.nx; i++) nw += w[(size_t)(j + 1) * F->g.nxp + i + 1];
+        info->nwet = nw;
+    }
+    return CMS2D_OK;
+}
+
+static double *field_ptr(cms2d_flow *F, const char *name) {
+    St &s = F->s;
+    struct { const char *n; double *p; } tab[] = {
+        {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
+        {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy}, {"z0", s.z0},
+        {"zb", s.zb}};
+    for (auto &t : tab)
+        if (!strcmp(t.n, name)) return t.p;
+    return nullptr;
+}
+
+extern "C" int cms2d_flow_get(cms2d_flow *F, const char *name, double *out) {
+    if (!F || !name || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const Geo &g = F->g;
+    std::vector<double> h(F->npad);
+    if (!strcmp(name, "iwet")) {
+        std::vector<uint8_t> w(F->npad);
+        HIPCHK(hipMemcpy(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost));
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) out[(size_t)j * g.nx + i] = w[(size_t)(j + 1) * g.nxp + i + 1];
+        return CMS2D_OK;
+    }
+    double *p = field_ptr(F, name);
+    if (!p) return set_err(CMS2D_ERR_ARG, std::string("unknown field ") + name);
+    HIPCHK(hipMemcpy(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) out[(size_t)j * g.nx + i] = h[(size_t)(j + 1) * g.nxp + i + 1];
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_flow_set(cms2d_flow *F, const char *name, const double *in) {
+    if (!F || !name || !in) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (strcmp(name, "u") && strcmp(name, "v") && strcmp(name, "p"))
+        return set_err(CMS2D_ERR_ARG, std::string("field not settable: ") + name);
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const Geo &g = F->g;
+    double *p = field_ptr(F, name);
+    std::vector<double> h(F->npad);
+    HIPCHK(hipMemcpy(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) h[(size_t)(j + 1) * g.nxp + i + 1] = in[(size_t)j * g.nx + i];
+    HIPCHK(hipMemcpy(p, h.data(), F->npad * sizeof(double), hipMemcpyHostToDevice));
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_flow_get_face(cms2d_flow *F, const char *name, double *out) {
+    if (!F || !name || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const Geo &g = F->g;
+    const double *E, *Nn;
+    bool anti;
+    if (!strcmp(name, "flux")) { E = F->s.fluxE; Nn = F->s.fluxN; anti = true; }
+    else if (!strcmp(name, "hk")) { E = F->s.hkE; Nn = F->s.hkN; anti = false; }
+    else if (!strcmp(name, "visk")) { E = F->s.viskE; Nn = F->s.viskN; anti = false; }
+    else return set_err(CMS2D_ERR_ARG, std::string("unknown face field ") + name);
+    std::vector<double> he(F->npad), hn(F->npad);
+    HIPCHK(hipMemcpy(he.data(), E, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hn.data(), Nn, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    const double sg = anti ? -1.0 : 1.0;
+    for (int j = 0; j < g.ny; j++)
+        for (int i = 0; i < g.nx; i++) {
+            size_t c = (size_t)(j + 1) * g.nxp + i + 1, o = 4 * ((size_t)j * g.nx + i);
+            out[o + KN] = hn[c];
+            out[o + KE] = he[c];
+            out[o + KS] = sg * hn[c - g.nxp];
+            out[o + KW] = sg * he[c - 1];
+        }
+    return CMS2D_OK;
+}
+
+extern "C" double cms2d_flow_time(const cms2d_flow *F) { return F ? F->timesecs : 0.0; }
+
+extern "C" int cms2d_flow_kernel_stats(const cms2d_flow *F, const char *name, double *avg_ms, long *count) {
+    if (!F || !name || !avg_ms || !count) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (strcmp(name, "spmv_pp")) return set_err(CMS2D_ERR_ARG, std::string("unknown kernel group ") + name);
+    *count = F->spmv_n;
+    *avg_ms = F->spmv_n ? F->spmv_ms / F->spmv_n : 0.0;
+    return CMS2D_OK;
+}
+
+// ----------------------------------------------------------------------------
+// cms2d_solve: solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
+extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const int *cell2cell, const int *iwet,
+                           double *ss, const double *sp, double *res, double *phi, double relax, double rtol,
+                           double atol, int kmax, double *rmom, int *iters) {
+    if (ncells <= 0 || ncellsD < ncells || !acoef || !cell2cell || !iwet || !ss || !sp || !res || !phi ||
+        !(relax > 0.0))
+        return set_err(CMS2D_ERR_ARG, "invalid argument");
+    for (long f = 0; f < 4L * ncells; f++)
+        if (cell2cell[f] < -1 || cell2cell[f] >= ncellsD) return set_err(CMS2D_ERR_ARG, "cell2cell out of range");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
+    std::vector<void *> mem;
+    auto cleanup = [&]() { for (void *p : mem) (void)hipFree(p); };
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return p;
+    };
+    const size_t nD = ncellsD, n = ncells;
+    int nblk = (ncells + BLOCK - 1) / BLOCK;
+    double *d_a = (double *)alloc(4 * nD * sizeof(double));
+    int *d_c2c = (int *)alloc(4 * nD * sizeof(int));
+    int *d_w = (int *)alloc(nD * sizeof(int));
+    double *d_ss = (double *)alloc(nD * sizeof(double)), *d_sp = (double *)alloc(nD * sizeof(double));
+    double *d_res = (double *)alloc(nD * sizeof(double)), *d_phi = (double *)alloc(nD * sizeof(double));
+    double *vec = (double *)alloc(6 * n * sizeof(double));
+    double *partials = (double *)alloc(((size_t)nblk * 8 + 64) * sizeof(double));
+    unsigned *counter = (unsigned *)alloc(sizeof(unsigned));
+    KryScal *d_sc = (KryScal *)alloc(sizeof(KryScal));
+    KryScal h_sc{};
+    if (!d_a || !d_c2c || !d_w || !d_ss || !d_sp || !d_res || !d_phi || !vec || !partials || !counter || !d_sc) {
+        cleanup();
+        return set_err(CMS2D_ERR_HIP, "device allocation failed");
+    }
+    hipStream_t st = nullptr;
+    int rc = 0;
+    do {
+        if (hipStreamCreate(&st) != hipSuccess) { rc = set_err(CMS2D_ERR_HIP, "stream"); break; }
+        bool ok = hipMemcpyAsync(d_a, acoef, 4 * nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_c2c, cell2cell, 4 * nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_w, iwet, nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_phi, phi, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemsetAsync(counter, 0, sizeof(unsigned), st) == hipSuccess;
+        if (!ok) { rc = set_err(CMS2D_ERR_HIP, "upload failed"); break; }
+        KrySys<1> K{};
+        K.x[0] = d_phi;
+        K.r[0] = vec; K.r0[0] = vec + n; K.p[0] = vec + 2 * n; K.v[0] = vec + 3 * n; K.t[0] = vec + 4 * n;
+        K.ap_w[0] = vec + 5 * n;
+        K.sc = d_sc;
+        Par P{};
+        P.rtol = rtol > 0 ? rtol : 1e-10;
+        P.atol = atol >= 0 ? atol : 1e-12;
+        P.kmax = kmax > 0 ? kmax : 4000;
+        EllOp A{ncells, d_c2c, d_a};
+        hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(BLOCK), 0, st, A, (const int *)d_w, d_ss,
+                           (const double *)d_sp, d_res, (const double *)d_phi, relax, P, K, partials, counter);
+        int prev = 8;
+        KryCtx X{st, nblk, P.kmax, partials, counter, d_sc, &h_sc, &prev, nullptr, nullptr, nullptr};
+        X.prev_iters = &prev;
+        // run_bicg indexes scalar mirrors from sc0=0
+        if ((rc = run_bicg(X, K, A, 0))) break;
+        ok = hipMemcpyAsync(ss, d_ss, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipMemcpyAsync(res, d_res, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipMemcpyAsync(phi, d_phi, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) { rc = set_err(CMS2D_ERR_HIP, "download failed"); break; }
+        if (rmom) *rmom = h_sc.rmom;
+        if (iters) *iters = h_sc.iters;
+    } while (0);
+    if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
+    cleanup();
+    return rc;
+}

@@ -1,0 +1,843 @@
+// flow_kernels.hip -- gfx950 kernels of CMS-Flow's implicit hydrodynamic step
+// (erdc/cms2d 5.4, source/flow/flow_imp.F90:29-312) on a uniform Cartesian grid.
+//
+// Layout: every cell field is a padded row-major array over the active cells
+// plus a one-cell ghost ring (common.h).  The reference's ghost cells (one per
+// missing neighbour, geospatial.F90:2877-2930) are exactly the ring cells.
+// Quantities that the reference stores twice per face with the same value
+// (hk, visk, pk, zbk) or opposite values (flux) are stored once per face:
+// xE[c] is the face between c and c+1, xN[c] the face between c and c+nxp.
+// Matrix coefficients are directed (aN,aE,aS,aW per cell) as in acoef(k,i).
+//
+// Arithmetic follows the reference expression by expression (build with
+// -ffp-contract=off) so that the non-iterative kernels reproduce the FP64
+// oracle bit for bit; the Krylov solves agree to their stopping tolerance.
+#include "flow_state.h"
+
+namespace cms {
+
+#define SMALL F4(1.0e-15)
+#define RELAX_UV F4(0.8)
+#define FACPP F4(0.2)
+#define CVISCON F4(1.0e-6)
+#define CVISBOT F4(0.0667)
+#define CVISMAX 20.0
+#define CVISWAV 0.5
+#define CVISWAVBRK F4(0.1)
+#define CFRICWAV F4(0.65)
+#define RHOW 1025.0
+
+__device__ __constant__ int c_kopp[4] = {KS, KW, KN, KE};
+
+// ---------------------------------------------------------------- helpers
+// comp_lib.f90:54-109
+__device__ __forceinline__ double schmcoef(int ndsch, double dk, double fk) {
+    double absfk = fabs(fk), peck;
+    switch (ndsch) {
+    case 2:
+        peck = absfk / fmax(dk, F4(1.0e-6));
+        return dk * fmax(0.0, 1.0 - 0.5 * peck) + fmax(0.0, -fk);
+    case 3:
+        peck = absfk / fmax(fmax(dk, absfk), F4(1.0e-6));
+        return dk * (1.0 - pow(peck, 5.0)) + fmax(0.0, -fk);
+    case 4:
+        peck = absfk / fmax(fmax(dk, F4(1.e-6)), absfk);
+        return dk * peck / fmax(exp(peck) - 1.0, F4(1.0e-6)) + fmax(0.0, -fk);
+    default:
+        return dk + fmax(0.0, -fk);
+    }
+}
+// fric_lib.f90:162-194
+__device__ __forceinline__ double wall_coef(double yp, double y0) {
+    double y0lim = fmin(fmax(y0, F4(1.0e-6)), yp * F4(0.9999));
+    double t = F4(0.4) / log(yp / y0lim);
+    return t * t;
+}
+// fric_lib.f90:944-996
+__device__ __forceinline__ double man2drag(double cman, double h) {
+    double hh = fmax(fmin(10.0, h), F4(0.1) * cman);
+    return GRAV * cman * cman / pow(hh, F4(0.3333));
+}
+__device__ __forceinline__ double drag2length(double cd, double h) {
+    double z0 = h / exp(F4(0.4) / sqrt(cd) + 1.0);
+    return fmax(z0, F4(1.0e-6));
+}
+
+// directed face values of cell c
+__device__ __forceinline__ double fdir(const double *E, const double *N, const Geo &g, int c, int k) {
+    return k == KN ? N[c] : k == KE ? E[c] : k == KS ? -N[c - g.nxp] : -E[c - 1];
+}
+__device__ __forceinline__ double fsym(const double *E, const double *N, const Geo &g, int c, int k) {
+    return k == KN ? N[c] : k == KE ? E[c] : k == KS ? N[c - g.nxp] : E[c - 1];
+}
+// is the neighbour of active cell (i,j) in direction k a ghost (ring) cell?
+__device__ __forceinline__ bool nb_ghost(const Geo &g, int i, int j, int k) {
+    return k == KN ? j == g.ny - 1 : k == KE ? i == g.nx - 1 : k == KS ? j == 0 : i == 0;
+}
+// Ring cell -> side it lies on (-1 for corners and active cells).
+__device__ __forceinline__ int ring_side(const Geo &g, int i, int j) {
+    bool ci = (i == -1 || i == g.nx), cj = (j == -1 || j == g.ny);
+    if (ci && cj) return -1;
+    if (j == g.ny) return KN;
+    if (i == g.nx) return KE;
+    if (j == -1) return KS;
+    if (i == -1) return KW;
+    return -1;
+}
+__device__ __forceinline__ void pad_ij(const Geo &g, int t, int &i, int &j) {
+    j = t / g.nxp - 1;
+    i = t - (j + 1) * g.nxp - 1;
+}
+
+// Cell-based Green-Gauss gradient with wet-only stencils (der_grad_cbgg with
+// gow%ibc=1, der_lib.f90:272-378, screened and re-centred as :360-376), then
+// der_grad_eval (:726-839).  Weights are rebuilt from iwet on every call,
+// which is what der_update keeps them equal to (derivatives.f90:204-296).
+__device__ __forceinline__ void cbgg_grad(const Geo &g, const uint8_t *iwet, int c, int nv,
+                                          const double *const *var, double *gx, double *gy) {
+    if (!iwet[c]) {
+        for (int q = 0; q < nv; q++) { gx[q] = 0.0; gy[q] = 0.0; }
+        return;
+    }
+    double wx[5] = {0, 0, 0, 0, 0}, wy[5] = {0, 0, 0, 0, 0};
+    int ix[5];
+    ix[0] = c;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int nb = nbr(g, c, k);
+        ix[k + 1] = nb;
+        if (iwet[nb] == 1) {
+            wx[0] += g.dsx[k] * (1.0 - g.fintp[k]);
+            wx[k + 1] = g.dsx[k] * g.fintp[k];
+            wy[0] += g.dsy[k] * (1.0 - g.fintp[k]);
+            wy[k + 1] = g.dsy[k] * g.fintp[k];
+        } else {
+            wx[0] += g.dsx[k];
+            wy[0] += g.dsy[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) { wx[k] /= g.area; wy[k] /= g.area; }
+    const double tolw = F4(1.0e-8);
+    int nx_ = 0, ny_ = 0;
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        if (fabs(wx[k]) > tolw) { nx_++; sx += wx[k]; }
+        if (fabs(wy[k]) > tolw) { ny_++; sy += wy[k]; }
+    }
+    double mx = nx_ ? sx / (double)nx_ : 0.0, my = ny_ ? sy / (double)ny_ : 0.0;
+    for (int q = 0; q < nv; q++) {
+        double ax = 0.0, ay = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            if (fabs(wx[k]) > tolw) ax += (wx[k] - mx) * var[q][ix[k]];
+            if (fabs(wy[k]) > tolw) ay += (wy[k] - my) * var[q][ix[k]];
+        }
+        gx[q] = ax;
+        gy[q] = ay;
+    }
+}
+
+// ------------------------------------------------------- wetting and drying
+// flow_wetdry(0), flow.F90:1257-1292: depth and wet flag on every cell.
+__global__ void k_wetdry_h(Geo g, St s, Par P) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    int i, j;
+    pad_ij(g, t, i, j);
+    double h = s.p[t] * GRAVINV - s.zb[t];
+    uint8_t w = 1;
+    if (h < P.hdry) { h = fmax(h, P.hmin); w = 0; }
+    if (!active_ij(g, i, j)) {
+        int sd = ring_side(g, i, j);
+        if (sd < 0 || g.side[sd] == 0) w = 0;     // corner or wall ghost
+    }
+    s.h[t] = h;
+    s.iwet[t] = w;
+}
+
+// Narrow-channel removal, flow.F90:1342-1455.  Every rule dries a cell when
+// fewer of its neighbours are wet, and cells only ever dry, so any order of
+// in-place updates repeated until a pass changes nothing reaches the same
+// fixed point as the reference's sequential sweep.
+__global__ void k_narrow(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (!s.iwet[c]) return;
+    const uint8_t *w = s.iwet;
+    int n = w[c + g.nxp], e = w[c + 1], so = w[c - g.nxp], we = w[c - 1];
+    double h = s.h[c];
+    bool dry = false;
+    if (n + e + so + we <= 1) dry = true;
+    if (((n + so) == 0 || (e + we) == 0) && h < P.hdry1) dry = true;
+    int ss2 = j > 0 ? w[c - 2 * g.nxp] : 0;
+    int ww2 = i > 0 ? w[c - 2] : 0;
+    int ee2 = i < g.nx - 1 ? w[c + 2] : 0;
+    if (n + ss2 == 0 && h < P.hdry2) dry = true;     // knss (tested twice in the reference)
+    if (e + ww2 == 0 && h < P.hdry2) dry = true;     // keww
+    if (we + ee2 == 0 && h < P.hdry2) dry = true;    // keew
+    if (dry) {
+        s.iwet[c] = 0;
+        s.flags[0] = 1;
+    }
+}
+
+// flow_pond (flow.F90:1807-1876): wet cells not connected through wet active
+// cells to a forcing-boundary cell are dried.  Seeds: boundary-string cells
+// and their ghosts.
+__global__ void k_pond_seed(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    int i, j;
+    pad_ij(g, t, i, j);
+    uint8_t v = 0;
+    if (active_ij(g, i, j)) {
+        if ((j == g.ny - 1 && g.side[KN]) || (i == g.nx - 1 && g.side[KE]) || (j == 0 && g.side[KS]) ||
+            (i == 0 && g.side[KW]))
+            v = s.iwet[t];
+    } else {
+        int sd = ring_side(g, i, j);
+        if (sd >= 0 && g.side[sd]) v = s.iwet[t];
+    }
+    s.ipond[t] = v;
+}
+
+// Reachability propagated inside a 64x16 tile held in LDS until the tile is
+// stable; repeated over the grid until no tile changes.
+constexpr int PT_X = 64, PT_Y = 16;
+__global__ void __launch_bounds__(256) k_pond_tile(Geo g, St s) {
+    __shared__ uint8_t sw[PT_Y + 2][PT_X + 2], sp[PT_Y + 2][PT_X + 2];
+    const int tx0 = blockIdx.x * PT_X - 1, ty0 = blockIdx.y * PT_Y - 1;   // padded coords of halo origin
+    for (int q = threadIdx.x; q < (PT_X + 2) * (PT_Y + 2); q += blockDim.x) {
+        int ly = q / (PT_X + 2), lx = q - ly * (PT_X + 2);
+        int X = tx0 + lx, Y = ty0 + ly;
+        uint8_t w = 0, pv = 0;
+        if (X >= 0 && X < g.nxp && Y >= 0 && Y < g.nyp) {
+            w = s.iwet[Y * g.nxp + X];
+            pv = s.ipond[Y * g.nxp + X];
+        }
+        sw[ly][lx] = w;
+        sp[ly][lx] = pv;
+    }
+    __syncthreads();
+    int changed_any = 0;
+    for (;;) {
+        int changed = 0;
+        for (int q = threadIdx.x; q < PT_X * PT_Y; q += blockDim.x) {
+            int ly = q / PT_X + 1, lx = q % PT_X + 1;
+            int X = tx0 + lx, Y = ty0 + ly;
+            if (X >= g.nxp || Y >= g.nyp) continue;
+            if (!sw[ly][lx] || sp[ly][lx]) continue;
+            // a reached ACTIVE neighbour propagates to this cell
+            bool hit = false;
+            const int dxs[4] = {0, 1, 0, -1}, dys[4] = {1, 0, -1, 0};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int nxq = X + dxs[k], nyq = Y + dys[k];
+                int ni = nxq - 1, nj = nyq - 1;
+                if (active_ij(g, ni, nj) && sp[ly + dys[k]][lx + dxs[k]]) hit = true;
+            }
+            if (hit) { sp[ly][lx] = 1; changed = 1; }
+        }
+        changed = __syncthreads_or(changed);
+        if (!changed) break;
+        changed_any = 1;
+    }
+    if (changed_any) {
+        for (int q = threadIdx.x; q < PT_X * PT_Y; q += blockDim.x) {
+            int ly = q / PT_X + 1, lx = q % PT_X + 1;
+            int X = tx0 + lx, Y = ty0 + ly;
+            if (X >= g.nxp || Y >= g.nyp) continue;
+            s.ipond[Y * g.nxp + X] = sp[ly][lx];
+        }
+        if (threadIdx.x == 0) s.flags[1] = 1;
+    }
+}
+
+__global__ void k_pond_final(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    if (s.ipond[t] == 0 && s.iwet[t] == 1) s.iwet[t] = 0;
+}
+
+// Corner flags, newly wet count, and zero flux/visk on faces not wet on both
+// sides (flow.F90:1459-1525).
+__global__ void k_wetdry_post(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const uint8_t *w = s.iwet;
+    uint8_t ic = 0;
+    if (w[c]) {
+        int nd = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) nd += w[nbr(g, c, k)] == 0;
+        if (nd > 1) ic = 1;
+    }
+    s.icorner[c] = ic;
+    if (w[c] == 1 && s.iwet1[c] == 0) atomicAdd(&s.flags[2], 1);
+    if (w[c] * w[c + 1] == 0) { s.fluxE[c] = 0.0; s.viskE[c] = 0.0; }
+    if (w[c] * w[c + g.nxp] == 0) { s.fluxN[c] = 0.0; s.viskN[c] = 0.0; }
+    if (i == 0 && w[c] * w[c - 1] == 0) { s.fluxE[c - 1] = 0.0; s.viskE[c - 1] = 0.0; }
+    if (j == 0 && w[c] * w[c - g.nxp] == 0) { s.fluxN[c - g.nxp] = 0.0; s.viskN[c - g.nxp] = 0.0; }
+}
+
+// Dry-to-wet initialisation (flow.F90:1477-1491), one sweep, every read from
+// the values before the sweep.
+__global__ void k_d2w_sweep(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (!(s.iwet[c] == 1 && s.iwet1[c] == 0)) return;
+    double us = 0.0, vs = 0.0, viss = 0.0;
+    int ns = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int nb = nbr(g, c, k);
+        us += s.u[nb]; vs += s.v[nb]; viss += s.vis[nb]; ns += s.iwet[nb];
+    }
+    double den = (double)(float)ns + SMALL;
+    s.tu[c] = us / den / F4(1.25);
+    s.tv[c] = vs / den / F4(1.25);
+    s.tvis[c] = viss / den;
+}
+__global__ void k_d2w_commit(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (!(s.iwet[c] == 1 && s.iwet1[c] == 0)) return;
+    s.u[c] = s.tu[c]; s.v[c] = s.tv[c]; s.vis[c] = s.tvis[c];
+}
+
+// ------------------------------------------------------------ momentum
+// flow_srcsnk_static, flow_imp.F90:336-403 (ntsch=1, wave forcing when noptset>=3)
+__global__ void k_srcsnk(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    double gdi = 1.0 / (GRAV * P.dtime), dti = 1.0 / P.dtime;
+    double fac1 = g.area * s.h1[c] * dti;
+    s.spuv0[c] = -fac1;
+    double su0 = fac1 * s.u1[c], sv0 = fac1 * s.v1[c];
+    s.sppp0[c] = -g.area * gdi;
+    s.sspp0[c] = -s.p1[c] * s.sppp0[c];
+    if (P.noptset >= 3) {
+        su0 = su0 + s.wavestrx[c] * g.area;
+        sv0 = sv0 + s.wavestry[c] * g.area;
+    }
+    s.ssu0[c] = su0;
+    s.ssv0[c] = sv0;
+}
+
+// coeffsourcesink_uv (flow_imp.F90:444-566) + bound_uv (boundaries.F90:3310-3490)
+// with bnd_vel_openwse (bnd_vel.f90:35-95) for harmonic water-level sides.
+__global__ void k_mom(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const double A = g.area, fc = 0.0;
+    double u = s.u[c], v = s.v[c], h = s.h[c];
+    double su = s.ssu0[c] + (fc * v - s.dpx[c]) * h * A;
+    double sv = s.ssv0[c] + (-fc * u - s.dpy[c]) * h * A;
+    double cbc = 1.0 * s.cfrict[c] * s.uelwc[c] * A;          // cmb=1 (no bed-slope factor)
+    double sp = s.spuv0[c] - cbc;
+    double a[4], hk[4], fl[4], vk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        hk[k] = fsym(s.hkE, s.hkN, g, c, k);
+        vk[k] = fsym(s.viskE, s.viskN, g, c, k);
+        fl[k] = fdir(s.fluxE, s.fluxN, g, c, k);
+        double ddk = vk[k] * hk[k] * g.dsxy[k];
+        a[k] = schmcoef(P.ndsch, ddk, fl[k]);
+    }
+    const int wet = s.iwet[c];
+    // harmonic water-level sides: fold the boundary face into su/sv/sp
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!(nb_ghost(g, i, j, k) && g.side[k] == 1)) continue;
+        int gc = nbr(g, c, k);
+        double ddk = vk[k] * hk[k] * g.dsxy[k];
+        double ab = schmcoef(1, ddk, fl[k]);
+        if (fabs(g.fny[k]) > F4(0.9999)) {
+            s.u[gc] = wet * u * h / s.h[gc];
+            double vg = g.fny[k] * fl[k] / (g.ds[k] * hk[k]);
+            s.v[gc] = vg;
+            sv += ab * vg;
+        } else {
+            double ug = g.fnx[k] * fl[k] / (g.ds[k] * hk[k]);
+            s.u[gc] = ug;
+            s.v[gc] = wet * v * h / s.h[gc];
+            su += ab * ug;
+        }
+        sp -= ab;
+        a[k] = 0.0;
+    }
+    double spu, spv;
+    if (P.wallfric) {
+        if (wet == 1) {
+            spu = sp;
+            spv = sp;
+            double z0w = s.z0[c] * 1.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (s.iwet[nbr(g, c, k)] != 0) continue;
+                if (fabs(g.fny[k]) > F4(0.999))
+                    spu = spu - g.ds[k] * hk[k] * wall_coef(fabs(g.ry[k]), z0w) * fabs(u);
+                else
+                    spv = spv - g.ds[k] * hk[k] * wall_coef(fabs(g.rx[k]), z0w) * fabs(v);
+            }
+        } else {
+            s.u[c] = 0.0; s.v[c] = 0.0;
+            spu = -1.0; spv = -1.0; sp = -1.0;
+            su = 0.0; sv = 0.0;
+        }
+    } else {
+        s.u[c] = wet * u;
+        s.v[c] = wet * v;
+        sp = wet * sp + (double)(wet - 1);
+        spu = sp; spv = sp;
+        su = wet * su; sv = wet * sv;
+    }
+    s.su[c] = su; s.sv[c] = sv; s.sp[c] = sp; s.spu[c] = spu; s.spv[c] = spv;
+    s.aN[c] = a[KN]; s.aE[c] = a[KE]; s.aS[c] = a[KS]; s.aW[c] = a[KW];
+}
+
+// ------------------------------------------------------- pressure correction
+// coefsourcesink_pp, flow_imp.F90:608-621
+__global__ void k_pp_prep(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    double asum = ((s.aN[c] + s.aE[c]) + s.aS[c]) + s.aW[c];
+    int w = s.iwet[c];
+    double apu = w * RELAX_UV / fmax(asum - s.sp[c], F4(1.0e-6));
+    double apa = apu * g.area;
+    s.apuareap[c] = apa;
+    s.sumu[c] = asum * apu;
+    s.Hu[c] = w * (s.u[c] + apa * s.dpx[c] * s.h[c]);
+    s.Hv[c] = w * (s.v[c] + apa * s.dpy[c] * s.h[c]);
+}
+
+// pp coefficient of the face of `own` in direction k (flow_imp.F90:628-647)
+__device__ __forceinline__ double pp_coef(const Geo &g, const St &s, int own, int oi, int oj, int k) {
+    int nck = nbr(g, own, k);
+    double hk = fsym(s.hkE, s.hkN, g, own, k);
+    if (s.iwet[own] * s.iwet[nck] == 0) return 0.0;
+    if (nb_ghost(g, oi, oj, k))
+        return hk * hk * g.ds[k] * s.apuareap[own] / (1.0 - s.sumu[own]) / g.dn[k];
+    double fk = g.fintp[k], fp = 1.0 - fk;
+    double apf = fk * s.apuareap[nck] + fp * s.apuareap[own];
+    double suf = fk * s.sumu[nck] + fp * s.sumu[own];
+    return hk * hk * g.ds[k] * apf / (1.0 - suf) / g.dn[k];
+}
+// Rhie-Chow face flux of `own` in direction k (flow_imp_flux.f90:382-406)
+__device__ __forceinline__ double rc_flux(const Geo &g, const St &s, int own, int oi, int oj, int k) {
+    int nck = nbr(g, own, k);
+    if (s.iwet[own] * s.iwet[nck] == 0) return 0.0;
+    double hk = fsym(s.hkE, s.hkN, g, own, k);
+    if (nb_ghost(g, oi, oj, k))
+        return hk * g.ds[k] * ((g.fnx[k] * s.Hu[own] + g.fny[k] * s.Hv[own])
+                               - s.apuareap[own] * hk * (s.p[nck] - s.p[own]) / g.dn[k]);
+    double fk = g.fintp[k], fp = 1.0 - fk;
+    double apf = fk * s.apuareap[nck] + fp * s.apuareap[own];
+    double Huf = fk * s.Hu[nck] + fp * s.Hu[own];
+    double Hvf = fk * s.Hv[nck] + fp * s.Hv[own];
+    return hk * g.ds[k] * ((g.fnx[k] * Huf + g.fny[k] * Hvf) - apf * hk * (s.p[nck] - s.p[own]) / g.dn[k]);
+}
+
+__global__ void k_pp_coef_flux(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    // coefficients: each face evaluated from its owner (the cell that lists it
+    // first in kxyface: the S/W cell of an interior face, the active cell of a
+    // boundary face)
+    double aN = pp_coef(g, s, c, i, j, KN);
+    double aE = pp_coef(g, s, c, i, j, KE);
+    double aS = j > 0 ? pp_coef(g, s, c - g.nxp, i, j - 1, KN) : pp_coef(g, s, c, i, j, KS);
+    double aW = i > 0 ? pp_coef(g, s, c - 1, i - 1, j, KE) : pp_coef(g, s, c, i, j, KW);
+    s.aN[c] = aN; s.aE[c] = aE; s.aS[c] = aS; s.aW[c] = aW;
+    s.fluxN[c] = rc_flux(g, s, c, i, j, KN);
+    s.fluxE[c] = rc_flux(g, s, c, i, j, KE);
+    if (j == 0) s.fluxN[c - g.nxp] = -rc_flux(g, s, c, i, j, KS);
+    if (i == 0) s.fluxE[c - 1] = -rc_flux(g, s, c, i, j, KW);
+}
+
+// bnd_wse_adjust (boundaries.F90:4620-4686) for one harmonic side: setup from
+// the radiation-stress forcing integrated outward from the centre cell.
+__global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int side, double wsebnd) {
+    extern __shared__ double term[];
+    const int n = (side == KE || side == KW) ? g.ny : g.nx;
+    double *wse = s.wseadj + side * P.lmax;
+    const int im0 = n / 2 - 1;
+    const bool adjust = n >= 3;
+    for (int im = threadIdx.x; im < n; im += blockDim.x) {
+        double tv = 0.0;
+        if (adjust && im != im0) {
+            int im1 = im > im0 ? im - 1 : im + 1;
+            int ci, cj, ci1, cj1;
+            if (side == KE || side == KW) { ci = ci1 = (side == KE ? g.nx - 1 : 0); cj = im; cj1 = im1; }
+            else { cj = cj1 = (side == KN ? g.ny - 1 : 0); ci = im; ci1 = im1; }
+            int c = cidx(g, ci, cj), c1 = cidx(g, ci1, cj1);
+            double fx = 0.0, fx1 = 0.0, fy = 0.0, fy1 = 0.0;
+            if (P.noptset >= 3) { fx = s.wavestrx[c]; fx1 = s.wavestrx[c1]; fy = s.wavestry[c]; fy1 = s.wavestry[c1]; }
+            double xd = (ci - ci1) * g.dx, yd = (cj - cj1) * g.dy;
+            double sx = xd * (fx + fx1) / (s.h[c] + s.h[c1]) * GRAVINV;
+            double sy = yd * (fy + fy1) / (s.h[c] + s.h[c1]) * GRAVINV;
+            if (fabs(g.ry[side]) >= F4(0.999)) tv = sx;
+            else if (fabs(g.rx[side]) >= F4(0.999)) tv = sy;
+            else tv = sx + sy;
+        }
+        term[im] = tv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && adjust) {
+        // sequential prefix in the reference's order
+        term[im0] = 0.0;
+        for (int im = im0 + 1; im < n; im++) term[im] = term[im - 1] + term[im];
+        for (int im = im0 - 1; im >= 0; im--) term[im] = term[im + 1] + term[im];
+    }
+    __syncthreads();
+    for (int im = threadIdx.x; im < n; im += blockDim.x) wse[im] = adjust ? wsebnd + term[im] : wsebnd;
+}
+
+// pp sources (flow_imp.F90:700-712) + bound_pp (boundaries.F90:3493-3646) +
+// solve prologue for n=1 (solvers.f90:232-240, relax(1)=1) + BiCGStab start.
+__global__ void __launch_bounds__(BLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> K) {
+    int i = 0, j = 0, c = 0;
+    double red[3] = {0.0, 0.0, 0.0};
+    if (act_thread(g, i, j, c)) {
+        double sp = s.sppp0[c];
+        double fs = ((s.fluxN[c] + s.fluxE[c]) + (-s.fluxN[c - g.nxp])) + (-s.fluxE[c - 1]);
+        double su = s.sspp0[c] - fs + s.p[c] * sp;
+        double a[4] = {s.aN[c], s.aE[c], s.aS[c], s.aW[c]};
+        const int wet = s.iwet[c];
+        if (wet == 0) {
+            int sumwet = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) if (!nb_ghost(g, i, j, k)) sumwet += s.iwet[nbr(g, c, k)];
+            double aploc = 0.0;
+            su = 0.0;
+            if (sumwet >= 1) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    int nb = nbr(g, c, k);
+                    a[k] = s.iwet[nb];
+                    aploc += a[k];
+                    su += a[k] * s.p[nb];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    int nb = nbr(g, c, k);
+                    if (nb_ghost(g, i, j, k)) a[k] = 0.0;
+                    else { a[k] = g.dsxy[k]; aploc += a[k]; su += a[k] * s.p[nb]; }
+                }
+            }
+            su -= aploc * s.p[c];
+            sp = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!(nb_ghost(g, i, j, k) && g.side[k] == 1)) continue;
+            int gc = nbr(g, c, k);
+            int im = (k == KE || k == KW) ? j : i;
+            double wse = s.wseadj[k * P.lmax + im];
+            double ppg = wse * GRAV - s.p[gc];
+            s.pp[gc] = ppg;
+            su += a[k] * ppg;
+            sp -= a[k];
+            a[k] = 0.0;
+        }
+        s.aN[c] = a[KN]; s.aE[c] = a[KE]; s.aS[c] = a[KS]; s.aW[c] = a[KW];
+        s.sp[c] = sp;
+        // solve prologue, relax(1) = 1
+        double asum = ((a[0] + a[1]) + a[2]) + a[3];
+        double ap = (asum - sp) * 1.0;
+        double phi = s.pp[c];
+        double ss = su + 0.0 * phi * ap;
+        s.su[c] = ss;
+        double anb = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) anb += a[k] * s.pp[nbr(g, c, k)];
+        double res = wet * (anb + ss - ap * phi) / ap;
+        s.resp[c] = res;
+        double r = anb + ss - ap * phi + SMALL;
+        K.ap_w[0][c] = ap;
+        K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
+        red[0] = res * res;
+        red[1] = r * r;
+    }
+    if (grid_sum<3>(red, s.partials, s.counter) && threadIdx.x == 0)
+        kry_start(K.sc[0], red[0], red[1], g.nact, P);
+}
+
+// pvfc (flow_imp.F90:856-908): wall ghosts copy pp, gradient of pp, velocity
+// and face-flux corrections.
+__global__ void k_pvfc1(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (nb_ghost(g, i, j, k) && g.side[k] == 0) s.pp[nbr(g, c, k)] = s.pp[c];
+    const double *vars[1] = {s.pp};
+    double gx, gy;
+    cbgg_grad(g, s.iwet, c, 1, vars, &gx, &gy);
+    s.dppx[c] = gx;
+    s.dppy[c] = gy;
+    if (s.iwet[c] == 0) return;
+    double val = s.h[c] * s.apuareap[c] / (1.0 - s.sumu[c] + SMALL);
+    s.u[c] = s.u[c] - val * gx;
+    s.v[c] = s.v[c] - val * gy;
+    const double ppc = s.pp[c];
+    int nb = c + g.nxp;
+    if (s.iwet[nb]) s.fluxN[c] = s.fluxN[c] - s.aN[c] * (s.pp[nb] - ppc);
+    nb = c + 1;
+    if (s.iwet[nb]) s.fluxE[c] = s.fluxE[c] - s.aE[c] * (s.pp[nb] - ppc);
+    if (j == 0) {
+        nb = c - g.nxp;
+        if (s.iwet[nb]) { double f = -s.fluxN[nb]; f = f - s.aS[c] * (s.pp[nb] - ppc); s.fluxN[nb] = -f; }
+    }
+    if (i == 0) {
+        nb = c - 1;
+        if (s.iwet[nb]) { double f = -s.fluxE[nb]; f = f - s.aW[c] * (s.pp[nb] - ppc); s.fluxE[nb] = -f; }
+    }
+}
+
+// pvfc (flow_imp.F90:965-969) on every cell incl. ghosts
+__global__ void k_pvfc2(Geo g, St s, Par P) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    double pp = s.pp[t];
+    double p = s.p[t] + 1.0 * pp;
+    s.p[t] = p;
+    s.h[t] = fmax(P.hmin, p * GRAVINV - s.zb[t]);
+    s.pp[t] = FACPP * pp;
+}
+
+// pvfc (flow_imp.F90:972-980): wall ghosts take the pressure of their cell
+__global__ void k_ghost_p(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int nr = 2 * g.nx + 2 * g.ny;
+    if (t >= nr) return;
+    int i, j, k;
+    if (t < g.nx) { i = t; j = g.ny - 1; k = KN; }
+    else if (t < 2 * g.nx) { i = t - g.nx; j = 0; k = KS; }
+    else if (t < 2 * g.nx + g.ny) { i = g.nx - 1; j = t - 2 * g.nx; k = KE; }
+    else { i = 0; j = t - 2 * g.nx - g.ny; k = KW; }
+    if (g.side[k] != 0) return;
+    int c = cidx(g, i, j);
+    s.p[nbr(g, c, k)] = s.p[c];
+}
+
+// flow_grad_interp (flow_imp.F90:1106-1127) + fric_eval (friction.F90:615-797,
+// Manning) + flow_eddyvis cell values (flow_eddyvis.f90:128-160, mturbul=5).
+__global__ void k_grad_fric_vis(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const double *vars[2] = {s.u, s.v};
+    double gx[2], gy[2];
+    cbgg_grad(g, s.iwet, c, 2, vars, gx, gy);
+    s.dux[c] = gx[0]; s.duy[c] = gy[0]; s.dvx[c] = gx[1]; s.dvy[c] = gy[1];
+    // interp_scal_cell2face(p, ibc=1)
+    const double pc = s.p[c];
+    const int wc = s.iwet[c];
+    double sx = 0.0, sy = 0.0, hkv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int nb = nbr(g, c, k);
+        int wn = s.iwet[nb];
+        double pk;
+        if (wc * wn == 1) pk = g.fintp[k] * s.p[nb] + (1.0 - g.fintp[k]) * pc;
+        else if (wc == 1) pk = pc;
+        else if (wn == 1) pk = s.p[nb];
+        else pk = 0.5 * (pc + s.p[nb]);
+        sx += g.dsx[k] * pk;
+        sy += g.dsy[k] * pk;
+        hkv[k] = fmax(P.hmin, pk * GRAVINV - fsym(s.zbkE, s.zbkN, g, c, k));
+    }
+    s.dpx[c] = sx / g.area;
+    s.dpy[c] = sy / g.area;
+    s.hkN[c] = hkv[KN];
+    s.hkE[c] = hkv[KE];
+    if (j == 0) s.hkN[c - g.nxp] = hkv[KS];
+    if (i == 0) s.hkE[c - 1] = hkv[KW];
+    // fric_eval
+    double h = s.h[c], u = s.u[c], v = s.v[c];
+    double cd = man2drag(s.coefman[c], h);
+    s.cfrict[c] = cd;
+    if (cd != 0.0) s.z0[c] = drag2length(cd, h);
+    double Um, Uwc, taum;
+    if (P.noptset >= 3) {
+        double Um2 = u * u + v * v;
+        Um = sqrt(Um2);
+        Uwc = sqrt(Um2 + CFRICWAV * 0.0 * 0.0);
+        taum = RHOW * cd * Um * Uwc;
+    } else {
+        Um = sqrt(u * u + v * v);
+        Uwc = Um;
+        taum = RHOW * cd * Um * Um;
+    }
+    double bsvel = sqrt(taum / RHOW + F4(1.0e-15));
+    s.uv[c] = Um; s.uelwc[c] = Uwc; s.bsvel[c] = bsvel;
+    // eddy viscosity, cell value
+    double vis;
+    if (wc == 1) {
+        double stens = sqrt(2.0 * (gx[0] * gx[0] + gy[1] * gy[1]) + (gy[0] + gx[1]) * (gy[0] + gx[1]));
+        vis = CVISCON + CVISBOT * bsvel * h + P.c2a * stens;
+        if (P.noptset >= 3) {
+            vis = vis + CVISWAV * 0.0 * 0.0;
+            vis = vis + CVISWAVBRK * h * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
+        }
+        vis = fmin(vis, CVISMAX);
+    } else {
+        vis = F4(1.0e-6);
+    }
+    s.visb[c] = vis;
+    // ghosts: forcing strings copy the cell value (bndcopy2ghost); walls keep theirs
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!nb_ghost(g, i, j, k)) continue;
+        int gc = nbr(g, c, k);
+        s.visb[gc] = g.side[k] == 1 ? vis : s.vis[gc];
+    }
+}
+
+// flow_eddyvis boundary fix-ups (flow_eddyvis.f90:168-191, Cartesian mesh),
+// race-free: every read is of the values before the loop.  A wet cell next to
+// a dry cell or a ghost takes the value of its opposite neighbour; that value
+// is also written to the dry/ghost neighbour, the last writer in cell order
+// (the highest-numbered wet neighbour: N, then E, then W, then S) winning.
+__device__ __forceinline__ double vis_from(const Geo &g, const double *vb, int w, int k) {
+    // wet cell w, boundary face k -> value from the cell opposite face k
+    double vissum = vb[nbr(g, w, c_kopp[k])] - CVISCON;
+    return CVISCON + vissum / (1.0 + SMALL);
+}
+__global__ void k_vis_fix(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    int i, j;
+    pad_ij(g, t, i, j);
+    const uint8_t *w = s.iwet;
+    const double *vb = s.visb;
+    double val = vb[t];
+    if (active_ij(g, i, j)) {
+        if (w[t] == 1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int nb = nbr(g, t, k);
+                if (w[nb] == 0 || nb_ghost(g, i, j, k)) val = vis_from(g, vb, t, k);
+            }
+        } else {
+            // writers in increasing cell order: S, W, E, N -> last wet one wins
+            const int order[4] = {KS, KW, KE, KN};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int k = order[q];
+                if (nb_ghost(g, i, j, k)) continue;
+                int nb = nbr(g, t, k);
+                if (w[nb] == 1) val = vis_from(g, vb, nb, c_kopp[k]);
+            }
+        }
+    } else {
+        int sd = ring_side(g, i, j);
+        if (sd < 0) return;
+        int own = nbr(g, t, c_kopp[sd]);
+        if (w[own] == 1) val = vis_from(g, vb, own, sd);
+    }
+    s.vis[t] = val;
+}
+
+// corner wet nodes (flow_eddyvis.f90:194-206), reading the fixed-up values
+__global__ void k_vis_corner(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (s.icorner[c] != 1) return;
+    double cid = 0.0, vs = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int nb = nbr(g, c, k);
+        cid += s.iwet[nb];
+        vs += (s.vis[nb] - CVISCON) * s.iwet[nb];
+    }
+    s.visc[c] = CVISCON + vs / (cid + SMALL);
+}
+
+// interp_scal_cell2face(vis, ibc=3) -> visk on owned faces; commits corners.
+__device__ __forceinline__ double vis_final(const Geo &g, const St &s, int x) {
+    int j = x / g.nxp - 1, i = x - (j + 1) * g.nxp - 1;
+    return (active_ij(g, i, j) && s.icorner[x] == 1) ? s.visc[x] : s.vis[x];
+}
+__global__ void k_visk(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const uint8_t *w = s.iwet;
+    double vc = vis_final(g, s, c);
+    int nb = c + g.nxp;
+    s.viskN[c] = (w[c] * w[nb] == 1) ? g.fintp[KN] * vis_final(g, s, nb) + (1.0 - g.fintp[KN]) * vc : 0.0;
+    nb = c + 1;
+    s.viskE[c] = (w[c] * w[nb] == 1) ? g.fintp[KE] * vis_final(g, s, nb) + (1.0 - g.fintp[KE]) * vc : 0.0;
+    if (j == 0) {
+        nb = c - g.nxp;
+        s.viskN[nb] = (w[c] * w[nb] == 1) ? g.fintp[KS] * vis_final(g, s, nb) + (1.0 - g.fintp[KS]) * vc : 0.0;
+    }
+    if (i == 0) {
+        nb = c - 1;
+        s.viskE[nb] = (w[c] * w[nb] == 1) ? g.fintp[KW] * vis_final(g, s, nb) + (1.0 - g.fintp[KW]) * vc : 0.0;
+    }
+    if (s.icorner[c] == 1) s.vis[c] = s.visc[c];
+}
+
+// velface_eta (flow_imp.F90:1296-1346, Cartesian) + final fric_eval
+__global__ void k_velface(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    double h = s.h[c], u = s.u[c], v = s.v[c];
+    if (h > P.hmin && s.iwet[c] == 1) {
+        double umin = 1.0e6, umax = -1.0e6, vmin = 1.0e6, vmax = -1.0e6;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double vel = fdir(s.fluxE, s.fluxN, g, c, k) / (fsym(s.hkE, s.hkN, g, c, k) * g.ds[k]);
+            double uf = g.fnx[k] * vel, vf = g.fny[k] * vel;
+            umin = fmin(umin, uf); umax = fmax(umax, uf);
+            vmin = fmin(vmin, vf); vmax = fmax(vmax, vf);
+        }
+        u = fmin(fmax(u, umin), umax);
+        v = fmin(fmax(v, vmin), vmax);
+        s.eta[c] = s.p[c] * GRAVINV;
+    } else {
+        h = fmax(P.hmin, h);
+        u = 0.0; v = 0.0;
+        s.eta[c] = -999.0;
+        s.h[c] = h;
+    }
+    s.u[c] = u; s.v[c] = v;
+    double cd = man2drag(s.coefman[c], h);
+    s.cfrict[c] = cd;
+    if (cd != 0.0) s.z0[c] = drag2length(cd, h);
+    double Um = sqrt(u * u + v * v), Uwc = Um, taum;
+    if (P.noptset >= 3) { Uwc = sqrt(u * u + v * v + CFRICWAV * 0.0 * 0.0); taum = RHOW * cd * Um * Uwc; }
+    else taum = RHOW * cd * Um * Um;
+    s.uv[c] = Um; s.uelwc[c] = Uwc;
+    s.bsvel[c] = sqrt(taum / RHOW + F4(1.0e-15));
+}
+
+// flow_update, CMS_Flow.f90:312-350
+__global__ void k_update(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    s.iwet1[t] = s.iwet[t];
+    s.u1[t] = s.u[t]; s.v1[t] = s.v[t]; s.p1[t] = s.p[t]; s.h1[t] = s.h[t];
+    s.pp[t] = 0.0; s.dppx[t] = 0.0; s.dppy[t] = 0.0;
+}
+
+// interp_scal_cell2face(zb, ibc=0) on owned faces (prestart.f90:320)
+__global__ void k_zbk(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    s.zbkN[c] = g.fintp[KN] * s.zb[c + g.nxp] + (1.0 - g.fintp[KN]) * s.zb[c];
+    s.zbkE[c] = g.fintp[KE] * s.zb[c + 1] + (1.0 - g.fintp[KE]) * s.zb[c];
+    if (j == 0) s.zbkN[c - g.nxp] = g.fintp[KS] * s.zb[c - g.nxp] + (1.0 - g.fintp[KS]) * s.zb[c];
+    if (i == 0) s.zbkE[c - 1] = g.fintp[KW] * s.zb[c - 1] + (1.0 - g.fintp[KW]) * s.zb[c];
+}
+
+__global__ void k_set_int(int *p, int n, int v) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+}  // namespace cms

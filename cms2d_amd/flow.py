@@ -1,0 +1,121 @@
+"""Python front end of the MI355X implicit step (thin layer over the C ABI).
+
+Mirrors the reference's entry points: ``FlowStep.step`` advances the time loop
+the way source/main/CMS_Flow.f90:43-87 does (timing, flow_imp, flow_update),
+and ``solve`` is the reference's solve(acoef,ss,sp,res,phi,n)
+(source/solvers/solvers.f90:205-304) on the general cell list.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import D, I, check
+
+SIDE = {"N": 0, "E": 1, "S": 2, "W": 3}
+
+
+def _dp(a):
+    return a.ctypes.data_as(D)
+
+
+class FlowStep:
+    """Device-resident implicit flow model on a uniform Cartesian grid."""
+
+    def __init__(self, case, *, rtol=1e-10, atol=1e-12, kmax=4000, maxit=0,
+                 precision=_lib.PREC_FP64, device=0):
+        L = _lib.lib()
+        self.case = case
+        keep = []
+
+        def arr(a):
+            if a is None:
+                return D()
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return _dp(a)
+
+        d = _lib.FlowDesc()
+        d.nx, d.ny, d.dx, d.dy = case.nx, case.ny, case.dx, case.dy
+        d.zb, d.mann = arr(case.zb), arr(case.mann)
+        d.wavestrx = arr(case.wavestrx)
+        d.wavestry = arr(case.wavestry)
+        for k in range(4):
+            d.side_type[k] = _lib.BND_WALL
+        if case.th_side >= 0:
+            d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
+        d.ntc = len(case.amp)
+        for k in range(d.ntc):
+            d.amp[k], d.speed[k], d.phase[k] = case.amp[k], case.speed_rad_hr[k], case.phase_rad[k]
+        d.dt, d.rampdur, d.hdry = case.dt, case.rampdur, float(np.float32(case.hdry))
+        d.noptset, d.wallfric, d.ndsch, d.maxit = case.noptset, case.wallfric, case.ndsch, maxit
+        d.rtol, d.atol, d.kmax, d.precision, d.device = rtol, atol, kmax, precision, device
+        h = C.c_void_p()
+        check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.ncells = case.nx * case.ny
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().cms2d_flow_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        info = _lib.StepInfo()
+        check(_lib.lib().cms2d_flow_step(self._h, n, C.byref(info)))
+        return {"niter": info.niter, "iconv": info.iconv, "rmom": list(info.rmom),
+                "kry": list(info.kry_iters), "nwet": info.nwet, "kernel_ms": info.kernel_ms}
+
+    def get(self, name):
+        out = np.empty(self.ncells)
+        check(_lib.lib().cms2d_flow_get(self._h, name.encode(), _dp(out)))
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        check(_lib.lib().cms2d_flow_set(self._h, name.encode(), _dp(a)))
+
+    def get_face(self, name):
+        out = np.empty(4 * self.ncells)
+        check(_lib.lib().cms2d_flow_get_face(self._h, name.encode(), _dp(out)))
+        return out.reshape(self.ncells, 4)
+
+    @property
+    def time(self):
+        return _lib.lib().cms2d_flow_time(self._h)
+
+    def kernel_stats(self, name="spmv_pp"):
+        ms, n = C.c_double(), C.c_long()
+        check(_lib.lib().cms2d_flow_kernel_stats(self._h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, atol=1e-12,
+          kmax=4000):
+    """solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
+
+    acoef, cell2cell: [ncellsD, 4] (faces N,E,S,W), cell2cell 0-based with -1
+    for no face; iwet, ss, sp, phi: [ncellsD].  ss and phi are updated in place.
+    Returns (res, rmom, iterations)."""
+    acoef = np.ascontiguousarray(acoef, dtype=np.float64)
+    c2c = np.ascontiguousarray(cell2cell, dtype=np.int32)
+    iw = np.ascontiguousarray(iwet, dtype=np.int32)
+    assert ss.dtype == np.float64 and ss.flags.c_contiguous
+    assert phi.dtype == np.float64 and phi.flags.c_contiguous
+    spa = np.ascontiguousarray(sp, dtype=np.float64)
+    ncellsD = acoef.shape[0]
+    res = np.zeros(ncellsD)
+    rmom = C.c_double()
+    iters = C.c_int()
+    check(_lib.lib().cms2d_solve(ncells, ncellsD, _dp(acoef), c2c.ctypes.data_as(I),
+                                 iw.ctypes.data_as(I), _dp(ss), _dp(spa), _dp(res), _dp(phi),
+                                 relax, rtol, atol, kmax, C.byref(rmom), C.byref(iters)))
+    return res, rmom.value, iters.value

@@ -1,0 +1,117 @@
+/*
+ * cms2d_amd.h -- C ABI of the MI355X implementation of CMS-Flow's implicit
+ * hydrodynamic time step (erdc/cms2d 5.4).
+ *
+ * Plain C types only: pointers, sizes, doubles and ints.  Every entry point
+ * returns 0 on success and a negative status on failure; cms2d_last_error()
+ * gives the message.  No C++ exception crosses this boundary.
+ *
+ * Reference interfaces replaced (paths relative to the erdc/cms2d tree):
+ *   cms2d_flow_step  <- flow_imp                       source/flow/flow_imp.F90:29-312
+ *                       driven as in the time loop     source/main/CMS_Flow.f90:43-87
+ *                       (timing :109-240, flow_update :243-411)
+ *   cms2d_solve      <- solve(acoef,ss,sp,res,phi,n)   source/solvers/solvers.f90:205-304
+ *                       with pbicgstab                 source/solvers/pbicgstab.f90:2-98
+ *   cms2d_flow_create <- prestart/flow_init/geo_cart   source/main/prestart.f90:2-329,
+ *                       source/flow/flow.F90:592-835, source/geospatial/geospatial.F90:2709-2828
+ */
+#ifndef CMS2D_AMD_H
+#define CMS2D_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMS2D_OK 0
+#define CMS2D_ERR_ARG (-1)        /* invalid argument or size */
+#define CMS2D_ERR_HIP (-2)        /* HIP runtime error */
+#define CMS2D_ERR_DIVERGED (-3)   /* outer iteration divergent (iconv=0) with fixed dt */
+#define CMS2D_ERR_NODEVICE (-4)   /* no HIP device */
+
+/* Side codes, in the reference's face order (geospatial.F90:2841-2875). */
+#define CMS2D_SIDE_N 0
+#define CMS2D_SIDE_E 1
+#define CMS2D_SIDE_S 2
+#define CMS2D_SIDE_W 3
+/* Boundary types of a side. */
+#define CMS2D_BND_WALL 0          /* closed wall (W_str, boundaries.F90:2413-2435)            */
+#define CMS2D_BND_WSE_HARMONIC 1  /* harmonic water level (TH_str, boundaries.F90:4098-4124) */
+
+/* Solver precision forms. */
+#define CMS2D_PREC_FP64 0         /* FP64 everywhere                                          */
+#define CMS2D_PREC_FP32_REFINE 1  /* FP32 Krylov + SpMV inside FP64 iterative refinement     */
+
+typedef struct {
+    int nx, ny;                   /* active cells per row / per column (row-major from SW)   */
+    double dx, dy;                /* uniform cell size [m]                                  */
+    const double *zb;             /* bed elevation [nx*ny], host memory [m]                 */
+    const double *mann;           /* Manning n [nx*ny] (MANNINGS_N_DATASET)                 */
+    const double *wavestrx;       /* radiation-stress forcing / rho [nx*ny] or NULL         */
+    const double *wavestry;       /* (wave_flowgrid_def wavestrx/wavestry)                  */
+    int side_type[4];             /* CMS2D_BND_* for N,E,S,W                                 */
+    int ntc;                      /* harmonic constituents (<= 8)                            */
+    double amp[8];                /* [m]                                                     */
+    double speed[8];              /* [rad/hr]                                                */
+    double phase[8];              /* [rad]                                                   */
+    double dt;                    /* HYDRO_TIMESTEP [s], fixed (VARIABLE_TIMESTEP OFF)       */
+    double rampdur;               /* DURATION_RAMP [hr]; 0 -> one time step                  */
+    double hdry;                  /* DRYING_DEPTH [m] (reference default 0.05)               */
+    int noptset;                  /* >= 3: wave branches of friction/eddy viscosity active   */
+    int wallfric;                 /* USE_WALL_FRICTION_TERMS                                 */
+    int ndsch;                    /* ADVECTION_SCHEME 1 upwind,2 hybrid,3 power-law,4 exponential */
+    int maxit;                    /* HYDRO_MAX_ITERATIONS; 0 -> 20 + ncells/10000            */
+    double rtol;                  /* Krylov stop: ||r||2 <= max(rtol*||r0||2, atol*sqrt(n))  */
+    double atol;
+    int kmax;                     /* Krylov iteration cap per solve                          */
+    int precision;                /* CMS2D_PREC_*                                            */
+    int device;                   /* HIP device ordinal                                      */
+} cms2d_flow_desc;
+
+typedef struct {
+    int niter;                    /* outer (SIMPLEC) iterations of the last step             */
+    int iconv;                    /* check_conv state of the last step                       */
+    double rmom[3];               /* normalized residual RMS: pp, u, v                       */
+    int kry_iters[3];             /* Krylov iterations over the last step: pp, u, v          */
+    int nwet;                     /* wet active cells                                        */
+    double kernel_ms;             /* device time of the steps run by this call [ms]          */
+} cms2d_step_info;
+
+typedef struct cms2d_flow cms2d_flow;
+
+const char *cms2d_last_error(void);
+int cms2d_device_count(int *count);
+
+/* Builds the grid, uploads the case and sets the cold-start state
+ * (prestart.f90:309-329). */
+int cms2d_flow_create(const cms2d_flow_desc *desc, cms2d_flow **out);
+int cms2d_flow_destroy(cms2d_flow *flow);
+/* Advances nsteps time steps: timing, flow_imp and flow_update per step. */
+int cms2d_flow_step(cms2d_flow *flow, int nsteps, cms2d_step_info *info);
+/* Copies an active-cell field to host memory, row-major [nx*ny]:
+ * "eta","u","v","p","h","iwet","vis","cfrict","uv","bsvel","pp","dpx","dpy". */
+int cms2d_flow_get(cms2d_flow *flow, const char *name, double *out);
+/* Directed face field of every active cell, out[4*c+k], k = N,E,S,W:
+ * "flux","hk","visk". */
+int cms2d_flow_get_face(cms2d_flow *flow, const char *name, double *out);
+/* Overwrites an active-cell field ("u","v","p"); ghost values follow at the
+ * next step. */
+int cms2d_flow_set(cms2d_flow *flow, const char *name, const double *in);
+double cms2d_flow_time(const cms2d_flow *flow);
+/* Average device time per launch of the named kernel group over the steps run
+ * so far ("spmv_pp": pp-system SpMV), and launches counted. */
+int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
+
+/* Drop-in for solve(acoef,ss,sp,res,phi,n) on the reference's general cell
+ * list (solvers.f90:205-304): nmaxfaces = 4, 0-based cell2cell with ghost cells
+ * numbered ncells..ncellsD-1 (-1 for none), host arrays in the reference layout
+ * (acoef[4*ncellsD] face-major per cell).  relax = relax(n).  Under-relaxation
+ * and the normalized residual follow the reference; the Krylov solve stops at
+ * ||r||2 <= max(rtol*||r0||2, atol*sqrt(ncells)).  Outputs: ss, res, phi,
+ * *rmom, *iters. */
+int cms2d_solve(int ncells, int ncellsD, const double *acoef, const int *cell2cell,
+                const int *iwet, double *ss, const double *sp, double *res, double *phi,
+                double relax, double rtol, double atol, int kmax, double *rmom, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
