@@ -8,6 +8,8 @@
 #include "krylov_kernels.hip"
 #include "../../include/cms2d_amd.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -16,6 +18,9 @@
 #include <vector>
 
 using namespace cms;
+
+// Fixed grid of the Krylov kernels (4 workgroups per CU on 256 CUs).
+static constexpr int KRY_BLOCKS = 1024;
 
 static thread_local std::string g_err;
 static int set_err(int code, const std::string &m) {
@@ -41,7 +46,7 @@ struct cms2d_flow {
     KryScal *h_sc = nullptr;   // pinned mirror
     int *h_flags = nullptr;    // pinned
     std::vector<void *> allocs;
-    int npad = 0, nblk_act = 0, nblk_pad = 0;
+    int npad = 0, nblk_act = 0, nblk_pad = 0, nblk_kry = 0;
     int ntc = 0;
     double amp[8]{}, speed[8]{}, phase[8]{};
     double timesecs = 0.0, rampdur = 0.0;
@@ -118,13 +123,12 @@ static int wetdry(cms2d_flow *F) {
     return 0;
 }
 
-// Krylov launch context: stream, reduction scratch and scalar mirrors.
+// Krylov launch context: stream, fixed grid and scalar mirrors.
 struct KryCtx {
     hipStream_t st;
-    int nblk;
+    int nblk;                      // fixed Krylov grid
     int kmax;
-    double *partials;
-    unsigned *counter;
+    double rtol, atol;
     KryScal *d_sc, *h_sc;
     int *prev_iters;
     std::vector<hipEvent_t> *ev;   // event pairs bracketing timed SpMV launches (or null)
@@ -132,42 +136,51 @@ struct KryCtx {
     long *ev_n;
 };
 
-// Runs BiCGStab iterations in batches until every system has stopped.  Each
+// Runs BiCGStab iterations in batches until every system has stopped.  A
 // batch is enqueued without host synchronisation; kernels launched after a
-// system has converged return at once (they read the device-side done flag).
+// system has stopped return at once (they read the device-side done flag).
 template <class Op, int NS>
 static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
-    int batch = std::max(1, X.prev_iters[sc0] - 1);
+    const dim3 grid(X.nblk), blk(BLOCK);
+    // start: thresholds, rmom and the initial stop test
+    hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
     HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
     HIPCHK(hipStreamSynchronize(X.st));
-    int done_all = 1;
-    for (int q = 0; q < NS; q++) done_all &= X.h_sc[sc0 + q].done;
-    int launched = 0;
-    while (!done_all && launched < X.kmax) {
-        int nb = std::min(batch, X.kmax - launched);
-        int it0 = X.h_sc[sc0].iters;
+    auto all_done = [&]() {
+        int d = 1;
+        for (int q = 0; q < NS; q++) d &= X.h_sc[sc0 + q].done;
+        return d;
+    };
+    int it = 0;
+    int batch = std::max(1, X.prev_iters[sc0] - 1);
+    while (!all_done() && it < X.kmax) {
+        int nb = std::min(batch, X.kmax - it);
         int nev = 0;
-        for (int b = 0; b < nb; b++) {
-            hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K);
+        const int it0 = it;
+        for (int b = 0; b < nb; b++, it++) {
+            hipLaunchKernelGGL((k_bicg_p<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
             bool rec = X.ev && nev < (int)X.ev->size() / 2;
-            if (rec) HIPCHK(hipEventRecord((*X.ev)[2 * nev], X.st));
-            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials,
-                               X.counter);
-            if (rec) { HIPCHK(hipEventRecord((*X.ev)[2 * nev + 1], X.st)); nev++; }
-            hipLaunchKernelGGL((k_bicg_s<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K);
-            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials,
-                               X.counter);
-            hipLaunchKernelGGL((k_bicg_r<Op, NS>), dim3(X.nblk), dim3(BLOCK), 0, X.st, A, K, X.partials, X.counter,
-                               X.kmax);
+            if (rec) {
+                // events stamped by the dispatch itself (start and end of this kernel)
+                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
+                                      (*X.ev)[2 * nev + 1], 0, A, K);
+                nev++;
+            } else {
+                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), grid, blk, 0, X.st, A, K);
+            }
+            hipLaunchKernelGGL((k_bicg_s<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk);
+            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2>), grid, blk, 0, X.st, A, K);
+            hipLaunchKernelGGL((k_bicg_r<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk);
         }
+        hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 1);
         HIPCHK(hipGetLastError());
-        launched += nb;
         HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
         HIPCHK(hipStreamSynchronize(X.st));
-        done_all = 1;
-        for (int q = 0; q < NS; q++) done_all &= X.h_sc[sc0 + q].done;
         if (nev > 0) {
-            int real = std::min(nev, X.h_sc[sc0].iters - it0);
+            // launches that did work: those of iterations the first system completed
+            const KryScal &s0 = X.h_sc[sc0];
+            int upto = s0.done ? s0.final_iters : it;
+            int real = std::min(nev, std::max(0, upto - it0));
             for (int e = 0; e < real; e++) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, (*X.ev)[2 * e], (*X.ev)[2 * e + 1]));
@@ -177,14 +190,14 @@ static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
         }
         batch = std::min(2 * std::max(batch, 2), 64);
     }
-    for (int q = 0; q < NS; q++) X.prev_iters[sc0 + q] = std::max(1, X.h_sc[sc0 + q].iters);
+    for (int q = 0; q < NS; q++) X.prev_iters[sc0 + q] = std::max(1, X.h_sc[sc0 + q].final_iters);
     return 0;
 }
 
 static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
     KryCtx X;
-    X.st = F->st; X.nblk = F->nblk_act; X.kmax = F->P.kmax;
-    X.partials = F->s.partials; X.counter = F->s.counter;
+    X.st = F->st; X.nblk = F->nblk_kry; X.kmax = F->P.kmax;
+    X.rtol = F->P.rtol; X.atol = F->P.atol;
     X.d_sc = F->d_sc; X.h_sc = F->h_sc; X.prev_iters = F->prev_iters;
     X.ev = (timed && F->stats) ? &F->ev : nullptr;
     X.ev_ms = &F->spmv_ms; X.ev_n = &F->spmv_n;
@@ -248,7 +261,7 @@ static int flow_imp(cms2d_flow *F) {
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
         LAUNCH(k_mom, F->nblk_act, g, s, F->P);
-        LAUNCH(k_uv_prologue, F->nblk_act, g, s, F->P, F->Kuv);
+        hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->Kuv);
         {
             KryCtx X = flow_ctx(F, false);
             if (int rc = run_bicg(X, F->Kuv, A, 1)) return rc;
@@ -260,7 +273,7 @@ static int flow_imp(cms2d_flow *F) {
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
             hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd, wsebnd);
         }
-        LAUNCH(k_pp_src, F->nblk_act, g, s, F->P, F->Kpp);
+        hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
             KryCtx X = flow_ctx(F, true);
             if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
@@ -272,7 +285,7 @@ static int flow_imp(cms2d_flow *F) {
         if (int rc = eddyvis_tail(F)) return rc;
         for (int n = 0; n < 3; n++) {
             F->rmom[n] = F->h_sc[n].rmom;
-            F->kry[n] += F->h_sc[n].iters;
+            F->kry[n] += F->h_sc[n].final_iters;
         }
         check_conv(F);
         if (F->iconv == 0) return set_err(CMS2D_ERR_DIVERGED, "Model is divergent (check_conv iconv=0)");
@@ -356,6 +369,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     F->npad = g.nxp * g.nyp;
     F->nblk_act = (g.nact + BLOCK - 1) / BLOCK;
     F->nblk_pad = (F->npad + BLOCK - 1) / BLOCK;
+    F->nblk_kry = std::min(KRY_BLOCKS, F->nblk_act);
     Par &P = F->P;
     P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
     P.hmin = P.hdry;                                  // flow.F90:690-694
@@ -397,6 +411,17 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (int rc = dalloc(F, &s.flags, 4)) return rc;
     if (int rc = dalloc(F, &s.partials, (size_t)F->nblk_act * 8 + 64)) return rc;
     if (int rc = dalloc(F, &s.counter, 1)) return rc;
+    {
+        // Krylov partial sums: 5 slots per system (pr: 2, pv: 1, pt: 2)
+        double *red;
+        if (int rc = dalloc(F, &red, 3 * 5 * (size_t)F->nblk_kry)) return rc;
+        const size_t nb = F->nblk_kry;
+        F->Kpp.red.pr[0] = red; F->Kpp.red.pv[0] = red + 2 * nb; F->Kpp.red.pt[0] = red + 3 * nb;
+        for (int q = 0; q < 2; q++) {
+            double *b = red + (q + 1) * 5 * nb;
+            F->Kuv.red.pr[q] = b; F->Kuv.red.pv[q] = b + 2 * nb; F->Kuv.red.pt[q] = b + 3 * nb;
+        }
+    }
     if (int rc = dalloc(F, &F->d_sc, 3)) return rc;
     // Krylov vectors
     {
@@ -593,18 +618,17 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
         return p;
     };
     const size_t nD = ncellsD, n = ncells;
-    int nblk = (ncells + BLOCK - 1) / BLOCK;
+    int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
     double *d_a = (double *)alloc(4 * nD * sizeof(double));
     int *d_c2c = (int *)alloc(4 * nD * sizeof(int));
     int *d_w = (int *)alloc(nD * sizeof(int));
     double *d_ss = (double *)alloc(nD * sizeof(double)), *d_sp = (double *)alloc(nD * sizeof(double));
     double *d_res = (double *)alloc(nD * sizeof(double)), *d_phi = (double *)alloc(nD * sizeof(double));
     double *vec = (double *)alloc(6 * n * sizeof(double));
-    double *partials = (double *)alloc(((size_t)nblk * 8 + 64) * sizeof(double));
-    unsigned *counter = (unsigned *)alloc(sizeof(unsigned));
+    double *partials = (double *)alloc((size_t)nblk * 5 * sizeof(double));
     KryScal *d_sc = (KryScal *)alloc(sizeof(KryScal));
     KryScal h_sc{};
-    if (!d_a || !d_c2c || !d_w || !d_ss || !d_sp || !d_res || !d_phi || !vec || !partials || !counter || !d_sc) {
+    if (!d_a || !d_c2c || !d_w || !d_ss || !d_sp || !d_res || !d_phi || !vec || !partials || !d_sc) {
         cleanup();
         return set_err(CMS2D_ERR_HIP, "device allocation failed");
     }
@@ -618,24 +642,20 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
                   hipMemcpyAsync(d_ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
                   hipMemcpyAsync(d_sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
                   hipMemcpyAsync(d_phi, phi, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemsetAsync(counter, 0, sizeof(unsigned), st) == hipSuccess;
+                  hipMemsetAsync(d_sc, 0, sizeof(KryScal), st) == hipSuccess;
         if (!ok) { rc = set_err(CMS2D_ERR_HIP, "upload failed"); break; }
         KrySys<1> K{};
         K.x[0] = d_phi;
         K.r[0] = vec; K.r0[0] = vec + n; K.p[0] = vec + 2 * n; K.v[0] = vec + 3 * n; K.t[0] = vec + 4 * n;
         K.ap_w[0] = vec + 5 * n;
         K.sc = d_sc;
-        Par P{};
-        P.rtol = rtol > 0 ? rtol : 1e-10;
-        P.atol = atol >= 0 ? atol : 1e-12;
-        P.kmax = kmax > 0 ? kmax : 4000;
+        K.red.pr[0] = partials; K.red.pv[0] = partials + 2 * nblk; K.red.pt[0] = partials + 3 * nblk;
         EllOp A{ncells, d_c2c, d_a};
         hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(BLOCK), 0, st, A, (const int *)d_w, d_ss,
-                           (const double *)d_sp, d_res, (const double *)d_phi, relax, P, K, partials, counter);
+                           (const double *)d_sp, d_res, (const double *)d_phi, relax, K);
         int prev = 8;
-        KryCtx X{st, nblk, P.kmax, partials, counter, d_sc, &h_sc, &prev, nullptr, nullptr, nullptr};
-        X.prev_iters = &prev;
-        // run_bicg indexes scalar mirrors from sc0=0
+        KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,
+                 d_sc, &h_sc, &prev, nullptr, nullptr, nullptr};
         if ((rc = run_bicg(X, K, A, 0))) break;
         ok = hipMemcpyAsync(ss, d_ss, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
              hipMemcpyAsync(res, d_res, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -643,7 +663,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
              hipStreamSynchronize(st) == hipSuccess;
         if (!ok) { rc = set_err(CMS2D_ERR_HIP, "download failed"); break; }
         if (rmom) *rmom = h_sc.rmom;
-        if (iters) *iters = h_sc.iters;
+        if (iters) *iters = h_sc.final_iters;
     } while (0);
     if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     cleanup();

@@ -497,10 +497,12 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
 
 // pp sources (flow_imp.F90:700-712) + bound_pp (boundaries.F90:3493-3646) +
 // solve prologue for n=1 (solvers.f90:232-240, relax(1)=1) + BiCGStab start.
+// Runs on the fixed Krylov grid and leaves r.r / res.res partials.
 __global__ void __launch_bounds__(BLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> K) {
-    int i = 0, j = 0, c = 0;
-    double red[3] = {0.0, 0.0, 0.0};
-    if (act_thread(g, i, j, c)) {
+    double red[2] = {0.0, 0.0};
+    const Chunk ch = block_chunk(g.nact);
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
         double sp = s.sppp0[c];
         double fs = ((s.fluxN[c] + s.fluxE[c]) + (-s.fluxN[c - g.nxp])) + (-s.fluxE[c - 1]);
         double su = s.sspp0[c] - fs + s.p[c] * sp;
@@ -559,11 +561,12 @@ __global__ void __launch_bounds__(BLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> 
         double r = anb + ss - ap * phi + SMALL;
         K.ap_w[0][c] = ap;
         K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
-        red[0] = res * res;
-        red[1] = r * r;
+        red[0] += r * r;
+        red[1] += res * res;
     }
-    if (grid_sum<3>(red, s.partials, s.counter) && threadIdx.x == 0)
-        kry_start(K.sc[0], red[0], red[1], g.nact, P);
+    double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
+    put_partials<2>(red, dst);
+    kry_reset(K.sc[0]);
 }
 
 // pvfc (flow_imp.F90:856-908): wall ghosts copy pp, gradient of pp, velocity

@@ -28,21 +28,4 @@ struct Par {
     int kmax, noptset, wallfric, ndsch, lmax;
 };
 
-// Stopping threshold and scalar state at the start of a Krylov solve.
-__device__ __forceinline__ void kry_start(KryScal &sc, double sumres2, double sumr2, int n, const Par &P) {
-    sc.rmom = sqrt(sumres2 / (double)(float)n);       // sqrt(adak/real(ncells))
-    double r0n = sqrt(sumr2);
-    sc.r0norm = r0n;
-    sc.rnorm = r0n;
-    sc.thr = fmax(P.rtol * r0n, P.atol * sqrt((double)n));
-    sc.rho = sumr2;
-    sc.alpha = 1.0;
-    sc.omega = 1.0;
-    sc.beta = 0.0;
-    sc.iters = 0;
-    sc.restart = 0;
-    sc.first = 1;
-    sc.done = (r0n <= sc.thr) ? 1 : 0;
-}
-
 }  // namespace cms

@@ -1,43 +1,78 @@
-// krylov.h -- right-preconditioned BiCGStab on the 5-point cell stencil.
+// krylov.h -- right-preconditioned BiCGStab on the cell stencil.
 //
 // Restates the iteration of pbicgstab (source/solvers/pbicgstab.f90:21-96):
 //   p = r + beta*(p - omega*v);  v = A M^-1 p;  alpha = rho/(r0.v)
 //   x += alpha M^-1 p;  s = r - alpha v;  t = A M^-1 s;  omega = (t.s)/(t.t)
 //   x += omega M^-1 s;  r = s - omega t
-// with M = diag(ap) (Jacobi) and the standard beta = (rho/rho_old)(alpha/omega);
-// the reference's beta denominator carries +1e-15 (pbicgstab.f90:43), which
-// stalls the recurrence once rho < ~1e-13, so the tight solve restarts the
-// shadow residual on rho breakdown instead.  Up to NS systems that share the
-// off-diagonal coefficients (u and v momentum) advance together so each
+// with M = diag(ap) (Jacobi) and the standard beta = (rho/rho_old)(alpha/omega).
+// The reference's beta denominator carries +1e-15 (pbicgstab.f90:43), which
+// stalls the recurrence once rho < ~1e-13, so the tight solve instead restarts
+// the shadow residual when rho breaks down.  Up to NS systems that share the
+// off-diagonal coefficients (u and v momentum) advance together so that every
 // coefficient is read once per sweep.
+//
+// Reductions without atomics: every Krylov kernel runs on the same fixed grid
+// of NB blocks (grid-stride over the cells).  A kernel that produces dot
+// products writes one partial per block; the kernel that consumes them sums
+// all NB partials in block order in every block (identical result everywhere),
+// and block 0 publishes the scalars for later kernels into slots indexed by
+// the iteration parity, so no block reads a slot that the same launch writes.
 #pragma once
 #include "common.h"
 
 namespace cms {
 
 struct KryScal {
-    double rho;        // r0.r of the current residual
-    double alpha, omega, beta;
-    double r0norm;     // ||r|| at the start (or at the last restart)
+    double rho[2];     // r0.r at the start of iteration it (slot it&1)
+    double r0n[2];     // ||r0|| of the current shadow residual
+    double alpha[2];   // alpha of iteration it
+    double omega[2];   // omega of iteration it
     double thr;        // stopping threshold on ||r||2
-    double rnorm;
     double rmom;       // normalized residual RMS of the solve prologue
-    double red[4];     // reduction results
-    int done, iters, restart, first;
+    double rnorm;      // ||r|| when the solve stopped
+    int iters[2];      // iterations completed at the start of iteration it
+    int done;          // solve finished
+    int final_iters;
 };
 
-// Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k), z = M^-1 x = x/ap,
-// z = 0 on ghost cells.  cell() maps a thread to its row.
+// Per-block partial sums, slot-major: pr[2*NB] (r.r | r0.r, or r.r | res.res
+// for the prologue), pv[NB] (r0.v), pt[2*NB] (t.s | t.t).
+template <int NS>
+struct KryRed {
+    double *pr[NS], *pv[NS], *pt[NS];
+};
+
 struct Idx {
     int c, i, j;
 };
 
-// Structured 5-point stencil on the padded grid (directed coefficients).
+// Fixed-grid mapping: block b (XCD-remapped) owns a contiguous chunk of the
+// cell list; its threads stride through the chunk 256 cells at a time.
+struct Chunk {
+    int begin, end;
+};
+__device__ __forceinline__ Chunk block_chunk(int n) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+    Chunk ch;
+    ch.begin = min(n, b * per);
+    ch.end = min(n, ch.begin + per);
+    return ch;
+}
+
+// Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k), z = M^-1 x = x/ap,
+// z = 0 on ghost cells.
 struct StencilOp {
     Geo g;
     const double *aN, *aE, *aS, *aW;
-    __device__ __forceinline__ bool cell(Idx &x) const { return act_thread(g, x.i, x.j, x.c); }
     __device__ __forceinline__ int n() const { return g.nact; }
+    __device__ __forceinline__ Idx at(int t) const {
+        Idx x;
+        x.j = t / g.nx;
+        x.i = t - x.j * g.nx;
+        x.c = cidx(g, x.i, x.j);
+        return x;
+    }
     __device__ __forceinline__ double apply(const double *xv, const double *ap, const Idx &x) const {
         const int c = x.c;
         const double zc = xv[c] / ap[c];
@@ -57,12 +92,13 @@ struct EllOp {
     int ncells;
     const int *c2c;
     const double *a;
-    __device__ __forceinline__ bool cell(Idx &x) const {
-        x.c = blockIdx.x * blockDim.x + threadIdx.x;
-        x.i = x.j = 0;
-        return x.c < ncells;
-    }
     __device__ __forceinline__ int n() const { return ncells; }
+    __device__ __forceinline__ Idx at(int t) const {
+        Idx x;
+        x.c = t;
+        x.i = x.j = 0;
+        return x;
+    }
     __device__ __forceinline__ double apply(const double *xv, const double *ap, const Idx &x) const {
         const int c = x.c;
         double sum = 0.0;
@@ -80,6 +116,36 @@ struct KrySys {
     double *x[NS], *r[NS], *r0[NS], *p[NS], *v[NS], *t[NS];
     double *ap_w[NS];  // diagonal ap of each system (written by the prologue)
     KryScal *sc;       // [NS]
+    KryRed<NS> red;
 };
+
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// Called by the kernel that starts a solve (its block 0 only writes the slot
+// no other block of that launch reads).
+__device__ __forceinline__ void kry_reset(KryScal &sc) {
+    if (lead()) { sc.done = 0; sc.final_iters = 0; }
+}
+
+// Totals of NV partial arrays of nb entries, summed in block order; every
+// thread of every block obtains the same values.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double *const *src, int nb, double (&out)[NV]) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += BLOCK)
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] += src[k][b];
+    block_sum<NV>(out);
+}
+
+// Block totals of v written as this block's partials.
+template <int NV>
+__device__ __forceinline__ void put_partials(double (&v)[NV], double *const *dst) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) dst[k][blockIdx.x] = v[k];
+}
 
 }  // namespace cms

@@ -1,6 +1,15 @@
 // krylov_kernels.hip -- solve() prologue and Jacobi-preconditioned BiCGStab
-// sweeps on the structured 5-point operator (restating solvers.f90:205-304
-// and pbicgstab.f90:21-96; see krylov.h for the recurrence).
+// sweeps (restating solvers.f90:205-304 and pbicgstab.f90:21-96; krylov.h
+// describes the recurrence and the atomic-free reductions).
+//
+// Iteration `it` is five launches on the fixed grid:
+//   k_bicg_p(it)     finish iteration it-1 from its r.r and r0.r partials
+//                    (stop test, restart, beta); p = r + beta (p - omega v)
+//   k_bicg_spmv1(it) v = A M^-1 p, partials r0.v
+//   k_bicg_s(it)     alpha = rho / r0.v;  x += alpha M^-1 p;  s = r - alpha v
+//   k_bicg_spmv2(it) t = A M^-1 s, partials t.s and t.t
+//   k_bicg_r(it)     omega = t.s / t.t;  x += omega M^-1 s;  r = s - omega t,
+//                    partials r.r and r0.r
 #include "flow_state.h"
 
 namespace cms {
@@ -19,10 +28,11 @@ __device__ __forceinline__ bool kdone(const KrySys<NS> &K) {
 
 // solve() prologue for u (n=2) and v (n=3), relax(2)=relax(3)=0.8
 // (solvers.f90:232-240), then the BiCGStab start (pbicgstab.f90:21-31).
-__global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, Par P, KrySys<2> K) {
-    int i = 0, j = 0, c = 0;
+__global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    if (act_thread(g, i, j, c)) {
+    const Chunk ch = block_chunk(g.nact);
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
         const double a[4] = {s.aN[c], s.aE[c], s.aS[c], s.aW[c]};
         const int nb[4] = {c + g.nxp, c + 1, c - g.nxp, c - 1};
         const double asum = ((a[0] + a[1]) + a[2]) + a[3];
@@ -45,159 +55,226 @@ __global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, Par P, KrySy
             (q ? s.resv : s.resu)[c] = res;
             K.ap_w[q][c] = ap;
             K.r[q][c] = r; K.r0[q][c] = r; K.p[q][c] = 0.0; K.v[q][c] = 0.0;
-            red[2 * q] = res * res;
-            red[2 * q + 1] = r * r;
+            red[2 * q] += r * r;
+            red[2 * q + 1] += res * res;
         }
     }
-    if (grid_sum<4>(red, s.partials, s.counter) && threadIdx.x == 0) {
-        kry_start(K.sc[0], red[0], red[1], g.nact, P);
-        kry_start(K.sc[1], red[2], red[3], g.nact, P);
-    }
+    double *const dst[4] = {K.red.pr[0], K.red.pr[0] + gridDim.x, K.red.pr[1], K.red.pr[1] + gridDim.x};
+    put_partials<4>(red, dst);
+    kry_reset(K.sc[0]);
+    kry_reset(K.sc[1]);
 }
 
-// p = r + beta (p - omega v); on a restart the shadow residual becomes r.
+// Finishes iteration it-1 (or starts the solve when it == 0) and, unless
+// fin_only, updates p.  Launched with fin_only on one block at the end of a
+// batch so the host can read the state; the next batch repeats the same
+// finish (same inputs, same result) before its first p update.
 template <class Op, int NS>
-__global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS> K) {
-    if (kdone(K)) return;
-    Idx x;
-    if (!A.cell(x)) return;
-    const int c = x.c;
+__global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS> K, int it, int nb, int kmax, double rtol,
+                                                  double atol, int fin_only) {
+    double tot[2 * NS];
+    const double *src[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) { src[2 * q] = K.red.pr[q]; src[2 * q + 1] = K.red.pr[q] + nb; }
+    sum_partials<2 * NS>(src, nb, tot);
+    bool act[NS];
+    int rst[NS];
+    double beta[NS], om[NS];
+    bool any = false;
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-        if (K.sc[q].done) continue;
-        const double beta = K.sc[q].beta, om = K.sc[q].omega;
-        const double r = K.r[q][c];
-        if (K.sc[q].restart) K.r0[q][c] = r;
-        K.p[q][c] = r + beta * (K.p[q][c] - om * K.v[q][c]);
+        KryScal &sc = K.sc[q];
+        act[q] = false; rst[q] = 0; beta[q] = 0.0; om[q] = 1.0;
+        if (sc.done) continue;
+        const double rr = tot[2 * q], nr = sqrt(rr);
+        if (it == 0) {
+            const double thr = fmax(rtol * nr, atol * sqrt((double)A.n()));
+            act[q] = !(nr <= thr);
+            if (lead()) {
+                sc.rmom = sqrt(tot[2 * q + 1] / (double)(float)A.n());   // sqrt(adak/real(ncells))
+                sc.thr = thr;
+                sc.rho[0] = rr;
+                sc.r0n[0] = nr;
+                sc.iters[0] = 0;
+                sc.rnorm = nr;
+                if (!act[q]) { sc.done = 1; sc.final_iters = 0; }
+            }
+        } else {
+            const int pq = (it - 1) & 1, cq = it & 1;
+            const double r0r = tot[2 * q + 1];
+            const int iters = sc.iters[pq] + 1;
+            const double rho_p = sc.rho[pq], r0n_p = sc.r0n[pq], al = sc.alpha[pq], omp = sc.omega[pq];
+            double rho = rho_p, r0n = r0n_p;
+            if (omp == 0.0 || nr <= sc.thr || iters >= kmax) {
+                act[q] = false;
+            } else if (fabs(r0r) <= 1e-12 * nr * r0n_p) {
+                rst[q] = 1; rho = rr; r0n = nr; beta[q] = 0.0; act[q] = true;
+            } else {
+                beta[q] = (r0r / rho_p) * (al / omp); rho = r0r; act[q] = true;
+            }
+            om[q] = omp;
+            if (lead()) {
+                sc.rnorm = nr;
+                sc.iters[cq] = iters;
+                if (act[q]) { sc.rho[cq] = rho; sc.r0n[cq] = r0n; }
+                else { sc.done = 1; sc.final_iters = iters; }
+            }
+        }
+        any = any || act[q];
+    }
+    if (fin_only || !any) return;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int c = A.at(t).c;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            if (!act[q]) continue;
+            const double r = K.r[q][c];
+            if (rst[q]) K.r0[q][c] = r;
+            K.p[q][c] = r + beta[q] * (K.p[q][c] - om[q] * K.v[q][c]);
+        }
     }
 }
 
-// y = A M^-1 x with M = diag(ap), z = 0 on ghost cells.
-// MODE 1: x = p, y -> v, reduce r0.v, then alpha = rho / (r0.v).
-// MODE 2: x = s (held in r), y -> t, reduce t.s and t.t, then omega = (t.s)/(t.t).
+// MODE 1: v = A M^-1 p, partials r0.v.  MODE 2: t = A M^-1 s (s held in r),
+// partials t.s and t.t.
 template <class Op, int NS, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K, double *partials, unsigned *counter) {
+__global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
     if (kdone(K)) return;
     constexpr int NV = MODE == 1 ? NS : 2 * NS;
     double red[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) red[q] = 0.0;
-    Idx x;
-    if (A.cell(x)) {
+    bool act[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) act[q] = !K.sc[q].done;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const Idx x = A.at(t);
         const int c = x.c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
-            if (K.sc[q].done) continue;
+            if (!act[q]) continue;
             const double *xv = MODE == 1 ? K.p[q] : K.r[q];
             const double y = A.apply(xv, K.ap_w[q], x);
             if (MODE == 1) {
                 K.v[q][c] = y;
-                red[q] = K.r0[q][c] * y;
+                red[q] += K.r0[q][c] * y;
             } else {
                 K.t[q][c] = y;
-                red[2 * q] = y * xv[c];
-                red[2 * q + 1] = y * y;
+                red[2 * q] += y * xv[c];
+                red[2 * q + 1] += y * y;
             }
         }
     }
-    if (grid_sum<NV>(red, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NS; q++) {
-            KryScal &sc = K.sc[q];
-            if (sc.done) continue;
-            if (MODE == 1) {
-                if (red[q] == 0.0) sc.done = 1;
-                else sc.alpha = sc.rho / red[q];
-            } else {
-                double ts = red[2 * q], tt = red[2 * q + 1];
-                sc.omega = tt > 0.0 ? ts / tt : 0.0;
-            }
-        }
-    }
-}
-
-// x += alpha M^-1 p ; s = r - alpha v (s overwrites r)
-template <class Op, int NS>
-__global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K) {
-    if (kdone(K)) return;
-    Idx x;
-    if (!A.cell(x)) return;
-    const int c = x.c;
+    double *dst[NV];
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-        if (K.sc[q].done) continue;
-        const double al = K.sc[q].alpha;
-        K.x[q][c] = K.x[q][c] + al * (K.p[q][c] / K.ap_w[q][c]);
-        K.r[q][c] = K.r[q][c] - al * K.v[q][c];
+        if (MODE == 1) dst[q] = K.red.pv[q];
+        else { dst[2 * q] = K.red.pt[q]; dst[2 * q + 1] = K.red.pt[q] + gridDim.x; }
+    }
+    put_partials<NV>(red, dst);
+}
+
+// alpha = rho / r0.v ; x += alpha M^-1 p ; s = r - alpha v (s overwrites r)
+template <class Op, int NS>
+__global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K, int it, int nb) {
+    if (kdone(K)) return;
+    double tot[NS];
+    const double *src[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) src[q] = K.red.pv[q];
+    sum_partials<NS>(src, nb, tot);
+    bool act[NS];
+    double al[NS];
+    bool any = false;
+    const int cq = it & 1;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        KryScal &sc = K.sc[q];
+        act[q] = !sc.done;
+        al[q] = 0.0;
+        if (!act[q]) continue;
+        if (tot[q] == 0.0) {                       // r0.v = 0: stop before the update
+            act[q] = false;
+            if (lead()) { sc.done = 1; sc.final_iters = sc.iters[cq]; }
+            continue;
+        }
+        al[q] = sc.rho[cq] / tot[q];
+        if (lead()) sc.alpha[cq] = al[q];
+        any = true;
+    }
+    if (!any) return;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int c = A.at(t).c;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            if (!act[q]) continue;
+            K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] / K.ap_w[q][c]);
+            K.r[q][c] = K.r[q][c] - al[q] * K.v[q][c];
+        }
     }
 }
 
-// x += omega M^-1 s ; r = s - omega t ; convergence, restart and next beta
+// omega = t.s / t.t ; x += omega M^-1 s ; r = s - omega t ; partials r.r, r0.r
 template <class Op, int NS>
-__global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, double *partials, unsigned *counter,
-                                                  int kmax) {
+__global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, int nb) {
     if (kdone(K)) return;
+    double tot[2 * NS];
+    const double *src[2 * NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) { src[2 * q] = K.red.pt[q]; src[2 * q + 1] = K.red.pt[q] + nb; }
+    sum_partials<2 * NS>(src, nb, tot);
+    bool act[NS];
+    double om[NS];
+    const int cq = it & 1;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        act[q] = !K.sc[q].done;
+        const double ts = tot[2 * q], tt = tot[2 * q + 1];
+        om[q] = tt > 0.0 ? ts / tt : 0.0;
+        if (act[q] && lead()) K.sc[q].omega[cq] = om[q];
+    }
     double red[2 * NS];
 #pragma unroll
     for (int q = 0; q < 2 * NS; q++) red[q] = 0.0;
-    Idx x;
-    if (A.cell(x)) {
-        const int c = x.c;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int c = A.at(t).c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
-            if (K.sc[q].done) continue;
-            const double om = K.sc[q].omega;
+            if (!act[q]) continue;
             const double sv = K.r[q][c];
-            K.x[q][c] = K.x[q][c] + om * (sv / K.ap_w[q][c]);
-            const double r = sv - om * K.t[q][c];
+            K.x[q][c] = K.x[q][c] + om[q] * (sv / K.ap_w[q][c]);
+            const double r = sv - om[q] * K.t[q][c];
             K.r[q][c] = r;
-            red[2 * q] = r * r;
-            red[2 * q + 1] = K.r0[q][c] * r;
+            red[2 * q] += r * r;
+            red[2 * q + 1] += K.r0[q][c] * r;
         }
     }
-    if (grid_sum<2 * NS>(red, partials, counter) && threadIdx.x == 0) {
+    double *dst[2 * NS];
 #pragma unroll
-        for (int q = 0; q < NS; q++) {
-            KryScal &sc = K.sc[q];
-            if (sc.done) continue;
-            sc.iters += 1;
-            sc.first = 0;
-            const double nr = sqrt(red[2 * q]);
-            sc.rnorm = nr;
-            const double r0r = red[2 * q + 1];
-            if (sc.omega == 0.0 || nr <= sc.thr || sc.iters >= kmax) {
-                sc.done = 1;
-            } else if (fabs(r0r) <= 1e-12 * nr * sc.r0norm) {
-                sc.restart = 1;
-                sc.rho = nr * nr;
-                sc.r0norm = nr;
-                sc.beta = 0.0;
-            } else {
-                sc.restart = 0;
-                sc.beta = (r0r / sc.rho) * (sc.alpha / sc.omega);
-                sc.rho = r0r;
-            }
-        }
-    }
+    for (int q = 0; q < NS; q++) { dst[2 * q] = K.red.pr[q]; dst[2 * q + 1] = K.red.pr[q] + gridDim.x; }
+    put_partials<2 * NS>(red, dst);
 }
 
 // solve() prologue on the reference's general cell list (solvers.f90:232-240)
 // and the BiCGStab start, for the drop-in cms2d_solve.
 __global__ void __launch_bounds__(BLOCK) k_ell_prologue(EllOp A, const int *iwet, double *ss, const double *sp,
-                                                        double *res, const double *phi, double relax, Par P,
-                                                        KrySys<1> K, double *partials, unsigned *counter) {
+                                                        double *res, const double *phi, double relax,
+                                                        KrySys<1> K) {
     double red[2] = {0.0, 0.0};
-    Idx x;
-    if (A.cell(x)) {
-        const int c = x.c;
-        const double rinv = 1.0 / relax, relaxm = 1.0 - relax;
+    const Chunk ch = block_chunk(A.ncells);
+    const double rinv = 1.0 / relax, relaxm = 1.0 - relax;
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += BLOCK) {
         double asum = 0.0, anb = 0.0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            int nb = A.c2c[4 * c + k];
-            if (nb < 0) continue;
+            int nbc = A.c2c[4 * c + k];
+            if (nbc < 0) continue;
             asum += A.a[4 * c + k];
-            anb += A.a[4 * c + k] * phi[nb];
+            anb += A.a[4 * c + k] * phi[nbc];
         }
         double ap = (asum - sp[c]) * rinv;
         double ph = phi[c];
@@ -208,10 +285,12 @@ __global__ void __launch_bounds__(BLOCK) k_ell_prologue(EllOp A, const int *iwet
         double r = anb + ssv - ap * ph + SMALL;
         K.ap_w[0][c] = ap;
         K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
-        red[0] = rs * rs;
-        red[1] = r * r;
+        red[0] += r * r;
+        red[1] += rs * rs;
     }
-    if (grid_sum<2>(red, partials, counter) && threadIdx.x == 0) kry_start(K.sc[0], red[0], red[1], A.ncells, P);
+    double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
+    put_partials<2>(red, dst);
+    kry_reset(K.sc[0]);
 }
 
 }  // namespace cms

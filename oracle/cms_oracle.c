@@ -1310,6 +1310,23 @@ void cms_oracle_volume(const cms_oracle *o, double *vol, double *bndvol)
     *bndvol = o->bndvol;
 }
 
+int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy)
+{
+    /* gradient of an active-cell field with the current wet-only CBGG weights;
+     * ghost values copy their cell (zero-gradient) */
+    double *v = DALLOC(o->ncellsD);
+    memcpy(v, var_act, sizeof(double) * o->ncells);
+    for (int i = 0; i < o->ncells; i++)
+        for (int k = 0; k < 4; k++) {
+            int nck = o->c2c[4 * i + k];
+            if (nck >= o->ncells) v[nck] = var_act[i];
+        }
+    der_grad_cbgg(o, o->gow, 1);
+    der_grad_eval(o, o->gow, v, gx, gy);
+    free(v);
+    return 0;
+}
+
 int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2c,
                      const int *iwet, double *ss, const double *sp, double *res,
                      double *phi, double relax, double rtol, double atol, int kmax,

@@ -81,6 +81,10 @@ double cms_oracle_wall_coef(double yp, double y0);
 double cms_oracle_man2drag(double g, double cman, double h, double hmax);
 double cms_oracle_drag2length(double cd, double h);
 
+/* Gradient of an active-cell field (row-major) with the current wet-only CBGG
+ * operator (der_grad_cbgg + der_grad_eval, der_lib.f90:272-378,726-839). */
+int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy);
+
 /* Stand-alone linear solve on a general cell list, the reference's
  * solve(acoef,ss,sp,res,phi,n) (solvers.f90:205-304) with the tight stopping rule.
  * acoef[4*ncellsD], c2c[4*ncellsD] (0-based, -1 none), iwet[ncellsD].

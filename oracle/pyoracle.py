@@ -60,6 +60,7 @@ def lib():
             f.restype = C.c_double
             f.argtypes = ([C.c_int] if fn == "cms_oracle_schmcoef" else []) + \
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
+        L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
         L.cms_oracle_solve.restype = C.c_int
         L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
                                        C.c_double, C.c_double, C.c_int, D]
@@ -131,6 +132,12 @@ class Oracle:
         if lib().cms_oracle_get_face(self.h, name.encode(), _dp(out)) != 0:
             raise KeyError(name)
         return out.reshape(self.ncells, 4)
+
+    def grad(self, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        gx, gy = np.empty(self.ncells), np.empty(self.ncells)
+        lib().cms_oracle_grad(self.h, _dp(a), _dp(gx), _dp(gy))
+        return gx, gy
 
     def volume(self):
         v, b = C.c_double(), C.c_double()

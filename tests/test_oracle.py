@@ -1,0 +1,171 @@
+"""CPU checks that pin the FP64 oracle without retyping its formulas.
+
+Each check comes from an independent source: Patankar's textbook forms of the
+advection-diffusion coefficients, the log law that ties Manning drag, roughness
+length and wall friction together, exactness of Green-Gauss gradients on linear
+fields, dense linear algebra, and the invariants of the shallow-water step
+(lake at rest, volume balance).  Reference-specific quirks that no textbook
+fixes are pinned by the compiled-reference vectors in test_reference_vectors.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cell_list import cart_cell_list, dense_system
+from cms2d_amd import cases
+from pyoracle import D, I, Oracle, lib
+
+
+def f4(x):
+    return float(np.float32(x))
+
+
+# ---------------------------------------------------------------- schemes
+@pytest.mark.parametrize("dk", [0.3, 1.0, 7.5])
+@pytest.mark.parametrize("ratio", [-0.9, -0.4, -1e-4, 0.25, 0.99])
+def test_exponential_scheme_is_patankar_within_unit_peclet(dk, ratio):
+    """For 1e-5 D < |F| <= D the scheme is Patankar's exponential a_E = F/(exp(F/D)-1)."""
+    fk = ratio * dk
+    a = lib().cms_oracle_schmcoef(4, dk, fk)
+    assert a == pytest.approx(fk / math.expm1(fk / dk), rel=1e-9)
+
+
+def test_exponential_scheme_vanishes_at_zero_flux():
+    """Reference quirk (comp_lib.f90:106): the denominator max(exp(P)-1, 1e-6)
+    makes the coefficient fall linearly to zero for |F| below ~1e-6 D."""
+    assert lib().cms_oracle_schmcoef(4, 2.0, 0.0) == 0.0
+    a = lib().cms_oracle_schmcoef(4, 2.0, 2.0e-8)
+    assert a == pytest.approx(2.0 * 1e-8 / f4(1e-6), rel=1e-6)
+
+
+def test_exponential_scheme_caps_peclet_at_one():
+    """Reference quirk (comp_lib.f90:105-107): P = |F|/max(D,|F|) <= 1."""
+    for dk, fk in ((1.0, 3.0), (2.0, -10.0)):
+        a = lib().cms_oracle_schmcoef(4, dk, fk)
+        assert a == pytest.approx(dk / (math.e - 1.0) + max(0.0, -fk), rel=1e-12)
+
+
+@pytest.mark.parametrize("dk,fk", [(1.0, 0.5), (1.0, -0.5), (1.0, 3.0), (2.0, -7.0), (0.5, 0.0)])
+def test_hybrid_and_upwind_are_patankar(dk, fk):
+    hyb = lib().cms_oracle_schmcoef(2, dk, fk)
+    assert hyb == pytest.approx(max(-fk, dk - fk / 2.0, 0.0), rel=1e-14)
+    upw = lib().cms_oracle_schmcoef(1, dk, fk)
+    assert upw == pytest.approx(dk + max(-fk, 0.0), rel=1e-15)
+
+
+def test_zero_diffusion_zero_flux_gives_zero_coefficient():
+    for nd in (1, 2, 3, 4):
+        assert lib().cms_oracle_schmcoef(nd, 0.0, 0.0) == 0.0
+
+
+# ---------------------------------------------------------------- friction
+def test_manning_drag_unit_depth_and_caps():
+    g, n = 9.81, f4(0.025)
+    L = lib()
+    assert L.cms_oracle_man2drag(g, n, 1.0, 10.0) == pytest.approx(g * n * n, rel=1e-15)
+    # h^(1/3) with the reference's 0.3333 exponent: 8 -> ~2
+    assert L.cms_oracle_man2drag(g, n, 8.0, 10.0) == pytest.approx(g * n * n / 2.0, rel=2e-4)
+    # depth capped at 10 m and floored at 0.1 n
+    assert L.cms_oracle_man2drag(g, n, 25.0, 10.0) == L.cms_oracle_man2drag(g, n, 10.0, 10.0)
+    assert L.cms_oracle_man2drag(g, n, 1e-6, 10.0) == L.cms_oracle_man2drag(g, n, 0.1 * n, 10.0)
+
+
+@pytest.mark.parametrize("cd,h", [(0.002, 1.0), (0.005, 3.7), (0.01, 0.4)])
+def test_roughness_length_inverts_the_log_law(cd, h):
+    """Depth-averaged log law: Cd = (kappa/(ln(h/z0)-1))^2 with kappa = 0.4."""
+    z0 = lib().cms_oracle_drag2length(cd, h)
+    kappa = f4(0.4)
+    assert (kappa / (math.log(h / z0) - 1.0)) ** 2 == pytest.approx(cd, rel=1e-12)
+
+
+def test_wall_coefficient_log_law():
+    kappa = f4(0.4)
+    yp = 5.0
+    y0 = yp / math.e ** 2
+    assert lib().cms_oracle_wall_coef(yp, y0) == pytest.approx((kappa / 2.0) ** 2, rel=1e-12)
+    # roughness is clipped below the distance to the wall
+    assert math.isfinite(lib().cms_oracle_wall_coef(yp, 2 * yp))
+
+
+# ---------------------------------------------------------------- operators
+def test_green_gauss_gradient_exact_on_linear_fields():
+    c = cases.basin(nx=20, ny=12, dx=10.0)
+    c.zb = np.full(c.ncells, -5.0)                    # all wet
+    o = Oracle(c)
+    jj, ii = np.divmod(np.arange(c.ncells), c.nx)
+    x, y = (ii + 0.5) * c.dx, (jj + 0.5) * c.dy
+    gx, gy = o.grad(0.3 * x - 0.7 * y + 2.0)
+    inner = (ii > 0) & (ii < c.nx - 1) & (jj > 0) & (jj < c.ny - 1)
+    assert np.abs(gx[inner] - 0.3).max() < 1e-13
+    assert np.abs(gy[inner] + 0.7).max() < 1e-13
+    # a constant field has zero gradient everywhere (weights sum to zero)
+    gx, gy = o.grad(np.full(c.ncells, 4.2))
+    assert np.abs(gx).max() < 1e-13 and np.abs(gy).max() < 1e-13
+
+
+def test_solve_matches_dense():
+    rng = np.random.default_rng(4)
+    act = rng.random((14, 17)) > 0.1
+    n, nd, c2c = cart_cell_list(act)
+    acoef = np.zeros((nd, 4))
+    acoef[:n] = rng.uniform(0.1, 3.0, (n, 4))
+    sp = np.zeros(nd)
+    sp[:n] = -rng.uniform(0.01, 1.0, n)
+    ss = np.zeros(nd)
+    ss[:n] = rng.normal(size=n)
+    phi = rng.normal(size=nd)
+    iwet = np.ones(nd, dtype=np.int32)
+    for relax in (1.0, f4(0.8)):
+        A, b = dense_system(n, acoef, c2c, sp, ss, phi, relax)
+        x = np.linalg.solve(A, b)
+        s2, p2, res = ss.copy(), phi.copy(), np.zeros(nd)
+        import ctypes as C
+        rm = C.c_double()
+        it = lib().cms_oracle_solve(n, nd, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
+                                    iwet.ctypes.data_as(I), s2.ctypes.data_as(D),
+                                    sp.ctypes.data_as(D), res.ctypes.data_as(D),
+                                    p2.ctypes.data_as(D), relax, 1e-12, 0.0, 2000, C.byref(rm))
+        assert it > 0
+        assert np.abs(p2[:n] - x).max() < 1e-9 * np.abs(x).max()
+        # rmom is the RMS of the normalized initial residual
+        r0 = (b - A @ phi[:n]) / np.diag(A)
+        assert rm.value == pytest.approx(math.sqrt((r0 ** 2).sum() / f4(n)), rel=1e-10)
+
+
+# ---------------------------------------------------------------- invariants
+def test_lake_at_rest_stays_at_rest():
+    c = cases.lake_at_rest()
+    o = Oracle(c)
+    for _ in range(5):
+        rc, info = o.step(1)
+        assert rc == 0
+    assert np.abs(o.get("u")).max() == 0.0
+    assert np.abs(o.get("v")).max() == 0.0
+    assert np.abs(o.get("eta")).max() == 0.0
+
+
+@pytest.mark.parametrize("serial", [0, 1])
+def test_volume_balance(serial):
+    """Water volume change equals the boundary inflow to the solver tolerance."""
+    c = cases.tidal(nx=40, ny=24, dx=20.0, seed=3)
+    o = Oracle(c, serial=serial)
+    v0, _ = o.volume()
+    for _ in range(15):
+        rc, _ = o.step(1)
+        assert rc == 0
+    v, b = o.volume()
+    assert abs((v - v0) - b) <= 1e-4 * abs(b) + 1e-3
+
+
+def test_serial_and_race_free_modes_agree_closely():
+    """The three loops whose OpenMP form races (dry-to-wet start values, eddy
+    viscosity fix-ups) change the result only at the level of the solver
+    tolerance on this case."""
+    c = cases.tidal(nx=40, ny=24, dx=20.0, seed=3)
+    a, b = Oracle(c, serial=0), Oracle(c, serial=1)
+    for _ in range(10):
+        a.step(1)
+        b.step(1)
+    assert np.array_equal(a.get("iwet"), b.get("iwet"))
+    assert np.abs(a.get("eta") - b.get("eta")).max() < 1e-8
