@@ -1,0 +1,193 @@
+!=============================================================================
+! ref_driver.f90 -- runs the reference's own implicit flow step on an ASCII
+! case and writes the state after every step.
+!
+! TEST INFRASTRUCTURE ONLY: a checker for the FP64 oracle.  Built by
+! oracle/build_ref.py against the reference's objects (compiled unmodified,
+! single precision as shipped) into oracle/_ref/ref_run.
+!
+! Usage: ref_run <case dir/> <case name> <nsteps> <output file> [forcing file]
+!   reads <case dir>/<case name>.cmcards (and the files it names) with the
+!   reference's card readers, initialises the model with the calls prestart
+!   makes (source/main/prestart.f90:2-329) except those of the output module
+!   (source/output/output.F90 needs Intel's IFPORT module and is not built;
+!   it writes result files only), then runs the time loop of CMS_Flow
+!   (source/main/CMS_Flow.f90:43-87): timing, flow_imp, flow_update.  After
+!   each step appends, as float64 stream data: ncells, then eta, u, v, iwet.
+!   The optional forcing file (float64 stream, 2*ncells values) replaces the
+!   radiation-stress arrays wavestrx, wavestry after initialisation; it is
+!   used with the CONSTANT_WAVE_HEIGHT card, which turns the wave branches on
+!   with zero wave height (source/waves/wave.F90:506-540).
+!=============================================================================
+program ref_run
+    use comvarbl,  only: ctlfile, flowpath, casename
+    use cms_def,   only: cmsflow, cmswave, noptset
+    use diag_def,  only: dgfile, dgunit, interactive
+    use geo_def,   only: telfile
+    use size_def,  only: ncells
+    use flow_def,  only: eta, u, v, iwet
+    use wave_flowgrid_def, only: wavestrx, wavestry, constant_waves
+    implicit none
+    character(len=400) :: dir, name, arg, outfile, forcefile
+    integer :: nsteps, n, i, ou, fu
+    real(8), allocatable :: buf(:)
+    interface
+      subroutine model_init()
+      end subroutine
+    end interface
+
+    call get_command_argument(1, dir)
+    call get_command_argument(2, name)
+    call get_command_argument(3, arg)
+    read(arg, *) nsteps
+    call get_command_argument(4, outfile)
+
+    ! the setup get_com_arg does (source/main/CMS2D_main.f90:110-390)
+    interactive = .false.
+    cmsflow = .true.
+    cmswave = .false.
+    noptset = 2
+    flowpath = trim(dir)
+    casename = trim(name)
+    ctlfile = trim(dir) // trim(name) // '.cmcards'
+    telfile = ' '
+    dgfile = trim(dir) // 'CMS_DIAG.txt'
+    dgunit = 9
+    open(dgunit, file=dgfile, status='unknown')
+    close(dgunit)
+
+    call model_init
+
+    forcefile = ' '
+    if (command_argument_count() >= 5) call get_command_argument(5, forcefile)
+    if (len_trim(forcefile) > 0) then
+      allocate(buf(2*ncells))
+      open(newunit=fu, file=trim(forcefile), access='stream', form='unformatted', status='old')
+      read(fu) buf
+      close(fu)
+      do i = 1, ncells
+        wavestrx(i) = buf(i)
+        wavestry(i) = buf(ncells + i)
+      enddo
+      deallocate(buf)
+    endif
+
+    open(newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
+    allocate(buf(ncells))
+    do n = 1, nsteps
+      call timing
+      if (constant_waves) call wave_const_update
+      call flow_imp
+      call flow_update
+      write(ou) real(ncells, 8)
+      do i = 1, ncells
+        buf(i) = real(eta(i), 8)
+      enddo
+      write(ou) buf
+      do i = 1, ncells
+        buf(i) = real(u(i), 8)
+      enddo
+      write(ou) buf
+      do i = 1, ncells
+        buf(i) = real(v(i), 8)
+      enddo
+      write(ou) buf
+      do i = 1, ncells
+        buf(i) = real(iwet(i), 8)
+      enddo
+      write(ou) buf
+    enddo
+    close(ou)
+end program ref_run
+
+!-----------------------------------------------------------------------------
+! The initialisation prestart performs (source/main/prestart.f90:63-329),
+! without the output module's routines, the hot start and the printouts.
+!-----------------------------------------------------------------------------
+subroutine model_init()
+    use comvarbl
+    use cms_def
+    use geo_def
+    use geo_lib,    only: assign_proj_names
+    use flow_def
+    use der_def
+    use der_lib,    only: der_grad_eval
+    use interp_lib, only: interp_scal_cell2face
+    use size_def
+    implicit none
+    integer :: ierr
+    character(len=37) :: cardname
+    logical :: foundcard
+
+    call geo_default
+    call flow_default
+    call solv_default
+    call fric_default
+    call bnd_default
+    call struct_default
+    call hot_default
+    call sed_default
+    call sal_default
+    call heat_default
+    call met_default
+    call rol_default
+    call stat_default
+    call der_default
+    call diag_default
+    call wave_default
+    call dredge_default
+
+    open(77, file=ctlfile, status='old')
+    do
+      read(77, *, iostat=ierr) cardname
+      if (ierr /= 0) exit
+      if (cardname(1:14) == 'END_PARAMETERS') exit
+      if (cardname(1:1) == '!' .or. cardname(1:1) == '#' .or. cardname(1:1) == '*') cycle
+      call ignr_cards(cardname, foundcard);          if (foundcard) cycle
+      call geo_cards(cardname, foundcard, .true.);   if (foundcard) cycle
+      call flow_cards(cardname, foundcard, .true.);  if (foundcard) cycle
+      call bnd_cards(cardname, foundcard);           if (foundcard) cycle
+      call struct_cards(cardname, foundcard);        if (foundcard) cycle
+      call fric_cards(cardname, foundcard);          if (foundcard) cycle
+      call hot_cards(cardname, foundcard);           if (foundcard) cycle
+      call sed_cards(cardname, foundcard);           if (foundcard) cycle
+      call sal_cards(cardname, foundcard);           if (foundcard) cycle
+      call heat_cards(cardname, foundcard);          if (foundcard) cycle
+      call met_cards(cardname, foundcard);           if (foundcard) cycle
+      call rol_cards(cardname, foundcard);           if (foundcard) cycle
+      call diag_cards(cardname, foundcard);          if (foundcard) cycle
+      call wave_cards(cardname, foundcard);          if (foundcard) cycle
+      call dredge_cards(cardname, foundcard);        if (foundcard) cycle
+      call der_cards(cardname, foundcard);           if (foundcard) cycle
+      call stat_cards(cardname, foundcard)
+    enddo
+    close(77)
+    call assign_proj_names(projfl)
+
+    call read_grid_cart
+    call geo_init
+    call bnd_init
+    call struct_init
+    call solv_init
+    call flow_init
+    call interp_init
+    call der_init
+    call fric_init
+    call met_init
+    call stat_init
+    call wave_init
+
+    call flow_wetdry(0)
+    if (sum(1 - iwet(1:ncells)) == ncells) then
+      ponding = .true.
+      call flow_wetdry(0)
+    endif
+    iwet1 = iwet
+    eta = iwet*p*gravinv - 999.0*(1 - iwet)
+    call der_grad_eval(goa, 0, zb, dzbx, dzby)
+    call interp_scal_cell2face(zb, 0, zbk, dzbx, dzby)
+    call flow_grad_interp
+    call flow_convflux
+    call fric_eval
+    call flow_eddyvis
+end subroutine model_init

@@ -1,0 +1,115 @@
+"""Writes a Case (cms2d_amd.cases) as the reference's own ASCII inputs and runs
+the reference build (oracle/_ref/ref_run) on it.
+
+Inputs written: <name>.cmcards, <name>.cart (GRID_* cards read by
+read_grid_cart, source/geospatial/geospatial.F90:2553-2706), <name>.bid (one
+cellstring, bnd_read_str_bid, source/boundaries/bnd_lib.f90:138-178),
+<name>_mann.txt (readscalTxt, source/input/in_lib.f90:68-119).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_RUN = os.path.join(ROOT, "oracle", "_ref", "ref_run")
+
+
+def available():
+    return os.path.exists(REF_RUN)
+
+
+def write_case(case, d, name="case", solver="BICGSTAB"):
+    os.makedirs(d, exist_ok=True)
+    nx, ny = case.nx, case.ny
+    depth = -np.asarray(case.zb, dtype=np.float32)          # .cart holds depths (zb = -Z)
+    with open(os.path.join(d, name + ".cart"), "w") as f:
+        f.write("GRID_TYPE REGULAR\n")
+        f.write(f"GRID_NUMBER_CELLS_X {nx}\nGRID_NUMBER_CELLS_Y {ny}\n")
+        f.write("GRID_RESOLUTION_X_BEGIN\n" + "".join(f"{case.dx:.9g}\n" for _ in range(nx))
+                + "GRID_RESOLUTION_X_END\n")
+        f.write("GRID_RESOLUTION_Y_BEGIN\n" + "".join(f"{case.dy:.9g}\n" for _ in range(ny))
+                + "GRID_RESOLUTION_Y_END\n")
+        f.write("GRID_DEPTHS_BEGIN\n" + "".join(f"{v:.9g}\n" for v in depth) + "GRID_DEPTHS_END\n")
+    with open(os.path.join(d, name + "_mann.txt"), "w") as f:
+        f.write(f"{nx * ny} 1\n")
+        f.write("".join(f"{float(np.float32(v)):.9g}\n" for v in case.mann))
+    side = case.th_side
+    if side == 1:
+        ids = [nx + j * nx for j in range(ny)]                  # east column, full-grid ids
+    elif side == 3:
+        ids = [1 + j * nx for j in range(ny)]
+    elif side == 0:
+        ids = [i + 1 + (ny - 1) * nx for i in range(nx)]
+    else:
+        ids = [i + 1 for i in range(nx)]
+    with open(os.path.join(d, name + ".bid"), "w") as f:
+        f.write("1\n")
+        f.write(f"{len(ids)}\n")
+        f.write(" ".join(str(v) for v in ids) + "\n")
+    harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"
+                   for s, a, p in zip(case.speed_deg_hr, case.amp, case.phase_deg))
+    cards = f"""CMS_VERSION                         5.4
+GRID_ANGLE                          0.0
+GRID_ORIGIN_X                       0.0
+GRID_ORIGIN_Y                       0.0
+GRID_FILE                           "{name}.cart"
+USE_WALL_FRICTION_TERMS             {"ON" if case.wallfric else "OFF"}
+DRYING_DEPTH                        {case.hdry:.9g}
+HYDRO_TIMESTEP                      {case.dt:.9g} 'sec'
+STARTING_DATE_TIME                  2001-01-01 00:00:00 UTC
+DURATION_RUN                        {max(1.0, case.dt * case.nsteps / 3600.0):.9g} 'hrs'
+VARIABLE_TIMESTEP                   OFF
+SOLUTION_SCHEME                     IMPLICIT
+MATRIX_SOLVER                       {solver}
+MANNINGS_N_DATASET                  "{name}_mann.txt" "ManningsN"
+{"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
+OUTPUT_FILE_TYPE                    ASCII
+HORIZONTAL_PROJECTION_BEGIN
+  DATUM                             None
+  COORDINATE_SYSTEM                 NONE
+  UNITS                             METERS
+  ZONE                              -1
+HORIZONTAL_PROJECTION_END
+VERTICAL_PROJECTION_BEGIN
+  DATUM                             LOCAL
+  UNITS                             METERS
+VERTICAL_PROJECTION_END
+BOUNDARY_BEGIN
+  NAME "tide"
+  CELLSTRING "{name}.bid" "1"
+  HARMONIC_BEGIN
+{harm}  HARMONIC_END
+BOUNDARY_END
+END_PARAMETERS
+"""
+    with open(os.path.join(d, name + ".cmcards"), "w") as f:
+        f.write(cards)
+    return d
+
+
+def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600):
+    """Returns a list of dicts (eta, u, v, iwet) after each step."""
+    write_case(case, d, name, solver)
+    out = os.path.join(d, "state.bin")
+    args = [REF_RUN, d.rstrip("/") + "/", name, str(nsteps), out]
+    if case.noptset >= 3 and case.wavestrx is not None:
+        force = os.path.join(d, "forcing.bin")
+        np.concatenate([np.asarray(case.wavestrx, np.float64),
+                        np.asarray(case.wavestry, np.float64)]).tofile(force)
+        args.append(force)
+    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"ref_run failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
+    raw = np.fromfile(out, dtype=np.float64)
+    res = []
+    pos = 0
+    while pos < raw.size:
+        n = int(raw[pos])
+        pos += 1
+        blk = raw[pos:pos + 4 * n].reshape(4, n)
+        pos += 4 * n
+        res.append({"eta": blk[0], "u": blk[1], "v": blk[2], "iwet": blk[3]})
+    return res

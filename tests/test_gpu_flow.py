@@ -65,3 +65,14 @@ def test_tidal_wetdry_radstress_matches_oracle(gpu_available):
     o, g = run_pair(c, 40)
     err = compare(o, g)
     print("max err", err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["basin", "tidal"])
+def test_gpu_matches_reference_vectors(gpu_available, name):
+    """The HIP path against the compiled reference's own outputs."""
+    from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold
+    print(f"reference vectors {name}", flush=True)
+    w = compare_to_gold(name, lambda c: FlowStep(c))
+    print(name, w)
+    assert w["eta"] <= TOL_ETA and w["u"] <= TOL_VEL and w["v"] <= TOL_VEL, w

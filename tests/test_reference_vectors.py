@@ -1,0 +1,59 @@
+"""The FP64 oracle against the reference's own implicit step.
+
+Golden vectors in tests/golden/ref_*.npz come from the reference's Fortran
+sources compiled unmodified (tests/golden/make_ref_vectors.py, oracle/build_ref.py).
+The reference build runs in single precision as shipped and stops each
+BiCGStab solve at a 1e-3 residual reduction (pbicgstab.f90:95), so agreement
+is bounded by those: water level within 2e-7 m, velocities within 5e-6 m/s,
+wet/dry flags identical.  Any dropped term or wrong sign in the restatement
+shows up orders of magnitude above these bounds (the mutation test below).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cms2d_amd import cases
+from pyoracle import Oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+TOL_ETA, TOL_VEL = 2e-7, 5e-6
+
+CASES = {
+    "basin": (lambda: cases.basin(nx=40, ny=20, dx=10.0, seed=0)),
+    "tidal": (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1)),
+}
+
+
+def compare_to_gold(name, model_factory):
+    g = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    m = model_factory(CASES[name]())
+    step = 0
+    worst = {"eta": 0.0, "u": 0.0, "v": 0.0}
+    for s in g["steps"]:
+        while step < s:
+            m.step(1)
+            step += 1
+        iw = g[f"iwet_{s}"]
+        assert np.array_equal(m.get("iwet"), iw), (name, s)
+        wet = iw > 0
+        for f in ("eta", "u", "v"):
+            worst[f] = max(worst[f], np.abs(m.get(f) - g[f"{f}_{s}"])[wet].max())
+    return worst
+
+
+@pytest.mark.parametrize("name", ["basin", "tidal"])
+def test_oracle_matches_reference_vectors(name):
+    w = compare_to_gold(name, lambda c: Oracle(c, serial=1))
+    print(name, w)
+    assert w["eta"] <= TOL_ETA and w["u"] <= TOL_VEL and w["v"] <= TOL_VEL, w
+
+
+def test_mutated_case_is_caught():
+    """A plausible slip -- Manning n off by 4 % -- already fails the bounds."""
+    def mutated(c):
+        c.mann = c.mann * 1.04
+        return Oracle(c, serial=1)
+    w = compare_to_gold("tidal", mutated)
+    assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
