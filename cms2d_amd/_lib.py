@@ -20,6 +20,8 @@ BND_WALL = 0
 BND_WSE_HARMONIC = 1
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
+PRECOND_MG = 0
+PRECOND_JACOBI = 1
 
 ERR = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "diverged", -4: "no device"}
 
@@ -40,7 +42,7 @@ class FlowDesc(C.Structure):
         ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
-        ("device", C.c_int),
+        ("device", C.c_int), ("precond", C.c_int),
     ]
 
 

@@ -6,6 +6,7 @@
 // iteration (Krylov flags and residual norms, wet/dry change flags).
 #include "flow_kernels.hip"
 #include "krylov_kernels.hip"
+#include "mg_kernels.hip"
 #include "../../include/cms2d_amd.h"
 
 #include <hip/hip_ext.h>
@@ -42,6 +43,8 @@ struct cms2d_flow {
     St s{};
     KrySys<2> Kuv{};
     KrySys<1> Kpp{};
+    int precond = CMS2D_PRECOND_MG;
+    Mg mg{};                   // pp preconditioner hierarchy (level 0 z is Kpp.zp / Kpp.zs)
     KryScal *d_sc = nullptr;   // [3]: pp, u, v
     KryScal *h_sc = nullptr;   // pinned mirror
     int *h_flags = nullptr;    // pinned
@@ -136,11 +139,16 @@ struct KryCtx {
     long *ev_n;
 };
 
+struct NoPrec {
+    void operator()(int) const {}
+};
+
 // Runs BiCGStab iterations in batches until every system has stopped.  A
 // batch is enqueued without host synchronisation; kernels launched after a
 // system has stopped return at once (they read the device-side done flag).
-template <class Op, int NS>
-static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
+// PREC 1: pre(0) enqueues zp = M^-1 p, pre(1) enqueues zs = M^-1 s.
+template <class Op, int NS, int PREC = 0, class Pre = NoPrec>
+static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0, Pre pre = Pre()) {
     const dim3 grid(X.nblk), blk(BLOCK);
     // start: thresholds, rmom and the initial stop test
     hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
@@ -159,18 +167,20 @@ static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
         const int it0 = it;
         for (int b = 0; b < nb; b++, it++) {
             hipLaunchKernelGGL((k_bicg_p<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
+            if (PREC) pre(0);
             bool rec = X.ev && nev < (int)X.ev->size() / 2;
             if (rec) {
                 // events stamped by the dispatch itself (start and end of this kernel)
-                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
+                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
                                       (*X.ev)[2 * nev + 1], 0, A, K);
                 nev++;
             } else {
-                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1>), grid, blk, 0, X.st, A, K);
+                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC>), grid, blk, 0, X.st, A, K);
             }
-            hipLaunchKernelGGL((k_bicg_s<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk);
-            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2>), grid, blk, 0, X.st, A, K);
-            hipLaunchKernelGGL((k_bicg_r<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk);
+            hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC>), grid, blk, 0, X.st, A, K, it, X.nblk);
+            if (PREC) pre(1);
+            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC>), grid, blk, 0, X.st, A, K);
+            hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC>), grid, blk, 0, X.st, A, K, it, X.nblk);
         }
         hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 1);
         HIPCHK(hipGetLastError());
@@ -192,6 +202,43 @@ static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0) {
     }
     for (int q = 0; q < NS; q++) X.prev_iters[sc0 + q] = std::max(1, X.h_sc[sc0 + q].final_iters);
     return 0;
+}
+
+static int mg_nblk(const MgLevel &L) { return (L.nx * L.ny + BLOCK - 1) / BLOCK; }
+
+// Builds the hierarchy from the assembled pp system (after the solve prologue
+// has written ap).
+static void mg_setup(cms2d_flow *F) {
+    Mg &M = F->mg;
+    const St &s = F->s;
+    LAUNCH(k_mg_setup0, mg_nblk(M.L[0]), F->g, s.aN, s.aE, s.aS, s.aW, F->Kpp.ap_w[0], M.L[0]);
+    for (int l = 0; l + 1 < M.nlev; l++) LAUNCH(k_mg_coarsen, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1]);
+}
+
+// z0 = V(b0): one V(1,1) cycle (mg.h) on the stream of F.
+static void mg_vcycle(cms2d_flow *F, const double *b0, float *z0) {
+    const Mg &M = F->mg;
+    const KryScal *sc = F->d_sc;      // pp system: stop when its solve has
+    MgLevel L0 = M.L[0];
+    L0.z = z0;
+    if (M.nlev == 1) {
+        hipLaunchKernelGGL(k_mg_coarsest<double>, dim3(1), dim3(1024), 0, F->st, L0, b0, sc);
+        return;
+    }
+    LAUNCH(k_mg_pre<double>, mg_nblk(L0), L0, b0, sc);
+    LAUNCH(k_mg_restrict, mg_nblk(M.L[1]), L0, M.L[1], sc);
+    for (int l = 1; l + 1 < M.nlev; l++) {
+        LAUNCH(k_mg_pre<float>, mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, sc);
+        LAUNCH(k_mg_restrict, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1], sc);
+    }
+    const MgLevel &Lc = M.L[M.nlev - 1];
+    hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(1024), 0, F->st, Lc, (const float *)Lc.b, sc);
+    for (int l = M.nlev - 2; l >= 1; l--) {
+        LAUNCH((k_mg_post<float, 1>), mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+        LAUNCH((k_mg_post<float, 0>), mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+    }
+    LAUNCH((k_mg_post<double, 1>), mg_nblk(L0), L0, b0, M.L[1], sc);
+    LAUNCH((k_mg_post<double, 0>), mg_nblk(L0), L0, b0, M.L[1], sc);
 }
 
 static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
@@ -276,7 +323,16 @@ static int flow_imp(cms2d_flow *F) {
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
             KryCtx X = flow_ctx(F, true);
-            if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
+            if (F->precond == CMS2D_PRECOND_MG) {
+                mg_setup(F);
+                auto pre = [F](int which) {
+                    if (which == 0) mg_vcycle(F, F->Kpp.p[0], F->Kpp.zp);
+                    else mg_vcycle(F, F->Kpp.r[0], F->Kpp.zs);
+                };
+                if (int rc = run_bicg<StencilOp, 1, 1>(X, F->Kpp, A, 0, pre)) return rc;
+            } else {
+                if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
+            }
         }
         LAUNCH(k_pvfc1, F->nblk_act, g, s);
         LAUNCH(k_pvfc2, F->nblk_pad, g, s, F->P);
@@ -358,6 +414,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC)
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
     if (d->precision != CMS2D_PREC_FP64) return set_err(CMS2D_ERR_ARG, "precision form not available");
+    if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
+        return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device)
         return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
@@ -409,8 +467,6 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     for (uint8_t **f : bfields)
         if (int rc = dalloc(F, f, N)) return rc;
     if (int rc = dalloc(F, &s.flags, 4)) return rc;
-    if (int rc = dalloc(F, &s.partials, (size_t)F->nblk_act * 8 + 64)) return rc;
-    if (int rc = dalloc(F, &s.counter, 1)) return rc;
     {
         // Krylov partial sums: 5 slots per system (pr: 2, pv: 1, pt: 2)
         double *red;
@@ -439,6 +495,31 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             F->Kuv.v[q] = kv[q + 1][3]; F->Kuv.t[q] = kv[q + 1][4]; F->Kuv.ap_w[q] = kv[q + 1][5];
         }
         F->Kuv.sc = F->d_sc + 1;
+    }
+    // multigrid hierarchy: level 0 shares the padded layout of the flow arrays
+    F->precond = d->precond;
+    {
+        Mg &M = F->mg;
+        int nx = g.nx, ny = g.ny;
+        M.nlev = 0;
+        for (;;) {
+            MgLevel &L = M.L[M.nlev];
+            L.nx = nx; L.ny = ny; L.nxp = nx + 2;
+            const size_t n = (size_t)(nx + 2) * (ny + 2);
+            float **arrs[] = {&L.aN, &L.aE, &L.aS, &L.aW, &L.dinv, &L.e};
+            for (float **a : arrs)
+                if (int rc = dalloc(F, a, n)) return rc;
+            if (M.nlev > 0) {
+                if (int rc = dalloc(F, &L.z, n)) return rc;
+                if (int rc = dalloc(F, &L.b, n)) return rc;
+            }
+            M.nlev++;
+            if ((long)nx * ny <= MG_COARSE_MAX || M.nlev == MG_MAX_LEVELS || nx < 2 || ny < 2) break;
+            nx = (nx + 1) / 2;
+            ny = (ny + 1) / 2;
+        }
+        if (int rc = dalloc(F, &F->Kpp.zp, N)) return rc;
+        if (int rc = dalloc(F, &F->Kpp.zs, N)) return rc;
     }
     HIPCHK(hipHostMalloc((void **)&F->h_sc, 3 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));

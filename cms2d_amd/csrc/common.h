@@ -84,32 +84,4 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     }
 }
 
-// Writes this block's NV partial sums; returns true in the one block that
-// finishes last, in which every thread then holds the grid totals in v (summed
-// in block order, so the result does not depend on scheduling).
-template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double *partials, unsigned *counter) {
-    block_sum<NV>(v);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) partials[blockIdx.x * NV + k] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = 0.0;
-    const volatile double *vp = partials;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLOCK)
-#pragma unroll
-        for (int k = 0; k < NV; k++) v[k] += vp[b * NV + k];
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
-
 }  // namespace cms

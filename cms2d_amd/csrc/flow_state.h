@@ -19,8 +19,6 @@ struct St {
     double *wseadj;            // [4*lmax] adjusted boundary water level per side
     uint8_t *iwet, *iwet1, *icorner, *ipond;
     int *flags;                // [0] narrow-channel change, [1] ponding change, [2] newly wet cells
-    double *partials;          // reduction scratch [nblocks*8]
-    unsigned *counter;         // last-block counter (returns to 0 after every reduction)
 };
 
 struct Par {

@@ -4,7 +4,9 @@
 //   p = r + beta*(p - omega*v);  v = A M^-1 p;  alpha = rho/(r0.v)
 //   x += alpha M^-1 p;  s = r - alpha v;  t = A M^-1 s;  omega = (t.s)/(t.t)
 //   x += omega M^-1 s;  r = s - omega t
-// with M = diag(ap) (Jacobi) and the standard beta = (rho/rho_old)(alpha/omega).
+// with M = diag(ap) (Jacobi, applied inline) or an explicit preconditioner
+// whose z = M^-1 p and M^-1 s are computed between the kernels (PREC = 1,
+// one system), and the standard beta = (rho/rho_old)(alpha/omega).
 // The reference's beta denominator carries +1e-15 (pbicgstab.f90:43), which
 // stalls the recurrence once rho < ~1e-13, so the tight solve instead restarts
 // the shadow residual when rho breaks down.  Up to NS systems that share the
@@ -84,6 +86,17 @@ struct StencilOp {
         const double sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
         return ap[c] * zc - sum;
     }
+    // y_i = ap_i z_i - sum_k a_k(i) z_nb(k) for an explicit preconditioned
+    // vector z (FP32, zero on the ghost ring).
+    __device__ __forceinline__ double apply_z(const float *z, const double *ap, const Idx &x) const {
+        const int c = x.c;
+        const double zN = x.j == g.ny - 1 ? 0.0 : (double)z[c + g.nxp];
+        const double zE = x.i == g.nx - 1 ? 0.0 : (double)z[c + 1];
+        const double zS = x.j == 0 ? 0.0 : (double)z[c - g.nxp];
+        const double zW = x.i == 0 ? 0.0 : (double)z[c - 1];
+        const double sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
+        return ap[c] * (double)z[c] - sum;
+    }
 };
 
 // The reference's general cell list: acoef(4,ncellsD) and cell2cell(4,ncellsD)
@@ -115,6 +128,7 @@ template <int NS>
 struct KrySys {
     double *x[NS], *r[NS], *r0[NS], *p[NS], *v[NS], *t[NS];
     double *ap_w[NS];  // diagonal ap of each system (written by the prologue)
+    float *zp, *zs;    // M^-1 p and M^-1 s of system 0 (explicit preconditioner only)
     KryScal *sc;       // [NS]
     KryRed<NS> red;
 };

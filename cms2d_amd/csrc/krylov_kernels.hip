@@ -137,9 +137,10 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS> K, int it, in
 }
 
 // MODE 1: v = A M^-1 p, partials r0.v.  MODE 2: t = A M^-1 s (s held in r),
-// partials t.s and t.t.
-template <class Op, int NS, int MODE>
+// partials t.s and t.t.  PREC 1: M^-1 p and M^-1 s are the explicit zp, zs.
+template <class Op, int NS, int MODE, int PREC = 0>
 __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
+    static_assert(PREC == 0 || NS == 1, "explicit preconditioner: one system");
     if (kdone(K)) return;
     constexpr int NV = MODE == 1 ? NS : 2 * NS;
     double red[NV];
@@ -156,7 +157,9 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
             const double *xv = MODE == 1 ? K.p[q] : K.r[q];
-            const double y = A.apply(xv, K.ap_w[q], x);
+            double y;
+            if constexpr (PREC == 1) y = A.apply_z(MODE == 1 ? K.zp : K.zs, K.ap_w[q], x);
+            else y = A.apply(xv, K.ap_w[q], x);
             if (MODE == 1) {
                 K.v[q][c] = y;
                 red[q] += K.r0[q][c] * y;
@@ -176,8 +179,9 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
     put_partials<NV>(red, dst);
 }
 
-// alpha = rho / r0.v ; x += alpha M^-1 p ; s = r - alpha v (s overwrites r)
-template <class Op, int NS>
+// alpha = rho / r0.v ; x += alpha M^-1 p ; s = r - alpha v (s overwrites r).
+// PREC 1 defers the x update to k_bicg_r.
+template <class Op, int NS, int PREC = 0>
 __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[NS];
@@ -211,14 +215,15 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K, int it, in
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
-            K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] / K.ap_w[q][c]);
+            if constexpr (PREC == 0) K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] / K.ap_w[q][c]);
             K.r[q][c] = K.r[q][c] - al[q] * K.v[q][c];
         }
     }
 }
 
 // omega = t.s / t.t ; x += omega M^-1 s ; r = s - omega t ; partials r.r, r0.r
-template <class Op, int NS>
+// PREC 1: x += alpha zp + omega zs.
+template <class Op, int NS, int PREC = 0>
 __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[2 * NS];
@@ -227,13 +232,14 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, in
     for (int q = 0; q < NS; q++) { src[2 * q] = K.red.pt[q]; src[2 * q + 1] = K.red.pt[q] + nb; }
     sum_partials<2 * NS>(src, nb, tot);
     bool act[NS];
-    double om[NS];
+    double om[NS], al[NS];
     const int cq = it & 1;
 #pragma unroll
     for (int q = 0; q < NS; q++) {
         act[q] = !K.sc[q].done;
         const double ts = tot[2 * q], tt = tot[2 * q + 1];
         om[q] = tt > 0.0 ? ts / tt : 0.0;
+        al[q] = K.sc[q].alpha[cq];
         if (act[q] && lead()) K.sc[q].omega[cq] = om[q];
     }
     double red[2 * NS];
@@ -246,7 +252,10 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, in
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
             const double sv = K.r[q][c];
-            K.x[q][c] = K.x[q][c] + om[q] * (sv / K.ap_w[q][c]);
+            if constexpr (PREC == 1)
+                K.x[q][c] = K.x[q][c] + (al[q] * (double)K.zp[c] + om[q] * (double)K.zs[c]);
+            else
+                K.x[q][c] = K.x[q][c] + om[q] * (sv / K.ap_w[q][c]);
             const double r = sv - om[q] * K.t[q][c];
             K.r[q][c] = r;
             red[2 * q] += r * r;

@@ -1,0 +1,190 @@
+!=============================================================================
+! cms2d_hip_mod.f90 -- Fortran host layer of the MI355X implicit CMS-Flow step.
+!
+! ISO_C_BINDING interfaces to the C ABI in include/cms2d_amd.h and the
+! reference-shaped wrappers a Fortran caller uses:
+!   cms2d_solve_ref   -- solve(acoef,ss,sp,res,phi,n) on the reference's cell
+!                        list (source/solvers/solvers.f90:205-304), with the
+!                        reference's array shapes acoef(nmaxfaces,ncellsD),
+!                        1-based cell2cell(nmaxfaces,ncellsD) and iwet(ncellsD)
+!   cms2d_flow_*      -- the whole implicit step (flow_imp,
+!                        source/flow/flow_imp.F90:29-312) kept on the device,
+!                        driven from a Fortran time loop as CMS_Flow does
+!                        (source/main/CMS_Flow.f90:43-87).
+!=============================================================================
+module cms2d_hip
+    use iso_c_binding
+    implicit none
+    private
+
+    integer(c_int), parameter, public :: CMS2D_OK = 0, CMS2D_ERR_ARG = -1, CMS2D_ERR_HIP = -2, &
+                                         CMS2D_ERR_DIVERGED = -3, CMS2D_ERR_NODEVICE = -4
+    integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1
+    integer(c_int), parameter, public :: CMS2D_PREC_FP64 = 0
+    integer(c_int), parameter, public :: CMS2D_PRECOND_MG = 0, CMS2D_PRECOND_JACOBI = 1
+
+    ! struct cms2d_flow_desc
+    type, bind(C), public :: cms2d_flow_desc
+        integer(c_int) :: nx = 0, ny = 0
+        real(c_double) :: dx = 0, dy = 0
+        type(c_ptr) :: zb = c_null_ptr, mann = c_null_ptr, wavestrx = c_null_ptr, wavestry = c_null_ptr
+        integer(c_int) :: side_type(4) = 0
+        integer(c_int) :: ntc = 0
+        real(c_double) :: amp(8) = 0, speed(8) = 0, phase(8) = 0
+        real(c_double) :: dt = 0, rampdur = 0, hdry = 0
+        integer(c_int) :: noptset = 2, wallfric = 1, ndsch = 4, maxit = 0
+        real(c_double) :: rtol = 1.0d-10, atol = 1.0d-12
+        integer(c_int) :: kmax = 4000, precision = 0, device = 0
+        integer(c_int) :: precond = 0
+    end type
+
+    ! struct cms2d_step_info
+    type, bind(C), public :: cms2d_step_info
+        integer(c_int) :: niter = 0, iconv = 0
+        real(c_double) :: rmom(3) = 0
+        integer(c_int) :: kry_iters(3) = 0
+        integer(c_int) :: nwet = 0
+        real(c_double) :: kernel_ms = 0
+    end type
+
+    interface
+        function cms2d_last_error() bind(C, name='cms2d_last_error') result(p)
+            import :: c_ptr
+            type(c_ptr) :: p
+        end function
+        function cms2d_device_count(n) bind(C, name='cms2d_device_count') result(rc)
+            import :: c_int
+            integer(c_int), intent(out) :: n
+            integer(c_int) :: rc
+        end function
+        function c_flow_create(desc, flow) bind(C, name='cms2d_flow_create') result(rc)
+            import :: c_int, c_ptr, cms2d_flow_desc
+            type(cms2d_flow_desc), intent(in) :: desc
+            type(c_ptr), intent(out) :: flow
+            integer(c_int) :: rc
+        end function
+        function c_flow_destroy(flow) bind(C, name='cms2d_flow_destroy') result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: flow
+            integer(c_int) :: rc
+        end function
+        function c_flow_step(flow, nsteps, info) bind(C, name='cms2d_flow_step') result(rc)
+            import :: c_int, c_ptr, cms2d_step_info
+            type(c_ptr), value :: flow
+            integer(c_int), value :: nsteps
+            type(cms2d_step_info), intent(out) :: info
+            integer(c_int) :: rc
+        end function
+        function c_flow_get(flow, name, out) bind(C, name='cms2d_flow_get') result(rc)
+            import :: c_int, c_ptr, c_char, c_double
+            type(c_ptr), value :: flow
+            character(kind=c_char), intent(in) :: name(*)
+            real(c_double), intent(out) :: out(*)
+            integer(c_int) :: rc
+        end function
+        function c_solve(ncells, ncellsD, acoef, c2c, iwet, ss, sp, res, phi, relax, rtol, atol, kmax, &
+                         rmom, iters) bind(C, name='cms2d_solve') result(rc)
+            import :: c_int, c_double
+            integer(c_int), value :: ncells, ncellsD, kmax
+            real(c_double), intent(in) :: acoef(*), sp(*)
+            integer(c_int), intent(in) :: c2c(*), iwet(*)
+            real(c_double), intent(inout) :: ss(*), phi(*)
+            real(c_double), intent(out) :: res(*)
+            real(c_double), value :: relax, rtol, atol
+            real(c_double), intent(out) :: rmom
+            integer(c_int), intent(out) :: iters
+            integer(c_int) :: rc
+        end function
+    end interface
+
+    public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
+    public :: cms2d_error_message, cms2d_device_count
+
+contains
+
+    function cms2d_error_message() result(msg)
+        character(len=256) :: msg
+        character(kind=c_char), pointer :: s(:)
+        type(c_ptr) :: p
+        integer :: i
+        msg = ' '
+        p = cms2d_last_error()
+        if (.not. c_associated(p)) return
+        call c_f_pointer(p, s, [256])
+        do i = 1, 256
+            if (s(i) == c_null_char) exit
+            msg(i:i) = s(i)
+        end do
+    end function
+
+    ! solve(acoef,ss,sp,res,phi,n) with the reference's shapes.  cell2cell is
+    ! 1-based as in geo_def (0 = no face); relax = relax(n) of comvarbl.
+    ! Returns the status; rmom = sqrt(adak(res)/ncells), iters = Krylov iterations.
+    function cms2d_solve_ref(ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi, relax, &
+                             rmom, iters, rtol) result(rc)
+        integer, intent(in) :: ncells, ncellsD, nmaxfaces
+        real(8), intent(in) :: acoef(nmaxfaces, ncellsD), sp(ncellsD)
+        integer, intent(in) :: cell2cell(nmaxfaces, ncellsD), iwet(ncellsD)
+        real(8), intent(inout) :: ss(ncellsD), phi(ncellsD)
+        real(8), intent(out) :: res(ncellsD)
+        real(8), intent(in) :: relax
+        real(8), intent(out) :: rmom
+        integer, intent(out) :: iters
+        real(8), intent(in), optional :: rtol
+        integer(c_int) :: rc
+        integer(c_int), allocatable :: c2c(:)
+        real(c_double), allocatable :: a4(:)
+        integer(c_int), allocatable :: iw(:)
+        integer :: i, k
+        real(c_double) :: tol
+        integer(c_int) :: it
+        if (nmaxfaces /= 4) then
+            rc = CMS2D_ERR_ARG
+            return
+        end if
+        allocate(c2c(4*ncellsD), a4(4*ncellsD), iw(ncellsD))
+        do i = 1, ncellsD
+            do k = 1, 4
+                c2c(4*(i-1)+k) = cell2cell(k, i) - 1         ! 0-based, -1 for none
+                a4(4*(i-1)+k) = acoef(k, i)
+            end do
+            iw(i) = iwet(i)
+        end do
+        tol = 1.0d-10
+        if (present(rtol)) tol = rtol
+        rc = c_solve(int(ncells, c_int), int(ncellsD, c_int), a4, c2c, iw, ss, sp, res, phi, &
+                     real(relax, c_double), tol, 1.0d-12, 4000_c_int, rmom, it)
+        iters = it
+    end function
+
+    function cms2d_flow_create(desc, flow) result(rc)
+        type(cms2d_flow_desc), intent(in) :: desc
+        type(c_ptr), intent(out) :: flow
+        integer(c_int) :: rc
+        rc = c_flow_create(desc, flow)
+    end function
+
+    function cms2d_flow_step(flow, nsteps, info) result(rc)
+        type(c_ptr), intent(in) :: flow
+        integer, intent(in) :: nsteps
+        type(cms2d_step_info), intent(out) :: info
+        integer(c_int) :: rc
+        rc = c_flow_step(flow, int(nsteps, c_int), info)
+    end function
+
+    function cms2d_flow_get(flow, name, out) result(rc)
+        type(c_ptr), intent(in) :: flow
+        character(len=*), intent(in) :: name
+        real(8), intent(out) :: out(*)
+        integer(c_int) :: rc
+        rc = c_flow_get(flow, trim(name)//c_null_char, out)
+    end function
+
+    function cms2d_flow_destroy(flow) result(rc)
+        type(c_ptr), intent(inout) :: flow
+        integer(c_int) :: rc
+        rc = c_flow_destroy(flow)
+        flow = c_null_ptr
+    end function
+
+end module cms2d_hip

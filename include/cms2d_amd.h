@@ -36,6 +36,11 @@ extern "C" {
 #define CMS2D_BND_WALL 0          /* closed wall (W_str, boundaries.F90:2413-2435)            */
 #define CMS2D_BND_WSE_HARMONIC 1  /* harmonic water level (TH_str, boundaries.F90:4098-4124) */
 
+/* Preconditioner of the pressure-correction (pp) BiCGStab.  The momentum
+ * systems are diagonally dominant and always use Jacobi. */
+#define CMS2D_PRECOND_MG 0        /* FP32 geometric multigrid V(1,1) cycle                    */
+#define CMS2D_PRECOND_JACOBI 1    /* diagonal                                                 */
+
 /* Solver precision forms. */
 #define CMS2D_PREC_FP64 0         /* FP64 everywhere                                          */
 #define CMS2D_PREC_FP32_REFINE 1  /* FP32 Krylov + SpMV inside FP64 iterative refinement     */
@@ -64,6 +69,7 @@ typedef struct {
     int kmax;                     /* Krylov iteration cap per solve                          */
     int precision;                /* CMS2D_PREC_*                                            */
     int device;                   /* HIP device ordinal                                      */
+    int precond;                  /* CMS2D_PRECOND_* for the pp system                       */
 } cms2d_flow_desc;
 
 typedef struct {

@@ -7,7 +7,7 @@ must be identical.
 import numpy as np
 import pytest
 
-from cms2d_amd import cases
+from cms2d_amd import _lib, cases
 from cms2d_amd.flow import FlowStep
 from pyoracle import Oracle
 
@@ -51,20 +51,35 @@ def test_lake_at_rest(gpu_available):
     assert np.abs(g.get("eta")).max() == 0.0
 
 
+PRECONDS = [pytest.param(_lib.PRECOND_MG, id="mg"), pytest.param(_lib.PRECOND_JACOBI, id="jacobi")]
+
+
 @pytest.mark.gpu
-def test_basin_small_matches_oracle(gpu_available):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_basin_small_matches_oracle(gpu_available, precond):
     c = cases.basin(nx=40, ny=20, dx=10.0, seed=0)
-    o, g = run_pair(c, 30)
+    o, g = run_pair(c, 30, precond=precond)
     err = compare(o, g)
     print("max err", err)
 
 
 @pytest.mark.gpu
-def test_tidal_wetdry_radstress_matches_oracle(gpu_available):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_tidal_wetdry_radstress_matches_oracle(gpu_available, precond):
     c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
-    o, g = run_pair(c, 40)
+    o, g = run_pair(c, 40, precond=precond)
     err = compare(o, g)
     print("max err", err)
+
+
+@pytest.mark.gpu
+def test_tidal_multigrid_levels_matches_oracle(gpu_available):
+    """Odd sizes through three multigrid levels (201x121 -> 101x61 -> 51x31)."""
+    c = cases.tidal(nx=201, ny=121, dx=20.0, seed=4)
+    o, g = run_pair(c, 8)
+    err = compare(o, g)
+    kry = g.step(0)
+    print("max err", err, kry)
 
 
 @pytest.mark.gpu
