@@ -24,9 +24,10 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of one launch of the pp-system SpMV (k_bicg_spmv, MODE 1)
-# per active cell, FP64: read aN, aE, aS, aW, ap, p, r0 and write v.
-SPMV_BYTES_PER_CELL = 8 * 8
+# Algorithmic bytes of one launch of the pp-system SpMV (k_bicg_spmv, MODE 1,
+# multigrid-preconditioned) per active cell: read aN, aE, aS, aW, ap, r0
+# (FP64) and z = M^-1 p (FP32), write v (FP64).
+SPMV_BYTES_PER_CELL = 7 * 8 + 4
 
 
 def parse():
@@ -37,6 +38,7 @@ def parse():
     ap.add_argument("--nx", type=int, default=2000)
     ap.add_argument("--ny", type=int, default=1500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-refinement form")
     ap.add_argument("--cpu-sample", default="500x375x2",
                     help="oracle sample: NXxNYxSTEPS of the same workload definition")
     return ap.parse_args()
@@ -60,6 +62,39 @@ def cpu_baseline(sample):
             "sample": f"FP64 oracle (ILU0-BiCGStab, rtol 1e-10, one thread) on the tidal workload "
                       f"generated at {nx}x{ny} cells (dx 20 m, dt 30 s), {steps} steps after one "
                       f"untimed cold-start step, {dt:.1f} s"}
+
+
+def fp32_form(a, case, g64, value64):
+    """The FP32 form (FP32 pressure-correction BiCGStab + SpMV inside FP64
+    refinement) on the same workload and step count, timed the same way, and
+    its state against the FP64 form's after the same steps."""
+    from cms2d_amd import _lib
+    from cms2d_amd.flow import FlowStep
+    g = FlowStep(case, precision=_lib.PREC_FP32_REFINE)
+    for _ in range(a.warmup):
+        g.step(1)
+    kry = 0.0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        kry += g.step(1)["kry"][0]
+    el = time.perf_counter() - t0
+    v32 = case.ncells * a.steps / el
+    wet = g64.get("iwet") > 0
+    wdiff = int((g.get("iwet") != g64.get("iwet")).sum())
+    err = {}
+    for name in ("eta", "u", "v"):
+        d = (g.get(name) - g64.get(name))[wet]
+        err[name + "_max"] = float(np.abs(d).max()) if d.size else 0.0
+        err[name + "_rms"] = float(np.sqrt(np.mean(d * d))) if d.size else 0.0
+    area = case.dx * case.dy
+    dvol = float(area * (g.get("h").sum() - g64.get("h").sum()))
+    nstep = a.warmup + a.steps
+    g.close()
+    return {"value": v32, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / a.steps,
+            "speedup_vs_fp64": v32 / value64, "pp_inner_krylov_iters_per_step": kry / a.steps,
+            "error_vs_fp64_form": err, "wet_cells_differ": wdiff,
+            "volume_difference_m3_after_steps": dvol, "steps_compared": nstep,
+            "volume_drift_m_per_step": abs(dvol) / (area * case.ncells) / nstep}
 
 
 def main():
@@ -115,7 +150,7 @@ def main():
                                       "v": kry[2] / a.steps},
         },
         "roofline": {
-            "kernel": "k_bicg_spmv (pp system, MODE 1)",
+            "kernel": "k_bicg_spmv (pp system, MODE 1: v = A z, z from the multigrid cycle)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -128,6 +163,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not a.no_fp32:
+        out["fp32_refine"] = fp32_form(a, case, g, value)
     g.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_sample)

@@ -132,6 +132,29 @@ def square(nx=1000, ny=1000, dx=20.0, seed=3, nsteps=720, dt=60.0) -> Case:
     return c
 
 
+def ensemble_members(n=256, seed=11, side_min=64, side_max=512, dx=20.0, dt=60.0, nsteps=720):
+    """Independent basins of the batched ensemble: nx and ny uniform in
+    [side_min, side_max], offshore depth slope, tide amplitude uniform in
+    0.2-1.0 m and lognormal Manning with median uniform in 0.015-0.035, each
+    member from its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        nx, ny = (int(v) for v in rng.integers(side_min, side_max + 1, size=2))
+        amp = float(rng.uniform(0.2, 1.0))
+        med = float(rng.uniform(0.015, 0.035))
+        deep = float(rng.uniform(5.0, 15.0))
+        r = np.random.default_rng([seed, m])
+        x = (np.arange(nx) + 0.5) / nx
+        depth = 1.0 + (deep - 1.0) * x[None, :] * np.ones((ny, 1))
+        depth = depth + _bumps(r, nx, ny, 8, 0.2, max(2, nx / 40), max(3, nx / 12))
+        zb = f32(-depth).ravel()
+        mann = f32(med * np.exp(0.2 * r.standard_normal(nx * ny)))
+        out.append(Case(f"member{m}", nx, ny, dx, dx, zb, mann, None, None, 1, [amp], [M2_SPEED_DEG_HR],
+                        [90.0], dt, nsteps))
+    return out
+
+
 def lake_at_rest(nx=40, ny=30, dx=10.0, seed=5) -> Case:
     """Closed basin with a still, flat water surface over a bumpy bed."""
     rng = np.random.default_rng(seed)

@@ -12,10 +12,12 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace cms;
@@ -44,8 +46,10 @@ struct cms2d_flow {
     KrySys<2> Kuv{};
     KrySys<1> Kpp{};
     int precond = CMS2D_PRECOND_MG;
+    int precision = CMS2D_PREC_FP64;
     Mg mg{};                   // pp preconditioner hierarchy (level 0 z is Kpp.zp / Kpp.zs)
-    KryScal *d_sc = nullptr;   // [3]: pp, u, v
+    KrySys<1, float> Kin{};    // inner FP32 solve of the refinement form
+    KryScal *d_sc = nullptr;   // [4]: pp, u, v, inner FP32 pp
     KryScal *h_sc = nullptr;   // pinned mirror
     int *h_flags = nullptr;    // pinned
     std::vector<void *> allocs;
@@ -55,7 +59,7 @@ struct cms2d_flow {
     double timesecs = 0.0, rampdur = 0.0;
     int ntime = 0, maxit = 0, niter = 0, iconv = 1;
     double rmom[3]{}, rmom0[3]{}, rmom1[3]{};
-    int kry[3]{}, prev_iters[3]{4, 2, 2};
+    int kry[3]{}, prev_iters[4]{4, 2, 2, 4};
     // timing of the pp-system SpMV launches (events bracket each launch)
     bool stats = true;
     std::vector<hipEvent_t> ev;
@@ -104,16 +108,15 @@ static int wetdry(cms2d_flow *F) {
         if (F->h_flags[0] == 0) break;
         if (it > 100000) return set_err(CMS2D_ERR_HIP, "narrow-channel pass did not settle");
     }
-    LAUNCH(k_pond_seed, F->nblk_pad, g, s);
-    dim3 pg((g.nxp + PT_X - 1) / PT_X, (g.nyp + PT_Y - 1) / PT_Y);
-    for (int it = 0;; it++) {
-        LAUNCH(k_set_int, 1, s.flags, 3, 0);
-        hipLaunchKernelGGL(k_pond_tile, pg, dim3(256), 0, F->st, g, s);
-        if (int rc = sync_flags(F)) return rc;
-        if (F->h_flags[1] == 0) break;
-        if (it > 1000000) return set_err(CMS2D_ERR_HIP, "ponding pass did not settle");
+    {
+        const int ntx = (g.nxp + PT - 1) / PT, nty = (g.nyp + PT - 1) / PT;
+        const int nedge = (ntx - 1) * g.nyp + (nty - 1) * g.nxp;
+        hipLaunchKernelGGL(k_pond_local, dim3(ntx, nty), dim3(256), 0, F->st, g, s);
+        if (nedge > 0) LAUNCH(k_pond_merge, (nedge + BLOCK - 1) / BLOCK, g, s);
+        LAUNCH(k_pond_compress, F->nblk_pad, g, s);
+        LAUNCH(k_pond_mark, F->nblk_pad, g, s);
+        LAUNCH(k_pond_final, F->nblk_pad, g, s);
     }
-    LAUNCH(k_pond_final, F->nblk_pad, g, s);
     LAUNCH(k_set_int, 1, s.flags, 3, 0);
     LAUNCH(k_wetdry_post, F->nblk_act, g, s);
     if (int rc = sync_flags(F)) return rc;
@@ -147,11 +150,11 @@ struct NoPrec {
 // batch is enqueued without host synchronisation; kernels launched after a
 // system has stopped return at once (they read the device-side done flag).
 // PREC 1: pre(0) enqueues zp = M^-1 p, pre(1) enqueues zs = M^-1 s.
-template <class Op, int NS, int PREC = 0, class Pre = NoPrec>
-static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0, Pre pre = Pre()) {
+template <class Op, int NS, int PREC = 0, class T = double, class Pre = NoPrec>
+static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre()) {
     const dim3 grid(X.nblk), blk(BLOCK);
     // start: thresholds, rmom and the initial stop test
-    hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
+    hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
     HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
     HIPCHK(hipStreamSynchronize(X.st));
     auto all_done = [&]() {
@@ -159,30 +162,33 @@ static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0, Pre pre = Pre()) {
         for (int q = 0; q < NS; q++) d &= X.h_sc[sc0 + q].done;
         return d;
     };
-    int it = 0;
+    // batches: one short of the last solve's count, then 2, 2, 4, 8, ... so
+    // that few launches follow convergence (each costs a few microseconds
+    // even when it returns at once) and few host round trips are needed
+    int it = 0, nbatch = 0, grow = 2;
     int batch = std::max(1, X.prev_iters[sc0] - 1);
     while (!all_done() && it < X.kmax) {
         int nb = std::min(batch, X.kmax - it);
         int nev = 0;
         const int it0 = it;
         for (int b = 0; b < nb; b++, it++) {
-            hipLaunchKernelGGL((k_bicg_p<Op, NS>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
+            hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
             if (PREC) pre(0);
             bool rec = X.ev && nev < (int)X.ev->size() / 2;
             if (rec) {
                 // events stamped by the dispatch itself (start and end of this kernel)
-                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
+                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC, T>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
                                       (*X.ev)[2 * nev + 1], 0, A, K);
                 nev++;
             } else {
-                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC>), grid, blk, 0, X.st, A, K);
+                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC, T>), grid, blk, 0, X.st, A, K);
             }
-            hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC>), grid, blk, 0, X.st, A, K, it, X.nblk);
+            hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, it, X.nblk);
             if (PREC) pre(1);
-            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC>), grid, blk, 0, X.st, A, K);
-            hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC>), grid, blk, 0, X.st, A, K, it, X.nblk);
+            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC, T>), grid, blk, 0, X.st, A, K);
+            hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, it, X.nblk);
         }
-        hipLaunchKernelGGL((k_bicg_p<Op, NS>), dim3(1), blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 1);
+        hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 1);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
         HIPCHK(hipStreamSynchronize(X.st));
@@ -198,7 +204,8 @@ static int run_bicg(KryCtx &X, KrySys<NS> &K, Op A, int sc0, Pre pre = Pre()) {
                 *X.ev_n += 1;
             }
         }
-        batch = std::min(2 * std::max(batch, 2), 64);
+        batch = grow;
+        if (++nbatch >= 2) grow = std::min(2 * grow, 64);
     }
     for (int q = 0; q < NS; q++) X.prev_iters[sc0 + q] = std::max(1, X.h_sc[sc0 + q].final_iters);
     return 0;
@@ -215,30 +222,67 @@ static void mg_setup(cms2d_flow *F) {
     for (int l = 0; l + 1 < M.nlev; l++) LAUNCH(k_mg_coarsen, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1]);
 }
 
-// z0 = V(b0): one V(1,1) cycle (mg.h) on the stream of F.
-static void mg_vcycle(cms2d_flow *F, const double *b0, float *z0) {
+// z0 = V(b0): one V(1,1) cycle (mg.h) on the stream of F; every launch
+// returns at once when the solve owning sc has stopped.
+template <class TB>
+static void mg_vcycle(cms2d_flow *F, const TB *b0, float *z0, const KryScal *sc) {
     const Mg &M = F->mg;
-    const KryScal *sc = F->d_sc;      // pp system: stop when its solve has
     MgLevel L0 = M.L[0];
-    L0.z = z0;
+    L0.zo = z0;
+    const MgLevel &Lc = M.nlev == 1 ? L0 : M.L[M.nlev - 1];
+    const size_t lds = sizeof(float) * (size_t)Lc.nxp * (Lc.ny + 2);
     if (M.nlev == 1) {
-        hipLaunchKernelGGL(k_mg_coarsest<double>, dim3(1), dim3(1024), 0, F->st, L0, b0, sc);
+        hipLaunchKernelGGL(k_mg_coarsest<TB>, dim3(1), dim3(MG_CT), lds, F->st, L0, b0, sc);
         return;
     }
-    LAUNCH(k_mg_pre<double>, mg_nblk(L0), L0, b0, sc);
-    LAUNCH(k_mg_restrict, mg_nblk(M.L[1]), L0, M.L[1], sc);
-    for (int l = 1; l + 1 < M.nlev; l++) {
-        LAUNCH(k_mg_pre<float>, mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, sc);
-        LAUNCH(k_mg_restrict, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1], sc);
+    LAUNCH(k_mg_down<TB>, mg_nblk(M.L[1]), L0, b0, M.L[1], sc);
+    for (int l = 1; l + 1 < M.nlev; l++)
+        LAUNCH(k_mg_down<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+    hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(MG_CT), lds, F->st, Lc, (const float *)Lc.b, sc);
+    for (int l = M.nlev - 2; l >= 1; l--)
+        LAUNCH(k_mg_up<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+    LAUNCH(k_mg_up<TB>, mg_nblk(M.L[1]), L0, b0, M.L[1], sc);
+}
+
+// Inner FP32 solve: relative stop and iteration cap per refinement sweep.
+static constexpr double REF_RTOL = 1e-5;
+static constexpr int REF_KMAX = 200;
+static constexpr int REF_SWEEPS = 20;
+
+// The pp solve of the FP32 form: FP32 BiCGStab (FP32 operator and vectors,
+// multigrid) for the correction, FP64 x and residual, FP64 stop test on the
+// FP64 solve's own threshold.  Krylov iterations are summed over the sweeps.
+static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
+    KrySys<1> &K = F->Kpp;
+    KrySys<1, float> &Ki = F->Kin;
+    const int nb = F->nblk_kry;
+    hipLaunchKernelGGL((k_bicg_p<StencilOp, 1, double>), dim3(1), dim3(BLOCK), 0, F->st, A, K, 0, nb, X.kmax,
+                       X.rtol, X.atol, 1);
+    HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const MgLevel &L0 = F->mg.L[0];
+    StencilOp32 A32{F->g, L0.aN, L0.aE, L0.aS, L0.aW};
+    KryCtx Xi = X;
+    Xi.kmax = REF_KMAX;
+    Xi.rtol = REF_RTOL;
+    Xi.atol = 0.0;
+    Xi.ev = nullptr;
+    auto pre = [F, &Ki](int which) {
+        if (which == 0) mg_vcycle<float>(F, Ki.p[0], Ki.zp, Ki.sc);
+        else mg_vcycle<float>(F, Ki.r[0], Ki.zs, Ki.sc);
+    };
+    int total = 0;
+    for (int sw = 0; sw < REF_SWEEPS && !F->h_sc[0].done; sw++) {
+        hipLaunchKernelGGL(k_ref_start, dim3(nb), dim3(BLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
+        if (int rc = run_bicg<StencilOp32, 1, 1, float>(Xi, Ki, A32, 3, pre)) return rc;
+        total += F->h_sc[3].final_iters;
+        hipLaunchKernelGGL(k_ref_update, dim3(nb), dim3(BLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
+        hipLaunchKernelGGL(k_ref_check, dim3(1), dim3(BLOCK), 0, F->st, K, nb);
+        HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
     }
-    const MgLevel &Lc = M.L[M.nlev - 1];
-    hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(1024), 0, F->st, Lc, (const float *)Lc.b, sc);
-    for (int l = M.nlev - 2; l >= 1; l--) {
-        LAUNCH((k_mg_post<float, 1>), mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
-        LAUNCH((k_mg_post<float, 0>), mg_nblk(M.L[l]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
-    }
-    LAUNCH((k_mg_post<double, 1>), mg_nblk(L0), L0, b0, M.L[1], sc);
-    LAUNCH((k_mg_post<double, 0>), mg_nblk(L0), L0, b0, M.L[1], sc);
+    F->h_sc[0].final_iters = total;
+    return 0;
 }
 
 static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
@@ -323,13 +367,16 @@ static int flow_imp(cms2d_flow *F) {
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
             KryCtx X = flow_ctx(F, true);
-            if (F->precond == CMS2D_PRECOND_MG) {
+            if (F->precision == CMS2D_PREC_FP32_REFINE) {
+                mg_setup(F);
+                if (int rc = pp_solve_refine(F, X, A)) return rc;
+            } else if (F->precond == CMS2D_PRECOND_MG) {
                 mg_setup(F);
                 auto pre = [F](int which) {
-                    if (which == 0) mg_vcycle(F, F->Kpp.p[0], F->Kpp.zp);
-                    else mg_vcycle(F, F->Kpp.r[0], F->Kpp.zs);
+                    if (which == 0) mg_vcycle<double>(F, F->Kpp.p[0], F->Kpp.zp, F->d_sc);
+                    else mg_vcycle<double>(F, F->Kpp.r[0], F->Kpp.zs, F->d_sc);
                 };
-                if (int rc = run_bicg<StencilOp, 1, 1>(X, F->Kpp, A, 0, pre)) return rc;
+                if (int rc = run_bicg<StencilOp, 1, 1, double>(X, F->Kpp, A, 0, pre)) return rc;
             } else {
                 if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
             }
@@ -413,9 +460,12 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     for (int k = 0; k < 4; k++)
         if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC)
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
-    if (d->precision != CMS2D_PREC_FP64) return set_err(CMS2D_ERR_ARG, "precision form not available");
+    if (d->precision != CMS2D_PREC_FP64 && d->precision != CMS2D_PREC_FP32_REFINE)
+        return set_err(CMS2D_ERR_ARG, "unknown precision form");
     if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
         return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
+    if (d->precision == CMS2D_PREC_FP32_REFINE && d->precond != CMS2D_PRECOND_MG)
+        return set_err(CMS2D_ERR_ARG, "the FP32 refinement form uses the multigrid preconditioner");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device)
         return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
@@ -467,18 +517,21 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     for (uint8_t **f : bfields)
         if (int rc = dalloc(F, f, N)) return rc;
     if (int rc = dalloc(F, &s.flags, 4)) return rc;
+    if (int rc = dalloc(F, &s.plab, N)) return rc;
     {
         // Krylov partial sums: 5 slots per system (pr: 2, pv: 1, pt: 2)
         double *red;
-        if (int rc = dalloc(F, &red, 3 * 5 * (size_t)F->nblk_kry)) return rc;
+        if (int rc = dalloc(F, &red, 4 * 5 * (size_t)F->nblk_kry)) return rc;
         const size_t nb = F->nblk_kry;
         F->Kpp.red.pr[0] = red; F->Kpp.red.pv[0] = red + 2 * nb; F->Kpp.red.pt[0] = red + 3 * nb;
         for (int q = 0; q < 2; q++) {
             double *b = red + (q + 1) * 5 * nb;
             F->Kuv.red.pr[q] = b; F->Kuv.red.pv[q] = b + 2 * nb; F->Kuv.red.pt[q] = b + 3 * nb;
         }
+        double *bi = red + 3 * 5 * nb;
+        F->Kin.red.pr[0] = bi; F->Kin.red.pv[0] = bi + 2 * nb; F->Kin.red.pt[0] = bi + 3 * nb;
     }
-    if (int rc = dalloc(F, &F->d_sc, 3)) return rc;
+    if (int rc = dalloc(F, &F->d_sc, 4)) return rc;
     // Krylov vectors
     {
         double *kv[3][6];
@@ -498,6 +551,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     }
     // multigrid hierarchy: level 0 shares the padded layout of the flow arrays
     F->precond = d->precond;
+    F->precision = d->precision;
     {
         Mg &M = F->mg;
         int nx = g.nx, ny = g.ny;
@@ -509,21 +563,35 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             float **arrs[] = {&L.aN, &L.aE, &L.aS, &L.aW, &L.dinv, &L.e};
             for (float **a : arrs)
                 if (int rc = dalloc(F, a, n)) return rc;
+            if (int rc = dalloc(F, &L.z, n)) return rc;
             if (M.nlev > 0) {
-                if (int rc = dalloc(F, &L.z, n)) return rc;
+                if (int rc = dalloc(F, &L.zo, n)) return rc;
                 if (int rc = dalloc(F, &L.b, n)) return rc;
+            } else {
+                if (int rc = dalloc(F, &L.ap, n)) return rc;
             }
             M.nlev++;
-            if ((long)nx * ny <= MG_COARSE_MAX || M.nlev == MG_MAX_LEVELS || nx < 2 || ny < 2) break;
+            if ((long)nx * ny <= MG_COARSE_MAX) break;
+            if (M.nlev == MG_MAX_LEVELS) return set_err(CMS2D_ERR_ARG, "grid too large for the multigrid hierarchy");
             nx = (nx + 1) / 2;
             ny = (ny + 1) / 2;
         }
         if (int rc = dalloc(F, &F->Kpp.zp, N)) return rc;
         if (int rc = dalloc(F, &F->Kpp.zs, N)) return rc;
+        KrySys<1, float> &Ki = F->Kin;
+        if (F->precision == CMS2D_PREC_FP32_REFINE) {
+            float **vec[] = {&Ki.x[0], &Ki.r[0], &Ki.r0[0], &Ki.p[0], &Ki.v[0], &Ki.t[0]};
+            for (float **v : vec)
+                if (int rc = dalloc(F, v, N)) return rc;
+        }
+        Ki.ap_w[0] = M.L[0].ap;
+        Ki.zp = F->Kpp.zp;
+        Ki.zs = F->Kpp.zs;
+        Ki.sc = F->d_sc + 3;
     }
-    HIPCHK(hipHostMalloc((void **)&F->h_sc, 3 * sizeof(KryScal), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
-    memset(F->h_sc, 0, 3 * sizeof(KryScal));
+    memset(F->h_sc, 0, 4 * sizeof(KryScal));
     F->ev.resize(2 * 64);
     for (auto &e : F->ev) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
@@ -591,6 +659,36 @@ extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info)
             for (int i = 0; i < F->g.nx; i++) nw += w[(size_t)(j + 1) * F->g.nxp + i + 1];
         info->nwet = nw;
     }
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_ensemble_step(cms2d_flow **flows, int nflows, int nsteps, int nthreads,
+                                   cms2d_step_info *infos) {
+    if (!flows || nflows < 0 || nsteps < 0) return set_err(CMS2D_ERR_ARG, "invalid argument");
+    for (int m = 0; m < nflows; m++)
+        if (!flows[m]) return set_err(CMS2D_ERR_ARG, "null member");
+    std::vector<int> order(nflows);
+    for (int m = 0; m < nflows; m++) order[m] = m;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return flows[a]->g.nact > flows[b]->g.nact; });
+    std::vector<int> rc(nflows, CMS2D_OK);
+    std::vector<std::string> msg(nflows);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int k; (k = next.fetch_add(1)) < nflows;) {
+            const int m = order[k];
+            rc[m] = cms2d_flow_step(flows[m], nsteps, infos ? &infos[m] : nullptr);
+            if (rc[m] != CMS2D_OK) msg[m] = g_err;
+        }
+    };
+    int nt = nthreads > 0 ? nthreads : 8;
+    nt = std::max(1, std::min(nt, nflows));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    for (int m = 0; m < nflows; m++)
+        if (rc[m] != CMS2D_OK) return set_err(rc[m], "member " + std::to_string(m) + ": " + msg[m]);
     return CMS2D_OK;
 }
 

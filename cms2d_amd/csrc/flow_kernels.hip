@@ -183,82 +183,146 @@ __global__ void k_narrow(Geo g, St s, Par P) {
     }
 }
 
-// flow_pond (flow.F90:1807-1876): wet cells not connected through wet active
-// cells to a forcing-boundary cell are dried.  Seeds: boundary-string cells
-// and their ghosts.
-__global__ void k_pond_seed(Geo g, St s) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+// flow_pond (flow.F90:1807-1876): wet cells not reached from a forcing
+// boundary are dried.  The reference seeds the boundary-string cells and their
+// ghosts and spreads from reached ACTIVE cells to wet neighbours until nothing
+// changes, so the reached set is every wet cell in a connected component that
+// holds a seed, where two neighbours are connected when both are wet and at
+// least one is active (ghosts receive but do not pass on).  The components
+// are labelled in one pass instead of by repeated sweeps:
+//   k_pond_local    union-find inside 32x32 tiles in LDS (root = smallest
+//                   padded index of the tile-local component)
+//   k_pond_merge    union across tile edges on the global label array
+//   k_pond_compress label = root (smallest padded index of the component)
+//   k_pond_mark     reached[root] = 1 for every wet seed
+//   k_pond_final    wet cells whose root was not reached are dried
+// Labels only ever decrease towards the component minimum, so the result does
+// not depend on the order in which the atomic unions land.
+constexpr int PT = 32;
+
+__device__ __forceinline__ bool pond_link(const Geo &g, const uint8_t *iwet, int c, int d) {
+    if (!iwet[c] || !iwet[d]) return false;
+    int i, j, k, l;
+    pad_ij(g, c, i, j);
+    pad_ij(g, d, k, l);
+    return active_ij(g, i, j) || active_ij(g, k, l);
+}
+
+__device__ __forceinline__ int lds_find(int *lab, int x) {
+    for (;;) {
+        int y = __hip_atomic_load(&lab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (y == x) return x;
+        x = y;
+    }
+}
+__device__ __forceinline__ void lds_union(int *lab, int a, int b) {
+    for (;;) {
+        a = lds_find(lab, a);
+        b = lds_find(lab, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }             // link the larger root a under b
+        int old = atomicMin(&lab[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+__device__ __forceinline__ int glb_find(int *P, int x) {
+    for (;;) {
+        int y = __hip_atomic_load(&P[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (y == x) return x;
+        x = y;
+    }
+}
+__device__ __forceinline__ void glb_union(int *P, int a, int b) {
+    for (;;) {
+        a = glb_find(P, a);
+        b = glb_find(P, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }
+        int old = atomicMin(&P[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pond_local(Geo g, St s) {
+    __shared__ int lab[PT * PT];
+    const int X0 = blockIdx.x * PT, Y0 = blockIdx.y * PT;
+    for (int q = threadIdx.x; q < PT * PT; q += blockDim.x) {
+        const int X = X0 + q % PT, Y = Y0 + q / PT;
+        lab[q] = (X < g.nxp && Y < g.nyp && s.iwet[Y * g.nxp + X]) ? q : -1;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < PT * PT; q += blockDim.x) {
+        const int lx = q % PT, ly = q / PT, X = X0 + lx, Y = Y0 + ly;
+        if (X >= g.nxp || Y >= g.nyp) continue;
+        const int c = Y * g.nxp + X;
+        if (lx > 0 && pond_link(g, s.iwet, c, c - 1)) lds_union(lab, q, q - 1);
+        if (ly > 0 && pond_link(g, s.iwet, c, c - g.nxp)) lds_union(lab, q, q - PT);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < PT * PT; q += blockDim.x) {
+        const int X = X0 + q % PT, Y = Y0 + q / PT;
+        if (X >= g.nxp || Y >= g.nyp) continue;
+        const int c = Y * g.nxp + X;
+        int r = -1;
+        if (lab[q] >= 0) {
+            const int rq = lds_find(lab, q);
+            r = (Y0 + rq / PT) * g.nxp + (X0 + rq % PT);
+        }
+        s.plab[c] = r;
+        s.ipond[c] = 0;
+    }
+}
+
+// Edges that cross a tile boundary: threads [0, nv) take the west edges of
+// tile columns 1.., threads [nv, nv+nh) the south edges of tile rows 1..
+__global__ void k_pond_merge(Geo g, St s) {
+    const int ntx = (g.nxp + PT - 1) / PT, nty = (g.nyp + PT - 1) / PT;
+    const int nv = (ntx - 1) * g.nyp, nh = (nty - 1) * g.nxp;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int c, d;
+    if (t < nv) {
+        const int Y = t / (ntx - 1), X = (t % (ntx - 1) + 1) * PT;
+        c = Y * g.nxp + X;
+        d = c - 1;
+    } else if (t < nv + nh) {
+        const int u = t - nv, X = u % g.nxp, Y = (u / g.nxp + 1) * PT;
+        c = Y * g.nxp + X;
+        d = c - g.nxp;
+    } else {
+        return;
+    }
+    if (pond_link(g, s.iwet, c, d)) glb_union(s.plab, c, d);
+}
+
+__global__ void k_pond_compress(Geo g, St s) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    if (s.plab[t] >= 0) s.plab[t] = glb_find(s.plab, t);
+}
+
+// Seeds: boundary-string cells and the ghosts across their boundary faces.
+__global__ void k_pond_mark(Geo g, St s) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.nxp * g.nyp) return;
     int i, j;
     pad_ij(g, t, i, j);
-    uint8_t v = 0;
+    bool seed;
     if (active_ij(g, i, j)) {
-        if ((j == g.ny - 1 && g.side[KN]) || (i == g.nx - 1 && g.side[KE]) || (j == 0 && g.side[KS]) ||
-            (i == 0 && g.side[KW]))
-            v = s.iwet[t];
+        seed = (j == g.ny - 1 && g.side[KN]) || (i == g.nx - 1 && g.side[KE]) || (j == 0 && g.side[KS]) ||
+               (i == 0 && g.side[KW]);
     } else {
-        int sd = ring_side(g, i, j);
-        if (sd >= 0 && g.side[sd]) v = s.iwet[t];
+        const int sd = ring_side(g, i, j);
+        seed = sd >= 0 && g.side[sd];
     }
-    s.ipond[t] = v;
-}
-
-// Reachability propagated inside a 64x16 tile held in LDS until the tile is
-// stable; repeated over the grid until no tile changes.
-constexpr int PT_X = 64, PT_Y = 16;
-__global__ void __launch_bounds__(256) k_pond_tile(Geo g, St s) {
-    __shared__ uint8_t sw[PT_Y + 2][PT_X + 2], sp[PT_Y + 2][PT_X + 2];
-    const int tx0 = blockIdx.x * PT_X - 1, ty0 = blockIdx.y * PT_Y - 1;   // padded coords of halo origin
-    for (int q = threadIdx.x; q < (PT_X + 2) * (PT_Y + 2); q += blockDim.x) {
-        int ly = q / (PT_X + 2), lx = q - ly * (PT_X + 2);
-        int X = tx0 + lx, Y = ty0 + ly;
-        uint8_t w = 0, pv = 0;
-        if (X >= 0 && X < g.nxp && Y >= 0 && Y < g.nyp) {
-            w = s.iwet[Y * g.nxp + X];
-            pv = s.ipond[Y * g.nxp + X];
-        }
-        sw[ly][lx] = w;
-        sp[ly][lx] = pv;
-    }
-    __syncthreads();
-    int changed_any = 0;
-    for (;;) {
-        int changed = 0;
-        for (int q = threadIdx.x; q < PT_X * PT_Y; q += blockDim.x) {
-            int ly = q / PT_X + 1, lx = q % PT_X + 1;
-            int X = tx0 + lx, Y = ty0 + ly;
-            if (X >= g.nxp || Y >= g.nyp) continue;
-            if (!sw[ly][lx] || sp[ly][lx]) continue;
-            // a reached ACTIVE neighbour propagates to this cell
-            bool hit = false;
-            const int dxs[4] = {0, 1, 0, -1}, dys[4] = {1, 0, -1, 0};
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                int nxq = X + dxs[k], nyq = Y + dys[k];
-                int ni = nxq - 1, nj = nyq - 1;
-                if (active_ij(g, ni, nj) && sp[ly + dys[k]][lx + dxs[k]]) hit = true;
-            }
-            if (hit) { sp[ly][lx] = 1; changed = 1; }
-        }
-        changed = __syncthreads_or(changed);
-        if (!changed) break;
-        changed_any = 1;
-    }
-    if (changed_any) {
-        for (int q = threadIdx.x; q < PT_X * PT_Y; q += blockDim.x) {
-            int ly = q / PT_X + 1, lx = q % PT_X + 1;
-            int X = tx0 + lx, Y = ty0 + ly;
-            if (X >= g.nxp || Y >= g.nyp) continue;
-            s.ipond[Y * g.nxp + X] = sp[ly][lx];
-        }
-        if (threadIdx.x == 0) s.flags[1] = 1;
-    }
+    if (seed && s.iwet[t]) s.ipond[s.plab[t]] = 1;
 }
 
 __global__ void k_pond_final(Geo g, St s) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.nxp * g.nyp) return;
-    if (s.ipond[t] == 0 && s.iwet[t] == 1) s.iwet[t] = 0;
+    if (s.iwet[t] == 1 && !s.ipond[s.plab[t]]) s.iwet[t] = 0;
 }
 
 // Corner flags, newly wet count, and zero flux/visk on faces not wet on both

@@ -63,10 +63,11 @@ __device__ __forceinline__ Chunk block_chunk(int n) {
 }
 
 // Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k), z = M^-1 x = x/ap,
-// z = 0 on ghost cells.
-struct StencilOp {
+// z = 0 on ghost cells.  C is the type of the coefficients and of the result.
+template <class C>
+struct StencilOpT {
     Geo g;
-    const double *aN, *aE, *aS, *aW;
+    const C *aN, *aE, *aS, *aW;
     __device__ __forceinline__ int n() const { return g.nact; }
     __device__ __forceinline__ Idx at(int t) const {
         Idx x;
@@ -87,17 +88,19 @@ struct StencilOp {
         return ap[c] * zc - sum;
     }
     // y_i = ap_i z_i - sum_k a_k(i) z_nb(k) for an explicit preconditioned
-    // vector z (FP32, zero on the ghost ring).
-    __device__ __forceinline__ double apply_z(const float *z, const double *ap, const Idx &x) const {
+    // vector z (FP32, zero on the ghost ring), evaluated in C.
+    __device__ __forceinline__ C apply_z(const float *z, const C *ap, const Idx &x) const {
         const int c = x.c;
-        const double zN = x.j == g.ny - 1 ? 0.0 : (double)z[c + g.nxp];
-        const double zE = x.i == g.nx - 1 ? 0.0 : (double)z[c + 1];
-        const double zS = x.j == 0 ? 0.0 : (double)z[c - g.nxp];
-        const double zW = x.i == 0 ? 0.0 : (double)z[c - 1];
-        const double sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
-        return ap[c] * (double)z[c] - sum;
+        const C zN = x.j == g.ny - 1 ? C(0) : (C)z[c + g.nxp];
+        const C zE = x.i == g.nx - 1 ? C(0) : (C)z[c + 1];
+        const C zS = x.j == 0 ? C(0) : (C)z[c - g.nxp];
+        const C zW = x.i == 0 ? C(0) : (C)z[c - 1];
+        const C sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
+        return ap[c] * (C)z[c] - sum;
     }
 };
+using StencilOp = StencilOpT<double>;
+using StencilOp32 = StencilOpT<float>;
 
 // The reference's general cell list: acoef(4,ncellsD) and cell2cell(4,ncellsD)
 // (0-based, ghosts numbered from ncells, -1 for no face).
@@ -124,10 +127,12 @@ struct EllOp {
     }
 };
 
-template <int NS>
+// Vectors of NS systems in precision T (double, or float for the inner solve
+// of the FP32 form); scalars and reductions are FP64 in both.
+template <int NS, class T = double>
 struct KrySys {
-    double *x[NS], *r[NS], *r0[NS], *p[NS], *v[NS], *t[NS];
-    double *ap_w[NS];  // diagonal ap of each system (written by the prologue)
+    T *x[NS], *r[NS], *r0[NS], *p[NS], *v[NS], *t[NS];
+    T *ap_w[NS];       // diagonal ap of each system (written by the prologue)
     float *zp, *zs;    // M^-1 p and M^-1 s of system 0 (explicit preconditioner only)
     KryScal *sc;       // [NS]
     KryRed<NS> red;

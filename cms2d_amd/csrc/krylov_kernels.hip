@@ -18,8 +18,8 @@ namespace cms {
 #define SMALL F4(1.0e-15)
 #endif
 
-template <int NS>
-__device__ __forceinline__ bool kdone(const KrySys<NS> &K) {
+template <int NS, class T>
+__device__ __forceinline__ bool kdone(const KrySys<NS, T> &K) {
 #pragma unroll
     for (int q = 0; q < NS; q++)
         if (!K.sc[q].done) return false;
@@ -69,8 +69,8 @@ __global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K)
 // fin_only, updates p.  Launched with fin_only on one block at the end of a
 // batch so the host can read the state; the next batch repeats the same
 // finish (same inputs, same result) before its first p update.
-template <class Op, int NS>
-__global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS> K, int it, int nb, int kmax, double rtol,
+template <class Op, int NS, class T = double>
+__global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it, int nb, int kmax, double rtol,
                                                   double atol, int fin_only) {
     double tot[2 * NS];
     const double *src[2 * NS];
@@ -129,17 +129,17 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS> K, int it, in
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
-            const double r = K.r[q][c];
+            const T r = K.r[q][c];
             if (rst[q]) K.r0[q][c] = r;
-            K.p[q][c] = r + beta[q] * (K.p[q][c] - om[q] * K.v[q][c]);
+            K.p[q][c] = r + (T)beta[q] * (K.p[q][c] - (T)om[q] * K.v[q][c]);
         }
     }
 }
 
 // MODE 1: v = A M^-1 p, partials r0.v.  MODE 2: t = A M^-1 s (s held in r),
 // partials t.s and t.t.  PREC 1: M^-1 p and M^-1 s are the explicit zp, zs.
-template <class Op, int NS, int MODE, int PREC = 0>
-__global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
+template <class Op, int NS, int MODE, int PREC = 0, class T = double>
+__global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
     static_assert(PREC == 0 || NS == 1, "explicit preconditioner: one system");
     if (kdone(K)) return;
     constexpr int NV = MODE == 1 ? NS : 2 * NS;
@@ -156,17 +156,18 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
-            const double *xv = MODE == 1 ? K.p[q] : K.r[q];
-            double y;
+            const T *xv = MODE == 1 ? K.p[q] : K.r[q];
+            T y;
             if constexpr (PREC == 1) y = A.apply_z(MODE == 1 ? K.zp : K.zs, K.ap_w[q], x);
             else y = A.apply(xv, K.ap_w[q], x);
+            const double yd = y;
             if (MODE == 1) {
                 K.v[q][c] = y;
-                red[q] += K.r0[q][c] * y;
+                red[q] += (double)K.r0[q][c] * yd;
             } else {
                 K.t[q][c] = y;
-                red[2 * q] += y * xv[c];
-                red[2 * q + 1] += y * y;
+                red[2 * q] += yd * (double)xv[c];
+                red[2 * q + 1] += yd * yd;
             }
         }
     }
@@ -181,8 +182,8 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS> K) {
 
 // alpha = rho / r0.v ; x += alpha M^-1 p ; s = r - alpha v (s overwrites r).
 // PREC 1 defers the x update to k_bicg_r.
-template <class Op, int NS, int PREC = 0>
-__global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K, int it, int nb) {
+template <class Op, int NS, int PREC = 0, class T = double>
+__global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[NS];
     const double *src[NS];
@@ -216,15 +217,15 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS> K, int it, in
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
             if constexpr (PREC == 0) K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] / K.ap_w[q][c]);
-            K.r[q][c] = K.r[q][c] - al[q] * K.v[q][c];
+            K.r[q][c] = K.r[q][c] - (T)al[q] * K.v[q][c];
         }
     }
 }
 
 // omega = t.s / t.t ; x += omega M^-1 s ; r = s - omega t ; partials r.r, r0.r
 // PREC 1: x += alpha zp + omega zs.
-template <class Op, int NS, int PREC = 0>
-__global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, int nb) {
+template <class Op, int NS, int PREC = 0, class T = double>
+__global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[2 * NS];
     const double *src[2 * NS];
@@ -251,15 +252,15 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS> K, int it, in
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
-            const double sv = K.r[q][c];
+            const T sv = K.r[q][c];
             if constexpr (PREC == 1)
-                K.x[q][c] = K.x[q][c] + (al[q] * (double)K.zp[c] + om[q] * (double)K.zs[c]);
+                K.x[q][c] = K.x[q][c] + ((T)al[q] * (T)K.zp[c] + (T)om[q] * (T)K.zs[c]);
             else
                 K.x[q][c] = K.x[q][c] + om[q] * (sv / K.ap_w[q][c]);
-            const double r = sv - om[q] * K.t[q][c];
+            const T r = sv - (T)om[q] * K.t[q][c];
             K.r[q][c] = r;
-            red[2 * q] += r * r;
-            red[2 * q + 1] += K.r0[q][c] * r;
+            red[2 * q] += (double)r * (double)r;
+            red[2 * q + 1] += (double)K.r0[q][c] * (double)r;
         }
     }
     double *dst[2 * NS];
@@ -300,6 +301,59 @@ __global__ void __launch_bounds__(BLOCK) k_ell_prologue(EllOp A, const int *iwet
     double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
     put_partials<2>(red, dst);
     kry_reset(K.sc[0]);
+}
+
+}  // namespace cms
+
+namespace cms {
+
+// --- FP32 Krylov inside FP64 iterative refinement ----------------------------
+// Outer sweep: d = inner FP32 BiCGStab solve of A d = r (FP32 matrix and
+// vectors, multigrid preconditioner); x += d and r -= A d in FP64; stop when
+// the FP64 ||r|| meets the FP64 solve's own threshold.
+
+// Inner start from the FP64 residual: r = r0 = (float) r64, p = v = x = 0.
+__global__ void __launch_bounds__(BLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
+    double red[2] = {0.0, 0.0};
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const int c = A.at(t).c;
+        const float r = (float)r64[c];
+        Ki.r[0][c] = r; Ki.r0[0][c] = r; Ki.p[0][c] = 0.0f; Ki.v[0][c] = 0.0f; Ki.x[0][c] = 0.0f;
+        red[0] += (double)r * (double)r;
+    }
+    red[1] = red[0];
+    double *const dst[2] = {Ki.red.pr[0], Ki.red.pr[0] + gridDim.x};
+    put_partials<2>(red, dst);
+    kry_reset(Ki.sc[0]);
+}
+
+// x += d, r -= A d (FP64 operator on the FP32 correction), partials ||r||^2.
+__global__ void __launch_bounds__(BLOCK) k_ref_update(StencilOp A, KrySys<1, double> K, const float *d) {
+    double red[2] = {0.0, 0.0};
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+        const Idx x = A.at(t);
+        const int c = x.c;
+        K.x[0][c] = K.x[0][c] + (double)d[c];
+        const double r = K.r[0][c] - A.apply_z(d, K.ap_w[0], x);
+        K.r[0][c] = r;
+        red[0] += r * r;
+    }
+    double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
+    put_partials<2>(red, dst);
+}
+
+// One block: the FP64 stop test of the outer sweep.
+__global__ void __launch_bounds__(BLOCK) k_ref_check(KrySys<1, double> K, int nb) {
+    double tot[1];
+    const double *src[1] = {K.red.pr[0]};
+    sum_partials<1>(src, nb, tot);
+    if (threadIdx.x == 0) {
+        const double nr = sqrt(tot[0]);
+        K.sc[0].rnorm = nr;
+        if (nr <= K.sc[0].thr) K.sc[0].done = 1;
+    }
 }
 
 }  // namespace cms

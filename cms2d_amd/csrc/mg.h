@@ -7,6 +7,8 @@
 //     MG_COARSE_MAX cells;
 //   * smoother: one red-black Gauss-Seidel sweep before (red first, from a zero
 //     guess) and after (red first) the coarse correction;
+//   * per level two launches (down: sweep + restriction; up: correction +
+//     sweep), one thread per coarse cell and its four children;
 //   * restriction: sum of the children's residuals (the equations are cell
 //     integrals); prolongation: the coarse value injected into its children;
 //   * coarse operators re-discretised from the fine coefficients: the
@@ -36,8 +38,10 @@ struct MgLevel {
     int nx, ny, nxp;
     float *aN, *aE, *aS, *aW;   // directed off-diagonal coefficients
     float *dinv;                // 1/ap
+    float *ap;                  // ap (level 0 only: the FP32 operator of the inner solve)
     float *e;                   // row excess ap - sum(a), summed when coarsening
-    float *z, *b;               // correction and right-hand side (z, b of level 0 belong to the caller)
+    float *z, *zo, *b;          // down-sweep z, cycle output z, right-hand side (zo of level 0 is the
+                                // caller's M^-1 x and level 0 takes its b from the caller)
     __device__ __forceinline__ int idx(int i, int j) const { return (j + 1) * nxp + (i + 1); }
     __device__ __forceinline__ int n() const { return nx * ny; }
 };

@@ -1,19 +1,18 @@
 // mg_kernels.hip -- the V(1,1) cycle of mg.h.
 //
-// Per level above the coarsest, one cycle is four launches:
-//   k_mg_pre        red-black sweep from z = 0: red z = b/ap, black
-//                   z = (b + sum a_k b_nb/ap_nb)/ap (the red values it needs
-//                   are recomputed, so one launch does both colours)
-//   k_mg_restrict   coarse b = sum over the red children of their residual,
-//                   which after that sweep is sum a_k z_nb exactly (red z =
-//                   b/ap cancels b - ap z) and zero on black children
+// Per level above the coarsest, one cycle is two launches:
+//   k_mg_down  red-black sweep from z = 0 (red z = b/ap, black
+//              z = (b + sum a_k b_nb/ap_nb)/ap) and the coarse right-hand
+//              side: the sum over the red children of their residual, which
+//              after that sweep is sum a_k z_nb exactly (red z = b/ap cancels
+//              b - ap z) and zero on black children
 //   (coarser levels, then k_mg_coarsest in one workgroup)
-//   k_mg_post<RED>  red cells: z = (b + sum a_k (z_nb + zc(parent(nb))))/ap;
-//                   the coarse correction of the red cell itself is replaced
-//                   by this update, so only the neighbours' is needed
-//   k_mg_post<BLACK> z = (b + sum a_k z_nb)/ap on black cells
-// Every kernel writes only cells of one colour or reads only what an earlier
-// launch wrote, so no launch reads values it also writes.
+//   k_mg_up    red cells: z = (b + sum a_k (z_nb + zc(parent(nb))))/ap -- the
+//              coarse correction of the red cell itself is replaced by this
+//              update, so only its neighbours' is needed; then black cells:
+//              z = (b + sum a_k z_nb)/ap with the new red values
+// The down sweep writes z and reads only b; the up sweep reads z and the
+// coarse zo and writes zo, so no launch reads values it also writes.
 #include "mg.h"
 #include "krylov.h"
 
@@ -42,6 +41,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_setup0(Geo g, const double *aN, co
     L.aS[c] = (float)s_;
     L.aW[c] = (float)w_;
     L.dinv[c] = d > 0.0 ? (float)(1.0 / d) : 0.0f;
+    L.ap[c] = (float)d;
     L.e[c] = (float)fmax(d - (((n_ + e_) + s_) + w_), 0.0);
 }
 
@@ -70,81 +70,141 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
     C.dinv[c] = ap > 0.0f ? 1.0f / ap : 0.0f;
 }
 
+// --- fused cycle: one launch down and one up per level ------------------------
+// A thread owns one coarse cell and its 2x2 children: (i0,j0) and (i0+1,j0+1)
+// are red, the other two black.  Values needed from children of other coarse
+// cells are recomputed by the same expressions, so they are bitwise the values
+// their owners store.
+
+// Black z after the red-black sweep from zero (red z = b/ap), or 0 outside
+// the grid.
 template <class TB>
-__global__ void __launch_bounds__(BLOCK) k_mg_pre(MgLevel L, const TB *b, const KryScal *sc) {
-    if (sc->done) return;
-    int i, j, c;
-    if (!mg_thread(L, i, j, c)) return;
-    const float bc = (float)b[c];
-    float z;
-    if (((i + j) & 1) == 0) {
-        z = bc * L.dinv[c];
-    } else {
-        const int cN = c + L.nxp, cE = c + 1, cS = c - L.nxp, cW = c - 1;
-        const float s = (L.aN[c] * ((float)b[cN] * L.dinv[cN]) + L.aE[c] * ((float)b[cE] * L.dinv[cE])) +
-                        (L.aS[c] * ((float)b[cS] * L.dinv[cS]) + L.aW[c] * ((float)b[cW] * L.dinv[cW]));
-        z = (bc + s) * L.dinv[c];
-    }
-    L.z[c] = z;
+__device__ __forceinline__ float mg_black0(const MgLevel &L, const TB *b, int i, int j) {
+    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
+    const int c = L.idx(i, j);
+    const int cN = c + L.nxp, cE = c + 1, cS = c - L.nxp, cW = c - 1;
+    const float s = (L.aN[c] * ((float)b[cN] * L.dinv[cN]) + L.aE[c] * ((float)b[cE] * L.dinv[cE])) +
+                    (L.aS[c] * ((float)b[cS] * L.dinv[cS]) + L.aW[c] * ((float)b[cW] * L.dinv[cW]));
+    return ((float)b[c] + s) * L.dinv[c];
 }
 
-__device__ __forceinline__ float mg_nbsum(const MgLevel &L, int c) {
-    return (L.aN[c] * L.z[c + L.nxp] + L.aE[c] * L.z[c + 1]) + (L.aS[c] * L.z[c - L.nxp] + L.aW[c] * L.z[c - 1]);
-}
-
-__global__ void __launch_bounds__(BLOCK) k_mg_restrict(MgLevel f, MgLevel C, const KryScal *sc) {
-    if (sc->done) return;
-    int I, J, c;
-    if (!mg_thread(C, I, J, c)) return;
-    const int i0 = 2 * I, j0 = 2 * J;
-    float r = mg_nbsum(f, f.idx(i0, j0));
-    if (i0 + 1 < f.nx && j0 + 1 < f.ny) r += mg_nbsum(f, f.idx(i0 + 1, j0 + 1));
-    C.b[c] = r;
-}
-
-// RED = 1: red cells with the coarse correction of their neighbours (zc on
-// the coarse level C); RED = 0: black cells.
-template <class TB, int RED>
-__global__ void __launch_bounds__(BLOCK) k_mg_post(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
-    if (sc->done) return;
-    int i, j, c;
-    if (!mg_thread(L, i, j, c)) return;
-    if ((((i + j) & 1) == 0) != (RED == 1)) return;
-    float s;
-    if (RED) {
-        // parents of the four neighbours; a neighbour outside the grid has a
-        // zero coefficient, its parent index is clamped to stay in the array
-        const int pN = C.idx(i >> 1, min(j + 1, L.ny - 1) >> 1), pS = C.idx(i >> 1, max(j - 1, 0) >> 1);
-        const int pE = C.idx(min(i + 1, L.nx - 1) >> 1, j >> 1), pW = C.idx(max(i - 1, 0) >> 1, j >> 1);
-        s = (L.aN[c] * (L.z[c + L.nxp] + C.z[pN]) + L.aE[c] * (L.z[c + 1] + C.z[pE])) +
-            (L.aS[c] * (L.z[c - L.nxp] + C.z[pS]) + L.aW[c] * (L.z[c - 1] + C.z[pW]));
-    } else {
-        s = mg_nbsum(L, c);
-    }
-    L.z[c] = ((float)b[c] + s) * L.dinv[c];
-}
-
-// Coarsest grid: MG_COARSE_SWEEPS red-black sweeps from z = 0 in one block.
+// Residual of red cell (i,j) after that sweep: sum a_k z_nb (b - ap z = 0).
 template <class TB>
-__global__ void __launch_bounds__(1024) k_mg_coarsest(MgLevel L, const TB *b, const KryScal *sc) {
+__device__ __forceinline__ float mg_red_res(const MgLevel &L, const TB *b, int i, int j, float zN, float zE,
+                                            float zS, float zW) {
+    const int c = L.idx(i, j);
+    return (L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW);
+}
+
+// Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
     if (sc->done) return;
-    const int n = L.nx * L.ny;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int j = t / L.nx, i = t - j * L.nx;
-        L.z[L.idx(i, j)] = 0.0f;
+    int I, J, cc;
+    if (!mg_thread(C, I, J, cc)) return;
+    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
+    const bool hi = i1 < f.nx, hj = j1 < f.ny;
+    // black children (i1,j0), (i0,j1) and the black neighbours of the red ones
+    const float zb10 = mg_black0(f, b, i1, j0), zb01 = mg_black0(f, b, i0, j1);
+    const float zbW = mg_black0(f, b, i0 - 1, j0), zbS = mg_black0(f, b, i0, j0 - 1);
+    const int c00 = f.idx(i0, j0);
+    f.z[c00] = (float)b[c00] * f.dinv[c00];
+    float r = mg_red_res(f, b, i0, j0, zb01, zb10, zbS, zbW);
+    if (hi) f.z[f.idx(i1, j0)] = zb10;
+    if (hj) f.z[f.idx(i0, j1)] = zb01;
+    if (hi && hj) {
+        const int c11 = f.idx(i1, j1);
+        f.z[c11] = (float)b[c11] * f.dinv[c11];
+        const float zbE = mg_black0(f, b, i1 + 1, j1), zbN = mg_black0(f, b, i1, j1 + 1);
+        r += mg_red_res(f, b, i1, j1, zbN, zbE, zb10, zb01);
+    }
+    C.b[cc] = r;
+}
+
+// Red z after the coarse correction and the red half-sweep, or 0 outside.
+// zd: z of the down sweep, zc: coarse correction (on C).
+template <class TB>
+__device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const MgLevel &C, int i, int j) {
+    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
+    const int c = L.idx(i, j);
+    const int pN = C.idx(i >> 1, min(j + 1, L.ny - 1) >> 1), pS = C.idx(i >> 1, max(j - 1, 0) >> 1);
+    const int pE = C.idx(min(i + 1, L.nx - 1) >> 1, j >> 1), pW = C.idx(max(i - 1, 0) >> 1, j >> 1);
+    const float s = (L.aN[c] * (L.z[c + L.nxp] + C.zo[pN]) + L.aE[c] * (L.z[c + 1] + C.zo[pE])) +
+                    (L.aS[c] * (L.z[c - L.nxp] + C.zo[pS]) + L.aW[c] * (L.z[c - 1] + C.zo[pW]));
+    return ((float)b[c] + s) * L.dinv[c];
+}
+
+// Up: coarse correction and post-smoothing sweep (red, then black) of level
+// L into L.zo.
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_mg_up(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
+    if (sc->done) return;
+    int I, J, cc;
+    if (!mg_thread(C, I, J, cc)) return;
+    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
+    const bool hi = i1 < L.nx, hj = j1 < L.ny;
+    const float r00 = mg_red1(L, b, C, i0, j0);
+    const float r11 = (hi && hj) ? mg_red1(L, b, C, i1, j1) : 0.0f;
+    L.zo[L.idx(i0, j0)] = r00;
+    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
+    if (hi) {
+        // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
+        const int c = L.idx(i1, j0);
+        const float zN = r11, zE = mg_red1(L, b, C, i1 + 1, j0), zS = mg_red1(L, b, C, i1, j0 - 1), zW = r00;
+        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    }
+    if (hj) {
+        // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
+        const int c = L.idx(i0, j1);
+        const float zN = mg_red1(L, b, C, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, i0 - 1, j1);
+        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    }
+}
+
+// Coarsest grid: MG_COARSE_SWEEPS red-black sweeps from z = 0 in one block
+// with z in LDS (padded layout of the level, ghost ring zero) and each
+// thread's cells in registers; result to L.zo.
+constexpr int MG_CT = 1024;
+constexpr int MG_CPT = MG_COARSE_MAX / MG_CT;   // cells per thread
+template <class TB>
+__global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, const KryScal *sc) {
+    if (sc->done) return;
+    extern __shared__ float zs[];
+    const int n = L.nx * L.ny, npad = L.nxp * (L.ny + 2);
+    for (int q = threadIdx.x; q < npad; q += MG_CT) zs[q] = 0.0f;
+    float aN[MG_CPT], aE[MG_CPT], aS[MG_CPT], aW[MG_CPT], di[MG_CPT], bb[MG_CPT];
+    int cix[MG_CPT], col[MG_CPT];
+#pragma unroll
+    for (int k = 0; k < MG_CPT; k++) {
+        const int t = threadIdx.x + k * MG_CT;
+        cix[k] = -1;
+        col[k] = -1;
+        aN[k] = aE[k] = aS[k] = aW[k] = di[k] = bb[k] = 0.0f;
+        if (t < n) {
+            const int j = t / L.nx, i = t - j * L.nx, c = L.idx(i, j);
+            cix[k] = c;
+            col[k] = (i + j) & 1;
+            aN[k] = L.aN[c]; aE[k] = L.aE[c]; aS[k] = L.aS[c]; aW[k] = L.aW[c];
+            di[k] = L.dinv[c];
+            bb[k] = (float)b[c];
+        }
     }
     __syncthreads();
     for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
-        for (int col = 0; col < 2; col++) {
-            for (int t = threadIdx.x; t < n; t += blockDim.x) {
-                const int j = t / L.nx, i = t - j * L.nx;
-                if (((i + j) & 1) != col) continue;
-                const int c = L.idx(i, j);
-                L.z[c] = ((float)b[c] + mg_nbsum(L, c)) * L.dinv[c];
+        for (int cl = 0; cl < 2; cl++) {
+#pragma unroll
+            for (int k = 0; k < MG_CPT; k++) {
+                if (col[k] != cl) continue;
+                const int c = cix[k];
+                const float s = (aN[k] * zs[c + L.nxp] + aE[k] * zs[c + 1]) + (aS[k] * zs[c - L.nxp] + aW[k] * zs[c - 1]);
+                zs[c] = (bb[k] + s) * di[k];
             }
             __syncthreads();
         }
     }
+#pragma unroll
+    for (int k = 0; k < MG_CPT; k++)
+        if (cix[k] >= 0) L.zo[cix[k]] = zs[cix[k]];
 }
 
 }  // namespace cms

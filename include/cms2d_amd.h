@@ -106,6 +106,17 @@ double cms2d_flow_time(const cms2d_flow *flow);
  * so far ("spmv_pp": pp-system SpMV), and launches counted. */
 int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
 
+/* Batched ensemble: advances each of nflows independent flow objects by
+ * nsteps time steps.  No counterpart in the reference (which runs one domain
+ * per process): it runs that many CMS_Flow time loops (CMS_Flow.f90:43-87)
+ * side by side, members taken largest first by nthreads host threads (0 -> 8),
+ * each member on its own stream, so small domains share the device.  infos
+ * (optional, [nflows]) receives each member's info as cms2d_flow_step gives
+ * it.  Every member completes its steps; the status is the first failing
+ * member's (in member order) or CMS2D_OK.  A flow object must not appear
+ * twice or be used by another thread during the call. */
+int cms2d_ensemble_step(cms2d_flow **flows, int nflows, int nsteps, int nthreads, cms2d_step_info *infos);
+
 /* Drop-in for solve(acoef,ss,sp,res,phi,n) on the reference's general cell
  * list (solvers.f90:205-304): nmaxfaces = 4, 0-based cell2cell with ghost cells
  * numbered ncells..ncellsD-1 (-1 for none), host arrays in the reference layout

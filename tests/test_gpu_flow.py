@@ -78,8 +78,10 @@ def test_tidal_multigrid_levels_matches_oracle(gpu_available):
     c = cases.tidal(nx=201, ny=121, dx=20.0, seed=4)
     o, g = run_pair(c, 8)
     err = compare(o, g)
-    kry = g.step(0)
-    print("max err", err, kry)
+    info = g.step(0)                    # counters of the last step
+    print("max err", err, info)
+    # the preconditioner's quality: a wrong cycle still converges, only slowly
+    assert info["kry"][0] <= 15 * info["niter"], info
 
 
 @pytest.mark.gpu
