@@ -37,6 +37,50 @@ static int set_err(int code, const std::string &m) {
             return set_err(CMS2D_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Live timing of selected kernel groups: a timed launch carries start/stop
+// events stamped by its own dispatch; after the host synchronises, the
+// launches of iterations before the solve stopped (those that did work) are
+// added up.
+enum { TG_SPMV_PP = 0, TG_MG_DOWN0 = 1, TG_MG_UP0 = 2, TG_N = 3 };
+static const char *const TG_NAME[TG_N] = {"spmv_pp", "mg_down0", "mg_up0"};
+struct KTimer {
+    struct Rec {
+        int group, it;
+    };
+    std::vector<hipEvent_t> pool;   // event pairs
+    std::vector<Rec> rec;           // one per pair in use
+    int it = 0;                     // Krylov iteration of the launches being enqueued
+    double ms[TG_N]{};
+    long n[TG_N]{};
+    bool take(int group, hipEvent_t &a, hipEvent_t &b) {
+        const size_t k = rec.size();
+        if (2 * k + 2 > pool.size()) return false;
+        a = pool[2 * k];
+        b = pool[2 * k + 1];
+        rec.push_back({group, it});
+        return true;
+    }
+    int harvest(int upto) {
+        for (size_t k = 0; k < rec.size(); k++) {
+            if (rec[k].it >= upto) continue;
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, pool[2 * k], pool[2 * k + 1]));
+            ms[rec[k].group] += t;
+            n[rec[k].group] += 1;
+        }
+        rec.clear();
+        return 0;
+    }
+};
+
+template <class Kern, class... Args>
+static void tlaunch(KTimer *tm, int group, Kern kern, dim3 grid, dim3 blk, size_t lds, hipStream_t st,
+                    Args... args) {
+    hipEvent_t a, b;
+    if (tm && tm->take(group, a, b)) hipExtLaunchKernelGGL(kern, grid, blk, lds, st, a, b, 0, args...);
+    else hipLaunchKernelGGL(kern, grid, blk, lds, st, args...);
+}
+
 struct cms2d_flow {
     int dev = 0;
     hipStream_t st = nullptr;
@@ -62,9 +106,8 @@ struct cms2d_flow {
     int kry[3]{}, prev_iters[4]{4, 2, 2, 4};
     // timing of the pp-system SpMV launches (events bracket each launch)
     bool stats = true;
-    std::vector<hipEvent_t> ev;
-    double spmv_ms = 0.0;
-    long spmv_n = 0;
+    KTimer tm;                 // live timing of the pp solve's main kernels
+    KTimer *tm_active = nullptr;
     hipEvent_t step_ev[2]{};
 };
 
@@ -137,9 +180,7 @@ struct KryCtx {
     double rtol, atol;
     KryScal *d_sc, *h_sc;
     int *prev_iters;
-    std::vector<hipEvent_t> *ev;   // event pairs bracketing timed SpMV launches (or null)
-    double *ev_ms;
-    long *ev_n;
+    KTimer *tm;                    // timed launches (or null)
 };
 
 struct NoPrec {
@@ -169,20 +210,11 @@ static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre())
     int batch = std::max(1, X.prev_iters[sc0] - 1);
     while (!all_done() && it < X.kmax) {
         int nb = std::min(batch, X.kmax - it);
-        int nev = 0;
-        const int it0 = it;
         for (int b = 0; b < nb; b++, it++) {
+            if (X.tm) X.tm->it = it;
             hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
             if (PREC) pre(0);
-            bool rec = X.ev && nev < (int)X.ev->size() / 2;
-            if (rec) {
-                // events stamped by the dispatch itself (start and end of this kernel)
-                hipExtLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC, T>), grid, blk, 0, X.st, (*X.ev)[2 * nev],
-                                      (*X.ev)[2 * nev + 1], 0, A, K);
-                nev++;
-            } else {
-                hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 1, PREC, T>), grid, blk, 0, X.st, A, K);
-            }
+            tlaunch(X.tm, TG_SPMV_PP, k_bicg_spmv<Op, NS, 1, PREC, T>, grid, blk, 0, X.st, A, K);
             hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, it, X.nblk);
             if (PREC) pre(1);
             hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC, T>), grid, blk, 0, X.st, A, K);
@@ -192,17 +224,10 @@ static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre())
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
         HIPCHK(hipStreamSynchronize(X.st));
-        if (nev > 0) {
+        if (X.tm) {
             // launches that did work: those of iterations the first system completed
             const KryScal &s0 = X.h_sc[sc0];
-            int upto = s0.done ? s0.final_iters : it;
-            int real = std::min(nev, std::max(0, upto - it0));
-            for (int e = 0; e < real; e++) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, (*X.ev)[2 * e], (*X.ev)[2 * e + 1]));
-                *X.ev_ms += ms;
-                *X.ev_n += 1;
-            }
+            if (int rc = X.tm->harvest(s0.done ? s0.final_iters : it)) return rc;
         }
         batch = grow;
         if (++nbatch >= 2) grow = std::min(2 * grow, 64);
@@ -231,17 +256,22 @@ static void mg_vcycle(cms2d_flow *F, const TB *b0, float *z0, const KryScal *sc)
     L0.zo = z0;
     const MgLevel &Lc = M.nlev == 1 ? L0 : M.L[M.nlev - 1];
     const size_t lds = sizeof(float) * (size_t)Lc.nxp * (Lc.ny + 2);
+    KTimer *tm = F->tm_active;
     if (M.nlev == 1) {
         hipLaunchKernelGGL(k_mg_coarsest<TB>, dim3(1), dim3(MG_CT), lds, F->st, L0, b0, sc);
         return;
     }
-    LAUNCH(k_mg_down<TB>, mg_nblk(M.L[1]), L0, b0, M.L[1], sc);
+    auto tiles = [](const MgLevel &L) { return dim3((L.nx + MT_X - 1) / MT_X, (L.ny + MT_Y - 1) / MT_Y); };
+    const dim3 tb(MT_THREADS);
+    tlaunch(tm, TG_MG_DOWN0, k_mg_down<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
     for (int l = 1; l + 1 < M.nlev; l++)
-        LAUNCH(k_mg_down<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+        hipLaunchKernelGGL(k_mg_down<float>, tiles(M.L[l]), tb, 0, F->st, M.L[l], (const float *)M.L[l].b,
+                           M.L[l + 1], sc);
     hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(MG_CT), lds, F->st, Lc, (const float *)Lc.b, sc);
     for (int l = M.nlev - 2; l >= 1; l--)
-        LAUNCH(k_mg_up<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
-    LAUNCH(k_mg_up<TB>, mg_nblk(M.L[1]), L0, b0, M.L[1], sc);
+        hipLaunchKernelGGL(k_mg_up<float>, tiles(M.L[l]), tb, 0, F->st, M.L[l], (const float *)M.L[l].b,
+                           M.L[l + 1], sc);
+    tlaunch(tm, TG_MG_UP0, k_mg_up<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
 }
 
 // Inner FP32 solve: relative stop and iteration cap per refinement sweep.
@@ -266,7 +296,7 @@ static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
     Xi.kmax = REF_KMAX;
     Xi.rtol = REF_RTOL;
     Xi.atol = 0.0;
-    Xi.ev = nullptr;
+    Xi.tm = nullptr;
     auto pre = [F, &Ki](int which) {
         if (which == 0) mg_vcycle<float>(F, Ki.p[0], Ki.zp, Ki.sc);
         else mg_vcycle<float>(F, Ki.r[0], Ki.zs, Ki.sc);
@@ -290,8 +320,7 @@ static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
     X.st = F->st; X.nblk = F->nblk_kry; X.kmax = F->P.kmax;
     X.rtol = F->P.rtol; X.atol = F->P.atol;
     X.d_sc = F->d_sc; X.h_sc = F->h_sc; X.prev_iters = F->prev_iters;
-    X.ev = (timed && F->stats) ? &F->ev : nullptr;
-    X.ev_ms = &F->spmv_ms; X.ev_n = &F->spmv_n;
+    X.tm = (timed && F->stats) ? &F->tm : nullptr;
     return X;
 }
 
@@ -376,7 +405,10 @@ static int flow_imp(cms2d_flow *F) {
                     if (which == 0) mg_vcycle<double>(F, F->Kpp.p[0], F->Kpp.zp, F->d_sc);
                     else mg_vcycle<double>(F, F->Kpp.r[0], F->Kpp.zs, F->d_sc);
                 };
-                if (int rc = run_bicg<StencilOp, 1, 1, double>(X, F->Kpp, A, 0, pre)) return rc;
+                F->tm_active = X.tm;
+                int rc = run_bicg<StencilOp, 1, 1, double>(X, F->Kpp, A, 0, pre);
+                F->tm_active = nullptr;
+                if (rc) return rc;
             } else {
                 if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
             }
@@ -445,7 +477,7 @@ extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
     for (void *p : F->allocs) (void)hipFree(p);
     if (F->h_sc) (void)hipHostFree(F->h_sc);
     if (F->h_flags) (void)hipHostFree(F->h_flags);
-    for (auto e : F->ev) (void)hipEventDestroy(e);
+    for (auto e : F->tm.pool) (void)hipEventDestroy(e);
     for (auto e : F->step_ev) if (e) (void)hipEventDestroy(e);
     if (F->st) (void)hipStreamDestroy(F->st);
     delete F;
@@ -534,18 +566,20 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (int rc = dalloc(F, &F->d_sc, 4)) return rc;
     // Krylov vectors
     {
-        double *kv[3][6];
+        double *kv[3][7];
         for (int q = 0; q < 3; q++)
-            for (int m = 0; m < 6; m++)
+            for (int m = 0; m < 7; m++)
                 if (int rc = dalloc(F, &kv[q][m], N)) return rc;
         F->Kpp.x[0] = s.pp;
         F->Kpp.r[0] = kv[0][0]; F->Kpp.r0[0] = kv[0][1]; F->Kpp.p[0] = kv[0][2];
         F->Kpp.v[0] = kv[0][3]; F->Kpp.t[0] = kv[0][4]; F->Kpp.ap_w[0] = kv[0][5];
+        F->Kpp.dinv_w[0] = kv[0][6];
         F->Kpp.sc = F->d_sc;
         for (int q = 0; q < 2; q++) {
             F->Kuv.x[q] = q ? s.v : s.u;
             F->Kuv.r[q] = kv[q + 1][0]; F->Kuv.r0[q] = kv[q + 1][1]; F->Kuv.p[q] = kv[q + 1][2];
             F->Kuv.v[q] = kv[q + 1][3]; F->Kuv.t[q] = kv[q + 1][4]; F->Kuv.ap_w[q] = kv[q + 1][5];
+            F->Kuv.dinv_w[q] = kv[q + 1][6];
         }
         F->Kuv.sc = F->d_sc + 1;
     }
@@ -592,8 +626,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
     memset(F->h_sc, 0, 4 * sizeof(KryScal));
-    F->ev.resize(2 * 64);
-    for (auto &e : F->ev) HIPCHK(hipEventCreate(&e));
+    F->tm.pool.resize(2 * 256);
+    for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
     // inputs
     if (int rc = upload_padded(F, s.zb, d->zb, true)) return rc;
@@ -770,10 +804,13 @@ extern "C" double cms2d_flow_time(const cms2d_flow *F) { return F ? F->timesecs 
 
 extern "C" int cms2d_flow_kernel_stats(const cms2d_flow *F, const char *name, double *avg_ms, long *count) {
     if (!F || !name || !avg_ms || !count) return set_err(CMS2D_ERR_ARG, "null argument");
-    if (strcmp(name, "spmv_pp")) return set_err(CMS2D_ERR_ARG, std::string("unknown kernel group ") + name);
-    *count = F->spmv_n;
-    *avg_ms = F->spmv_n ? F->spmv_ms / F->spmv_n : 0.0;
-    return CMS2D_OK;
+    for (int k = 0; k < TG_N; k++)
+        if (!strcmp(name, TG_NAME[k])) {
+            *count = F->tm.n[k];
+            *avg_ms = F->tm.n[k] ? F->tm.ms[k] / F->tm.n[k] : 0.0;
+            return CMS2D_OK;
+        }
+    return set_err(CMS2D_ERR_ARG, std::string("unknown kernel group ") + name);
 }
 
 // ----------------------------------------------------------------------------
@@ -803,7 +840,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
     int *d_w = (int *)alloc(nD * sizeof(int));
     double *d_ss = (double *)alloc(nD * sizeof(double)), *d_sp = (double *)alloc(nD * sizeof(double));
     double *d_res = (double *)alloc(nD * sizeof(double)), *d_phi = (double *)alloc(nD * sizeof(double));
-    double *vec = (double *)alloc(6 * n * sizeof(double));
+    double *vec = (double *)alloc(7 * n * sizeof(double));
     double *partials = (double *)alloc((size_t)nblk * 5 * sizeof(double));
     KryScal *d_sc = (KryScal *)alloc(sizeof(KryScal));
     KryScal h_sc{};
@@ -827,6 +864,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
         K.x[0] = d_phi;
         K.r[0] = vec; K.r0[0] = vec + n; K.p[0] = vec + 2 * n; K.v[0] = vec + 3 * n; K.t[0] = vec + 4 * n;
         K.ap_w[0] = vec + 5 * n;
+        K.dinv_w[0] = vec + 6 * n;
         K.sc = d_sc;
         K.red.pr[0] = partials; K.red.pv[0] = partials + 2 * nblk; K.red.pt[0] = partials + 3 * nblk;
         EllOp A{ncells, d_c2c, d_a};
@@ -834,7 +872,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
                            (const double *)d_sp, d_res, (const double *)d_phi, relax, K);
         int prev = 8;
         KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,
-                 d_sc, &h_sc, &prev, nullptr, nullptr, nullptr};
+                 d_sc, &h_sc, &prev, nullptr};
         if ((rc = run_bicg(X, K, A, 0))) break;
         ok = hipMemcpyAsync(ss, d_ss, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
              hipMemcpyAsync(res, d_res, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&

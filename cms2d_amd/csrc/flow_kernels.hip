@@ -550,10 +550,12 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
     }
     __syncthreads();
     if (threadIdx.x == 0 && adjust) {
-        // sequential prefix in the reference's order
+        // sequential prefix in the reference's order, carried in a register
         term[im0] = 0.0;
-        for (int im = im0 + 1; im < n; im++) term[im] = term[im - 1] + term[im];
-        for (int im = im0 - 1; im >= 0; im--) term[im] = term[im + 1] + term[im];
+        double acc = 0.0;
+        for (int im = im0 + 1; im < n; im++) { acc = acc + term[im]; term[im] = acc; }
+        acc = 0.0;
+        for (int im = im0 - 1; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
     }
     __syncthreads();
     for (int im = threadIdx.x; im < n; im += blockDim.x) wse[im] = adjust ? wsebnd + term[im] : wsebnd;
@@ -624,6 +626,7 @@ __global__ void __launch_bounds__(BLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> 
         s.resp[c] = res;
         double r = anb + ss - ap * phi + SMALL;
         K.ap_w[0][c] = ap;
+        K.dinv_w[0][c] = 1.0 / ap;
         K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
         red[0] += r * r;
         red[1] += res * res;

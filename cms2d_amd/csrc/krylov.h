@@ -62,8 +62,10 @@ __device__ __forceinline__ Chunk block_chunk(int n) {
     return ch;
 }
 
-// Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k), z = M^-1 x = x/ap,
-// z = 0 on ghost cells.  C is the type of the coefficients and of the result.
+// Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k).  apply() is the
+// Jacobi-preconditioned form, z = M^-1 x = x/ap with 1/ap stored (dinv), so
+// y_i = x_i - sum_k a_k(i) x_nb dinv_nb; z = 0 on ghost cells.  C is the type
+// of the coefficients and of the result.
 template <class C>
 struct StencilOpT {
     Geo g;
@@ -76,16 +78,15 @@ struct StencilOpT {
         x.c = cidx(g, x.i, x.j);
         return x;
     }
-    __device__ __forceinline__ double apply(const double *xv, const double *ap, const Idx &x) const {
+    __device__ __forceinline__ double apply(const double *xv, const double *dinv, const Idx &x) const {
         const int c = x.c;
-        const double zc = xv[c] / ap[c];
         const int cN = c + g.nxp, cE = c + 1, cS = c - g.nxp, cW = c - 1;
-        const double zN = x.j == g.ny - 1 ? 0.0 : xv[cN] / ap[cN];
-        const double zE = x.i == g.nx - 1 ? 0.0 : xv[cE] / ap[cE];
-        const double zS = x.j == 0 ? 0.0 : xv[cS] / ap[cS];
-        const double zW = x.i == 0 ? 0.0 : xv[cW] / ap[cW];
+        const double zN = x.j == g.ny - 1 ? 0.0 : xv[cN] * dinv[cN];
+        const double zE = x.i == g.nx - 1 ? 0.0 : xv[cE] * dinv[cE];
+        const double zS = x.j == 0 ? 0.0 : xv[cS] * dinv[cS];
+        const double zW = x.i == 0 ? 0.0 : xv[cW] * dinv[cW];
         const double sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
-        return ap[c] * zc - sum;
+        return xv[c] - sum;
     }
     // y_i = ap_i z_i - sum_k a_k(i) z_nb(k) for an explicit preconditioned
     // vector z (FP32, zero on the ghost ring), evaluated in C.
@@ -115,15 +116,15 @@ struct EllOp {
         x.i = x.j = 0;
         return x;
     }
-    __device__ __forceinline__ double apply(const double *xv, const double *ap, const Idx &x) const {
+    __device__ __forceinline__ double apply(const double *xv, const double *dinv, const Idx &x) const {
         const int c = x.c;
         double sum = 0.0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             int nb = c2c[4 * c + k];
-            if (nb >= 0 && nb < ncells) sum += a[4 * c + k] * (xv[nb] / ap[nb]);
+            if (nb >= 0 && nb < ncells) sum += a[4 * c + k] * (xv[nb] * dinv[nb]);
         }
-        return ap[c] * (xv[c] / ap[c]) - sum;
+        return xv[c] - sum;
     }
 };
 
@@ -133,6 +134,7 @@ template <int NS, class T = double>
 struct KrySys {
     T *x[NS], *r[NS], *r0[NS], *p[NS], *v[NS], *t[NS];
     T *ap_w[NS];       // diagonal ap of each system (written by the prologue)
+    T *dinv_w[NS];     // 1/ap (Jacobi form)
     float *zp, *zs;    // M^-1 p and M^-1 s of system 0 (explicit preconditioner only)
     KryScal *sc;       // [NS]
     KryRed<NS> red;

@@ -54,6 +54,7 @@ __global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K)
             double r = anb + ssv - ap * ph + SMALL;
             (q ? s.resv : s.resu)[c] = res;
             K.ap_w[q][c] = ap;
+            K.dinv_w[q][c] = 1.0 / ap;
             K.r[q][c] = r; K.r0[q][c] = r; K.p[q][c] = 0.0; K.v[q][c] = 0.0;
             red[2 * q] += r * r;
             red[2 * q + 1] += res * res;
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
             const T *xv = MODE == 1 ? K.p[q] : K.r[q];
             T y;
             if constexpr (PREC == 1) y = A.apply_z(MODE == 1 ? K.zp : K.zs, K.ap_w[q], x);
-            else y = A.apply(xv, K.ap_w[q], x);
+            else y = A.apply(xv, K.dinv_w[q], x);
             const double yd = y;
             if (MODE == 1) {
                 K.v[q][c] = y;
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it,
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
-            if constexpr (PREC == 0) K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] / K.ap_w[q][c]);
+            if constexpr (PREC == 0) K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] * K.dinv_w[q][c]);
             K.r[q][c] = K.r[q][c] - (T)al[q] * K.v[q][c];
         }
     }
@@ -256,7 +257,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it,
             if constexpr (PREC == 1)
                 K.x[q][c] = K.x[q][c] + ((T)al[q] * (T)K.zp[c] + (T)om[q] * (T)K.zs[c]);
             else
-                K.x[q][c] = K.x[q][c] + om[q] * (sv / K.ap_w[q][c]);
+                K.x[q][c] = K.x[q][c] + om[q] * (sv * K.dinv_w[q][c]);
             const T r = sv - (T)om[q] * K.t[q][c];
             K.r[q][c] = r;
             red[2 * q] += (double)r * (double)r;
@@ -294,6 +295,7 @@ __global__ void __launch_bounds__(BLOCK) k_ell_prologue(EllOp A, const int *iwet
         res[c] = rs;
         double r = anb + ssv - ap * ph + SMALL;
         K.ap_w[0][c] = ap;
+        K.dinv_w[0][c] = 1.0 / ap;
         K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
         red[0] += r * r;
         red[1] += rs * rs;

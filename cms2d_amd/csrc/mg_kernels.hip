@@ -1,6 +1,7 @@
 // mg_kernels.hip -- the V(1,1) cycle of mg.h.
 //
-// Per level above the coarsest, one cycle is two launches:
+// Per level above the coarsest, one cycle is two launches over tiles of the
+// level (LDS holds the sweep's intermediate values over a halo):
 //   k_mg_down  red-black sweep from z = 0 (red z = b/ap, black
 //              z = (b + sum a_k b_nb/ap_nb)/ap) and the coarse right-hand
 //              side: the sum over the red children of their residual, which
@@ -71,93 +72,108 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 }
 
 // --- fused cycle: one launch down and one up per level ------------------------
-// A thread owns one coarse cell and its 2x2 children: (i0,j0) and (i0+1,j0+1)
-// are red, the other two black.  Values needed from children of other coarse
-// cells are recomputed by the same expressions, so they are bitwise the values
-// their owners store.
+// A workgroup owns a tile of MT_X x MT_Y fine cells (MT_X/2 x MT_Y/2 coarse
+// cells, one per thread).  Intermediate values of the sweep are built in LDS
+// over the tile and a halo; halo values are recomputed by the same expressions
+// as in the neighbouring tile, so they are bitwise the values stored there.
+constexpr int MT_X = 64, MT_Y = 16, MT_THREADS = (MT_X / 2) * (MT_Y / 2);
+static_assert(MT_THREADS == BLOCK, "one coarse cell per thread");
 
-// Black z after the red-black sweep from zero (red z = b/ap), or 0 outside
-// the grid.
-template <class TB>
-__device__ __forceinline__ float mg_black0(const MgLevel &L, const TB *b, int i, int j) {
-    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
-    const int c = L.idx(i, j);
-    const int cN = c + L.nxp, cE = c + 1, cS = c - L.nxp, cW = c - 1;
-    const float s = (L.aN[c] * ((float)b[cN] * L.dinv[cN]) + L.aE[c] * ((float)b[cE] * L.dinv[cE])) +
-                    (L.aS[c] * ((float)b[cS] * L.dinv[cS]) + L.aW[c] * ((float)b[cW] * L.dinv[cW]));
-    return ((float)b[c] + s) * L.dinv[c];
-}
-
-// Residual of red cell (i,j) after that sweep: sum a_k z_nb (b - ap z = 0).
-template <class TB>
-__device__ __forceinline__ float mg_red_res(const MgLevel &L, const TB *b, int i, int j, float zN, float zE,
-                                            float zS, float zW) {
-    const int c = L.idx(i, j);
-    return (L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW);
+__device__ __forceinline__ bool mg_inside(const MgLevel &L, int i, int j) {
+    return i >= 0 && j >= 0 && i < L.nx && j < L.ny;
 }
 
 // Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
+//   q  = b/ap over the tile + 2   (red z of the sweep from zero)
+//   zt = z over the tile + 1      (black z = (b + sum a_k q_nb)/ap)
+//   coarse b = sum over the red children of sum a_k zt_nb (their residual)
 template <class TB>
-__global__ void __launch_bounds__(BLOCK) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
+__global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
     if (sc->done) return;
-    int I, J, cc;
-    if (!mg_thread(C, I, J, cc)) return;
-    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
-    const bool hi = i1 < f.nx, hj = j1 < f.ny;
-    // black children (i1,j0), (i0,j1) and the black neighbours of the red ones
-    const float zb10 = mg_black0(f, b, i1, j0), zb01 = mg_black0(f, b, i0, j1);
-    const float zbW = mg_black0(f, b, i0 - 1, j0), zbS = mg_black0(f, b, i0, j0 - 1);
-    const int c00 = f.idx(i0, j0);
-    f.z[c00] = (float)b[c00] * f.dinv[c00];
-    float r = mg_red_res(f, b, i0, j0, zb01, zb10, zbS, zbW);
-    if (hi) f.z[f.idx(i1, j0)] = zb10;
-    if (hj) f.z[f.idx(i0, j1)] = zb01;
-    if (hi && hj) {
-        const int c11 = f.idx(i1, j1);
-        f.z[c11] = (float)b[c11] * f.dinv[c11];
-        const float zbE = mg_black0(f, b, i1 + 1, j1), zbN = mg_black0(f, b, i1, j1 + 1);
-        r += mg_red_res(f, b, i1, j1, zbN, zbE, zb10, zb01);
+    __shared__ float q[MT_Y + 4][MT_X + 4];
+    __shared__ float zt[MT_Y + 2][MT_X + 2];
+    const int X0 = blockIdx.x * MT_X, Y0 = blockIdx.y * MT_Y;
+    for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
+        const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
+        float v = 0.0f;
+        if (mg_inside(f, i, j)) {
+            const int c = f.idx(i, j);
+            v = (float)b[c] * f.dinv[c];
+        }
+        q[ly][lx] = v;
     }
-    C.b[cc] = r;
-}
-
-// Red z after the coarse correction and the red half-sweep, or 0 outside.
-// zd: z of the down sweep, zc: coarse correction (on C).
-template <class TB>
-__device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const MgLevel &C, int i, int j) {
-    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
-    const int c = L.idx(i, j);
-    const int pN = C.idx(i >> 1, min(j + 1, L.ny - 1) >> 1), pS = C.idx(i >> 1, max(j - 1, 0) >> 1);
-    const int pE = C.idx(min(i + 1, L.nx - 1) >> 1, j >> 1), pW = C.idx(max(i - 1, 0) >> 1, j >> 1);
-    const float s = (L.aN[c] * (L.z[c + L.nxp] + C.zo[pN]) + L.aE[c] * (L.z[c + 1] + C.zo[pE])) +
-                    (L.aS[c] * (L.z[c - L.nxp] + C.zo[pS]) + L.aW[c] * (L.z[c - 1] + C.zo[pW]));
-    return ((float)b[c] + s) * L.dinv[c];
+    __syncthreads();
+    for (int t = threadIdx.x; t < (MT_X + 2) * (MT_Y + 2); t += MT_THREADS) {
+        const int ly = t / (MT_X + 2), lx = t - ly * (MT_X + 2), i = X0 - 1 + lx, j = Y0 - 1 + ly;
+        float z = 0.0f;
+        if (mg_inside(f, i, j)) {
+            const int c = f.idx(i, j);
+            if (((i + j) & 1) == 0) {
+                z = q[ly + 1][lx + 1];
+            } else {
+                const float s = (f.aN[c] * q[ly + 2][lx + 1] + f.aE[c] * q[ly + 1][lx + 2]) +
+                                (f.aS[c] * q[ly][lx + 1] + f.aW[c] * q[ly + 1][lx]);
+                z = ((float)b[c] + s) * f.dinv[c];
+            }
+            if (lx >= 1 && lx <= MT_X && ly >= 1 && ly <= MT_Y) f.z[c] = z;
+        }
+        zt[ly][lx] = z;
+    }
+    __syncthreads();
+    const int cx = threadIdx.x % (MT_X / 2), cy = threadIdx.x / (MT_X / 2);
+    const int I = X0 / 2 + cx, J = Y0 / 2 + cy;
+    if (I >= C.nx || J >= C.ny) return;
+    const int i0 = 2 * I, j0 = 2 * J, lx = i0 - X0 + 1, ly = j0 - Y0 + 1;
+    const int c00 = f.idx(i0, j0);
+    float r = (f.aN[c00] * zt[ly + 1][lx] + f.aE[c00] * zt[ly][lx + 1]) +
+              (f.aS[c00] * zt[ly - 1][lx] + f.aW[c00] * zt[ly][lx - 1]);
+    if (i0 + 1 < f.nx && j0 + 1 < f.ny) {
+        const int c11 = f.idx(i0 + 1, j0 + 1);
+        r += (f.aN[c11] * zt[ly + 2][lx + 1] + f.aE[c11] * zt[ly + 1][lx + 2]) +
+             (f.aS[c11] * zt[ly][lx + 1] + f.aW[c11] * zt[ly + 1][lx]);
+    }
+    C.b[C.idx(I, J)] = r;
 }
 
 // Up: coarse correction and post-smoothing sweep (red, then black) of level
 // L into L.zo.
+//   w  = z + zc(parent) over the tile + 2
+//   rt = new red z = (b + sum a_k w_nb)/ap over the tile + 1
+//   black zo = (b + sum a_k rt_nb)/ap on the tile
+// (the coarse correction of a red cell itself is replaced by its update).
 template <class TB>
-__global__ void __launch_bounds__(BLOCK) k_mg_up(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
+__global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
     if (sc->done) return;
-    int I, J, cc;
-    if (!mg_thread(C, I, J, cc)) return;
-    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
-    const bool hi = i1 < L.nx, hj = j1 < L.ny;
-    const float r00 = mg_red1(L, b, C, i0, j0);
-    const float r11 = (hi && hj) ? mg_red1(L, b, C, i1, j1) : 0.0f;
-    L.zo[L.idx(i0, j0)] = r00;
-    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
-    if (hi) {
-        // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
-        const int c = L.idx(i1, j0);
-        const float zN = r11, zE = mg_red1(L, b, C, i1 + 1, j0), zS = mg_red1(L, b, C, i1, j0 - 1), zW = r00;
-        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    __shared__ float w[MT_Y + 4][MT_X + 4];
+    __shared__ float rt[MT_Y + 2][MT_X + 2];
+    const int X0 = blockIdx.x * MT_X, Y0 = blockIdx.y * MT_Y;
+    for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
+        const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
+        float v = 0.0f;
+        if (mg_inside(L, i, j)) v = L.z[L.idx(i, j)] + C.zo[C.idx(i >> 1, j >> 1)];
+        w[ly][lx] = v;
     }
-    if (hj) {
-        // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
-        const int c = L.idx(i0, j1);
-        const float zN = mg_red1(L, b, C, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, i0 - 1, j1);
-        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    __syncthreads();
+    for (int t = threadIdx.x; t < (MT_X + 2) * (MT_Y + 2); t += MT_THREADS) {
+        const int ly = t / (MT_X + 2), lx = t - ly * (MT_X + 2), i = X0 - 1 + lx, j = Y0 - 1 + ly;
+        float z = 0.0f;
+        if (mg_inside(L, i, j) && ((i + j) & 1) == 0) {
+            const int c = L.idx(i, j);
+            const float s = (L.aN[c] * w[ly + 2][lx + 1] + L.aE[c] * w[ly + 1][lx + 2]) +
+                            (L.aS[c] * w[ly][lx + 1] + L.aW[c] * w[ly + 1][lx]);
+            z = ((float)b[c] + s) * L.dinv[c];
+            if (lx >= 1 && lx <= MT_X && ly >= 1 && ly <= MT_Y) L.zo[c] = z;
+        }
+        rt[ly][lx] = z;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MT_X * MT_Y; t += MT_THREADS) {
+        const int ly = t / MT_X + 1, lx = t % MT_X + 1, i = X0 - 1 + lx, j = Y0 - 1 + ly;
+        if (!mg_inside(L, i, j) || ((i + j) & 1) == 0) continue;
+        const int c = L.idx(i, j);
+        const float s = (L.aN[c] * rt[ly + 1][lx] + L.aE[c] * rt[ly][lx + 1]) +
+                        (L.aS[c] * rt[ly - 1][lx] + L.aW[c] * rt[ly][lx - 1]);
+        L.zo[c] = ((float)b[c] + s) * L.dinv[c];
     }
 }
 
