@@ -22,6 +22,8 @@ PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PRECOND_MG = 0
 PRECOND_JACOBI = 1
+SOLVER_BICGSTAB = 0
+SOLVER_GMRES = 1
 
 ERR = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "diverged", -4: "no device"}
 
@@ -42,7 +44,7 @@ class FlowDesc(C.Structure):
         ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
-        ("device", C.c_int), ("precond", C.c_int),
+        ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int),
     ]
 
 

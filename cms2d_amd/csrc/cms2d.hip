@@ -91,6 +91,10 @@ struct cms2d_flow {
     KrySys<1> Kpp{};
     int precond = CMS2D_PRECOND_MG;
     int precision = CMS2D_PREC_FP64;
+    int solver = CMS2D_SOLVER_BICGSTAB;
+    GmSys gm{};                // GMRES basis and scalars (pp system)
+    GmScal *h_gm = nullptr;    // pinned mirror
+    int prev_gm = 8;
     Mg mg{};                   // pp preconditioner hierarchy (level 0 z is Kpp.zp / Kpp.zs)
     KrySys<1, float> Kin{};    // inner FP32 solve of the refinement form
     KryScal *d_sc = nullptr;   // [4]: pp, u, v, inner FP32 pp
@@ -193,7 +197,7 @@ struct NoPrec {
 // PREC 1: pre(0) enqueues zp = M^-1 p, pre(1) enqueues zs = M^-1 s.
 template <class Op, int NS, int PREC = 0, class T = double, class Pre = NoPrec>
 static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre()) {
-    const dim3 grid(X.nblk), blk(BLOCK);
+    const dim3 grid(X.nblk), blk(KBLOCK);
     // start: thresholds, rmom and the initial stop test
     hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
     HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
@@ -286,7 +290,7 @@ static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
     KrySys<1> &K = F->Kpp;
     KrySys<1, float> &Ki = F->Kin;
     const int nb = F->nblk_kry;
-    hipLaunchKernelGGL((k_bicg_p<StencilOp, 1, double>), dim3(1), dim3(BLOCK), 0, F->st, A, K, 0, nb, X.kmax,
+    hipLaunchKernelGGL((k_bicg_p<StencilOp, 1, double>), dim3(1), dim3(KBLOCK), 0, F->st, A, K, 0, nb, X.kmax,
                        X.rtol, X.atol, 1);
     HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
     HIPCHK(hipStreamSynchronize(F->st));
@@ -303,15 +307,70 @@ static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
     };
     int total = 0;
     for (int sw = 0; sw < REF_SWEEPS && !F->h_sc[0].done; sw++) {
-        hipLaunchKernelGGL(k_ref_start, dim3(nb), dim3(BLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
+        hipLaunchKernelGGL(k_ref_start, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
         if (int rc = run_bicg<StencilOp32, 1, 1, float>(Xi, Ki, A32, 3, pre)) return rc;
         total += F->h_sc[3].final_iters;
-        hipLaunchKernelGGL(k_ref_update, dim3(nb), dim3(BLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
-        hipLaunchKernelGGL(k_ref_check, dim3(1), dim3(BLOCK), 0, F->st, K, nb);
+        hipLaunchKernelGGL(k_ref_update, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
+        hipLaunchKernelGGL(k_ref_check, dim3(1), dim3(KBLOCK), 0, F->st, K, nb);
         HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
         HIPCHK(hipStreamSynchronize(F->st));
     }
     F->h_sc[0].final_iters = total;
+    return 0;
+}
+
+// The pp solve by restarted GMRES with the multigrid preconditioner: cycles
+// of Arnoldi steps enqueued in batches (launches after the cycle ended return
+// at once), each cycle closed by x += M^-1 V y and the true residual.
+static int pp_solve_gmres(cms2d_flow *F, KryCtx &X, StencilOp A) {
+    GmSys &G = F->gm;
+    const int nb = F->nblk_kry, n = F->g.nact;
+    const dim3 grid(nb), blk(KBLOCK);
+    auto sync = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(F->h_gm, G.sc, sizeof(GmScal), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        return 0;
+    };
+    hipLaunchKernelGGL(k_gm_begin, dim3(1), blk, 0, F->st, G, (const double *)F->Kpp.red.pr[0],
+                       (const double *)(F->Kpp.red.pr[0] + nb), nb, 1, n, X.rtol, X.atol);
+    if (int rc = sync()) return rc;
+    int batch = std::max(1, F->prev_gm - 1);
+    while (!F->h_gm->ks.done && F->h_gm->iters < X.kmax) {
+        hipLaunchKernelGGL(k_gm_v0<StencilOp>, grid, blk, 0, F->st, A, G);
+        int j = 0, grow = 2, nbatch = 0;
+        for (;;) {
+            const int cnt = std::min(batch, GM_M - j);
+            for (int b = 0; b < cnt; b++, j++) {
+                mg_vcycle<double>(F, G.V[j], G.z, &G.sc->ks);
+                hipLaunchKernelGGL(k_gm_spmv_dot<StencilOp>, grid, blk, 0, F->st, A, G, j);
+                hipLaunchKernelGGL(k_gm_hred, dim3(1), blk, 0, F->st, G, j, nb);
+                hipLaunchKernelGGL(k_gm_orth<StencilOp>, grid, blk, 0, F->st, A, G, j);
+                hipLaunchKernelGGL(k_gm_finish, dim3(1), blk, 0, F->st, G, j, nb);
+                hipLaunchKernelGGL(k_gm_scale<StencilOp>, grid, blk, 0, F->st, A, G, j);
+            }
+            HIPCHK(hipGetLastError());
+            if (int rc = sync()) return rc;
+            if (F->h_gm->cycle_end || j >= GM_M) break;
+            batch = grow;
+            if (++nbatch >= 2) grow = std::min(2 * grow, 64);
+        }
+        const int k = F->h_gm->j;
+        if (k > 0) {
+            hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, F->st, G);
+            hipLaunchKernelGGL(k_gm_combine<StencilOp>, grid, blk, 0, F->st, A, G, k);
+            mg_vcycle<double>(F, G.V[k], G.z, &G.sc->ks_final);
+            hipLaunchKernelGGL(k_gm_update<StencilOp>, grid, blk, 0, F->st, A, G);
+        }
+        // next cycle (or the stop) from the true residual
+        hipLaunchKernelGGL(k_gm_begin, dim3(1), blk, 0, F->st, G, (const double *)G.part, (const double *)nullptr,
+                           nb, 0, n, X.rtol, X.atol);
+        if (int rc = sync()) return rc;
+        batch = std::max(1, std::min(F->prev_gm, GM_M) - 1);
+        if (k == 0) break;
+    }
+    F->prev_gm = std::max(1, F->h_gm->iters);
+    F->h_sc[0].rmom = F->h_gm->ks.rmom;
+    F->h_sc[0].final_iters = F->h_gm->iters;
     return 0;
 }
 
@@ -381,7 +440,7 @@ static int flow_imp(cms2d_flow *F) {
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
         LAUNCH(k_mom, F->nblk_act, g, s, F->P);
-        hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->Kuv);
+        hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->Kuv);
         {
             KryCtx X = flow_ctx(F, false);
             if (int rc = run_bicg(X, F->Kuv, A, 1)) return rc;
@@ -393,12 +452,15 @@ static int flow_imp(cms2d_flow *F) {
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
             hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd, wsebnd);
         }
-        hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(BLOCK), 0, F->st, g, s, F->P, F->Kpp);
+        hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
             KryCtx X = flow_ctx(F, true);
             if (F->precision == CMS2D_PREC_FP32_REFINE) {
                 mg_setup(F);
                 if (int rc = pp_solve_refine(F, X, A)) return rc;
+            } else if (F->solver == CMS2D_SOLVER_GMRES) {
+                mg_setup(F);
+                if (int rc = pp_solve_gmres(F, X, A)) return rc;
             } else if (F->precond == CMS2D_PRECOND_MG) {
                 mg_setup(F);
                 auto pre = [F](int which) {
@@ -477,6 +539,7 @@ extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
     for (void *p : F->allocs) (void)hipFree(p);
     if (F->h_sc) (void)hipHostFree(F->h_sc);
     if (F->h_flags) (void)hipHostFree(F->h_flags);
+    if (F->h_gm) (void)hipHostFree(F->h_gm);
     for (auto e : F->tm.pool) (void)hipEventDestroy(e);
     for (auto e : F->step_ev) if (e) (void)hipEventDestroy(e);
     if (F->st) (void)hipStreamDestroy(F->st);
@@ -498,6 +561,10 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
     if (d->precision == CMS2D_PREC_FP32_REFINE && d->precond != CMS2D_PRECOND_MG)
         return set_err(CMS2D_ERR_ARG, "the FP32 refinement form uses the multigrid preconditioner");
+    if (d->solver != CMS2D_SOLVER_BICGSTAB && d->solver != CMS2D_SOLVER_GMRES)
+        return set_err(CMS2D_ERR_ARG, "unknown solver");
+    if (d->solver == CMS2D_SOLVER_GMRES && (d->precond != CMS2D_PRECOND_MG || d->precision != CMS2D_PREC_FP64))
+        return set_err(CMS2D_ERR_ARG, "GMRES is built for the FP64 form with the multigrid preconditioner");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device)
         return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
@@ -586,6 +653,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     // multigrid hierarchy: level 0 shares the padded layout of the flow arrays
     F->precond = d->precond;
     F->precision = d->precision;
+    F->solver = d->solver;
     {
         Mg &M = F->mg;
         int nx = g.nx, ny = g.ny;
@@ -617,6 +685,19 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             float **vec[] = {&Ki.x[0], &Ki.r[0], &Ki.r0[0], &Ki.p[0], &Ki.v[0], &Ki.t[0]};
             for (float **v : vec)
                 if (int rc = dalloc(F, v, N)) return rc;
+        }
+        if (F->solver == CMS2D_SOLVER_GMRES) {
+            GmSys &G = F->gm;
+            for (int i = 0; i <= GM_M; i++)
+                if (int rc = dalloc(F, &G.V[i], N)) return rc;
+            if (int rc = dalloc(F, &G.z, N)) return rc;
+            if (int rc = dalloc(F, &G.part, (size_t)(GM_M + 2) * F->nblk_kry)) return rc;
+            if (int rc = dalloc(F, &G.sc, 1)) return rc;
+            G.x = s.pp;
+            G.r = F->Kpp.r[0];
+            G.ap = F->Kpp.ap_w[0];
+            HIPCHK(hipHostMalloc((void **)&F->h_gm, sizeof(GmScal), hipHostMallocDefault));
+            memset(F->h_gm, 0, sizeof(GmScal));
         }
         Ki.ap_w[0] = M.L[0].ap;
         Ki.zp = F->Kpp.zp;
@@ -868,7 +949,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
         K.sc = d_sc;
         K.red.pr[0] = partials; K.red.pv[0] = partials + 2 * nblk; K.red.pt[0] = partials + 3 * nblk;
         EllOp A{ncells, d_c2c, d_a};
-        hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(BLOCK), 0, st, A, (const int *)d_w, d_ss,
+        hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(KBLOCK), 0, st, A, (const int *)d_w, d_ss,
                            (const double *)d_sp, d_res, (const double *)d_phi, relax, K);
         int prev = 8;
         KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,

@@ -12,6 +12,9 @@ namespace cms {
 
 constexpr int KN = 0, KE = 1, KS = 2, KW = 3;
 constexpr int BLOCK = 256;
+// Threads per block of the kernels on the fixed Krylov grid (8 waves per
+// SIMD with 4 blocks per CU).
+constexpr int KBLOCK = 512;
 constexpr double GRAV = 9.81;                       // 9.81_ikind, flow.F90:35
 constexpr double GRAVINV = 1.0 / 9.81;              // flow.F90:37
 
@@ -59,9 +62,9 @@ __device__ __forceinline__ bool act_thread(const Geo &g, int &i, int &j, int &c)
 }
 
 // --- deterministic reductions -------------------------------------------------
-template <int NV>
+template <int NV, int BS = BLOCK>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[NV][BLOCK / 64];
+    __shared__ double sh[NV][BS / 64];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double x = v[k];
@@ -79,7 +82,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     for (int k = 0; k < NV; k++) {
         double s = sh[k][0];
 #pragma unroll
-        for (int q = 1; q < BLOCK / 64; q++) s += sh[k][q];
+        for (int q = 1; q < BS / 64; q++) s += sh[k][q];
         v[k] = s;
     }
 }

@@ -564,10 +564,10 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
 // pp sources (flow_imp.F90:700-712) + bound_pp (boundaries.F90:3493-3646) +
 // solve prologue for n=1 (solvers.f90:232-240, relax(1)=1) + BiCGStab start.
 // Runs on the fixed Krylov grid and leaves r.r / res.res partials.
-__global__ void __launch_bounds__(BLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> K) {
+__global__ void __launch_bounds__(KBLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> K) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(g.nact);
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
         double sp = s.sppp0[c];
         double fs = ((s.fluxN[c] + s.fluxE[c]) + (-s.fluxN[c - g.nxp])) + (-s.fluxE[c - 1]);

@@ -55,7 +55,7 @@ struct Chunk {
 };
 __device__ __forceinline__ Chunk block_chunk(int n) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+    const int per = ((n + (int)gridDim.x - 1) / (int)gridDim.x + KBLOCK - 1) / KBLOCK * KBLOCK;
     Chunk ch;
     ch.begin = min(n, b * per);
     ch.end = min(n, ch.begin + per);
@@ -140,6 +140,32 @@ struct KrySys {
     KryRed<NS> red;
 };
 
+// Restarted GMRES(m) (pgmres, source/solvers/pgmres.f90:2-146: GMRES(3) with
+// restarts) for one system with an explicit right preconditioner z = M^-1 v.
+// Arnoldi by classical Gram-Schmidt with the dot products fused into the
+// SpMV; Givens rotations and the back substitution in one workgroup; each
+// cycle ends with x += M^-1 V y and the true residual r -= A M^-1 V y, whose
+// norm decides the restart.
+constexpr int GM_M = 20;
+struct GmScal {
+    KryScal ks;                     // ks.done stops the Arnoldi launches; thr, rmom, rnorm
+    KryScal ks_final;               // never done: the cycle's closing V-cycle
+    double H[(GM_M + 1) * GM_M];    // Hessenberg, column j at H[(GM_M + 1) * j], rotated in place
+    double cs[GM_M], sn[GM_M], g[GM_M + 1], y[GM_M];
+    double beta, hinv;
+    int j;                          // Arnoldi steps of this cycle
+    int iters;                      // steps over all cycles
+    int cycle_end;
+};
+struct GmSys {
+    double *V[GM_M + 1];
+    double *x, *r;
+    const double *ap;
+    float *z;
+    double *part;                   // (GM_M + 2) slots of nb partial sums
+    GmScal *sc;
+};
+
 __device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
 // Called by the kernel that starts a solve (its block 0 only writes the slot
@@ -154,16 +180,16 @@ template <int NV>
 __device__ __forceinline__ void sum_partials(const double *const *src, int nb, double (&out)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; k++) out[k] = 0.0;
-    for (int b = threadIdx.x; b < nb; b += BLOCK)
+    for (int b = threadIdx.x; b < nb; b += KBLOCK)
 #pragma unroll
         for (int k = 0; k < NV; k++) out[k] += src[k][b];
-    block_sum<NV>(out);
+    block_sum<NV, KBLOCK>(out);
 }
 
 // Block totals of v written as this block's partials.
 template <int NV>
 __device__ __forceinline__ void put_partials(double (&v)[NV], double *const *dst) {
-    block_sum<NV>(v);
+    block_sum<NV, KBLOCK>(v);
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) dst[k][blockIdx.x] = v[k];

@@ -28,10 +28,10 @@ __device__ __forceinline__ bool kdone(const KrySys<NS, T> &K) {
 
 // solve() prologue for u (n=2) and v (n=3), relax(2)=relax(3)=0.8
 // (solvers.f90:232-240), then the BiCGStab start (pbicgstab.f90:21-31).
-__global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K) {
+__global__ void __launch_bounds__(KBLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
     const Chunk ch = block_chunk(g.nact);
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
         const double a[4] = {s.aN[c], s.aE[c], s.aS[c], s.aW[c]};
         const int nb[4] = {c + g.nxp, c + 1, c - g.nxp, c - 1};
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(BLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K)
 // batch so the host can read the state; the next batch repeats the same
 // finish (same inputs, same result) before its first p update.
 template <class Op, int NS, class T = double>
-__global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it, int nb, int kmax, double rtol,
+__global__ void __launch_bounds__(KBLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it, int nb, int kmax, double rtol,
                                                   double atol, int fin_only) {
     double tot[2 * NS];
     const double *src[2 * NS];
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it,
     }
     if (fin_only || !any) return;
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int c = A.at(t).c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it,
 // MODE 1: v = A M^-1 p, partials r0.v.  MODE 2: t = A M^-1 s (s held in r),
 // partials t.s and t.t.  PREC 1: M^-1 p and M^-1 s are the explicit zp, zs.
 template <class Op, int NS, int MODE, int PREC = 0, class T = double>
-__global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
+__global__ void __launch_bounds__(KBLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
     static_assert(PREC == 0 || NS == 1, "explicit preconditioner: one system");
     if (kdone(K)) return;
     constexpr int NV = MODE == 1 ? NS : 2 * NS;
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
 #pragma unroll
     for (int q = 0; q < NS; q++) act[q] = !K.sc[q].done;
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const Idx x = A.at(t);
         const int c = x.c;
 #pragma unroll
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
 // alpha = rho / r0.v ; x += alpha M^-1 p ; s = r - alpha v (s overwrites r).
 // PREC 1 defers the x update to k_bicg_r.
 template <class Op, int NS, int PREC = 0, class T = double>
-__global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it, int nb) {
+__global__ void __launch_bounds__(KBLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[NS];
     const double *src[NS];
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it,
     }
     if (!any) return;
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int c = A.at(t).c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it,
 // omega = t.s / t.t ; x += omega M^-1 s ; r = s - omega t ; partials r.r, r0.r
 // PREC 1: x += alpha zp + omega zs.
 template <class Op, int NS, int PREC = 0, class T = double>
-__global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it, int nb) {
+__global__ void __launch_bounds__(KBLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it, int nb) {
     if (kdone(K)) return;
     double tot[2 * NS];
     const double *src[2 * NS];
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it,
 #pragma unroll
     for (int q = 0; q < 2 * NS; q++) red[q] = 0.0;
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int c = A.at(t).c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
@@ -272,13 +272,13 @@ __global__ void __launch_bounds__(BLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it,
 
 // solve() prologue on the reference's general cell list (solvers.f90:232-240)
 // and the BiCGStab start, for the drop-in cms2d_solve.
-__global__ void __launch_bounds__(BLOCK) k_ell_prologue(EllOp A, const int *iwet, double *ss, const double *sp,
+__global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwet, double *ss, const double *sp,
                                                         double *res, const double *phi, double relax,
                                                         KrySys<1> K) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(A.ncells);
     const double rinv = 1.0 / relax, relaxm = 1.0 - relax;
-    for (int c = ch.begin + threadIdx.x; c < ch.end; c += BLOCK) {
+    for (int c = ch.begin + threadIdx.x; c < ch.end; c += KBLOCK) {
         double asum = 0.0, anb = 0.0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -315,10 +315,10 @@ namespace cms {
 // the FP64 ||r|| meets the FP64 solve's own threshold.
 
 // Inner start from the FP64 residual: r = r0 = (float) r64, p = v = x = 0.
-__global__ void __launch_bounds__(BLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
+__global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int c = A.at(t).c;
         const float r = (float)r64[c];
         Ki.r[0][c] = r; Ki.r0[0][c] = r; Ki.p[0][c] = 0.0f; Ki.v[0][c] = 0.0f; Ki.x[0][c] = 0.0f;
@@ -331,10 +331,10 @@ __global__ void __launch_bounds__(BLOCK) k_ref_start(StencilOp32 A, KrySys<1, fl
 }
 
 // x += d, r -= A d (FP64 operator on the FP32 correction), partials ||r||^2.
-__global__ void __launch_bounds__(BLOCK) k_ref_update(StencilOp A, KrySys<1, double> K, const float *d) {
+__global__ void __launch_bounds__(KBLOCK) k_ref_update(StencilOp A, KrySys<1, double> K, const float *d) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += BLOCK) {
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const Idx x = A.at(t);
         const int c = x.c;
         K.x[0][c] = K.x[0][c] + (double)d[c];
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(BLOCK) k_ref_update(StencilOp A, KrySys<1, dou
 }
 
 // One block: the FP64 stop test of the outer sweep.
-__global__ void __launch_bounds__(BLOCK) k_ref_check(KrySys<1, double> K, int nb) {
+__global__ void __launch_bounds__(KBLOCK) k_ref_check(KrySys<1, double> K, int nb) {
     double tot[1];
     const double *src[1] = {K.red.pr[0]};
     sum_partials<1>(src, nb, tot);
@@ -356,6 +356,201 @@ __global__ void __launch_bounds__(BLOCK) k_ref_check(KrySys<1, double> K, int nb
         K.sc[0].rnorm = nr;
         if (nr <= K.sc[0].thr) K.sc[0].done = 1;
     }
+}
+
+}  // namespace cms
+
+namespace cms {
+
+// --- GMRES(m) -----------------------------------------------------------------
+
+// Start of a cycle (one block): beta = ||r|| from partials pr0 (pr1: res.res
+// of the solve prologue on the first cycle, for rmom).
+__global__ void __launch_bounds__(KBLOCK) k_gm_begin(GmSys G, const double *pr0, const double *pr1, int nb,
+                                                     int first, int n, double rtol, double atol) {
+    double tot[2];
+    const double *src[2] = {pr0, pr1 ? pr1 : pr0};
+    sum_partials<2>(src, nb, tot);
+    if (threadIdx.x != 0) return;
+    GmScal &s = *G.sc;
+    const double beta = sqrt(tot[0]);
+    if (first) {
+        s.ks.thr = fmax(rtol * beta, atol * sqrt((double)n));
+        s.ks.rmom = sqrt(tot[1] / (double)(float)n);
+        s.iters = 0;
+    }
+    s.beta = beta;
+    s.g[0] = beta;
+    s.j = 0;
+    s.cycle_end = 0;
+    s.ks.rnorm = beta;
+    s.ks.done = beta <= s.ks.thr ? 1 : 0;
+    s.ks.final_iters = s.iters;
+    s.ks_final.done = 0;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_v0(Op A, GmSys G) {
+    if (G.sc->ks.done) return;
+    const double bi = 1.0 / G.sc->beta;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const int c = A.at(t).c;
+        G.V[0][c] = G.r[c] * bi;
+    }
+}
+
+// w = A z into V[j+1]; partial sums of (w, V[i]), i <= j, in slot i.
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_spmv_dot(Op A, GmSys G, int j) {
+    if (G.sc->ks.done) return;
+    double acc[GM_M + 1];
+#pragma unroll
+    for (int i = 0; i <= GM_M; i++) acc[i] = 0.0;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const Idx x = A.at(t);
+        const double w = A.apply_z(G.z, G.ap, x);
+        G.V[j + 1][x.c] = w;
+#pragma unroll
+        for (int i = 0; i <= GM_M; i++)
+            if (i <= j) acc[i] += w * G.V[i][x.c];
+    }
+#pragma unroll
+    for (int i = 0; i <= GM_M; i++) {
+        if (i <= j) {                   // j is uniform: every thread takes the same branches
+            double v[1] = {acc[i]};
+            double *const dst[1] = {G.part + (size_t)i * gridDim.x};
+            put_partials<1>(v, dst);
+        }
+    }
+}
+
+// One block: column j of H from the partials.
+__global__ void __launch_bounds__(KBLOCK) k_gm_hred(GmSys G, int j, int nb) {
+    if (G.sc->ks.done) return;
+    for (int i = 0; i <= j; i++) {
+        double tot[1];
+        const double *src[1] = {G.part + (size_t)i * nb};
+        sum_partials<1>(src, nb, tot);
+        if (threadIdx.x == 0) G.sc->H[(GM_M + 1) * j + i] = tot[0];
+    }
+}
+
+// w -= sum_i H[i,j] V[i]; partial ||w||^2 in slot GM_M + 1.
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_orth(Op A, GmSys G, int j) {
+    if (G.sc->ks.done) return;
+    double h[GM_M + 1];
+#pragma unroll
+    for (int i = 0; i <= GM_M; i++) h[i] = i <= j ? G.sc->H[(GM_M + 1) * j + i] : 0.0;
+    double acc[1] = {0.0};
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const int c = A.at(t).c;
+        double w = G.V[j + 1][c];
+#pragma unroll
+        for (int i = 0; i <= GM_M; i++)
+            if (i <= j) w -= h[i] * G.V[i][c];
+        G.V[j + 1][c] = w;
+        acc[0] += w * w;
+    }
+    double *const dst[1] = {G.part + (size_t)(GM_M + 1) * gridDim.x};
+    put_partials<1>(acc, dst);
+}
+
+// One block: H[j+1,j] = ||w||, Givens rotations, residual estimate, stop test.
+__global__ void __launch_bounds__(KBLOCK) k_gm_finish(GmSys G, int j, int nb) {
+    if (G.sc->ks.done) return;
+    double tot[1];
+    const double *src[1] = {G.part + (size_t)(GM_M + 1) * nb};
+    sum_partials<1>(src, nb, tot);
+    if (threadIdx.x != 0) return;
+    GmScal &s = *G.sc;
+    double *Hc = s.H + (GM_M + 1) * j;
+    const double nw = sqrt(tot[0]);
+    Hc[j + 1] = nw;
+    for (int i = 0; i < j; i++) {
+        const double a = Hc[i], b = Hc[i + 1];
+        Hc[i] = s.cs[i] * a + s.sn[i] * b;
+        Hc[i + 1] = -s.sn[i] * a + s.cs[i] * b;
+    }
+    const double a = Hc[j], b = Hc[j + 1], rho = hypot(a, b);
+    const double c = rho > 0.0 ? a / rho : 1.0, sn = rho > 0.0 ? b / rho : 0.0;
+    Hc[j] = rho;
+    Hc[j + 1] = 0.0;
+    s.cs[j] = c;
+    s.sn[j] = sn;
+    s.g[j + 1] = -sn * s.g[j];
+    s.g[j] = c * s.g[j];
+    s.iters += 1;
+    s.j = j + 1;
+    s.hinv = nw > 0.0 ? 1.0 / nw : 0.0;
+    const double res = fabs(s.g[j + 1]);
+    s.ks.rnorm = res;
+    if (res <= s.ks.thr || j + 1 == GM_M || nw == 0.0) {
+        s.ks.done = 1;
+        s.cycle_end = 1;
+        s.ks.final_iters = s.iters;
+    }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_scale(Op A, GmSys G, int j) {
+    if (G.sc->ks.done) return;
+    const double hi = G.sc->hinv;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const int c = A.at(t).c;
+        G.V[j + 1][c] *= hi;
+    }
+}
+
+// One block: y = R^-1 g over the cycle's k steps.
+__global__ void k_gm_solve(GmSys G) {
+    if (threadIdx.x != 0) return;
+    GmScal &s = *G.sc;
+    const int k = s.j;
+    for (int i = k - 1; i >= 0; i--) {
+        double v = s.g[i];
+        for (int l = i + 1; l < k; l++) v -= s.H[(GM_M + 1) * l + i] * s.y[l];
+        const double d = s.H[(GM_M + 1) * i + i];
+        s.y[i] = d != 0.0 ? v / d : 0.0;
+    }
+}
+
+// u = sum_{i<k} y_i V[i] into V[k].
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_combine(Op A, GmSys G, int k) {
+    double y[GM_M];
+#pragma unroll
+    for (int i = 0; i < GM_M; i++) y[i] = i < k ? G.sc->y[i] : 0.0;
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const int c = A.at(t).c;
+        double u = 0.0;
+#pragma unroll
+        for (int i = 0; i < GM_M; i++)
+            if (i < k) u += y[i] * G.V[i][c];
+        G.V[k][c] = u;
+    }
+}
+
+// x += z, r -= A z (z = M^-1 V y); partials r.r in slot 0.
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_gm_update(Op A, GmSys G) {
+    double acc[1] = {0.0};
+    const Chunk ch = block_chunk(A.n());
+    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
+        const Idx x = A.at(t);
+        const int c = x.c;
+        G.x[c] = G.x[c] + (double)G.z[c];
+        const double r = G.r[c] - A.apply_z(G.z, G.ap, x);
+        G.r[c] = r;
+        acc[0] += r * r;
+    }
+    double *const dst[1] = {G.part};
+    put_partials<1>(acc, dst);
 }
 
 }  // namespace cms

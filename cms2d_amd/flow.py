@@ -25,7 +25,7 @@ class FlowStep:
     """Device-resident implicit flow model on a uniform Cartesian grid."""
 
     def __init__(self, case, *, rtol=1e-10, atol=1e-12, kmax=4000, maxit=0,
-                 precision=_lib.PREC_FP64, device=0, precond=None):
+                 precision=_lib.PREC_FP64, device=0, precond=None, solver=None):
         L = _lib.lib()
         self.case = case
         keep = []
@@ -53,6 +53,7 @@ class FlowStep:
         d.noptset, d.wallfric, d.ndsch, d.maxit = case.noptset, case.wallfric, case.ndsch, maxit
         d.rtol, d.atol, d.kmax, d.precision, d.device = rtol, atol, kmax, precision, device
         d.precond = _lib.PRECOND_MG if precond is None else precond
+        d.solver = _lib.SOLVER_BICGSTAB if solver is None else solver
         h = C.c_void_p()
         check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h

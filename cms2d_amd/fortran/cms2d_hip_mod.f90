@@ -22,6 +22,7 @@ module cms2d_hip
     integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1
     integer(c_int), parameter, public :: CMS2D_PREC_FP64 = 0
     integer(c_int), parameter, public :: CMS2D_PRECOND_MG = 0, CMS2D_PRECOND_JACOBI = 1
+    integer(c_int), parameter, public :: CMS2D_SOLVER_BICGSTAB = 0, CMS2D_SOLVER_GMRES = 1
 
     ! struct cms2d_flow_desc
     type, bind(C), public :: cms2d_flow_desc
@@ -35,7 +36,7 @@ module cms2d_hip
         integer(c_int) :: noptset = 2, wallfric = 1, ndsch = 4, maxit = 0
         real(c_double) :: rtol = 1.0d-10, atol = 1.0d-12
         integer(c_int) :: kmax = 4000, precision = 0, device = 0
-        integer(c_int) :: precond = 0
+        integer(c_int) :: precond = 0, solver = 0
     end type
 
     ! struct cms2d_step_info

@@ -14,6 +14,13 @@
  *                       with pbicgstab                 source/solvers/pbicgstab.f90:2-98
  *   cms2d_flow_create <- prestart/flow_init/geo_cart   source/main/prestart.f90:2-329,
  *                       source/flow/flow.F90:592-835, source/geospatial/geospatial.F90:2709-2828
+ *   cms2d_flow_get/_get_face/_set <- reads/writes of the flow_def module state
+ *                       (eta, u, v, p, h, iwet, flux, hk, visk; source/flow/flow_def.f90:32-106)
+ *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
+ *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
+ * Without a counterpart in the reference: cms2d_device_count (device probe),
+ * cms2d_flow_kernel_stats (live kernel timing for measurement) and
+ * cms2d_ensemble_step (independent domains side by side).
  */
 #ifndef CMS2D_AMD_H
 #define CMS2D_AMD_H
@@ -40,6 +47,10 @@ extern "C" {
  * systems are diagonally dominant and always use Jacobi. */
 #define CMS2D_PRECOND_MG 0        /* FP32 geometric multigrid V(1,1) cycle                    */
 #define CMS2D_PRECOND_JACOBI 1    /* diagonal                                                 */
+
+/* Krylov method of the pp system (the momentum systems use BiCGStab). */
+#define CMS2D_SOLVER_BICGSTAB 0   /* pbicgstab (pbicgstab.f90:2-98)                           */
+#define CMS2D_SOLVER_GMRES 1      /* restarted GMRES(20) (pgmres.f90:2-146 runs GMRES(3))     */
 
 /* Solver precision forms. */
 #define CMS2D_PREC_FP64 0         /* FP64 everywhere                                          */
@@ -70,6 +81,7 @@ typedef struct {
     int precision;                /* CMS2D_PREC_*                                            */
     int device;                   /* HIP device ordinal                                      */
     int precond;                  /* CMS2D_PRECOND_* for the pp system                       */
+    int solver;                   /* CMS2D_SOLVER_* for the pp system (GMRES: FP64, multigrid) */
 } cms2d_flow_desc;
 
 typedef struct {
@@ -103,7 +115,9 @@ int cms2d_flow_get_face(cms2d_flow *flow, const char *name, double *out);
 int cms2d_flow_set(cms2d_flow *flow, const char *name, const double *in);
 double cms2d_flow_time(const cms2d_flow *flow);
 /* Average device time per launch of the named kernel group over the steps run
- * so far ("spmv_pp": pp-system SpMV), and launches counted. */
+ * so far, counting the launches that did work, and their number.  Groups:
+ * "spmv_pp" (pp-system SpMV v = A z), "mg_down0" and "mg_up0" (multigrid
+ * level-0 down and up sweeps).  Measurement only: no reference counterpart. */
 int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
 
 /* Batched ensemble: advances each of nflows independent flow objects by

@@ -93,3 +93,17 @@ def test_gpu_matches_reference_vectors(gpu_available, name):
     w = compare_to_gold(name, lambda c: FlowStep(c))
     print(name, w)
     assert w["eta"] <= TOL_ETA and w["u"] <= TOL_VEL and w["v"] <= TOL_VEL, w
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,steps", [((48, 32), 40), ((201, 121), 8)])
+def test_gmres_matches_oracle(gpu_available, shape, steps):
+    """The pp system by restarted GMRES (pgmres) with the multigrid
+    preconditioner: same converged solutions as the oracle's BiCGStab."""
+    nx, ny = shape
+    c = cases.tidal(nx=nx, ny=ny, dx=20.0, seed=1 if nx < 100 else 4)
+    o, g = run_pair(c, steps, solver=_lib.SOLVER_GMRES)
+    err = compare(o, g)
+    info = g.step(0)
+    print("max err", err, info)
+    assert info["kry"][0] <= 20 * info["niter"], info
