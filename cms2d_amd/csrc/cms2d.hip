@@ -198,21 +198,20 @@ struct NoPrec {
 template <class Op, int NS, int PREC = 0, class T = double, class Pre = NoPrec>
 static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre()) {
     const dim3 grid(X.nblk), blk(KBLOCK);
-    // start: thresholds, rmom and the initial stop test
+    // start: thresholds, rmom and the initial stop test (read with the first
+    // batch: a solve that needs no iteration only costs no-op launches)
     hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
-    HIPCHK(hipMemcpyAsync(X.h_sc + sc0, X.d_sc + sc0, NS * sizeof(KryScal), hipMemcpyDeviceToHost, X.st));
-    HIPCHK(hipStreamSynchronize(X.st));
     auto all_done = [&]() {
         int d = 1;
         for (int q = 0; q < NS; q++) d &= X.h_sc[sc0 + q].done;
         return d;
     };
-    // batches: one short of the last solve's count, then 2, 2, 4, 8, ... so
-    // that few launches follow convergence (each costs a few microseconds
-    // even when it returns at once) and few host round trips are needed
+    // batches: the last solve's count, then 2, 2, 4, 8, ... so that few
+    // launches follow convergence (each costs a few microseconds even when it
+    // returns at once) and few host round trips are needed
     int it = 0, nbatch = 0, grow = 2;
-    int batch = std::max(1, X.prev_iters[sc0] - 1);
-    while (!all_done() && it < X.kmax) {
+    int batch = std::max(1, X.prev_iters[sc0]);
+    for (bool first = true; first || (!all_done() && it < X.kmax); first = false) {
         int nb = std::min(batch, X.kmax - it);
         for (int b = 0; b < nb; b++, it++) {
             if (X.tm) X.tm->it = it;

@@ -627,7 +627,7 @@ __global__ void __launch_bounds__(KBLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1>
         double r = anb + ss - ap * phi + SMALL;
         K.ap_w[0][c] = ap;
         K.dinv_w[0][c] = 1.0 / ap;
-        K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
+        K.r[0][c] = r; K.r0[0][c] = r;
         red[0] += r * r;
         red[1] += res * res;
     }

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(KBLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K
             (q ? s.resv : s.resu)[c] = res;
             K.ap_w[q][c] = ap;
             K.dinv_w[q][c] = 1.0 / ap;
-            K.r[q][c] = r; K.r0[q][c] = r; K.p[q][c] = 0.0; K.v[q][c] = 0.0;
+            K.r[q][c] = r; K.r0[q][c] = r;
             red[2 * q] += r * r;
             red[2 * q + 1] += res * res;
         }
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it
             if (!act[q]) continue;
             const T r = K.r[q][c];
             if (rst[q]) K.r0[q][c] = r;
-            K.p[q][c] = r + (T)beta[q] * (K.p[q][c] - (T)om[q] * K.v[q][c]);
+            // the first iteration starts from p = r (p and v of an earlier solve are not read)
+            K.p[q][c] = it == 0 ? r : r + (T)beta[q] * (K.p[q][c] - (T)om[q] * K.v[q][c]);
         }
     }
 }
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwe
         double r = anb + ssv - ap * ph + SMALL;
         K.ap_w[0][c] = ap;
         K.dinv_w[0][c] = 1.0 / ap;
-        K.r[0][c] = r; K.r0[0][c] = r; K.p[0][c] = 0.0; K.v[0][c] = 0.0;
+        K.r[0][c] = r; K.r0[0][c] = r;
         red[0] += r * r;
         red[1] += rs * rs;
     }
@@ -314,14 +315,14 @@ namespace cms {
 // vectors, multigrid preconditioner); x += d and r -= A d in FP64; stop when
 // the FP64 ||r|| meets the FP64 solve's own threshold.
 
-// Inner start from the FP64 residual: r = r0 = (float) r64, p = v = x = 0.
+// Inner start from the FP64 residual: r = r0 = (float) r64, x = 0 (p starts from r).
 __global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(A.n());
     for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
         const int c = A.at(t).c;
         const float r = (float)r64[c];
-        Ki.r[0][c] = r; Ki.r0[0][c] = r; Ki.p[0][c] = 0.0f; Ki.v[0][c] = 0.0f; Ki.x[0][c] = 0.0f;
+        Ki.r[0][c] = r; Ki.r0[0][c] = r; Ki.x[0][c] = 0.0f;
         red[0] += (double)r * (double)r;
     }
     red[1] = red[0];

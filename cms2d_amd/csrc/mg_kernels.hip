@@ -73,11 +73,12 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 
 // --- fused cycle: one launch down and one up per level ------------------------
 // A workgroup owns a tile of MT_X x MT_Y fine cells (MT_X/2 x MT_Y/2 coarse
-// cells, one per thread).  Intermediate values of the sweep are built in LDS
+// cells, two per thread).  Intermediate values of the sweep are built in LDS
 // over the tile and a halo; halo values are recomputed by the same expressions
 // as in the neighbouring tile, so they are bitwise the values stored there.
-constexpr int MT_X = 64, MT_Y = 16, MT_THREADS = (MT_X / 2) * (MT_Y / 2);
-static_assert(MT_THREADS == BLOCK, "one coarse cell per thread");
+// At 2000 x 1500 the 1504 tiles of level 0 fit the chip's workgroup slots in
+// one round.
+constexpr int MT_X = 64, MT_Y = 32, MT_THREADS = 256;
 
 __device__ __forceinline__ bool mg_inside(const MgLevel &L, int i, int j) {
     return i >= 0 && j >= 0 && i < L.nx && j < L.ny;
@@ -120,19 +121,21 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
         zt[ly][lx] = z;
     }
     __syncthreads();
-    const int cx = threadIdx.x % (MT_X / 2), cy = threadIdx.x / (MT_X / 2);
-    const int I = X0 / 2 + cx, J = Y0 / 2 + cy;
-    if (I >= C.nx || J >= C.ny) return;
-    const int i0 = 2 * I, j0 = 2 * J, lx = i0 - X0 + 1, ly = j0 - Y0 + 1;
-    const int c00 = f.idx(i0, j0);
-    float r = (f.aN[c00] * zt[ly + 1][lx] + f.aE[c00] * zt[ly][lx + 1]) +
-              (f.aS[c00] * zt[ly - 1][lx] + f.aW[c00] * zt[ly][lx - 1]);
-    if (i0 + 1 < f.nx && j0 + 1 < f.ny) {
-        const int c11 = f.idx(i0 + 1, j0 + 1);
-        r += (f.aN[c11] * zt[ly + 2][lx + 1] + f.aE[c11] * zt[ly + 1][lx + 2]) +
-             (f.aS[c11] * zt[ly][lx + 1] + f.aW[c11] * zt[ly + 1][lx]);
+    for (int t = threadIdx.x; t < (MT_X / 2) * (MT_Y / 2); t += MT_THREADS) {
+        const int cx = t % (MT_X / 2), cy = t / (MT_X / 2);
+        const int I = X0 / 2 + cx, J = Y0 / 2 + cy;
+        if (I >= C.nx || J >= C.ny) continue;
+        const int i0 = 2 * I, j0 = 2 * J, lx = i0 - X0 + 1, ly = j0 - Y0 + 1;
+        const int c00 = f.idx(i0, j0);
+        float r = (f.aN[c00] * zt[ly + 1][lx] + f.aE[c00] * zt[ly][lx + 1]) +
+                  (f.aS[c00] * zt[ly - 1][lx] + f.aW[c00] * zt[ly][lx - 1]);
+        if (i0 + 1 < f.nx && j0 + 1 < f.ny) {
+            const int c11 = f.idx(i0 + 1, j0 + 1);
+            r += (f.aN[c11] * zt[ly + 2][lx + 1] + f.aE[c11] * zt[ly + 1][lx + 2]) +
+                 (f.aS[c11] * zt[ly][lx + 1] + f.aW[c11] * zt[ly + 1][lx]);
+        }
+        C.b[C.idx(I, J)] = r;
     }
-    C.b[C.idx(I, J)] = r;
 }
 
 // Up: coarse correction and post-smoothing sweep (red, then black) of level

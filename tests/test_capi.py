@@ -48,3 +48,22 @@ def test_invalid_arguments_rejected():
     out = np.zeros(4)
     assert L.cms2d_solve(0, 0, out.ctypes.data_as(_lib.D), None, None, None, None, None, None,
                          1.0, 1e-10, 0.0, 10, None, None) == -1
+
+
+@pytest.mark.parametrize("field,value", [("precision", 7), ("precond", 9), ("solver", 4)])
+def test_unknown_options_rejected_before_device(field, value):
+    """Option codes are validated (CMS2D_ERR_ARG) before any device is touched."""
+    from cms2d_amd import _lib, cases
+    from cms2d_amd.flow import FlowStep
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(cases.lake_at_rest(), **{field: value})
+    assert e.value.code == -1
+
+
+def test_gmres_requires_multigrid_fp64():
+    from cms2d_amd import _lib, cases
+    from cms2d_amd.flow import FlowStep
+    for kw in ({"precond": _lib.PRECOND_JACOBI}, {"precision": _lib.PREC_FP32_REFINE}):
+        with pytest.raises(_lib.Cms2dError) as e:
+            FlowStep(cases.lake_at_rest(), solver=_lib.SOLVER_GMRES, **kw)
+        assert e.value.code == -1
