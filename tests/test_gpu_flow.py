@@ -106,4 +106,6 @@ def test_gmres_matches_oracle(gpu_available, shape, steps):
     err = compare(o, g)
     info = g.step(0)
     print("max err", err, info)
-    assert info["kry"][0] <= 20 * info["niter"], info
+    # one restart cycle (20 steps) or two per solve; a broken Arnoldi process
+    # would still converge through restarts, only far more slowly
+    assert info["kry"][0] <= 2 * 20 * info["niter"], info
