@@ -566,9 +566,8 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
 // Runs on the fixed Krylov grid and leaves r.r / res.res partials.
 __global__ void __launch_bounds__(KBLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1> K) {
     double red[2] = {0.0, 0.0};
-    const Chunk ch = block_chunk(g.nact);
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
+    grid_walk(g, [&](const Idx &x_) {
+        const int i = x_.i, j = x_.j, c = x_.c;
         double sp = s.sppp0[c];
         double fs = ((s.fluxN[c] + s.fluxE[c]) + (-s.fluxN[c - g.nxp])) + (-s.fluxE[c - 1]);
         double su = s.sspp0[c] - fs + s.p[c] * sp;
@@ -630,7 +629,7 @@ __global__ void __launch_bounds__(KBLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1>
         K.r[0][c] = r; K.r0[0][c] = r;
         red[0] += r * r;
         red[1] += res * res;
-    }
+    });
     double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
     put_partials<2>(red, dst);
     kry_reset(K.sc[0]);

@@ -14,7 +14,7 @@
 // coefficient is read once per sweep.
 //
 // Reductions without atomics: every Krylov kernel runs on the same fixed grid
-// of NB blocks (grid-stride over the cells).  A kernel that produces dot
+// of NB blocks (each block walks a fixed set of cells, grid_walk below).  A kernel that produces dot
 // products writes one partial per block; the kernel that consumes them sums
 // all NB partials in block order in every block (identical result everywhere),
 // and block 0 publishes the scalars for later kernels into slots indexed by
@@ -48,8 +48,8 @@ struct Idx {
     int c, i, j;
 };
 
-// Fixed-grid mapping: block b (XCD-remapped) owns a contiguous chunk of the
-// cell list; its threads stride through the chunk 256 cells at a time.
+// Fixed-grid mapping of a cell list (EllOp): block b (XCD-remapped) owns a
+// contiguous chunk of the list; its threads stride through the chunk.
 struct Chunk {
     int begin, end;
 };
@@ -62,6 +62,38 @@ __device__ __forceinline__ Chunk block_chunk(int n) {
     return ch;
 }
 
+// Cells of this block on the fixed Krylov grid of a structured grid: strips
+// KW_STRIP columns wide split into row bands, one (strip, band) tile per
+// block, consecutive blocks (vertically adjacent tiles, on one XCD) sharing
+// their halo rows in L2; a thread walks one column of its tile.  Every active
+// cell is visited once, whatever the grid size.
+constexpr int KW_STRIP = 256;
+template <class F>
+__device__ __forceinline__ void grid_walk(const Geo &g, F &&f) {
+    constexpr int RP = KBLOCK / KW_STRIP;                 // rows per pass
+    const int G = (int)gridDim.x;
+    const int sx = (g.nx + KW_STRIP - 1) / KW_STRIP;
+    const int bands = max(1, G / sx);
+    const int nbx = G / bands;                            // strips in parallel (>= 1)
+    const int b = xcd_block(blockIdx.x, G);
+    const int bx = b / bands, by = b % bands;
+    if (bx >= nbx) return;
+    const int R = (g.ny + bands - 1) / bands;
+    const int r0 = by * R, r1 = min(g.ny, r0 + R);
+    const int tx = (int)threadIdx.x % KW_STRIP, ty = (int)threadIdx.x / KW_STRIP;
+    for (int s = bx; s < sx; s += nbx) {
+        const int col = s * KW_STRIP + tx;
+        if (col >= g.nx) continue;
+        for (int row = r0 + ty; row < r1; row += RP) {
+            Idx x;
+            x.i = col;
+            x.j = row;
+            x.c = cidx(g, col, row);
+            f(x);
+        }
+    }
+}
+
 // Matrix operators: y_i = ap_i z_i - sum_k a_k(i) z_nb(k).  apply() is the
 // Jacobi-preconditioned form, z = M^-1 x = x/ap with 1/ap stored (dinv), so
 // y_i = x_i - sum_k a_k(i) x_nb dinv_nb; z = 0 on ghost cells.  C is the type
@@ -71,6 +103,8 @@ struct StencilOpT {
     Geo g;
     const C *aN, *aE, *aS, *aW;
     __device__ __forceinline__ int n() const { return g.nact; }
+    template <class F>
+    __device__ __forceinline__ void each(F &&f) const { grid_walk(g, f); }
     __device__ __forceinline__ Idx at(int t) const {
         Idx x;
         x.j = t / g.nx;
@@ -110,6 +144,11 @@ struct EllOp {
     const int *c2c;
     const double *a;
     __device__ __forceinline__ int n() const { return ncells; }
+    template <class F>
+    __device__ __forceinline__ void each(F &&f) const {
+        const Chunk ch = block_chunk(ncells);
+        for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) f(at(t));
+    }
     __device__ __forceinline__ Idx at(int t) const {
         Idx x;
         x.c = t;

@@ -30,9 +30,8 @@ __device__ __forceinline__ bool kdone(const KrySys<NS, T> &K) {
 // (solvers.f90:232-240), then the BiCGStab start (pbicgstab.f90:21-31).
 __global__ void __launch_bounds__(KBLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K) {
     double red[4] = {0.0, 0.0, 0.0, 0.0};
-    const Chunk ch = block_chunk(g.nact);
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int j = t / g.nx, i = t - j * g.nx, c = cidx(g, i, j);
+    grid_walk(g, [&](const Idx &x_) {
+        const int i = x_.i, j = x_.j, c = x_.c;
         const double a[4] = {s.aN[c], s.aE[c], s.aS[c], s.aW[c]};
         const int nb[4] = {c + g.nxp, c + 1, c - g.nxp, c - 1};
         const double asum = ((a[0] + a[1]) + a[2]) + a[3];
@@ -59,7 +58,7 @@ __global__ void __launch_bounds__(KBLOCK) k_uv_prologue(Geo g, St s, KrySys<2> K
             red[2 * q] += r * r;
             red[2 * q + 1] += res * res;
         }
-    }
+    });
     double *const dst[4] = {K.red.pr[0], K.red.pr[0] + gridDim.x, K.red.pr[1], K.red.pr[1] + gridDim.x};
     put_partials<4>(red, dst);
     kry_reset(K.sc[0]);
@@ -124,9 +123,8 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it
         any = any || act[q];
     }
     if (fin_only || !any) return;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
@@ -135,7 +133,7 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_p(Op A, KrySys<NS, T> K, int it
             // the first iteration starts from p = r (p and v of an earlier solve are not read)
             K.p[q][c] = it == 0 ? r : r + (T)beta[q] * (K.p[q][c] - (T)om[q] * K.v[q][c]);
         }
-    }
+    });
 }
 
 // MODE 1: v = A M^-1 p, partials r0.v.  MODE 2: t = A M^-1 s (s held in r),
@@ -151,9 +149,7 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
     bool act[NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) act[q] = !K.sc[q].done;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const Idx x = A.at(t);
+    A.each([&](const Idx &x) {
         const int c = x.c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
@@ -172,7 +168,7 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_spmv(Op A, KrySys<NS, T> K) {
                 red[2 * q + 1] += yd * yd;
             }
         }
-    }
+    });
     double *dst[NV];
 #pragma unroll
     for (int q = 0; q < NS; q++) {
@@ -212,16 +208,15 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_s(Op A, KrySys<NS, T> K, int it
         any = true;
     }
     if (!any) return;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
             if constexpr (PREC == 0) K.x[q][c] = K.x[q][c] + al[q] * (K.p[q][c] * K.dinv_w[q][c]);
             K.r[q][c] = K.r[q][c] - (T)al[q] * K.v[q][c];
         }
-    }
+    });
 }
 
 // omega = t.s / t.t ; x += omega M^-1 s ; r = s - omega t ; partials r.r, r0.r
@@ -248,9 +243,8 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it
     double red[2 * NS];
 #pragma unroll
     for (int q = 0; q < 2 * NS; q++) red[q] = 0.0;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
 #pragma unroll
         for (int q = 0; q < NS; q++) {
             if (!act[q]) continue;
@@ -264,7 +258,7 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it
             red[2 * q] += (double)r * (double)r;
             red[2 * q + 1] += (double)K.r0[q][c] * (double)r;
         }
-    }
+    });
     double *dst[2 * NS];
 #pragma unroll
     for (int q = 0; q < NS; q++) { dst[2 * q] = K.red.pr[q]; dst[2 * q + 1] = K.red.pr[q] + gridDim.x; }
@@ -318,13 +312,12 @@ namespace cms {
 // Inner start from the FP64 residual: r = r0 = (float) r64, x = 0 (p starts from r).
 __global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
     double red[2] = {0.0, 0.0};
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
         const float r = (float)r64[c];
         Ki.r[0][c] = r; Ki.r0[0][c] = r; Ki.x[0][c] = 0.0f;
         red[0] += (double)r * (double)r;
-    }
+    });
     red[1] = red[0];
     double *const dst[2] = {Ki.red.pr[0], Ki.red.pr[0] + gridDim.x};
     put_partials<2>(red, dst);
@@ -334,15 +327,13 @@ __global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, f
 // x += d, r -= A d (FP64 operator on the FP32 correction), partials ||r||^2.
 __global__ void __launch_bounds__(KBLOCK) k_ref_update(StencilOp A, KrySys<1, double> K, const float *d) {
     double red[2] = {0.0, 0.0};
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const Idx x = A.at(t);
+    A.each([&](const Idx &x) {
         const int c = x.c;
         K.x[0][c] = K.x[0][c] + (double)d[c];
         const double r = K.r[0][c] - A.apply_z(d, K.ap_w[0], x);
         K.r[0][c] = r;
         red[0] += r * r;
-    }
+    });
     double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
     put_partials<2>(red, dst);
 }
@@ -394,11 +385,10 @@ template <class Op>
 __global__ void __launch_bounds__(KBLOCK) k_gm_v0(Op A, GmSys G) {
     if (G.sc->ks.done) return;
     const double bi = 1.0 / G.sc->beta;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
         G.V[0][c] = G.r[c] * bi;
-    }
+    });
 }
 
 // w = A z into V[j+1]; partial sums of (w, V[i]), i <= j, in slot i.
@@ -408,15 +398,13 @@ __global__ void __launch_bounds__(KBLOCK) k_gm_spmv_dot(Op A, GmSys G, int j) {
     double acc[GM_M + 1];
 #pragma unroll
     for (int i = 0; i <= GM_M; i++) acc[i] = 0.0;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const Idx x = A.at(t);
+    A.each([&](const Idx &x) {
         const double w = A.apply_z(G.z, G.ap, x);
         G.V[j + 1][x.c] = w;
 #pragma unroll
         for (int i = 0; i <= GM_M; i++)
             if (i <= j) acc[i] += w * G.V[i][x.c];
-    }
+    });
 #pragma unroll
     for (int i = 0; i <= GM_M; i++) {
         if (i <= j) {                   // j is uniform: every thread takes the same branches
@@ -446,16 +434,15 @@ __global__ void __launch_bounds__(KBLOCK) k_gm_orth(Op A, GmSys G, int j) {
 #pragma unroll
     for (int i = 0; i <= GM_M; i++) h[i] = i <= j ? G.sc->H[(GM_M + 1) * j + i] : 0.0;
     double acc[1] = {0.0};
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
         double w = G.V[j + 1][c];
 #pragma unroll
         for (int i = 0; i <= GM_M; i++)
             if (i <= j) w -= h[i] * G.V[i][c];
         G.V[j + 1][c] = w;
         acc[0] += w * w;
-    }
+    });
     double *const dst[1] = {G.part + (size_t)(GM_M + 1) * gridDim.x};
     put_partials<1>(acc, dst);
 }
@@ -500,11 +487,10 @@ template <class Op>
 __global__ void __launch_bounds__(KBLOCK) k_gm_scale(Op A, GmSys G, int j) {
     if (G.sc->ks.done) return;
     const double hi = G.sc->hinv;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
         G.V[j + 1][c] *= hi;
-    }
+    });
 }
 
 // One block: y = R^-1 g over the cycle's k steps.
@@ -526,30 +512,27 @@ __global__ void __launch_bounds__(KBLOCK) k_gm_combine(Op A, GmSys G, int k) {
     double y[GM_M];
 #pragma unroll
     for (int i = 0; i < GM_M; i++) y[i] = i < k ? G.sc->y[i] : 0.0;
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const int c = A.at(t).c;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
         double u = 0.0;
 #pragma unroll
         for (int i = 0; i < GM_M; i++)
             if (i < k) u += y[i] * G.V[i][c];
         G.V[k][c] = u;
-    }
+    });
 }
 
 // x += z, r -= A z (z = M^-1 V y); partials r.r in slot 0.
 template <class Op>
 __global__ void __launch_bounds__(KBLOCK) k_gm_update(Op A, GmSys G) {
     double acc[1] = {0.0};
-    const Chunk ch = block_chunk(A.n());
-    for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) {
-        const Idx x = A.at(t);
+    A.each([&](const Idx &x) {
         const int c = x.c;
         G.x[c] = G.x[c] + (double)G.z[c];
         const double r = G.r[c] - A.apply_z(G.z, G.ap, x);
         G.r[c] = r;
         acc[0] += r * r;
-    }
+    });
     double *const dst[1] = {G.part};
     put_partials<1>(acc, dst);
 }

@@ -72,78 +72,123 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 }
 
 // --- fused cycle: one launch down and one up per level ------------------------
-// A workgroup owns a tile of MT_X x MT_Y fine cells (MT_X/2 x MT_Y/2 coarse
-// cells, two per thread).  Intermediate values of the sweep are built in LDS
-// over the tile and a halo; halo values are recomputed by the same expressions
-// as in the neighbouring tile, so they are bitwise the values stored there.
-// At 2000 x 1500 the 1504 tiles of level 0 fit the chip's workgroup slots in
-// one round.
-constexpr int MT_X = 64, MT_Y = 32, MT_THREADS = 256;
+// A workgroup owns a tile of MT_X x MT_Y fine cells; each thread owns two
+// coarse cells of it and their eight children, whose coefficients it loads
+// once into registers and keeps for both phases of the sweep.  Neighbour
+// values come from LDS over the tile and a halo; halo values are recomputed
+// by the same expressions as in the neighbouring tile (mg_black, mg_red), so
+// they are bitwise the values stored there.  At 2000 x 1500 the 1504 tiles of
+// level 0 fit the chip's workgroup slots in one round.
+constexpr int MT_X = 64, MT_Y = 32, MT_THREADS = 256, MT_OWN = 8;
+static_assert(MT_THREADS * 2 == (MT_X / 2) * (MT_Y / 2), "two coarse cells per thread");
 
 __device__ __forceinline__ bool mg_inside(const MgLevel &L, int i, int j) {
     return i >= 0 && j >= 0 && i < L.nx && j < L.ny;
 }
+// Fine cell k (0..7) of thread t in tile coordinates: coarse cells (cx, cy)
+// and (cx, cy + MT_Y/4), children (sub & 1, sub >> 1).
+__device__ __forceinline__ void mg_own(int t, int k, int &fx, int &fy) {
+    const int cx = t % (MT_X / 2), cy = t / (MT_X / 2) + (k >> 2) * (MT_Y / 4), sub = k & 3;
+    fx = 2 * cx + (sub & 1);
+    fy = 2 * cy + (sub >> 1);
+}
+// Cells of the one-cell ring around the tile, in tile coordinates.
+constexpr int MT_RING = 2 * (MT_X + 2) + 2 * MT_Y;
+__device__ __forceinline__ void mg_ring(int t, int &fx, int &fy) {
+    if (t < MT_X + 2) { fx = t - 1; fy = -1; }
+    else if (t < 2 * (MT_X + 2)) { fx = t - (MT_X + 2) - 1; fy = MT_Y; }
+    else { const int u = t - 2 * (MT_X + 2); fx = u < MT_Y ? -1 : MT_X; fy = u % MT_Y; }
+}
+
+// Black z of the sweep from zero: (b + sum a_k q_nb)/ap = q + (sum a_k q_nb)/ap,
+// q = b/ap (red z).
+__device__ __forceinline__ float mg_black(float aN, float aE, float aS, float aW, float qN, float qE, float qS,
+                                          float qW, float qc, float d) {
+    const float s = (aN * qN + aE * qE) + (aS * qS + aW * qW);
+    return qc + s * d;
+}
+// Red z after the coarse correction: (b + sum a_k w_nb)/ap, w = z + zc.
+__device__ __forceinline__ float mg_red(float aN, float aE, float aS, float aW, float wN, float wE, float wS,
+                                        float wW, float bc, float d) {
+    const float s = (aN * wN + aE * wE) + (aS * wS + aW * wW);
+    return (bc + s) * d;
+}
 
 // Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
-//   q  = b/ap over the tile + 2   (red z of the sweep from zero)
-//   zt = z over the tile + 1      (black z = (b + sum a_k q_nb)/ap)
-//   coarse b = sum over the red children of sum a_k zt_nb (their residual)
+//   q, 1/ap over the tile + 2; z over the tile + 1 in LDS; coarse b = sum over
+//   the red children of sum a_k z_nb (their residual after the sweep).
 template <class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
     if (sc->done) return;
-    __shared__ float q[MT_Y + 4][MT_X + 4];
+    __shared__ float q[MT_Y + 4][MT_X + 4], dv[MT_Y + 4][MT_X + 4];
     __shared__ float zt[MT_Y + 2][MT_X + 2];
     const int X0 = blockIdx.x * MT_X, Y0 = blockIdx.y * MT_Y;
     for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
         const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
-        float v = 0.0f;
+        float v = 0.0f, d = 0.0f;
         if (mg_inside(f, i, j)) {
             const int c = f.idx(i, j);
-            v = (float)b[c] * f.dinv[c];
+            d = f.dinv[c];
+            v = (float)b[c] * d;
         }
         q[ly][lx] = v;
+        dv[ly][lx] = d;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < (MT_X + 2) * (MT_Y + 2); t += MT_THREADS) {
-        const int ly = t / (MT_X + 2), lx = t - ly * (MT_X + 2), i = X0 - 1 + lx, j = Y0 - 1 + ly;
+    float a[MT_OWN][4];
+#pragma unroll
+    for (int k = 0; k < MT_OWN; k++) {
+        int fx, fy;
+        mg_own(threadIdx.x, k, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
+        a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
         if (mg_inside(f, i, j)) {
             const int c = f.idx(i, j);
-            if (((i + j) & 1) == 0) {
-                z = q[ly + 1][lx + 1];
-            } else {
-                const float s = (f.aN[c] * q[ly + 2][lx + 1] + f.aE[c] * q[ly + 1][lx + 2]) +
-                                (f.aS[c] * q[ly][lx + 1] + f.aW[c] * q[ly + 1][lx]);
-                z = ((float)b[c] + s) * f.dinv[c];
-            }
-            if (lx >= 1 && lx <= MT_X && ly >= 1 && ly <= MT_Y) f.z[c] = z;
+            a[k][0] = f.aN[c]; a[k][1] = f.aE[c]; a[k][2] = f.aS[c]; a[k][3] = f.aW[c];
+            if (((i + j) & 1) == 0) z = q[ly][lx];
+            else z = mg_black(a[k][0], a[k][1], a[k][2], a[k][3], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
+                              q[ly][lx - 1], q[ly][lx], dv[ly][lx]);
+            f.z[c] = z;
         }
-        zt[ly][lx] = z;
+        zt[fy + 1][fx + 1] = z;
+    }
+    for (int t = threadIdx.x; t < MT_RING; t += MT_THREADS) {
+        int fx, fy;
+        mg_ring(t, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
+        float z = 0.0f;
+        if (mg_inside(f, i, j)) {
+            if (((i + j) & 1) == 0) {
+                z = q[ly][lx];
+            } else {
+                const int c = f.idx(i, j);
+                z = mg_black(f.aN[c], f.aE[c], f.aS[c], f.aW[c], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
+                             q[ly][lx - 1], q[ly][lx], dv[ly][lx]);
+            }
+        }
+        zt[fy + 1][fx + 1] = z;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < (MT_X / 2) * (MT_Y / 2); t += MT_THREADS) {
-        const int cx = t % (MT_X / 2), cy = t / (MT_X / 2);
-        const int I = X0 / 2 + cx, J = Y0 / 2 + cy;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        int fx, fy;
+        mg_own(threadIdx.x, 4 * h, fx, fy);                 // child (0,0): red
+        const int I = (X0 + fx) / 2, J = (Y0 + fy) / 2;
         if (I >= C.nx || J >= C.ny) continue;
-        const int i0 = 2 * I, j0 = 2 * J, lx = i0 - X0 + 1, ly = j0 - Y0 + 1;
-        const int c00 = f.idx(i0, j0);
-        float r = (f.aN[c00] * zt[ly + 1][lx] + f.aE[c00] * zt[ly][lx + 1]) +
-                  (f.aS[c00] * zt[ly - 1][lx] + f.aW[c00] * zt[ly][lx - 1]);
-        if (i0 + 1 < f.nx && j0 + 1 < f.ny) {
-            const int c11 = f.idx(i0 + 1, j0 + 1);
-            r += (f.aN[c11] * zt[ly + 2][lx + 1] + f.aE[c11] * zt[ly + 1][lx + 2]) +
-                 (f.aS[c11] * zt[ly][lx + 1] + f.aW[c11] * zt[ly + 1][lx]);
-        }
+        const int x = fx + 1, y = fy + 1;                    // zt frame
+        const float *a0 = a[4 * h], *a3 = a[4 * h + 3];
+        float r = (a0[0] * zt[y + 1][x] + a0[1] * zt[y][x + 1]) + (a0[2] * zt[y - 1][x] + a0[3] * zt[y][x - 1]);
+        // child (1,1), red; its coefficients are zero outside the grid
+        r += (a3[0] * zt[y + 2][x + 1] + a3[1] * zt[y + 1][x + 2]) + (a3[2] * zt[y][x + 1] + a3[3] * zt[y + 1][x]);
         C.b[C.idx(I, J)] = r;
     }
 }
 
 // Up: coarse correction and post-smoothing sweep (red, then black) of level
-// L into L.zo.
-//   w  = z + zc(parent) over the tile + 2
-//   rt = new red z = (b + sum a_k w_nb)/ap over the tile + 1
-//   black zo = (b + sum a_k rt_nb)/ap on the tile
-// (the coarse correction of a red cell itself is replaced by its update).
+// L into L.zo.  w = z + zc(parent) over the tile + 2 and the new red z over
+// the tile + 1 in LDS; the coarse correction of a red cell itself is
+// replaced by its update.
 template <class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
     if (sc->done) return;
@@ -157,26 +202,49 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
         w[ly][lx] = v;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < (MT_X + 2) * (MT_Y + 2); t += MT_THREADS) {
-        const int ly = t / (MT_X + 2), lx = t - ly * (MT_X + 2), i = X0 - 1 + lx, j = Y0 - 1 + ly;
+    float a[MT_OWN][4], bb[MT_OWN], dd[MT_OWN];
+#pragma unroll
+    for (int k = 0; k < MT_OWN; k++) {
+        int fx, fy;
+        mg_own(threadIdx.x, k, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
+        float z = 0.0f;
+        a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
+        bb[k] = dd[k] = 0.0f;
+        if (mg_inside(L, i, j)) {
+            const int c = L.idx(i, j);
+            a[k][0] = L.aN[c]; a[k][1] = L.aE[c]; a[k][2] = L.aS[c]; a[k][3] = L.aW[c];
+            bb[k] = (float)b[c];
+            dd[k] = L.dinv[c];
+            if (((i + j) & 1) == 0) {
+                z = mg_red(a[k][0], a[k][1], a[k][2], a[k][3], w[ly + 1][lx], w[ly][lx + 1], w[ly - 1][lx],
+                           w[ly][lx - 1], bb[k], dd[k]);
+                L.zo[c] = z;
+            }
+        }
+        rt[fy + 1][fx + 1] = z;
+    }
+    for (int t = threadIdx.x; t < MT_RING; t += MT_THREADS) {
+        int fx, fy;
+        mg_ring(t, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
         if (mg_inside(L, i, j) && ((i + j) & 1) == 0) {
             const int c = L.idx(i, j);
-            const float s = (L.aN[c] * w[ly + 2][lx + 1] + L.aE[c] * w[ly + 1][lx + 2]) +
-                            (L.aS[c] * w[ly][lx + 1] + L.aW[c] * w[ly + 1][lx]);
-            z = ((float)b[c] + s) * L.dinv[c];
-            if (lx >= 1 && lx <= MT_X && ly >= 1 && ly <= MT_Y) L.zo[c] = z;
+            z = mg_red(L.aN[c], L.aE[c], L.aS[c], L.aW[c], w[ly + 1][lx], w[ly][lx + 1], w[ly - 1][lx],
+                       w[ly][lx - 1], (float)b[c], L.dinv[c]);
         }
-        rt[ly][lx] = z;
+        rt[fy + 1][fx + 1] = z;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < MT_X * MT_Y; t += MT_THREADS) {
-        const int ly = t / MT_X + 1, lx = t % MT_X + 1, i = X0 - 1 + lx, j = Y0 - 1 + ly;
+#pragma unroll
+    for (int k = 0; k < MT_OWN; k++) {
+        int fx, fy;
+        mg_own(threadIdx.x, k, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, x = fx + 1, y = fy + 1;
         if (!mg_inside(L, i, j) || ((i + j) & 1) == 0) continue;
-        const int c = L.idx(i, j);
-        const float s = (L.aN[c] * rt[ly + 1][lx] + L.aE[c] * rt[ly][lx + 1]) +
-                        (L.aS[c] * rt[ly - 1][lx] + L.aW[c] * rt[ly][lx - 1]);
-        L.zo[c] = ((float)b[c] + s) * L.dinv[c];
+        const float s = (a[k][0] * rt[y + 1][x] + a[k][1] * rt[y][x + 1]) + (a[k][2] * rt[y - 1][x] + a[k][3] * rt[y][x - 1]);
+        L.zo[L.idx(i, j)] = (bb[k] + s) * dd[k];
     }
 }
 
