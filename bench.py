@@ -32,6 +32,13 @@ HBM_PEAK_GBS = 8000.0
 #             z (FP32) and a quarter-size coarse b (FP32)   8 + 5*4 + 4 + 1 = 33
 #   mg_up0    k_mg_up on level 0: read b (FP64), aN..aW, 1/ap, z (FP32) and a
 #             quarter-size coarse z, write z (FP32)      8 + 6*4 + 1 + 4 = 37
+# HBM bytes per active cell of one launch as measured by rocprofv3 counters
+# (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a copy kernel of known
+# size) on this workload at 2000 x 1500; the commands and the table are in
+# profiles/traffic_tidal_2000x1500.md.
+TRAFFIC_PER_CELL = {"spmv_pp": (173.0 + 24.5) * 1e6 / 3.0e6,
+                    "mg_down0": (139.6 + 20.0) * 1e6 / 3.0e6,
+                    "mg_up0": (179.1 + 25.5) * 1e6 / 3.0e6}
 GROUPS = {
     "spmv_pp": (60, "k_bicg_spmv<StencilOp,1,1,1> (pp system, v = A z, z from the multigrid cycle)"),
     "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
@@ -53,6 +60,9 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=3)
     ap.add_argument("--cpu-sample", default="500x375x2",
                     help="oracle sample: NXxNYxSTEPS of the same workload definition")
+    ap.add_argument("--ref-sample", default="500x375x3",
+                    help="compiled-reference sample: NXxNYxSTEPS of the same workload definition")
+    ap.add_argument("--ref-threads", type=int, default=min(16, os.cpu_count() or 1))
     return ap.parse_args()
 
 
@@ -74,6 +84,46 @@ def cpu_baseline(sample):
             "sample": f"FP64 oracle (ILU0-BiCGStab, rtol 1e-10, one thread) on the tidal workload "
                       f"generated at {nx}x{ny} cells (dx 20 m, dt 30 s), {steps} steps after one "
                       f"untimed cold-start step, {dt:.1f} s"}
+
+
+def cpu_reference(sample, threads):
+    """Times the reference's own implicit step (oracle/_ref/ref_run: the
+    reference's Fortran sources compiled unmodified, single precision and its
+    own solver settings as shipped, OpenMP) on a bounded sample of the same
+    workload; the first step (cold start) is not counted."""
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import refcase  # writes the reference's ASCII inputs for a case
+    from cms2d_amd import cases
+    if not refcase.available():
+        return None
+    nx, ny, steps = (int(v) for v in sample.split("x"))
+    c = cases.tidal(nx=nx, ny=ny, dx=20.0, seed=1)
+    with tempfile.TemporaryDirectory() as d:
+        d = d + "/"
+        refcase.write_case(c, d, "tidal")
+        force = os.path.join(d, "forcing.bin")
+        np.concatenate([np.asarray(c.wavestrx, np.float64), np.asarray(c.wavestry, np.float64)]).tofile(force)
+        # the reference keeps work arrays on the stack (automatic arrays sized
+        # by the cell count): unlimited main stack, large OpenMP thread stacks
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="512M")
+
+        def stack():
+            import resource
+            resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+        r = subprocess.run([refcase.REF_RUN, d, "tidal", str(steps + 1), "-", force], cwd=d, env=env,
+                           capture_output=True, text=True, timeout=600, preexec_fn=stack)
+    t = [float(l.split()[2]) for l in r.stdout.splitlines() if l.startswith("step_seconds")]
+    if r.returncode != 0 or len(t) != steps + 1:
+        return None
+    el = sum(t[1:])
+    return {"value": nx * ny * steps / el, "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"the reference's flow_imp/flow_update (its Fortran compiled unmodified with flang -O2 "
+                      f"-fopenmp, single precision and BiCGStab-ILUT settings as shipped) on the tidal workload "
+                      f"generated at {nx}x{ny} cells, {steps} steps after one untimed cold-start step, "
+                      f"{threads} OpenMP threads, {el:.1f} s"}
 
 
 def fp32_form(a, case, g64, value64):
@@ -137,6 +187,12 @@ def main():
         sys.exit(2)
     from cms2d_amd import cases
     from cms2d_amd.flow import FlowStep
+    # CPU baselines first, before this process touches the GPU (the compiled
+    # reference runs as a child process)
+    cpu = port = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_reference(a.ref_sample, a.ref_threads)
+        port = cpu_baseline(a.cpu_sample)
     case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
     ncells = case.ncells
     g = FlowStep(case)
@@ -196,7 +252,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": TRAFFIC_PER_CELL[dom] * ncells,
             "avg_launch_ms": D["avg_launch_ms"],
             "launches": D["launches"],
             "bytes_per_launch": D["bytes_per_launch"],
@@ -210,7 +266,8 @@ def main():
     if not a.no_ensemble:
         out["ensemble"] = ensemble_form(a)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_sample)
+        out["cpu_baseline"] = cpu if cpu is not None else port
+        out["cpu_baseline_port"] = port
     print(json.dumps(out), flush=True)
 
 

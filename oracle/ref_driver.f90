@@ -6,7 +6,7 @@
 ! oracle/build_ref.py against the reference's objects (compiled unmodified,
 ! single precision as shipped) into oracle/_ref/ref_run.
 !
-! Usage: ref_run <case dir/> <case name> <nsteps> <output file> [forcing file]
+! Usage: ref_run <case dir/> <case name> <nsteps> <output file|-> [forcing file]
 !   reads <case dir>/<case name>.cmcards (and the files it names) with the
 !   reference's card readers, initialises the model with the calls prestart
 !   makes (source/main/prestart.f90:2-329) except those of the output module
@@ -17,7 +17,9 @@
 !   The optional forcing file (float64 stream, 2*ncells values) replaces the
 !   radiation-stress arrays wavestrx, wavestry after initialisation; it is
 !   used with the CONSTANT_WAVE_HEIGHT card, which turns the wave branches on
-!   with zero wave height (source/waves/wave.F90:506-540).
+!   with zero wave height (source/waves/wave.F90:506-540).  With output file
+!   "-" nothing is written.  Prints the wall time of every step (timing,
+!   flow_imp, flow_update; output excluded) as "step_seconds <n> <t>".
 !=============================================================================
 program ref_run
     use comvarbl,  only: ctlfile, flowpath, casename
@@ -30,6 +32,8 @@ program ref_run
     implicit none
     character(len=400) :: dir, name, arg, outfile, forcefile
     integer :: nsteps, n, i, ou, fu
+    integer(8) :: t0, t1, rate
+    logical :: dump
     real(8), allocatable :: buf(:)
     interface
       subroutine model_init()
@@ -72,13 +76,18 @@ program ref_run
       deallocate(buf)
     endif
 
-    open(newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
+    dump = trim(outfile) /= '-'
+    if (dump) open(newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
     allocate(buf(ncells))
     do n = 1, nsteps
+      call system_clock(t0, rate)
       call timing
       if (constant_waves) call wave_const_update
       call flow_imp
       call flow_update
+      call system_clock(t1)
+      write(*, '(a,i0,1x,es14.6)') 'step_seconds ', n, real(t1 - t0, 8) / real(rate, 8)
+      if (.not. dump) cycle
       write(ou) real(ncells, 8)
       do i = 1, ncells
         buf(i) = real(eta(i), 8)
@@ -97,7 +106,7 @@ program ref_run
       enddo
       write(ou) buf
     enddo
-    close(ou)
+    if (dump) close(ou)
 end program ref_run
 
 !-----------------------------------------------------------------------------
