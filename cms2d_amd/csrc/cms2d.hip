@@ -268,12 +268,10 @@ static void mg_vcycle(cms2d_flow *F, const TB *b0, float *z0, const KryScal *sc)
     const dim3 tb(MT_THREADS);
     tlaunch(tm, TG_MG_DOWN0, k_mg_down<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
     for (int l = 1; l + 1 < M.nlev; l++)
-        hipLaunchKernelGGL(k_mg_down<float>, tiles(M.L[l]), tb, 0, F->st, M.L[l], (const float *)M.L[l].b,
-                           M.L[l + 1], sc);
+        LAUNCH(k_mg_down_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
     hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(MG_CT), lds, F->st, Lc, (const float *)Lc.b, sc);
     for (int l = M.nlev - 2; l >= 1; l--)
-        hipLaunchKernelGGL(k_mg_up<float>, tiles(M.L[l]), tb, 0, F->st, M.L[l], (const float *)M.L[l].b,
-                           M.L[l + 1], sc);
+        LAUNCH(k_mg_up_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
     tlaunch(tm, TG_MG_UP0, k_mg_up<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
 }
 

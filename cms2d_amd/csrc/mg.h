@@ -8,7 +8,8 @@
 //   * smoother: one red-black Gauss-Seidel sweep before (red first, from a zero
 //     guess) and after (red first) the coarse correction;
 //   * per level two launches (down: sweep + restriction; up: correction +
-//     sweep), one thread per coarse cell and its four children;
+//     sweep): LDS tiles on level 0, one thread per coarse cell and its four
+//     children on the coarser levels;
 //   * restriction: sum of the children's residuals (the equations are cell
 //     integrals); prolongation: the coarse value injected into its children;
 //   * coarse operators re-discretised from the fine coefficients: the

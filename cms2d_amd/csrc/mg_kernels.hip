@@ -1,7 +1,7 @@
 // mg_kernels.hip -- the V(1,1) cycle of mg.h.
 //
-// Per level above the coarsest, one cycle is two launches over tiles of the
-// level (LDS holds the sweep's intermediate values over a halo):
+// Per level above the coarsest, one cycle is two launches (over LDS tiles on
+// level 0, one thread per coarse cell on the coarser levels):
 //   k_mg_down  red-black sweep from z = 0 (red z = b/ap, black
 //              z = (b + sum a_k b_nb/ap_nb)/ap) and the coarse right-hand
 //              side: the sum over the red children of their residual, which
@@ -79,7 +79,10 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 // by the same expressions as in the neighbouring tile (mg_black, mg_red), so
 // they are bitwise the values stored there.  At 2000 x 1500 the 1504 tiles of
 // level 0 fit the chip's workgroup slots in one round.
-constexpr int MT_X = 64, MT_Y = 32, MT_THREADS = 256, MT_OWN = 8;
+#ifndef CMS2D_MG_TILE_X
+#define CMS2D_MG_TILE_X 128
+#endif
+constexpr int MT_X = CMS2D_MG_TILE_X, MT_Y = 2048 / CMS2D_MG_TILE_X, MT_THREADS = 256, MT_OWN = 8;
 static_assert(MT_THREADS * 2 == (MT_X / 2) * (MT_Y / 2), "two coarse cells per thread");
 
 __device__ __forceinline__ bool mg_inside(const MgLevel &L, int i, int j) {
@@ -245,6 +248,97 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
         if (!mg_inside(L, i, j) || ((i + j) & 1) == 0) continue;
         const float s = (a[k][0] * rt[y + 1][x] + a[k][1] * rt[y][x + 1]) + (a[k][2] * rt[y - 1][x] + a[k][3] * rt[y][x - 1]);
         L.zo[L.idx(i, j)] = (bb[k] + s) * dd[k];
+    }
+}
+
+// --- the same cycle on coarse levels: one thread per coarse cell ----------------
+// On coarse levels a tile kernel is latency-bound (few tiles, three dependent
+// phases); here every thread issues all its loads at once and recomputes the
+// values it needs from neighbouring children by the same expressions
+// (mg_black0, mg_red1), so they are bitwise the values stored by their owners.
+
+// Black z after the red-black sweep from zero (red z = b/ap), or 0 outside
+// the grid.
+template <class TB>
+__device__ __forceinline__ float mg_black0(const MgLevel &L, const TB *b, int i, int j) {
+    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
+    const int c = L.idx(i, j);
+    const int cN = c + L.nxp, cE = c + 1, cS = c - L.nxp, cW = c - 1;
+    const float s = (L.aN[c] * ((float)b[cN] * L.dinv[cN]) + L.aE[c] * ((float)b[cE] * L.dinv[cE])) +
+                    (L.aS[c] * ((float)b[cS] * L.dinv[cS]) + L.aW[c] * ((float)b[cW] * L.dinv[cW]));
+    return ((float)b[c] + s) * L.dinv[c];
+}
+
+// Residual of red cell (i,j) after that sweep: sum a_k z_nb (b - ap z = 0).
+template <class TB>
+__device__ __forceinline__ float mg_red_res(const MgLevel &L, const TB *b, int i, int j, float zN, float zE,
+                                            float zS, float zW) {
+    const int c = L.idx(i, j);
+    return (L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW);
+}
+
+// Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
+    if (sc->done) return;
+    int I, J, cc;
+    if (!mg_thread(C, I, J, cc)) return;
+    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
+    const bool hi = i1 < f.nx, hj = j1 < f.ny;
+    // black children (i1,j0), (i0,j1) and the black neighbours of the red ones
+    const float zb10 = mg_black0(f, b, i1, j0), zb01 = mg_black0(f, b, i0, j1);
+    const float zbW = mg_black0(f, b, i0 - 1, j0), zbS = mg_black0(f, b, i0, j0 - 1);
+    const int c00 = f.idx(i0, j0);
+    f.z[c00] = (float)b[c00] * f.dinv[c00];
+    float r = mg_red_res(f, b, i0, j0, zb01, zb10, zbS, zbW);
+    if (hi) f.z[f.idx(i1, j0)] = zb10;
+    if (hj) f.z[f.idx(i0, j1)] = zb01;
+    if (hi && hj) {
+        const int c11 = f.idx(i1, j1);
+        f.z[c11] = (float)b[c11] * f.dinv[c11];
+        const float zbE = mg_black0(f, b, i1 + 1, j1), zbN = mg_black0(f, b, i1, j1 + 1);
+        r += mg_red_res(f, b, i1, j1, zbN, zbE, zb10, zb01);
+    }
+    C.b[cc] = r;
+}
+
+// Red z after the coarse correction and the red half-sweep, or 0 outside.
+// zd: z of the down sweep, zc: coarse correction (on C).
+template <class TB>
+__device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const MgLevel &C, int i, int j) {
+    if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
+    const int c = L.idx(i, j);
+    const int pN = C.idx(i >> 1, min(j + 1, L.ny - 1) >> 1), pS = C.idx(i >> 1, max(j - 1, 0) >> 1);
+    const int pE = C.idx(min(i + 1, L.nx - 1) >> 1, j >> 1), pW = C.idx(max(i - 1, 0) >> 1, j >> 1);
+    const float s = (L.aN[c] * (L.z[c + L.nxp] + C.zo[pN]) + L.aE[c] * (L.z[c + 1] + C.zo[pE])) +
+                    (L.aS[c] * (L.z[c - L.nxp] + C.zo[pS]) + L.aW[c] * (L.z[c - 1] + C.zo[pW]));
+    return ((float)b[c] + s) * L.dinv[c];
+}
+
+// Up: coarse correction and post-smoothing sweep (red, then black) of level
+// L into L.zo.
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_mg_up_pc(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
+    if (sc->done) return;
+    int I, J, cc;
+    if (!mg_thread(C, I, J, cc)) return;
+    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
+    const bool hi = i1 < L.nx, hj = j1 < L.ny;
+    const float r00 = mg_red1(L, b, C, i0, j0);
+    const float r11 = (hi && hj) ? mg_red1(L, b, C, i1, j1) : 0.0f;
+    L.zo[L.idx(i0, j0)] = r00;
+    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
+    if (hi) {
+        // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
+        const int c = L.idx(i1, j0);
+        const float zN = r11, zE = mg_red1(L, b, C, i1 + 1, j0), zS = mg_red1(L, b, C, i1, j0 - 1), zW = r00;
+        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    }
+    if (hj) {
+        // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
+        const int c = L.idx(i0, j1);
+        const float zN = mg_red1(L, b, C, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, i0 - 1, j1);
+        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
     }
 }
 
