@@ -670,7 +670,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                 if (int rc = dalloc(F, &L.ap, n)) return rc;
             }
             M.nlev++;
-            if ((long)nx * ny <= MG_COARSE_MAX) break;
+            if ((long)nx * ny <= MG_COARSE_MAX && (long)(nx + 2) * (ny + 2) <= MG_COARSE_LDS) break;
             if (M.nlev == MG_MAX_LEVELS) return set_err(CMS2D_ERR_ARG, "grid too large for the multigrid hierarchy");
             nx = (nx + 1) / 2;
             ny = (ny + 1) / 2;

@@ -4,7 +4,7 @@
 // The reference preconditions its Krylov solvers with ILUT (ilutp.f90,
 // solvers.f90:185-199).  Here M^-1 is one V(1,1) cycle:
 //   * cell-centred 2x2 coarsening, levels until the grid has at most
-//     MG_COARSE_MAX cells;
+//     MG_COARSE_MAX cells and its padded layout fits MG_COARSE_LDS floats;
 //   * smoother: one red-black Gauss-Seidel sweep before (red first, from a zero
 //     guess) and after (red first) the coarse correction;
 //   * per level two launches (down: sweep + restriction; up: correction +
@@ -32,8 +32,15 @@
 namespace cms {
 
 constexpr int MG_MAX_LEVELS = 16;
-constexpr int MG_COARSE_MAX = 4096;
-constexpr int MG_COARSE_SWEEPS = 6;
+#ifndef CMS2D_MG_COARSE_MAX
+#define CMS2D_MG_COARSE_MAX 4096
+#endif
+#ifndef CMS2D_MG_COARSE_SWEEPS
+#define CMS2D_MG_COARSE_SWEEPS 6
+#endif
+constexpr int MG_COARSE_MAX = CMS2D_MG_COARSE_MAX;   // cells of the coarsest grid (a multiple of 1024)
+constexpr int MG_COARSE_LDS = 16384;                 // its padded size in floats (64 KB of LDS)
+constexpr int MG_COARSE_SWEEPS = CMS2D_MG_COARSE_SWEEPS;
 
 struct MgLevel {
     int nx, ny, nxp;

@@ -344,7 +344,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up_pc(MgLevel L, const TB *b, MgLe
 
 // Coarsest grid: MG_COARSE_SWEEPS red-black sweeps from z = 0 in one block
 // with z in LDS (padded layout of the level, ghost ring zero) and each
-// thread's cells in registers; result to L.zo.
+// thread's cells (t = thread + k * MG_CT) in registers; result to L.zo.  The
+// cell coordinates are stepped, not stored, to keep the registers for the
+// coefficients.
 constexpr int MG_CT = 1024;
 constexpr int MG_CPT = MG_COARSE_MAX / MG_CT;   // cells per thread
 template <class TB>
@@ -352,40 +354,46 @@ __global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, c
     if (sc->done) return;
     extern __shared__ float zs[];
     const int n = L.nx * L.ny, npad = L.nxp * (L.ny + 2);
+    const int j0 = (int)threadIdx.x / L.nx, i0 = (int)threadIdx.x - j0 * L.nx;
+    const int dj = MG_CT / L.nx, di_ = MG_CT - dj * L.nx;
     for (int q = threadIdx.x; q < npad; q += MG_CT) zs[q] = 0.0f;
     float aN[MG_CPT], aE[MG_CPT], aS[MG_CPT], aW[MG_CPT], di[MG_CPT], bb[MG_CPT];
-    int cix[MG_CPT], col[MG_CPT];
+    {
+        int i = i0, j = j0;
 #pragma unroll
-    for (int k = 0; k < MG_CPT; k++) {
-        const int t = threadIdx.x + k * MG_CT;
-        cix[k] = -1;
-        col[k] = -1;
-        aN[k] = aE[k] = aS[k] = aW[k] = di[k] = bb[k] = 0.0f;
-        if (t < n) {
-            const int j = t / L.nx, i = t - j * L.nx, c = L.idx(i, j);
-            cix[k] = c;
-            col[k] = (i + j) & 1;
-            aN[k] = L.aN[c]; aE[k] = L.aE[c]; aS[k] = L.aS[c]; aW[k] = L.aW[c];
-            di[k] = L.dinv[c];
-            bb[k] = (float)b[c];
+        for (int k = 0; k < MG_CPT; k++) {
+            aN[k] = aE[k] = aS[k] = aW[k] = di[k] = bb[k] = 0.0f;
+            if ((int)threadIdx.x + k * MG_CT < n) {
+                const int c = L.idx(i, j);
+                aN[k] = L.aN[c]; aE[k] = L.aE[c]; aS[k] = L.aS[c]; aW[k] = L.aW[c];
+                di[k] = L.dinv[c];
+                bb[k] = (float)b[c];
+            }
+            i += di_; j += dj;
+            if (i >= L.nx) { i -= L.nx; j++; }
         }
     }
     __syncthreads();
     for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
         for (int cl = 0; cl < 2; cl++) {
+            int i = i0, j = j0;
 #pragma unroll
             for (int k = 0; k < MG_CPT; k++) {
-                if (col[k] != cl) continue;
-                const int c = cix[k];
-                const float s = (aN[k] * zs[c + L.nxp] + aE[k] * zs[c + 1]) + (aS[k] * zs[c - L.nxp] + aW[k] * zs[c - 1]);
-                zs[c] = (bb[k] + s) * di[k];
+                if ((int)threadIdx.x + k * MG_CT < n && ((i + j) & 1) == cl) {
+                    const int c = L.idx(i, j);
+                    const float s = (aN[k] * zs[c + L.nxp] + aE[k] * zs[c + 1]) + (aS[k] * zs[c - L.nxp] + aW[k] * zs[c - 1]);
+                    zs[c] = (bb[k] + s) * di[k];
+                }
+                i += di_; j += dj;
+                if (i >= L.nx) { i -= L.nx; j++; }
             }
             __syncthreads();
         }
     }
-#pragma unroll
-    for (int k = 0; k < MG_CPT; k++)
-        if (cix[k] >= 0) L.zo[cix[k]] = zs[cix[k]];
+    for (int t = threadIdx.x; t < n; t += MG_CT) {
+        const int j = t / L.nx, i = t - j * L.nx, c = L.idx(i, j);
+        L.zo[c] = zs[c];
+    }
 }
 
 }  // namespace cms

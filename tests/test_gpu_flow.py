@@ -74,9 +74,9 @@ def test_tidal_wetdry_radstress_matches_oracle(gpu_available, precond):
 
 @pytest.mark.gpu
 def test_tidal_multigrid_levels_matches_oracle(gpu_available):
-    """Odd sizes through three multigrid levels (201x121 -> 101x61 -> 51x31)."""
-    c = cases.tidal(nx=201, ny=121, dx=20.0, seed=4)
-    o, g = run_pair(c, 8)
+    """Odd sizes through four multigrid levels (403x243 -> 202x122 -> 101x61 -> 51x31)."""
+    c = cases.tidal(nx=403, ny=243, dx=20.0, seed=4)
+    o, g = run_pair(c, 6)
     err = compare(o, g)
     info = g.step(0)                    # counters of the last step
     print("max err", err, info)
