@@ -44,13 +44,14 @@ class FlowDesc(C.Structure):
         ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
-        ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int),
+        ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int), ("dtvar", C.c_int),
     ]
 
 
 class StepInfo(C.Structure):
     _fields_ = [("niter", C.c_int), ("iconv", C.c_int), ("rmom", C.c_double * 3),
-                ("kry_iters", C.c_int * 3), ("nwet", C.c_int), ("kernel_ms", C.c_double)]
+                ("kry_iters", C.c_int * 3), ("nwet", C.c_int), ("kernel_ms", C.c_double),
+                ("dt", C.c_double)]
 
 
 _LIB = None

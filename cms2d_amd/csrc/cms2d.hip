@@ -104,7 +104,9 @@ struct cms2d_flow {
     int npad = 0, nblk_act = 0, nblk_pad = 0, nblk_kry = 0;
     int ntc = 0;
     double amp[8]{}, speed[8]{}, phase[8]{};
-    double timesecs = 0.0, rampdur = 0.0;
+    double timesecs = 0.0, rampdur = 0.0, ramp = 0.0;
+    int dtvar = 0, jtime = 0, mtime = 0;       // variable time step (flow.F90:107, 664-666)
+    double dtimebeg = 0.0, deltime = 0.0;
     int ntime = 0, maxit = 0, niter = 0, iconv = 1;
     double rmom[3]{}, rmom0[3]{}, rmom1[3]{};
     int kry[3]{}, prev_iters[4]{4, 2, 2, 4};
@@ -402,11 +404,14 @@ static void check_conv(cms2d_flow *F) {
     const double RP = F4(0.5), RUV = 1.0;
     if ((rp >= RP * 10.0 || ruv >= RUV * 10.0) && rmom[0] > F4(1.0e-4)) { F->iconv = 0; return; }
     if (niter == maxit && (rp >= RP || ruv >= RUV) && rmom[0] > F4(1.0e-3)) { F->iconv = 0; return; }
-    if (niter == maxit && (rp >= RP / 5.0 || ruv >= RUV / 5.0) && rmom[0] > F4(1.0e-4)) { F->iconv = 0; return; }
+    if (niter == maxit && (rp >= RP / 5.0 || ruv >= RUV / 5.0) && rmom[0] > F4(1.0e-4)) {
+        F->iconv = F->dtvar ? 3 : 0;                 // 3: reduce the next time step
+        return;
+    }
     double rmomuv = std::max(rmom[1], rmom[2]);
     if (rmom[0] <= F4(1.0e-8) && rmomuv <= F4(1.0e-7)) { F->iconv = 2; return; }
     if (niter == maxit && (rmom[0] > F4(1.0e-3) || rmomuv > F4(1.0e-2))) { F->iconv = 0; return; }
-    if (niter == maxit && (rmom[0] > F4(1.0e-4) || rmomuv > F4(1.0e-3))) { F->iconv = 0; return; }
+    if (niter == maxit && (rmom[0] > F4(1.0e-4) || rmomuv > F4(1.0e-3))) { F->iconv = F->dtvar ? 3 : 0; return; }
     if (niter <= nhalf || niter % 2 != 0) return;
     double rchp = r1[0] - rmom[0];
     double rchuv = std::max(r1[1], r1[2]) - rmomuv;
@@ -422,12 +427,41 @@ static int eddyvis_tail(cms2d_flow *F) {
     return 0;
 }
 
-// flow_imp, flow_imp.F90:29-312 (fixed time step)
+// flow_recalculate, flow_imp.F90:1133-1242: back to the previous time level
+// with half the time step (the ramp keeps the value timing gave it).
+static int flow_recalculate(cms2d_flow *F) {
+    F->mtime = 1;
+    F->jtime += 1;
+    F->timesecs = std::max(0.0, F->timesecs - F->deltime);
+    F->deltime = F->dtimebeg / std::ldexp(1.0, F->jtime);
+    F->timesecs += F->deltime;
+    F->P.dtime = F->deltime;
+    if (F->P.dtime <= F->dtimebeg / 256.0)
+        return set_err(CMS2D_ERR_DIVERGED, "Model is divergent (time step reduced below 1/256 of its start value)");
+    LAUNCH(k_recalc, F->nblk_pad, F->g, F->s);
+    if (int rc = wetdry(F)) return rc;
+    // bndflux: flux and cross-shore strings only, none on these grids
+    LAUNCH(k_grad_fric_vis, F->nblk_act, F->g, F->s, F->P);
+    return eddyvis_tail(F);
+}
+
+static int flow_imp_once(cms2d_flow *F);
+
+// flow_imp, flow_imp.F90:29-312; with dtvar a divergent outer iteration
+// recalculates the step with half the time step (:157-161, 262-272)
 static int flow_imp(cms2d_flow *F) {
+    for (;;) {
+        const int rc = flow_imp_once(F);
+        if (rc != CMS2D_ERR_DIVERGED || !F->dtvar) return rc;
+        if (int r2 = flow_recalculate(F)) return r2;
+    }
+}
+
+static int flow_imp_once(cms2d_flow *F) {
     Geo &g = F->g;
     St &s = F->s;
     const double timehrs = F->timesecs / 3600.0;
-    const double ramp = ramp_func(timehrs, F->rampdur);
+    const double ramp = F->ramp;
     double w = 0.0;
     for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
     const double wsebnd = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);     // bnd_eval, boundaries.F90:4109-4122
@@ -544,6 +578,29 @@ extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
     return CMS2D_OK;
 }
 
+// Initial state from u, v, p as flow_init and prestart build it
+// (flow.F90:688-707, prestart.f90:309-329): wet/dry, previous time level,
+// gradients and face values, fluxes (flow_convflux), friction, eddy viscosity.
+static int init_state(cms2d_flow *F) {
+    const Geo &g = F->g;
+    St &s = F->s;
+    const size_t N = F->npad;
+    if (int rc = wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.u1, s.u, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.v1, s.v, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.p1, s.p, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.h1, s.h, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    if (int rc = wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.iwet1, s.iwet, N, hipMemcpyDeviceToDevice, F->st));
+    LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, F->P);
+    LAUNCH(k_convflux, F->nblk_act, g, s);
+    HIPCHK(hipMemcpyAsync(s.fluxE1, s.fluxE, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.fluxN1, s.fluxN, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    if (int rc = eddyvis_tail(F)) return rc;
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
 static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
         d->ntc > 8 || !(d->dt > 0))
@@ -593,6 +650,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     P.lmax = std::max(g.nx, g.ny);
     F->maxit = d->maxit > 0 ? d->maxit : 20 + g.nact / 10000;   // flow.F90:784
     F->rampdur = std::max(d->rampdur, d->dt / 3600.0);
+    F->dtvar = d->dtvar != 0;
+    F->dtimebeg = F->deltime = d->dt;
     F->ntc = d->ntc;
     for (int k = 0; k < d->ntc; k++) { F->amp[k] = d->amp[k]; F->speed[k] = d->speed[k]; F->phase[k] = d->phase[k]; }
     for (int n = 0; n < 3; n++) { F->rmom0[n] = 1e10; F->rmom[n] = 0.0; }
@@ -605,7 +664,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                           &s.spv, &s.ssu0, &s.ssv0, &s.spuv0, &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu,
                           &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
                           &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
-                          &s.tu, &s.tv, &s.tvis};
+                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1};
     for (double **f : dfields)
         if (int rc = dalloc(F, f, N)) return rc;
     if (int rc = dalloc(F, &s.wseadj, 4 * (size_t)P.lmax)) return rc;
@@ -712,20 +771,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (int rc = upload_padded(F, s.coefman, d->mann, false)) return rc;
     if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
-    // flow_init (flow.F90:688-707) and prestart (prestart.f90:309-329)
-    if (int rc = wetdry(F)) return rc;
-    HIPCHK(hipMemcpyAsync(s.u1, s.u, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
-    HIPCHK(hipMemcpyAsync(s.v1, s.v, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
-    HIPCHK(hipMemcpyAsync(s.p1, s.p, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
-    HIPCHK(hipMemcpyAsync(s.h1, s.h, N * sizeof(double), hipMemcpyDeviceToDevice, F->st));
-    if (int rc = wetdry(F)) return rc;
-    HIPCHK(hipMemcpyAsync(s.iwet1, s.iwet, N, hipMemcpyDeviceToDevice, F->st));
     LAUNCH(k_zbk, F->nblk_act, g, s);
-    LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, P);
-    // flow_convflux with u = v = 0 leaves every flux at zero
-    if (int rc = eddyvis_tail(F)) return rc;
-    HIPCHK(hipStreamSynchronize(F->st));
-    return 0;
+    return init_state(F);
 }
 
 extern "C" int cms2d_flow_create(const cms2d_flow_desc *d, cms2d_flow **out) {
@@ -749,9 +796,33 @@ extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info)
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipEventRecord(F->step_ev[0], F->st));
     for (int n = 0; n < nsteps; n++) {
-        // timing, CMS_Flow.f90:109-240 (fixed dt)
+        // timing, CMS_Flow.f90:109-240; with dtvar the time step is halved
+        // after a step that ended with iconv = 3 and doubled back once the pp
+        // residual is small and the time falls on the coarser step (:180-205)
         F->ntime++;
-        F->timesecs += F->P.dtime;
+        if (F->dtvar) {
+            if (F->iconv == 3 && F->dtimebeg / F->deltime <= 10.0) {
+                F->mtime = 0;
+                F->jtime += 1;
+                F->deltime = F->dtimebeg / std::ldexp(1.0, F->jtime);
+            } else if (F->jtime >= 1) {
+                if ((F->mtime >= 2 && F->rmom[0] < F4(1.0e-4) / 100.0) ||
+                    (F->mtime >= 4 && F->rmom[0] < F4(1.0e-4) / 10.0)) {
+                    const double dtt = F->dtimebeg / std::ldexp(1.0, F->jtime - 1);
+                    const double rtime = std::fmod(F->timesecs + dtt, dtt);
+                    const double err = std::fabs(rtime) / dtt;
+                    if (rtime < F4(0.01) || err < F4(0.01)) {
+                        F->deltime = dtt;
+                        F->mtime = 0;
+                        F->jtime -= 1;
+                    }
+                }
+            }
+            F->mtime += 1;
+            F->P.dtime = F->deltime;
+        }
+        F->timesecs += F->deltime;
+        F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
         if (int rc = flow_imp(F)) return rc;
         LAUNCH(k_update, F->nblk_pad, F->g, F->s);
     }
@@ -764,6 +835,7 @@ extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info)
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, F->step_ev[0], F->step_ev[1]);
         info->kernel_ms = ms;
+        info->dt = F->deltime;
         std::vector<uint8_t> w(F->npad);
         HIPCHK(hipMemcpy(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost));
         int nw = 0;
@@ -849,7 +921,8 @@ extern "C" int cms2d_flow_set(cms2d_flow *F, const char *name, const double *in)
     for (int j = 0; j < g.ny; j++)
         for (int i = 0; i < g.nx; i++) h[(size_t)(j + 1) * g.nxp + i + 1] = in[(size_t)j * g.nx + i];
     HIPCHK(hipMemcpy(p, h.data(), F->npad * sizeof(double), hipMemcpyHostToDevice));
-    return CMS2D_OK;
+    // an initial condition: rebuild the state as at the start of a run
+    return init_state(F);
 }
 
 extern "C" int cms2d_flow_get_face(cms2d_flow *F, const char *name, double *out) {

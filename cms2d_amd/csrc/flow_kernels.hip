@@ -550,12 +550,30 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
     }
     __syncthreads();
     if (threadIdx.x == 0 && adjust) {
-        // sequential prefix in the reference's order, carried in a register
+        // sequential prefix in the reference's order, carried in a register;
+        // terms are read 16 at a time so the LDS loads overlap
+        constexpr int U = 16;
         term[im0] = 0.0;
         double acc = 0.0;
-        for (int im = im0 + 1; im < n; im++) { acc = acc + term[im]; term[im] = acc; }
+        int im = im0 + 1;
+        for (; im + U <= n; im += U) {
+            double v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) v[k] = term[im + k];
+#pragma unroll
+            for (int k = 0; k < U; k++) { acc = acc + v[k]; term[im + k] = acc; }
+        }
+        for (; im < n; im++) { acc = acc + term[im]; term[im] = acc; }
         acc = 0.0;
-        for (int im = im0 - 1; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
+        im = im0 - 1;
+        for (; im - U + 1 >= 0; im -= U) {
+            double v[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) v[k] = term[im - k];
+#pragma unroll
+            for (int k = 0; k < U; k++) { acc = acc + v[k]; term[im - k] = acc; }
+        }
+        for (; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
     }
     __syncthreads();
     for (int im = threadIdx.x; im < n; im += blockDim.x) wse[im] = adjust ? wsebnd + term[im] : wsebnd;
@@ -891,7 +909,34 @@ __global__ void k_update(Geo g, St s) {
     if (t >= g.nxp * g.nyp) return;
     s.iwet1[t] = s.iwet[t];
     s.u1[t] = s.u[t]; s.v1[t] = s.v[t]; s.p1[t] = s.p[t]; s.h1[t] = s.h[t];
+    s.fluxE1[t] = s.fluxE[t]; s.fluxN1[t] = s.fluxN[t];
     s.pp[t] = 0.0; s.dppx[t] = 0.0; s.dppy[t] = 0.0;
+}
+
+// flow_recalculate (flow_imp.F90:1161-1173): back to the previous time level.
+__global__ void k_recalc(Geo g, St s) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    s.u[t] = s.u1[t]; s.v[t] = s.v1[t]; s.p[t] = s.p1[t]; s.h[t] = s.h1[t];
+    s.pp[t] = 0.0; s.dppx[t] = 0.0; s.dppy[t] = 0.0;
+    s.iwet[t] = s.iwet1[t];
+    s.fluxE[t] = s.fluxE1[t]; s.fluxN[t] = s.fluxN1[t];
+}
+
+// flow_convflux (flow.F90:1879-1910): face fluxes from cell velocities,
+// fintp-weighted normal velocity times hk*ds, zero unless both sides are wet.
+__device__ __forceinline__ double conv_face(const Geo &g, const St &s, int c, int nb, int k, double hk) {
+    const double uni = g.fnx[k] * s.u[c] + g.fny[k] * s.v[c];
+    const double unk = g.fnx[k] * s.u[nb] + g.fny[k] * s.v[nb];
+    return (double)(s.iwet[c] * s.iwet[nb]) * g.ds[k] * hk * (g.fintp[k] * unk + (1.0 - g.fintp[k]) * uni);
+}
+__global__ void k_convflux(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    s.fluxN[c] = conv_face(g, s, c, c + g.nxp, KN, s.hkN[c]);
+    s.fluxE[c] = conv_face(g, s, c, c + 1, KE, s.hkE[c]);
+    if (j == 0) s.fluxN[c - g.nxp] = -conv_face(g, s, c, c - g.nxp, KS, s.hkN[c - g.nxp]);
+    if (i == 0) s.fluxE[c - 1] = -conv_face(g, s, c, c - 1, KW, s.hkE[c - 1]);
 }
 
 // interp_scal_cell2face(zb, ibc=0) on owned faces (prestart.f90:320)

@@ -18,6 +18,7 @@ struct St {
     double *tu, *tv, *tvis;
     double *wseadj;            // [4*lmax] adjusted boundary water level per side
     uint8_t *iwet, *iwet1, *icorner, *ipond;
+    double *fluxE1, *fluxN1;   // face fluxes of the previous time level (flux1)
     int *plab;                 // ponding component labels (padded index of the root, -1 dry)
     int *flags;                // [0] narrow-channel change, [1] ponding change, [2] newly wet cells
 };

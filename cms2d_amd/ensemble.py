@@ -23,7 +23,7 @@ class Ensemble:
     def step(self, n=1):
         infos = (_lib.StepInfo * len(self.members))()
         check(_lib.lib().cms2d_ensemble_step(self._handles, len(self.members), n, self.nthreads, infos))
-        return [{"niter": i.niter, "iconv": i.iconv, "kry": list(i.kry_iters), "nwet": i.nwet}
+        return [{"niter": i.niter, "iconv": i.iconv, "kry": list(i.kry_iters), "nwet": i.nwet, "dt": i.dt}
                 for i in infos]
 
     def close(self):

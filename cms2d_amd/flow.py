@@ -25,7 +25,7 @@ class FlowStep:
     """Device-resident implicit flow model on a uniform Cartesian grid."""
 
     def __init__(self, case, *, rtol=1e-10, atol=1e-12, kmax=4000, maxit=0,
-                 precision=_lib.PREC_FP64, device=0, precond=None, solver=None):
+                 precision=_lib.PREC_FP64, device=0, precond=None, solver=None, dtvar=0):
         L = _lib.lib()
         self.case = case
         keep = []
@@ -54,6 +54,7 @@ class FlowStep:
         d.rtol, d.atol, d.kmax, d.precision, d.device = rtol, atol, kmax, precision, device
         d.precond = _lib.PRECOND_MG if precond is None else precond
         d.solver = _lib.SOLVER_BICGSTAB if solver is None else solver
+        d.dtvar = dtvar
         h = C.c_void_p()
         check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -74,7 +75,8 @@ class FlowStep:
         info = _lib.StepInfo()
         check(_lib.lib().cms2d_flow_step(self._h, n, C.byref(info)))
         return {"niter": info.niter, "iconv": info.iconv, "rmom": list(info.rmom),
-                "kry": list(info.kry_iters), "nwet": info.nwet, "kernel_ms": info.kernel_ms}
+                "kry": list(info.kry_iters), "nwet": info.nwet, "kernel_ms": info.kernel_ms,
+                "dt": info.dt}
 
     def get(self, name):
         out = np.empty(self.ncells)

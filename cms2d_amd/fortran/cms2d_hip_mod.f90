@@ -36,7 +36,7 @@ module cms2d_hip
         integer(c_int) :: noptset = 2, wallfric = 1, ndsch = 4, maxit = 0
         real(c_double) :: rtol = 1.0d-10, atol = 1.0d-12
         integer(c_int) :: kmax = 4000, precision = 0, device = 0
-        integer(c_int) :: precond = 0, solver = 0
+        integer(c_int) :: precond = 0, solver = 0, dtvar = 0
     end type
 
     ! struct cms2d_step_info
@@ -45,7 +45,7 @@ module cms2d_hip
         real(c_double) :: rmom(3) = 0
         integer(c_int) :: kry_iters(3) = 0
         integer(c_int) :: nwet = 0
-        real(c_double) :: kernel_ms = 0
+        real(c_double) :: kernel_ms = 0, dt = 0
     end type
 
     interface

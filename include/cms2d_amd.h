@@ -82,6 +82,10 @@ typedef struct {
     int device;                   /* HIP device ordinal                                      */
     int precond;                  /* CMS2D_PRECOND_* for the pp system                       */
     int solver;                   /* CMS2D_SOLVER_* for the pp system (GMRES: FP64, multigrid) */
+    int dtvar;                    /* VARIABLE_TIMESTEP: a divergent step is redone with half the
+                                     time step (flow_recalculate, flow_imp.F90:1133-1242), dt
+                                     recovers in timing (CMS_Flow.f90:180-205); 0: divergence
+                                     returns CMS2D_ERR_DIVERGED                                   */
 } cms2d_flow_desc;
 
 typedef struct {
@@ -91,6 +95,7 @@ typedef struct {
     int kry_iters[3];             /* Krylov iterations over the last step: pp, u, v          */
     int nwet;                     /* wet active cells                                        */
     double kernel_ms;             /* device time of the steps run by this call [ms]          */
+    double dt;                    /* time step of the last step [s] (varies with dtvar)      */
 } cms2d_step_info;
 
 typedef struct cms2d_flow cms2d_flow;
@@ -110,8 +115,10 @@ int cms2d_flow_get(cms2d_flow *flow, const char *name, double *out);
 /* Directed face field of every active cell, out[4*c+k], k = N,E,S,W:
  * "flux","hk","visk". */
 int cms2d_flow_get_face(cms2d_flow *flow, const char *name, double *out);
-/* Overwrites an active-cell field ("u","v","p"); ghost values follow at the
- * next step. */
+/* Initial condition: overwrites an active-cell field ("u","v","p") and
+ * rebuilds the state from it as a run starts (wet/dry, previous time level,
+ * face fluxes by flow_convflux, friction, eddy viscosity: prestart.f90:309-329,
+ * flow.F90:1879-1910), as a hot start does. */
 int cms2d_flow_set(cms2d_flow *flow, const char *name, const double *in);
 double cms2d_flow_time(const cms2d_flow *flow);
 /* Average device time per launch of the named kernel group over the steps run

@@ -87,6 +87,8 @@ struct cms_oracle {
     gradop *gow;
     /* control (comvarbl) */
     double dtime, timesecs, ctime, timehrs, ramp, rampdur, hmin, hdry, hdry1, hdry2;
+    int dtvar, jtime, mtime;          /* variable time step (flow.F90:107, 664-666) */
+    double dtimebeg, deltime;
     int maxit, niter, iconv, ntime;
     double rmom[3], rmom0[3], rmom1[3];
     int kry[3];
@@ -1047,11 +1049,14 @@ static void check_conv(cms_oracle *o)
     double ruv = fmax(rmom[1] / r0[1], rmom[2] / r0[2]);
     if ((rp >= RMOMRATIOP * 10.0 || ruv >= RMOMRATIOUV * 10.0) && rmom[0] > RMOMTARGETP) { o->iconv = 0; return; }
     if (niter == o->maxit && (rp >= RMOMRATIOP || ruv >= RMOMRATIOUV) && rmom[0] > RMOMMAXP) { o->iconv = 0; return; }
-    if (niter == o->maxit && (rp >= RMOMRATIOP / 5.0 || ruv >= RMOMRATIOUV / 5.0) && rmom[0] > RMOMTARGETP) { o->iconv = 0; return; }
+    if (niter == o->maxit && (rp >= RMOMRATIOP / 5.0 || ruv >= RMOMRATIOUV / 5.0) && rmom[0] > RMOMTARGETP) {
+        o->iconv = o->dtvar ? 3 : 0;                 /* 3: reduce the next time step */
+        return;
+    }
     double rmomuv = fmax(rmom[1], rmom[2]);
     if (rmom[0] <= RMOMMINP && rmomuv <= RMOMMINUV) { o->iconv = 2; return; }
     if (niter == o->maxit && (rmom[0] > RMOMMAXP || rmomuv > RMOMMAXUV)) { o->iconv = 0; return; }
-    if (niter == o->maxit && (rmom[0] > RMOMTARGETP || rmomuv > RMOMTARGETUV)) { o->iconv = 0; return; }
+    if (niter == o->maxit && (rmom[0] > RMOMTARGETP || rmomuv > RMOMTARGETUV)) { o->iconv = o->dtvar ? 3 : 0; return; }
     if (niter <= nhalf || niter % 2 != 0) return;
     double rchp = r1[0] - rmom[0];
     double rchuv = fmax(r1[1], r1[2]) - rmomuv;
@@ -1085,9 +1090,41 @@ static void velface_eta(cms_oracle *o)
     }
 }
 
-/* flow_imp, flow_imp.F90:29-312, fixed time step (VARIABLE_TIMESTEP OFF) */
+/* flow_recalculate, flow_imp.F90:1133-1242: back to the previous time level
+ * with half the time step (the ramp keeps the value timing gave it).
+ * Returns -1 when the step would fall below dtimebeg/2^8 (divergent). */
+static int flow_recalculate(cms_oracle *o)
+{
+    o->mtime = 1;
+    o->jtime += 1;
+    o->timesecs = fmax(0.0, o->timesecs - o->deltime);
+    o->deltime = o->dtimebeg / ldexp(1.0, o->jtime);
+    o->timesecs += o->deltime;
+    o->dtime = o->deltime;
+    o->ctime = o->timesecs;
+    o->timehrs = o->ctime / 3600.0;
+    if (o->dtime <= o->dtimebeg / 256.0) return -1;
+    const int nd = o->ncellsD;
+    for (int i = 0; i < nd; i++) {
+        o->u[i] = o->u1[i]; o->v[i] = o->v1[i]; o->p[i] = o->p1[i]; o->h[i] = o->h1[i];
+        o->pp[i] = 0.0; o->dppx[i] = 0.0; o->dppy[i] = 0.0;
+        o->iwet[i] = o->iwet1[i];
+    }
+    memcpy(o->flux, o->flux1, sizeof(double) * 4 * nd);
+    flow_wetdry(o);
+    /* bndflux: flux and cross-shore strings only; none here */
+    der_grad_cbgg(o, o->gow, 1);
+    flow_grad_interp(o);
+    fric_eval(o);
+    flow_eddyvis(o);
+    return 0;
+}
+
+/* flow_imp, flow_imp.F90:29-312; with dtvar a divergent outer iteration
+ * recalculates the step with half the time step (:262-272) */
 static int flow_imp(cms_oracle *o)
 {
+restart:
     bnd_eval(o);
     flow_wetdry(o);
     der_grad_cbgg(o, o->gow, 1);                 /* der_update: weights follow iwet */
@@ -1106,7 +1143,10 @@ static int flow_imp(cms_oracle *o)
         fric_eval(o);
         flow_eddyvis(o);
         check_conv(o);
-        if (o->iconv == 0) return -1;
+        if (o->iconv == 0) {
+            if (!o->dtvar || flow_recalculate(o) != 0) return -1;
+            goto restart;
+        }
         if (o->iconv == 2) break;
     }
     if (o->niter > o->maxit) o->niter = o->maxit;
@@ -1149,6 +1189,8 @@ static void flow_convflux(cms_oracle *o)
         }
 }
 
+static void init_state(cms_oracle *o);
+
 cms_oracle *cms_oracle_create(const cms_case *c)
 {
     cms_oracle *o = (cms_oracle *)xcalloc(1, sizeof(cms_oracle));
@@ -1179,6 +1221,11 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     build_bnd(o);
     /* flow_init (flow.F90:592-835) */
     o->dtime = c->dt;
+    o->dtvar = c->dtvar;
+    o->dtimebeg = c->dt;
+    o->deltime = c->dt;
+    o->jtime = 0;
+    o->mtime = 0;
     o->hdry = c->hdry > 0 ? c->hdry : F4(0.05);
     o->hmin = o->hdry;
     o->hdry1 = F4(5.0) * o->hdry;
@@ -1189,6 +1236,17 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     for (int n = 0; n < 3; n++) { o->rmom0[n] = 1.e10; o->rmom[n] = 0.0; }
     if (o->c.rtol <= 0) o->c.rtol = 1e-10;
     if (o->c.kmax <= 0) o->c.kmax = 2000;
+    interp_cell2face(o, o->zb, 0, o->zbk);       /* zbk: every face of every active cell */
+    init_state(o);
+    return o;
+}
+
+/* Initial state from u, v, p as flow_init and prestart build it
+ * (flow.F90:688-707, prestart.f90:309-329): wet/dry, previous time level,
+ * gradients, face values, fluxes (flow_convflux), friction, eddy viscosity. */
+static void init_state(cms_oracle *o)
+{
+    const int nd = o->ncellsD;
     flow_wetdry(o);
     memcpy(o->u1, o->u, sizeof(double) * nd); memcpy(o->v1, o->v, sizeof(double) * nd);
     memcpy(o->p1, o->p, sizeof(double) * nd); memcpy(o->h1, o->h, sizeof(double) * nd);
@@ -1197,14 +1255,12 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     flow_wetdry(o);
     memcpy(o->iwet1, o->iwet, sizeof(int) * nd);
     for (int i = 0; i < nd; i++) o->eta[i] = o->iwet[i] * o->p[i] * (1.0 / GRAV) - 999.0 * (1 - o->iwet[i]);
-    /* zbk from interp_scal_cell2face(zb,0,...) covers every face of every active cell */
-    interp_cell2face(o, o->zb, 0, o->zbk);
     der_grad_cbgg(o, o->gow, 1);
     flow_grad_interp(o);
     flow_convflux(o);
+    memcpy(o->flux1, o->flux, sizeof(double) * 4 * nd);
     fric_eval(o);
     flow_eddyvis(o);
-    return o;
 }
 
 void cms_oracle_destroy(cms_oracle *o)
@@ -1230,9 +1286,34 @@ double cms_oracle_time(const cms_oracle *o) { return o->timesecs; }
 int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info)
 {
     for (int s = 0; s < nsteps; s++) {
-        /* timing, CMS_Flow.f90:109-240 (fixed dt) */
+        /* timing, CMS_Flow.f90:109-240; with dtvar the time step is halved
+         * after a step that ended with iconv = 3 and doubled back once the
+         * pp residual is small and the time falls on the coarser step
+         * (:180-205) */
         o->ntime++;
-        o->timesecs += o->dtime;
+        if (o->dtvar) {
+            if (o->iconv == 3 && o->dtimebeg / o->deltime <= 10.0) {
+                o->mtime = 0;
+                o->jtime += 1;
+                o->deltime = o->dtimebeg / ldexp(1.0, o->jtime);
+                o->dtime = o->deltime;
+            } else if (o->jtime >= 1) {
+                if ((o->mtime >= 2 && o->rmom[0] < RMOMTARGETP / 100.0) ||
+                    (o->mtime >= 4 && o->rmom[0] < RMOMTARGETP / 10.0)) {
+                    double dtt = o->dtimebeg / ldexp(1.0, o->jtime - 1);
+                    double rtime = fmod(o->timesecs + dtt, dtt);
+                    double err = fabs(rtime) / dtt;
+                    if (rtime < F4(0.01) || err < F4(0.01)) {
+                        o->deltime = dtt;
+                        o->dtime = o->deltime;
+                        o->mtime = 0;
+                        o->jtime -= 1;
+                    }
+                }
+            }
+            o->mtime += 1;
+        }
+        o->timesecs += o->deltime;
         o->ctime = o->timesecs;
         o->timehrs = o->ctime / 3600.0;
         o->ramp = ramp_func(o->timehrs, o->rampdur);
@@ -1282,11 +1363,14 @@ int cms_oracle_get(const cms_oracle *o, const char *name, double *out)
     return 0;
 }
 
+/* Initial condition ("u", "v" or "p" of the active cells): the state is then
+ * rebuilt as at the start of a run, so both time levels agree. */
 int cms_oracle_set(cms_oracle *o, const char *name, const double *in)
 {
+    if (strcmp(name, "u") && strcmp(name, "v") && strcmp(name, "p")) return -1;
     double *f = (double *)field(o, name);
-    if (!f) return -1;
     memcpy(f, in, sizeof(double) * o->ncells);
+    init_state(o);
     return 0;
 }
 

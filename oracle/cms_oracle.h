@@ -46,6 +46,8 @@ typedef struct {
     int serial;               /* 1: reference serial loop order for the three loops whose
                                  OpenMP form races; 0: race-free (all reads from the values
                                  before the loop), the semantics of the HIP path      */
+    int dtvar;                /* VARIABLE_TIMESTEP: a divergent step is redone with half
+                                 the time step, which recovers later (0: stop with an error) */
 } cms_case;
 
 typedef struct {

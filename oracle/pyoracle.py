@@ -26,7 +26,7 @@ class CmsCase(C.Structure):
         ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
-        ("serial", C.c_int),
+        ("serial", C.c_int), ("dtvar", C.c_int),
     ]
 
 
@@ -76,7 +76,7 @@ class Oracle:
     """FP64 restatement run on a Case (see cms2d_amd.cases)."""
 
     def __init__(self, case, *, serial=0, solver_mode=0, rtol=1e-10, atol=1e-12, kmax=4000,
-                 maxit=0):
+                 maxit=0, dtvar=0):
         L = lib()
         self.case = case
         self._keep = []
@@ -101,6 +101,7 @@ class Oracle:
         c.noptset, c.wallfric, c.ndsch = case.noptset, case.wallfric, case.ndsch
         c.maxit, c.solver_mode, c.rtol, c.atol, c.kmax = maxit, solver_mode, rtol, atol, kmax
         c.serial = serial
+        c.dtvar = dtvar
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)
