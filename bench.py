@@ -44,6 +44,14 @@ GROUPS = {
     "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
     "mg_up0": (37, "k_mg_up<double> (multigrid level 0: correction + red-black post-smoothing)"),
 }
+# Whole-step byte model (FP64, per active cell): once per step 358 (static
+# sources, wet/dry, velface_eta, level rotation, mass balance); per outer
+# iteration 1156 (momentum assembly 193, three solve prologues 80+80+64, three
+# ILU0 factorisations 3*48, pp assembly 185, pvfc 168, gradients/interpolation
+# 105, friction 64, eddy viscosity 73); per BiCGStab iteration of any of the
+# three systems 376 (two SpMVs 7*8 each, two preconditioner applications 9*8
+# each, vector updates and dot products 15*8).
+STEP_MODEL = {"per_step": 358, "per_outer": 1156, "per_krylov_iter": 376}
 
 
 def parse():
@@ -58,6 +66,7 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--ens-members", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=3)
+    ap.add_argument("--ens-threads", type=int, default=8)
     ap.add_argument("--cpu-sample", default="500x375x2",
                     help="oracle sample: NXxNYxSTEPS of the same workload definition")
     ap.add_argument("--ref-sample", default="500x375x3",
@@ -165,14 +174,15 @@ def ensemble_form(a):
     from cms2d_amd import _lib, cases
     from cms2d_amd.ensemble import Ensemble
     ms = cases.ensemble_members(n=a.ens_members, seed=11)
-    e = Ensemble(ms, nthreads=8, precision=_lib.PREC_FP32_REFINE)
+    e = Ensemble(ms, nthreads=a.ens_threads, precision=_lib.PREC_FP32_REFINE)
     e.step(1)
     t0 = time.perf_counter()
     infos = e.step(a.ens_steps)
     el = time.perf_counter() - t0
     ok = all(i["iconv"] in (1, 2) for i in infos)
     out = {"value": e.ncells * a.ens_steps / el, "unit": "cell-steps/s", "members": len(ms),
-           "cells": e.ncells, "steps": a.ens_steps, "seconds": el, "host_threads": 8,
+           "cells": e.ncells, "steps": a.ens_steps, "seconds": el, "host_threads": a.ens_threads,
+           "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default"),
            "form": "FP32 Krylov + FP64 refinement, dt 60 s", "all_converged": ok,
            "note": "members run on host threads with one stream each (not one persistent launch)"}
     e.close()
@@ -223,6 +233,9 @@ def main():
     D = kern[dom]
     achieved = D["achieved_GBs"]
     value = ncells * a.steps / elapsed
+    model_b = (STEP_MODEL["per_step"] + STEP_MODEL["per_outer"] * outer / a.steps
+               + STEP_MODEL["per_krylov_iter"] * kry.sum() / a.steps)
+    model_gbs = model_b * value / 1e9
     out = {
         "metric": "implicit cell-steps/s",
         "value": value,
@@ -257,6 +270,13 @@ def main():
             "launches": D["launches"],
             "bytes_per_launch": D["bytes_per_launch"],
             "timed_groups": kern,
+        },
+        "step_model_roofline": {
+            "bytes_per_cell_step": model_b, "achieved": model_gbs, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": model_gbs / HBM_PEAK_GBS,
+            "model": "whole step, FP64 bytes per active cell: 358 per step + 1156 per outer "
+                     "iteration + 376 per BiCGStab iteration (u, v and pp), with this run's "
+                     "iteration counts, at the measured cell-steps/s",
         },
         "cpu_baseline": None,
     }

@@ -84,7 +84,7 @@ def lib():
     L.cms2d_flow_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, D, C.POINTER(C.c_long)]
     L.cms2d_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int,
                                       C.POINTER(StepInfo)]
-    L.cms2d_solve.argtypes = [C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double, C.c_double,
+    L.cms2d_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double, C.c_double,
                               C.c_double, C.c_int, D, I]
     _LIB = L
     return L

@@ -966,13 +966,16 @@ extern "C" int cms2d_flow_kernel_stats(const cms2d_flow *F, const char *name, do
 
 // ----------------------------------------------------------------------------
 // cms2d_solve: solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
-extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const int *cell2cell, const int *iwet,
-                           double *ss, const double *sp, double *res, double *phi, double relax, double rtol,
-                           double atol, int kmax, double *rmom, int *iters) {
+extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
+                           const int *iwet, double *ss, const double *sp, double *res, double *phi, double relax,
+                           double rtol, double atol, int kmax, double *rmom, int *iters) {
     if (ncells <= 0 || ncellsD < ncells || !acoef || !cell2cell || !iwet || !ss || !sp || !res || !phi ||
         !(relax > 0.0))
         return set_err(CMS2D_ERR_ARG, "invalid argument");
-    for (long f = 0; f < 4L * ncells; f++)
+    if (nmaxfaces < 1 || nmaxfaces > ELL_MAX_FACES)
+        return set_err(CMS2D_ERR_ARG, "nmaxfaces must be 1.." + std::to_string(ELL_MAX_FACES));
+    const long nf = nmaxfaces;
+    for (long f = 0; f < nf * ncells; f++)
         if (cell2cell[f] < -1 || cell2cell[f] >= ncellsD) return set_err(CMS2D_ERR_ARG, "cell2cell out of range");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
@@ -986,8 +989,8 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
     };
     const size_t nD = ncellsD, n = ncells;
     int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
-    double *d_a = (double *)alloc(4 * nD * sizeof(double));
-    int *d_c2c = (int *)alloc(4 * nD * sizeof(int));
+    double *d_a = (double *)alloc(nf * nD * sizeof(double));
+    int *d_c2c = (int *)alloc(nf * nD * sizeof(int));
     int *d_w = (int *)alloc(nD * sizeof(int));
     double *d_ss = (double *)alloc(nD * sizeof(double)), *d_sp = (double *)alloc(nD * sizeof(double));
     double *d_res = (double *)alloc(nD * sizeof(double)), *d_phi = (double *)alloc(nD * sizeof(double));
@@ -1003,8 +1006,8 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
     int rc = 0;
     do {
         if (hipStreamCreate(&st) != hipSuccess) { rc = set_err(CMS2D_ERR_HIP, "stream"); break; }
-        bool ok = hipMemcpyAsync(d_a, acoef, 4 * nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_c2c, cell2cell, 4 * nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+        bool ok = hipMemcpyAsync(d_a, acoef, nf * nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+                  hipMemcpyAsync(d_c2c, cell2cell, nf * nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
                   hipMemcpyAsync(d_w, iwet, nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
                   hipMemcpyAsync(d_ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
                   hipMemcpyAsync(d_sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
@@ -1018,7 +1021,7 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, const double *acoef, const i
         K.dinv_w[0] = vec + 6 * n;
         K.sc = d_sc;
         K.red.pr[0] = partials; K.red.pv[0] = partials + 2 * nblk; K.red.pt[0] = partials + 3 * nblk;
-        EllOp A{ncells, d_c2c, d_a};
+        EllOp A{ncells, nmaxfaces, d_c2c, d_a};
         hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(KBLOCK), 0, st, A, (const int *)d_w, d_ss,
                            (const double *)d_sp, d_res, (const double *)d_phi, relax, K);
         int prev = 8;

@@ -139,8 +139,9 @@ using StencilOp32 = StencilOpT<float>;
 
 // The reference's general cell list: acoef(4,ncellsD) and cell2cell(4,ncellsD)
 // (0-based, ghosts numbered from ncells, -1 for no face).
+constexpr int ELL_MAX_FACES = 8;   // nmaxfaces: 4 Cartesian, 6 telescoping (read_tel, geospatial.F90:433)
 struct EllOp {
-    int ncells;
+    int ncells, nf;      // cells, faces per cell (row stride of c2c and a)
     const int *c2c;
     const double *a;
     __device__ __forceinline__ int n() const { return ncells; }
@@ -159,9 +160,10 @@ struct EllOp {
         const int c = x.c;
         double sum = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int nb = c2c[4 * c + k];
-            if (nb >= 0 && nb < ncells) sum += a[4 * c + k] * (xv[nb] * dinv[nb]);
+        for (int k = 0; k < ELL_MAX_FACES; k++) {
+            if (k >= nf) break;
+            const int nb = c2c[nf * c + k];
+            if (nb >= 0 && nb < ncells) sum += a[nf * c + k] * (xv[nb] * dinv[nb]);
         }
         return xv[c] - sum;
     }

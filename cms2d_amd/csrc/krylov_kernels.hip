@@ -276,11 +276,12 @@ __global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwe
     for (int c = ch.begin + threadIdx.x; c < ch.end; c += KBLOCK) {
         double asum = 0.0, anb = 0.0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            int nbc = A.c2c[4 * c + k];
+        for (int k = 0; k < ELL_MAX_FACES; k++) {
+            if (k >= A.nf) break;
+            const int nbc = A.c2c[A.nf * c + k];
             if (nbc < 0) continue;
-            asum += A.a[4 * c + k];
-            anb += A.a[4 * c + k] * phi[nbc];
+            asum += A.a[A.nf * c + k];
+            anb += A.a[A.nf * c + k] * phi[nbc];
         }
         double ap = (asum - sp[c]) * rinv;
         double ph = phi[c];

@@ -280,7 +280,7 @@ __device__ __forceinline__ float mg_red_res(const MgLevel &L, const TB *b, int i
 // Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
 template <class TB>
 __global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
-    if (sc->done) return;
+    const bool done = sc->done;   // tested before the stores: its load overlaps the others
     int I, J, cc;
     if (!mg_thread(C, I, J, cc)) return;
     const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
@@ -289,16 +289,20 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, Mg
     const float zb10 = mg_black0(f, b, i1, j0), zb01 = mg_black0(f, b, i0, j1);
     const float zbW = mg_black0(f, b, i0 - 1, j0), zbS = mg_black0(f, b, i0, j0 - 1);
     const int c00 = f.idx(i0, j0);
-    f.z[c00] = (float)b[c00] * f.dinv[c00];
+    const float z00 = (float)b[c00] * f.dinv[c00];
     float r = mg_red_res(f, b, i0, j0, zb01, zb10, zbS, zbW);
-    if (hi) f.z[f.idx(i1, j0)] = zb10;
-    if (hj) f.z[f.idx(i0, j1)] = zb01;
+    float z11 = 0.0f;
     if (hi && hj) {
         const int c11 = f.idx(i1, j1);
-        f.z[c11] = (float)b[c11] * f.dinv[c11];
+        z11 = (float)b[c11] * f.dinv[c11];
         const float zbE = mg_black0(f, b, i1 + 1, j1), zbN = mg_black0(f, b, i1, j1 + 1);
         r += mg_red_res(f, b, i1, j1, zbN, zbE, zb10, zb01);
     }
+    if (done) return;
+    f.z[c00] = z00;
+    if (hi) f.z[f.idx(i1, j0)] = zb10;
+    if (hj) f.z[f.idx(i0, j1)] = zb01;
+    if (hi && hj) f.z[f.idx(i1, j1)] = z11;
     C.b[cc] = r;
 }
 
@@ -319,27 +323,31 @@ __device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const Mg
 // L into L.zo.
 template <class TB>
 __global__ void __launch_bounds__(BLOCK) k_mg_up_pc(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
-    if (sc->done) return;
+    const bool done = sc->done;   // tested before the stores: its load overlaps the others
     int I, J, cc;
     if (!mg_thread(C, I, J, cc)) return;
     const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
     const bool hi = i1 < L.nx, hj = j1 < L.ny;
     const float r00 = mg_red1(L, b, C, i0, j0);
     const float r11 = (hi && hj) ? mg_red1(L, b, C, i1, j1) : 0.0f;
-    L.zo[L.idx(i0, j0)] = r00;
-    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
+    float z10 = 0.0f, z01 = 0.0f;
     if (hi) {
         // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
         const int c = L.idx(i1, j0);
         const float zN = r11, zE = mg_red1(L, b, C, i1 + 1, j0), zS = mg_red1(L, b, C, i1, j0 - 1), zW = r00;
-        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+        z10 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
     }
     if (hj) {
         // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
         const int c = L.idx(i0, j1);
         const float zN = mg_red1(L, b, C, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, i0 - 1, j1);
-        L.zo[c] = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+        z01 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
     }
+    if (done) return;
+    L.zo[L.idx(i0, j0)] = r00;
+    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
+    if (hi) L.zo[L.idx(i1, j0)] = z10;
+    if (hj) L.zo[L.idx(i0, j1)] = z01;
 }
 
 // Coarsest grid: MG_COARSE_SWEEPS red-black sweeps from z = 0 in one block
@@ -351,7 +359,7 @@ constexpr int MG_CT = 1024;
 constexpr int MG_CPT = MG_COARSE_MAX / MG_CT;   // cells per thread
 template <class TB>
 __global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, const KryScal *sc) {
-    if (sc->done) return;
+    const bool done = sc->done;   // tested after the loads (uniform over the block)
     extern __shared__ float zs[];
     const int n = L.nx * L.ny, npad = L.nxp * (L.ny + 2);
     const int j0 = (int)threadIdx.x / L.nx, i0 = (int)threadIdx.x - j0 * L.nx;
@@ -373,6 +381,7 @@ __global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, c
             if (i >= L.nx) { i -= L.nx; j++; }
         }
     }
+    if (done) return;
     __syncthreads();
     for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
         for (int cl = 0; cl < 2; cl++) {

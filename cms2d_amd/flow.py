@@ -106,8 +106,9 @@ def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, ato
           kmax=4000):
     """solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
 
-    acoef, cell2cell: [ncellsD, 4] (faces N,E,S,W), cell2cell 0-based with -1
-    for no face; iwet, ss, sp, phi: [ncellsD].  ss and phi are updated in place.
+    acoef, cell2cell: [ncellsD, nmaxfaces] (4 faces N,E,S,W on Cartesian grids,
+    up to 6 in any order on telescoping grids), cell2cell 0-based with -1 for
+    no face; iwet, ss, sp, phi: [ncellsD].  ss and phi are updated in place.
     Returns (res, rmom, iterations)."""
     acoef = np.ascontiguousarray(acoef, dtype=np.float64)
     c2c = np.ascontiguousarray(cell2cell, dtype=np.int32)
@@ -116,10 +117,11 @@ def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, ato
     assert phi.dtype == np.float64 and phi.flags.c_contiguous
     spa = np.ascontiguousarray(sp, dtype=np.float64)
     ncellsD = acoef.shape[0]
+    assert c2c.shape == acoef.shape
     res = np.zeros(ncellsD)
     rmom = C.c_double()
     iters = C.c_int()
-    check(_lib.lib().cms2d_solve(ncells, ncellsD, _dp(acoef), c2c.ctypes.data_as(I),
+    check(_lib.lib().cms2d_solve(ncells, ncellsD, acoef.shape[1], _dp(acoef), c2c.ctypes.data_as(I),
                                  iw.ctypes.data_as(I), _dp(ss), _dp(spa), _dp(res), _dp(phi),
                                  relax, rtol, atol, kmax, C.byref(rmom), C.byref(iters)))
     return res, rmom.value, iters.value

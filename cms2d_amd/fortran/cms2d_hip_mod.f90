@@ -83,10 +83,10 @@ module cms2d_hip
             real(c_double), intent(out) :: out(*)
             integer(c_int) :: rc
         end function
-        function c_solve(ncells, ncellsD, acoef, c2c, iwet, ss, sp, res, phi, relax, rtol, atol, kmax, &
-                         rmom, iters) bind(C, name='cms2d_solve') result(rc)
+        function c_solve(ncells, ncellsD, nmaxfaces, acoef, c2c, iwet, ss, sp, res, phi, relax, rtol, atol, &
+                         kmax, rmom, iters) bind(C, name='cms2d_solve') result(rc)
             import :: c_int, c_double
-            integer(c_int), value :: ncells, ncellsD, kmax
+            integer(c_int), value :: ncells, ncellsD, nmaxfaces, kmax
             real(c_double), intent(in) :: acoef(*), sp(*)
             integer(c_int), intent(in) :: c2c(*), iwet(*)
             real(c_double), intent(inout) :: ss(*), phi(*)
@@ -139,21 +139,17 @@ contains
         integer :: i, k
         real(c_double) :: tol
         integer(c_int) :: it
-        if (nmaxfaces /= 4) then
-            rc = CMS2D_ERR_ARG
-            return
-        end if
-        allocate(c2c(4*ncellsD), a4(4*ncellsD), iw(ncellsD))
+        allocate(c2c(nmaxfaces*ncellsD), a4(nmaxfaces*ncellsD), iw(ncellsD))
         do i = 1, ncellsD
-            do k = 1, 4
-                c2c(4*(i-1)+k) = cell2cell(k, i) - 1         ! 0-based, -1 for none
-                a4(4*(i-1)+k) = acoef(k, i)
+            do k = 1, nmaxfaces
+                c2c(nmaxfaces*(i-1)+k) = cell2cell(k, i) - 1         ! 0-based, -1 for none
+                a4(nmaxfaces*(i-1)+k) = acoef(k, i)
             end do
             iw(i) = iwet(i)
         end do
         tol = 1.0d-10
         if (present(rtol)) tol = rtol
-        rc = c_solve(int(ncells, c_int), int(ncellsD, c_int), a4, c2c, iw, ss, sp, res, phi, &
+        rc = c_solve(int(ncells, c_int), int(ncellsD, c_int), int(nmaxfaces, c_int), a4, c2c, iw, ss, sp, res, phi, &
                      real(relax, c_double), tol, 1.0d-12, 4000_c_int, rmom, it)
         iters = it
     end function

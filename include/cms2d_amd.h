@@ -139,13 +139,17 @@ int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *av
 int cms2d_ensemble_step(cms2d_flow **flows, int nflows, int nsteps, int nthreads, cms2d_step_info *infos);
 
 /* Drop-in for solve(acoef,ss,sp,res,phi,n) on the reference's general cell
- * list (solvers.f90:205-304): nmaxfaces = 4, 0-based cell2cell with ghost cells
- * numbered ncells..ncellsD-1 (-1 for none), host arrays in the reference layout
- * (acoef[4*ncellsD] face-major per cell).  relax = relax(n).  Under-relaxation
+ * list (solvers.f90:205-304): nmaxfaces faces per cell (1..8; 4 on Cartesian
+ * grids, geospatial.F90:2572, 6 on telescoping grids, :433), 0-based cell2cell
+ * with ghost cells numbered ncells..ncellsD-1 (-1 for no face), host arrays in
+ * the reference layout (acoef[nmaxfaces*ncellsD], the faces of a cell
+ * contiguous).  Any caller of solve (flow n=1..3, sediment n=4, salinity and
+ * heat n=5,6: sed_imp.f90:230, salinity.F90:510,599, heat.f90:480,559) passes
+ * its own system.  relax = relax(n).  Under-relaxation
  * and the normalized residual follow the reference; the Krylov solve stops at
  * ||r||2 <= max(rtol*||r0||2, atol*sqrt(ncells)).  Outputs: ss, res, phi,
  * *rmom, *iters. */
-int cms2d_solve(int ncells, int ncellsD, const double *acoef, const int *cell2cell,
+int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
                 const int *iwet, double *ss, const double *sp, double *res, double *phi,
                 double relax, double rtol, double atol, int kmax, double *rmom, int *iters);
 

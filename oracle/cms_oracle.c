@@ -598,7 +598,7 @@ static void bound_uv(cms_oracle *o)
  * iterates, not the converged solution.
  * ------------------------------------------------------------------------- */
 typedef struct {
-    int n, nd;
+    int n, nd, nf;            /* cells, cells + ghosts, faces per cell (nmaxfaces) */
     const double *acoef;
     const int *c2c;
     const double *ap;
@@ -611,11 +611,11 @@ static void ilu0_build(sysmat *S)
     int n = S->n, nz = 0;
     for (int i = 0; i < n; i++) {
         S->ia[i] = nz;
-        int cols[5]; double vals[5]; int m = 0;
+        int cols[CMS_MAX_FACES + 1]; double vals[CMS_MAX_FACES + 1]; int m = 0;
         cols[m] = i; vals[m] = S->ap[i]; m++;
-        for (int k = 0; k < 4; k++) {
-            int nck = S->c2c[4 * i + k];
-            if (nck >= 0 && nck < n) { cols[m] = nck; vals[m] = -S->acoef[4 * i + k]; m++; }
+        for (int k = 0; k < S->nf; k++) {
+            int nck = S->c2c[S->nf * i + k];
+            if (nck >= 0 && nck < n) { cols[m] = nck; vals[m] = -S->acoef[S->nf * i + k]; m++; }
         }
         for (int a = 1; a < m; a++) {          /* insertion sort by column */
             int cc = cols[a]; double vv = vals[a]; int b = a - 1;
@@ -665,9 +665,9 @@ static void spmv(const sysmat *S, const double *z, double *y)
 {
     for (int i = 0; i < S->n; i++) {
         double s = 0.0;
-        for (int k = 0; k < 4; k++) {
-            int nck = S->c2c[4 * i + k];
-            if (nck >= 0 && nck < S->n) s += S->acoef[4 * i + k] * z[nck];
+        for (int k = 0; k < S->nf; k++) {
+            int nck = S->c2c[S->nf * i + k];
+            if (nck >= 0 && nck < S->n) s += S->acoef[S->nf * i + k] * z[nck];
         }
         y[i] = S->ap[i] * z[i] - s;
     }
@@ -683,9 +683,9 @@ static int bicgstab(sysmat *S, const double *su, double *phi, int mode, int nmax
     double res0 = 0.0, n20 = 0.0;
     for (int i = 0; i < n; i++) {
         double term = 0.0;
-        for (int k = 0; k < 4; k++) {
-            int nck = S->c2c[4 * i + k];
-            if (nck >= 0) term += S->acoef[4 * i + k] * phi[nck];
+        for (int k = 0; k < S->nf; k++) {
+            int nck = S->c2c[S->nf * i + k];
+            if (nck >= 0) term += S->acoef[S->nf * i + k] * phi[nck];
         }
         term = term + su[i] - S->ap[i] * phi[i];
         r[i] = term + SMALL;
@@ -765,7 +765,7 @@ static int solve_sys(cms_oracle *o, double *ss, const double *sp, double *res, d
         adak += res[i] * res[i];
     }
     *rmom_out = sqrt(adak / (double)(float)nc);
-    sysmat S = {nc, o->ncellsD, o->acoef, o->c2c, ap, o->ia, o->ja, o->diagpos, o->lu};
+    sysmat S = {nc, o->ncellsD, 4, o->acoef, o->c2c, ap, o->ia, o->ja, o->diagpos, o->lu};
     int nswp = (n == 1) ? 15 : 5;                       /* flow.F90:779-783 */
     int nmax = o->c.solver_mode == 1 ? nswp : o->c.kmax;
     return bicgstab(&S, ss, phi, o->c.solver_mode, nmax, o->c.rtol, o->c.atol, o->wk);
@@ -1411,20 +1411,21 @@ int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy
     return 0;
 }
 
-int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2c,
+int cms_oracle_solve(int ncells, int ncellsD, int nf, const double *acoef, const int *c2c,
                      const int *iwet, double *ss, const double *sp, double *res,
                      double *phi, double relax, double rtol, double atol, int kmax,
                      double *rmom)
 {
+    if (nf < 1 || nf > CMS_MAX_FACES) return -1;
     double rinv = 1.0 / relax, relaxm = 1.0 - relax, adak = 0.0;
     double *ap = DALLOC(ncells), *wk = DALLOC(6 * (size_t)ncells);
     for (int i = 0; i < ncells; i++) {
         double asum = 0.0, anb = 0.0;
-        for (int k = 0; k < 4; k++) {
-            int nck = c2c[4 * i + k];
+        for (int k = 0; k < nf; k++) {
+            int nck = c2c[nf * i + k];
             if (nck < 0) continue;
-            asum += acoef[4 * i + k];
-            anb += acoef[4 * i + k] * phi[nck];
+            asum += acoef[nf * i + k];
+            anb += acoef[nf * i + k] * phi[nck];
         }
         ap[i] = (asum - sp[i]) * rinv;
         ss[i] = ss[i] + relaxm * phi[i] * ap[i];
@@ -1432,9 +1433,9 @@ int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2
         adak += res[i] * res[i];
     }
     *rmom = sqrt(adak / (double)(float)ncells);
-    int *ia = IALLOC(ncells + 1), *ja = IALLOC(5 * ncells), *dp = IALLOC(ncells);
-    double *lu = DALLOC(5 * ncells);
-    sysmat S = {ncells, ncellsD, acoef, c2c, ap, ia, ja, dp, lu};
+    int *ia = IALLOC(ncells + 1), *ja = IALLOC((nf + 1) * (size_t)ncells), *dp = IALLOC(ncells);
+    double *lu = DALLOC((nf + 1) * (size_t)ncells);
+    sysmat S = {ncells, ncellsD, nf, acoef, c2c, ap, ia, ja, dp, lu};
     int it = bicgstab(&S, ss, phi, 0, kmax, rtol, atol, wk);
     free(ap); free(wk); free(ia); free(ja); free(dp); free(lu);
     return it;

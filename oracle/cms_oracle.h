@@ -89,9 +89,11 @@ int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy
 
 /* Stand-alone linear solve on a general cell list, the reference's
  * solve(acoef,ss,sp,res,phi,n) (solvers.f90:205-304) with the tight stopping rule.
- * acoef[4*ncellsD], c2c[4*ncellsD] (0-based, -1 none), iwet[ncellsD].
- * Returns iterations used, rmom via *rmom. */
-int cms_oracle_solve(int ncells, int ncellsD, const double *acoef, const int *c2c,
+ * nf = nmaxfaces (4 on Cartesian grids, 6 on telescoping grids, read_tel,
+ * geospatial.F90:433), acoef[nf*ncellsD], c2c[nf*ncellsD] (0-based, -1 none),
+ * iwet[ncellsD].  Returns iterations used (-1: nf out of range), rmom via *rmom. */
+#define CMS_MAX_FACES 8
+int cms_oracle_solve(int ncells, int ncellsD, int nf, const double *acoef, const int *c2c,
                      const int *iwet, double *ss, const double *sp, double *res,
                      double *phi, double relax, double rtol, double atol, int kmax,
                      double *rmom);

@@ -62,7 +62,7 @@ def lib():
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
         L.cms_oracle_solve.restype = C.c_int
-        L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
+        L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
                                        C.c_double, C.c_double, C.c_int, D]
         _LIB = L
     return _LIB

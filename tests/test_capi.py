@@ -46,7 +46,7 @@ def test_invalid_arguments_rejected():
     assert L.cms2d_flow_create(C.byref(d), C.byref(h)) == -1
     assert h.value is None
     out = np.zeros(4)
-    assert L.cms2d_solve(0, 0, out.ctypes.data_as(_lib.D), None, None, None, None, None, None,
+    assert L.cms2d_solve(0, 0, 4, out.ctypes.data_as(_lib.D), None, None, None, None, None, None,
                          1.0, 1e-10, 0.0, 10, None, None) == -1
 
 

@@ -1,12 +1,16 @@
-"""cms2d_solve (drop-in for the reference's solve) on a general cell list."""
+"""cms2d_solve (drop-in for the reference's solve) on a general cell list:
+Cartesian lists with land cells (nmaxfaces 4), two-level telescoping lists
+(nmaxfaces 6) and the scalar-transport system the salinity/heat callers build
+(salinity.F90:510,599, heat.f90:480,559) from the flow's face fluxes."""
 import ctypes as C
 
 import numpy as np
 import pytest
 
-from cell_list import cart_cell_list, dense_system
+from cell_list import cart_cell_list, dense_system, tel_cell_list
+from cms2d_amd import cases
 from cms2d_amd.flow import solve
-from pyoracle import D, I, lib as olib
+from pyoracle import D, I, Oracle, lib as olib
 
 
 def make_system(seed, nx=24, ny=18, land=0.15):
@@ -37,7 +41,7 @@ def test_solve_matches_dense_and_oracle(gpu_available, seed, relax):
     res_g, rmom_g, it_g = solve(acoef, c2c, iwet, ss_g, sp, phi_g, relax, ncells=n)
     ss_o, phi_o, res_o = ss.copy(), phi.copy(), np.zeros(nd)
     rm = C.c_double()
-    olib().cms_oracle_solve(n, nd, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
+    olib().cms_oracle_solve(n, nd, 4, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
                             iwet.ctypes.data_as(I), ss_o.ctypes.data_as(D), sp.ctypes.data_as(D),
                             res_o.ctypes.data_as(D), phi_o.ctypes.data_as(D), relax, 1e-10, 1e-12,
                             4000, C.byref(rm))
@@ -49,3 +53,81 @@ def test_solve_matches_dense_and_oracle(gpu_available, seed, relax):
     assert np.abs(res_g[:n] - res_o[:n]).max() <= 1e-12 * np.abs(res_o).max()
     assert np.abs(ss_g[:n] - ss_o[:n]).max() <= 1e-13 * np.abs(ss_o).max()
     assert it_g > 0
+
+
+def oracle_solve(n, nd, acoef, c2c, iwet, ss, sp, phi, relax):
+    ss_o, phi_o, res_o = ss.copy(), phi.copy(), np.zeros(nd)
+    rm = C.c_double()
+    it = olib().cms_oracle_solve(n, nd, acoef.shape[1], acoef.ctypes.data_as(D),
+                                 c2c.ctypes.data_as(I), iwet.ctypes.data_as(I),
+                                 ss_o.ctypes.data_as(D), sp.ctypes.data_as(D),
+                                 res_o.ctypes.data_as(D), phi_o.ctypes.data_as(D), relax, 1e-10,
+                                 1e-12, 4000, C.byref(rm))
+    assert it > 0
+    return ss_o, phi_o, res_o, rm.value
+
+
+def check_against(n, nd, c2c, acoef, sp, ss, phi, iwet, relax):
+    A, b = dense_system(n, acoef, c2c, sp, ss, phi, relax)
+    x = np.linalg.solve(A, b)
+    ss_g, phi_g = ss.copy(), phi.copy()
+    res_g, rmom_g, it_g = solve(acoef, c2c, iwet, ss_g, sp, phi_g, relax, ncells=n)
+    ss_o, phi_o, res_o, rm = oracle_solve(n, nd, acoef, c2c, iwet, ss, sp, phi, relax)
+    scale = np.abs(x).max()
+    assert np.abs(phi_g[:n] - x).max() <= 1e-8 * scale
+    assert np.abs(phi_o[:n] - x).max() <= 1e-8 * scale
+    assert np.array_equal(phi_g[n:], phi[n:])
+    assert np.abs(rmom_g - rm) <= 1e-12 * max(rm, 1e-30)
+    assert np.abs(res_g[:n] - res_o[:n]).max() <= 1e-12 * np.abs(res_o).max()
+    assert np.abs(ss_g[:n] - ss_o[:n]).max() <= 1e-13 * np.abs(ss_o).max()
+    assert it_g > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1])
+def test_solve_telescoping_cell_list(gpu_available, seed):
+    """nmaxfaces = 6; a coarse cell in the inner corner of the refined region
+    has two fine neighbours on two sides."""
+    rng = np.random.default_rng(seed)
+    refined = np.zeros((30, 40), bool)
+    refined[6:20, 0:14] = True
+    refined[18:26, 12:22] = True
+    n, nd, c2c, level, _ = tel_cell_list(refined)
+    assert (c2c[:n] >= 0).sum(1).max() == 6
+    print(f"telescoping solve seed {seed}: {n} cells, {nd - n} ghosts", flush=True)
+    acoef = np.where(c2c >= 0, rng.uniform(0.2, 2.0, c2c.shape), 0.0)
+    acoef[n:] = 0.0
+    sp = np.zeros(nd)
+    sp[:n] = -rng.uniform(0.05, 0.5, n)
+    ss = np.zeros(nd)
+    ss[:n] = rng.normal(size=n)
+    phi = rng.normal(scale=0.1, size=nd)
+    check_against(n, nd, c2c, acoef, sp, ss, phi, np.ones(nd, dtype=np.int32), 1.0)
+
+
+@pytest.mark.gpu
+def test_solve_scalar_transport_system(gpu_available):
+    """Implicit advection-diffusion of a scalar (salinity) over one flow step:
+    a_k = K hk ds/dn + max(-F_k, 0) (upwind, Patankar's form), ap = sum a -
+    sp with the storage sp = -A h/dt, ss = A h1 c1/dt, the boundary value on
+    the ghost cells; F_k, hk and h from 20 steps of the 80x48 basin."""
+    c = cases.basin(nx=80, ny=48, dx=10.0, seed=4, dt=60.0)
+    o = Oracle(c, serial=0)
+    h1 = o.get("h").copy()
+    for _ in range(20):
+        assert o.step(1)[0] == 0
+    flux, hk, h = o.get_face("flux"), o.get_face("hk"), o.get("h")
+    n, nd, c2c = cart_cell_list(np.ones((c.ny, c.nx), bool))
+    area, kdiff = c.dx * c.dy, 1.0
+    acoef = np.zeros((nd, 4))
+    acoef[:n] = kdiff * hk * (c.dx / c.dx) + np.maximum(-flux, 0.0)   # ds/dn = 1 on a uniform grid
+    rng = np.random.default_rng(5)
+    c1 = 30.0 + 5.0 * rng.random(n)
+    sp = np.zeros(nd)
+    sp[:n] = -area * h / c.dt
+    ss = np.zeros(nd)
+    ss[:n] = area * h1 * c1 / c.dt
+    phi = np.full(nd, 35.0)                 # ocean salinity on the ghost cells
+    phi[:n] = c1
+    print(f"transport system {n} cells", flush=True)
+    check_against(n, nd, c2c, acoef, sp, ss, phi, np.ones(nd, dtype=np.int32), 1.0)

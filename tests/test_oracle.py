@@ -122,7 +122,7 @@ def test_solve_matches_dense():
         s2, p2, res = ss.copy(), phi.copy(), np.zeros(nd)
         import ctypes as C
         rm = C.c_double()
-        it = lib().cms_oracle_solve(n, nd, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
+        it = lib().cms_oracle_solve(n, nd, 4, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
                                     iwet.ctypes.data_as(I), s2.ctypes.data_as(D),
                                     sp.ctypes.data_as(D), res.ctypes.data_as(D),
                                     p2.ctypes.data_as(D), relax, 1e-12, 0.0, 2000, C.byref(rm))
@@ -132,6 +132,43 @@ def test_solve_matches_dense():
         r0 = (b - A @ phi[:n]) / np.diag(A)
         assert rm.value == pytest.approx(math.sqrt((r0 ** 2).sum() / f4(n)), rel=1e-10)
 
+
+
+def test_solve_matches_dense_telescoping():
+    """nmaxfaces = 6: the cell list of a two-level telescoping grid, faces in
+    any order, unused face slots -1."""
+    import ctypes as C
+    from cell_list import tel_cell_list
+    rng = np.random.default_rng(9)
+    refined = np.zeros((9, 12), bool)
+    refined[2:6, 0:5] = True
+    refined[5:7, 4:7] = True
+    n, nd, c2c, level, _ = tel_cell_list(refined)
+    assert (c2c[:n] >= 0).sum(1).max() == 6
+    acoef = np.where(c2c >= 0, rng.uniform(0.1, 3.0, c2c.shape), 0.0)
+    acoef[n:] = 0.0
+    sp = np.zeros(nd)
+    sp[:n] = -rng.uniform(0.01, 1.0, n)
+    ss = np.zeros(nd)
+    ss[:n] = rng.normal(size=n)
+    phi = rng.normal(size=nd)
+    iwet = np.ones(nd, dtype=np.int32)
+    A, b = dense_system(n, acoef, c2c, sp, ss, phi, 1.0)
+    x = np.linalg.solve(A, b)
+    s2, p2, res = ss.copy(), phi.copy(), np.zeros(nd)
+    rm = C.c_double()
+    it = lib().cms_oracle_solve(n, nd, 6, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
+                                iwet.ctypes.data_as(I), s2.ctypes.data_as(D),
+                                sp.ctypes.data_as(D), res.ctypes.data_as(D),
+                                p2.ctypes.data_as(D), 1.0, 1e-12, 0.0, 2000, C.byref(rm))
+    assert it > 0
+    assert np.abs(p2[:n] - x).max() < 1e-9 * np.abs(x).max()
+    r0 = (b - A @ phi[:n]) / np.diag(A)
+    assert rm.value == pytest.approx(math.sqrt((r0 ** 2).sum() / f4(n)), rel=1e-10)
+    assert lib().cms_oracle_solve(n, nd, 9, acoef.ctypes.data_as(D), c2c.ctypes.data_as(I),
+                                  iwet.ctypes.data_as(I), s2.ctypes.data_as(D),
+                                  sp.ctypes.data_as(D), res.ctypes.data_as(D),
+                                  p2.ctypes.data_as(D), 1.0, 1e-12, 0.0, 2000, C.byref(rm)) == -1
 
 # ---------------------------------------------------------------- invariants
 def test_lake_at_rest_stays_at_rest():
