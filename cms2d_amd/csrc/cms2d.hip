@@ -729,7 +729,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                 if (int rc = dalloc(F, &L.ap, n)) return rc;
             }
             M.nlev++;
-            if ((long)nx * ny <= MG_COARSE_MAX && (long)(nx + 2) * (ny + 2) <= MG_COARSE_LDS) break;
+            const bool fits = (long)nx * ny <= MG_COARSE_MAX && (long)(nx + 2) * (ny + 2) <= MG_COARSE_LDS;
+            if (fits && (M.nlev >= MG_MIN_LEVELS || (nx == 1 && ny == 1))) break;
             if (M.nlev == MG_MAX_LEVELS) return set_err(CMS2D_ERR_ARG, "grid too large for the multigrid hierarchy");
             nx = (nx + 1) / 2;
             ny = (ny + 1) / 2;

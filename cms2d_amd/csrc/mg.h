@@ -4,7 +4,8 @@
 // The reference preconditions its Krylov solvers with ILUT (ilutp.f90,
 // solvers.f90:185-199).  Here M^-1 is one V(1,1) cycle:
 //   * cell-centred 2x2 coarsening, levels until the grid has at most
-//     MG_COARSE_MAX cells and its padded layout fits MG_COARSE_LDS floats;
+//     MG_COARSE_MAX cells, its padded layout fits MG_COARSE_LDS floats and
+//     there are at least MG_MIN_LEVELS levels (or the grid is one cell);
 //   * smoother: one red-black Gauss-Seidel sweep before (red first, from a zero
 //     guess) and after (red first) the coarse correction;
 //   * per level two launches (down: sweep + restriction; up: correction +
@@ -41,6 +42,12 @@ constexpr int MG_MAX_LEVELS = 16;
 constexpr int MG_COARSE_MAX = CMS2D_MG_COARSE_MAX;   // cells of the coarsest grid (a multiple of 1024)
 constexpr int MG_COARSE_LDS = 16384;                 // its padded size in floats (64 KB of LDS)
 constexpr int MG_COARSE_SWEEPS = CMS2D_MG_COARSE_SWEEPS;
+// Each coarsening quadruples the storage term of a row against its face
+// coefficients, so three levels down the coarsest operator is diagonally
+// dominant enough for MG_COARSE_SWEEPS sweeps to act as its solve.  One level
+// down (the 64 x 64 coarsest of a 128 x 128 grid) the sweeps barely touch its
+// smooth error and the pp BiCGStab needs about three times the iterations.
+constexpr int MG_MIN_LEVELS = 4;
 
 struct MgLevel {
     int nx, ny, nxp;
