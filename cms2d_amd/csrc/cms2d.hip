@@ -15,6 +15,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -114,6 +115,10 @@ struct cms2d_flow {
     bool stats = true;
     KTimer tm;                 // live timing of the pp solve's main kernels
     KTimer *tm_active = nullptr;
+    // launch-bound grids: BiCGStab iteration pairs replayed from captured
+    // graphs, one per Krylov system (indexed like d_sc)
+    bool use_graphs = false;
+    hipGraphExec_t kgraph[4]{};
     hipEvent_t step_ev[2]{};
 };
 
@@ -187,7 +192,33 @@ struct KryCtx {
     KryScal *d_sc, *h_sc;
     int *prev_iters;
     KTimer *tm;                    // timed launches (or null)
+    hipGraphExec_t *graph = nullptr;   // per-system graph slots (or null: direct launches)
 };
+
+// Grids up to this many cells replay BiCGStab iteration pairs from a graph
+// captured on first use: on them a launch lasts about as long as the host
+// takes to issue it, and a graph issues the ~20 launches of a pair at once.
+// The kernels and their arguments are those of direct launches, so results
+// are bitwise the same either way.
+#ifndef CMS2D_GRAPH_CELLS
+#define CMS2D_GRAPH_CELLS (1 << 20)
+#endif
+
+template <class Body>
+static int capture_graph(hipStream_t st, hipGraphExec_t &ge, Body body) {
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    body();
+    const hipError_t e = hipStreamEndCapture(st, &gr);
+    if (e != hipSuccess) return set_err(CMS2D_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e2 != hipSuccess) {
+        ge = nullptr;
+        return set_err(CMS2D_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e2));
+    }
+    return 0;
+}
 
 struct NoPrec {
     void operator()(int) const {}
@@ -197,9 +228,34 @@ struct NoPrec {
 // batch is enqueued without host synchronisation; kernels launched after a
 // system has stopped return at once (they read the device-side done flag).
 // PREC 1: pre(0) enqueues zp = M^-1 p, pre(1) enqueues zs = M^-1 s.
+// One BiCGStab iteration (index i) of the systems in K.
+template <class Op, int NS, int PREC, class T, class Pre>
+static void bicg_iter(const KryCtx &X, KrySys<NS, T> &K, Op A, Pre &pre, int i) {
+    const dim3 grid(X.nblk), blk(KBLOCK);
+    if (X.tm) X.tm->it = i;
+    hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), grid, blk, 0, X.st, A, K, i, X.nblk, X.kmax, X.rtol, X.atol, 0);
+    if (PREC) pre(0);
+    tlaunch(X.tm, TG_SPMV_PP, k_bicg_spmv<Op, NS, 1, PREC, T>, grid, blk, 0, X.st, A, K);
+    hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, i, X.nblk);
+    if (PREC) pre(1);
+    hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC, T>), grid, blk, 0, X.st, A, K);
+    hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, i, X.nblk);
+}
+
+// Captures iterations 2 and 3 of the systems in K into ge.  Iterations from 2
+// on depend on their index only through its parity (k_bicg_p/s/r use i & 1
+// and i == 0), so the pair replays as any later pair (2m, 2m+1).
+template <class Op, int NS, int PREC = 0, class T = double, class Pre = NoPrec>
+static int capture_bicg_pair(const KryCtx &X, KrySys<NS, T> &K, Op A, hipGraphExec_t &ge, Pre pre = Pre()) {
+    return capture_graph(X.st, ge, [&]() {
+        bicg_iter<Op, NS, PREC, T>(X, K, A, pre, 2);
+        bicg_iter<Op, NS, PREC, T>(X, K, A, pre, 3);
+    });
+}
+
 template <class Op, int NS, int PREC = 0, class T = double, class Pre = NoPrec>
 static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre()) {
-    const dim3 grid(X.nblk), blk(KBLOCK);
+    const dim3 blk(KBLOCK);
     // start: thresholds, rmom and the initial stop test (read with the first
     // batch: a solve that needs no iteration only costs no-op launches)
     hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, 0, X.nblk, X.kmax, X.rtol, X.atol, 1);
@@ -213,17 +269,19 @@ static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre())
     // returns at once) and few host round trips are needed
     int it = 0, nbatch = 0, grow = 2;
     int batch = std::max(1, X.prev_iters[sc0]);
+    const hipGraphExec_t ge = X.graph ? X.graph[sc0] : nullptr;
     for (bool first = true; first || (!all_done() && it < X.kmax); first = false) {
-        int nb = std::min(batch, X.kmax - it);
-        for (int b = 0; b < nb; b++, it++) {
-            if (X.tm) X.tm->it = it;
-            hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), grid, blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 0);
-            if (PREC) pre(0);
-            tlaunch(X.tm, TG_SPMV_PP, k_bicg_spmv<Op, NS, 1, PREC, T>, grid, blk, 0, X.st, A, K);
-            hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, it, X.nblk);
-            if (PREC) pre(1);
-            hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC, T>), grid, blk, 0, X.st, A, K);
-            hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, it, X.nblk);
+        const int nb = std::min(batch, X.kmax - it);
+        for (int b = 0; b < nb;) {
+            if (ge && it >= 2 && (it & 1) == 0 && nb - b >= 2) {
+                HIPCHK(hipGraphLaunch(ge, X.st));
+                b += 2;
+                it += 2;
+            } else {
+                bicg_iter<Op, NS, PREC, T>(X, K, A, pre, it);
+                b++;
+                it++;
+            }
         }
         hipLaunchKernelGGL((k_bicg_p<Op, NS, T>), dim3(1), blk, 0, X.st, A, K, it, X.nblk, X.kmax, X.rtol, X.atol, 1);
         HIPCHK(hipGetLastError());
@@ -282,6 +340,38 @@ static constexpr double REF_RTOL = 1e-5;
 static constexpr int REF_KMAX = 200;
 static constexpr int REF_SWEEPS = 20;
 
+// Multigrid preconditioners of the pp solve (FP64 Krylov vectors) and of the
+// inner FP32 solve.
+struct PpPre {
+    cms2d_flow *F;
+    void operator()(int which) const {
+        if (which == 0) mg_vcycle<double>(F, F->Kpp.p[0], F->Kpp.zp, F->d_sc);
+        else mg_vcycle<double>(F, F->Kpp.r[0], F->Kpp.zs, F->d_sc);
+    }
+};
+struct InnerPre {
+    cms2d_flow *F;
+    void operator()(int which) const {
+        KrySys<1, float> &Ki = F->Kin;
+        if (which == 0) mg_vcycle<float>(F, Ki.p[0], Ki.zp, Ki.sc);
+        else mg_vcycle<float>(F, Ki.r[0], Ki.zs, Ki.sc);
+    }
+};
+
+// Context and operator of the inner FP32 solve.
+static KryCtx inner_ctx(const KryCtx &X) {
+    KryCtx Xi = X;
+    Xi.kmax = REF_KMAX;
+    Xi.rtol = REF_RTOL;
+    Xi.atol = 0.0;
+    Xi.tm = nullptr;
+    return Xi;
+}
+static StencilOp32 inner_op(const cms2d_flow *F) {
+    const MgLevel &L0 = F->mg.L[0];
+    return StencilOp32{F->g, L0.aN, L0.aE, L0.aS, L0.aW};
+}
+
 // The pp solve of the FP32 form: FP32 BiCGStab (FP32 operator and vectors,
 // multigrid) for the correction, FP64 x and residual, FP64 stop test on the
 // FP64 solve's own threshold.  Krylov iterations are summed over the sweeps.
@@ -293,17 +383,9 @@ static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
                        X.rtol, X.atol, 1);
     HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
     HIPCHK(hipStreamSynchronize(F->st));
-    const MgLevel &L0 = F->mg.L[0];
-    StencilOp32 A32{F->g, L0.aN, L0.aE, L0.aS, L0.aW};
-    KryCtx Xi = X;
-    Xi.kmax = REF_KMAX;
-    Xi.rtol = REF_RTOL;
-    Xi.atol = 0.0;
-    Xi.tm = nullptr;
-    auto pre = [F, &Ki](int which) {
-        if (which == 0) mg_vcycle<float>(F, Ki.p[0], Ki.zp, Ki.sc);
-        else mg_vcycle<float>(F, Ki.r[0], Ki.zs, Ki.sc);
-    };
+    const StencilOp32 A32 = inner_op(F);
+    KryCtx Xi = inner_ctx(X);
+    InnerPre pre{F};
     int total = 0;
     for (int sw = 0; sw < REF_SWEEPS && !F->h_sc[0].done; sw++) {
         hipLaunchKernelGGL(k_ref_start, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
@@ -378,8 +460,32 @@ static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
     X.st = F->st; X.nblk = F->nblk_kry; X.kmax = F->P.kmax;
     X.rtol = F->P.rtol; X.atol = F->P.atol;
     X.d_sc = F->d_sc; X.h_sc = F->h_sc; X.prev_iters = F->prev_iters;
-    X.tm = (timed && F->stats) ? &F->tm : nullptr;
+    X.tm = (timed && F->stats && !F->use_graphs) ? &F->tm : nullptr;
+    X.graph = F->use_graphs ? F->kgraph : nullptr;
     return X;
+}
+
+// Captures the BiCGStab iteration pairs of the flow's systems (u/v, pp or the
+// inner FP32 pp solve) once, at creation, in the creating thread; solves only
+// replay them.
+static int capture_graphs(cms2d_flow *F) {
+    if (!F->use_graphs) return 0;
+    const St &s = F->s;
+    const StencilOp A{F->g, s.aN, s.aE, s.aS, s.aW};
+    const KryCtx X = flow_ctx(F, false);
+    if (int rc = capture_bicg_pair<StencilOp, 2, 0, double>(X, F->Kuv, A, F->kgraph[1])) return rc;
+    if (F->precision == CMS2D_PREC_FP32_REFINE) {
+        if (int rc = capture_bicg_pair<StencilOp32, 1, 1, float>(inner_ctx(X), F->Kin, inner_op(F), F->kgraph[3],
+                                                                 InnerPre{F}))
+            return rc;
+    } else if (F->solver == CMS2D_SOLVER_GMRES) {
+        // the pp solve is GMRES
+    } else if (F->precond == CMS2D_PRECOND_MG) {
+        if (int rc = capture_bicg_pair<StencilOp, 1, 1, double>(X, F->Kpp, A, F->kgraph[0], PpPre{F})) return rc;
+    } else {
+        if (int rc = capture_bicg_pair<StencilOp, 1, 0, double>(X, F->Kpp, A, F->kgraph[0])) return rc;
+    }
+    return 0;
 }
 
 static double ramp_func(double timehrs, double rampdur) {
@@ -494,12 +600,8 @@ static int flow_imp_once(cms2d_flow *F) {
                 if (int rc = pp_solve_gmres(F, X, A)) return rc;
             } else if (F->precond == CMS2D_PRECOND_MG) {
                 mg_setup(F);
-                auto pre = [F](int which) {
-                    if (which == 0) mg_vcycle<double>(F, F->Kpp.p[0], F->Kpp.zp, F->d_sc);
-                    else mg_vcycle<double>(F, F->Kpp.r[0], F->Kpp.zs, F->d_sc);
-                };
                 F->tm_active = X.tm;
-                int rc = run_bicg<StencilOp, 1, 1, double>(X, F->Kpp, A, 0, pre);
+                int rc = run_bicg<StencilOp, 1, 1, double>(X, F->Kpp, A, 0, PpPre{F});
                 F->tm_active = nullptr;
                 if (rc) return rc;
             } else {
@@ -567,6 +669,8 @@ extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
     if (!F) return CMS2D_OK;
     (void)hipSetDevice(F->dev);
     if (F->st) (void)hipStreamSynchronize(F->st);
+    for (auto &ge : F->kgraph)
+        if (ge) (void)hipGraphExecDestroy(ge);
     for (void *p : F->allocs) (void)hipFree(p);
     if (F->h_sc) (void)hipHostFree(F->h_sc);
     if (F->h_flags) (void)hipHostFree(F->h_flags);
@@ -631,6 +735,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     F->nblk_act = (g.nact + BLOCK - 1) / BLOCK;
     F->nblk_pad = (F->npad + BLOCK - 1) / BLOCK;
     F->nblk_kry = std::min(KRY_BLOCKS, F->nblk_act);
+    F->use_graphs = (long)g.nact <= CMS2D_GRAPH_CELLS;
+    if (const char *e = getenv("CMS2D_GRAPHS")) F->use_graphs = atoi(e) != 0;   // 0: direct launches, 1: graphs
     Par &P = F->P;
     P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
     P.hmin = P.hdry;                                  // flow.F90:690-694
@@ -773,7 +879,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
     LAUNCH(k_zbk, F->nblk_act, g, s);
-    return init_state(F);
+    if (int rc = init_state(F)) return rc;
+    return capture_graphs(F);
 }
 
 extern "C" int cms2d_flow_create(const cms2d_flow_desc *d, cms2d_flow **out) {

@@ -109,3 +109,23 @@ def test_gmres_matches_oracle(gpu_available, shape, steps):
     # one restart cycle (20 steps) or two per solve; a broken Arnoldi process
     # would still converge through restarts, only far more slowly
     assert info["kry"][0] <= 2 * 20 * info["niter"], info
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_matches_direct_launches(gpu_available, monkeypatch, precision):
+    """BiCGStab iteration pairs replayed from a captured graph (the launch
+    mode of grids up to 1M cells) give bitwise the state and iteration counts
+    of direct launches."""
+    c = cases.tidal(nx=96, ny=64, dx=20.0, seed=5)
+    runs = []
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("CMS2D_GRAPHS", graphs)
+        print(f"graphs {graphs} precision {precision} tidal 96x64, 12 steps", flush=True)
+        g = FlowStep(c, precision=precision)
+        kry = [g.step(1)["kry"] for _ in range(12)]
+        runs.append((kry, [g.get(n) for n in ("eta", "u", "v", "p")]))
+        g.close()
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
