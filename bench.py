@@ -180,11 +180,19 @@ def ensemble_form(a):
     infos = e.step(a.ens_steps)
     el = time.perf_counter() - t0
     ok = all(i["iconv"] in (1, 2) for i in infos)
+    tiles = {}
+    for m in e.members:
+        t, gr = m.tuning()
+        key = f"{t}{'+graphs' if gr else ''}"
+        tiles[key] = tiles.get(key, 0) + 1
     out = {"value": e.ncells * a.ens_steps / el, "unit": "cell-steps/s", "members": len(ms),
            "cells": e.ncells, "steps": a.ens_steps, "seconds": el, "host_threads": a.ens_threads,
            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "runtime default"),
            "form": "FP32 Krylov + FP64 refinement, dt 60 s", "all_converged": ok,
-           "note": "members run on host threads with one stream each (not one persistent launch)"}
+           "members_per_tuning": tiles,
+           "note": "members run on host threads with one stream each (not one persistent launch); "
+                   "per member, the level-0 multigrid tile width timed at creation and graph replay "
+                   "of BiCGStab iteration pairs (members up to 1M cells)"}
     e.close()
     return out
 
@@ -206,6 +214,7 @@ def main():
     case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
     ncells = case.ncells
     g = FlowStep(case)
+    mg_tile, graphs = g.tuning()
     for _ in range(a.warmup):
         g.step(1)
     st0 = {k: g.kernel_stats(k) for k in GROUPS}
@@ -253,7 +262,7 @@ def main():
             "workload": f"tidal basin {a.nx}x{a.ny} uniform Cartesian dx=20m (the 2-level "
                         f"telescoping band of the reference case is not built yet), wet/dry, "
                         f"radiation stress, M2 tide, dt=30s, FP64 BiCGStab rtol 1e-10",
-            "cells": ncells, "dt_s": case.dt,
+            "cells": ncells, "dt_s": case.dt, "mg_tile_x_autotuned": mg_tile, "graph_replay": graphs,
             "outer_iters_per_step": outer / a.steps,
             "krylov_iters_per_step": {"pp": kry[0] / a.steps, "u": kry[1] / a.steps,
                                       "v": kry[2] / a.steps},

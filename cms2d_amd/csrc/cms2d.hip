@@ -119,6 +119,7 @@ struct cms2d_flow {
     // graphs, one per Krylov system (indexed like d_sc)
     bool use_graphs = false;
     hipGraphExec_t kgraph[4]{};
+    int mg_tile = 128;         // level-0 tile width of the multigrid cycle (timed at creation)
     hipEvent_t step_ev[2]{};
 };
 
@@ -310,6 +311,21 @@ static void mg_setup(cms2d_flow *F) {
     for (int l = 0; l + 1 < M.nlev; l++) LAUNCH(k_mg_coarsen, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1]);
 }
 
+// The level-0 launch of the cycle (down or up) with tile width TX.
+template <int TX, class TB>
+static void mg_level0_tx(cms2d_flow *F, KTimer *tm, bool down, const MgLevel &L0, const TB *b0, const KryScal *sc) {
+    using T = MTile<TX>;
+    const dim3 tiles((L0.nx + T::X - 1) / T::X, (L0.ny + T::Y - 1) / T::Y), tb(MT_THREADS);
+    if (down) tlaunch(tm, TG_MG_DOWN0, k_mg_down<TX, TB>, tiles, tb, 0, F->st, L0, b0, F->mg.L[1], sc);
+    else tlaunch(tm, TG_MG_UP0, k_mg_up<TX, TB>, tiles, tb, 0, F->st, L0, b0, F->mg.L[1], sc);
+}
+template <class TB>
+static void mg_level0(cms2d_flow *F, KTimer *tm, bool down, const MgLevel &L0, const TB *b0, const KryScal *sc) {
+    if (F->mg_tile == 64) mg_level0_tx<64>(F, tm, down, L0, b0, sc);
+    else if (F->mg_tile == 256) mg_level0_tx<256>(F, tm, down, L0, b0, sc);
+    else mg_level0_tx<128>(F, tm, down, L0, b0, sc);
+}
+
 // z0 = V(b0): one V(1,1) cycle (mg.h) on the stream of F; every launch
 // returns at once when the solve owning sc has stopped.
 template <class TB>
@@ -324,15 +340,52 @@ static void mg_vcycle(cms2d_flow *F, const TB *b0, float *z0, const KryScal *sc)
         hipLaunchKernelGGL(k_mg_coarsest<TB>, dim3(1), dim3(MG_CT), lds, F->st, L0, b0, sc);
         return;
     }
-    auto tiles = [](const MgLevel &L) { return dim3((L.nx + MT_X - 1) / MT_X, (L.ny + MT_Y - 1) / MT_Y); };
-    const dim3 tb(MT_THREADS);
-    tlaunch(tm, TG_MG_DOWN0, k_mg_down<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
+    mg_level0(F, tm, true, L0, b0, sc);
     for (int l = 1; l + 1 < M.nlev; l++)
         LAUNCH(k_mg_down_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
     hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(MG_CT), lds, F->st, Lc, (const float *)Lc.b, sc);
     for (int l = M.nlev - 2; l >= 1; l--)
         LAUNCH(k_mg_up_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
-    tlaunch(tm, TG_MG_UP0, k_mg_up<TB>, tiles(L0), tb, 0, F->st, L0, b0, M.L[1], sc);
+    mg_level0(F, tm, false, L0, b0, sc);
+}
+
+// Picks the level-0 tile width of the cycle by timing the down and up launches
+// of each candidate on this grid; every width gives the same values.
+static int tune_mg_tile(cms2d_flow *F) {
+    static constexpr int cand[3] = {64, 128, 256};
+    if (F->mg.nlev < 2) return 0;
+    if (const char *e = getenv("CMS2D_MG_TILE")) {   // fixed width (tests)
+        const int t = atoi(e);
+        if (t == 64 || t == 128 || t == 256) { F->mg_tile = t; return 0; }
+    }
+    MgLevel L0 = F->mg.L[0];
+    L0.zo = F->Kpp.zp;
+    const double *b0 = F->Kpp.p[0];
+    HIPCHK(hipMemsetAsync(F->d_sc, 0, sizeof(KryScal), F->st));   // done = 0: the launches work
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreate(&ev[0]));
+    HIPCHK(hipEventCreate(&ev[1]));
+    float best = 0.0f;
+    int pick = F->mg_tile;
+    for (int t : cand) {
+        F->mg_tile = t;
+        mg_level0(F, nullptr, true, L0, b0, F->d_sc);          // warm-up
+        mg_level0(F, nullptr, false, L0, b0, F->d_sc);
+        HIPCHK(hipEventRecord(ev[0], F->st));
+        for (int r = 0; r < 4; r++) {
+            mg_level0(F, nullptr, true, L0, b0, F->d_sc);
+            mg_level0(F, nullptr, false, L0, b0, F->d_sc);
+        }
+        HIPCHK(hipEventRecord(ev[1], F->st));
+        HIPCHK(hipEventSynchronize(ev[1]));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        if (t == cand[0] || ms < best) { best = ms; pick = t; }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    F->mg_tile = pick;
+    return 0;
 }
 
 // Inner FP32 solve: relative stop and iteration cap per refinement sweep.
@@ -880,6 +933,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
     LAUNCH(k_zbk, F->nblk_act, g, s);
     if (int rc = init_state(F)) return rc;
+    if (int rc = tune_mg_tile(F)) return rc;
     return capture_graphs(F);
 }
 
@@ -993,6 +1047,13 @@ static double *field_ptr(cms2d_flow *F, const char *name) {
     for (auto &t : tab)
         if (!strcmp(t.n, name)) return t.p;
     return nullptr;
+}
+
+extern "C" int cms2d_flow_tuning(cms2d_flow *F, int *mg_tile_x, int *graphs) {
+    if (!F || !mg_tile_x || !graphs) return set_err(CMS2D_ERR_ARG, "null argument");
+    *mg_tile_x = F->mg.nlev > 1 ? F->mg_tile : 0;
+    *graphs = F->use_graphs ? 1 : 0;
+    return CMS2D_OK;
 }
 
 extern "C" int cms2d_flow_get(cms2d_flow *F, const char *name, double *out) {

@@ -72,35 +72,41 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 }
 
 // --- fused cycle: one launch down and one up per level ------------------------
-// A workgroup owns a tile of MT_X x MT_Y fine cells; each thread owns two
+// A workgroup owns a tile of TX x 2048/TX fine cells; each thread owns two
 // coarse cells of it and their eight children, whose coefficients it loads
 // once into registers and keeps for both phases of the sweep.  Neighbour
 // values come from LDS over the tile and a halo; halo values are recomputed
 // by the same expressions as in the neighbouring tile (mg_black, mg_red), so
-// they are bitwise the values stored there.  At 2000 x 1500 the 1504 tiles of
-// level 0 fit the chip's workgroup slots in one round.
-#ifndef CMS2D_MG_TILE_X
-#define CMS2D_MG_TILE_X 128
-#endif
-constexpr int MT_X = CMS2D_MG_TILE_X, MT_Y = 2048 / CMS2D_MG_TILE_X, MT_THREADS = 256, MT_OWN = 8;
-static_assert(MT_THREADS * 2 == (MT_X / 2) * (MT_Y / 2), "two coarse cells per thread");
+// they are bitwise the values stored there and the result does not depend on
+// the tile shape.  The shape (TX = 64, 128 or 256) is picked per grid by
+// timing the variants when the flow is created.  At 2000 x 1500 the 1504
+// tiles of 128 x 16 fit the chip's workgroup slots in one round.
+constexpr int MT_THREADS = 256, MT_OWN = 8;
+template <int TX>
+struct MTile {
+    static constexpr int X = TX, Y = 2048 / TX, RING = 2 * (X + 2) + 2 * Y;
+    static_assert(MT_THREADS * 2 == (X / 2) * (Y / 2) && Y % 4 == 0, "two coarse cells per thread");
+};
 
 __device__ __forceinline__ bool mg_inside(const MgLevel &L, int i, int j) {
     return i >= 0 && j >= 0 && i < L.nx && j < L.ny;
 }
 // Fine cell k (0..7) of thread t in tile coordinates: coarse cells (cx, cy)
-// and (cx, cy + MT_Y/4), children (sub & 1, sub >> 1).
+// and (cx, cy + Y/4), children (sub & 1, sub >> 1).
+template <int TX>
 __device__ __forceinline__ void mg_own(int t, int k, int &fx, int &fy) {
-    const int cx = t % (MT_X / 2), cy = t / (MT_X / 2) + (k >> 2) * (MT_Y / 4), sub = k & 3;
+    using T = MTile<TX>;
+    const int cx = t % (T::X / 2), cy = t / (T::X / 2) + (k >> 2) * (T::Y / 4), sub = k & 3;
     fx = 2 * cx + (sub & 1);
     fy = 2 * cy + (sub >> 1);
 }
 // Cells of the one-cell ring around the tile, in tile coordinates.
-constexpr int MT_RING = 2 * (MT_X + 2) + 2 * MT_Y;
+template <int TX>
 __device__ __forceinline__ void mg_ring(int t, int &fx, int &fy) {
-    if (t < MT_X + 2) { fx = t - 1; fy = -1; }
-    else if (t < 2 * (MT_X + 2)) { fx = t - (MT_X + 2) - 1; fy = MT_Y; }
-    else { const int u = t - 2 * (MT_X + 2); fx = u < MT_Y ? -1 : MT_X; fy = u % MT_Y; }
+    using T = MTile<TX>;
+    if (t < T::X + 2) { fx = t - 1; fy = -1; }
+    else if (t < 2 * (T::X + 2)) { fx = t - (T::X + 2) - 1; fy = T::Y; }
+    else { const int u = t - 2 * (T::X + 2); fx = u < T::Y ? -1 : T::X; fy = u % T::Y; }
 }
 
 // Black z of the sweep from zero: (b + sum a_k q_nb)/ap = q + (sum a_k q_nb)/ap,
@@ -120,8 +126,9 @@ __device__ __forceinline__ float mg_red(float aN, float aE, float aS, float aW, 
 // Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
 //   q, 1/ap over the tile + 2; z over the tile + 1 in LDS; coarse b = sum over
 //   the red children of sum a_k z_nb (their residual after the sweep).
-template <class TB>
+template <int TX, class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
+    constexpr int MT_X = MTile<TX>::X, MT_Y = MTile<TX>::Y, MT_RING = MTile<TX>::RING;
     if (sc->done) return;
     __shared__ float q[MT_Y + 4][MT_X + 4], dv[MT_Y + 4][MT_X + 4];
     __shared__ float zt[MT_Y + 2][MT_X + 2];
@@ -142,7 +149,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
-        mg_own(threadIdx.x, k, fx, fy);
+        mg_own<TX>(threadIdx.x, k, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
         a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
@@ -158,7 +165,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
     }
     for (int t = threadIdx.x; t < MT_RING; t += MT_THREADS) {
         int fx, fy;
-        mg_ring(t, fx, fy);
+        mg_ring<TX>(t, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
         if (mg_inside(f, i, j)) {
@@ -176,7 +183,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         int fx, fy;
-        mg_own(threadIdx.x, 4 * h, fx, fy);                 // child (0,0): red
+        mg_own<TX>(threadIdx.x, 4 * h, fx, fy);                 // child (0,0): red
         const int I = (X0 + fx) / 2, J = (Y0 + fy) / 2;
         if (I >= C.nx || J >= C.ny) continue;
         const int x = fx + 1, y = fy + 1;                    // zt frame
@@ -192,8 +199,9 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
 // L into L.zo.  w = z + zc(parent) over the tile + 2 and the new red z over
 // the tile + 1 in LDS; the coarse correction of a red cell itself is
 // replaced by its update.
-template <class TB>
+template <int TX, class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, MgLevel C, const KryScal *sc) {
+    constexpr int MT_X = MTile<TX>::X, MT_Y = MTile<TX>::Y, MT_RING = MTile<TX>::RING;
     if (sc->done) return;
     __shared__ float w[MT_Y + 4][MT_X + 4];
     __shared__ float rt[MT_Y + 2][MT_X + 2];
@@ -209,7 +217,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
-        mg_own(threadIdx.x, k, fx, fy);
+        mg_own<TX>(threadIdx.x, k, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
         a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
@@ -229,7 +237,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
     }
     for (int t = threadIdx.x; t < MT_RING; t += MT_THREADS) {
         int fx, fy;
-        mg_ring(t, fx, fy);
+        mg_ring<TX>(t, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
         if (mg_inside(L, i, j) && ((i + j) & 1) == 0) {
@@ -243,7 +251,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
-        mg_own(threadIdx.x, k, fx, fy);
+        mg_own<TX>(threadIdx.x, k, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, x = fx + 1, y = fy + 1;
         if (!mg_inside(L, i, j) || ((i + j) & 1) == 0) continue;
         const float s = (a[k][0] * rt[y + 1][x] + a[k][1] * rt[y][x + 1]) + (a[k][2] * rt[y - 1][x] + a[k][3] * rt[y][x - 1]);

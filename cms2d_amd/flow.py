@@ -96,6 +96,12 @@ class FlowStep:
     def time(self):
         return _lib.lib().cms2d_flow_time(self._h)
 
+    def tuning(self):
+        """(level-0 multigrid tile width, graph replay on) picked at creation."""
+        t, gr = C.c_int(), C.c_int()
+        check(_lib.lib().cms2d_flow_tuning(self._h, C.byref(t), C.byref(gr)))
+        return t.value, bool(gr.value)
+
     def kernel_stats(self, name="spmv_pp"):
         ms, n = C.c_double(), C.c_long()
         check(_lib.lib().cms2d_flow_kernel_stats(self._h, name.encode(), C.byref(ms), C.byref(n)))

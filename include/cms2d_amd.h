@@ -19,7 +19,8 @@
  *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
- * cms2d_flow_kernel_stats (live kernel timing for measurement) and
+ * cms2d_flow_kernel_stats (live kernel timing for measurement),
+ * cms2d_flow_tuning (the launch configuration picked for a flow) and
  * cms2d_ensemble_step (independent domains side by side).
  */
 #ifndef CMS2D_AMD_H
@@ -126,6 +127,12 @@ double cms2d_flow_time(const cms2d_flow *flow);
  * "spmv_pp" (pp-system SpMV v = A z), "mg_down0" and "mg_up0" (multigrid
  * level-0 down and up sweeps).  Measurement only: no reference counterpart. */
 int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
+/* Launch configuration picked when the flow was created: the tile width of
+ * the level-0 multigrid kernels (64, 128 or 256, the fastest when timed on
+ * this grid; 0 without a level 0 cycle) and whether BiCGStab iterations are
+ * replayed from captured graphs (grids up to 1M cells).  Neither changes any
+ * result.  No reference counterpart. */
+int cms2d_flow_tuning(cms2d_flow *flow, int *mg_tile_x, int *graphs);
 
 /* Batched ensemble: advances each of nflows independent flow objects by
  * nsteps time steps.  No counterpart in the reference (which runs one domain

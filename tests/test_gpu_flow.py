@@ -129,3 +129,28 @@ def test_graph_replay_matches_direct_launches(gpu_available, monkeypatch, precis
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_multigrid_tile_widths_match(gpu_available, monkeypatch):
+    """Every level-0 tile width the autotuner can pick gives bitwise the same
+    step (halo values are recomputed by the same expressions), on a grid whose
+    sizes are no multiple of any tile."""
+    c = cases.tidal(nx=301, ny=203, dx=20.0, seed=6)
+    runs = []
+    for tile in ("64", "128", "256"):
+        monkeypatch.setenv("CMS2D_MG_TILE", tile)
+        print(f"tile {tile} tidal 301x203, 6 steps", flush=True)
+        g = FlowStep(c)
+        assert g.tuning()[0] == int(tile)
+        kry = [g.step(1)["kry"] for _ in range(6)]
+        runs.append((kry, [g.get(n) for n in ("eta", "u", "v")]))
+        g.close()
+    for r in runs[1:]:
+        assert r[0] == runs[0][0]
+        for a, b in zip(r[1], runs[0][1]):
+            assert np.array_equal(a, b)
+    monkeypatch.delenv("CMS2D_MG_TILE")
+    g = FlowStep(c)
+    assert g.tuning()[0] in (64, 128, 256)
+    g.close()
