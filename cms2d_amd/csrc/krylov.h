@@ -63,28 +63,31 @@ __device__ __forceinline__ Chunk block_chunk(int n) {
 }
 
 // Cells of this block on the fixed Krylov grid of a structured grid: strips
-// KW_STRIP columns wide split into row bands, one (strip, band) tile per
-// block, consecutive blocks (vertically adjacent tiles, on one XCD) sharing
-// their halo rows in L2; a thread walks one column of its tile.  Every active
-// cell is visited once, whatever the grid size.
+// KW_STRIP columns wide; the rows of a strip go in groups of RP (one pass of
+// the block) to its blocks in turn, so in every pass the blocks of a strip
+// cover one run of consecutive rows.  Consecutive blocks run on one XCD
+// (xcd_block), so the rows above and below a block's rows are the ones its
+// neighbours load at the same time and meet in that XCD's L2; a thread walks
+// one column.  Every active cell is visited once, whatever the grid size.
 constexpr int KW_STRIP = 256;
 template <class F>
 __device__ __forceinline__ void grid_walk(const Geo &g, F &&f) {
     constexpr int RP = KBLOCK / KW_STRIP;                 // rows per pass
     const int G = (int)gridDim.x;
     const int sx = (g.nx + KW_STRIP - 1) / KW_STRIP;
-    const int bands = max(1, G / sx);
+    const int bands = max(1, G / sx);                     // blocks per strip
     const int nbx = G / bands;                            // strips in parallel (>= 1)
     const int b = xcd_block(blockIdx.x, G);
     const int bx = b / bands, by = b % bands;
     if (bx >= nbx) return;
-    const int R = (g.ny + bands - 1) / bands;
-    const int r0 = by * R, r1 = min(g.ny, r0 + R);
+    const int ngroups = (g.ny + RP - 1) / RP;
     const int tx = (int)threadIdx.x % KW_STRIP, ty = (int)threadIdx.x / KW_STRIP;
     for (int s = bx; s < sx; s += nbx) {
         const int col = s * KW_STRIP + tx;
         if (col >= g.nx) continue;
-        for (int row = r0 + ty; row < r1; row += RP) {
+        for (int grp = by; grp < ngroups; grp += bands) {
+            const int row = grp * RP + ty;
+            if (row >= g.ny) continue;
             Idx x;
             x.i = col;
             x.j = row;

@@ -80,7 +80,8 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(MgLevel f, MgLevel C) {
 // they are bitwise the values stored there and the result does not depend on
 // the tile shape.  The shape (TX = 64, 128 or 256) is picked per grid by
 // timing the variants when the flow is created.  At 2000 x 1500 the 1504
-// tiles of 128 x 16 fit the chip's workgroup slots in one round.
+// tiles of 128 x 16 fit the chip's workgroup slots (seven down or eight up
+// workgroups per CU) in one round.
 constexpr int MT_THREADS = 256, MT_OWN = 8;
 template <int TX>
 struct MTile {
@@ -99,6 +100,19 @@ __device__ __forceinline__ void mg_own(int t, int k, int &fx, int &fy) {
     const int cx = t % (T::X / 2), cy = t / (T::X / 2) + (k >> 2) * (T::Y / 4), sub = k & 3;
     fx = 2 * cx + (sub & 1);
     fy = 2 * cy + (sub >> 1);
+}
+// Origin of this workgroup's tile.  Tiles are numbered row-major and dealt
+// to the XCDs in contiguous runs (xcd_block), so a tile and the neighbours
+// whose halo lines it shares mostly run on one XCD and meet in its L2; the
+// hardware's own round-robin would put horizontal neighbours on different
+// XCDs, each fetching the shared lines from HBM.
+template <int TX>
+__device__ __forceinline__ void mg_tile_origin(int &X0, int &Y0) {
+    const int nt = (int)(gridDim.x * gridDim.y);
+    const int t = xcd_block((int)(blockIdx.y * gridDim.x + blockIdx.x), nt);
+    const int ty = t / (int)gridDim.x, tx = t - ty * (int)gridDim.x;
+    X0 = tx * MTile<TX>::X;
+    Y0 = ty * MTile<TX>::Y;
 }
 // Cells of the one-cell ring around the tile, in tile coordinates.
 template <int TX>
@@ -124,25 +138,28 @@ __device__ __forceinline__ float mg_red(float aN, float aE, float aS, float aW, 
 }
 
 // Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
-//   q, 1/ap over the tile + 2; z over the tile + 1 in LDS; coarse b = sum over
+//   q = b/ap over the tile + 2, z over the tile + 1 in LDS; coarse b = sum over
 //   the red children of sum a_k z_nb (their residual after the sweep).
 template <int TX, class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
     constexpr int MT_X = MTile<TX>::X, MT_Y = MTile<TX>::Y, MT_RING = MTile<TX>::RING;
     if (sc->done) return;
-    __shared__ float q[MT_Y + 4][MT_X + 4], dv[MT_Y + 4][MT_X + 4];
+    // 1/ap of the black cells is read again from global memory (a cache hit
+    // after the staging below) rather than kept in LDS: without that second
+    // array the workgroup fits seven times on a CU, and the 1504 tiles of a
+    // 2000 x 1500 grid run in one round instead of two
+    __shared__ float q[MT_Y + 4][MT_X + 4];
     __shared__ float zt[MT_Y + 2][MT_X + 2];
-    const int X0 = blockIdx.x * MT_X, Y0 = blockIdx.y * MT_Y;
+    int X0, Y0;
+    mg_tile_origin<TX>(X0, Y0);
     for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
         const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
-        float v = 0.0f, d = 0.0f;
+        float v = 0.0f;
         if (mg_inside(f, i, j)) {
             const int c = f.idx(i, j);
-            d = f.dinv[c];
-            v = (float)b[c] * d;
+            v = (float)b[c] * f.dinv[c];
         }
         q[ly][lx] = v;
-        dv[ly][lx] = d;
     }
     __syncthreads();
     float a[MT_OWN][4];
@@ -158,7 +175,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
             a[k][0] = f.aN[c]; a[k][1] = f.aE[c]; a[k][2] = f.aS[c]; a[k][3] = f.aW[c];
             if (((i + j) & 1) == 0) z = q[ly][lx];
             else z = mg_black(a[k][0], a[k][1], a[k][2], a[k][3], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
-                              q[ly][lx - 1], q[ly][lx], dv[ly][lx]);
+                              q[ly][lx - 1], q[ly][lx], f.dinv[c]);
             f.z[c] = z;
         }
         zt[fy + 1][fx + 1] = z;
@@ -174,7 +191,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
             } else {
                 const int c = f.idx(i, j);
                 z = mg_black(f.aN[c], f.aE[c], f.aS[c], f.aW[c], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
-                             q[ly][lx - 1], q[ly][lx], dv[ly][lx]);
+                             q[ly][lx - 1], q[ly][lx], f.dinv[c]);
             }
         }
         zt[fy + 1][fx + 1] = z;
@@ -205,7 +222,8 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
     if (sc->done) return;
     __shared__ float w[MT_Y + 4][MT_X + 4];
     __shared__ float rt[MT_Y + 2][MT_X + 2];
-    const int X0 = blockIdx.x * MT_X, Y0 = blockIdx.y * MT_Y;
+    int X0, Y0;
+    mg_tile_origin<TX>(X0, Y0);
     for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
         const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
         float v = 0.0f;

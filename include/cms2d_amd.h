@@ -58,7 +58,7 @@ extern "C" {
 #define CMS2D_PREC_FP32_REFINE 1  /* FP32 Krylov + SpMV inside FP64 iterative refinement     */
 
 typedef struct {
-    int nx, ny;                   /* active cells per row / per column (row-major from SW)   */
+    int nx, ny;                   /* active cells per row / per column (row-major from SW), >= 3 */
     double dx, dy;                /* uniform cell size [m]                                  */
     const double *zb;             /* bed elevation [nx*ny], host memory [m]                 */
     const double *mann;           /* Manning n [nx*ny] (MANNINGS_N_DATASET)                 */

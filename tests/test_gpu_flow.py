@@ -85,6 +85,29 @@ def test_tidal_multigrid_levels_matches_oracle(gpu_available):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(3, 3), (3, 40), (40, 3), (3, 5), (7, 3)],
+                         ids=lambda s: f"{s[0]}x{s[1]}")
+def test_small_shapes_match_oracle(gpu_available, shape):
+    """The smallest grids accepted (three cells a side): hierarchies of one-
+    and two-cell levels, tiles and Krylov strips mostly empty, one Krylov
+    block."""
+    c = cases.tidal(nx=shape[0], ny=shape[1], dx=20.0, seed=7)
+    o, g = run_pair(c, 20)
+    err = compare(o, g)
+    print("max err", err)
+
+
+@pytest.mark.parametrize("shape", [(1, 40), (40, 1), (2, 2), (1, 1)], ids=lambda s: f"{s[0]}x{s[1]}")
+def test_grids_under_three_cells_a_side_rejected(shape):
+    """cms2d_flow_create refuses grids with fewer than three cells a side
+    (CMS2D_ERR_ARG) before touching a device."""
+    c = cases.tidal(nx=shape[0], ny=shape[1], dx=20.0, seed=7)
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(c)
+    assert e.value.code == -1                       # CMS2D_ERR_ARG
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["basin", "tidal"])
 def test_gpu_matches_reference_vectors(gpu_available, name):
     """The HIP path against the compiled reference's own outputs."""
