@@ -389,7 +389,10 @@ static int tune_mg_tile(cms2d_flow *F) {
 }
 
 // Inner FP32 solve: relative stop and iteration cap per refinement sweep.
-static constexpr double REF_RTOL = 1e-5;
+#ifndef CMS2D_REF_RTOL
+#define CMS2D_REF_RTOL 1e-5
+#endif
+static constexpr double REF_RTOL = CMS2D_REF_RTOL;
 static constexpr int REF_KMAX = 200;
 static constexpr int REF_SWEEPS = 20;
 

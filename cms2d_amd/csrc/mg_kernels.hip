@@ -303,12 +303,10 @@ __device__ __forceinline__ float mg_red_res(const MgLevel &L, const TB *b, int i
     return (L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW);
 }
 
-// Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
+// Down, coarse cell (I, J): pre-smoothing sweep of its children on level f
+// (stored to f.z when store) and their restricted residual, returned.
 template <class TB>
-__global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
-    const bool done = sc->done;   // tested before the stores: its load overlaps the others
-    int I, J, cc;
-    if (!mg_thread(C, I, J, cc)) return;
+__device__ __forceinline__ float mg_down_cell(const MgLevel &f, const TB *b, int I, int J, bool store) {
     const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
     const bool hi = i1 < f.nx, hj = j1 < f.ny;
     // black children (i1,j0), (i0,j1) and the black neighbours of the red ones
@@ -324,25 +322,67 @@ __global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, Mg
         const float zbE = mg_black0(f, b, i1 + 1, j1), zbN = mg_black0(f, b, i1, j1 + 1);
         r += mg_red_res(f, b, i1, j1, zbN, zbE, zb10, zb01);
     }
-    if (done) return;
-    f.z[c00] = z00;
-    if (hi) f.z[f.idx(i1, j0)] = zb10;
-    if (hj) f.z[f.idx(i0, j1)] = zb01;
-    if (hi && hj) f.z[f.idx(i1, j1)] = z11;
-    C.b[cc] = r;
+    if (store) {
+        f.z[c00] = z00;
+        if (hi) f.z[f.idx(i1, j0)] = zb10;
+        if (hj) f.z[f.idx(i0, j1)] = zb01;
+        if (hi && hj) f.z[f.idx(i1, j1)] = z11;
+    }
+    return r;
+}
+
+// Down: pre-smoothing sweep of level f (z of its cells) and coarse b.
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_mg_down_pc(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
+    const bool done = sc->done;   // tested before the stores: its load overlaps the others
+    int I, J, cc;
+    if (!mg_thread(C, I, J, cc)) return;
+    const float r = mg_down_cell(f, b, I, J, !done);
+    if (!done) C.b[cc] = r;
 }
 
 // Red z after the coarse correction and the red half-sweep, or 0 outside.
-// zd: z of the down sweep, zc: coarse correction (on C).
+// zd: z of the down sweep, zc: coarse correction (padded layout of C, in
+// global memory or LDS).
 template <class TB>
-__device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const MgLevel &C, int i, int j) {
+__device__ __forceinline__ float mg_red1(const MgLevel &L, const TB *b, const MgLevel &C, const float *zc, int i,
+                                         int j) {
     if (i < 0 || j < 0 || i >= L.nx || j >= L.ny) return 0.0f;
     const int c = L.idx(i, j);
     const int pN = C.idx(i >> 1, min(j + 1, L.ny - 1) >> 1), pS = C.idx(i >> 1, max(j - 1, 0) >> 1);
     const int pE = C.idx(min(i + 1, L.nx - 1) >> 1, j >> 1), pW = C.idx(max(i - 1, 0) >> 1, j >> 1);
-    const float s = (L.aN[c] * (L.z[c + L.nxp] + C.zo[pN]) + L.aE[c] * (L.z[c + 1] + C.zo[pE])) +
-                    (L.aS[c] * (L.z[c - L.nxp] + C.zo[pS]) + L.aW[c] * (L.z[c - 1] + C.zo[pW]));
+    const float s = (L.aN[c] * (L.z[c + L.nxp] + zc[pN]) + L.aE[c] * (L.z[c + 1] + zc[pE])) +
+                    (L.aS[c] * (L.z[c - L.nxp] + zc[pS]) + L.aW[c] * (L.z[c - 1] + zc[pW]));
     return ((float)b[c] + s) * L.dinv[c];
+}
+
+// Up, coarse cell (I, J): coarse correction and post-smoothing sweep (red,
+// then black) of its children on level L into L.zo (when store).
+template <class TB>
+__device__ __forceinline__ void mg_up_cell(const MgLevel &L, const TB *b, const MgLevel &C, const float *zc, int I,
+                                           int J, bool store) {
+    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
+    const bool hi = i1 < L.nx, hj = j1 < L.ny;
+    const float r00 = mg_red1(L, b, C, zc, i0, j0);
+    const float r11 = (hi && hj) ? mg_red1(L, b, C, zc, i1, j1) : 0.0f;
+    float z10 = 0.0f, z01 = 0.0f;
+    if (hi) {
+        // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
+        const int c = L.idx(i1, j0);
+        const float zN = r11, zE = mg_red1(L, b, C, zc, i1 + 1, j0), zS = mg_red1(L, b, C, zc, i1, j0 - 1), zW = r00;
+        z10 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    }
+    if (hj) {
+        // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
+        const int c = L.idx(i0, j1);
+        const float zN = mg_red1(L, b, C, zc, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, zc, i0 - 1, j1);
+        z01 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
+    }
+    if (!store) return;
+    L.zo[L.idx(i0, j0)] = r00;
+    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
+    if (hi) L.zo[L.idx(i1, j0)] = z10;
+    if (hj) L.zo[L.idx(i0, j1)] = z01;
 }
 
 // Up: coarse correction and post-smoothing sweep (red, then black) of level
@@ -352,45 +392,21 @@ __global__ void __launch_bounds__(BLOCK) k_mg_up_pc(MgLevel L, const TB *b, MgLe
     const bool done = sc->done;   // tested before the stores: its load overlaps the others
     int I, J, cc;
     if (!mg_thread(C, I, J, cc)) return;
-    const int i0 = 2 * I, j0 = 2 * J, i1 = i0 + 1, j1 = j0 + 1;
-    const bool hi = i1 < L.nx, hj = j1 < L.ny;
-    const float r00 = mg_red1(L, b, C, i0, j0);
-    const float r11 = (hi && hj) ? mg_red1(L, b, C, i1, j1) : 0.0f;
-    float z10 = 0.0f, z01 = 0.0f;
-    if (hi) {
-        // black (i1,j0): red neighbours (i1,j1) N, (i1+1,j0) E, (i1,j0-1) S, (i0,j0) W
-        const int c = L.idx(i1, j0);
-        const float zN = r11, zE = mg_red1(L, b, C, i1 + 1, j0), zS = mg_red1(L, b, C, i1, j0 - 1), zW = r00;
-        z10 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
-    }
-    if (hj) {
-        // black (i0,j1): red neighbours (i0,j1+1) N, (i1,j1) E, (i0,j0) S, (i0-1,j1) W
-        const int c = L.idx(i0, j1);
-        const float zN = mg_red1(L, b, C, i0, j1 + 1), zE = r11, zS = r00, zW = mg_red1(L, b, C, i0 - 1, j1);
-        z01 = ((float)b[c] + ((L.aN[c] * zN + L.aE[c] * zE) + (L.aS[c] * zS + L.aW[c] * zW))) * L.dinv[c];
-    }
-    if (done) return;
-    L.zo[L.idx(i0, j0)] = r00;
-    if (hi && hj) L.zo[L.idx(i1, j1)] = r11;
-    if (hi) L.zo[L.idx(i1, j0)] = z10;
-    if (hj) L.zo[L.idx(i0, j1)] = z01;
+    mg_up_cell(L, b, C, C.zo, I, J, !done);
 }
 
 // Coarsest grid: MG_COARSE_SWEEPS red-black sweeps from z = 0 in one block
-// with z in LDS (padded layout of the level, ghost ring zero) and each
-// thread's cells (t = thread + k * MG_CT) in registers; result to L.zo.  The
-// cell coordinates are stepped, not stored, to keep the registers for the
-// coefficients.
+// with z in LDS (zs, padded layout of the level, ghost ring zero) and each
+// thread's cells (t = thread + k * MG_CT) in registers; bsrc(c) gives the
+// right-hand side of padded cell c.  The cell coordinates are stepped, not
+// stored, to keep the registers for the coefficients.  Ends with a barrier.
 constexpr int MG_CT = 1024;
 constexpr int MG_CPT = MG_COARSE_MAX / MG_CT;   // cells per thread
-template <class TB>
-__global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, const KryScal *sc) {
-    const bool done = sc->done;   // tested after the loads (uniform over the block)
-    extern __shared__ float zs[];
-    const int n = L.nx * L.ny, npad = L.nxp * (L.ny + 2);
+template <class BSrc>
+__device__ __forceinline__ void mg_coarse_sweeps(const MgLevel &L, BSrc bsrc, float *zs) {
+    const int n = L.nx * L.ny;
     const int j0 = (int)threadIdx.x / L.nx, i0 = (int)threadIdx.x - j0 * L.nx;
     const int dj = MG_CT / L.nx, di_ = MG_CT - dj * L.nx;
-    for (int q = threadIdx.x; q < npad; q += MG_CT) zs[q] = 0.0f;
     float aN[MG_CPT], aE[MG_CPT], aS[MG_CPT], aW[MG_CPT], di[MG_CPT], bb[MG_CPT];
     {
         int i = i0, j = j0;
@@ -401,13 +417,12 @@ __global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, c
                 const int c = L.idx(i, j);
                 aN[k] = L.aN[c]; aE[k] = L.aE[c]; aS[k] = L.aS[c]; aW[k] = L.aW[c];
                 di[k] = L.dinv[c];
-                bb[k] = (float)b[c];
+                bb[k] = bsrc(c);
             }
             i += di_; j += dj;
             if (i >= L.nx) { i -= L.nx; j++; }
         }
     }
-    if (done) return;
     __syncthreads();
     for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
         for (int cl = 0; cl < 2; cl++) {
@@ -425,6 +440,16 @@ __global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, c
             __syncthreads();
         }
     }
+}
+
+// The coarsest grid: result to L.zo.
+template <class TB>
+__global__ void __launch_bounds__(MG_CT) k_mg_coarsest(MgLevel L, const TB *b, const KryScal *sc) {
+    if (sc->done) return;
+    extern __shared__ float zs[];
+    const int n = L.nx * L.ny, npad = L.nxp * (L.ny + 2);
+    for (int q = threadIdx.x; q < npad; q += MG_CT) zs[q] = 0.0f;
+    mg_coarse_sweeps(L, [&](int c) { return (float)b[c]; }, zs);
     for (int t = threadIdx.x; t < n; t += MG_CT) {
         const int j = t / L.nx, i = t - j * L.nx, c = L.idx(i, j);
         L.zo[c] = zs[c];
