@@ -388,13 +388,17 @@ static int tune_mg_tile(cms2d_flow *F) {
     return 0;
 }
 
-// Inner FP32 solve: relative stop and iteration cap per refinement sweep.
+// Inner FP32 solve: relative stop and iteration cap per refinement sweep.  A
+// loose inner stop is the fastest on the tidal workload at 2000 x 1500 (pp
+// iterations per step: 60 at 1e-2, 64 at 1e-3, 78 at 1e-5, where FP32
+// BiCGStab starts to stall); the FP64 sweeps still reach the FP64 solve's
+// own threshold.
 #ifndef CMS2D_REF_RTOL
-#define CMS2D_REF_RTOL 1e-5
+#define CMS2D_REF_RTOL 1e-2
 #endif
 static constexpr double REF_RTOL = CMS2D_REF_RTOL;
 static constexpr int REF_KMAX = 200;
-static constexpr int REF_SWEEPS = 20;
+static constexpr int REF_SWEEPS = 40;
 
 // Multigrid preconditioners of the pp solve (FP64 Krylov vectors) and of the
 // inner FP32 solve.
@@ -1046,10 +1050,70 @@ static double *field_ptr(cms2d_flow *F, const char *name) {
     struct { const char *n; double *p; } tab[] = {
         {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
         {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy}, {"z0", s.z0},
-        {"zb", s.zb}};
+        {"zb", s.zb}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}};
     for (auto &t : tab)
         if (!strcmp(t.n, name)) return t.p;
     return nullptr;
+}
+
+// wave_rad_stress_fl (waves/wave_rad_stress_fl.f90:2-59): the wave fields of
+// the active cells in, the kinematic radiation-stress forcing of the next
+// steps out (k_wave_stress, k_wave_force).
+extern "C" int cms2d_flow_set_waves(cms2d_flow *F, const double *whgt, const double *wper, const double *wlen,
+                                    const double *wunitx, const double *wunity) {
+    if (!F || !whgt || !wper || !wlen || !wunitx || !wunity) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->P.noptset < 3) return set_err(CMS2D_ERR_ARG, "wave forcing needs a wave mode (noptset >= 3)");
+    HIPCHK(hipSetDevice(F->dev));
+    const Geo &g = F->g;
+    const size_t N = F->npad;
+    std::vector<void *> tmp;
+    auto cleanup = [&]() { for (void *p : tmp) (void)hipFree(p); };
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    double *in[4], *S[3], *out[2];
+    for (auto &p : in) p = (double *)alloc(N * sizeof(double));
+    for (auto &p : S) p = (double *)alloc(N * sizeof(double));
+    for (auto &p : out) p = (double *)alloc(N * sizeof(double));
+    int *bad = (int *)alloc(sizeof(int));
+    for (double *p : {in[0], in[1], in[2], in[3], S[0], S[1], S[2], out[0], out[1]})
+        if (!p) { cleanup(); return set_err(CMS2D_ERR_HIP, "device allocation failed"); }
+    if (!bad) { cleanup(); return set_err(CMS2D_ERR_HIP, "device allocation failed"); }
+    const double *src[4] = {whgt, wlen, wunitx, wunity};   // the period only gives c = L/T, unused (:26)
+    int rc = 0;
+    for (int q = 0; q < 4 && !rc; q++) rc = upload_padded(F, in[q], src[q], false);
+    if (!rc) {
+        bool ok = true;
+        for (double *p : {S[0], S[1], S[2], out[0], out[1]})
+            ok = ok && hipMemsetAsync(p, 0, N * sizeof(double), F->st) == hipSuccess;
+        ok = ok && hipMemsetAsync(bad, 0, sizeof(int), F->st) == hipSuccess;
+        if (!ok) rc = set_err(CMS2D_ERR_HIP, "memset failed");
+    }
+    int h_bad = 0;
+    if (!rc) {
+        LAUNCH(k_wave_stress, F->nblk_act, g, F->s, (const double *)in[0], (const double *)in[1],
+               (const double *)in[2], (const double *)in[3], S[0], S[1], S[2]);
+        LAUNCH(k_wave_force, F->nblk_act, g, F->s, (const double *)S[0], (const double *)S[1],
+               (const double *)S[2], out[0], out[1], bad);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&h_bad, bad, sizeof(int), hipMemcpyDeviceToHost, F->st) != hipSuccess ||
+            hipStreamSynchronize(F->st) != hipSuccess)
+            rc = set_err(CMS2D_ERR_HIP, "radiation-stress kernels failed");
+    }
+    if (!rc && h_bad)
+        rc = set_err(CMS2D_ERR_ARG, "radiation-stress forcing is NaN or above 10 at " + std::to_string(h_bad) +
+                                        " cells (wave_rad_stress_fl)");
+    if (!rc) {
+        if (hipMemcpyAsync(F->s.wavestrx, out[0], N * sizeof(double), hipMemcpyDeviceToDevice, F->st) != hipSuccess ||
+            hipMemcpyAsync(F->s.wavestry, out[1], N * sizeof(double), hipMemcpyDeviceToDevice, F->st) != hipSuccess ||
+            hipStreamSynchronize(F->st) != hipSuccess)
+            rc = set_err(CMS2D_ERR_HIP, "copy of the forcing failed");
+    }
+    cleanup();
+    return rc;
 }
 
 extern "C" int cms2d_flow_tuning(cms2d_flow *F, int *mg_tile_x, int *graphs) {

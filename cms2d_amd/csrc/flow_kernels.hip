@@ -784,6 +784,50 @@ __global__ void k_grad_fric_vis(Geo g, St s, Par P) {
     }
 }
 
+// wave_rad_stress_fl (waves/wave_rad_stress_fl.f90:2-59), part 1: the
+// radiation stresses (divided by the density) of each active cell from the
+// wave height H, length L and unit direction (ux, uy) with the current depth:
+//   kh = min(2 pi h / L, 10), n = 1/2 + kh / sinh(2 kh), E = g H^2 / 16,
+//   Sxx = E (n (1 + ux^2) - 1/2), Syy = E (n (1 + uy^2) - 1/2), Sxy = E n uy uy
+// (the reference's uy*uy, :31).  Ghost cells on the grid edge take their
+// cell's values (the reference reads its uninitialised stack array there).
+__global__ void k_wave_stress(Geo g, St s, const double *H, const double *L, const double *ux, const double *uy,
+                              double *sxx, double *sxy, double *syy) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const double twopi = 2.0 * 3.141592653589793;   // const_def twopi = 2 acos(-1), definitions.F90:86-87
+    double kh = twopi / L[c] * s.h[c];
+    kh = fmin(kh, 10.0);
+    const double cn = 0.5 + kh / sinh(2.0 * kh);
+    const double e = 0.0625 * GRAV * H[c] * H[c];
+    const double vxx = e * (cn * (1.0 + ux[c] * ux[c]) - 0.5);
+    const double vyy = e * (cn * (1.0 + uy[c] * uy[c]) - 0.5);
+    const double vxy = e * cn * uy[c] * uy[c];
+    sxx[c] = vxx; sxy[c] = vxy; syy[c] = vyy;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!nb_ghost(g, i, j, k)) continue;
+        const int gc = nbr(g, c, k);
+        sxx[gc] = vxx; sxy[gc] = vxy; syy[gc] = vyy;
+    }
+}
+
+// Part 2: wavestrx = -dSxx/dx - dSxy/dy, wavestry = -dSxy/dx - dSyy/dy with the
+// wet-only gradient; counts components that are NaN or above 10 in bad[0] (the
+// reference's error stop, :42-56).
+__global__ void k_wave_force(Geo g, St s, const double *sxx, const double *sxy, const double *syy, double *fx,
+                             double *fy, int *bad) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    const double *vars[3] = {sxx, sxy, syy};
+    double gx[3], gy[3];
+    cbgg_grad(g, s.iwet, c, 3, vars, gx, gy);
+    const double wx = -gx[0] - gy[1], wy = -gx[1] - gy[2];
+    fx[c] = wx;
+    fy[c] = wy;
+    if (isnan(wx) || isnan(wy) || fabs(wx) > 10.0 || fabs(wy) > 10.0) atomicAdd(bad, 1);
+}
+
 // flow_eddyvis boundary fix-ups (flow_eddyvis.f90:168-191, Cartesian mesh),
 // race-free: every read is of the values before the loop.  A wet cell next to
 // a dry cell or a ghost takes the value of its opposite neighbour; that value

@@ -96,6 +96,14 @@ class FlowStep:
     def time(self):
         return _lib.lib().cms2d_flow_time(self._h)
 
+    def set_waves(self, whgt, wper, wlen, wunitx, wunity):
+        """Radiation-stress forcing from wave fields on the flow grid
+        (cms2d_flow_set_waves); arrays over the active cells, row-major."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (whgt, wper, wlen, wunitx, wunity)]
+        for a in arrs:
+            assert a.size == self.ncells
+        check(_lib.lib().cms2d_flow_set_waves(self._h, *[_dp(a) for a in arrs]))
+
     def tuning(self):
         """(level-0 multigrid tile width, graph replay on) picked at creation."""
         t, gr = C.c_int(), C.c_int()

@@ -17,6 +17,7 @@
  *   cms2d_flow_get/_get_face/_set <- reads/writes of the flow_def module state
  *                       (eta, u, v, p, h, iwet, flux, hk, visk; source/flow/flow_def.f90:32-106)
  *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
+ *   cms2d_flow_set_waves <- wave_rad_stress_fl         source/waves/wave_rad_stress_fl.f90:2-59
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
  * cms2d_flow_kernel_stats (live kernel timing for measurement),
@@ -127,6 +128,18 @@ double cms2d_flow_time(const cms2d_flow *flow);
  * "spmv_pp" (pp-system SpMV v = A z), "mg_down0" and "mg_up0" (multigrid
  * level-0 down and up sweeps).  Measurement only: no reference counterpart. */
 int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
+/* Wave forcing from wave fields on the flow grid: wave_rad_stress_fl
+ * (source/waves/wave_rad_stress_fl.f90:2-59), called where the reference
+ * calls it after interpolating the CMS-Wave fields (wave.F90:125-190).
+ * Inputs per active cell (row-major): significant wave height whgt (m),
+ * period wper (s, unused as in the reference), wave length wlen (m), unit
+ * wave direction (wunitx, wunity).  With the current depth the radiation
+ * stresses and their wet-only gradient give the kinematic forcing used by the
+ * following steps (readable as "wavestrx"/"wavestry").  Needs a wave mode
+ * (noptset >= 3).  CMS2D_ERR_ARG, forcing unchanged, if a component is NaN
+ * or above 10 (the reference's error stop). */
+int cms2d_flow_set_waves(cms2d_flow *flow, const double *whgt, const double *wper, const double *wlen,
+                         const double *wunitx, const double *wunity);
 /* Launch configuration picked when the flow was created: the tile width of
  * the level-0 multigrid kernels (64, 128 or 256, the fastest when timed on
  * this grid; 0 without a level 0 cycle) and whether BiCGStab iterations are

@@ -1345,6 +1345,8 @@ static const double *field(const cms_oracle *o, const char *name)
     if (!strcmp(name, "dpy")) return o->dpy;
     if (!strcmp(name, "cfrict")) return o->cfrict;
     if (!strcmp(name, "uv")) return o->uv;
+    if (!strcmp(name, "wavestrx")) return o->wavestrx;
+    if (!strcmp(name, "wavestry")) return o->wavestry;
     if (!strcmp(name, "bsvel")) return o->bsvel;
     if (!strcmp(name, "zb")) return o->zb;
     if (!strcmp(name, "z0")) return o->z0;
@@ -1409,6 +1411,52 @@ int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy
     der_grad_eval(o, o->gow, v, gx, gy);
     free(v);
     return 0;
+}
+
+/* wave_rad_stress_fl (waves/wave_rad_stress_fl.f90:2-59): the kinematic
+ * radiation-stress forcing from the wave height, period, length and unit
+ * direction on the flow grid (active cells, row-major), with the current depth
+ * and the wet-only gradient operator:
+ *   kh = min(2 pi h / L, 10), n = 1/2 + kh / sinh(2 kh), E = g H^2 / 16,
+ *   Sxx = E (n (1 + ux^2) - 1/2), Syy = E (n (1 + uy^2) - 1/2),
+ *   Sxy = E n uy uy   (the reference's uy*uy, :31, kept as it is),
+ *   wavestrx = -dSxx/dx - dSxy/dy, wavestry = -dSxy/dx - dSyy/dy.
+ * Ghost values copy their cell (the reference reads its uninitialised stack
+ * array there).  Returns -1, leaving the forcing unchanged, on a NaN or a
+ * component above 10 (the reference's error stop, :42-56). */
+int cms_oracle_set_waves(cms_oracle *o, const double *whgt, const double *wper, const double *wlen,
+                         const double *ux, const double *uy)
+{
+    (void)wper;                               /* c = L/T is computed and unused (:26) */
+    const int n = o->ncells;
+    const double twopi = 2.0 * acos(-1.0);    /* const_def twopi, definitions.F90:86-87 */
+    double *sxx = DALLOC(n), *sxy = DALLOC(n), *syy = DALLOC(n);
+    double *g1x = DALLOC(n), *g1y = DALLOC(n), *g2x = DALLOC(n), *g2y = DALLOC(n), *g3x = DALLOC(n),
+           *g3y = DALLOC(n);
+    for (int i = 0; i < n; i++) {
+        double kh = twopi / wlen[i] * o->h[i];
+        kh = fmin(kh, 10.0);
+        const double cn = 0.5 + kh / sinh(2.0 * kh);
+        const double e = 0.0625 * GRAV * whgt[i] * whgt[i];
+        sxx[i] = e * (cn * (1.0 + ux[i] * ux[i]) - 0.5);
+        syy[i] = e * (cn * (1.0 + uy[i] * uy[i]) - 0.5);
+        sxy[i] = e * cn * uy[i] * uy[i];
+    }
+    cms_oracle_grad(o, sxx, g1x, g1y);
+    cms_oracle_grad(o, sxy, g2x, g2y);
+    cms_oracle_grad(o, syy, g3x, g3y);
+    int bad = 0;
+    for (int i = 0; i < n; i++) {
+        const double fx = -g1x[i] - g2y[i], fy = -g2x[i] - g3y[i];
+        if (isnan(fx) || isnan(fy) || fabs(fx) > 10.0 || fabs(fy) > 10.0) bad = 1;
+    }
+    if (!bad)
+        for (int i = 0; i < n; i++) {
+            o->wavestrx[i] = -g1x[i] - g2y[i];
+            o->wavestry[i] = -g2x[i] - g3y[i];
+        }
+    free(sxx); free(sxy); free(syy); free(g1x); free(g1y); free(g2x); free(g2y); free(g3x); free(g3y);
+    return bad ? -1 : 0;
 }
 
 int cms_oracle_solve(int ncells, int ncellsD, int nf, const double *acoef, const int *c2c,

@@ -87,6 +87,11 @@ double cms_oracle_drag2length(double cd, double h);
  * operator (der_grad_cbgg + der_grad_eval, der_lib.f90:272-378,726-839). */
 int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy);
 
+/* Radiation-stress forcing from wave fields, wave_rad_stress_fl
+ * (waves/wave_rad_stress_fl.f90:2-59); -1 on NaN or |forcing| > 10. */
+int cms_oracle_set_waves(cms_oracle *o, const double *whgt, const double *wper, const double *wlen,
+                         const double *ux, const double *uy);
+
 /* Stand-alone linear solve on a general cell list, the reference's
  * solve(acoef,ss,sp,res,phi,n) (solvers.f90:205-304) with the tight stopping rule.
  * nf = nmaxfaces (4 on Cartesian grids, 6 on telescoping grids, read_tel,

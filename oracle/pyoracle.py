@@ -61,6 +61,7 @@ def lib():
             f.argtypes = ([C.c_int] if fn == "cms_oracle_schmcoef" else []) + \
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
+        L.cms_oracle_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
         L.cms_oracle_solve.restype = C.c_int
         L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
                                        C.c_double, C.c_double, C.c_int, D]
@@ -139,6 +140,12 @@ class Oracle:
         gx, gy = np.empty(self.ncells), np.empty(self.ncells)
         lib().cms_oracle_grad(self.h, _dp(a), _dp(gx), _dp(gy))
         return gx, gy
+
+    def set_waves(self, whgt, wper, wlen, wunitx, wunity):
+        """Radiation-stress forcing from wave fields (active cells); False on
+        the reference's error (NaN or a component above 10)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (whgt, wper, wlen, wunitx, wunity)]
+        return lib().cms_oracle_set_waves(self.h, *[_dp(a) for a in arrs]) == 0
 
     def volume(self):
         v, b = C.c_double(), C.c_double()
