@@ -102,6 +102,7 @@ struct cms2d_flow {
     KryScal *h_sc = nullptr;   // pinned mirror
     int *h_flags = nullptr;    // pinned
     std::vector<void *> allocs;
+    std::vector<std::pair<void *, size_t>> state;   // device arrays of the model state (checkpointed)
     int npad = 0, nblk_act = 0, nblk_pad = 0, nblk_kry = 0;
     int ntc = 0;
     double amp[8]{}, speed[8]{}, phase[8]{};
@@ -831,14 +832,20 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                           &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
                           &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
                           &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1};
-    for (double **f : dfields)
+    for (double **f : dfields) {
         if (int rc = dalloc(F, f, N)) return rc;
+        F->state.push_back({*f, N * sizeof(double)});
+    }
     if (int rc = dalloc(F, &s.wseadj, 4 * (size_t)P.lmax)) return rc;
+    F->state.push_back({s.wseadj, 4 * (size_t)P.lmax * sizeof(double)});
     uint8_t **bfields[] = {&s.iwet, &s.iwet1, &s.icorner, &s.ipond};
-    for (uint8_t **f : bfields)
+    for (uint8_t **f : bfields) {
         if (int rc = dalloc(F, f, N)) return rc;
+        F->state.push_back({*f, N});
+    }
     if (int rc = dalloc(F, &s.flags, 4)) return rc;
     if (int rc = dalloc(F, &s.plab, N)) return rc;
+    F->state.push_back({s.plab, N * sizeof(int)});
     {
         // Krylov partial sums: 5 slots per system (pr: 2, pv: 1, pt: 2)
         double *red;
@@ -1114,6 +1121,91 @@ extern "C" int cms2d_flow_set_waves(cms2d_flow *F, const double *whgt, const dou
     }
     cleanup();
     return rc;
+}
+
+// Exact checkpoint: a header, the host-side time and solver state, then every
+// state array in allocation order.  Scratch (Krylov vectors, multigrid levels,
+// partial sums) is rebuilt by each step and not saved.
+namespace {
+struct CkHeader {
+    char magic[8];
+    int version, nx, ny, npad, lmax, narrays;
+    long long bytes;
+};
+struct CkHost {
+    Par P;
+    double timesecs, rampdur, ramp, dtimebeg, deltime;
+    double rmom[3], rmom0[3], rmom1[3];
+    int dtvar, jtime, mtime, ntime, maxit, niter, iconv;
+    int kry[3], prev_iters[4], prev_gm;
+};
+const char CK_MAGIC[8] = {'C', 'M', 'S', '2', 'D', 'C', 'K', '1'};
+}  // namespace
+
+static size_t ck_size(const cms2d_flow *F) {
+    size_t n = sizeof(CkHeader) + sizeof(CkHost);
+    for (auto &a : F->state) n += a.second;
+    return n;
+}
+
+extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
+    if (!F || !nbytes) return set_err(CMS2D_ERR_ARG, "null argument");
+    const size_t need = ck_size(F);
+    if (!buf) { *nbytes = need; return CMS2D_OK; }
+    if (*nbytes < need) return set_err(CMS2D_ERR_ARG, "checkpoint buffer too small: " + std::to_string(need) + " bytes needed");
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamSynchronize(F->st));
+    char *q = (char *)buf;
+    CkHeader h{};
+    memcpy(h.magic, CK_MAGIC, 8);
+    h.version = 1; h.nx = F->g.nx; h.ny = F->g.ny; h.npad = F->npad; h.lmax = F->P.lmax;
+    h.narrays = (int)F->state.size(); h.bytes = (long long)need;
+    memcpy(q, &h, sizeof h);
+    q += sizeof h;
+    CkHost c{};
+    c.P = F->P;
+    c.timesecs = F->timesecs; c.rampdur = F->rampdur; c.ramp = F->ramp; c.dtimebeg = F->dtimebeg; c.deltime = F->deltime;
+    for (int n = 0; n < 3; n++) { c.rmom[n] = F->rmom[n]; c.rmom0[n] = F->rmom0[n]; c.rmom1[n] = F->rmom1[n]; c.kry[n] = F->kry[n]; }
+    c.dtvar = F->dtvar; c.jtime = F->jtime; c.mtime = F->mtime; c.ntime = F->ntime; c.maxit = F->maxit;
+    c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
+    for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
+    memcpy(q, &c, sizeof c);
+    q += sizeof c;
+    for (auto &a : F->state) {
+        HIPCHK(hipMemcpy(q, a.first, a.second, hipMemcpyDeviceToHost));
+        q += a.second;
+    }
+    *nbytes = need;
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes) {
+    if (!F || !buf) return set_err(CMS2D_ERR_ARG, "null argument");
+    const size_t need = ck_size(F);
+    CkHeader h{};
+    if (nbytes < sizeof h) return set_err(CMS2D_ERR_ARG, "not a checkpoint");
+    memcpy(&h, buf, sizeof h);
+    if (memcmp(h.magic, CK_MAGIC, 8) || h.version != 1) return set_err(CMS2D_ERR_ARG, "not a checkpoint of this library");
+    if (h.nx != F->g.nx || h.ny != F->g.ny || h.npad != F->npad || h.lmax != F->P.lmax ||
+        h.narrays != (int)F->state.size() || (size_t)h.bytes != need || nbytes < need)
+        return set_err(CMS2D_ERR_ARG, "checkpoint of a different grid");
+    const char *q = (const char *)buf + sizeof h;
+    CkHost c{};
+    memcpy(&c, q, sizeof c);
+    q += sizeof c;
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamSynchronize(F->st));
+    for (auto &a : F->state) {
+        HIPCHK(hipMemcpy(a.first, q, a.second, hipMemcpyHostToDevice));
+        q += a.second;
+    }
+    F->P = c.P;
+    F->timesecs = c.timesecs; F->rampdur = c.rampdur; F->ramp = c.ramp; F->dtimebeg = c.dtimebeg; F->deltime = c.deltime;
+    for (int n = 0; n < 3; n++) { F->rmom[n] = c.rmom[n]; F->rmom0[n] = c.rmom0[n]; F->rmom1[n] = c.rmom1[n]; F->kry[n] = c.kry[n]; }
+    F->dtvar = c.dtvar; F->jtime = c.jtime; F->mtime = c.mtime; F->ntime = c.ntime; F->maxit = c.maxit;
+    F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
+    for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
+    return CMS2D_OK;
 }
 
 extern "C" int cms2d_flow_tuning(cms2d_flow *F, int *mg_tile_x, int *graphs) {

@@ -104,6 +104,19 @@ class FlowStep:
             assert a.size == self.ncells
         check(_lib.lib().cms2d_flow_set_waves(self._h, *[_dp(a) for a in arrs]))
 
+    def checkpoint(self):
+        """Exact checkpoint of the device state (bytes), cms2d_flow_checkpoint."""
+        n = C.c_size_t(0)
+        check(_lib.lib().cms2d_flow_checkpoint(self._h, None, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(_lib.lib().cms2d_flow_checkpoint(self._h, buf, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def restore(self, data):
+        """Continues from a checkpoint of a flow on the same grid."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(_lib.lib().cms2d_flow_restore(self._h, buf, len(data)))
+
     def tuning(self):
         """(level-0 multigrid tile width, graph replay on) picked at creation."""
         t, gr = C.c_int(), C.c_int()

@@ -96,10 +96,32 @@ module cms2d_hip
             integer(c_int), intent(out) :: iters
             integer(c_int) :: rc
         end function
+        ! hot start, exact-resume variant (hot_start.F90:1512-1560 write, :693-1344 read)
+        function cms2d_flow_checkpoint(flow, buf, nbytes) bind(C, name='cms2d_flow_checkpoint') result(rc)
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: flow, buf
+            integer(c_size_t), intent(inout) :: nbytes
+            integer(c_int) :: rc
+        end function
+        function cms2d_flow_restore(flow, buf, nbytes) bind(C, name='cms2d_flow_restore') result(rc)
+            import :: c_int, c_ptr, c_size_t
+            type(c_ptr), value :: flow, buf
+            integer(c_size_t), value :: nbytes
+            integer(c_int) :: rc
+        end function
+        ! wave_rad_stress_fl (waves/wave_rad_stress_fl.f90:2-59)
+        function cms2d_flow_set_waves(flow, whgt, wper, wlen, wunitx, wunity) &
+                bind(C, name='cms2d_flow_set_waves') result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: flow
+            real(c_double), intent(in) :: whgt(*), wper(*), wlen(*), wunitx(*), wunity(*)
+            integer(c_int) :: rc
+        end function
     end interface
 
     public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
     public :: cms2d_error_message, cms2d_device_count
+    public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves
 
 contains
 

@@ -18,6 +18,8 @@
  *                       (eta, u, v, p, h, iwet, flux, hk, visk; source/flow/flow_def.f90:32-106)
  *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
  *   cms2d_flow_set_waves <- wave_rad_stress_fl         source/waves/wave_rad_stress_fl.f90:2-59
+ *   cms2d_flow_checkpoint/_restore <- hot start write/read (exact-resume variant)
+ *                       source/hot_start/hot_start.F90:693-1122,1125-1344,1512-1560
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
  * cms2d_flow_kernel_stats (live kernel timing for measurement),
@@ -26,6 +28,7 @@
  */
 #ifndef CMS2D_AMD_H
 #define CMS2D_AMD_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -128,6 +131,18 @@ double cms2d_flow_time(const cms2d_flow *flow);
  * "spmv_pp" (pp-system SpMV v = A z), "mg_down0" and "mg_up0" (multigrid
  * level-0 down and up sweeps).  Measurement only: no reference counterpart. */
 int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *avg_ms, long *count);
+/* Exact checkpoint of a flow: every array of the model state (both time
+ * levels, fluxes, wet/dry flags, boundary set-up, forcing) and the time and
+ * time-step control, so that cms2d_flow_restore into a flow created from the
+ * same description continues bitwise as the original would.  The reference's
+ * hot start (hot_write_xmdf, source/hot_start/hot_start.F90:1512-1560; read
+ * :693-1122 and hot_init :1125-1344) stores p, eta, u, v (the fluxes too in
+ * its DEV_MODE build) and rebuilds the rest, which cms2d_flow_set does here;
+ * this is the exact-resume variant.  buf = NULL: *nbytes receives the size. */
+int cms2d_flow_checkpoint(cms2d_flow *flow, void *buf, size_t *nbytes);
+/* CMS2D_ERR_ARG if buf is not a checkpoint of a flow on the same grid. */
+int cms2d_flow_restore(cms2d_flow *flow, const void *buf, size_t nbytes);
+
 /* Wave forcing from wave fields on the flow grid: wave_rad_stress_fl
  * (source/waves/wave_rad_stress_fl.f90:2-59), called where the reference
  * calls it after interpolating the CMS-Wave fields (wave.F90:125-190).
