@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a copy kernel of known
 # size) on this workload at 2000 x 1500; the commands and the table are in
 # profiles/traffic_tidal_2000x1500.md.
-TRAFFIC_PER_CELL = {"spmv_pp": (173.0 + 24.5) * 1e6 / 3.0e6,
-                    "mg_down0": (139.6 + 20.0) * 1e6 / 3.0e6,
-                    "mg_up0": (179.1 + 25.5) * 1e6 / 3.0e6}
+TRAFFIC_PER_CELL = {"spmv_pp": (166.8 + 24.5) * 1e6 / 3.0e6,
+                    "mg_down0": (107.8 + 17.7) * 1e6 / 3.0e6,
+                    "mg_up0": (114.3 + 23.1) * 1e6 / 3.0e6}
 GROUPS = {
     "spmv_pp": (60, "k_bicg_spmv<StencilOp,1,1,1> (pp system, v = A z, z from the multigrid cycle)"),
     "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
