@@ -637,6 +637,7 @@ static int flow_imp_once(cms2d_flow *F) {
     StencilOp A{g, s.aN, s.aE, s.aS, s.aW};
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
+        if (F->P.ndsch >= 5 && F->P.ndsch <= 9) LAUNCH(k_defcor, F->nblk_act, g, s, F->P);
         LAUNCH(k_mom, F->nblk_act, g, s, F->P);
         hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->Kuv);
         {

@@ -30,10 +30,14 @@ namespace cms {
 __device__ __constant__ int c_kopp[4] = {KS, KW, KN, KE};
 
 // ---------------------------------------------------------------- helpers
-// comp_lib.f90:54-109
+// comp_lib.f90:40-109 (ndsch 0 zero, 1 upwind, 2 hybrid, 3 power law,
+// 4 exponential; every other scheme takes upwind coefficients,
+// flow_imp.F90:120-126)
 __device__ __forceinline__ double schmcoef(int ndsch, double dk, double fk) {
     double absfk = fabs(fk), peck;
     switch (ndsch) {
+    case 0:
+        return 0.0;
     case 2:
         peck = absfk / fmax(dk, F4(1.0e-6));
         return dk * fmax(0.0, 1.0 - 0.5 * peck) + fmax(0.0, -fk);
@@ -392,6 +396,87 @@ __global__ void k_srcsnk(Geo g, St s, Par P) {
 
 // coeffsourcesink_uv (flow_imp.F90:444-566) + bound_uv (boundaries.F90:3310-3490)
 // with bnd_vel_openwse (bnd_vel.f90:35-95) for harmonic water-level sides.
+// NVD deferred corrections on a Cartesian grid (flow_imp.F90:140-147):
+// defcorhlpavec (ndsch 5, computational.F90:337-414) and defcorgammavec
+// (ndsch 6-9, :495-581) with the limiters of comp_lib.f90:112-208.  The
+// deferred flux of a face is evaluated from its owner (the S/W cell, which
+// lists it in kxyface) with the owner's outward flux: C the upwind cell, D
+// the downwind one, U the cell upwind of C (a ghost included for the gamma
+// family; HLPA skips the face when U is a ghost or dry).
+__device__ __forceinline__ double nvd_gamma(int ndsch, double x) {
+    switch (ndsch) {
+    case 6: return (x > 0.0 && x <= 0.5) ? 2.0 * x : (x > 0.5 && x <= 1.0) ? 1.0 : 0.0;
+    case 7:
+        return (x > 0.0 && x <= 0.375)  ? 1.5 * x / (1.0 - x)
+               : (x > 0.375 && x <= 0.75) ? (3.0 - 2.0 * x) / (4.0 - 4.0 * x)
+               : (x > 0.75 && x <= 1.0)   ? 1.5
+                                          : 0.0;
+    case 8:
+        return (x > 0.0 && x <= 0.25)  ? 2.5 * x / (1.0 - x)
+               : (x > 0.25 && x <= 0.75) ? (3.0 - 2.0 * x) / (4.0 - 4.0 * x)
+               : (x > 0.75 && x <= 1.0)  ? 1.5
+                                         : 0.0;
+    case 9:
+        return (x > 0.0 && x <= F4(0.1667))  ? 5.0 * x / (1.0 - x)
+               : (x > F4(0.1667) && x <= 0.5) ? 1.0
+               : (x > 0.5 && x <= 0.75)       ? 0.5 / (1.0 - x)
+               : (x > 0.75 && x <= 1.0)       ? 2.0
+                                              : 0.0;
+    default: return 0.0;
+    }
+}
+// Deferred flux (u, v) of face k (KN or KE) owned by cell (i, j).
+__device__ __forceinline__ void nvd_face(const Geo &g, const St &s, int ndsch, int i, int j, int k, double &fu,
+                                         double &fv) {
+    fu = fv = 0.0;
+    const int c = cidx(g, i, j);
+    if (!s.iwet[c] || nb_ghost(g, i, j, k)) return;
+    const int nck = nbr(g, c, k);
+    if (!s.iwet[nck]) return;
+    const double F = k == KN ? s.fluxN[c] : s.fluxE[c];
+    int cc, cd, cu;
+    bool u_ghost;
+    if (F >= 0.0) {
+        cc = c; cd = nck; cu = nbr(g, c, c_kopp[k]);
+        u_ghost = nb_ghost(g, i, j, c_kopp[k]);
+    } else {
+        const int ni = k == KE ? i + 1 : i, nj = k == KN ? j + 1 : j;
+        cc = nck; cd = c; cu = nbr(g, nck, k);
+        u_ghost = nb_ghost(g, ni, nj, k);
+    }
+    if (ndsch == 5 && (u_ghost || !s.iwet[cu])) return;
+    const double tiny = F4(1.0e-20);
+    const double uc = s.u[cc], ud = s.u[cd], uu = s.u[cu];
+    const double vc = s.v[cc], vd = s.v[cd], vu = s.v[cu];
+    if (fabs(ud - uu) > tiny) {
+        const double un = (uc - uu) / (ud - uu);
+        if (un > 0.0 && un < 1.0) {
+            if (ndsch == 5) fu = F * (ud - uc) * un;
+            else fu = F * nvd_gamma(ndsch, un) * ((g.fintp[k] * s.u[nck] + (1.0 - g.fintp[k]) * s.u[c]) - uc);
+        }
+    }
+    if (fabs(vd - vu) > tiny) {
+        const double vn = (vc - vu) / (vd - vu);
+        if (vn > 0.0 && vn < 1.0) {
+            if (ndsch == 5) fv = F * (vd - vc) * vn;
+            else fv = F * nvd_gamma(ndsch, vn) * ((g.fintp[k] * s.v[nck] + (1.0 - g.fintp[k]) * s.v[c]) - vc);
+        }
+    }
+}
+// Sums over each cell's faces (N, E, S, W, the reference's face order) into
+// tu, tv; k_mom subtracts them from su, sv.
+__global__ void k_defcor(Geo g, St s, Par P) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    double nu, nv, eu, ev, su_ = 0.0, sv_ = 0.0, wu = 0.0, wv = 0.0;
+    nvd_face(g, s, P.ndsch, i, j, KN, nu, nv);
+    nvd_face(g, s, P.ndsch, i, j, KE, eu, ev);
+    if (j > 0) { nvd_face(g, s, P.ndsch, i, j - 1, KN, su_, sv_); su_ = -su_; sv_ = -sv_; }
+    if (i > 0) { nvd_face(g, s, P.ndsch, i - 1, j, KE, wu, wv); wu = -wu; wv = -wv; }
+    s.tu[c] = ((nu + eu) + su_) + wu;
+    s.tv[c] = ((nv + ev) + sv_) + wv;
+}
+
 __global__ void k_mom(Geo g, St s, Par P) {
     int i, j, c;
     if (!act_thread(g, i, j, c)) return;
@@ -399,6 +484,10 @@ __global__ void k_mom(Geo g, St s, Par P) {
     double u = s.u[c], v = s.v[c], h = s.h[c];
     double su = s.ssu0[c] + (fc * v - s.dpx[c]) * h * A;
     double sv = s.ssv0[c] + (-fc * u - s.dpy[c]) * h * A;
+    if (P.ndsch >= 5 && P.ndsch <= 9) {   // deferred corrections of k_defcor
+        su = su - s.tu[c];
+        sv = sv - s.tv[c];
+    }
     double cbc = 1.0 * s.cfrict[c] * s.uelwc[c] * A;          // cmb=1 (no bed-slope factor)
     double sp = s.spuv0[c] - cbc;
     double a[4], hk[4], fl[4], vk[4];

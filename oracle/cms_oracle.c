@@ -107,11 +107,15 @@ static void *xcalloc(size_t n, size_t s) { void *p = calloc(n ? n : 1, s); if (!
  * Closed-form helpers
  * ------------------------------------------------------------------------- */
 
-/* comp_lib.f90:54-109 upwindcoef / hybridcoef / powerlawcoef / exponentialcoef */
+/* comp_lib.f90:40-109 zerocoef / upwindcoef / hybridcoef / powerlawcoef /
+ * exponentialcoef (ndsch 0-4; every other scheme takes upwind coefficients,
+ * flow_imp.F90:120-126) */
 double cms_oracle_schmcoef(int ndsch, double dk, double fk)
 {
     double absfk = fabs(fk), peck;
     switch (ndsch) {
+    case 0:
+        return 0.0;
     case 2:
         peck = absfk / fmax(dk, F4(1.0e-6));
         return dk * fmax(0.0, 1.0 - 0.5 * peck) + fmax(0.0, -fk);
@@ -123,6 +127,37 @@ double cms_oracle_schmcoef(int ndsch, double dk, double fk)
         return dk * peck / fmax(exp(peck) - 1.0, F4(1.0e-6)) + fmax(0.0, -fk);
     default:
         return dk + fmax(0.0, -fk);
+    }
+}
+
+/* NVD deferred-correction limiters gamma(phi~) of the gamma family,
+ * comp_lib.f90:112-208 (ndsch 6 GAMMA, 7 CUBISTA, 8 ALVSMART, 9 HOAB; the
+ * reference's 0.1667 is a single-precision literal). */
+double cms_oracle_nvd_gamma(int ndsch, double phin)
+{
+    switch (ndsch) {
+    case 6:
+        if (phin > 0.0 && phin <= 0.5) return 2.0 * phin;
+        if (phin > 0.5 && phin <= 1.0) return 1.0;
+        return 0.0;
+    case 7:
+        if (phin > 0.0 && phin <= 0.375) return 1.5 * phin / (1.0 - phin);
+        if (phin > 0.375 && phin <= 0.75) return (3.0 - 2.0 * phin) / (4.0 - 4.0 * phin);
+        if (phin > 0.75 && phin <= 1.0) return 1.5;
+        return 0.0;
+    case 8:
+        if (phin > 0.0 && phin <= 0.25) return 2.5 * phin / (1.0 - phin);
+        if (phin > 0.25 && phin <= 0.75) return (3.0 - 2.0 * phin) / (4.0 - 4.0 * phin);
+        if (phin > 0.75 && phin <= 1.0) return 1.5;
+        return 0.0;
+    case 9:
+        if (phin > 0.0 && phin <= F4(0.1667)) return 5.0 * phin / (1.0 - phin);
+        if (phin > F4(0.1667) && phin <= 0.5) return 1.0;
+        if (phin > 0.5 && phin <= 0.75) return 0.5 / (1.0 - phin);
+        if (phin > 0.75 && phin <= 1.0) return 2.0;
+        return 0.0;
+    default:
+        return 0.0;
     }
 }
 
@@ -532,6 +567,82 @@ static void coeffsourcesink_uv(cms_oracle *o)
             o->acoef[f] = cms_oracle_schmcoef(o->c.ndsch, ddk, o->flux[f]);
         }
     }
+}
+
+/* Deferred anti-diffusion corrections of the NVD schemes on a Cartesian grid
+ * (flow_imp.F90:140-147): defcorhlpavec (ndsch 5, computational.F90:337-414)
+ * and defcorgammavec (ndsch 6-9, :495-581).  Each face is visited once by the
+ * cell listing it in kxyface; with C the upwind cell, D the downwind one and U
+ * the cell upwind of C, phi~ = (phiC - phiU)/(phiD - phiU) and, for
+ * 0 < phi~ < 1, the deferred normal flux is F (phiD - phiC) phi~ (HLPA) or
+ * F gamma(phi~) (phif - phiC) with the central phif (gamma family); the
+ * neighbour gets its negative.  HLPA skips a face whose U is a ghost or dry,
+ * the gamma family reads U whatever it is (ghost values included).  Returns
+ * the sums over each cell's faces in dsu, dsv (su -= dsu, sv -= dsv). */
+static void defcor_nvd(const cms_oracle *o, double *dsu, double *dsv)
+{
+    const int nd = o->ncellsD, nc = o->ncells, ndsch = o->c.ndsch;
+    double *uf = DALLOC(4 * (size_t)nd), *vf = DALLOC(4 * (size_t)nd);
+    const double tiny = F4(1.0e-20);
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] == 0) continue;
+        for (int j = 0; j < o->nxy[i]; j++) {
+            const int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            if (nck >= nc || o->iwet[nck] == 0) continue;
+            const int jcn = o->llec[f];
+            int nckk;
+            double uc, ud, uu, vc, vd, vu;
+            if (o->flux[f] >= 0.0) {
+                nckk = o->c2c[4 * i + kopp[k]];
+                uc = o->u[i]; ud = o->u[nck];
+                vc = o->v[i]; vd = o->v[nck];
+            } else {
+                nckk = o->c2c[4 * nck + k];
+                uc = o->u[nck]; ud = o->u[i];
+                vc = o->v[nck]; vd = o->v[i];
+            }
+            if (ndsch == 5 && (nckk >= nc || o->iwet[nckk] == 0)) continue;
+            uu = o->u[nckk]; vu = o->v[nckk];
+            if (fabs(ud - uu) > tiny) {
+                const double un = (uc - uu) / (ud - uu);
+                if (un > 0.0 && un < 1.0) {
+                    if (ndsch == 5) uf[f] = o->flux[f] * (ud - uc) * un;
+                    else {
+                        const double ufc = o->fintp[f] * o->u[nck] + (1.0 - o->fintp[f]) * o->u[i];
+                        uf[f] = o->flux[f] * cms_oracle_nvd_gamma(ndsch, un) * (ufc - uc);
+                    }
+                    uf[4 * nck + jcn] = -uf[f];
+                }
+            }
+            if (fabs(vd - vu) > tiny) {
+                const double vn = (vc - vu) / (vd - vu);
+                if (vn > 0.0 && vn < 1.0) {
+                    if (ndsch == 5) vf[f] = o->flux[f] * (vd - vc) * vn;
+                    else {
+                        const double vfc = o->fintp[f] * o->v[nck] + (1.0 - o->fintp[f]) * o->v[i];
+                        vf[f] = o->flux[f] * cms_oracle_nvd_gamma(ndsch, vn) * (vfc - vc);
+                    }
+                    vf[4 * nck + jcn] = -vf[f];
+                }
+            }
+        }
+    }
+    for (int i = 0; i < nc; i++) {
+        dsu[i] = ((uf[4 * i] + uf[4 * i + 1]) + uf[4 * i + 2]) + uf[4 * i + 3];
+        dsv[i] = ((vf[4 * i] + vf[4 * i + 1]) + vf[4 * i + 2]) + vf[4 * i + 3];
+    }
+    free(uf); free(vf);
+}
+
+int cms_oracle_defcor(cms_oracle *o, double *dsu, double *dsv)
+{
+    if (o->c.ndsch < 5 || o->c.ndsch > 9) {
+        memset(dsu, 0, sizeof(double) * o->ncells);
+        memset(dsv, 0, sizeof(double) * o->ncells);
+        return 0;
+    }
+    defcor_nvd(o, dsu, dsv);
+    return 0;
 }
 
 /* bound_uv, boundaries.F90:3310-3490, with bnd_vel_openwse (bnd_vel.f90:35-95) */
@@ -1132,6 +1243,12 @@ restart:
     o->kry[0] = o->kry[1] = o->kry[2] = 0;
     for (o->niter = 1; o->niter <= o->maxit; o->niter++) {
         coeffsourcesink_uv(o);
+        if (o->c.ndsch >= 5 && o->c.ndsch <= 9) {   /* deferred corrections, flow_imp.F90:140-147 */
+            double *dsu = DALLOC(o->ncells), *dsv = DALLOC(o->ncells);
+            defcor_nvd(o, dsu, dsv);
+            for (int i = 0; i < o->ncells; i++) { o->su[i] -= dsu[i]; o->sv[i] -= dsv[i]; }
+            free(dsu); free(dsv);
+        }
         bound_uv(o);
         o->kry[1] += solve_sys(o, o->su, o->spu, o->rsu, o->u, 2, &o->rmom[1]);
         o->kry[2] += solve_sys(o, o->sv, o->spv, o->rsv, o->v, 3, &o->rmom[2]);

@@ -87,6 +87,12 @@ double cms_oracle_drag2length(double cd, double h);
  * operator (der_grad_cbgg + der_grad_eval, der_lib.f90:272-378,726-839). */
 int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy);
 
+/* NVD limiter gamma(phi~) of ndsch 6-9 (comp_lib.f90:112-208) and the
+ * deferred-correction source sums of the current state for ndsch 5-9
+ * (computational.F90:337-414,495-581; zero for other schemes). */
+double cms_oracle_nvd_gamma(int ndsch, double phin);
+int cms_oracle_defcor(cms_oracle *o, double *dsu, double *dsv);
+
 /* Radiation-stress forcing from wave fields, wave_rad_stress_fl
  * (waves/wave_rad_stress_fl.f90:2-59); -1 on NaN or |forcing| > 10. */
 int cms_oracle_set_waves(cms_oracle *o, const double *whgt, const double *wper, const double *wlen,

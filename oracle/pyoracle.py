@@ -62,6 +62,9 @@ def lib():
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
         L.cms_oracle_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
+        L.cms_oracle_nvd_gamma.restype = C.c_double
+        L.cms_oracle_nvd_gamma.argtypes = [C.c_int, C.c_double]
+        L.cms_oracle_defcor.argtypes = [C.c_void_p, D, D]
         L.cms_oracle_solve.restype = C.c_int
         L.cms_oracle_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double,
                                        C.c_double, C.c_double, C.c_int, D]
@@ -146,6 +149,12 @@ class Oracle:
         the reference's error (NaN or a component above 10)."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (whgt, wper, wlen, wunitx, wunity)]
         return lib().cms_oracle_set_waves(self.h, *[_dp(a) for a in arrs]) == 0
+
+    def defcor(self):
+        """Deferred-correction sums (dsu, dsv) of the current state."""
+        du, dv = np.empty(self.ncells), np.empty(self.ncells)
+        lib().cms_oracle_defcor(self.h, _dp(du), _dp(dv))
+        return du, dv
 
     def volume(self):
         v, b = C.c_double(), C.c_double()

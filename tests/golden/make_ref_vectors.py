@@ -25,13 +25,25 @@ CASES = {
     "basin": (lambda: cases.basin(nx=40, ny=20, dx=10.0, seed=0), 30, [1, 10, 20, 30]),
     "tidal": (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1), 40, [1, 10, 20, 40]),
 }
+# the NVD deferred corrections (ndsch 5 HLPA, 7 CUBISTA, 9 HOAB) on the tidal case
+for _nd, _nm in ((5, "hlpa"), (7, "cubista"), (9, "hoab")):
+    def _mk(nd=_nd):
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.ndsch = nd
+        return c
+    CASES[f"tidal_{_nm}"] = (_mk, 40, [1, 10, 20, 40])
+
+
+def only(names):
+    return {k: v for k, v in CASES.items() if k in names}
 
 
 def main():
     if not refcase.available():
         print("oracle/_ref/ref_run not built (python oracle/build_ref.py)", file=sys.stderr)
         return 1
-    for name, (mk, nsteps, keep) in CASES.items():
+    todo = only(sys.argv[1:]) if len(sys.argv) > 1 else CASES
+    for name, (mk, nsteps, keep) in todo.items():
         c = mk()
         with tempfile.TemporaryDirectory() as d:
             res = refcase.run(c, d + "/", nsteps)

@@ -21,6 +21,11 @@ def available():
     return os.path.exists(REF_RUN)
 
 
+# advection scheme names of the ADVECTION_SCHEME card, in ndsch order
+# (source/main/definitions.F90:167-172)
+ADVSC = ["NONE", "UPWIND", "HYBRID", "POWERLAW", "EXPONENTIAL", "HLPA", "GAMMA", "CUBISTA", "ALVSMART", "HOAB"]
+
+
 def write_case(case, d, name="case", solver="BICGSTAB"):
     os.makedirs(d, exist_ok=True)
     nx, ny = case.nx, case.ny
@@ -64,6 +69,7 @@ DURATION_RUN                        {max(1.0, case.dt * case.nsteps / 3600.0):.9
 VARIABLE_TIMESTEP                   OFF
 SOLUTION_SCHEME                     IMPLICIT
 MATRIX_SOLVER                       {solver}
+{"ADVECTION_SCHEME                    " + ADVSC[case.ndsch] if case.ndsch != 4 else ""}
 MANNINGS_N_DATASET                  "{name}_mann.txt" "ManningsN"
 {"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
 OUTPUT_FILE_TYPE                    ASCII

@@ -108,7 +108,21 @@ def test_grids_under_three_cells_a_side_rejected(shape):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["basin", "tidal"])
+@pytest.mark.parametrize("ndsch", [0, 1, 2, 3, 5, 6, 7, 8, 9])
+def test_advection_schemes_match_oracle(gpu_available, ndsch):
+    """Every ADVECTION_SCHEME the Cartesian path has (the default 4 is in the
+    other tests): none, upwind, hybrid, power law, and upwind plus the NVD
+    deferred corrections (HLPA, GAMMA, CUBISTA, ALVSMART, HOAB), on the
+    wet/dry tidal beach."""
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.ndsch = ndsch
+    o, g = run_pair(c, 40)
+    err = compare(o, g)
+    print("max err", err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab"])
 def test_gpu_matches_reference_vectors(gpu_available, name):
     """The HIP path against the compiled reference's own outputs."""
     from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold

@@ -59,6 +59,15 @@ def test_zero_diffusion_zero_flux_gives_zero_coefficient():
         assert lib().cms_oracle_schmcoef(nd, 0.0, 0.0) == 0.0
 
 
+def test_scheme_none_and_nvd_coefficients():
+    """ADVECTION_SCHEME NONE has zero coefficients (zerocoef); the NVD and TVD
+    schemes (ndsch >= 5) take upwind coefficients (flow_imp.F90:120-126)."""
+    for dk, fk in ((1.0, 0.5), (2.0, -7.0)):
+        assert lib().cms_oracle_schmcoef(0, dk, fk) == 0.0
+        for nd in (5, 9, 12, 18):
+            assert lib().cms_oracle_schmcoef(nd, dk, fk) == dk + max(-fk, 0.0)
+
+
 # ---------------------------------------------------------------- friction
 def test_manning_drag_unit_depth_and_caps():
     g, n = 9.81, f4(0.025)
@@ -206,3 +215,45 @@ def test_serial_and_race_free_modes_agree_closely():
         b.step(1)
     assert np.array_equal(a.get("iwet"), b.get("iwet"))
     assert np.abs(a.get("eta") - b.get("eta")).max() < 1e-8
+
+
+# ---------------------------------------------------------------- NVD schemes
+def test_nvd_limiters_published_forms():
+    """gamma(phi~) of the gamma family against the published piecewise forms:
+    GAMMA (Jasak, Weller and Gosman 1999), CUBISTA (Alves, Oliveira and Pinho
+    2003), ALVSMART (Przulj and Basara 2001), HOAB (Wei et al. 2003), as the
+    reference writes them (comp_lib.f90:112-208, 0.1667 a single-precision
+    literal); zero outside (0, 1]."""
+    g = lambda nd, x: lib().cms_oracle_nvd_gamma(nd, x)
+    for x, want in ((0.2, 0.4), (0.5, 1.0), (0.7, 1.0), (1.0, 1.0), (0.0, 0.0), (1.2, 0.0)):
+        assert g(6, x) == pytest.approx(want, abs=1e-15)
+    for x, want in ((0.3, 1.5 * 0.3 / 0.7), (0.375, 1.5 * 0.375 / 0.625), (0.5, 2.0 / 2.0), (0.6, 1.8 / 1.6),
+                    (0.8, 1.5), (-0.1, 0.0)):
+        assert g(7, x) == pytest.approx(want, rel=1e-15)
+    for x, want in ((0.2, 2.5 * 0.2 / 0.8), (0.25, 2.5 * 0.25 / 0.75), (0.5, 1.0), (0.9, 1.5)):
+        assert g(8, x) == pytest.approx(want, rel=1e-15)
+    for x, want in ((0.1, 5.0 * 0.1 / 0.9), (0.3, 1.0), (0.6, 0.5 / 0.4), (0.8, 2.0)):
+        assert g(9, x) == pytest.approx(want, rel=1e-15)
+    # the HOAB breakpoint is the single-precision 0.1667
+    b = f4(0.1667)
+    assert g(9, b) == pytest.approx(5.0 * b / (1.0 - b), rel=1e-15)
+    assert g(9, np.nextafter(b, 1.0)) == 1.0
+
+
+@pytest.mark.parametrize("ndsch", [5, 6, 7, 8, 9])
+def test_nvd_deferred_correction_is_conservative(ndsch):
+    """Each face's deferred flux enters its two cells with opposite signs, so
+    the corrections sum to zero over the grid; they vanish for ndsch 1-4."""
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.ndsch = ndsch
+    o = Oracle(c)
+    for _ in range(5):
+        assert o.step(1)[0] == 0
+    du, dv = o.defcor()
+    assert np.abs(du).max() > 0 and np.abs(dv).max() > 0
+    assert abs(du.sum()) <= 1e-12 * np.abs(du).sum()
+    assert abs(dv.sum()) <= 1e-12 * np.abs(dv).sum()
+    c.ndsch = 4
+    o4 = Oracle(c)
+    o4.step(2)
+    assert not np.any(o4.defcor()[0])
