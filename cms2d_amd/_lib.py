@@ -32,7 +32,7 @@ EXPORTS = [
     "cms2d_last_error", "cms2d_device_count", "cms2d_flow_create", "cms2d_flow_destroy",
     "cms2d_flow_step", "cms2d_flow_get", "cms2d_flow_get_face", "cms2d_flow_set",
     "cms2d_flow_time", "cms2d_flow_kernel_stats", "cms2d_flow_tuning", "cms2d_flow_set_waves",
-    "cms2d_flow_checkpoint", "cms2d_flow_restore",
+    "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
     "cms2d_ensemble_step", "cms2d_solve",
 ]
 
@@ -47,6 +47,7 @@ class FlowDesc(C.Structure):
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
         ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int), ("dtvar", C.c_int),
+        ("latitude", C.c_double), ("coriolis_plane", C.c_int),
     ]
 
 
@@ -86,6 +87,7 @@ def lib():
     L.cms2d_flow_kernel_stats.argtypes = [C.c_void_p, C.c_char_p, D, C.POINTER(C.c_long)]
     L.cms2d_flow_tuning.argtypes = [C.c_void_p, I, I]
     L.cms2d_flow_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
+    L.cms2d_flow_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.cms2d_flow_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_size_t)]
     L.cms2d_flow_restore.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     L.cms2d_ensemble_step.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int,

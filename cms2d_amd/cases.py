@@ -58,6 +58,17 @@ class Case:
     noptset: int = 2          # >=3: wave branches (radiation-stress forcing) active
     wallfric: int = 1
     ndsch: int = 4            # exponential scheme
+    latitude: float = 0.0     # AVERAGE_LATITUDE, degrees (Coriolis)
+    coriolis_plane: int = 0   # CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane
+    wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
+    wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
+
+    def wind_vector(self):
+        """(wndx, wndy) in grid coordinates as windcurve_init makes them
+        (meteorological.F90:700-710, grid angle 0, 10-m height, factor 1):
+        direction + 180 degrees, x = W sin, y = W cos."""
+        d = np.deg2rad(self.wind_dir) + np.pi
+        return self.wind_speed * np.sin(d), self.wind_speed * np.cos(d)
 
     @property
     def ncells(self) -> int:

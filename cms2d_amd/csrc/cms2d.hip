@@ -121,6 +121,8 @@ struct cms2d_flow {
     bool use_graphs = false;
     hipGraphExec_t kgraph[4]{};
     int mg_tile = 128;         // level-0 tile width of the multigrid cycle (timed at creation)
+    int windconst = 0;         // spatially constant wind (met_def windconst) and its 10-m vector
+    double wndx = 0.0, wndy = 0.0;
     hipEvent_t step_ev[2]{};
 };
 
@@ -549,6 +551,16 @@ static int capture_graphs(cms2d_flow *F) {
     return 0;
 }
 
+// wind_drag_hsu (meteorological/met_lib.f90:33-66): Hsu (1988), reduced above
+// 30 m/s after Powell et al. (2003); wkappa = 0.4 (met_def)
+static double wind_drag_hsu(double w) {
+    if (w <= 30.0) {
+        const double t = F4(0.4) / (F4(14.56) - 2.0 * std::log(std::max(w, F4(0.01))));
+        return t * t;
+    }
+    return std::max(F4(3.86e-3) - F4(4.0e-5) * w, F4(0.0015));
+}
+
 static double ramp_func(double timehrs, double rampdur) {
     return 0.5 - 0.5 * std::cos(M_PI * std::min(timehrs, rampdur) / rampdur);
 }
@@ -633,6 +645,14 @@ static int flow_imp_once(cms2d_flow *F) {
     for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
     const double wsebnd = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);     // bnd_eval, boundaries.F90:4109-4122
     if (int rc = wetdry(F)) return rc;
+    F->P.tauwx = F->P.tauwy = 0.0;
+    if (F->windconst) {
+        // windcurve_eval (meteorological.F90:1020-1027), called each step in flow_imp (:85)
+        const double w = std::sqrt(F->wndx * F->wndx + F->wndy * F->wndy);
+        const double val = F->ramp * wind_drag_hsu(w) * F4(0.00120) * w;
+        F->P.tauwx = val * F->wndx;
+        F->P.tauwy = val * F->wndy;
+    }
     LAUNCH(k_srcsnk, F->nblk_act, g, s, F->P);
     StencilOp A{g, s.aN, s.aE, s.aS, s.aW};
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
@@ -769,7 +789,8 @@ static int init_state(cms2d_flow *F) {
 
 static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
-        d->ntc > 8 || !(d->dt > 0))
+        d->ntc > 8 || !(d->dt > 0) || !(std::fabs(d->latitude) <= 90.0) || d->coriolis_plane < 0 ||
+        d->coriolis_plane > 1)
         return set_err(CMS2D_ERR_ARG, "invalid flow description");
     if ((long)(d->nx + 2) * (d->ny + 2) > (1L << 30)) return set_err(CMS2D_ERR_ARG, "grid too large");
     for (int k = 0; k < 4; k++)
@@ -832,7 +853,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                           &s.spv, &s.ssu0, &s.ssv0, &s.spuv0, &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu,
                           &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
                           &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
-                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1};
+                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1, &s.fc};
     for (double **f : dfields) {
         if (int rc = dalloc(F, f, N)) return rc;
         F->state.push_back({*f, N * sizeof(double)});
@@ -946,6 +967,24 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (int rc = upload_padded(F, s.coefman, d->mann, false)) return rc;
     if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
+    {
+        // Coriolis, flow.F90:720-736: f-plane 2 omega sin(lat); beta-plane adds
+        // 2 omega cos(lat)/R (y - mean y), y the cell centres from the south
+        // edge; omega = 2 pi/86400, R = 6371 km (definitions.F90:86-96)
+        const double pi = std::acos(-1.0), omega = 2.0 * pi / 86400.0, deg2rad = pi / 180.0;
+        const double f0 = 2.0 * omega * std::sin(d->latitude * deg2rad);
+        std::vector<double> fc((size_t)g.nx * g.ny, f0);
+        if (d->coriolis_plane == 1) {
+            const double beta = 2.0 * omega * std::cos(d->latitude * deg2rad) / 6371000.0;
+            double ysum = 0.0;
+            for (int j = 0; j < g.ny; j++)
+                for (int i = 0; i < g.nx; i++) ysum += (j + 1) * g.dy - 0.5 * g.dy;
+            const double yavg = ysum / (double)((long)g.nx * g.ny);
+            for (int j = 0; j < g.ny; j++)
+                for (int i = 0; i < g.nx; i++) fc[(size_t)j * g.nx + i] = f0 + beta * (((j + 1) * g.dy - 0.5 * g.dy) - yavg);
+        }
+        if (int rc = upload_padded(F, s.fc, fc.data(), false)) return rc;
+    }
     LAUNCH(k_zbk, F->nblk_act, g, s);
     if (int rc = init_state(F)) return rc;
     if (int rc = tune_mg_tile(F)) return rc;
@@ -1058,7 +1097,7 @@ static double *field_ptr(cms2d_flow *F, const char *name) {
     struct { const char *n; double *p; } tab[] = {
         {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
         {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy}, {"z0", s.z0},
-        {"zb", s.zb}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}};
+        {"zb", s.zb}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}, {"fc", s.fc}};
     for (auto &t : tab)
         if (!strcmp(t.n, name)) return t.p;
     return nullptr;
@@ -1139,6 +1178,8 @@ struct CkHost {
     double rmom[3], rmom0[3], rmom1[3];
     int dtvar, jtime, mtime, ntime, maxit, niter, iconv;
     int kry[3], prev_iters[4], prev_gm;
+    int windconst;
+    double wndx, wndy;
 };
 const char CK_MAGIC[8] = {'C', 'M', 'S', '2', 'D', 'C', 'K', '1'};
 }  // namespace
@@ -1170,6 +1211,7 @@ extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     c.dtvar = F->dtvar; c.jtime = F->jtime; c.mtime = F->mtime; c.ntime = F->ntime; c.maxit = F->maxit;
     c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
     for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
+    c.windconst = F->windconst; c.wndx = F->wndx; c.wndy = F->wndy;
     memcpy(q, &c, sizeof c);
     q += sizeof c;
     for (auto &a : F->state) {
@@ -1206,6 +1248,19 @@ extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes)
     F->dtvar = c.dtvar; F->jtime = c.jtime; F->mtime = c.mtime; F->ntime = c.ntime; F->maxit = c.maxit;
     F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
     for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
+    F->windconst = c.windconst; F->wndx = c.wndx; F->wndy = c.wndy;
+    return CMS2D_OK;
+}
+
+// Spatially constant wind: windcurve_eval's 10-m vector in grid coordinates;
+// each following step adds ramp Cd(W) rho_air/rho W (wndx, wndy) to the
+// momentum sources (flow_srcsnk_static, flow_imp.F90:386-394).
+extern "C" int cms2d_flow_set_wind(cms2d_flow *F, double wndx, double wndy) {
+    if (!F) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (!std::isfinite(wndx) || !std::isfinite(wndy)) return set_err(CMS2D_ERR_ARG, "wind not finite");
+    F->windconst = 1;
+    F->wndx = wndx;
+    F->wndy = wndy;
     return CMS2D_OK;
 }
 

@@ -390,6 +390,10 @@ __global__ void k_srcsnk(Geo g, St s, Par P) {
         su0 = su0 + s.wavestrx[c] * g.area;
         sv0 = sv0 + s.wavestry[c] * g.area;
     }
+    if (P.tauwx != 0.0 || P.tauwy != 0.0) {   // constant wind, flow_imp.F90:386-394
+        su0 = su0 + P.tauwx * g.area;
+        sv0 = sv0 + P.tauwy * g.area;
+    }
     s.ssu0[c] = su0;
     s.ssv0[c] = sv0;
 }
@@ -480,7 +484,7 @@ __global__ void k_defcor(Geo g, St s, Par P) {
 __global__ void k_mom(Geo g, St s, Par P) {
     int i, j, c;
     if (!act_thread(g, i, j, c)) return;
-    const double A = g.area, fc = 0.0;
+    const double A = g.area, fc = s.fc[c];
     double u = s.u[c], v = s.v[c], h = s.h[c];
     double su = s.ssu0[c] + (fc * v - s.dpx[c]) * h * A;
     double sv = s.ssv0[c] + (-fc * u - s.dpy[c]) * h * A;
@@ -628,6 +632,9 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
             int c = cidx(g, ci, cj), c1 = cidx(g, ci1, cj1);
             double fx = 0.0, fx1 = 0.0, fy = 0.0, fy1 = 0.0;
             if (P.noptset >= 3) { fx = s.wavestrx[c]; fx1 = s.wavestrx[c1]; fy = s.wavestry[c]; fy1 = s.wavestry[c1]; }
+            if (P.tauwx != 0.0 || P.tauwy != 0.0) {   // forcex/forcey add the wind stress, boundaries.F90:3978-3982
+                fx = fx + P.tauwx; fx1 = fx1 + P.tauwx; fy = fy + P.tauwy; fy1 = fy1 + P.tauwy;
+            }
             double xd = (ci - ci1) * g.dx, yd = (cj - cj1) * g.dy;
             double sx = xd * (fx + fx1) / (s.h[c] + s.h[c1]) * GRAVINV;
             double sy = yd * (fy + fy1) / (s.h[c] + s.h[c1]) * GRAVINV;

@@ -15,6 +15,7 @@ struct St {
     double *fluxE, *fluxN, *hkE, *hkN, *viskE, *viskN, *zbkE, *zbkN;   // unique faces
     double *aN, *aE, *aS, *aW;                                         // directed coefficients
     double *wavestrx, *wavestry;
+    double *fc;                // Coriolis parameter (flow.F90:720-736)
     double *tu, *tv, *tvis;
     double *wseadj;            // [4*lmax] adjusted boundary water level per side
     uint8_t *iwet, *iwet1, *icorner, *ipond;
@@ -25,6 +26,7 @@ struct St {
 
 struct Par {
     double hmin, hdry, hdry1, hdry2, dtime, c2a, rtol, atol;
+    double tauwx, tauwy;       // constant wind stress / rho of the step (windcurve_eval)
     int kmax, noptset, wallfric, ndsch, lmax;
 };
 

@@ -55,9 +55,12 @@ class FlowStep:
         d.precond = _lib.PRECOND_MG if precond is None else precond
         d.solver = _lib.SOLVER_BICGSTAB if solver is None else solver
         d.dtvar = dtvar
+        d.latitude, d.coriolis_plane = case.latitude, case.coriolis_plane
         h = C.c_void_p()
         check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h
+        if case.wind_speed:
+            self.set_wind(*case.wind_vector())
         self.ncells = case.nx * case.ny
 
     def close(self):
@@ -103,6 +106,11 @@ class FlowStep:
         for a in arrs:
             assert a.size == self.ncells
         check(_lib.lib().cms2d_flow_set_waves(self._h, *[_dp(a) for a in arrs]))
+
+    def set_wind(self, wndx, wndy):
+        """Spatially constant 10-m wind vector [m/s] in grid coordinates
+        (cms2d_flow_set_wind)."""
+        check(_lib.lib().cms2d_flow_set_wind(self._h, float(wndx), float(wndy)))
 
     def checkpoint(self):
         """Exact checkpoint of the device state (bytes), cms2d_flow_checkpoint."""

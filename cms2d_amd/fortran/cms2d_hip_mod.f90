@@ -37,6 +37,8 @@ module cms2d_hip
         real(c_double) :: rtol = 1.0d-10, atol = 1.0d-12
         integer(c_int) :: kmax = 4000, precision = 0, device = 0
         integer(c_int) :: precond = 0, solver = 0, dtvar = 0
+        real(c_double) :: latitude = 0
+        integer(c_int) :: coriolis_plane = 0
     end type
 
     ! struct cms2d_step_info
@@ -109,6 +111,13 @@ module cms2d_hip
             integer(c_size_t), value :: nbytes
             integer(c_int) :: rc
         end function
+        ! windcurve_eval, spatially constant wind (meteorological.F90:995-1033)
+        function cms2d_flow_set_wind(flow, wndx, wndy) bind(C, name='cms2d_flow_set_wind') result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: flow
+            real(c_double), value :: wndx, wndy
+            integer(c_int) :: rc
+        end function
         ! wave_rad_stress_fl (waves/wave_rad_stress_fl.f90:2-59)
         function cms2d_flow_set_waves(flow, whgt, wper, wlen, wunitx, wunity) &
                 bind(C, name='cms2d_flow_set_waves') result(rc)
@@ -121,7 +130,7 @@ module cms2d_hip
 
     public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
     public :: cms2d_error_message, cms2d_device_count
-    public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves
+    public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves, cms2d_flow_set_wind
 
 contains
 

@@ -18,6 +18,7 @@
  *                       (eta, u, v, p, h, iwet, flux, hk, visk; source/flow/flow_def.f90:32-106)
  *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
  *   cms2d_flow_set_waves <- wave_rad_stress_fl         source/waves/wave_rad_stress_fl.f90:2-59
+ *   cms2d_flow_set_wind  <- windcurve_eval (constant wind) source/meteorological/meteorological.F90:995-1033
  *   cms2d_flow_checkpoint/_restore <- hot start write/read (exact-resume variant)
  *                       source/hot_start/hot_start.F90:693-1122,1125-1344,1512-1560
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
@@ -78,7 +79,9 @@ typedef struct {
     double hdry;                  /* DRYING_DEPTH [m] (reference default 0.05)               */
     int noptset;                  /* >= 3: wave branches of friction/eddy viscosity active   */
     int wallfric;                 /* USE_WALL_FRICTION_TERMS                                 */
-    int ndsch;                    /* ADVECTION_SCHEME 1 upwind,2 hybrid,3 power-law,4 exponential */
+    int ndsch;                    /* ADVECTION_SCHEME 0 none, 1 upwind, 2 hybrid, 3 power law,
+                                     4 exponential, 5-9 HLPA, GAMMA, CUBISTA, ALVSMART, HOAB
+                                     (upwind plus deferred anti-diffusion), 10-18 upwind   */
     int maxit;                    /* HYDRO_MAX_ITERATIONS; 0 -> 20 + ncells/10000            */
     double rtol;                  /* Krylov stop: ||r||2 <= max(rtol*||r0||2, atol*sqrt(n))  */
     double atol;
@@ -91,6 +94,9 @@ typedef struct {
                                      time step (flow_recalculate, flow_imp.F90:1133-1242), dt
                                      recovers in timing (CMS_Flow.f90:180-205); 0: divergence
                                      returns CMS2D_ERR_DIVERGED                                   */
+    double latitude;              /* AVERAGE_LATITUDE [deg] for the Coriolis parameter          */
+    int coriolis_plane;           /* CORIOLIS_APPROXIMATION 0 f-plane, 1 beta-plane
+                                     (flow.F90:261-269,720-736)                               */
 } cms2d_flow_desc;
 
 typedef struct {
@@ -142,6 +148,15 @@ int cms2d_flow_kernel_stats(const cms2d_flow *flow, const char *name, double *av
 int cms2d_flow_checkpoint(cms2d_flow *flow, void *buf, size_t *nbytes);
 /* CMS2D_ERR_ARG if buf is not a checkpoint of a flow on the same grid. */
 int cms2d_flow_restore(cms2d_flow *flow, const void *buf, size_t nbytes);
+
+/* Spatially constant wind (the reference's windconst: WIND_INPUT_CURVE /
+ * WIND_SPEED_CURVE + WIND_DIRECTION_CURVE, meteorological.F90:176-202):
+ * the 10-m wind vector in grid coordinates [m/s] as windcurve_init and
+ * windcurve_eval make it from the curve (meteorological.F90:700-710,
+ * 1015-1019; the host interpolates a curve in time and calls this before each
+ * step).  Each step then adds ramp Cd(W) 0.0012 W (wndx, wndy) with Hsu's drag
+ * (met_lib.f90:33-66) to the momentum sources (flow_imp.F90:386-394). */
+int cms2d_flow_set_wind(cms2d_flow *flow, double wndx, double wndy);
 
 /* Wave forcing from wave fields on the flow grid: wave_rad_stress_fl
  * (source/waves/wave_rad_stress_fl.f90:2-59), called where the reference

@@ -75,6 +75,9 @@ struct cms_oracle {
     int nTH, *THc, *THk;
     double *wsebnd, *wseadj;
     int wseadjust;
+    /* spatially constant wind (met_def windconst, wndx, wndy) */
+    int windconst;
+    double wndx, wndy, tauwx, tauwy;   /* the vector and the stress of the current step */
     /* state (flow_def, fric_def, wave_flowgrid_def) */
     double *u, *v, *p, *h, *pp, *dppx, *dppy, *dux, *duy, *dvx, *dvy, *dpx, *dpy;
     double *vis, *zb, *eta, *u1, *v1, *p1, *h1;
@@ -159,6 +162,17 @@ double cms_oracle_nvd_gamma(int ndsch, double phin)
     default:
         return 0.0;
     }
+}
+
+/* wind_drag_hsu (meteorological/met_lib.f90:33-66): Hsu (1988), reduced
+ * above 30 m/s after Powell et al. (2003); wkappa = 0.4 (met_def) */
+double cms_oracle_wind_drag_hsu(double w)
+{
+    if (w <= 30.0) {
+        const double t = F4(0.4) / (F4(14.56) - 2.0 * log(fmax(w, F4(0.01))));
+        return t * t;
+    }
+    return fmax(F4(3.86e-3) - F4(4.0e-5) * w, F4(0.0015));
 }
 
 /* fric_lib.f90:162-194 wall_coef */
@@ -551,6 +565,32 @@ static void flow_srcsnk_static(cms_oracle *o)
             o->ssu0[i] += o->wavestrx[i] * o->area[i];
             o->ssv0[i] += o->wavestry[i] * o->area[i];
         }
+    /* spatially constant wind, Eulerian frame (:386-394), its stress from
+     * windcurve_eval (meteorological.F90:1020-1027), called each step in
+     * flow_imp (:85): tau = ramp Cd rho_air/rho W (wndx, wndy), rho_air/rho
+     * = 0.00120 (met_def) */
+    if (o->windconst) {
+        const double w = sqrt(o->wndx * o->wndx + o->wndy * o->wndy);
+        const double val = o->ramp * cms_oracle_wind_drag_hsu(w) * F4(0.00120) * w;
+        const double tx = val * o->wndx, ty = val * o->wndy;
+        o->tauwx = tx;
+        o->tauwy = ty;
+        for (int i = 0; i < o->ncells; i++) {
+            o->ssu0[i] += tx * o->area[i];
+            o->ssv0[i] += ty * o->area[i];
+        }
+    }
+}
+
+/* Spatially constant wind: the 10-m wind vector (wndx, wndy) in grid
+ * coordinates [m/s], as windcurve_init/windcurve_eval turn the curve's speed
+ * and direction into it (meteorological.F90:700-710,1015-1019). */
+int cms_oracle_set_wind(cms_oracle *o, double wndx, double wndy)
+{
+    o->windconst = 1;
+    o->wndx = wndx;
+    o->wndy = wndy;
+    return 0;
 }
 
 /* coeffsourcesink_uv, flow_imp.F90:409-574 */
@@ -980,6 +1020,9 @@ static void bound_pp(cms_oracle *o)
                 int i = o->THc[im], i1 = o->THc[im - 1], f = 4 * i + o->THk[im];
                 double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
                 double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
+                if (o->windconst) {   /* forcex/forcey add the wind stress, boundaries.F90:3978-3982 */
+                    fx += o->tauwx; fx1 += o->tauwx; fy += o->tauwy; fy1 += o->tauwy;
+                }
                 double sx = (o->x[i] - o->x[i1]) * (fx + fx1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
                 double sy = (o->y[i] - o->y[i1]) * (fy + fy1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
                 if (fabs(o->ry[f]) >= F4(0.999)) ws[im] = ws[im - 1] + sx;
@@ -990,6 +1033,9 @@ static void bound_pp(cms_oracle *o)
                 int i = o->THc[im], i1 = o->THc[im + 1], f = 4 * i + o->THk[im];
                 double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
                 double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
+                if (o->windconst) {   /* forcex/forcey add the wind stress, boundaries.F90:3978-3982 */
+                    fx += o->tauwx; fx1 += o->tauwx; fy += o->tauwy; fy1 += o->tauwy;
+                }
                 double sx = (o->x[i] - o->x[i1]) * (fx + fx1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
                 double sy = (o->y[i] - o->y[i1]) * (fy + fy1) / (o->h[i] + o->h[i1]) * (1.0 / GRAV);
                 if (fabs(o->ry[f]) >= F4(0.999)) ws[im] = ws[im + 1] + sx;
@@ -1335,6 +1381,22 @@ cms_oracle *cms_oracle_create(const cms_case *c)
         o->wavestrx[i] = c->wavestrx ? c->wavestrx[i] : 0.0;
         o->wavestry[i] = c->wavestry ? c->wavestry[i] : 0.0;
     }
+    /* Coriolis, flow.F90:720-736: f-plane fc = 2 omega sin(lat), beta-plane
+     * fc + 2 omega cos(lat)/R (y - mean y); omega = 2 pi/86400, R = 6371 km
+     * (definitions.F90:86-96) */
+    {
+        const double pi = acos(-1.0), omega = 2.0 * pi / 86400.0, deg2rad = pi / 180.0;
+        const double f0 = 2.0 * omega * sin(c->latitude * deg2rad);
+        if (c->coriolis_plane == 1) {
+            const double beta = 2.0 * omega * cos(c->latitude * deg2rad) / 6371000.0;
+            double ysum = 0.0;
+            for (int i = 0; i < nc; i++) ysum += o->y[i];
+            const double yavg = ysum / (double)nc;
+            for (int i = 0; i < nd; i++) o->fc[i] = f0 + beta * (o->y[i] - yavg);
+        } else {
+            for (int i = 0; i < nd; i++) o->fc[i] = f0;
+        }
+    }
     build_bnd(o);
     /* flow_init (flow.F90:592-835) */
     o->dtime = c->dt;
@@ -1462,6 +1524,7 @@ static const double *field(const cms_oracle *o, const char *name)
     if (!strcmp(name, "dpy")) return o->dpy;
     if (!strcmp(name, "cfrict")) return o->cfrict;
     if (!strcmp(name, "uv")) return o->uv;
+    if (!strcmp(name, "fc")) return o->fc;
     if (!strcmp(name, "wavestrx")) return o->wavestrx;
     if (!strcmp(name, "wavestry")) return o->wavestry;
     if (!strcmp(name, "bsvel")) return o->bsvel;

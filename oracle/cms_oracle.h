@@ -48,6 +48,8 @@ typedef struct {
                                  before the loop), the semantics of the HIP path      */
     int dtvar;                /* VARIABLE_TIMESTEP: a divergent step is redone with half
                                  the time step, which recovers later (0: stop with an error) */
+    double latitude;          /* AVERAGE_LATITUDE [deg] (0: no Coriolis on the f-plane)   */
+    int coriolis_plane;       /* CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane         */
 } cms_case;
 
 typedef struct {
@@ -86,6 +88,12 @@ double cms_oracle_drag2length(double cd, double h);
 /* Gradient of an active-cell field (row-major) with the current wet-only CBGG
  * operator (der_grad_cbgg + der_grad_eval, der_lib.f90:272-378,726-839). */
 int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy);
+
+/* Spatially constant wind (windconst): the 10-m wind vector in grid
+ * coordinates [m/s]; the stress of each step is ramp * Cd(W) * 0.0012 * W *
+ * (wndx, wndy) with Hsu's drag (met_lib.f90:33-66). */
+int cms_oracle_set_wind(cms_oracle *o, double wndx, double wndy);
+double cms_oracle_wind_drag_hsu(double w);
 
 /* NVD limiter gamma(phi~) of ndsch 6-9 (comp_lib.f90:112-208) and the
  * deferred-correction source sums of the current state for ndsch 5-9

@@ -26,7 +26,7 @@ class CmsCase(C.Structure):
         ("dt", C.c_double), ("rampdur", C.c_double), ("hdry", C.c_double),
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
-        ("serial", C.c_int), ("dtvar", C.c_int),
+        ("serial", C.c_int), ("dtvar", C.c_int), ("latitude", C.c_double), ("coriolis_plane", C.c_int),
     ]
 
 
@@ -62,6 +62,9 @@ def lib():
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
         L.cms_oracle_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
+        L.cms_oracle_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.cms_oracle_wind_drag_hsu.restype = C.c_double
+        L.cms_oracle_wind_drag_hsu.argtypes = [C.c_double]
         L.cms_oracle_nvd_gamma.restype = C.c_double
         L.cms_oracle_nvd_gamma.argtypes = [C.c_int, C.c_double]
         L.cms_oracle_defcor.argtypes = [C.c_void_p, D, D]
@@ -106,9 +109,12 @@ class Oracle:
         c.maxit, c.solver_mode, c.rtol, c.atol, c.kmax = maxit, solver_mode, rtol, atol, kmax
         c.serial = serial
         c.dtvar = dtvar
+        c.latitude, c.coriolis_plane = case.latitude, case.coriolis_plane
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)
+        if getattr(case, "wind_speed", 0.0):
+            self.set_wind(*case.wind_vector())
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -149,6 +155,10 @@ class Oracle:
         the reference's error (NaN or a component above 10)."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (whgt, wper, wlen, wunitx, wunity)]
         return lib().cms_oracle_set_waves(self.h, *[_dp(a) for a in arrs]) == 0
+
+    def set_wind(self, wndx, wndy):
+        """Spatially constant 10-m wind vector [m/s] in grid coordinates."""
+        lib().cms_oracle_set_wind(self.h, float(wndx), float(wndy))
 
     def defcor(self):
         """Deferred-correction sums (dsu, dsv) of the current state."""

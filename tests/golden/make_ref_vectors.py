@@ -34,6 +34,25 @@ for _nd, _nm in ((5, "hlpa"), (7, "cubista"), (9, "hoab")):
     CASES[f"tidal_{_nm}"] = (_mk, 40, [1, 10, 20, 40])
 
 
+# Coriolis at 45 degrees north, f-plane and beta-plane
+for _pl, _nm in ((0, "fplane"), (1, "betaplane")):
+    def _mk(pl=_pl):
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.latitude, c.coriolis_plane = 45.0, pl
+        return c
+    CASES[f"tidal_{_nm}"] = (_mk, 40, [1, 10, 20, 40])
+
+
+# a constant 12 m/s wind from the south-west over the tidal case
+def _mk_wind():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.wind_speed, c.wind_dir = 12.0, 225.0
+    return c
+
+
+CASES["tidal_wind"] = (_mk_wind, 40, [1, 10, 20, 40])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

@@ -56,6 +56,12 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
         f.write(" ".join(str(v) for v in ids) + "\n")
     harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"
                    for s, a, p in zip(case.speed_deg_hr, case.amp, case.phase_deg))
+    if case.wind_speed:
+        # constant wind as an SMS XYS curve pair (read_xys, input/in_lib.f90:22-65):
+        # hours, speed [m/s] / direction it comes from [deg clockwise from north]
+        for tag, val in (("wspd", case.wind_speed), ("wdir", case.wind_dir)):
+            with open(os.path.join(d, f"{name}_{tag}.xys"), "w") as f:
+                f.write(f"XYS 1 2\n0.0 {val:.9g}\n10000.0 {val:.9g}\n")
     cards = f"""CMS_VERSION                         5.4
 GRID_ANGLE                          0.0
 GRID_ORIGIN_X                       0.0
@@ -70,6 +76,10 @@ VARIABLE_TIMESTEP                   OFF
 SOLUTION_SCHEME                     IMPLICIT
 MATRIX_SOLVER                       {solver}
 {"ADVECTION_SCHEME                    " + ADVSC[case.ndsch] if case.ndsch != 4 else ""}
+{f"AVERAGE_LATITUDE                    {case.latitude:.9g}" if case.latitude else ""}
+{"CORIOLIS_APPROXIMATION              BETA" if case.coriolis_plane == 1 else ""}
+{f'WIND_SPEED_CURVE                    "{name}_wspd.xys"' if case.wind_speed else ""}
+{f'WIND_DIRECTION_CURVE                "{name}_wdir.xys"' if case.wind_speed else ""}
 MANNINGS_N_DATASET                  "{name}_mann.txt" "ManningsN"
 {"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
 OUTPUT_FILE_TYPE                    ASCII

@@ -122,7 +122,25 @@ def test_advection_schemes_match_oracle(gpu_available, ndsch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab"])
+@pytest.mark.parametrize("plane", [0, 1])
+def test_coriolis_matches_oracle(gpu_available, plane):
+    """The Coriolis parameter per cell (f- and beta-plane at 30 N) and the flow
+    it turns, on a basin 20 km across (beta (y - mean y) up to 2 % of f)."""
+    c = cases.tidal(nx=100, ny=60, dx=200.0, seed=3)
+    c.latitude, c.coriolis_plane = 30.0, plane
+    o = Oracle(c, serial=0)
+    g = FlowStep(c)
+    a, b = o.get("fc"), g.get("fc")
+    assert np.abs(a - b).max() <= 1e-15 * np.abs(a).max()
+    print(f"coriolis plane {plane}: fc spread {a.max() - a.min():.2e}", flush=True)
+    g.close()
+    o, g = run_pair(c, 20)
+    print("max err", compare(o, g))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab", "tidal_fplane",
+                                  "tidal_betaplane", "tidal_wind"])
 def test_gpu_matches_reference_vectors(gpu_available, name):
     """The HIP path against the compiled reference's own outputs."""
     from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold

@@ -257,3 +257,44 @@ def test_nvd_deferred_correction_is_conservative(ndsch):
     o4 = Oracle(c)
     o4.step(2)
     assert not np.any(o4.defcor()[0])
+
+
+# ---------------------------------------------------------------- Coriolis
+@pytest.mark.parametrize("plane", [0, 1])
+def test_coriolis_parameter_closed_form(plane):
+    """f = 2 Omega sin(lat) on the f-plane; on the beta-plane plus
+    2 Omega cos(lat)/R (y - mean y), Omega = 2 pi/86400 s, R = 6371 km."""
+    c = cases.basin(nx=40, ny=30, dx=100.0, seed=0)
+    c.latitude, c.coriolis_plane = 30.0, plane
+    o = Oracle(c)
+    om = 2.0 * math.pi / 86400.0
+    f0 = 2.0 * om * math.sin(math.radians(30.0))
+    jj = np.arange(c.ncells) // c.nx
+    y = (jj + 0.5) * c.dy
+    want = f0 + (2.0 * om * math.cos(math.radians(30.0)) / 6371000.0 * (y - y.mean()) if plane else 0.0)
+    assert np.abs(o.get("fc") - want).max() <= 1e-18 + 1e-13 * f0
+
+
+# ---------------------------------------------------------------- wind
+def test_wind_drag_hsu_known_values():
+    """Hsu (1988): Cd = (kappa / (14.56 - 2 ln W))^2 up to 30 m/s, then the
+    Powell et al. (2003) reduction max(3.86e-3 - 4e-5 W, 1.5e-3); the
+    reference's single-precision literals (met_lib.f90:57-62)."""
+    d = lambda w: lib().cms_oracle_wind_drag_hsu(w)
+    for w in (0.5, 5.0, 10.0, 30.0):
+        assert d(w) == pytest.approx((f4(0.4) / (f4(14.56) - 2.0 * math.log(w))) ** 2, rel=1e-15)
+    assert d(10.0) == pytest.approx(1.6146e-3, rel=1e-4)           # textbook value at 10 m/s
+    assert d(0.0) == d(f4(0.01))                                  # speed floor 0.01 m/s
+    assert d(40.0) == pytest.approx(f4(3.86e-3) - f4(4.0e-5) * 40.0, rel=1e-15)
+    assert d(80.0) == f4(0.0015)
+
+
+def test_wind_vector_follows_met_convention():
+    """A wind from 225 degrees (south-west) blows toward +x, +y."""
+    c = cases.basin(nx=10, ny=10)
+    c.wind_speed, c.wind_dir = 10.0, 225.0
+    wx, wy = c.wind_vector()
+    assert wx == pytest.approx(10.0 / math.sqrt(2.0)) and wy == pytest.approx(10.0 / math.sqrt(2.0))
+    c.wind_dir = 270.0                                            # from the west
+    wx, wy = c.wind_vector()
+    assert wx == pytest.approx(10.0) and abs(wy) < 1e-12

@@ -41,6 +41,30 @@ for _name, _nd in NVD.items():
     CASES[_name] = _tidal_scheme(_nd)
 
 
+def _tidal_coriolis(plane):
+    def mk():
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.latitude, c.coriolis_plane = 45.0, plane
+        return c
+    return mk
+
+
+# Coriolis at 45 N (AVERAGE_LATITUDE 45, CORIOLIS_APPROXIMATION F / BETA)
+CORIOLIS = {"tidal_fplane": 0, "tidal_betaplane": 1}
+for _name, _pl in CORIOLIS.items():
+    CASES[_name] = _tidal_coriolis(_pl)
+
+
+def _tidal_wind():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.wind_speed, c.wind_dir = 12.0, 225.0       # WIND_SPEED_CURVE / WIND_DIRECTION_CURVE
+    return c
+
+
+CASES["tidal_wind"] = _tidal_wind
+ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind"]
+
+
 def compare_to_gold(name, model_factory):
     g = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
     m = model_factory(CASES[name]())
@@ -58,7 +82,7 @@ def compare_to_gold(name, model_factory):
     return worst
 
 
-@pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab"])
+@pytest.mark.parametrize("name", ALL)
 def test_oracle_matches_reference_vectors(name):
     w = compare_to_gold(name, lambda c: Oracle(c, serial=1))
     print(name, w)
@@ -82,4 +106,24 @@ def test_nvd_correction_is_what_the_reference_adds(name):
         c.ndsch = 1
         return Oracle(c, serial=1)
     w = compare_to_gold(name, upwind)
+    assert w["u"] > 100 * TOL_VEL or w["v"] > 100 * TOL_VEL, w
+
+
+def test_coriolis_is_what_the_reference_adds():
+    """Without the Coriolis term the oracle misses the 45 N reference run by far
+    more than the bounds."""
+    def flat(c):
+        c.latitude = 0.0
+        return Oracle(c, serial=1)
+    w = compare_to_gold("tidal_fplane", flat)
+    assert w["u"] > 100 * TOL_VEL or w["v"] > 100 * TOL_VEL, w
+
+
+def test_wind_is_what_the_reference_adds():
+    """Without its wind stress (in the momentum sources and in the boundary
+    set-up) the oracle misses the 12 m/s wind run by far more than the bounds."""
+    def calm(c):
+        c.wind_speed = 0.0
+        return Oracle(c, serial=1)
+    w = compare_to_gold("tidal_wind", calm)
     assert w["u"] > 100 * TOL_VEL or w["v"] > 100 * TOL_VEL, w
