@@ -22,6 +22,7 @@ PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PRECOND_MG = 0
 PRECOND_JACOBI = 1
+TURB_SUBGRID, TURB_CONSTANT, TURB_FALCONER, TURB_PARABOLIC, TURB_SUBGRID_WU = 0, 1, 2, 3, 4
 SOLVER_BICGSTAB = 0
 SOLVER_GMRES = 1
 
@@ -47,7 +48,7 @@ class FlowDesc(C.Structure):
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
         ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int), ("dtvar", C.c_int),
-        ("latitude", C.c_double), ("coriolis_plane", C.c_int),
+        ("latitude", C.c_double), ("coriolis_plane", C.c_int), ("turbulence", C.c_int),
     ]
 
 

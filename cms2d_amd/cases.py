@@ -60,6 +60,7 @@ class Case:
     ndsch: int = 4            # exponential scheme
     latitude: float = 0.0     # AVERAGE_LATITUDE, degrees (Coriolis)
     coriolis_plane: int = 0   # CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane
+    mturbul: int = 5          # TURBULENCE_MODEL: 0 constant, 1 Falconer, 2 parabolic, 3 subgrid-Wu, 5 subgrid
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 

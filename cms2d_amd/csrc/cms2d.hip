@@ -790,7 +790,7 @@ static int init_state(cms2d_flow *F) {
 static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
         d->ntc > 8 || !(d->dt > 0) || !(std::fabs(d->latitude) <= 90.0) || d->coriolis_plane < 0 ||
-        d->coriolis_plane > 1)
+        d->coriolis_plane > 1 || d->turbulence < 0 || d->turbulence > 4)
         return set_err(CMS2D_ERR_ARG, "invalid flow description");
     if ((long)(d->nx + 2) * (d->ny + 2) > (1L << 30)) return set_err(CMS2D_ERR_ARG, "grid too large");
     for (int k = 0; k < 4; k++)
@@ -836,6 +836,10 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     P.noptset = d->noptset;
     P.wallfric = d->wallfric;
     P.ndsch = d->ndsch;
+    {
+        static const int turb[5] = {5, 0, 1, 2, 3};   // CMS2D_TURB_* -> mturbul
+        P.mturbul = turb[d->turbulence];
+    }
     P.lmax = std::max(g.nx, g.ny);
     F->maxit = d->maxit > 0 ? d->maxit : 20 + g.nact / 10000;   // flow.F90:784
     F->rampdur = std::max(d->rampdur, d->dt / 3600.0);

@@ -857,16 +857,35 @@ __global__ void k_grad_fric_vis(Geo g, St s, Par P) {
     }
     double bsvel = sqrt(taum / RHOW + F4(1.0e-15));
     s.uv[c] = Um; s.uelwc[c] = Uwc; s.bsvel[c] = bsvel;
-    // eddy viscosity, cell value
-    double vis;
-    if (wc == 1) {
-        double stens = sqrt(2.0 * (gx[0] * gx[0] + gy[1] * gy[1]) + (gy[0] + gx[1]) * (gy[0] + gx[1]));
-        vis = CVISCON + CVISBOT * bsvel * h + P.c2a * stens;
+    // eddy viscosity, cell value (flow_eddyvis.f90:23-160): TURBULENCE_MODEL
+    // CONSTANT (0), FALCONER (1), PARABOLIC (2), SUBGRID-WU (3), SUBGRID (5);
+    // the coefficients keep their mturbul-5 defaults (flow.F90:81-92)
+    auto waves = [&](double vv) {
         if (P.noptset >= 3) {
-            vis = vis + CVISWAV * 0.0 * 0.0;
-            vis = vis + CVISWAVBRK * h * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
+            vv = vv + CVISWAV * 0.0 * 0.0;
+            vv = vv + CVISWAVBRK * h * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
         }
-        vis = fmin(vis, CVISMAX);
+        return vv;
+    };
+    double vis;
+    if (P.mturbul == 0) {
+        vis = CVISCON;
+    } else if (P.mturbul == 1) {
+        vis = fmin(waves(CVISCON + CVISBOT * cd * Um * h), CVISMAX);
+    } else if (P.mturbul == 2) {
+        vis = fmin(waves(CVISCON + CVISBOT * bsvel * h), CVISMAX);
+    } else if (P.mturbul == 3) {
+        if (wc == 1) {
+            const double ss12 = 0.5 * (gy[0] + gx[1]);
+            const double stens = 2.0 * (gx[0] * gx[0] + 2.0 * ss12 * ss12 + gy[1] * gy[1]);
+            const double b = CVISBOT * bsvel * h;
+            vis = fmin(waves(CVISCON + sqrt(P.c2a * P.c2a * stens + b * b)), CVISMAX);
+        } else {
+            vis = F4(1.0e-6);
+        }
+    } else if (wc == 1) {
+        double stens = sqrt(2.0 * (gx[0] * gx[0] + gy[1] * gy[1]) + (gy[0] + gx[1]) * (gy[0] + gx[1]));
+        vis = fmin(waves(CVISCON + CVISBOT * bsvel * h + P.c2a * stens), CVISMAX);
     } else {
         vis = F4(1.0e-6);
     }

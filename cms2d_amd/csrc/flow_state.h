@@ -28,6 +28,7 @@ struct Par {
     double hmin, hdry, hdry1, hdry2, dtime, c2a, rtol, atol;
     double tauwx, tauwy;       // constant wind stress / rho of the step (windcurve_eval)
     int kmax, noptset, wallfric, ndsch, lmax;
+    int mturbul;               // turbulence model (flow_eddyvis.f90:23-160)
 };
 
 }  // namespace cms

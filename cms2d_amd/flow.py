@@ -56,6 +56,7 @@ class FlowStep:
         d.solver = _lib.SOLVER_BICGSTAB if solver is None else solver
         d.dtvar = dtvar
         d.latitude, d.coriolis_plane = case.latitude, case.coriolis_plane
+        d.turbulence = {5: 0, 0: 1, 1: 2, 2: 3, 3: 4}[case.mturbul]   # CMS2D_TURB_* from mturbul
         h = C.c_void_p()
         check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -23,6 +23,8 @@ module cms2d_hip
     integer(c_int), parameter, public :: CMS2D_PREC_FP64 = 0
     integer(c_int), parameter, public :: CMS2D_PRECOND_MG = 0, CMS2D_PRECOND_JACOBI = 1
     integer(c_int), parameter, public :: CMS2D_SOLVER_BICGSTAB = 0, CMS2D_SOLVER_GMRES = 1
+    integer(c_int), parameter, public :: CMS2D_TURB_SUBGRID = 0, CMS2D_TURB_CONSTANT = 1, CMS2D_TURB_FALCONER = 2, &
+                                         CMS2D_TURB_PARABOLIC = 3, CMS2D_TURB_SUBGRID_WU = 4
 
     ! struct cms2d_flow_desc
     type, bind(C), public :: cms2d_flow_desc
@@ -39,6 +41,7 @@ module cms2d_hip
         integer(c_int) :: precond = 0, solver = 0, dtvar = 0
         real(c_double) :: latitude = 0
         integer(c_int) :: coriolis_plane = 0
+        integer(c_int) :: turbulence = 0
     end type
 
     ! struct cms2d_step_info

@@ -57,6 +57,12 @@ extern "C" {
 /* Krylov method of the pp system (the momentum systems use BiCGStab). */
 #define CMS2D_SOLVER_BICGSTAB 0   /* pbicgstab (pbicgstab.f90:2-98)                           */
 #define CMS2D_SOLVER_GMRES 1      /* restarted GMRES(20) (pgmres.f90:2-146 runs GMRES(3))     */
+/* Turbulence models (flow_eddyvis.f90:23-160); 0 is the reference's default */
+#define CMS2D_TURB_SUBGRID 0      /* SUBGRID, mturbul 5                                          */
+#define CMS2D_TURB_CONSTANT 1     /* CONSTANT, mturbul 0                                         */
+#define CMS2D_TURB_FALCONER 2     /* FALCONER, mturbul 1                                         */
+#define CMS2D_TURB_PARABOLIC 3    /* PARABOLIC, mturbul 2                                        */
+#define CMS2D_TURB_SUBGRID_WU 4   /* SUBGRID-WU, mturbul 3                                       */
 
 /* Solver precision forms. */
 #define CMS2D_PREC_FP64 0         /* FP64 everywhere                                          */
@@ -97,6 +103,7 @@ typedef struct {
     double latitude;              /* AVERAGE_LATITUDE [deg] for the Coriolis parameter          */
     int coriolis_plane;           /* CORIOLIS_APPROXIMATION 0 f-plane, 1 beta-plane
                                      (flow.F90:261-269,720-736)                               */
+    int turbulence;               /* TURBULENCE_MODEL, CMS2D_TURB_* (flow.F90:362-384)        */
 } cms2d_flow_desc;
 
 typedef struct {

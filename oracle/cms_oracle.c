@@ -1132,21 +1132,47 @@ static void fric_eval(cms_oracle *o)
     }
 }
 
-/* flow_eddyvis, flow_eddyvis.f90:2-225 (mturbul=5, Cartesian mesh) */
+/* flow_eddyvis, flow_eddyvis.f90:2-225 (Cartesian mesh): TURBULENCE_MODEL
+ * CONSTANT (mturbul 0, :24-29), FALCONER (1, :31-48), PARABOLIC (2, :50-67),
+ * SUBGRID-WU (3, :69-98) and SUBGRID (5, the default, :133-160).  The
+ * coefficients keep the values flow_default gives them for mturbul 5
+ * (flow.F90:81-92) whichever model a card selects later. */
+static double eddy_waves(const cms_oracle *o, int i, double vv)
+{
+    if (o->c.noptset < 3) return vv;
+    vv += CVISWAV * 0.0 * 0.0;              /* cviswav Whgt Worb: no wave height on this path */
+    vv += CVISWAVBRK * o->h[i] * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
+    return vv;
+}
+
 static void flow_eddyvis(cms_oracle *o)
 {
-    const int nc = o->ncells, nd = o->ncellsD;
-    const double c2a = CVISHOR * CVISHOR * o->areaavg;
+    const int nc = o->ncells, nd = o->ncellsD, mt = o->c.mturbul;
+    const double c2a = CVISHOR * CVISHOR * o->areaavg, c2a2 = c2a * c2a;
     for (int i = 0; i < nc; i++) {
-        if (o->iwet[i] == 1) {
+        if (mt == 0) {
+            o->vis[i] = CVISCON;
+        } else if (mt == 1) {
+            const double vv = CVISCON + CVISBOT * o->cfrict[i] * o->uv[i] * o->h[i];
+            o->vis[i] = fmin(eddy_waves(o, i, vv), CVISMAX);
+        } else if (mt == 2) {
+            const double vv = CVISCON + CVISBOT * o->bsvel[i] * o->h[i];
+            o->vis[i] = fmin(eddy_waves(o, i, vv), CVISMAX);
+        } else if (mt == 3) {
+            if (o->iwet[i] == 1) {
+                const double ss12 = 0.5 * (o->duy[i] + o->dvx[i]);
+                const double stens = 2.0 * (o->dux[i] * o->dux[i] + 2.0 * ss12 * ss12 + o->dvy[i] * o->dvy[i]);
+                const double b = CVISBOT * o->bsvel[i] * o->h[i];
+                const double vv = CVISCON + sqrt(c2a2 * stens + b * b);
+                o->vis[i] = fmin(eddy_waves(o, i, vv), CVISMAX);
+            } else {
+                o->vis[i] = F4(1.0e-6);
+            }
+        } else if (o->iwet[i] == 1) {
             double stens = sqrt(2.0 * (o->dux[i] * o->dux[i] + o->dvy[i] * o->dvy[i])
                                 + (o->duy[i] + o->dvx[i]) * (o->duy[i] + o->dvx[i]));
             double vv = CVISCON + CVISBOT * o->bsvel[i] * o->h[i] + c2a * stens;
-            if (o->c.noptset >= 3) {
-                vv += CVISWAV * 0.0 * 0.0;
-                vv += CVISWAVBRK * o->h[i] * pow(fmax(0.0, F4(1.0e-15)) / RHOW, F4(0.33333));
-            }
-            o->vis[i] = fmin(vv, CVISMAX);
+            o->vis[i] = fmin(eddy_waves(o, i, vv), CVISMAX);
         } else {
             o->vis[i] = F4(1.0e-6);
         }

@@ -50,6 +50,8 @@ typedef struct {
                                  the time step, which recovers later (0: stop with an error) */
     double latitude;          /* AVERAGE_LATITUDE [deg] (0: no Coriolis on the f-plane)   */
     int coriolis_plane;       /* CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane         */
+    int mturbul;              /* TURBULENCE_MODEL: 0 constant, 1 Falconer, 2 parabolic,
+                                 3 subgrid (Wu), 5 subgrid (the reference's default)   */
 } cms_case;
 
 typedef struct {

@@ -27,6 +27,7 @@ class CmsCase(C.Structure):
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
         ("serial", C.c_int), ("dtvar", C.c_int), ("latitude", C.c_double), ("coriolis_plane", C.c_int),
+        ("mturbul", C.c_int),
     ]
 
 
@@ -110,6 +111,7 @@ class Oracle:
         c.serial = serial
         c.dtvar = dtvar
         c.latitude, c.coriolis_plane = case.latitude, case.coriolis_plane
+        c.mturbul = case.mturbul
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

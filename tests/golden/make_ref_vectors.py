@@ -52,6 +52,14 @@ def _mk_wind():
 
 CASES["tidal_wind"] = (_mk_wind, 40, [1, 10, 20, 40])
 
+# the other turbulence models of the Cartesian path
+for _mt, _nm in ((0, "constant"), (1, "falconer"), (2, "parabolic"), (3, "subgridwu")):
+    def _mk(mt=_mt):
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.mturbul = mt
+        return c
+    CASES[f"tidal_turb_{_nm}"] = (_mk, 40, [1, 10, 20, 40])
+
 
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}

@@ -24,6 +24,8 @@ def available():
 # advection scheme names of the ADVECTION_SCHEME card, in ndsch order
 # (source/main/definitions.F90:167-172)
 ADVSC = ["NONE", "UPWIND", "HYBRID", "POWERLAW", "EXPONENTIAL", "HLPA", "GAMMA", "CUBISTA", "ALVSMART", "HOAB"]
+# TURBULENCE_MODEL names by mturbul (flow.F90:362-384)
+TURB = {0: "CONSTANT", 1: "FALCONER", 2: "PARABOLIC", 3: "SUBGRID-WU", 5: "SUBGRID"}
 
 
 def write_case(case, d, name="case", solver="BICGSTAB"):
@@ -78,6 +80,7 @@ MATRIX_SOLVER                       {solver}
 {"ADVECTION_SCHEME                    " + ADVSC[case.ndsch] if case.ndsch != 4 else ""}
 {f"AVERAGE_LATITUDE                    {case.latitude:.9g}" if case.latitude else ""}
 {"CORIOLIS_APPROXIMATION              BETA" if case.coriolis_plane == 1 else ""}
+{"TURBULENCE_MODEL                    " + TURB[case.mturbul] if case.mturbul != 5 else ""}
 {f'WIND_SPEED_CURVE                    "{name}_wspd.xys"' if case.wind_speed else ""}
 {f'WIND_DIRECTION_CURVE                "{name}_wdir.xys"' if case.wind_speed else ""}
 MANNINGS_N_DATASET                  "{name}_mann.txt" "ManningsN"

@@ -140,7 +140,8 @@ def test_coriolis_matches_oracle(gpu_available, plane):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab", "tidal_fplane",
-                                  "tidal_betaplane", "tidal_wind"])
+                                  "tidal_betaplane", "tidal_wind", "tidal_turb_constant", "tidal_turb_falconer",
+                                  "tidal_turb_parabolic", "tidal_turb_subgridwu"])
 def test_gpu_matches_reference_vectors(gpu_available, name):
     """The HIP path against the compiled reference's own outputs."""
     from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold
