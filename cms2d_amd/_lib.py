@@ -23,6 +23,7 @@ PREC_FP32_REFINE = 1
 PRECOND_MG = 0
 PRECOND_JACOBI = 1
 TURB_SUBGRID, TURB_CONSTANT, TURB_FALCONER, TURB_PARABOLIC, TURB_SUBGRID_WU = 0, 1, 2, 3, 4
+FRIC_MANNING, FRIC_DRAG, FRIC_ROUGHNESS, FRIC_LINEAR = 0, 1, 2, 3
 SOLVER_BICGSTAB = 0
 SOLVER_GMRES = 1
 
@@ -49,6 +50,7 @@ class FlowDesc(C.Structure):
         ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int), ("precision", C.c_int),
         ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int), ("dtvar", C.c_int),
         ("latitude", C.c_double), ("coriolis_plane", C.c_int), ("turbulence", C.c_int),
+        ("friction", C.c_int),
     ]
 
 

@@ -61,6 +61,8 @@ class Case:
     latitude: float = 0.0     # AVERAGE_LATITUDE, degrees (Coriolis)
     coriolis_plane: int = 0   # CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane
     mturbul: int = 5          # TURBULENCE_MODEL: 0 constant, 1 Falconer, 2 parabolic, 3 subgrid-Wu, 5 subgrid
+    mbedfric: int = 2         # bottom friction law of `mann`: 2 Manning n, 1 drag coefficient,
+                              # 3 roughness length z0 [m] (height / 30), 4 linear coefficient [1/s]
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 

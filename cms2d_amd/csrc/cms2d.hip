@@ -790,7 +790,7 @@ static int init_state(cms2d_flow *F) {
 static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
         d->ntc > 8 || !(d->dt > 0) || !(std::fabs(d->latitude) <= 90.0) || d->coriolis_plane < 0 ||
-        d->coriolis_plane > 1 || d->turbulence < 0 || d->turbulence > 4)
+        d->coriolis_plane > 1 || d->turbulence < 0 || d->turbulence > 4 || d->friction < 0 || d->friction > 3)
         return set_err(CMS2D_ERR_ARG, "invalid flow description");
     if ((long)(d->nx + 2) * (d->ny + 2) > (1L << 30)) return set_err(CMS2D_ERR_ARG, "grid too large");
     for (int k = 0; k < 4; k++)
@@ -839,6 +839,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     {
         static const int turb[5] = {5, 0, 1, 2, 3};   // CMS2D_TURB_* -> mturbul
         P.mturbul = turb[d->turbulence];
+        static const int fric[4] = {2, 1, 3, 4};      // CMS2D_FRIC_* -> mbedfric
+        P.mbedfric = fric[d->friction];
     }
     P.lmax = std::max(g.nx, g.ny);
     F->maxit = d->maxit > 0 ? d->maxit : 20 + g.nact / 10000;   // flow.F90:784
@@ -857,7 +859,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                           &s.spv, &s.ssu0, &s.ssv0, &s.spuv0, &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu,
                           &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
                           &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
-                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1, &s.fc};
+                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1, &s.fc, &s.cbf};
     for (double **f : dfields) {
         if (int rc = dalloc(F, f, N)) return rc;
         F->state.push_back({*f, N * sizeof(double)});
@@ -968,7 +970,15 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
     // inputs
     if (int rc = upload_padded(F, s.zb, d->zb, true)) return rc;
-    if (int rc = upload_padded(F, s.coefman, d->mann, false)) return rc;
+    {
+        // fric_init (friction.F90:315-319,340-390): z0 = 1e-5 until the first
+        // evaluation, the law's input array from the dataset
+        std::vector<double> z0(N, (double)1.0e-5f);
+        HIPCHK(hipMemcpyAsync(s.z0, z0.data(), N * sizeof(double), hipMemcpyHostToDevice, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        double *in = P.mbedfric == 1 ? s.cfrict : P.mbedfric == 3 ? s.z0 : P.mbedfric == 4 ? s.cbf : s.coefman;
+        if (int rc = upload_padded(F, in, d->mann, false)) return rc;
+    }
     if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
     {
@@ -1100,6 +1110,7 @@ static double *field_ptr(cms2d_flow *F, const char *name) {
     St &s = F->s;
     struct { const char *n; double *p; } tab[] = {
         {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
+        {"coefman", s.coefman},
         {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy}, {"z0", s.z0},
         {"zb", s.zb}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}, {"fc", s.fc}};
     for (auto &t : tab)

@@ -66,6 +66,44 @@ __device__ __forceinline__ double drag2length(double cd, double h) {
     double z0 = h / exp(F4(0.4) / sqrt(cd) + 1.0);
     return fmax(z0, F4(1.0e-6));
 }
+// fric_lib.f90:916-941, 999-1023
+__device__ __forceinline__ double drag2man(double cd, double h, double hmax) {
+    return sqrt(cd * pow(fmin(hmax, h), F4(0.3333)) / GRAV);
+}
+__device__ __forceinline__ double length2drag(double z0, double h) {
+    double z0lim = fmax(fmin(z0, 0.5 * h), F4(1.0e-5));
+    double t = F4(0.4) / (log(h / z0lim) - 1.0);
+    return t * t;
+}
+// The per-law part of fric_eval (friction.F90:638-679): the law's input
+// (cfrict, z0, cbf or coefman) stays, the other coefficients follow from it
+// and the depth; returns the drag coefficient.
+__device__ __forceinline__ double fric_law(const Par &P, const St &s, int c, double h) {
+    double cd;
+    switch (P.mbedfric) {
+    case 1:
+        cd = s.cfrict[c];
+        s.coefman[c] = drag2man(cd, h, 10.0);
+        s.z0[c] = drag2length(cd, h);
+        break;
+    case 3:
+        cd = length2drag(s.z0[c], h);
+        s.cfrict[c] = cd;
+        s.coefman[c] = drag2man(cd, h, 10.0);
+        break;
+    case 4:   // uelwc is the previous evaluation's
+        cd = s.cbf[c] * h / fmax(s.uelwc[c], F4(1.0e-15));
+        s.cfrict[c] = cd;
+        s.coefman[c] = drag2man(cd, h, h);
+        s.z0[c] = drag2length(cd, h);
+        break;
+    default:
+        cd = man2drag(s.coefman[c], h);
+        s.cfrict[c] = cd;
+        if (cd != 0.0) s.z0[c] = drag2length(cd, h);
+    }
+    return cd;
+}
 
 // directed face values of cell c
 __device__ __forceinline__ double fdir(const double *E, const double *N, const Geo &g, int c, int k) {
@@ -841,9 +879,7 @@ __global__ void k_grad_fric_vis(Geo g, St s, Par P) {
     if (i == 0) s.hkE[c - 1] = hkv[KW];
     // fric_eval
     double h = s.h[c], u = s.u[c], v = s.v[c];
-    double cd = man2drag(s.coefman[c], h);
-    s.cfrict[c] = cd;
-    if (cd != 0.0) s.z0[c] = drag2length(cd, h);
+    double cd = fric_law(P, s, c, h);
     double Um, Uwc, taum;
     if (P.noptset >= 3) {
         double Um2 = u * u + v * v;
@@ -1052,9 +1088,7 @@ __global__ void k_velface(Geo g, St s, Par P) {
         s.h[c] = h;
     }
     s.u[c] = u; s.v[c] = v;
-    double cd = man2drag(s.coefman[c], h);
-    s.cfrict[c] = cd;
-    if (cd != 0.0) s.z0[c] = drag2length(cd, h);
+    double cd = fric_law(P, s, c, h);
     double Um = sqrt(u * u + v * v), Uwc = Um, taum;
     if (P.noptset >= 3) { Uwc = sqrt(u * u + v * v + CFRICWAV * 0.0 * 0.0); taum = RHOW * cd * Um * Uwc; }
     else taum = RHOW * cd * Um * Um;

@@ -10,6 +10,7 @@ struct St {
     double *u, *v, *p, *h, *pp, *zb, *vis, *visb, *visc, *u1, *v1, *p1, *h1, *eta;
     double *dppx, *dppy, *dux, *duy, *dvx, *dvy, *dpx, *dpy;
     double *coefman, *cfrict, *z0, *uv, *uelwc, *bsvel;
+    double *cbf;               // linear friction coefficient (cbotfric of mbedfric 4)
     double *su, *sv, *sp, *spu, *spv, *ssu0, *ssv0, *spuv0, *sppp0, *sspp0;
     double *apuareap, *sumu, *Hu, *Hv, *resu, *resv, *resp;
     double *fluxE, *fluxN, *hkE, *hkN, *viskE, *viskN, *zbkE, *zbkN;   // unique faces
@@ -29,6 +30,7 @@ struct Par {
     double tauwx, tauwy;       // constant wind stress / rho of the step (windcurve_eval)
     int kmax, noptset, wallfric, ndsch, lmax;
     int mturbul;               // turbulence model (flow_eddyvis.f90:23-160)
+    int mbedfric;              // bottom friction law (fric_eval, friction.F90:638-688)
 };
 
 }  // namespace cms

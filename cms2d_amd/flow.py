@@ -57,6 +57,7 @@ class FlowStep:
         d.dtvar = dtvar
         d.latitude, d.coriolis_plane = case.latitude, case.coriolis_plane
         d.turbulence = {5: 0, 0: 1, 1: 2, 2: 3, 3: 4}[case.mturbul]   # CMS2D_TURB_* from mturbul
+        d.friction = {2: 0, 1: 1, 3: 2, 4: 3}[case.mbedfric]           # CMS2D_FRIC_* from mbedfric
         h = C.c_void_p()
         check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -25,6 +25,8 @@ module cms2d_hip
     integer(c_int), parameter, public :: CMS2D_SOLVER_BICGSTAB = 0, CMS2D_SOLVER_GMRES = 1
     integer(c_int), parameter, public :: CMS2D_TURB_SUBGRID = 0, CMS2D_TURB_CONSTANT = 1, CMS2D_TURB_FALCONER = 2, &
                                          CMS2D_TURB_PARABOLIC = 3, CMS2D_TURB_SUBGRID_WU = 4
+    integer(c_int), parameter, public :: CMS2D_FRIC_MANNING = 0, CMS2D_FRIC_DRAG = 1, CMS2D_FRIC_ROUGHNESS = 2, &
+                                         CMS2D_FRIC_LINEAR = 3
 
     ! struct cms2d_flow_desc
     type, bind(C), public :: cms2d_flow_desc
@@ -42,6 +44,7 @@ module cms2d_hip
         real(c_double) :: latitude = 0
         integer(c_int) :: coriolis_plane = 0
         integer(c_int) :: turbulence = 0
+        integer(c_int) :: friction = 0
     end type
 
     ! struct cms2d_step_info

@@ -63,6 +63,14 @@ extern "C" {
 #define CMS2D_TURB_FALCONER 2     /* FALCONER, mturbul 1                                         */
 #define CMS2D_TURB_PARABOLIC 3    /* PARABOLIC, mturbul 2                                        */
 #define CMS2D_TURB_SUBGRID_WU 4   /* SUBGRID-WU, mturbul 3                                       */
+/* Bottom friction laws (fric_eval, friction.F90:638-688); the field `mann` holds
+   the law's input per cell.  0 is the reference's default. */
+#define CMS2D_FRIC_MANNING 0      /* Manning n, mbedfric 2 (MANNINGS_N_DATASET)                  */
+#define CMS2D_FRIC_DRAG 1         /* drag coefficient Cd, mbedfric 1 (BOTTOM_FRICTION_COEF_*)    */
+#define CMS2D_FRIC_ROUGHNESS 2    /* hydraulic roughness length z0 [m] = roughness height / 30,
+                                     mbedfric 3 (ROUGHNESS_HEIGHT_*, friction.F90:344,385-387) */
+#define CMS2D_FRIC_LINEAR 3       /* linear coefficient c [1/s] (bed stress rho c h U), mbedfric 4
+                                     (LINEAR_BOTTOM_FRICTION_COEF_CONSTANT)                     */
 
 /* Solver precision forms. */
 #define CMS2D_PREC_FP64 0         /* FP64 everywhere                                          */
@@ -72,7 +80,8 @@ typedef struct {
     int nx, ny;                   /* active cells per row / per column (row-major from SW), >= 3 */
     double dx, dy;                /* uniform cell size [m]                                  */
     const double *zb;             /* bed elevation [nx*ny], host memory [m]                 */
-    const double *mann;           /* Manning n [nx*ny] (MANNINGS_N_DATASET)                 */
+    const double *mann;           /* bottom friction input [nx*ny] of the law `friction`:
+                                     Manning n by default (MANNINGS_N_DATASET)              */
     const double *wavestrx;       /* radiation-stress forcing / rho [nx*ny] or NULL         */
     const double *wavestry;       /* (wave_flowgrid_def wavestrx/wavestry)                  */
     int side_type[4];             /* CMS2D_BND_* for N,E,S,W                                 */
@@ -104,6 +113,7 @@ typedef struct {
     int coriolis_plane;           /* CORIOLIS_APPROXIMATION 0 f-plane, 1 beta-plane
                                      (flow.F90:261-269,720-736)                               */
     int turbulence;               /* TURBULENCE_MODEL, CMS2D_TURB_* (flow.F90:362-384)        */
+    int friction;                 /* bottom friction law, CMS2D_FRIC_* (friction.F90:201-256) */
 } cms2d_flow_desc;
 
 typedef struct {

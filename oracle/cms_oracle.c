@@ -86,6 +86,7 @@ struct cms_oracle {
     double *su, *sv, *sp, *spu, *spv, *ssu0, *ssv0, *spuv0, *sppp0, *sspp0;
     double *apuareap, *sumu, *Hu, *Hv, *rsu, *rsv, *rsp;
     double *coefman, *cfrict, *z0, *uv, *uelwc, *bsxy, *bsvel, *cmb, *cbcfuwcap;
+    double *cbf;   /* linear friction coefficient (cbotfric of mbedfric 4) */
     double *wavestrx, *wavestry, *fc;
     gradop *gow;
     /* control (comvarbl) */
@@ -195,6 +196,20 @@ double cms_oracle_drag2length(double cd, double h)
 {
     double z0 = h / exp(F4(0.4) / sqrt(cd) + 1.0);
     return fmax(z0, F4(1.0e-6));
+}
+
+/* fric_lib.f90:916-941 fric_conv_drag2man */
+double cms_oracle_drag2man(double g, double cd, double h, double hmax)
+{
+    return sqrt(cd * pow(fmin(hmax, h), F4(0.3333)) / g);
+}
+
+/* fric_lib.f90:999-1023 fric_conv_length2drag */
+double cms_oracle_length2drag(double z0, double h)
+{
+    double z0lim = fmax(fmin(z0, 0.5 * h), F4(1.0e-5));
+    double t = F4(0.4) / (log(h / z0lim) - 1.0);
+    return t * t;
 }
 
 /* time_lib.F90:234-256 ramp_func */
@@ -1107,12 +1122,31 @@ static void flow_grad_interp(cms_oracle *o)
     }
 }
 
-/* fric_eval, friction.F90:615-797 (mbedfric=2 Manning; mwavcurint=1 or no waves) */
+/* fric_eval, friction.F90:615-797 (mbedfric 1-4; mwavcurint=1 or no waves).
+ * The law's input (cfrict, coefman, z0 or the linear coefficient cbf) is the
+ * array fric_init filled (friction.F90:340-390); the others follow from it. */
 static void fric_eval(cms_oracle *o)
 {
     for (int i = 0; i < o->ncells; i++) {
-        o->cfrict[i] = cms_oracle_man2drag(GRAV, o->coefman[i], o->h[i], HMAXFRIC);
-        if (o->cfrict[i] != 0.0) o->z0[i] = cms_oracle_drag2length(o->cfrict[i], o->h[i]);
+        const double h = o->h[i];
+        switch (o->c.mbedfric) {
+        case 1:   /* :639-645 */
+            o->coefman[i] = cms_oracle_drag2man(GRAV, o->cfrict[i], h, HMAXFRIC);
+            o->z0[i] = cms_oracle_drag2length(o->cfrict[i], h);
+            break;
+        case 3:   /* :664-670 */
+            o->cfrict[i] = cms_oracle_length2drag(o->z0[i], h);
+            o->coefman[i] = cms_oracle_drag2man(GRAV, o->cfrict[i], h, HMAXFRIC);
+            break;
+        case 4:   /* :672-679: uelwc is the previous evaluation's */
+            o->cfrict[i] = o->cbf[i] * h / fmax(o->uelwc[i], F4(1.0e-15));
+            o->coefman[i] = cms_oracle_drag2man(GRAV, o->cfrict[i], h, h);
+            o->z0[i] = cms_oracle_drag2length(o->cfrict[i], h);
+            break;
+        default:  /* 2, :647-662 */
+            o->cfrict[i] = cms_oracle_man2drag(GRAV, o->coefman[i], h, HMAXFRIC);
+            if (o->cfrict[i] != 0.0) o->z0[i] = cms_oracle_drag2length(o->cfrict[i], h);
+        }
     }
     for (int i = 0; i < o->ncells; i++) {
         double Um, Uwc, taum;
@@ -1393,7 +1427,7 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     A(su); A(sv); A(sp); A(spu); A(spv); A(ssu0); A(ssv0); A(spuv0); A(sppp0); A(sspp0);
     A(apuareap); A(sumu); A(Hu); A(Hv); A(rsu); A(rsv); A(rsp);
     A(coefman); A(cfrict); A(z0); A(uv); A(uelwc); A(bsxy); A(bsvel); A(cmb); A(cbcfuwcap);
-    A(wavestrx); A(wavestry); A(fc);
+    A(wavestrx); A(wavestry); A(fc); A(cbf);
 #undef A
     o->iwet = IALLOC(nd); o->iwet1 = IALLOC(nd); o->icorner = IALLOC(nd);
     o->flux = DALLOC(F); o->flux1 = DALLOC(F); o->hk = DALLOC(F); o->pk = DALLOC(F);
@@ -1401,8 +1435,16 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     o->gow = (gradop *)xcalloc((size_t)nc, sizeof(gradop));
     o->wk = DALLOC(7 * (size_t)nc);
     o->ia = IALLOC(nc + 1); o->ja = IALLOC(5 * nc); o->diagpos = IALLOC(nc); o->lu = DALLOC(5 * nc);
+    /* fric_init (friction.F90:315-319): coefman = cfrict = 0, z0 = 1e-5, then
+     * the law's input array from the dataset */
+    for (int i = 0; i < nd; i++) o->z0[i] = F4(1.0e-5);
     for (int i = 0; i < nc; i++) {
-        o->coefman[i] = c->mann[i];
+        switch (c->mbedfric) {
+        case 1: o->cfrict[i] = c->mann[i]; break;
+        case 3: o->z0[i] = c->mann[i]; break;
+        case 4: o->cbf[i] = c->mann[i]; break;
+        default: o->coefman[i] = c->mann[i];
+        }
         o->cmb[i] = 1.0;
         o->wavestrx[i] = c->wavestrx ? c->wavestrx[i] : 0.0;
         o->wavestry[i] = c->wavestry ? c->wavestry[i] : 0.0;
@@ -1479,7 +1521,7 @@ void cms_oracle_destroy(cms_oracle *o)
                     o->su, o->sv, o->sp, o->spu, o->spv, o->ssu0, o->ssv0, o->spuv0, o->sppp0, o->sspp0,
                     o->apuareap, o->sumu, o->Hu, o->Hv, o->rsu, o->rsv, o->rsp, o->coefman, o->cfrict, o->z0,
                     o->uv, o->uelwc, o->bsxy, o->bsvel, o->cmb, o->cbcfuwcap, o->wavestrx, o->wavestry, o->fc,
-                    o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
+                    o->cbf, o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(o);
 }
@@ -1549,6 +1591,7 @@ static const double *field(const cms_oracle *o, const char *name)
     if (!strcmp(name, "dpx")) return o->dpx;
     if (!strcmp(name, "dpy")) return o->dpy;
     if (!strcmp(name, "cfrict")) return o->cfrict;
+    if (!strcmp(name, "coefman")) return o->coefman;
     if (!strcmp(name, "uv")) return o->uv;
     if (!strcmp(name, "fc")) return o->fc;
     if (!strcmp(name, "wavestrx")) return o->wavestrx;

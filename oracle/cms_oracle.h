@@ -52,6 +52,8 @@ typedef struct {
     int coriolis_plane;       /* CORIOLIS_APPROXIMATION: 0 f-plane, 1 beta-plane         */
     int mturbul;              /* TURBULENCE_MODEL: 0 constant, 1 Falconer, 2 parabolic,
                                  3 subgrid (Wu), 5 subgrid (the reference's default)   */
+    int mbedfric;             /* bottom friction law of `mann`: 1 drag coefficient, 2 Manning n
+                                 (0 is read as 2), 3 roughness length z0, 4 linear coefficient */
 } cms_case;
 
 typedef struct {
@@ -86,6 +88,8 @@ double cms_oracle_schmcoef(int ndsch, double dk, double fk);
 double cms_oracle_wall_coef(double yp, double y0);
 double cms_oracle_man2drag(double g, double cman, double h, double hmax);
 double cms_oracle_drag2length(double cd, double h);
+double cms_oracle_drag2man(double g, double cd, double h, double hmax);
+double cms_oracle_length2drag(double z0, double h);
 
 /* Gradient of an active-cell field (row-major) with the current wet-only CBGG
  * operator (der_grad_cbgg + der_grad_eval, der_lib.f90:272-378,726-839). */

@@ -27,7 +27,7 @@ class CmsCase(C.Structure):
         ("noptset", C.c_int), ("wallfric", C.c_int), ("ndsch", C.c_int), ("maxit", C.c_int),
         ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
         ("serial", C.c_int), ("dtvar", C.c_int), ("latitude", C.c_double), ("coriolis_plane", C.c_int),
-        ("mturbul", C.c_int),
+        ("mturbul", C.c_int), ("mbedfric", C.c_int),
     ]
 
 
@@ -56,7 +56,8 @@ def lib():
         L.cms_oracle_time.argtypes = [C.c_void_p]
         L.cms_oracle_volume.argtypes = [C.c_void_p, D, D]
         for fn, na in (("cms_oracle_schmcoef", 3), ("cms_oracle_wall_coef", 2),
-                       ("cms_oracle_man2drag", 4), ("cms_oracle_drag2length", 2)):
+                       ("cms_oracle_man2drag", 4), ("cms_oracle_drag2length", 2),
+                       ("cms_oracle_drag2man", 4), ("cms_oracle_length2drag", 2)):
             f = getattr(L, fn)
             f.restype = C.c_double
             f.argtypes = ([C.c_int] if fn == "cms_oracle_schmcoef" else []) + \
@@ -112,6 +113,7 @@ class Oracle:
         c.dtvar = dtvar
         c.latitude, c.coriolis_plane = case.latitude, case.coriolis_plane
         c.mturbul = case.mturbul
+        c.mbedfric = getattr(case, "mbedfric", 2)
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

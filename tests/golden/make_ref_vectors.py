@@ -61,6 +61,31 @@ for _mt, _nm in ((0, "constant"), (1, "falconer"), (2, "parabolic"), (3, "subgri
     CASES[f"tidal_turb_{_nm}"] = (_mk, 40, [1, 10, 20, 40])
 
 
+
+
+# the other bottom friction laws on the tidal case: a drag coefficient dataset,
+# a roughness height dataset (z0 = k / 4096 m, so the height 30 z0 and its
+# division by 30 are exact in single precision) and the linear law's constant
+def _mk_fric(law):
+    def mk():
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        rng = np.random.default_rng(7)
+        n = c.nx * c.ny
+        c.mbedfric = law
+        if law == 1:
+            c.mann = np.float32(0.002 + 0.004 * rng.random(n)).astype(np.float64)
+        elif law == 3:
+            c.mann = rng.integers(4, 41, size=n) / 4096.0
+        else:
+            c.mann = np.full(n, float(np.float32(1.0e-4)))
+        return c
+    return mk
+
+
+for _law, _nm in ((1, "drag"), (3, "roughness"), (4, "linear")):
+    CASES[f"tidal_fric_{_nm}"] = (_mk_fric(_law), 40, [1, 10, 20, 40])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

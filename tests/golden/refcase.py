@@ -4,7 +4,8 @@ the reference build (oracle/_ref/ref_run) on it.
 Inputs written: <name>.cmcards, <name>.cart (GRID_* cards read by
 read_grid_cart, source/geospatial/geospatial.F90:2553-2706), <name>.bid (one
 cellstring, bnd_read_str_bid, source/boundaries/bnd_lib.f90:138-178),
-<name>_mann.txt (readscalTxt, source/input/in_lib.f90:68-119).
+<name>_mann.txt (readscalTxt, source/input/in_lib.f90:68-119: Manning n, drag
+coefficients or roughness heights by the case's friction law).
 """
 from __future__ import annotations
 
@@ -26,6 +27,8 @@ def available():
 ADVSC = ["NONE", "UPWIND", "HYBRID", "POWERLAW", "EXPONENTIAL", "HLPA", "GAMMA", "CUBISTA", "ALVSMART", "HOAB"]
 # TURBULENCE_MODEL names by mturbul (flow.F90:362-384)
 TURB = {0: "CONSTANT", 1: "FALCONER", 2: "PARABOLIC", 3: "SUBGRID-WU", 5: "SUBGRID"}
+# bottom friction dataset cards by mbedfric (friction.F90:201-245)
+FRIC = {1: "BOTTOM_FRICTION_COEF_DATASET", 2: "MANNINGS_N_DATASET", 3: "ROUGHNESS_HEIGHT_DATASET"}
 
 
 def write_case(case, d, name="case", solver="BICGSTAB"):
@@ -40,9 +43,18 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
         f.write("GRID_RESOLUTION_Y_BEGIN\n" + "".join(f"{case.dy:.9g}\n" for _ in range(ny))
                 + "GRID_RESOLUTION_Y_END\n")
         f.write("GRID_DEPTHS_BEGIN\n" + "".join(f"{v:.9g}\n" for v in depth) + "GRID_DEPTHS_END\n")
+    mbedfric = getattr(case, "mbedfric", 2)
+    fric = np.asarray(case.mann, dtype=np.float64)
+    if mbedfric == 3:
+        fric = fric * 30.0          # the dataset holds roughness heights (z0 = height / 30)
     with open(os.path.join(d, name + "_mann.txt"), "w") as f:
         f.write(f"{nx * ny} 1\n")
-        f.write("".join(f"{float(np.float32(v)):.9g}\n" for v in case.mann))
+        f.write("".join(f"{float(np.float32(v)):.9g}\n" for v in fric))
+    if mbedfric == 4:
+        assert np.all(case.mann == case.mann[0]), "the linear law takes one constant"
+        fric_card = f"LINEAR_BOTTOM_FRICTION_COEF_CONSTANT {float(np.float32(case.mann[0])):.9g}"
+    else:
+        fric_card = FRIC[mbedfric] + f' "{name}_mann.txt" "Friction"'
     side = case.th_side
     if side == 1:
         ids = [nx + j * nx for j in range(ny)]                  # east column, full-grid ids
@@ -83,7 +95,7 @@ MATRIX_SOLVER                       {solver}
 {"TURBULENCE_MODEL                    " + TURB[case.mturbul] if case.mturbul != 5 else ""}
 {f'WIND_SPEED_CURVE                    "{name}_wspd.xys"' if case.wind_speed else ""}
 {f'WIND_DIRECTION_CURVE                "{name}_wdir.xys"' if case.wind_speed else ""}
-MANNINGS_N_DATASET                  "{name}_mann.txt" "ManningsN"
+{fric_card}
 {"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
 OUTPUT_FILE_TYPE                    ASCII
 HORIZONTAL_PROJECTION_BEGIN

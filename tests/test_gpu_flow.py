@@ -10,6 +10,7 @@ import pytest
 from cms2d_amd import _lib, cases
 from cms2d_amd.flow import FlowStep
 from pyoracle import Oracle
+from test_reference_vectors import ALL as REF_CASES
 
 TOL_ETA = 1e-6   # m
 TOL_VEL = 1e-6   # m/s
@@ -107,6 +108,25 @@ def test_grids_under_three_cells_a_side_rejected(shape):
     assert e.value.code == -1                       # CMS2D_ERR_ARG
 
 
+@pytest.mark.parametrize("field,value", [("friction", 4), ("friction", -1), ("turbulence", 5),
+                                         ("coriolis_plane", 2), ("latitude", 91.0)])
+def test_out_of_range_options_rejected(field, value):
+    """cms2d_flow_create refuses option values outside the documented sets
+    (CMS2D_ERR_ARG) before touching a device."""
+    import ctypes as C
+    zb = np.full(16, -2.0)
+    mann = np.full(16, 0.02)
+    d = _lib.FlowDesc()
+    d.nx = d.ny = 4
+    d.dx = d.dy = 10.0
+    d.zb, d.mann = zb.ctypes.data_as(_lib.D), mann.ctypes.data_as(_lib.D)
+    d.dt, d.hdry, d.kmax, d.rtol = 10.0, 0.05, 100, 1e-10
+    setattr(d, field, value)
+    h = C.c_void_p()
+    assert _lib.lib().cms2d_flow_create(C.byref(d), C.byref(h)) == -1
+    assert not h.value
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("ndsch", [0, 1, 2, 3, 5, 6, 7, 8, 9])
 def test_advection_schemes_match_oracle(gpu_available, ndsch):
@@ -139,9 +159,7 @@ def test_coriolis_matches_oracle(gpu_available, plane):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["basin", "tidal", "tidal_hlpa", "tidal_cubista", "tidal_hoab", "tidal_fplane",
-                                  "tidal_betaplane", "tidal_wind", "tidal_turb_constant", "tidal_turb_falconer",
-                                  "tidal_turb_parabolic", "tidal_turb_subgridwu"])
+@pytest.mark.parametrize("name", REF_CASES)
 def test_gpu_matches_reference_vectors(gpu_available, name):
     """The HIP path against the compiled reference's own outputs."""
     from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold
@@ -210,3 +228,35 @@ def test_multigrid_tile_widths_match(gpu_available, monkeypatch):
     g = FlowStep(c)
     assert g.tuning()[0] in (64, 128, 256)
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("law", [1, 3, 4])
+def test_friction_laws_match_oracle(gpu_available, law):
+    """Drag coefficient, roughness height and linear friction (fric_eval's
+    mbedfric 1, 3, 4) on a 201 x 121 tidal case with wet/dry cells: state
+    within the step tolerances, and the derived coefficients cfrict, coefman
+    and z0 within 1e-6 relative per wet cell (functions of the depth, which
+    agrees to about 1e-9 m; the linear law's cfrict divides by the previous
+    current speed, so it is compared where that speed is above 1 mm/s)."""
+    from test_reference_vectors import _tidal_fric
+    c = _tidal_fric(law)()
+    c2 = cases.tidal(nx=201, ny=121, dx=20.0, seed=4)
+    rng = np.random.default_rng(7)
+    n = c2.nx * c2.ny
+    c2.mbedfric = law
+    c2.mann = (np.float32(0.002 + 0.004 * rng.random(n)).astype(np.float64) if law == 1 else
+               rng.integers(4, 41, size=n) / 4096.0 if law == 3 else np.full(n, float(np.float32(1e-4))))
+    for case, steps in ((c, 20), (c2, 8)):
+        o, g = run_pair(case, steps)
+        err = compare(o, g)
+        sel = o.get("iwet") > 0
+        if law == 4:
+            sel &= o.get("uv") > 1e-3
+        assert sel.sum() > 50
+        for f in ("cfrict", "coefman", "z0"):
+            a, b = o.get(f)[sel], g.get(f)[sel]
+            d = (np.abs(a - b) / np.abs(a)).max()
+            assert d <= 1e-6, (f, d)
+        print(f"friction law {law} {case.nx}x{case.ny}: {err}", flush=True)
+        g.close()

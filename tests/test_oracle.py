@@ -88,6 +88,36 @@ def test_roughness_length_inverts_the_log_law(cd, h):
     assert (kappa / (math.log(h / z0) - 1.0)) ** 2 == pytest.approx(cd, rel=1e-12)
 
 
+@pytest.mark.parametrize("cd,h", [(0.002, 1.0), (0.005, 3.7), (0.01, 0.4)])
+def test_length_to_drag_inverts_drag_to_length(cd, h):
+    """fric_conv_length2drag is the log law solved for Cd: it returns the Cd
+    that fric_conv_drag2length turned into z0 (z0 well inside its clamps)."""
+    L = lib()
+    z0 = L.cms_oracle_drag2length(cd, h)
+    assert L.cms_oracle_length2drag(z0, h) == pytest.approx(cd, rel=1e-12)
+
+
+def test_length_to_drag_clamps():
+    """z0 above half the depth acts as h/2 (Cd = (kappa/(ln 2 - 1))^2), below
+    1e-5 m as 1e-5 m."""
+    L = lib()
+    kappa = f4(0.4)
+    assert L.cms_oracle_length2drag(3.0, 2.0) == pytest.approx((kappa / (math.log(2.0) - 1.0)) ** 2, rel=1e-12)
+    lo = (kappa / (math.log(2.0 / f4(1e-5)) - 1.0)) ** 2
+    assert L.cms_oracle_length2drag(1e-9, 2.0) == pytest.approx(lo, rel=1e-12)
+
+
+@pytest.mark.parametrize("n,h", [(0.02, 1.0), (0.035, 6.0), (0.012, 0.3)])
+def test_drag_to_manning_inverts_manning_to_drag(n, h):
+    """fric_conv_drag2man undoes fric_conv_man2drag below the 10 m depth cap;
+    above it both use the capped depth."""
+    g = 9.81
+    L = lib()
+    for hh in (h, 25.0):
+        cd = L.cms_oracle_man2drag(g, n, hh, 10.0)
+        assert L.cms_oracle_drag2man(g, cd, hh, 10.0) == pytest.approx(n, rel=1e-12)
+
+
 def test_wall_coefficient_log_law():
     kappa = f4(0.4)
     yp = 5.0

@@ -76,7 +76,33 @@ def _tidal_turb(mt):
 TURB = {"tidal_turb_constant": 0, "tidal_turb_falconer": 1, "tidal_turb_parabolic": 2, "tidal_turb_subgridwu": 3}
 for _name, _mt in TURB.items():
     CASES[_name] = _tidal_turb(_mt)
-ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB]
+
+
+def _tidal_fric(law):
+    """The tidal case under another bottom friction law, as
+    tests/golden/make_ref_vectors.py writes it: drag coefficients 0.002-0.006,
+    roughness lengths k / 4096 m (heights 30 z0 in the dataset), or the linear
+    law's constant 1e-4 1/s."""
+    def mk():
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        rng = np.random.default_rng(7)
+        n = c.nx * c.ny
+        c.mbedfric = law
+        if law == 1:
+            c.mann = np.float32(0.002 + 0.004 * rng.random(n)).astype(np.float64)
+        elif law == 3:
+            c.mann = rng.integers(4, 41, size=n) / 4096.0
+        else:
+            c.mann = np.full(n, float(np.float32(1.0e-4)))
+        return c
+    return mk
+
+
+# BOTTOM_FRICTION_COEF_DATASET / ROUGHNESS_HEIGHT_DATASET / LINEAR_BOTTOM_FRICTION_COEF_CONSTANT
+FRIC = {"tidal_fric_drag": 1, "tidal_fric_roughness": 3, "tidal_fric_linear": 4}
+for _name, _law in FRIC.items():
+    CASES[_name] = _tidal_fric(_law)
+ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC]
 
 
 def compare_to_gold(name, model_factory):
@@ -152,4 +178,17 @@ def test_turbulence_model_is_what_the_reference_uses(name):
         c.mturbul = 5
         return Oracle(c, serial=1)
     w = compare_to_gold(name, default)
+    assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
+
+
+@pytest.mark.parametrize("name", list(FRIC))
+def test_friction_law_is_what_the_reference_uses(name):
+    """Read as Manning n instead, the same friction field misses these runs by
+    far more than the bounds."""
+    def manning(c):
+        c.mbedfric = 2
+        if name == "tidal_fric_linear":
+            c.mann = np.full_like(c.mann, 0.025)
+        return Oracle(c, serial=1)
+    w = compare_to_gold(name, manning)
     assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
