@@ -152,6 +152,20 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
     __shared__ float zt[MT_Y + 2][MT_X + 2];
     int X0, Y0;
     mg_tile_origin<TX>(X0, Y0);
+    // the owned cells' coefficients are requested before the staging loads,
+    // so both are in flight together instead of one after the barrier
+    float a[MT_OWN][4];
+#pragma unroll
+    for (int k = 0; k < MT_OWN; k++) {
+        int fx, fy;
+        mg_own<TX>(threadIdx.x, k, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy;
+        a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
+        if (mg_inside(f, i, j)) {
+            const int c = f.idx(i, j);
+            a[k][0] = f.aN[c]; a[k][1] = f.aE[c]; a[k][2] = f.aS[c]; a[k][3] = f.aW[c];
+        }
+    }
     for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
         const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
         float v = 0.0f;
@@ -162,17 +176,14 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
         q[ly][lx] = v;
     }
     __syncthreads();
-    float a[MT_OWN][4];
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
         mg_own<TX>(threadIdx.x, k, fx, fy);
         const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
         float z = 0.0f;
-        a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
         if (mg_inside(f, i, j)) {
             const int c = f.idx(i, j);
-            a[k][0] = f.aN[c]; a[k][1] = f.aE[c]; a[k][2] = f.aS[c]; a[k][3] = f.aW[c];
             if (((i + j) & 1) == 0) z = q[ly][lx];
             else z = mg_black(a[k][0], a[k][1], a[k][2], a[k][3], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
                               q[ly][lx - 1], q[ly][lx], f.dinv[c]);
@@ -224,20 +235,13 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
     __shared__ float rt[MT_Y + 2][MT_X + 2];
     int X0, Y0;
     mg_tile_origin<TX>(X0, Y0);
-    for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
-        const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
-        float v = 0.0f;
-        if (mg_inside(L, i, j)) v = L.z[L.idx(i, j)] + C.zo[C.idx(i >> 1, j >> 1)];
-        w[ly][lx] = v;
-    }
-    __syncthreads();
+    // the owned cells' operands are requested before the staging loads (see k_mg_down)
     float a[MT_OWN][4], bb[MT_OWN], dd[MT_OWN];
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
         mg_own<TX>(threadIdx.x, k, fx, fy);
-        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
-        float z = 0.0f;
+        const int i = X0 + fx, j = Y0 + fy;
         a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
         bb[k] = dd[k] = 0.0f;
         if (mg_inside(L, i, j)) {
@@ -245,6 +249,23 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_up(MgLevel L, const TB *b, Mg
             a[k][0] = L.aN[c]; a[k][1] = L.aE[c]; a[k][2] = L.aS[c]; a[k][3] = L.aW[c];
             bb[k] = (float)b[c];
             dd[k] = L.dinv[c];
+        }
+    }
+    for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
+        const int ly = t / (MT_X + 4), lx = t - ly * (MT_X + 4), i = X0 - 2 + lx, j = Y0 - 2 + ly;
+        float v = 0.0f;
+        if (mg_inside(L, i, j)) v = L.z[L.idx(i, j)] + C.zo[C.idx(i >> 1, j >> 1)];
+        w[ly][lx] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MT_OWN; k++) {
+        int fx, fy;
+        mg_own<TX>(threadIdx.x, k, fx, fy);
+        const int i = X0 + fx, j = Y0 + fy, lx = fx + 2, ly = fy + 2;
+        float z = 0.0f;
+        if (mg_inside(L, i, j)) {
+            const int c = L.idx(i, j);
             if (((i + j) & 1) == 0) {
                 z = mg_red(a[k][0], a[k][1], a[k][2], a[k][3], w[ly + 1][lx], w[ly][lx + 1], w[ly - 1][lx],
                            w[ly][lx - 1], bb[k], dd[k]);
