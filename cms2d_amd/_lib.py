@@ -18,6 +18,7 @@ I = C.POINTER(C.c_int)
 
 BND_WALL = 0
 BND_WSE_HARMONIC = 1
+BND_WSE_CURVE = 2
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PRECOND_MG = 0
@@ -51,6 +52,7 @@ class FlowDesc(C.Structure):
         ("device", C.c_int), ("precond", C.c_int), ("solver", C.c_int), ("dtvar", C.c_int),
         ("latitude", C.c_double), ("coriolis_plane", C.c_int), ("turbulence", C.c_int),
         ("friction", C.c_int),
+        ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
     ]
 
 

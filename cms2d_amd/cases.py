@@ -63,6 +63,9 @@ class Case:
     mturbul: int = 5          # TURBULENCE_MODEL: 0 constant, 1 Falconer, 2 parabolic, 3 subgrid-Wu, 5 subgrid
     mbedfric: int = 2         # bottom friction law of `mann`: 2 Manning n, 1 drag coefficient,
                               # 3 roughness length z0 [m] (height / 30), 4 linear coefficient [1/s]
+    wse_times: object = None  # water-level curve on the open side instead of the harmonics:
+    wse_values: object = None # times [hr] and levels [m] (WSE_CURVE), Lagrange order wse_order
+    wse_order: int = 0        # (0: the reference's default 2)
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 

@@ -123,8 +123,53 @@ struct cms2d_flow {
     int mg_tile = 128;         // level-0 tile width of the multigrid cycle (timed at creation)
     int windconst = 0;         // spatially constant wind (met_def windconst) and its 10-m vector
     double wndx = 0.0, wndy = 0.0;
+    int side_type[4]{};        // CMS2D_BND_* per side (the kernels see 1 for every water-level side)
+    std::vector<double> wse_t, wse_v;   // water-level curve of CMS2D_BND_WSE_CURVE sides
+    int wse_order = 1, wse_inc = 1;     // interpolation order, window start (H_str%inc, 1-based)
     hipEvent_t step_ev[2]{};
 };
+
+// Lagrange interpolation of the curve (t, v) at time xi with the window search
+// of plagr_fit (plagr_lib.f90:20-138): order np = min(ni, m - 1); constant
+// extension outside the curve; the window start k (1-based, kept between calls
+// like H_str%inc) moves by a forward search, else backwards, else bisection,
+// so that x(k) <= xi <= x(k+np).  Weights normalised by their sum.
+static double wse_curve_eval(const std::vector<double> &t, const std::vector<double> &v, int ni, double xi, int &k) {
+    const int m = (int)t.size();
+    const int np = std::min(ni, m - 1);
+    auto x = [&](int i) { return t[i - 1]; };
+    if (xi < x(1)) { k = 1; return v[0]; }
+    if (xi > x(m)) { k = m - np; return v[m - 1]; }
+    if (k >= 1) {
+        const int i0 = k;
+        for (k = i0; k <= m - np; k++)
+            if (xi <= x(k + np)) break;
+        if (k > m - np || xi < x(k) || xi > x(k + np)) {
+            for (k = std::min(i0, m); k >= 1; k--)
+                if (x(k) <= xi) break;
+            if (k < 1 || k + np > m || xi < x(k) || xi > x(k + np)) k = 0;
+        }
+    }
+    if (k <= 0) {
+        int i = 1, j = m - np + 1;
+        while (j > i + 1) {
+            k = (i + j) / 2;
+            if (xi < x(k)) j = k;
+            else i = k;
+        }
+        k = i;
+    }
+    double lb[4], sum = 0.0;
+    for (int i = 1; i <= np + 1; i++) {
+        lb[i - 1] = 1.0;
+        for (int j = 1; j <= np + 1; j++)
+            if (i != j) lb[i - 1] = lb[i - 1] * (xi - x(j + k - 1)) / (x(i + k - 1) - x(j + k - 1));
+        sum += lb[i - 1];
+    }
+    double w = 0.0;
+    for (int i = 1; i <= np + 1; i++) w += lb[i - 1] / sum * v[i + k - 2];
+    return w;
+}
 
 extern "C" const char *cms2d_last_error(void) { return g_err.c_str(); }
 
@@ -644,6 +689,9 @@ static int flow_imp_once(cms2d_flow *F) {
     double w = 0.0;
     for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
     const double wsebnd = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);     // bnd_eval, boundaries.F90:4109-4122
+    double wsecurve = 0.0;                                             // bnd_eval, boundaries.F90:4127-4150
+    if (!F->wse_t.empty())
+        wsecurve = (1.0 - ramp) * 0.0 + ramp * (wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc) + 0.0);
     if (int rc = wetdry(F)) return rc;
     F->P.tauwx = F->P.tauwy = 0.0;
     if (F->windconst) {
@@ -669,7 +717,8 @@ static int flow_imp_once(cms2d_flow *F) {
         for (int sd = 0; sd < 4; sd++) {
             if (g.side[sd] != 1) continue;
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
-            hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd, wsebnd);
+            hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd,
+                               F->side_type[sd] == CMS2D_BND_WSE_CURVE ? wsecurve : wsebnd);
         }
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
@@ -793,9 +842,21 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         d->coriolis_plane > 1 || d->turbulence < 0 || d->turbulence > 4 || d->friction < 0 || d->friction > 3)
         return set_err(CMS2D_ERR_ARG, "invalid flow description");
     if ((long)(d->nx + 2) * (d->ny + 2) > (1L << 30)) return set_err(CMS2D_ERR_ARG, "grid too large");
-    for (int k = 0; k < 4; k++)
-        if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC)
+    bool curve = false;
+    for (int k = 0; k < 4; k++) {
+        if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC &&
+            d->side_type[k] != CMS2D_BND_WSE_CURVE)
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
+        curve = curve || d->side_type[k] == CMS2D_BND_WSE_CURVE;
+    }
+    if (curve) {
+        if (d->wse_ntimes < 1 || !d->wse_times || !d->wse_values || d->wse_order < 0 || d->wse_order > 3)
+            return set_err(CMS2D_ERR_ARG, "water-level curve missing or invalid");
+        for (int i = 0; i < d->wse_ntimes; i++)
+            if (!std::isfinite(d->wse_times[i]) || !std::isfinite(d->wse_values[i]) ||
+                (i > 0 && !(d->wse_times[i] > d->wse_times[i - 1])))
+                return set_err(CMS2D_ERR_ARG, "water-level curve times must increase, values be finite");
+    }
     if (d->precision != CMS2D_PREC_FP64 && d->precision != CMS2D_PREC_FP32_REFINE)
         return set_err(CMS2D_ERR_ARG, "unknown precision form");
     if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
@@ -812,7 +873,23 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     F->dev = d->device;
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipStreamCreate(&F->st));
-    fill_geo(F->g, d->nx, d->ny, d->dx, d->dy, d->side_type);
+    {
+        int open[4];
+        for (int k = 0; k < 4; k++) {
+            F->side_type[k] = d->side_type[k];
+            open[k] = d->side_type[k] == CMS2D_BND_WALL ? 0 : 1;
+        }
+        fill_geo(F->g, d->nx, d->ny, d->dx, d->dy, open);
+        bool curve = false;
+        for (int k = 0; k < 4; k++) curve = curve || d->side_type[k] == CMS2D_BND_WSE_CURVE;
+        if (curve) {
+            F->wse_t.assign(d->wse_times, d->wse_times + d->wse_ntimes);
+            F->wse_v.assign(d->wse_values, d->wse_values + d->wse_ntimes);
+            // H_str%nti = 2 by default, limited to 1..ntimes-1 (boundaries.F90:1417,1869)
+            F->wse_order = std::max(std::min(d->wse_order ? d->wse_order : 2, d->wse_ntimes - 1), 1);
+            F->wse_inc = 1;
+        }
+    }
     Geo &g = F->g;
     F->npad = g.nxp * g.nyp;
     F->nblk_act = (g.nact + BLOCK - 1) / BLOCK;
@@ -1195,6 +1272,7 @@ struct CkHost {
     int kry[3], prev_iters[4], prev_gm;
     int windconst;
     double wndx, wndy;
+    int wse_inc;
 };
 const char CK_MAGIC[8] = {'C', 'M', 'S', '2', 'D', 'C', 'K', '1'};
 }  // namespace
@@ -1227,6 +1305,7 @@ extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
     for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
     c.windconst = F->windconst; c.wndx = F->wndx; c.wndy = F->wndy;
+    c.wse_inc = F->wse_inc;
     memcpy(q, &c, sizeof c);
     q += sizeof c;
     for (auto &a : F->state) {
@@ -1264,6 +1343,7 @@ extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes)
     F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
     for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
     F->windconst = c.windconst; F->wndx = c.wndx; F->wndy = c.wndy;
+    F->wse_inc = c.wse_inc;
     return CMS2D_OK;
 }
 

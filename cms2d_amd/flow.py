@@ -45,7 +45,12 @@ class FlowStep:
         for k in range(4):
             d.side_type[k] = _lib.BND_WALL
         if case.th_side >= 0:
-            d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
+            if case.wse_times is not None:
+                d.side_type[case.th_side] = _lib.BND_WSE_CURVE
+                d.wse_times, d.wse_values = arr(case.wse_times), arr(case.wse_values)
+                d.wse_ntimes, d.wse_order = len(case.wse_times), case.wse_order
+            else:
+                d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
         d.ntc = len(case.amp)
         for k in range(d.ntc):
             d.amp[k], d.speed[k], d.phase[k] = case.amp[k], case.speed_rad_hr[k], case.phase_rad[k]

@@ -19,7 +19,7 @@ module cms2d_hip
 
     integer(c_int), parameter, public :: CMS2D_OK = 0, CMS2D_ERR_ARG = -1, CMS2D_ERR_HIP = -2, &
                                          CMS2D_ERR_DIVERGED = -3, CMS2D_ERR_NODEVICE = -4
-    integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1
+    integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1, CMS2D_BND_WSE_CURVE = 2
     integer(c_int), parameter, public :: CMS2D_PREC_FP64 = 0
     integer(c_int), parameter, public :: CMS2D_PRECOND_MG = 0, CMS2D_PRECOND_JACOBI = 1
     integer(c_int), parameter, public :: CMS2D_SOLVER_BICGSTAB = 0, CMS2D_SOLVER_GMRES = 1
@@ -45,6 +45,8 @@ module cms2d_hip
         integer(c_int) :: coriolis_plane = 0
         integer(c_int) :: turbulence = 0
         integer(c_int) :: friction = 0
+        type(c_ptr) :: wse_times = c_null_ptr, wse_values = c_null_ptr
+        integer(c_int) :: wse_ntimes = 0, wse_order = 0
     end type
 
     ! struct cms2d_step_info

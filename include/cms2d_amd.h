@@ -48,6 +48,10 @@ extern "C" {
 /* Boundary types of a side. */
 #define CMS2D_BND_WALL 0          /* closed wall (W_str, boundaries.F90:2413-2435)            */
 #define CMS2D_BND_WSE_HARMONIC 1  /* harmonic water level (TH_str, boundaries.F90:4098-4124) */
+#define CMS2D_BND_WSE_CURVE 2     /* single water level from a time series, WSE_CURVE / WSE_CONSTANT
+                                     (H_str, boundaries.F90:4127-4150): the curve wse_times/
+                                     wse_values, Lagrange interpolation of order wse_order
+                                     (plagr_fit, plagr_lib.f90:20-138)                       */
 
 /* Preconditioner of the pressure-correction (pp) BiCGStab.  The momentum
  * systems are diagonally dominant and always use Jacobi. */
@@ -114,6 +118,11 @@ typedef struct {
                                      (flow.F90:261-269,720-736)                               */
     int turbulence;               /* TURBULENCE_MODEL, CMS2D_TURB_* (flow.F90:362-384)        */
     int friction;                 /* bottom friction law, CMS2D_FRIC_* (friction.F90:201-256) */
+    const double *wse_times;      /* CMS2D_BND_WSE_CURVE sides: curve times [hr], increasing,  */
+    const double *wse_values;     /* water levels [m] (offset included), wse_ntimes each;      */
+    int wse_ntimes;               /* one point is a constant level (WSE_CONSTANT)             */
+    int wse_order;                /* TIME_INTERP_ORDER 1-3 (0 -> 2, the reference's default),
+                                     limited to wse_ntimes - 1 (boundaries.F90:1417,1869)     */
 } cms2d_flow_desc;
 
 typedef struct {

@@ -75,6 +75,7 @@ struct cms_oracle {
     int nTH, *THc, *THk;
     double *wsebnd, *wseadj;
     int wseadjust;
+    int wse_inc;          /* window start of the water-level curve (H_str%inc) */
     /* spatially constant wind (met_def windconst, wndx, wndy) */
     int windconst;
     double wndx, wndy, tauwx, tauwy;   /* the vector and the stress of the current step */
@@ -342,6 +343,7 @@ static void build_bnd(cms_oracle *o)
     o->wsebnd = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadj = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadjust = o->nTH >= 3;          /* boundaries.F90:1370,1808 */
+    o->wse_inc = 1;
     char *claimed = (char *)xcalloc((size_t)o->ncellsD, 1);
     for (int j = 0; j < o->nTH; j++) claimed[o->c2c[4 * o->THc[j] + o->THk[j]]] = 1;
     int n = 0;
@@ -550,11 +552,64 @@ static void flow_wetdry(cms_oracle *o)
         }
 }
 
+/* plagr_fit (plagr_lib.f90:20-138) followed by bnd_eval's weighted sum
+ * (boundaries.F90:4129-4130).  Arrays are 1-based through X(i), Y(i). */
+double cms_oracle_plagr(int m, const double *x, const double *y, int ni, double xi, int *kk)
+{
+#define X(i) x[(i) - 1]
+#define Y(i) y[(i) - 1]
+    int np = ni < m - 1 ? ni : m - 1;
+    int k = *kk;
+    if (xi < X(1)) { *kk = 1; return Y(1); }
+    if (xi > X(m)) { *kk = m - np; return Y(m); }
+    if (k >= 1) {                       /* sequential search, forwards then backwards */
+        int i = k;
+        for (k = i; k <= m - np; k++)
+            if (xi <= X(k + np)) break;
+        if (k > m - np || xi < X(k) || xi > X(k + np)) {
+            for (k = i; k >= 1; k--)
+                if (X(k) <= xi) break;
+            if (k < 1 || k + np > m || xi < X(k) || xi > X(k + np)) k = 0;
+        }
+    }
+    if (k <= 0) {                       /* bisection */
+        int i = 1, j = m - np + 1;
+        while (j > i + 1) {
+            k = (i + j) / 2;
+            if (xi < X(k)) j = k; else i = k;
+        }
+        k = i;
+    }
+    double lb[4], sum = 0.0, w = 0.0;
+    for (int i = 1; i <= np + 1; i++) {
+        lb[i - 1] = 1.0;
+        for (int j = 1; j <= np + 1; j++)
+            if (i != j) lb[i - 1] = lb[i - 1] * (xi - X(j + k - 1)) / (X(i + k - 1) - X(j + k - 1));
+        sum += lb[i - 1];
+    }
+    for (int i = 1; i <= np + 1; i++) w += lb[i - 1] / sum * Y(i + k - 1);
+    *kk = k;
+    return w;
+#undef X
+#undef Y
+}
+
 /* ---------------------------------------------------------------------------
- * Boundary values: bnd_eval harmonic branch (boundaries.F90:4098-4124)
+ * Boundary values: bnd_eval harmonic branch (boundaries.F90:4098-4124) or the
+ * single water level curve (H_str, :4127-4150; nti = 2 by default, limited to
+ * 1..ntimes-1, :1417,1869)
  * ------------------------------------------------------------------------- */
 static void bnd_eval(cms_oracle *o)
 {
+    if (o->c.wse_ntimes > 0) {
+        int ni = o->c.wse_order ? o->c.wse_order : 2;
+        if (ni > o->c.wse_ntimes - 1) ni = o->c.wse_ntimes - 1;
+        if (ni < 1) ni = 1;
+        const double w = cms_oracle_plagr(o->c.wse_ntimes, o->c.wse_times, o->c.wse_values, ni, o->timehrs,
+                                          &o->wse_inc);
+        for (int j = 0; j < o->nTH; j++) o->wsebnd[j] = (1.0 - o->ramp) * 0.0 + o->ramp * (w + 0.0);
+        return;
+    }
     for (int j = 0; j < o->nTH; j++) {
         double w = 0.0;
         for (int k = 0; k < o->c.ntc; k++)

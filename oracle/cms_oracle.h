@@ -54,6 +54,10 @@ typedef struct {
                                  3 subgrid (Wu), 5 subgrid (the reference's default)   */
     int mbedfric;             /* bottom friction law of `mann`: 1 drag coefficient, 2 Manning n
                                  (0 is read as 2), 3 roughness length z0, 4 linear coefficient */
+    const double *wse_times;  /* wse_ntimes > 0: the th_side string is a single water level  */
+    const double *wse_values; /* curve (H_str) [hr], [m] instead of the harmonics          */
+    int wse_ntimes;
+    int wse_order;            /* TIME_INTERP_ORDER (0 -> 2)                               */
 } cms_case;
 
 typedef struct {
@@ -89,6 +93,9 @@ double cms_oracle_wall_coef(double yp, double y0);
 double cms_oracle_man2drag(double g, double cman, double h, double hmax);
 double cms_oracle_drag2length(double cd, double h);
 double cms_oracle_drag2man(double g, double cd, double h, double hmax);
+/* plagr_fit + the weighted sum of bnd_eval: curve (x, y) of m points at xi,
+ * order ni, window start *k (1-based, updated) */
+double cms_oracle_plagr(int m, const double *x, const double *y, int ni, double xi, int *k);
 double cms_oracle_length2drag(double z0, double h);
 
 /* Gradient of an active-cell field (row-major) with the current wet-only CBGG

@@ -28,6 +28,7 @@ class CmsCase(C.Structure):
         ("solver_mode", C.c_int), ("rtol", C.c_double), ("atol", C.c_double), ("kmax", C.c_int),
         ("serial", C.c_int), ("dtvar", C.c_int), ("latitude", C.c_double), ("coriolis_plane", C.c_int),
         ("mturbul", C.c_int), ("mbedfric", C.c_int),
+        ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
     ]
 
 
@@ -62,6 +63,8 @@ def lib():
             f.restype = C.c_double
             f.argtypes = ([C.c_int] if fn == "cms_oracle_schmcoef" else []) + \
                 [C.c_double] * (na - (1 if fn == "cms_oracle_schmcoef" else 0))
+        L.cms_oracle_plagr.restype = C.c_double
+        L.cms_oracle_plagr.argtypes = [C.c_int, D, D, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
         L.cms_oracle_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
         L.cms_oracle_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
@@ -114,6 +117,9 @@ class Oracle:
         c.latitude, c.coriolis_plane = case.latitude, case.coriolis_plane
         c.mturbul = case.mturbul
         c.mbedfric = getattr(case, "mbedfric", 2)
+        if getattr(case, "wse_times", None) is not None:
+            c.wse_times, c.wse_values = arr(case.wse_times), arr(case.wse_values)
+            c.wse_ntimes, c.wse_order = len(case.wse_times), case.wse_order
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

@@ -86,6 +86,26 @@ for _law, _nm in ((1, "drag"), (3, "roughness"), (4, "linear")):
     CASES[f"tidal_fric_{_nm}"] = (_mk_fric(_law), 40, [1, 10, 20, 40])
 
 
+# single water level boundaries on the tidal case instead of the tide: a
+# five-point curve over the run with the default quadratic interpolation, the
+# same with cubic interpolation, and a constant level (WSE_CONSTANT)
+WSE_T = [0.0, 0.1, 0.2, 0.3, 0.4]
+WSE_V = [0.0, 0.125, 0.0625, 0.25, 0.1875]
+
+
+def _mk_wse(times, values, order):
+    def mk():
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.wse_times, c.wse_values, c.wse_order = list(times), list(values), order
+        return c
+    return mk
+
+
+CASES["tidal_wse_curve"] = (_mk_wse(WSE_T, WSE_V, 0), 40, [1, 10, 20, 40])
+CASES["tidal_wse_cubic"] = (_mk_wse(WSE_T, WSE_V, 3), 40, [1, 10, 20, 40])
+CASES["tidal_wse_const"] = (_mk_wse([0.0], [0.125], 0), 40, [1, 10, 20, 40])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

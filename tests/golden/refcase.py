@@ -68,8 +68,20 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
         f.write("1\n")
         f.write(f"{len(ids)}\n")
         f.write(" ".join(str(v) for v in ids) + "\n")
-    harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"
-                   for s, a, p in zip(case.speed_deg_hr, case.amp, case.phase_deg))
+    if getattr(case, "wse_times", None) is not None:
+        # single water level from an XYS curve (wse_block, boundaries.F90:482-485), hours / m
+        with open(os.path.join(d, f"{name}_wse.xys"), "w") as f:
+            f.write(f"XYS 1 {len(case.wse_times)}\n")
+            f.write("".join(f"{t:.9g} {v:.9g}\n" for t, v in zip(case.wse_times, case.wse_values)))
+        order = f"    TIME_INTERP_ORDER {case.wse_order}\n" if case.wse_order else ""
+        if len(case.wse_times) == 1:        # one point: the constant level of WSE_CONSTANT (:476-480)
+            forcing = f"  WSE_BEGIN\n    WSE_CONSTANT {case.wse_values[0]:.9g}\n  WSE_END\n"
+        else:
+            forcing = f'  WSE_BEGIN\n    CURVE "{name}_wse.xys"\n{order}  WSE_END\n'
+    else:
+        harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"
+                       for s, a, p in zip(case.speed_deg_hr, case.amp, case.phase_deg))
+        forcing = f"  HARMONIC_BEGIN\n{harm}  HARMONIC_END\n"
     if case.wind_speed:
         # constant wind as an SMS XYS curve pair (read_xys, input/in_lib.f90:22-65):
         # hours, speed [m/s] / direction it comes from [deg clockwise from north]
@@ -111,9 +123,7 @@ VERTICAL_PROJECTION_END
 BOUNDARY_BEGIN
   NAME "tide"
   CELLSTRING "{name}.bid" "1"
-  HARMONIC_BEGIN
-{harm}  HARMONIC_END
-BOUNDARY_END
+{forcing}BOUNDARY_END
 END_PARAMETERS
 """
     with open(os.path.join(d, name + ".cmcards"), "w") as f:

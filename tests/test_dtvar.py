@@ -43,9 +43,16 @@ def test_oracle_recovers_with_variable_step():
 
 
 @pytest.mark.gpu
-def test_gpu_variable_step_matches_oracle(gpu_available):
+@pytest.mark.parametrize("curve", [False, True], ids=["tide", "wse_curve"])
+def test_gpu_variable_step_matches_oracle(gpu_available, curve):
+    """With a water-level curve on the open side (quadratic interpolation over
+    uneven times), redone steps move the time back inside the curve's windows,
+    so the window search state is exercised too."""
     c, p = dam_break()
-    print("dam break 48x32 dt 300", flush=True)
+    if curve:
+        c.wse_times = [0.0, 0.05, 0.1, 0.2, 0.3, 0.5, 1.0]
+        c.wse_values = [0.0, 0.2, -0.1, 0.3, 0.1, 0.4, 0.2]
+    print(f"dam break 48x32 dt 300 curve={curve}", flush=True)
     g = FlowStep(c)
     g.set("p", p)
     with pytest.raises(_lib.Cms2dError) as e:

@@ -328,3 +328,45 @@ def test_wind_vector_follows_met_convention():
     c.wind_dir = 270.0                                            # from the west
     wx, wy = c.wind_vector()
     assert wx == pytest.approx(10.0) and abs(wy) < 1e-12
+
+
+def _plagr(x, y, order, xi, k):
+    import ctypes as C
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    D = C.POINTER(C.c_double)
+    kk = C.c_int(k)
+    v = lib().cms_oracle_plagr(len(x), x.ctypes.data_as(D), y.ctypes.data_as(D), order, xi, C.byref(kk))
+    return v, kk.value
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_curve_interpolation_reproduces_polynomials(order):
+    """plagr_fit's Lagrange weights reproduce polynomials up to their order on
+    uneven times; outside the curve the end values hold."""
+    x = np.array([0.0, 0.13, 0.3, 0.42, 0.7, 1.0])
+    p = np.poly1d(np.arange(1, order + 2, dtype=float))
+    k = 1
+    for xi in np.linspace(0.0, 1.0, 23):
+        v, k = _plagr(x, p(x), order, xi, k)
+        assert v == pytest.approx(p(xi), rel=1e-12, abs=1e-12)
+    assert _plagr(x, p(x), order, -0.5, k)[0] == p(x[0])
+    assert _plagr(x, p(x), order, 1.5, k)[0] == p(x[-1])
+
+
+def test_curve_window_follows_the_search_state():
+    """The window start is the first one whose span reaches the time when the
+    search moves forwards, and stays put when the time goes back inside it
+    (a redone step): on y = 0,1,0,1,0 the quadratic through the first three
+    points gives 0.96 at 0.3, the one through points 2-4 gives 0.64."""
+    x = [0.0, 0.25, 0.5, 0.75, 1.0]
+    y = [0.0, 1.0, 0.0, 1.0, 0.0]
+    v, k = _plagr(x, y, 2, 0.3, 1)
+    assert (v, k) == (pytest.approx(0.96, abs=1e-14), 1)
+    v, k = _plagr(x, y, 2, 0.6, k)
+    assert k == 2
+    v, k = _plagr(x, y, 2, 0.3, k)
+    assert (v, k) == (pytest.approx(0.64, abs=1e-14), 2)
+    # back before the window: the backward search takes the last start at or before the time
+    v, k = _plagr(x, y, 2, 0.1, k)
+    assert k == 1 and v == pytest.approx(1 - ((0.1 - 0.25) / 0.25) ** 2, abs=1e-14)

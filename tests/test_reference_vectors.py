@@ -102,7 +102,24 @@ def _tidal_fric(law):
 FRIC = {"tidal_fric_drag": 1, "tidal_fric_roughness": 3, "tidal_fric_linear": 4}
 for _name, _law in FRIC.items():
     CASES[_name] = _tidal_fric(_law)
-ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC]
+
+
+def _tidal_wse(times, values, order):
+    def mk():
+        c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+        c.wse_times, c.wse_values, c.wse_order = list(times), list(values), order
+        return c
+    return mk
+
+
+# single water level boundaries (WSE_BEGIN ... CURVE / WSE_CONSTANT ... WSE_END)
+WSE_T = [0.0, 0.1, 0.2, 0.3, 0.4]
+WSE_V = [0.0, 0.125, 0.0625, 0.25, 0.1875]
+WSE = {"tidal_wse_curve": (WSE_T, WSE_V, 0), "tidal_wse_cubic": (WSE_T, WSE_V, 3),
+       "tidal_wse_const": ([0.0], [0.125], 0)}
+for _name, _args in WSE.items():
+    CASES[_name] = _tidal_wse(*_args)
+ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE]
 
 
 def compare_to_gold(name, model_factory):
@@ -192,3 +209,18 @@ def test_friction_law_is_what_the_reference_uses(name):
         return Oracle(c, serial=1)
     w = compare_to_gold(name, manning)
     assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_wse_curve_order_is_what_the_reference_uses(order):
+    """The default quadratic interpolation of the curve, replaced by linear or
+    cubic, misses the reference's run by more than the bounds (or wets and
+    dries other cells)."""
+    def other(c):
+        c.wse_order = order
+        return Oracle(c, serial=1)
+    try:
+        w = compare_to_gold("tidal_wse_curve", other)
+    except AssertionError:
+        return                     # wet/dry flags differ already
+    assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
