@@ -144,8 +144,8 @@ template <int TX, class TB>
 __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, MgLevel C, const KryScal *sc) {
     constexpr int MT_X = MTile<TX>::X, MT_Y = MTile<TX>::Y, MT_RING = MTile<TX>::RING;
     if (sc->done) return;
-    // 1/ap of the black cells is read again from global memory (a cache hit
-    // after the staging below) rather than kept in LDS: without that second
+    // 1/ap of the black cells is read into registers with the coefficients
+    // (the staging reads the same lines) rather than kept in LDS: without that second
     // array the workgroup fits seven times on a CU, and the 1504 tiles of a
     // 2000 x 1500 grid run in one round instead of two
     __shared__ float q[MT_Y + 4][MT_X + 4];
@@ -154,16 +154,18 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
     mg_tile_origin<TX>(X0, Y0);
     // the owned cells' coefficients are requested before the staging loads,
     // so both are in flight together instead of one after the barrier
-    float a[MT_OWN][4];
+    float a[MT_OWN][4], dk[MT_OWN];
 #pragma unroll
     for (int k = 0; k < MT_OWN; k++) {
         int fx, fy;
         mg_own<TX>(threadIdx.x, k, fx, fy);
         const int i = X0 + fx, j = Y0 + fy;
         a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0f;
+        dk[k] = 0.0f;
         if (mg_inside(f, i, j)) {
             const int c = f.idx(i, j);
             a[k][0] = f.aN[c]; a[k][1] = f.aE[c]; a[k][2] = f.aS[c]; a[k][3] = f.aW[c];
+            if ((i + j) & 1) dk[k] = f.dinv[c];
         }
     }
     for (int t = threadIdx.x; t < (MT_X + 4) * (MT_Y + 4); t += MT_THREADS) {
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(MT_THREADS) k_mg_down(MgLevel f, const TB *b, 
             const int c = f.idx(i, j);
             if (((i + j) & 1) == 0) z = q[ly][lx];
             else z = mg_black(a[k][0], a[k][1], a[k][2], a[k][3], q[ly + 1][lx], q[ly][lx + 1], q[ly - 1][lx],
-                              q[ly][lx - 1], q[ly][lx], f.dinv[c]);
+                              q[ly][lx - 1], q[ly][lx], dk[k]);
             f.z[c] = z;
         }
         zt[fy + 1][fx + 1] = z;
