@@ -106,6 +106,20 @@ CASES["tidal_wse_cubic"] = (_mk_wse(WSE_T, WSE_V, 3), 40, [1, 10, 20, 40])
 CASES["tidal_wse_const"] = (_mk_wse([0.0], [0.125], 0), 40, [1, 10, 20, 40])
 
 
+# the tidal case without wall friction (USE_WALL_FRICTION_TERMS OFF), and the
+# tidal case run by the reference's default MATRIX_SOLVER GMRES (its restarted
+# GMRES with ILUT; the converged answer does not depend on the Krylov method)
+def _mk_nowall():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.wallfric = 0
+    return c
+
+
+CASES["tidal_nowallfric"] = (_mk_nowall, 40, [1, 10, 20, 40])
+CASES["tidal_refgmres"] = (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1), 40, [1, 10, 20, 40],
+                           {"solver": "GMRES"})
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 
@@ -115,10 +129,10 @@ def main():
         print("oracle/_ref/ref_run not built (python oracle/build_ref.py)", file=sys.stderr)
         return 1
     todo = only(sys.argv[1:]) if len(sys.argv) > 1 else CASES
-    for name, (mk, nsteps, keep) in todo.items():
+    for name, (mk, nsteps, keep, *opt) in todo.items():
         c = mk()
         with tempfile.TemporaryDirectory() as d:
-            res = refcase.run(c, d + "/", nsteps)
+            res = refcase.run(c, d + "/", nsteps, **(opt[0] if opt else {}))
         out = {"steps": np.array(keep)}
         for s in keep:
             for f in ("eta", "u", "v", "iwet"):

@@ -119,7 +119,18 @@ WSE = {"tidal_wse_curve": (WSE_T, WSE_V, 0), "tidal_wse_cubic": (WSE_T, WSE_V, 3
        "tidal_wse_const": ([0.0], [0.125], 0)}
 for _name, _args in WSE.items():
     CASES[_name] = _tidal_wse(*_args)
-ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE]
+
+
+def _tidal_nowall():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.wallfric = 0                                  # USE_WALL_FRICTION_TERMS OFF
+    return c
+
+
+CASES["tidal_nowallfric"] = _tidal_nowall
+# the plain tidal case as the reference runs it with MATRIX_SOLVER GMRES
+CASES["tidal_refgmres"] = CASES["tidal"]
+ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
@@ -224,3 +235,13 @@ def test_wse_curve_order_is_what_the_reference_uses(order):
     except AssertionError:
         return                     # wet/dry flags differ already
     assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
+
+
+def test_wall_friction_switch_is_what_the_reference_uses():
+    """With wall friction on, the oracle misses the USE_WALL_FRICTION_TERMS OFF
+    run by about four times the velocity bound."""
+    def withwall(c):
+        c.wallfric = 1
+        return Oracle(c, serial=1)
+    w = compare_to_gold("tidal_nowallfric", withwall)
+    assert w["u"] > 3 * TOL_VEL or w["v"] > 3 * TOL_VEL, w
