@@ -936,7 +936,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                           &s.spv, &s.ssu0, &s.ssv0, &s.spuv0, &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu,
                           &s.Hv, &s.resu, &s.resv, &s.resp, &s.fluxE, &s.fluxN, &s.hkE, &s.hkN, &s.viskE,
                           &s.viskN, &s.zbkE, &s.zbkN, &s.aN, &s.aE, &s.aS, &s.aW, &s.wavestrx, &s.wavestry,
-                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1, &s.fc, &s.cbf};
+                          &s.tu, &s.tv, &s.tvis, &s.fluxE1, &s.fluxN1, &s.fc, &s.cbf, &s.maxeta};
     for (double **f : dfields) {
         if (int rc = dalloc(F, f, N)) return rc;
         F->state.push_back({*f, N * sizeof(double)});
@@ -1189,7 +1189,7 @@ static double *field_ptr(cms2d_flow *F, const char *name) {
         {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
         {"coefman", s.coefman},
         {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy}, {"z0", s.z0},
-        {"zb", s.zb}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}, {"fc", s.fc}};
+        {"zb", s.zb}, {"maxeta", s.maxeta}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry}, {"fc", s.fc}};
     for (auto &t : tab)
         if (!strcmp(t.n, name)) return t.p;
     return nullptr;

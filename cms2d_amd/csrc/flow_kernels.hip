@@ -1104,6 +1104,7 @@ __global__ void k_update(Geo g, St s) {
     s.u1[t] = s.u[t]; s.v1[t] = s.v[t]; s.p1[t] = s.p[t]; s.h1[t] = s.h[t];
     s.fluxE1[t] = s.fluxE[t]; s.fluxN1[t] = s.fluxN[t];
     s.pp[t] = 0.0; s.dppx[t] = 0.0; s.dppy[t] = 0.0;
+    s.maxeta[t] = fmax(s.eta[t], s.maxeta[t]);   // maximum water level, CMS_Flow.f90:352-355
 }
 
 // flow_recalculate (flow_imp.F90:1161-1173): back to the previous time level.

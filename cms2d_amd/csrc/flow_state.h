@@ -11,6 +11,7 @@ struct St {
     double *dppx, *dppy, *dux, *duy, *dvx, *dvy, *dpx, *dpy;
     double *coefman, *cfrict, *z0, *uv, *uelwc, *bsvel;
     double *cbf;               // linear friction coefficient (cbotfric of mbedfric 4)
+    double *maxeta;            // running maximum of eta (flow_def maxeta)
     double *su, *sv, *sp, *spu, *spv, *ssu0, *ssv0, *spuv0, *sppp0, *sspp0;
     double *apuareap, *sumu, *Hu, *Hv, *resu, *resv, *resp;
     double *fluxE, *fluxN, *hkE, *hkN, *viskE, *viskN, *zbkE, *zbkN;   // unique faces

@@ -147,7 +147,9 @@ int cms2d_flow_destroy(cms2d_flow *flow);
 /* Advances nsteps time steps: timing, flow_imp and flow_update per step. */
 int cms2d_flow_step(cms2d_flow *flow, int nsteps, cms2d_step_info *info);
 /* Copies an active-cell field to host memory, row-major [nx*ny]:
- * "eta","u","v","p","h","iwet","vis","cfrict","uv","bsvel","pp","dpx","dpy". */
+ * "eta","u","v","p","h","iwet","vis","cfrict","coefman","z0","uv","bsvel","pp","dpx","dpy",
+ * "zb","fc","wavestrx","wavestry", and "maxeta" (running maximum of eta over the steps,
+ * CMS_Flow.f90:352-355). */
 int cms2d_flow_get(cms2d_flow *flow, const char *name, double *out);
 /* Directed face field of every active cell, out[4*c+k], k = N,E,S,W:
  * "flux","hk","visk". */

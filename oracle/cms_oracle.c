@@ -88,6 +88,7 @@ struct cms_oracle {
     double *apuareap, *sumu, *Hu, *Hv, *rsu, *rsv, *rsp;
     double *coefman, *cfrict, *z0, *uv, *uelwc, *bsxy, *bsvel, *cmb, *cbcfuwcap;
     double *cbf;   /* linear friction coefficient (cbotfric of mbedfric 4) */
+    double *maxeta;   /* running maximum of eta over the steps taken */
     double *wavestrx, *wavestry, *fc;
     gradop *gow;
     /* control (comvarbl) */
@@ -1443,6 +1444,8 @@ static void flow_update(cms_oracle *o)
         o->pp[i] = 0.0; o->dppx[i] = 0.0; o->dppy[i] = 0.0;
     }
     memcpy(o->flux1, o->flux, sizeof(double) * 4 * nd);
+    /* maximum water level, :352-355 (maxeta starts at 0, flow.F90:874) */
+    for (int i = 0; i < nd; i++) o->maxeta[i] = fmax(o->eta[i], o->maxeta[i]);
 }
 
 /* boundary inflow volume over one step from the final boundary fluxes */
@@ -1482,7 +1485,7 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     A(su); A(sv); A(sp); A(spu); A(spv); A(ssu0); A(ssv0); A(spuv0); A(sppp0); A(sspp0);
     A(apuareap); A(sumu); A(Hu); A(Hv); A(rsu); A(rsv); A(rsp);
     A(coefman); A(cfrict); A(z0); A(uv); A(uelwc); A(bsxy); A(bsvel); A(cmb); A(cbcfuwcap);
-    A(wavestrx); A(wavestry); A(fc); A(cbf);
+    A(wavestrx); A(wavestry); A(fc); A(cbf); A(maxeta);
 #undef A
     o->iwet = IALLOC(nd); o->iwet1 = IALLOC(nd); o->icorner = IALLOC(nd);
     o->flux = DALLOC(F); o->flux1 = DALLOC(F); o->hk = DALLOC(F); o->pk = DALLOC(F);
@@ -1576,7 +1579,7 @@ void cms_oracle_destroy(cms_oracle *o)
                     o->su, o->sv, o->sp, o->spu, o->spv, o->ssu0, o->ssv0, o->spuv0, o->sppp0, o->sspp0,
                     o->apuareap, o->sumu, o->Hu, o->Hv, o->rsu, o->rsv, o->rsp, o->coefman, o->cfrict, o->z0,
                     o->uv, o->uelwc, o->bsxy, o->bsvel, o->cmb, o->cbcfuwcap, o->wavestrx, o->wavestry, o->fc,
-                    o->cbf, o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
+                    o->cbf, o->maxeta, o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(o);
 }
@@ -1654,6 +1657,7 @@ static const double *field(const cms_oracle *o, const char *name)
     if (!strcmp(name, "bsvel")) return o->bsvel;
     if (!strcmp(name, "zb")) return o->zb;
     if (!strcmp(name, "z0")) return o->z0;
+    if (!strcmp(name, "maxeta")) return o->maxeta;
     return NULL;
 }
 

@@ -260,3 +260,16 @@ def test_friction_laws_match_oracle(gpu_available, law):
             assert d <= 1e-6, (f, d)
         print(f"friction law {law} {case.nx}x{case.ny}: {err}", flush=True)
         g.close()
+
+
+@pytest.mark.gpu
+def test_maxeta_matches_oracle(gpu_available):
+    """The running maximum of the water level (maxeta) after 30 steps of the
+    wet/dry tidal beach, within the water-level tolerance."""
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    o, g = run_pair(c, 30)
+    compare(o, g)
+    e = np.abs(o.get("maxeta") - g.get("maxeta")).max()
+    print("maxeta err", e, flush=True)
+    assert e <= TOL_ETA
+    g.close()

@@ -370,3 +370,17 @@ def test_curve_window_follows_the_search_state():
     # back before the window: the backward search takes the last start at or before the time
     v, k = _plagr(x, y, 2, 0.1, k)
     assert k == 1 and v == pytest.approx(1 - ((0.1 - 0.25) / 0.25) ** 2, abs=1e-14)
+
+
+def test_maxeta_is_the_running_maximum():
+    """maxeta (CMS_Flow.f90:352-355) is the maximum of 0 and every step's eta,
+    dry cells (eta = -999) included."""
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    o = Oracle(c, serial=0)
+    run = np.zeros(c.nx * c.ny)
+    for _ in range(15):
+        rc, _ = o.step(1)
+        assert rc == 0
+        run = np.maximum(run, o.get("eta"))
+    assert np.array_equal(o.get("maxeta"), run)
+    assert (o.get("eta") == -999.0).any()          # some cells stayed dry
