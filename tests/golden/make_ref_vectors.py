@@ -26,7 +26,8 @@ CASES = {
     "tidal": (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1), 40, [1, 10, 20, 40]),
 }
 # the NVD deferred corrections (ndsch 5 HLPA, 7 CUBISTA, 9 HOAB) on the tidal case
-for _nd, _nm in ((5, "hlpa"), (7, "cubista"), (9, "hoab")):
+for _nd, _nm in ((5, "hlpa"), (7, "cubista"), (9, "hoab"), (0, "none"), (1, "upwind"), (2, "hybrid"),
+                 (3, "powerlaw"), (6, "gamma"), (8, "alvsmart")):
     def _mk(nd=_nd):
         c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
         c.ndsch = nd

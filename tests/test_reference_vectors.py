@@ -39,6 +39,11 @@ def _tidal_scheme(ndsch):
 NVD = {"tidal_hlpa": 5, "tidal_cubista": 7, "tidal_hoab": 9}
 for _name, _nd in NVD.items():
     CASES[_name] = _tidal_scheme(_nd)
+# the other schemes: NONE, UPWIND, HYBRID, POWERLAW and the NVD GAMMA, ALVSMART
+SCHEMES = {"tidal_none": 0, "tidal_upwind": 1, "tidal_hybrid": 2, "tidal_powerlaw": 3, "tidal_gamma": 6,
+           "tidal_alvsmart": 8}
+for _name, _nd in SCHEMES.items():
+    CASES[_name] = _tidal_scheme(_nd)
 
 
 def _tidal_coriolis(plane):
@@ -130,7 +135,7 @@ def _tidal_nowall():
 CASES["tidal_nowallfric"] = _tidal_nowall
 # the plain tidal case as the reference runs it with MATRIX_SOLVER GMRES
 CASES["tidal_refgmres"] = CASES["tidal"]
-ALL = ["basin", "tidal", *NVD, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+ALL = ["basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
@@ -245,3 +250,14 @@ def test_wall_friction_switch_is_what_the_reference_uses():
         return Oracle(c, serial=1)
     w = compare_to_gold("tidal_nowallfric", withwall)
     assert w["u"] > 3 * TOL_VEL or w["v"] > 3 * TOL_VEL, w
+
+
+@pytest.mark.parametrize("name", list(SCHEMES))
+def test_scheme_is_what_the_reference_uses(name):
+    """With the default exponential scheme instead, the oracle misses each of
+    these runs by more than ten times the velocity bound."""
+    def default(c):
+        c.ndsch = 4
+        return Oracle(c, serial=1)
+    w = compare_to_gold(name, default)
+    assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
