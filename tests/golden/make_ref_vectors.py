@@ -121,6 +121,16 @@ CASES["tidal_refgmres"] = (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1), 4
                            {"solver": "GMRES"})
 
 
+# the tide ramped in over a quarter of an hour (DURATION_RAMP, ramp_func)
+def _mk_ramp():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.rampdur = 0.25
+    return c
+
+
+CASES["tidal_ramp"] = (_mk_ramp, 40, [1, 10, 20, 40])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

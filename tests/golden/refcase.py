@@ -99,6 +99,7 @@ HYDRO_TIMESTEP                      {case.dt:.9g} 'sec'
 STARTING_DATE_TIME                  2001-01-01 00:00:00 UTC
 DURATION_RUN                        {max(1.0, case.dt * case.nsteps / 3600.0):.9g} 'hrs'
 VARIABLE_TIMESTEP                   OFF
+{f"DURATION_RAMP                       {case.rampdur:.9g} 'hrs'" if case.rampdur else ""}
 SOLUTION_SCHEME                     IMPLICIT
 MATRIX_SOLVER                       {solver}
 {"ADVECTION_SCHEME                    " + ADVSC[case.ndsch] if case.ndsch != 4 else ""}

@@ -135,7 +135,14 @@ def _tidal_nowall():
 CASES["tidal_nowallfric"] = _tidal_nowall
 # the plain tidal case as the reference runs it with MATRIX_SOLVER GMRES
 CASES["tidal_refgmres"] = CASES["tidal"]
-ALL = ["basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+def _tidal_ramp():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    c.rampdur = 0.25                                # DURATION_RAMP 0.25 hrs
+    return c
+
+
+CASES["tidal_ramp"] = _tidal_ramp
+ALL = ["tidal_ramp", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
@@ -261,3 +268,16 @@ def test_scheme_is_what_the_reference_uses(name):
         return Oracle(c, serial=1)
     w = compare_to_gold(name, default)
     assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
+
+
+def test_ramp_is_what_the_reference_uses():
+    """Without the quarter-hour ramp the oracle misses the DURATION_RAMP run by
+    far more than the bounds."""
+    def noramp(c):
+        c.rampdur = 0.0
+        return Oracle(c, serial=1)
+    try:
+        w = compare_to_gold("tidal_ramp", noramp)
+    except AssertionError:
+        return                     # wet/dry flags differ already
+    assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
