@@ -142,7 +142,9 @@ def _tidal_ramp():
 
 
 CASES["tidal_ramp"] = _tidal_ramp
-ALL = ["tidal_ramp", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+# the basin at its full 200 x 100 size, 30 steps of 60 s
+CASES["basin_200x100"] = lambda: cases.basin(nx=200, ny=100, dx=10.0, seed=0)
+ALL = ["tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
