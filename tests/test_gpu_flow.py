@@ -273,3 +273,19 @@ def test_maxeta_matches_oracle(gpu_available):
     print("maxeta err", e, flush=True)
     assert e <= TOL_ETA
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["fp32_refine", "gmres", "jacobi"])
+@pytest.mark.parametrize("name", ["tidal", "basin_200x100"])
+def test_gpu_forms_match_reference_vectors(gpu_available, form, name):
+    """The FP32-refinement form, GMRES(20) and the Jacobi-preconditioned pp
+    solve against the compiled reference's own outputs, to the same bounds as
+    the default form."""
+    from test_reference_vectors import TOL_ETA, TOL_VEL, compare_to_gold
+    kw = {"fp32_refine": {"precision": _lib.PREC_FP32_REFINE}, "gmres": {"solver": _lib.SOLVER_GMRES},
+          "jacobi": {"precond": _lib.PRECOND_JACOBI}}[form]
+    print(f"reference vectors {name} {form}", flush=True)
+    w = compare_to_gold(name, lambda c: FlowStep(c, **kw))
+    print(name, form, w)
+    assert w["eta"] <= TOL_ETA and w["u"] <= TOL_VEL and w["v"] <= TOL_VEL, w
