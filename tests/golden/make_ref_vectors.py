@@ -137,6 +137,20 @@ CASES["tidal_ramp"] = (_mk_ramp, 40, [1, 10, 20, 40])
 CASES["basin_200x100"] = (lambda: cases.basin(nx=200, ny=100, dx=10.0, seed=0), 30, [1, 10, 20, 30])
 
 
+# the options together: HOAB scheme, beta-plane Coriolis at 30 N, a 9 m/s wind
+# from the north-east, the FALCONER model, roughness heights and the
+# water-level curve on the open side
+def _mk_combo():
+    c = _mk_fric(3)()
+    c.ndsch, c.latitude, c.coriolis_plane, c.mturbul = 9, 30.0, 1, 1
+    c.wind_speed, c.wind_dir = 9.0, 45.0
+    c.wse_times, c.wse_values, c.wse_order = list(WSE_T), list(WSE_V), 0
+    return c
+
+
+CASES["tidal_combo"] = (_mk_combo, 40, [1, 10, 20, 40])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

@@ -144,7 +144,18 @@ def _tidal_ramp():
 CASES["tidal_ramp"] = _tidal_ramp
 # the basin at its full 200 x 100 size, 30 steps of 60 s
 CASES["basin_200x100"] = lambda: cases.basin(nx=200, ny=100, dx=10.0, seed=0)
-ALL = ["tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+def _tidal_combo():
+    """HOAB, beta-plane Coriolis at 30 N, a 9 m/s north-east wind, FALCONER,
+    roughness heights and the water-level curve together."""
+    c = _tidal_fric(3)()
+    c.ndsch, c.latitude, c.coriolis_plane, c.mturbul = 9, 30.0, 1, 1
+    c.wind_speed, c.wind_dir = 9.0, 45.0
+    c.wse_times, c.wse_values, c.wse_order = list(WSE_T), list(WSE_V), 0
+    return c
+
+
+CASES["tidal_combo"] = _tidal_combo
+ALL = ["tidal_combo", "tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
