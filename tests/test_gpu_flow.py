@@ -289,3 +289,28 @@ def test_gpu_forms_match_reference_vectors(gpu_available, form, name):
     w = compare_to_gold(name, lambda c: FlowStep(c, **kw))
     print(name, form, w)
     assert w["eta"] <= TOL_ETA and w["u"] <= TOL_VEL and w["v"] <= TOL_VEL, w
+
+
+@pytest.mark.parametrize("times,values", [(None, None), ([0.0, 0.2, 0.1], [0.0, 0.1, 0.2]),
+                                          ([0.0, 0.1], [0.0, float("nan")]), ([], [])],
+                         ids=["missing", "not_increasing", "nan", "empty"])
+def test_bad_water_level_curves_rejected(times, values):
+    """A CMS2D_BND_WSE_CURVE side without a usable curve is refused
+    (CMS2D_ERR_ARG) before touching a device."""
+    import ctypes as C
+    zb = np.full(16, -2.0)
+    mann = np.full(16, 0.02)
+    d = _lib.FlowDesc()
+    d.nx = d.ny = 4
+    d.dx = d.dy = 10.0
+    d.zb, d.mann = zb.ctypes.data_as(_lib.D), mann.ctypes.data_as(_lib.D)
+    d.dt, d.hdry, d.kmax, d.rtol = 10.0, 0.05, 100, 1e-10
+    d.side_type[1] = _lib.BND_WSE_CURVE
+    keep = []
+    if times is not None:
+        t, v = np.array(times, dtype=np.float64), np.array(values, dtype=np.float64)
+        keep += [t, v]
+        d.wse_times, d.wse_values, d.wse_ntimes = t.ctypes.data_as(_lib.D), v.ctypes.data_as(_lib.D), len(t)
+    h = C.c_void_p()
+    assert _lib.lib().cms2d_flow_create(C.byref(d), C.byref(h)) == -1
+    assert not h.value
