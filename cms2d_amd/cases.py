@@ -66,6 +66,7 @@ class Case:
     wse_times: object = None  # water-level curve on the open side instead of the harmonics:
     wse_values: object = None # times [hr] and levels [m] (WSE_CURVE), Lagrange order wse_order
     wse_order: int = 0        # (0: the reference's default 2)
+    wse_adjust: bool = True   # WSE_ADJUSTMENT of the curve string (wind and wave set-up)
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 

@@ -126,6 +126,7 @@ struct cms2d_flow {
     int side_type[4]{};        // CMS2D_BND_* per side (the kernels see 1 for every water-level side)
     std::vector<double> wse_t, wse_v;   // water-level curve of CMS2D_BND_WSE_CURVE sides
     int wse_order = 1, wse_inc = 1;     // interpolation order, window start (H_str%inc, 1-based)
+    int wse_adjust_off = 0;             // WSE_ADJUSTMENT OFF on the curve sides
     hipEvent_t step_ev[2]{};
 };
 
@@ -717,8 +718,9 @@ static int flow_imp_once(cms2d_flow *F) {
         for (int sd = 0; sd < 4; sd++) {
             if (g.side[sd] != 1) continue;
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
+            const bool curve = F->side_type[sd] == CMS2D_BND_WSE_CURVE;
             hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd,
-                               F->side_type[sd] == CMS2D_BND_WSE_CURVE ? wsecurve : wsebnd);
+                               curve ? wsecurve : wsebnd, (int)(curve && F->wse_adjust_off));
         }
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
@@ -888,6 +890,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             // H_str%nti = 2 by default, limited to 1..ntimes-1 (boundaries.F90:1417,1869)
             F->wse_order = std::max(std::min(d->wse_order ? d->wse_order : 2, d->wse_ntimes - 1), 1);
             F->wse_inc = 1;
+            F->wse_adjust_off = d->wse_adjust_off != 0;
         }
     }
     Geo &g = F->g;

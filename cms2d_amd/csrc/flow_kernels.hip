@@ -654,12 +654,12 @@ __global__ void k_pp_coef_flux(Geo g, St s) {
 
 // bnd_wse_adjust (boundaries.F90:4620-4686) for one harmonic side: setup from
 // the radiation-stress forcing integrated outward from the centre cell.
-__global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int side, double wsebnd) {
+__global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int side, double wsebnd, int noadj) {
     extern __shared__ double term[];
     const int n = (side == KE || side == KW) ? g.ny : g.nx;
     double *wse = s.wseadj + side * P.lmax;
     const int im0 = n / 2 - 1;
-    const bool adjust = n >= 3;
+    const bool adjust = n >= 3 && !noadj;
     for (int im = threadIdx.x; im < n; im += blockDim.x) {
         double tv = 0.0;
         if (adjust && im != im0) {

@@ -49,6 +49,7 @@ class FlowStep:
                 d.side_type[case.th_side] = _lib.BND_WSE_CURVE
                 d.wse_times, d.wse_values = arr(case.wse_times), arr(case.wse_values)
                 d.wse_ntimes, d.wse_order = len(case.wse_times), case.wse_order
+                d.wse_adjust_off = int(not case.wse_adjust)
             else:
                 d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
         d.ntc = len(case.amp)

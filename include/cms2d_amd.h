@@ -123,6 +123,8 @@ typedef struct {
     int wse_ntimes;               /* one point is a constant level (WSE_CONSTANT)             */
     int wse_order;                /* TIME_INTERP_ORDER 1-3 (0 -> 2, the reference's default),
                                      limited to wse_ntimes - 1 (boundaries.F90:1417,1869)     */
+    int wse_adjust_off;           /* WSE_ADJUSTMENT OFF on the curve sides: no wind and wave
+                                     set-up along the string (boundaries.F90:552-554,3569)    */
 } cms2d_flow_desc;
 
 typedef struct {

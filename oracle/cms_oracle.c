@@ -344,6 +344,7 @@ static void build_bnd(cms_oracle *o)
     o->wsebnd = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadj = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadjust = o->nTH >= 3;          /* boundaries.F90:1370,1808 */
+    if (o->c.wse_ntimes > 0 && o->c.wse_adjust_off) o->wseadjust = 0;   /* WSE_ADJUSTMENT OFF, :552-554 */
     o->wse_inc = 1;
     char *claimed = (char *)xcalloc((size_t)o->ncellsD, 1);
     for (int j = 0; j < o->nTH; j++) claimed[o->c2c[4 * o->THc[j] + o->THk[j]]] = 1;

@@ -58,6 +58,7 @@ typedef struct {
     const double *wse_values; /* curve (H_str) [hr], [m] instead of the harmonics          */
     int wse_ntimes;
     int wse_order;            /* TIME_INTERP_ORDER (0 -> 2)                               */
+    int wse_adjust_off;       /* WSE_ADJUSTMENT OFF on the curve string                   */
 } cms_case;
 
 typedef struct {

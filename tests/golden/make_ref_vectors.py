@@ -107,6 +107,15 @@ CASES["tidal_wse_cubic"] = (_mk_wse(WSE_T, WSE_V, 3), 40, [1, 10, 20, 40])
 CASES["tidal_wse_const"] = (_mk_wse([0.0], [0.125], 0), 40, [1, 10, 20, 40])
 
 
+def _mk_wse_noadj():
+    c = _mk_wse(WSE_T, WSE_V, 0)()
+    c.wse_adjust = False                            # WSE_ADJUSTMENT OFF
+    return c
+
+
+CASES["tidal_wse_noadj"] = (_mk_wse_noadj, 40, [1, 10, 20, 40])
+
+
 # the tidal case without wall friction (USE_WALL_FRICTION_TERMS OFF), and the
 # tidal case run by the reference's default MATRIX_SOLVER GMRES (its restarted
 # GMRES with ILUT; the converged answer does not depend on the Krylov method)

@@ -77,7 +77,8 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
         if len(case.wse_times) == 1:        # one point: the constant level of WSE_CONSTANT (:476-480)
             forcing = f"  WSE_BEGIN\n    WSE_CONSTANT {case.wse_values[0]:.9g}\n  WSE_END\n"
         else:
-            forcing = f'  WSE_BEGIN\n    CURVE "{name}_wse.xys"\n{order}  WSE_END\n'
+            adj = "" if getattr(case, "wse_adjust", True) else "    WSE_ADJUSTMENT OFF\n"
+            forcing = f'  WSE_BEGIN\n    CURVE "{name}_wse.xys"\n{order}{adj}  WSE_END\n'
     else:
         harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"
                        for s, a, p in zip(case.speed_deg_hr, case.amp, case.phase_deg))

@@ -126,6 +126,16 @@ for _name, _args in WSE.items():
     CASES[_name] = _tidal_wse(*_args)
 
 
+def _tidal_wse_noadj():
+    c = _tidal_wse(WSE_T, WSE_V, 0)()
+    c.wse_adjust = False                            # WSE_ADJUSTMENT OFF
+    return c
+
+
+WSE["tidal_wse_noadj"] = None
+CASES["tidal_wse_noadj"] = _tidal_wse_noadj
+
+
 def _tidal_nowall():
     c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
     c.wallfric = 0                                  # USE_WALL_FRICTION_TERMS OFF
@@ -291,6 +301,19 @@ def test_ramp_is_what_the_reference_uses():
         return Oracle(c, serial=1)
     try:
         w = compare_to_gold("tidal_ramp", noramp)
+    except AssertionError:
+        return                     # wet/dry flags differ already
+    assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
+
+
+def test_wse_adjustment_switch_is_what_the_reference_uses():
+    """With the set-up adjustment left on, the oracle misses the WSE_ADJUSTMENT
+    OFF run by more than the bounds."""
+    def adjusted(c):
+        c.wse_adjust = True
+        return Oracle(c, serial=1)
+    try:
+        w = compare_to_gold("tidal_wse_noadj", adjusted)
     except AssertionError:
         return                     # wet/dry flags differ already
     assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
