@@ -67,6 +67,7 @@ class Case:
     wse_values: object = None # times [hr] and levels [m] (WSE_CURVE), Lagrange order wse_order
     wse_order: int = 0        # (0: the reference's default 2)
     wse_adjust: bool = True   # WSE_ADJUSTMENT of the curve string (wind and wave set-up)
+    wse_grad: tuple = (0.0, 0.0)   # WSE_GRADIENTS of the curve string [m/m]
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 

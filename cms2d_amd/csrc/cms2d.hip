@@ -127,6 +127,7 @@ struct cms2d_flow {
     std::vector<double> wse_t, wse_v;   // water-level curve of CMS2D_BND_WSE_CURVE sides
     int wse_order = 1, wse_inc = 1;     // interpolation order, window start (H_str%inc, 1-based)
     int wse_adjust_off = 0;             // WSE_ADJUSTMENT OFF on the curve sides
+    double wse_gx = 0.0, wse_gy = 0.0;  // WSE_GRADIENTS of the curve sides
     hipEvent_t step_ev[2]{};
 };
 
@@ -689,10 +690,10 @@ static int flow_imp_once(cms2d_flow *F) {
     const double ramp = F->ramp;
     double w = 0.0;
     for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
-    const double wsebnd = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);     // bnd_eval, boundaries.F90:4109-4122
-    double wsecurve = 0.0;                                             // bnd_eval, boundaries.F90:4127-4150
-    if (!F->wse_t.empty())
-        wsecurve = (1.0 - ramp) * 0.0 + ramp * (wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc) + 0.0);
+    // bnd_eval (boundaries.F90:4098-4150): the harmonic sum, the curve's level;
+    // ramp and gradient terms are applied per cell in k_wse_adjust
+    double wsecurve = 0.0;
+    if (!F->wse_t.empty()) wsecurve = wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc);
     if (int rc = wetdry(F)) return rc;
     F->P.tauwx = F->P.tauwy = 0.0;
     if (F->windconst) {
@@ -720,7 +721,8 @@ static int flow_imp_once(cms2d_flow *F) {
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
             const bool curve = F->side_type[sd] == CMS2D_BND_WSE_CURVE;
             hipLaunchKernelGGL(k_wse_adjust, dim3(1), dim3(1024), n * sizeof(double), F->st, g, s, F->P, sd,
-                               curve ? wsecurve : wsebnd, (int)(curve && F->wse_adjust_off));
+                               curve ? wsecurve : w, ramp, curve ? F->wse_gx : 0.0, curve ? F->wse_gy : 0.0,
+                               (int)(curve && F->wse_adjust_off));
         }
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->P, F->Kpp);
         {
@@ -891,6 +893,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             F->wse_order = std::max(std::min(d->wse_order ? d->wse_order : 2, d->wse_ntimes - 1), 1);
             F->wse_inc = 1;
             F->wse_adjust_off = d->wse_adjust_off != 0;
+            F->wse_gx = d->wse_gradx;
+            F->wse_gy = d->wse_grady;
         }
     }
     Geo &g = F->g;

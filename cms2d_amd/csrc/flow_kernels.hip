@@ -654,12 +654,22 @@ __global__ void k_pp_coef_flux(Geo g, St s) {
 
 // bnd_wse_adjust (boundaries.F90:4620-4686) for one harmonic side: setup from
 // the radiation-stress forcing integrated outward from the centre cell.
-__global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int side, double wsebnd, int noadj) {
+// The string's level before set-up: (1 - ramp) 0 + ramp (w + wsevar) with the
+// temporally constant gradient term wsevar = dist . (gx, gy) from the string's
+// middle cell (bnd_eval :4121-4122,4148; bnd_init :1892-1901).
+__global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int side, double w, double ramp, double gx,
+                                                     double gy, int noadj) {
     extern __shared__ double term[];
     const int n = (side == KE || side == KW) ? g.ny : g.nx;
     double *wse = s.wseadj + side * P.lmax;
     const int im0 = n / 2 - 1;
     const bool adjust = n >= 3 && !noadj;
+    const int iref = n / 2 - 1 >= 0 ? n / 2 - 1 : 0;
+    auto base = [&](int im) {
+        const double dist = (double)(im - iref) * ((side == KE || side == KW) ? g.dy : g.dx);
+        const double wv = (side == KE || side == KW) ? 0.0 * gx + dist * gy : dist * gx + 0.0 * gy;
+        return (1.0 - ramp) * 0.0 + ramp * (w + wv);
+    };
     for (int im = threadIdx.x; im < n; im += blockDim.x) {
         double tv = 0.0;
         if (adjust && im != im0) {
@@ -710,7 +720,7 @@ __global__ void __launch_bounds__(1024) k_wse_adjust(Geo g, St s, Par P, int sid
         for (; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
     }
     __syncthreads();
-    for (int im = threadIdx.x; im < n; im += blockDim.x) wse[im] = adjust ? wsebnd + term[im] : wsebnd;
+    for (int im = threadIdx.x; im < n; im += blockDim.x) wse[im] = adjust ? base(im) + term[im] : base(im);
 }
 
 // pp sources (flow_imp.F90:700-712) + bound_pp (boundaries.F90:3493-3646) +

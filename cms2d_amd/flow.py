@@ -50,6 +50,7 @@ class FlowStep:
                 d.wse_times, d.wse_values = arr(case.wse_times), arr(case.wse_values)
                 d.wse_ntimes, d.wse_order = len(case.wse_times), case.wse_order
                 d.wse_adjust_off = int(not case.wse_adjust)
+                d.wse_gradx, d.wse_grady = case.wse_grad
             else:
                 d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
         d.ntc = len(case.amp)

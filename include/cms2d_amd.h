@@ -125,6 +125,9 @@ typedef struct {
                                      limited to wse_ntimes - 1 (boundaries.F90:1417,1869)     */
     int wse_adjust_off;           /* WSE_ADJUSTMENT OFF on the curve sides: no wind and wave
                                      set-up along the string (boundaries.F90:552-554,3569)    */
+    double wse_gradx, wse_grady;  /* WSE_GRADIENTS of the curve sides [m/m], grid axes: the level
+                                     varies along the side from its middle cell
+                                     (boundaries.F90:556-572,1892-1901)                        */
 } cms2d_flow_desc;
 
 typedef struct {

@@ -609,7 +609,13 @@ static void bnd_eval(cms_oracle *o)
         if (ni < 1) ni = 1;
         const double w = cms_oracle_plagr(o->c.wse_ntimes, o->c.wse_times, o->c.wse_values, ni, o->timehrs,
                                           &o->wse_inc);
-        for (int j = 0; j < o->nTH; j++) o->wsebnd[j] = (1.0 - o->ramp) * 0.0 + o->ramp * (w + 0.0);
+        /* wsevar: temporally constant gradients from the string's middle cell (:1892-1901) */
+        const int ir = o->THc[o->nTH / 2 - 1 >= 0 ? o->nTH / 2 - 1 : 0];
+        for (int j = 0; j < o->nTH; j++) {
+            const int ii = o->THc[j];
+            const double wv = (o->x[ii] - o->x[ir]) * o->c.wse_gradx + (o->y[ii] - o->y[ir]) * o->c.wse_grady;
+            o->wsebnd[j] = (1.0 - o->ramp) * 0.0 + o->ramp * (w + wv);
+        }
         return;
     }
     for (int j = 0; j < o->nTH; j++) {

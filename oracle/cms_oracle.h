@@ -59,6 +59,7 @@ typedef struct {
     int wse_ntimes;
     int wse_order;            /* TIME_INTERP_ORDER (0 -> 2)                               */
     int wse_adjust_off;       /* WSE_ADJUSTMENT OFF on the curve string                   */
+    double wse_gradx, wse_grady;   /* WSE_GRADIENTS of the curve string [m/m]            */
 } cms_case;
 
 typedef struct {

@@ -29,7 +29,7 @@ class CmsCase(C.Structure):
         ("serial", C.c_int), ("dtvar", C.c_int), ("latitude", C.c_double), ("coriolis_plane", C.c_int),
         ("mturbul", C.c_int), ("mbedfric", C.c_int),
         ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
-        ("wse_adjust_off", C.c_int),
+        ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
     ]
 
 
@@ -122,6 +122,7 @@ class Oracle:
             c.wse_times, c.wse_values = arr(case.wse_times), arr(case.wse_values)
             c.wse_ntimes, c.wse_order = len(case.wse_times), case.wse_order
             c.wse_adjust_off = int(not getattr(case, "wse_adjust", True))
+            c.wse_gradx, c.wse_grady = getattr(case, "wse_grad", (0.0, 0.0))
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

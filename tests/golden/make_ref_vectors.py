@@ -116,6 +116,15 @@ def _mk_wse_noadj():
 CASES["tidal_wse_noadj"] = (_mk_wse_noadj, 40, [1, 10, 20, 40])
 
 
+def _mk_wse_grad():
+    c = _mk_wse(WSE_T, WSE_V, 0)()
+    c.wse_grad = (0.0, 2.0e-5)                      # WSE_GRADIENTS: 0.02 mm/m along the east side
+    return c
+
+
+CASES["tidal_wse_grad"] = (_mk_wse_grad, 40, [1, 10, 20, 40])
+
+
 # the tidal case without wall friction (USE_WALL_FRICTION_TERMS OFF), and the
 # tidal case run by the reference's default MATRIX_SOLVER GMRES (its restarted
 # GMRES with ILUT; the converged answer does not depend on the Krylov method)

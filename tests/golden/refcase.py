@@ -78,6 +78,9 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
             forcing = f"  WSE_BEGIN\n    WSE_CONSTANT {case.wse_values[0]:.9g}\n  WSE_END\n"
         else:
             adj = "" if getattr(case, "wse_adjust", True) else "    WSE_ADJUSTMENT OFF\n"
+            gx, gy = getattr(case, "wse_grad", (0.0, 0.0))
+            if gx or gy:
+                adj += f"    WSE_GRADIENTS {gx:.9g} {gy:.9g}\n"
             forcing = f'  WSE_BEGIN\n    CURVE "{name}_wse.xys"\n{order}{adj}  WSE_END\n'
     else:
         harm = "".join(f"    HARMONIC {s:.12g} {a:.9g} {p:.9g}\n"

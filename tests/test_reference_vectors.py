@@ -136,6 +136,16 @@ WSE["tidal_wse_noadj"] = None
 CASES["tidal_wse_noadj"] = _tidal_wse_noadj
 
 
+def _tidal_wse_grad():
+    c = _tidal_wse(WSE_T, WSE_V, 0)()
+    c.wse_grad = (0.0, 2.0e-5)                      # WSE_GRADIENTS 0 2e-5
+    return c
+
+
+WSE["tidal_wse_grad"] = None
+CASES["tidal_wse_grad"] = _tidal_wse_grad
+
+
 def _tidal_nowall():
     c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
     c.wallfric = 0                                  # USE_WALL_FRICTION_TERMS OFF
@@ -314,6 +324,19 @@ def test_wse_adjustment_switch_is_what_the_reference_uses():
         return Oracle(c, serial=1)
     try:
         w = compare_to_gold("tidal_wse_noadj", adjusted)
+    except AssertionError:
+        return                     # wet/dry flags differ already
+    assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
+
+
+def test_wse_gradient_is_what_the_reference_uses():
+    """Without its along-side gradient the oracle misses the WSE_GRADIENTS run
+    by more than the bounds."""
+    def flat(c):
+        c.wse_grad = (0.0, 0.0)
+        return Oracle(c, serial=1)
+    try:
+        w = compare_to_gold("tidal_wse_grad", flat)
     except AssertionError:
         return                     # wet/dry flags differ already
     assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
