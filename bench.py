@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 # size) on this workload at 2000 x 1500; the commands and the table are in
 # profiles/traffic_tidal_2000x1500.md.
 TRAFFIC_PER_CELL = {"spmv_pp": (166.8 + 24.5) * 1e6 / 3.0e6,
-                    "mg_down0": (114.9 + 15.2) * 1e6 / 3.0e6,
-                    "mg_up0": (128.0 + 17.0) * 1e6 / 3.0e6}
+                    "mg_down0": (114.3 + 15.2) * 1e6 / 3.0e6,
+                    "mg_up0": (128.1 + 16.9) * 1e6 / 3.0e6}
 GROUPS = {
     "spmv_pp": (60, "k_bicg_spmv<StencilOp,1,1,1> (pp system, v = A z, z from the multigrid cycle)"),
     "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
