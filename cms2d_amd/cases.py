@@ -19,6 +19,9 @@ corner, as source/geospatial/geospatial.F90:2709-2828 numbers them):
   up to 0.05 N/m^2, M2 tide on the east side, dt = 30 s.
 * ``square``  -- 1000 x 1000 cells of 20 m, the same physics as ``tidal``
   without the radiation stress.
+* ``tidal_tel`` -- the ``tidal`` physics on a telescoping grid: base cells of
+  20 m with a two-level band (10 m around a 5 m core) along the shore, cells
+  numbered base cell by base cell (read_tel, geospatial.F90:403-795).
 """
 from __future__ import annotations
 
@@ -71,6 +74,10 @@ class Case:
     wind_speed: float = 0.0   # constant 10-m wind speed [m/s] (0: no wind)
     wind_dir: float = 0.0     # the direction it comes from, degrees clockwise from north
 
+    tel: object = None        # TelGrid: the cells of a telescoping grid (nx, ny, dx, dy then
+                              # describe its base grid; zb, mann, wavestr are per cell)
+    noskew: int = 0           # SKEWNESS_CORRECTION OFF (telescoping grids)
+
     def wind_vector(self):
         """(wndx, wndy) in grid coordinates as windcurve_init makes them
         (meteorological.F90:700-710, grid angle 0, 10-m height, factor 1):
@@ -80,7 +87,7 @@ class Case:
 
     @property
     def ncells(self) -> int:
-        return self.nx * self.ny
+        return self.nx * self.ny if self.tel is None else len(self.tel.geom)
 
     @property
     def speed_rad_hr(self):
@@ -181,3 +188,175 @@ def lake_at_rest(nx=40, ny=30, dx=10.0, seed=5) -> Case:
     zb = f32(-depth).ravel()
     return Case("lake", nx, ny, dx, dx, zb, f32(np.full(nx * ny, 0.03)), None, None, 1,
                 [0.0], [M2_SPEED_DEG_HR], [0.0], 60.0, 10)
+
+
+# ---------------------------------------------------------------------------
+# Telescoping grids
+@dataclasses.dataclass
+class TelGrid:
+    """Cells of a telescoping (quadtree) grid in the layout of a .tel file
+    (read_tel, source/geospatial/geospatial.F90:403-795).
+
+    geom [n, 4]: centre x, y and size dx, dy [m]; nb [n, 8]: neighbour slots
+    N1 N2 E1 E2 S1 S2 W1 W2 as 1-based cell numbers, 0 for none.  A side with
+    one neighbour uses the first slot; a side with two (finer) neighbours lists
+    them clockwise around the cell: north west then east, east north then
+    south, south east then west, west south then north (the order the dummy
+    cells of read_tel assume, :552-690).  base [n]: the base cell j*nx+i that
+    holds the cell; level [n]: its refinement level.  bnd_cells: 0-based cells
+    of the water-level string in string order."""
+    geom: np.ndarray
+    nb: np.ndarray
+    base: np.ndarray
+    level: np.ndarray
+    bnd_cells: np.ndarray
+
+
+def telescope(levels, dx, side=1) -> TelGrid:
+    """Quadtree over an [ny, nx] base grid of dx x dx cells: base cell (i, j)
+    is split into 2^L x 2^L cells with L = levels[j, i] (0..2).  Base cells that
+    share an edge differ by one level at most.  Cells are numbered base cell
+    by base cell (row-major), the cells of a split base cell row-major from its
+    south-west corner.  side: the domain edge of the water-level string."""
+    levels = np.asarray(levels, dtype=np.int64)
+    ny, nx = levels.shape
+    assert levels.min() >= 0 and levels.max() <= 2
+    assert np.abs(np.diff(levels, axis=0)).max(initial=0) <= 1
+    assert np.abs(np.diff(levels, axis=1)).max(initial=0) <= 1
+    ids = {}
+    geom, base, lev = [], [], []
+    for j in range(ny):
+        for i in range(nx):
+            L = int(levels[j, i])
+            m = 1 << L
+            h = dx / m
+            for b in range(m):
+                for a in range(m):
+                    I, J = i * m + a, j * m + b
+                    ids[(L, I, J)] = len(geom)
+                    geom.append((I * h + 0.5 * h, J * h + 0.5 * h, h, h))
+                    base.append(j * nx + i)
+                    lev.append(L)
+    n = len(geom)
+    nb = np.zeros((n, 8), dtype=np.int32)
+    # per side: (di, dj), the two finer neighbours' offsets in slot order
+    sides = [((0, 1), [(0, 1), (1, 1)]),     # N: west, east  (fine offsets from 2I, 2J)
+             ((1, 0), [(1, 1), (1, 0)]),     # E: north, south
+             ((0, -1), [(1, 0), (0, 0)]),    # S: east, west
+             ((-1, 0), [(0, 0), (0, 1)])]    # W: south, north
+    for (L, I, J), c in ids.items():
+        for s_, ((di, dj), fine) in enumerate(sides):
+            In, Jn = I + di, J + dj
+            m = 1 << L
+            bi, bj = In // m if In >= 0 else -1, Jn // m if Jn >= 0 else -1
+            if not (0 <= In < nx * m and 0 <= Jn < ny * m):
+                continue
+            Lb = int(levels[bj, bi])
+            if Lb == L:
+                nb[c, 2 * s_] = ids[(L, In, Jn)] + 1
+            elif Lb == L - 1:
+                nb[c, 2 * s_] = ids[(L - 1, In // 2, Jn // 2)] + 1
+            else:
+                # the two cells of level L+1 across this side
+                for q, (oa, ob) in enumerate(fine):
+                    if di:
+                        fi = 2 * In + (0 if di > 0 else 1)
+                        fj = 2 * J + ob
+                    else:
+                        fi = 2 * I + oa
+                        fj = 2 * Jn + (0 if dj > 0 else 1)
+                    nb[c, 2 * s_ + q] = ids[(L + 1, fi, fj)] + 1
+    nfaces = np.maximum((nb[:, 0::2] > 0) + (nb[:, 1::2] > 0), 1).sum(axis=1)
+    assert nfaces.max() <= 6, "a cell with finer neighbours on more than two sides (nmaxfaces = 6)"
+    geom = f32(np.array(geom))
+    # the water-level string: the cells along the edge `side`, in order along it
+    g = geom
+    Lx, Ly = nx * dx, ny * dx
+    if side == 1:
+        sel = np.nonzero(np.isclose(g[:, 0] + 0.5 * g[:, 2], Lx))[0]
+        order = sel[np.argsort(g[sel, 1], kind="stable")]
+    elif side == 3:
+        sel = np.nonzero(np.isclose(g[:, 0] - 0.5 * g[:, 2], 0.0))[0]
+        order = sel[np.argsort(g[sel, 1], kind="stable")]
+    elif side == 0:
+        sel = np.nonzero(np.isclose(g[:, 1] + 0.5 * g[:, 3], Ly))[0]
+        order = sel[np.argsort(g[sel, 0], kind="stable")]
+    else:
+        sel = np.nonzero(np.isclose(g[:, 1] - 0.5 * g[:, 3], 0.0))[0]
+        order = sel[np.argsort(g[sel, 0], kind="stable")]
+    return TelGrid(geom, nb, np.array(base, dtype=np.int32), np.array(lev, dtype=np.int32),
+                   order.astype(np.int32))
+
+
+def band_levels(nx, ny, c1, c2):
+    """Levels of a shore-parallel band: base columns [c1[0], c1[1]) at level 1
+    around the columns [c2[0], c2[1]) at level 2."""
+    lev = np.zeros((ny, nx), dtype=np.int32)
+    lev[:, c1[0]:c1[1]] = 1
+    lev[:, c2[0]:c2[1]] = 2
+    return lev
+
+
+def _smooth_at(rng, xn, yn, nmodes, peak):
+    f = np.zeros_like(xn)
+    for _ in range(nmodes):
+        kx, ky = rng.integers(1, 4), rng.integers(1, 4)
+        ph1, ph2 = rng.uniform(0, 2 * np.pi, 2)
+        f += rng.normal() * np.sin(np.pi * kx * xn + ph1) * np.sin(np.pi * ky * yn + ph2)
+    return f * (peak / max(np.abs(f).max(), 1e-30))
+
+
+def tidal_tel(nx=2000, ny=1500, dx=20.0, seed=1, nsteps=2880, dt=30.0, band=None, radstress=True,
+              levels=None) -> Case:
+    """The tidal workload on a telescoping grid: the beach of ``tidal``
+    (bed from +1 m at the west edge to -15 m at the east), wetting and drying,
+    lognormal Manning (median 0.025, sigma_log 0.2), a smooth radiation-stress
+    field up to 0.05 N/m^2, M2 tide on the east side, and a band of 10 m cells
+    with a 5 m core along the shoreline (base columns band = ((a1, b1), (a2,
+    b2)); by default 5.4-7.4 % and 5.8-7.0 % of the width, around the
+    shoreline at 6.25 %); or any level map ``levels`` [ny, nx].  Fields are
+    evaluated at the cell centres."""
+    if levels is None:
+        if band is None:
+            band = ((int(round(0.054 * nx)), int(round(0.074 * nx))),
+                    (int(round(0.058 * nx)), int(round(0.070 * nx))))
+        levels = band_levels(nx, ny, band[0], band[1])
+    tel = telescope(levels, dx, side=1)
+    rng = np.random.default_rng(seed)
+    x, y = tel.geom[:, 0], tel.geom[:, 1]
+    xn, yn = x / (nx * dx), y / (ny * dx)
+    depth = -1.0 + 16.0 * xn
+    # Gaussian bumps in base-cell units (sigma 0.2 m)
+    xi, yj = x / dx, y / dx
+    for _ in range(40):
+        ci, cj = rng.uniform(0, nx), rng.uniform(0, ny)
+        r = rng.uniform(max(2, nx / 80), max(3, nx / 20))
+        a = rng.normal(0.0, 0.2)
+        depth = depth + a * np.exp(-((xi - ci) ** 2 + (yj - cj) ** 2) / (2 * r * r))
+    zb = f32(-depth)
+    n = len(x)
+    mann = f32(0.025 * np.exp(0.2 * rng.standard_normal(n)))
+    wx = wy = None
+    noptset = 2
+    if radstress:
+        tx = _smooth_at(rng, xn, yn, 6, 0.05)
+        ty = _smooth_at(rng, xn, yn, 6, 0.05)
+        mag = np.sqrt(tx * tx + ty * ty)
+        scale = np.where(mag > 0.05, 0.05 / np.maximum(mag, 1e-30), 1.0)
+        wx = f32(tx * scale / RHOW)
+        wy = f32(ty * scale / RHOW)
+        noptset = 3
+    return Case("tidal_tel", nx, ny, dx, dx, zb, mann, wx, wy, 1, [0.5], [M2_SPEED_DEG_HR], [90.0],
+                dt, nsteps, noptset=noptset, tel=tel)
+
+
+def patch_levels(nx, ny):
+    """An L-shaped refined patch that touches the south edge: level 1 around a
+    level-2 core; at its inner corner a coarse cell meets finer cells on two
+    sides (six faces)."""
+    lev = np.zeros((ny, nx), dtype=np.int32)
+    a, b = nx // 10, nx // 10 + max(6, nx // 5)
+    lev[0:ny // 2, a:b] = 1
+    lev[0:ny // 4, a:b + max(4, nx // 8)] = 1
+    lev[0:ny // 2 - 2, a + 2:b - 2] = 2
+    return lev

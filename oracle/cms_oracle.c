@@ -24,6 +24,7 @@
 #include <stdio.h>
 
 #define F4(x) ((double)(float)(x))
+#define NF (o->nf)
 #define KN 0
 #define KE 1
 #define KS 2
@@ -56,17 +57,34 @@ static const double GRAV = 9.81;                  /* 9.81_ikind */
 #define RMOMMINUV   F4(1.0e-7)
 #define RMOMRATIOUV 1.0
 
+/* Gradient operator of one cell (der_go_type, der_def.f90): cell-based
+ * Green-Gauss weights (wcx, wcy over cells icx, icy) and, on joint cells of a
+ * telescoping grid (ider 2), the deferred cell-reconstruction weights applied
+ * to the previous perpendicular derivative (wgxy over igx, wgyx over igy). */
+#define NSC 7                                     /* nmaxfaces + 1 */
 typedef struct {
     int ncx, ncy;
-    int icx[5], icy[5];
-    double wcx[5], wcy[5];
+    int icx[NSC], icy[NSC];
+    double wcx[NSC], wcy[NSC];
+    int ngx, ngy;
+    int igx[NSC], igy[NSC];
+    double wgxy[NSC], wgyx[NSC];
 } gradop;
 
 struct cms_oracle {
     cms_case c;
     int nx, ny, ncells, ncellsD;
-    /* connectivity / geometry (geo_def) */
+    /* connectivity / geometry (geo_def): nf faces per cell slot (nmaxfaces), ncf
+     * used by each cell (ncface), idir its directions 0..3 = N,E,S,W (idirface) */
+    int nf, *ncf, *idir;
     int *c2c, *llec, *nxy, *kxy;
+    int *nxf, *kxf, *nyf, *kyf;       /* no-repeat x (E/W) and y (N/S) face lists */
+    int *isjoint, njoint, *joint;     /* joint cells (telescoping), ascending */
+    int ider2;                        /* gradients with the joint-cell corrections (nder 2) */
+    int skewcor;                      /* skewness corrections on (joint cells present) */
+    double *rpx, *rpy;
+    double *dHux, *dHuy, *dHvx, *dHvy, *dapux, *dapuy, *dzbx, *dzby;
+    gradop *goa;
     double *dx, *dy, *x, *y, *area;
     double *ds, *dn, *fnx, *fny, *rx, *ry, *dsx, *dsy, *dsxy, *fintp;
     double areaavg;
@@ -221,10 +239,25 @@ static double ramp_func(double timehrs, double rampdur)
 }
 
 /* ---------------------------------------------------------------------------
- * Grid: geo_cart (geospatial.F90:2709-2828), geo_init_cart (:1400-1611),
- * interp_coef_cell2face (interp_lib.f90:2669-2724)
+ * Grid.  Topology of a rectangle (geo_cart, geospatial.F90:2709-2930) or of a
+ * telescoping grid (read_tel, :403-795); geometry of both (geo_init_cart,
+ * :1400-1611, interp_coef_cell2face, interp_lib.f90:2669-2724).
  * ------------------------------------------------------------------------- */
-static void build_grid(cms_oracle *o)
+static void alloc_topology(cms_oracle *o, int nf, int nc, int nd)
+{
+    o->nf = nf;
+    o->ncells = nc;
+    o->ncellsD = nd;
+    o->ncf = IALLOC(nd);
+    o->c2c = IALLOC(NF * nd);
+    o->llec = IALLOC(NF * nd);
+    o->idir = IALLOC(NF * nd);
+    for (int f = 0; f < NF * nd; f++) { o->c2c[f] = -1; o->llec[f] = -1; o->idir[f] = -1; }
+    o->dx = DALLOC(nd); o->dy = DALLOC(nd); o->x = DALLOC(nd); o->y = DALLOC(nd);
+    o->zb = DALLOC(nd);
+}
+
+static void build_topology_cart(cms_oracle *o)
 {
     const int nx = o->nx, ny = o->ny, nc = nx * ny;
     int nd = nc;
@@ -232,32 +265,29 @@ static void build_grid(cms_oracle *o)
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++)
             nd += (j == ny - 1) + (i == nx - 1) + (j == 0) + (i == 0);
-    o->ncells = nc;
-    o->ncellsD = nd;
-    o->c2c = IALLOC(4 * nd);
-    o->llec = IALLOC(4 * nd);
-    for (int f = 0; f < 4 * nd; f++) { o->c2c[f] = -1; o->llec[f] = -1; }
-    o->dx = DALLOC(nd); o->dy = DALLOC(nd); o->x = DALLOC(nd); o->y = DALLOC(nd);
-    o->zb = DALLOC(nd);
+    alloc_topology(o, 4, nc, nd);
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nx; i++) {
             int c = j * nx + i;
+            o->ncf[c] = 4;
+            for (int k = 0; k < 4; k++) o->idir[NF * c + k] = k;
             o->dx[c] = o->c.dx; o->dy[c] = o->c.dy;
             o->x[c] = (i + 1) * o->c.dx - 0.5 * o->c.dx;
             o->y[c] = (j + 1) * o->c.dy - 0.5 * o->c.dy;
             o->zb[c] = o->c.zb[c];
-            if (j < ny - 1) o->c2c[4 * c + KN] = c + nx;
-            if (i < nx - 1) o->c2c[4 * c + KE] = c + 1;
-            if (j > 0) o->c2c[4 * c + KS] = c - nx;
-            if (i > 0) o->c2c[4 * c + KW] = c - 1;
+            if (j < ny - 1) o->c2c[NF * c + KN] = c + nx;
+            if (i < nx - 1) o->c2c[NF * c + KE] = c + 1;
+            if (j > 0) o->c2c[NF * c + KS] = c - nx;
+            if (i > 0) o->c2c[NF * c + KW] = c - 1;
         }
     /* dummy cells, in the reference's order (cell by cell, N,E,S,W) */
     int g = nc;
     for (int c = 0; c < nc; c++) {
         for (int k = 0; k < 4; k++) {
-            if (o->c2c[4 * c + k] >= 0) continue;
-            o->c2c[4 * c + k] = g;
-            o->c2c[4 * g + kopp[k]] = c;
+            if (o->c2c[NF * c + k] >= 0) continue;
+            o->c2c[NF * c + k] = g;
+            o->ncf[g] = 4;
+            o->c2c[NF * g + kopp[k]] = c;
             o->dx[g] = o->dx[c]; o->dy[g] = o->dy[c];
             o->x[g] = o->x[c] + (k == KE ? o->dx[c] : k == KW ? -o->dx[c] : 0.0);
             o->y[g] = o->y[c] + (k == KN ? o->dy[c] : k == KS ? -o->dy[c] : 0.0);
@@ -265,32 +295,101 @@ static void build_grid(cms_oracle *o)
             g++;
         }
     }
+}
+
+/* read_tel (geospatial.F90:403-795) for a grid whose cells are all active:
+ * faces in the order of the file's neighbour slots N1 N2 E1 E2 S1 S2 W1 W2,
+ * one dummy (ghost) cell for each side without a neighbour, placed one cell
+ * size beyond it with the cell's size and bed (:552-690). */
+static void build_topology_tel(cms_oracle *o)
+{
+    const int nc = o->c.tel_n;
+    const int *nb = o->c.tel_nb;
+    int nd = nc;
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < 8; k += 2)
+            if (nb[8 * i + k] == 0 && nb[8 * i + k + 1] == 0) nd++;
+    alloc_topology(o, 6, nc, nd);
+    int g = nc;
+    for (int i = 0; i < nc; i++) {
+        const double *gm = &o->c.tel_geom[4 * i];
+        o->x[i] = gm[0]; o->y[i] = gm[1]; o->dx[i] = gm[2]; o->dy[i] = gm[3];
+        o->zb[i] = o->c.zb[i];
+    }
+    for (int i = 0; i < nc; i++) {
+        int k = 0;
+        for (int j = 0; j < 8; j++) {
+            const int dir = j / 2;                     /* N, E, S, W */
+            const int nck = nb[8 * i + j];
+            const int open = (j % 2 == 0) && nck == 0 && nb[8 * i + j + 1] == 0;
+            if (open) {
+                o->c2c[NF * i + k] = g;
+                o->idir[NF * i + k] = dir;
+                k++;
+                o->ncf[g] = 4;
+                o->c2c[NF * g + kopp[dir]] = i;
+                o->x[g] = o->x[i] + (dir == KE ? o->dx[i] : dir == KW ? -o->dx[i] : 0.0);
+                o->y[g] = o->y[i] + (dir == KN ? o->dy[i] : dir == KS ? -o->dy[i] : 0.0);
+                o->dx[g] = o->dx[i];
+                o->dy[g] = o->dy[i];
+                o->zb[g] = o->zb[i];
+                g++;
+            } else if (nck > 0) {
+                o->c2c[NF * i + k] = nck - 1;
+                o->idir[NF * i + k] = dir;
+                k++;
+            }
+            if (k > NF) { fprintf(stderr, "cms_oracle: cell %d has more than 6 faces\n", i + 1); abort(); }
+        }
+        o->ncf[i] = k;
+    }
+}
+
+static void geo_init(cms_oracle *o)
+{
+    const int nc = o->ncells, nd = o->ncellsD;
     o->area = DALLOC(nd);
     double asum = 0.0;
     for (int i = 0; i < nd; i++) o->area[i] = o->dx[i] * o->dy[i];
     for (int i = 0; i < nc; i++) asum += o->area[i];
     o->areaavg = asum / (double)(float)nc;           /* sum(areap)/real(ncells) */
-    /* backward connectivity */
+    /* backward connectivity (:1446-1468); ghosts list their cell in the
+     * direction-indexed slot */
     for (int i = 0; i < nc; i++)
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            const int f = NF * i + k, nck = o->c2c[f];
             if (nck < nc) {
-                for (int kk = 0; kk < 4; kk++)
-                    if (o->c2c[4 * nck + kk] == i) o->llec[4 * i + k] = kk;
+                for (int kk = 0; kk < o->ncf[nck]; kk++)
+                    if (o->c2c[NF * nck + kk] == i) o->llec[f] = kk;
             } else {
-                o->llec[4 * i + k] = kopp[k];
+                o->llec[f] = kopp[o->idir[f]];
             }
         }
-    int F = 4 * nd;
+    /* joint cells: a neighbour whose size across the face differs by more
+     * than 20 % (:1490-1505) */
+    o->isjoint = IALLOC(nd);
+    for (int i = 0; i < nc; i++)
+        for (int k = 0; k < o->ncf[i]; k++) {
+            const int f = NF * i + k, nck = o->c2c[f];
+            if (nck >= nc) continue;
+            const double r = (o->idir[f] == KE || o->idir[f] == KW) ? o->dy[i] / o->dy[nck] : o->dx[i] / o->dx[nck];
+            if (r > F4(1.2) || r < F4(0.8)) { o->isjoint[i] = 1; o->isjoint[nck] = 1; }
+        }
+    o->njoint = 0;
+    for (int i = 0; i < nc; i++) o->njoint += o->isjoint[i];
+    o->joint = IALLOC(o->njoint);
+    for (int i = 0, n = 0; i < nc; i++)
+        if (o->isjoint[i]) o->joint[n++] = i;
+    int F = NF * nd;
     o->ds = DALLOC(F); o->dn = DALLOC(F); o->fnx = DALLOC(F); o->fny = DALLOC(F);
     o->rx = DALLOC(F); o->ry = DALLOC(F); o->dsx = DALLOC(F); o->dsy = DALLOC(F);
-    o->dsxy = DALLOC(F); o->fintp = DALLOC(F);
+    o->dsxy = DALLOC(F); o->fintp = DALLOC(F); o->rpx = DALLOC(F); o->rpy = DALLOC(F);
     for (int i = 0; i < nc; i++)
-        for (int k = 0; k < 4; k++) {
-            int f = 4 * i + k, nck = o->c2c[f];
-            if (k == KN || k == KS) o->ds[f] = fmin(o->dx[i], o->dx[nck]);
+        for (int k = 0; k < o->ncf[i]; k++) {
+            const int f = NF * i + k, nck = o->c2c[f], d = o->idir[f];
+            if (d == KN || d == KS) o->ds[f] = fmin(o->dx[i], o->dx[nck]);
             else o->ds[f] = fmin(o->dy[i], o->dy[nck]);
-            switch (k) {
+            switch (d) {
             case KN: o->fnx[f] = 0; o->fny[f] = 1; o->dn[f] = fabs(o->y[nck] - o->y[i]);
                 o->rx[f] = o->x[nck] - o->x[i]; o->ry[f] = o->dy[i] / 2.0; break;
             case KE: o->fnx[f] = 1; o->fny[f] = 0; o->dn[f] = fabs(o->x[nck] - o->x[i]);
@@ -300,65 +399,104 @@ static void build_grid(cms_oracle *o)
             default: o->fnx[f] = -1; o->fny[f] = 0; o->dn[f] = fabs(o->x[nck] - o->x[i]);
                 o->rx[f] = -o->dx[i] / 2.0; o->ry[f] = o->y[nck] - o->y[i]; break;
             }
+            /* perpendicular centroid-to-face vectors (:1592-1611) */
+            o->rpx[f] = o->rx[f] - o->fnx[f] * fabs(o->rx[f]);
+            o->rpy[f] = o->ry[f] - o->fny[f] * fabs(o->ry[f]);
             o->dsx[f] = o->fnx[f] * o->ds[f];
             o->dsy[f] = o->fny[f] * o->ds[f];
             o->dsxy[f] = o->ds[f] / o->dn[f];
-            /* interp_coef_cell2face, simple cells */
-            if (k == KN || k == KS) o->fintp[f] = o->dy[i] / (o->dy[i] + o->dy[nck]);
+            /* interp_coef_cell2face (simple and joint cells alike) */
+            if (d == KN || d == KS) o->fintp[f] = o->dy[i] / (o->dy[i] + o->dy[nck]);
             else o->fintp[f] = o->dx[i] / (o->dx[i] + o->dx[nck]);
         }
-    /* no-repeat face lists kxyface (geospatial.F90:1544-1568) */
-    o->nxy = IALLOC(nd);
-    o->kxy = IALLOC(4 * nd);
+    /* no-repeat face lists kxyface, kxface, kyface (:1528-1552) */
+    o->nxy = IALLOC(nd); o->nxf = IALLOC(nd); o->nyf = IALLOC(nd);
+    o->kxy = IALLOC(NF * nd); o->kxf = IALLOC(NF * nd); o->kyf = IALLOC(NF * nd);
     char *ind = (char *)xcalloc((size_t)F, 1);
     memset(ind, 1, (size_t)F);
     for (int i = 0; i < nc; i++)
-        for (int k = 0; k < 4; k++)
-            if (ind[4 * i + k]) {
-                int nck = o->c2c[4 * i + k];
-                ind[4 * i + k] = 0;
-                ind[4 * nck + o->llec[4 * i + k]] = 0;
-                o->kxy[4 * i + o->nxy[i]++] = k;
+        for (int k = 0; k < o->ncf[i]; k++)
+            if (ind[NF * i + k]) {
+                const int f = NF * i + k, nck = o->c2c[f];
+                ind[f] = 0;
+                ind[NF * nck + o->llec[f]] = 0;
+                o->kxy[NF * i + o->nxy[i]++] = k;
+                if (o->idir[f] == KE || o->idir[f] == KW) o->kxf[NF * i + o->nxf[i]++] = k;
+                else o->kyf[NF * i + o->nyf[i]++] = k;
             }
     free(ind);
 }
 
-/* Boundary strings: harmonic WSE string on one side, walls elsewhere
- * (boundaries.F90:2380-2435 wall-string construction). */
+static void build_grid(cms_oracle *o)
+{
+    if (o->c.tel_n > 0) build_topology_tel(o);
+    else build_topology_cart(o);
+    geo_init(o);
+    /* skewcor is turned off without joint cells (flow.F90:832); gradients then
+     * take no corrections (nder 2 -> 1, derivatives.f90:109-112) */
+    o->skewcor = !o->c.noskew && o->njoint > 0;
+    o->ider2 = o->skewcor;
+}
+
+/* Boundary strings: the water-level string (TH_str or H_str) from its cells
+ * by cellstr_bnd (bnd_lib.f90:215-316: the edge faces of every inner cell of
+ * the string; the end cells take the direction of their neighbour in the
+ * string), walls on every other edge face (W_str, boundaries.F90:2380-2435). */
 static void build_bnd(cms_oracle *o)
 {
     const int nx = o->nx, ny = o->ny, nc = o->ncells;
     int side = o->c.th_side;
-    o->nTH = 0;
-    if (side == KE || side == KW) o->nTH = ny;
-    else if (side == KN || side == KS) o->nTH = nx;
-    o->THc = IALLOC(o->nTH); o->THk = IALLOC(o->nTH);
-    for (int j = 0; j < o->nTH; j++) {
-        int c;
-        if (side == KE) c = j * nx + (nx - 1);
-        else if (side == KW) c = j * nx;
-        else if (side == KN) c = (ny - 1) * nx + j;
-        else c = j;
-        o->THc[j] = c; o->THk[j] = side;
+    int ns = 0, *cells = NULL;
+    if (side >= 0 && o->c.tel_n > 0) {
+        ns = o->c.tel_nbnd;
+        cells = IALLOC(ns);
+        for (int j = 0; j < ns; j++) cells[j] = o->c.tel_bnd[j];
+    } else if (side >= 0) {
+        ns = (side == KE || side == KW) ? ny : nx;
+        cells = IALLOC(ns);
+        for (int j = 0; j < ns; j++) {
+            if (side == KE) cells[j] = j * nx + (nx - 1);
+            else if (side == KW) cells[j] = j * nx;
+            else if (side == KN) cells[j] = (ny - 1) * nx + j;
+            else cells[j] = j;
+        }
     }
+    int cap = 4 * ns + 1, n = 0;
+    o->THc = IALLOC(cap); o->THk = IALLOC(cap);
+    for (int j = 0; j < ns && ns >= 2; j++) {
+        const int id = cells[j];
+        const int ref = j == 0 ? cells[1] : j == ns - 1 ? cells[ns - 2] : id;
+        for (int k = 0; k < o->ncf[ref]; k++) {
+            if (o->c2c[NF * ref + k] < nc) continue;
+            int kf = k;
+            if (ref != id) {
+                kf = -1;
+                for (int kk = 0; kk < o->ncf[id]; kk++)
+                    if (o->idir[NF * id + kk] == o->idir[NF * ref + k]) { kf = kk; break; }
+            }
+            o->THc[n] = id; o->THk[n] = kf; n++;
+        }
+    }
+    o->nTH = n;
+    free(cells);
     o->wsebnd = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadj = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadjust = o->nTH >= 3;          /* boundaries.F90:1370,1808 */
     if (o->c.wse_ntimes > 0 && o->c.wse_adjust_off) o->wseadjust = 0;   /* WSE_ADJUSTMENT OFF, :552-554 */
     o->wse_inc = 1;
     char *claimed = (char *)xcalloc((size_t)o->ncellsD, 1);
-    for (int j = 0; j < o->nTH; j++) claimed[o->c2c[4 * o->THc[j] + o->THk[j]]] = 1;
-    int n = 0;
+    for (int j = 0; j < o->nTH; j++) claimed[o->c2c[NF * o->THc[j] + o->THk[j]]] = 1;
+    n = 0;
     for (int i = 0; i < nc; i++)
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int nck = o->c2c[NF * i + k];
             if (nck >= nc && !claimed[nck]) n++;
         }
     o->nW = n; o->Wc = IALLOC(n); o->Wk = IALLOC(n);
     n = 0;
     for (int i = 0; i < nc; i++)
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int nck = o->c2c[NF * i + k];
             if (nck >= nc && !claimed[nck]) { o->Wc[n] = i; o->Wk[n] = k; n++; }
         }
     free(claimed);
@@ -366,18 +504,21 @@ static void build_bnd(cms_oracle *o)
 
 /* ---------------------------------------------------------------------------
  * Gradient operator: der_grad_cbgg (der_lib.f90:272-378) + screen_weights
- * (:681-715), gow%ibc=1 (wet cells only) or goa%ibc=0.
+ * (:681-715), gow%ibc=1 (wet cells only) or goa%ibc=0; on joint cells of a
+ * telescoping grid also der_grad_cbggcr (:381-504).
  * ------------------------------------------------------------------------- */
 static void der_grad_cbgg(cms_oracle *o, gradop *go, int ibc)
 {
+    const double tolw = F4(1.0e-8);
     for (int i = 0; i < o->ncells; i++) {
         gradop *G = &go[i];
-        if (o->iwet[i] == 0 && ibc == 1) { G->ncx = 0; G->ncy = 0; continue; }
-        double wx[5] = {0}, wy[5] = {0};
-        int ix[5], iy[5];
+        if (o->iwet[i] == 0 && ibc == 1) { G->ncx = 0; G->ncy = 0; G->ngx = 0; G->ngy = 0; continue; }
+        const int nf = o->ncf[i];
+        double wx[NSC] = {0}, wy[NSC] = {0};
+        int ix[NSC], iy[NSC];
         ix[0] = iy[0] = i;
-        for (int k = 0; k < 4; k++) {
-            int f = 4 * i + k, nck = o->c2c[f];
+        for (int k = 0; k < nf; k++) {
+            int f = NF * i + k, nck = o->c2c[f];
             ix[k + 1] = iy[k + 1] = nck;
             if (o->iwet[nck] == 1 || ibc == 0) {
                 wx[0] += o->dsx[f] * (1.0 - o->fintp[f]);
@@ -389,10 +530,9 @@ static void der_grad_cbgg(cms_oracle *o, gradop *go, int ibc)
                 wy[0] += o->dsy[f];
             }
         }
-        for (int k = 0; k < 5; k++) { wx[k] /= o->area[i]; wy[k] /= o->area[i]; }
-        const double tolw = F4(1.0e-8);
+        for (int k = 0; k <= nf; k++) { wx[k] /= o->area[i]; wy[k] /= o->area[i]; }
         G->ncx = G->ncy = 0;
-        for (int k = 0; k < 5; k++) {
+        for (int k = 0; k <= nf; k++) {
             if (fabs(wx[k]) > tolw) { G->icx[G->ncx] = ix[k]; G->wcx[G->ncx] = wx[k]; G->ncx++; }
             if (fabs(wy[k]) > tolw) { G->icy[G->ncy] = iy[k]; G->wcy[G->ncy] = wy[k]; G->ncy++; }
         }
@@ -402,11 +542,57 @@ static void der_grad_cbgg(cms_oracle *o, gradop *go, int ibc)
         for (int k = 0; k < G->ncx; k++) G->wcx[k] -= sx / (double)G->ncx;
         for (int k = 0; k < G->ncy; k++) G->wcy[k] -= sy / (double)G->ncy;
     }
+    if (!o->ider2) return;
+    /* der_grad_cbggcr, joint cells: weights of the perpendicular derivative of
+     * the previous evaluation, from the linear reconstruction along the faces
+     * whose neighbour differs in size (rpx, rpy) */
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        gradop *G = &go[i];
+        if (o->iwet[i] == 0 && ibc == 1) { G->ngx = 0; G->ngy = 0; continue; }
+        double wxy[NSC] = {0}, wyx[NSC] = {0};
+        int igx[NSC], igy[NSC];
+        for (int k = 0; k < NSC; k++) igx[k] = igy[k] = i;
+        igx[0] = igy[0] = i;
+        for (int j = 0; j < o->nxf[i]; j++) {
+            const int k = o->kxf[NF * i + j], f = NF * i + k, nck = o->c2c[f];
+            if (nck >= o->ncells || fabs(o->dy[i] / o->dy[nck] - 1.0) < F4(0.1)) continue;
+            if (o->iwet[nck] == 0 && ibc == 1) continue;
+            const int jcn = o->llec[f];
+            wxy[0] += o->dsx[f] * (1.0 - o->fintp[f]) * o->rpy[f] / o->area[i];
+            wxy[k + 1] = o->dsx[f] * o->fintp[f] * o->rpy[NF * nck + jcn] / o->area[i];
+            igx[k + 1] = nck;
+        }
+        for (int j = 0; j < o->nyf[i]; j++) {
+            const int k = o->kyf[NF * i + j], f = NF * i + k, nck = o->c2c[f];
+            igy[k + 1] = nck;
+            if (nck >= o->ncells || fabs(o->dx[i] / o->dx[nck] - 1.0) < F4(0.1)) continue;
+            if (o->iwet[nck] == 0 && ibc == 1) continue;
+            const int jcn = o->llec[f];
+            wyx[0] += o->dsy[f] * (1.0 - o->fintp[f]) * o->rpx[f] / o->area[i];
+            wyx[k + 1] = o->dsy[f] * o->fintp[f] * o->rpx[NF * nck + jcn] / o->area[i];
+        }
+        G->ngx = G->ngy = 0;
+        for (int k = 0; k <= o->ncf[i]; k++) {
+            if (fabs(wxy[k]) > tolw) { G->igx[G->ngx] = igx[k]; G->wgxy[G->ngx] = wxy[k]; G->ngx++; }
+            if (fabs(wyx[k]) > tolw) { G->igy[G->ngy] = igy[k]; G->wgyx[G->ngy] = wyx[k]; G->ngy++; }
+        }
+    }
 }
 
-/* der_grad_eval, der_lib.f90:726-839 (first order, nder=1 on Cartesian grids) */
+/* der_grad_eval, der_lib.f90:726-839: first-order Green-Gauss on every cell;
+ * on joint cells (nder 2) plus the deferred correction from the previous
+ * perpendicular derivative, which is what gx, gy hold on entry. */
 static void der_grad_eval(cms_oracle *o, const gradop *go, const double *var, double *gx, double *gy)
 {
+    double *gxm = NULL, *gym = NULL;
+    if (o->ider2) {
+        gxm = DALLOC(o->ncellsD); gym = DALLOC(o->ncellsD);
+        for (int jj = 0; jj < o->njoint; jj++) {
+            const int i = o->joint[jj];
+            gxm[i] = gx[i]; gym[i] = gy[i];
+        }
+    }
     for (int i = 0; i < o->ncells; i++) {
         const gradop *G = &go[i];
         double sx = 0.0, sy = 0.0;
@@ -414,15 +600,27 @@ static void der_grad_eval(cms_oracle *o, const gradop *go, const double *var, do
         for (int k = 0; k < G->ncy; k++) sy += G->wcy[k] * var[G->icy[k]];
         gx[i] = sx; gy[i] = sy;
     }
+    if (o->ider2) {
+        for (int jj = 0; jj < o->njoint; jj++) {
+            const int i = o->joint[jj];
+            const gradop *G = &go[i];
+            for (int k = 0; k < G->ngx; k++) gx[i] += G->wgxy[k] * gym[G->igx[k]];
+            for (int k = 0; k < G->ngy; k++) gy[i] += G->wgyx[k] * gxm[G->igy[k]];
+        }
+        free(gxm); free(gym);
+    }
 }
 
-/* interp_scal_cell2face, interp_lib.f90:2727-2906, ibc = 0, 1, 3 (no skew terms
- * on Cartesian grids). */
-static void interp_cell2face(cms_oracle *o, const double *phi, int ibc, double *phik)
+/* interp_scal_cell2face, interp_lib.f90:2727-2906, ibc = 0, 1, 3; with the
+ * cell gradients given and skewness corrections on, the faces of joint cells
+ * add the linear reconstruction along the face and are clipped to the two
+ * cell values (:2853-2873). */
+static void interp_cell2face(cms_oracle *o, const double *phi, int ibc, double *phik, const double *dphix,
+                             const double *dphiy)
 {
     for (int i = 0; i < o->ncells; i++)
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k;
+            int k = o->kxy[NF * i + j], f = NF * i + k;
             int nck = o->c2c[f], jcn = o->llec[f];
             double fk = o->fintp[f], val;
             int wi = o->iwet[i], wn = o->iwet[nck];
@@ -438,8 +636,24 @@ static void interp_cell2face(cms_oracle *o, const double *phi, int ibc, double *
                 else val = 0.0;
             }
             phik[f] = val;
-            phik[4 * nck + jcn] = val;
+            phik[NF * nck + jcn] = val;
         }
+    if (!o->skewcor || !dphiy) return;
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        for (int j = 0; j < o->nxy[i]; j++) {
+            const int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f], jcn = o->llec[f];
+            const int b = NF * nck + jcn;
+            double val = phik[f];
+            if (fabs(o->fnx[f]) > F4(0.999))
+                val = val + o->fintp[f] * o->rpy[b] * dphiy[nck] + (1.0 - o->fintp[f]) * o->rpy[f] * dphiy[i];
+            else
+                val = val + o->fintp[f] * o->rpx[b] * dphix[nck] + (1.0 - o->fintp[f]) * o->rpx[f] * dphix[i];
+            val = fmin(fmax(val, fmin(phi[i], phi[nck])), fmax(phi[i], phi[nck]));
+            phik[f] = val;
+            phik[b] = val;
+        }
+    }
 }
 
 /* ---------------------------------------------------------------------------
@@ -450,7 +664,7 @@ static void flow_pond(cms_oracle *o)
     const int nc = o->ncells, nd = o->ncellsD;
     int *ipond = IALLOC(nd);
     for (int j = 0; j < o->nTH; j++) {
-        int i = o->THc[j], nck = o->c2c[4 * i + o->THk[j]];
+        int i = o->THc[j], nck = o->c2c[NF * i + o->THk[j]];
         ipond[i] = o->iwet[i];
         ipond[nck] = o->iwet[nck];
     }
@@ -459,16 +673,16 @@ static void flow_pond(cms_oracle *o)
         npond = 0;
         for (int i = 0; i < nc; i++)
             if (ipond[i] == 1)
-                for (int k = 0; k < 4; k++) {
-                    int nck = o->c2c[4 * i + k];
+                for (int k = 0; k < o->ncf[i]; k++) {
+                    int nck = o->c2c[NF * i + k];
                     int d = o->iwet[nck] - ipond[nck];
                     npond += d > 0 ? d : 0;
                     ipond[nck] = o->iwet[nck];
                 }
         for (int i = nc - 1; i >= 0; i--)
             if (ipond[i] == 1)
-                for (int k = 0; k < 4; k++) {
-                    int nck = o->c2c[4 * i + k];
+                for (int k = 0; k < o->ncf[i]; k++) {
+                    int nck = o->c2c[NF * i + k];
                     int d = o->iwet[nck] - ipond[nck];
                     npond += d > 0 ? d : 0;
                     ipond[nck] = o->iwet[nck];
@@ -487,38 +701,49 @@ static void flow_wetdry(cms_oracle *o)
         if (o->h[i] < o->hdry) { o->h[i] = fmax(o->h[i], o->hmin); o->iwet[i] = 0; }
         else o->iwet[i] = 1;
     }
-    for (int j = 0; j < o->nW; j++) o->iwet[o->c2c[4 * o->Wc[j] + o->Wk[j]]] = 0;
-    /* narrow-channel removal (Cartesian branch, :1342-1455), sequential in place */
+    for (int j = 0; j < o->nW; j++) o->iwet[o->c2c[NF * o->Wc[j] + o->Wk[j]]] = 0;
+    /* narrow-channel removal (Cartesian and telescoping branch, :1342-1455),
+     * sequential in place; wdir(i, d) sums the wet flags of the neighbours
+     * of i in direction d, wdir2 those two steps away through an active cell
+     * (a ghost lists no directions) */
+#define WDIR(c, d, acc) do { for (int q_ = 0; q_ < o->ncf[c]; q_++) \
+        if (o->idir[NF * (c) + q_] == (d)) acc += o->iwet[o->c2c[NF * (c) + q_]]; } while (0)
+#define WDIR2(c, d, acc) do { for (int q_ = 0; q_ < o->ncf[c]; q_++) { \
+        const int n_ = o->c2c[NF * (c) + q_]; \
+        if (o->idir[NF * (c) + q_] == (d) && n_ < nc) WDIR(n_, d, acc); } } while (0)
     int nsw = 1;
     while (nsw > 0) {
         nsw = 0;
         for (int i = 0; i < nc; i++) {
             if (o->iwet[i] == 0) continue;
-            const int *nb = &o->c2c[4 * i];
-            int ksum = o->iwet[nb[0]] + o->iwet[nb[1]] + o->iwet[nb[2]] + o->iwet[nb[3]];
+            int ksum = 0;
+            for (int k = 0; k < o->ncf[i]; k++) ksum += o->iwet[o->c2c[NF * i + k]];
             if (ksum <= 1) { o->iwet[i] = 0; nsw++; }
-            int kns = o->iwet[nb[KN]] + o->iwet[nb[KS]];
-            int kew = o->iwet[nb[KE]] + o->iwet[nb[KW]];
+            int kns = 0, kew = 0;
+            WDIR(i, KN, kns); WDIR(i, KS, kns);
+            WDIR(i, KE, kew); WDIR(i, KW, kew);
             if ((kns == 0 || kew == 0) && o->h[i] < o->hdry1) { o->iwet[i] = 0; nsw++; }
-            /* second neighbour in a direction exists only through an active cell */
-#define NB2(c, k) ((c) < nc ? o->iwet[o->c2c[4 * (c) + (k)]] : 0)
-            int knss = o->iwet[nb[KN]] + NB2(nb[KS], KS);
+            int knss = 0;
+            WDIR(i, KN, knss); WDIR2(i, KS, knss);
             if (knss == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
             /* the reference tests knss a second time here (flow.F90:1403) */
             if (knss == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
-            int keww = o->iwet[nb[KE]] + NB2(nb[KW], KW);
+            int keww = 0;
+            WDIR(i, KE, keww); WDIR2(i, KW, keww);
             if (keww == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
-            int keew = o->iwet[nb[KW]] + NB2(nb[KE], KE);
+            int keew = 0;
+            WDIR(i, KW, keew); WDIR2(i, KE, keew);
             if (keew == 0 && o->h[i] < o->hdry2) { o->iwet[i] = 0; nsw++; }
-#undef NB2
         }
     }
+#undef WDIR
+#undef WDIR2
     flow_pond(o);
     for (int i = 0; i < nc; i++) {
         o->icorner[i] = 0;
         if (o->iwet[i] == 1) {
             int nds = 0;
-            for (int k = 0; k < 4; k++) nds += o->iwet[o->c2c[4 * i + k]] == 0;
+            for (int k = 0; k < o->ncf[i]; k++) nds += o->iwet[o->c2c[NF * i + k]] == 0;
             if (nds > 1) o->icorner[i] = 1;
         }
     }
@@ -534,9 +759,9 @@ static void flow_wetdry(cms_oracle *o)
         }
         for (int i = 0; i < nc; i++)
             if (o->iwet[i] == 1 && o->iwet1[i] == 0) {
-                const int *nb = &o->c2c[4 * i];
+                const int *nb = &o->c2c[NF * i];
                 double us = 0, vs = 0, viss = 0; int ns = 0;
-                for (int k = 0; k < 4; k++) { us += ru[nb[k]]; vs += rv[nb[k]]; viss += rvis[nb[k]]; ns += o->iwet[nb[k]]; }
+                for (int k = 0; k < o->ncf[i]; k++) { us += ru[nb[k]]; vs += rv[nb[k]]; viss += rvis[nb[k]]; ns += o->iwet[nb[k]]; }
                 o->u[i] = us / ((double)(float)ns + SMALL) / F4(1.25);
                 o->v[i] = vs / ((double)(float)ns + SMALL) / F4(1.25);
                 o->vis[i] = viss / ((double)(float)ns + SMALL);
@@ -546,10 +771,10 @@ static void flow_wetdry(cms_oracle *o)
     /* zero fluxes and viscosities on faces that are not wet on both sides */
     for (int i = 0; i < nc; i++)
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f], jcn = o->llec[f];
+            int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f], jcn = o->llec[f];
             if (o->iwet[i] * o->iwet[nck] == 0) {
                 o->flux[f] = 0; o->visk[f] = 0;
-                o->flux[4 * nck + jcn] = 0; o->visk[4 * nck + jcn] = 0;
+                o->flux[NF * nck + jcn] = 0; o->visk[NF * nck + jcn] = 0;
             }
         }
 }
@@ -679,8 +904,8 @@ static void coeffsourcesink_uv(cms_oracle *o)
         o->sv[i] = o->ssv0[i] + (-o->fc[i] * o->u[i] - o->dpy[i]) * o->h[i] * o->area[i];
         o->cbcfuwcap[i] = o->cmb[i] * o->cfrict[i] * o->uelwc[i] * o->area[i];
         o->sp[i] = o->spuv0[i] - o->cbcfuwcap[i];
-        for (int k = 0; k < 4; k++) {
-            int f = 4 * i + k;
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int f = NF * i + k;
             double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
             o->acoef[f] = cms_oracle_schmcoef(o->c.ndsch, ddk, o->flux[f]);
         }
@@ -700,22 +925,22 @@ static void coeffsourcesink_uv(cms_oracle *o)
 static void defcor_nvd(const cms_oracle *o, double *dsu, double *dsv)
 {
     const int nd = o->ncellsD, nc = o->ncells, ndsch = o->c.ndsch;
-    double *uf = DALLOC(4 * (size_t)nd), *vf = DALLOC(4 * (size_t)nd);
+    double *uf = DALLOC(NF * (size_t)nd), *vf = DALLOC(NF * (size_t)nd);
     const double tiny = F4(1.0e-20);
     for (int i = 0; i < nc; i++) {
         if (o->iwet[i] == 0) continue;
         for (int j = 0; j < o->nxy[i]; j++) {
-            const int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            const int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
             if (nck >= nc || o->iwet[nck] == 0) continue;
             const int jcn = o->llec[f];
             int nckk;
             double uc, ud, uu, vc, vd, vu;
             if (o->flux[f] >= 0.0) {
-                nckk = o->c2c[4 * i + kopp[k]];
+                nckk = o->c2c[NF * i + kopp[k]];
                 uc = o->u[i]; ud = o->u[nck];
                 vc = o->v[i]; vd = o->v[nck];
             } else {
-                nckk = o->c2c[4 * nck + k];
+                nckk = o->c2c[NF * nck + k];
                 uc = o->u[nck]; ud = o->u[i];
                 vc = o->v[nck]; vd = o->v[i];
             }
@@ -729,7 +954,7 @@ static void defcor_nvd(const cms_oracle *o, double *dsu, double *dsv)
                         const double ufc = o->fintp[f] * o->u[nck] + (1.0 - o->fintp[f]) * o->u[i];
                         uf[f] = o->flux[f] * cms_oracle_nvd_gamma(ndsch, un) * (ufc - uc);
                     }
-                    uf[4 * nck + jcn] = -uf[f];
+                    uf[NF * nck + jcn] = -uf[f];
                 }
             }
             if (fabs(vd - vu) > tiny) {
@@ -740,16 +965,55 @@ static void defcor_nvd(const cms_oracle *o, double *dsu, double *dsv)
                         const double vfc = o->fintp[f] * o->v[nck] + (1.0 - o->fintp[f]) * o->v[i];
                         vf[f] = o->flux[f] * cms_oracle_nvd_gamma(ndsch, vn) * (vfc - vc);
                     }
-                    vf[4 * nck + jcn] = -vf[f];
+                    vf[NF * nck + jcn] = -vf[f];
                 }
             }
         }
     }
     for (int i = 0; i < nc; i++) {
-        dsu[i] = ((uf[4 * i] + uf[4 * i + 1]) + uf[4 * i + 2]) + uf[4 * i + 3];
-        dsv[i] = ((vf[4 * i] + vf[4 * i + 1]) + vf[4 * i + 2]) + vf[4 * i + 3];
+        dsu[i] = ((uf[NF * i] + uf[NF * i + 1]) + uf[NF * i + 2]) + uf[NF * i + 3];
+        dsv[i] = ((vf[NF * i] + vf[NF * i + 1]) + vf[NF * i + 2]) + vf[NF * i + 3];
     }
     free(uf); free(vf);
+}
+
+/* defcorparagradvec (computational.F90:149-264): deferred skewness
+ * correction of the momentum sources on joint cells, called for every scheme
+ * without an NVD correction on a grid that is not simple Cartesian
+ * (flow_imp.F90:127-139), from the previous velocity gradients. */
+static void defcor_skew(cms_oracle *o)
+{
+    const int nc = o->ncells;
+    double *us = DALLOC(NF * (size_t)o->ncellsD), *vs = DALLOC(NF * (size_t)o->ncellsD);
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        if (o->iwet[i] == 0) continue;
+        for (int j = 0; j < o->nxy[i]; j++) {
+            const int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
+            if (nck >= nc || o->iwet[nck] == 0) continue;
+            const int b = NF * nck + o->llec[f];
+            const int xdir = o->idir[f] == KE || o->idir[f] == KW;
+            const double rpi = xdir ? o->rpy[f] : o->rpx[f], rpn = xdir ? o->rpy[b] : o->rpx[b];
+            const double *du = xdir ? o->duy : o->dux, *dv = xdir ? o->dvy : o->dvx;
+            if (fabs(rpi) > F4(1.0e-6)) {
+                const double val = (o->acoef[f] + o->flux[f]) * rpi;
+                us[f] = val * du[i]; vs[f] = val * dv[i];
+                us[b] = -us[f]; vs[b] = -vs[f];
+            } else if (fabs(rpn) > F4(1.0e-6)) {
+                const double val = -o->acoef[f] * rpn;
+                us[f] = val * du[nck]; vs[f] = val * dv[nck];
+                us[b] = -us[f]; vs[b] = -vs[f];
+            }
+        }
+    }
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        double su = 0.0, sv = 0.0;
+        for (int k = 0; k < o->ncf[i]; k++) { su += us[NF * i + k]; sv += vs[NF * i + k]; }
+        o->su[i] = o->su[i] - su;
+        o->sv[i] = o->sv[i] - sv;
+    }
+    free(us); free(vs);
 }
 
 int cms_oracle_defcor(cms_oracle *o, double *dsu, double *dsv)
@@ -767,7 +1031,7 @@ int cms_oracle_defcor(cms_oracle *o, double *dsu, double *dsv)
 static void bound_uv(cms_oracle *o)
 {
     for (int j = 0; j < o->nTH; j++) {
-        int i = o->THc[j], k = o->THk[j], f = 4 * i + k, nck = o->c2c[f];
+        int i = o->THc[j], k = o->THk[j], f = NF * i + k, nck = o->c2c[f];
         double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
         o->acoef[f] = cms_oracle_schmcoef(1, ddk, o->flux[f]);
         if (fabs(o->fny[f]) > F4(0.9999)) {
@@ -787,8 +1051,8 @@ static void bound_uv(cms_oracle *o)
             if (o->iwet[i] == 1) {
                 o->spu[i] = o->sp[i];
                 o->spv[i] = o->sp[i];
-                for (int k = 0; k < 4; k++) {
-                    int f = 4 * i + k, nck = o->c2c[f];
+                for (int k = 0; k < o->ncf[i]; k++) {
+                    int f = NF * i + k, nck = o->c2c[f];
                     if (o->iwet[nck] == 0) {
                         double z0w = o->z0[i] * 1.0;   /* wallfac = 1.0 */
                         if (fabs(o->fny[f]) > F4(0.999)) {
@@ -984,9 +1248,9 @@ static int solve_sys(cms_oracle *o, double *ss, const double *sp, double *res, d
     double adak = 0.0;
     for (int i = 0; i < nc; i++) {
         double asum = 0.0, anb = 0.0;
-        for (int k = 0; k < 4; k++) {
-            asum += o->acoef[4 * i + k];
-            anb += o->acoef[4 * i + k] * phi[o->c2c[4 * i + k]];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            asum += o->acoef[NF * i + k];
+            anb += o->acoef[NF * i + k] * phi[o->c2c[NF * i + k]];
         }
         ap[i] = (asum - sp[i]) * rinv;
         ss[i] = ss[i] + relaxm * phi[i] * ap[i];
@@ -994,7 +1258,7 @@ static int solve_sys(cms_oracle *o, double *ss, const double *sp, double *res, d
         adak += res[i] * res[i];
     }
     *rmom_out = sqrt(adak / (double)(float)nc);
-    sysmat S = {nc, o->ncellsD, 4, o->acoef, o->c2c, ap, o->ia, o->ja, o->diagpos, o->lu};
+    sysmat S = {nc, o->ncellsD, NF, o->acoef, o->c2c, ap, o->ia, o->ja, o->diagpos, o->lu};
     int nswp = (n == 1) ? 15 : 5;                       /* flow.F90:779-783 */
     int nmax = o->c.solver_mode == 1 ? nswp : o->c.kmax;
     return bicgstab(&S, ss, phi, o->c.solver_mode, nmax, o->c.rtol, o->c.atol, o->wk);
@@ -1010,7 +1274,7 @@ static void coefsourcesink_pp(cms_oracle *o)
     const int nc = o->ncells;
     for (int i = 0; i < nc; i++) {
         double asum = 0.0;
-        for (int k = 0; k < 4; k++) asum += o->acoef[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) asum += o->acoef[NF * i + k];
         double apu = o->iwet[i] * RELAX_UV / fmax(asum - o->sp[i], F4(1.0e-6));
         o->apuareap[i] = apu * o->area[i];
         o->sumu[i] = asum * apu;
@@ -1019,7 +1283,7 @@ static void coefsourcesink_pp(cms_oracle *o)
     }
     for (int i = 0; i < nc; i++)
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
             double a;
             if (o->iwet[i] * o->iwet[nck] == 0) a = 0.0;
             else if (nck >= nc)
@@ -1031,18 +1295,49 @@ static void coefsourcesink_pp(cms_oracle *o)
                 a = o->hk[f] * o->hk[f] * o->ds[f] * apf / (1.0 - suf) / o->dn[f];
             }
             o->acoef[f] = a;
-            o->acoef[4 * nck + o->llec[f]] = a;
+            o->acoef[NF * nck + o->llec[f]] = a;
         }
-    /* flow_imp_flux_noskew */
+    /* Rhie-Chow fluxes: flow_imp_flux_noskew (flow_imp_flux.f90:361-409), or
+     * with skewness corrections flow_imp_flux_skew (:2-169): on the faces a
+     * joint cell lists, the side whose perpendicular offset rp is nonzero is
+     * reconstructed linearly from the gradients of H, apuareap and p */
+    if (o->skewcor) {
+        der_grad_eval(o, o->gow, o->Hu, o->dHux, o->dHuy);
+        der_grad_eval(o, o->gow, o->Hv, o->dHvx, o->dHvy);
+        der_grad_eval(o, o->gow, o->apuareap, o->dapux, o->dapuy);
+    }
     for (int i = 0; i < nc; i++)
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
             double fl;
             if (o->iwet[i] * o->iwet[nck] == 0) fl = 0.0;
             else if (nck >= nc)
                 fl = o->hk[f] * o->ds[f] * ((o->fnx[f] * o->Hu[i] + o->fny[f] * o->Hv[i])
                      - o->apuareap[i] * o->hk[f] * (o->p[nck] - o->p[i]) / o->dn[f]);
-            else {
+            else if (o->skewcor && o->isjoint[i]) {
+                const double fk = o->fintp[f], fp = 1.0 - fk;
+                const int b = NF * nck + o->llec[f];
+                const int xdir = o->idir[f] == KE || o->idir[f] == KW;
+                /* x faces reconstruct along y (rpy, d/dy), y faces along x */
+                const double rpi = xdir ? o->rpy[f] : o->rpx[f], rpn = xdir ? o->rpy[b] : o->rpx[b];
+                const double *dH = xdir ? o->dHuy : o->dHvx, *dap = xdir ? o->dapuy : o->dapux;
+                const double *dp = xdir ? o->dpy : o->dpx, *H = xdir ? o->Hu : o->Hv;
+                double apf, Hf, dpf;
+                if (fabs(rpi) > F4(1.0e-6)) {            /* coarse to fine */
+                    apf = fk * o->apuareap[nck] + fp * (o->apuareap[i] + rpi * dap[i]);
+                    Hf = fk * H[nck] + fp * (H[i] + rpi * dH[i]);
+                    dpf = (o->p[nck] - o->p[i] - rpi * dp[i]) / o->dn[f];
+                } else if (fabs(rpn) > F4(1.0e-6)) {     /* fine to coarse */
+                    apf = fk * (o->apuareap[nck] + rpn * dap[nck]) + fp * o->apuareap[i];
+                    Hf = fk * (H[nck] + rpn * dH[nck]) + fp * H[i];
+                    dpf = (o->p[nck] + rpn * dp[nck] - o->p[i]) / o->dn[f];
+                } else {
+                    apf = fk * o->apuareap[nck] + fp * o->apuareap[i];
+                    Hf = fk * H[nck] + fp * H[i];
+                    dpf = (o->p[nck] - o->p[i]) / o->dn[f];
+                }
+                fl = o->hk[f] * o->ds[f] * ((xdir ? o->fnx[f] : o->fny[f]) * Hf - apf * o->hk[f] * dpf);
+            } else {
                 double fk = o->fintp[f], fp = 1.0 - fk;
                 double apf = fk * o->apuareap[nck] + fp * o->apuareap[i];
                 double Huf = fk * o->Hu[nck] + fp * o->Hu[i];
@@ -1051,14 +1346,44 @@ static void coefsourcesink_pp(cms_oracle *o)
                      - apf * o->hk[f] * (o->p[nck] - o->p[i]) / o->dn[f]);
             }
             o->flux[f] = fl;
-            o->flux[4 * nck + o->llec[f]] = -fl;
+            o->flux[NF * nck + o->llec[f]] = -fl;
         }
     for (int i = 0; i < nc; i++) {
         o->sp[i] = o->sppp0[i];
         double fs = 0.0;
-        for (int k = 0; k < 4; k++) fs += o->flux[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) fs += o->flux[NF * i + k];
         o->su[i] = o->sspp0[i] - fs + o->p[i] * o->sp[i];
     }
+    if (!o->skewcor) return;
+    /* deferred skewness correction of the pp sources on joint cells
+     * (flow_imp.F90:718-789), from the previous pp gradient, scaled by 0.2 */
+    double *fcorr = DALLOC(NF * (size_t)o->ncellsD);
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        if (o->iwet[i] == 0) continue;
+        for (int j = 0; j < o->nxy[i]; j++) {
+            const int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
+            if (o->iwet[nck] == 0) continue;
+            const int b = NF * nck + o->llec[f];
+            const int xdir = o->idir[f] == KE || o->idir[f] == KW;
+            const double rpi = xdir ? o->rpy[f] : o->rpx[f], rpn = xdir ? o->rpy[b] : o->rpx[b];
+            const double *dpp = xdir ? o->dppy : o->dppx;
+            if (fabs(rpi) > F4(1.0e-6)) {
+                fcorr[f] = -o->acoef[f] * rpi * dpp[i];
+                fcorr[b] = -fcorr[f];
+            } else if (fabs(rpn) > F4(1.0e-6)) {
+                fcorr[f] = o->acoef[f] * rpn * dpp[nck];
+                fcorr[b] = -fcorr[f];
+            }
+        }
+    }
+    for (int jj = 0; jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        double fs = 0.0;
+        for (int k = 0; k < o->ncf[i]; k++) fs += fcorr[NF * i + k];
+        o->su[i] = o->su[i] + F4(0.2) * fs;
+    }
+    free(fcorr);
 }
 
 static void bound_pp(cms_oracle *o)
@@ -1067,19 +1392,19 @@ static void bound_pp(cms_oracle *o)
     for (int i = 0; i < nc; i++) {
         if (o->iwet[i] != 0) continue;
         int sumwet = 0;
-        for (int k = 0; k < 4; k++) { int nck = o->c2c[4 * i + k]; if (nck < nc) sumwet += o->iwet[nck]; }
+        for (int k = 0; k < o->ncf[i]; k++) { int nck = o->c2c[NF * i + k]; if (nck < nc) sumwet += o->iwet[nck]; }
         double aploc = 0.0;
         o->su[i] = 0.0;
         if (sumwet >= 1) {
-            for (int k = 0; k < 4; k++) {
-                int f = 4 * i + k, nck = o->c2c[f];
+            for (int k = 0; k < o->ncf[i]; k++) {
+                int f = NF * i + k, nck = o->c2c[f];
                 o->acoef[f] = o->iwet[nck];
                 aploc += o->acoef[f];
                 o->su[i] += o->acoef[f] * o->p[nck];
             }
         } else {
-            for (int k = 0; k < 4; k++) {
-                int f = 4 * i + k, nck = o->c2c[f];
+            for (int k = 0; k < o->ncf[i]; k++) {
+                int f = NF * i + k, nck = o->c2c[f];
                 if (nck >= nc) o->acoef[f] = 0.0;
                 else { o->acoef[f] = o->dsxy[f]; aploc += o->acoef[f]; o->su[i] += o->acoef[f] * o->p[nck]; }
             }
@@ -1095,7 +1420,7 @@ static void bound_pp(cms_oracle *o)
             double *ws = DALLOC(n);
             ws[im0] = 0.0;
             for (int im = im0 + 1; im < n; im++) {
-                int i = o->THc[im], i1 = o->THc[im - 1], f = 4 * i + o->THk[im];
+                int i = o->THc[im], i1 = o->THc[im - 1], f = NF * i + o->THk[im];
                 double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
                 double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
                 if (o->windconst) {   /* forcex/forcey add the wind stress, boundaries.F90:3978-3982 */
@@ -1108,7 +1433,7 @@ static void bound_pp(cms_oracle *o)
                 else ws[im] = ws[im - 1] + sx + sy;
             }
             for (int im = im0 - 1; im >= 0; im--) {
-                int i = o->THc[im], i1 = o->THc[im + 1], f = 4 * i + o->THk[im];
+                int i = o->THc[im], i1 = o->THc[im + 1], f = NF * i + o->THk[im];
                 double fx = o->c.noptset >= 3 ? o->wavestrx[i] : 0.0, fx1 = o->c.noptset >= 3 ? o->wavestrx[i1] : 0.0;
                 double fy = o->c.noptset >= 3 ? o->wavestry[i] : 0.0, fy1 = o->c.noptset >= 3 ? o->wavestry[i1] : 0.0;
                 if (o->windconst) {   /* forcex/forcey add the wind stress, boundaries.F90:3978-3982 */
@@ -1125,7 +1450,7 @@ static void bound_pp(cms_oracle *o)
             wse = o->wseadj;
         }
         for (int j = 0; j < o->nTH; j++) {
-            int i = o->THc[j], f = 4 * i + o->THk[j], nck = o->c2c[f];
+            int i = o->THc[j], f = NF * i + o->THk[j], nck = o->c2c[f];
             o->pp[nck] = wse[j] * GRAV - o->p[nck];
             o->su[i] += o->acoef[f] * o->pp[nck];
             o->sp[i] -= o->acoef[f];
@@ -1140,7 +1465,7 @@ static void pvfc(cms_oracle *o)
     const int nc = o->ncells, nd = o->ncellsD;
     for (int j = 0; j < o->nW; j++) {
         int i = o->Wc[j];
-        o->pp[o->c2c[4 * i + o->Wk[j]]] = o->pp[i];
+        o->pp[o->c2c[NF * i + o->Wk[j]]] = o->pp[i];
     }
     der_grad_eval(o, o->gow, o->pp, o->dppx, o->dppy);
     for (int i = 0; i < nc; i++) {
@@ -1149,10 +1474,30 @@ static void pvfc(cms_oracle *o)
         o->u[i] -= val * o->dppx[i];
         o->v[i] -= val * o->dppy[i];
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
             if (o->iwet[nck] == 0) continue;
             o->flux[f] -= o->acoef[f] * (o->pp[nck] - o->pp[i]);
-            o->flux[4 * nck + o->llec[f]] = -o->flux[f];
+            o->flux[NF * nck + o->llec[f]] = -o->flux[f];
+        }
+    }
+    /* skewness fixes of the corrected fluxes on joint cells (flow_imp.F90:912-962) */
+    for (int jj = 0; o->skewcor && jj < o->njoint; jj++) {
+        const int i = o->joint[jj];
+        if (o->iwet[i] == 0) continue;
+        for (int j = 0; j < o->nxy[i]; j++) {
+            const int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
+            if (nck >= nc || o->iwet[nck] == 0) continue;
+            const int b = NF * nck + o->llec[f];
+            const int xdir = o->idir[f] == KE || o->idir[f] == KW;
+            const double rpi = xdir ? o->rpy[f] : o->rpx[f], rpn = xdir ? o->rpy[b] : o->rpx[b];
+            const double *dpp = xdir ? o->dppy : o->dppx;
+            if (fabs(rpi) > F4(1.0e-6)) {
+                o->flux[f] = o->flux[f] + o->acoef[f] * rpi * dpp[i];
+                o->flux[b] = -o->flux[f];
+            } else if (fabs(rpn) > F4(1.0e-6)) {
+                o->flux[f] = o->flux[f] - o->acoef[f] * rpn * dpp[nck];
+                o->flux[b] = -o->flux[f];
+            }
         }
     }
     for (int i = 0; i < nd; i++) {
@@ -1162,7 +1507,7 @@ static void pvfc(cms_oracle *o)
     }
     for (int j = 0; j < o->nW; j++) {
         int i = o->Wc[j];
-        o->p[o->c2c[4 * i + o->Wk[j]]] = o->p[i];
+        o->p[o->c2c[NF * i + o->Wk[j]]] = o->p[i];
     }
 }
 
@@ -1171,11 +1516,11 @@ static void flow_grad_interp(cms_oracle *o)
 {
     der_grad_eval(o, o->gow, o->u, o->dux, o->duy);
     der_grad_eval(o, o->gow, o->v, o->dvx, o->dvy);
-    interp_cell2face(o, o->p, 1, o->pk);
+    interp_cell2face(o, o->p, 1, o->pk, o->dpx, o->dpy);
     for (int i = 0; i < o->ncells; i++) {
         double sx = 0.0, sy = 0.0;
-        for (int k = 0; k < 4; k++) {
-            int f = 4 * i + k;
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int f = NF * i + k;
             sx += o->dsx[f] * o->pk[f];
             sy += o->dsy[f] * o->pk[f];
             o->hk[f] = fmax(o->hmin, o->pk[f] * (1.0 / GRAV) - o->zbk[f]);
@@ -1277,20 +1622,26 @@ static void flow_eddyvis(cms_oracle *o)
     /* bndcopy2ghost: forcing strings */
     for (int j = 0; j < o->nTH; j++) {
         int i = o->THc[j];
-        o->vis[o->c2c[4 * i + o->THk[j]]] = o->vis[i];
+        o->vis[o->c2c[NF * i + o->THk[j]]] = o->vis[i];
     }
-    /* boundaries near dry cells (Cartesian branch) */
+    /* boundaries near dry cells (Cartesian and telescoping branch, :166-186) */
     double *rv = o->vis;
     double *old = NULL;
     if (!o->c.serial) { old = DALLOC(nd); memcpy(old, o->vis, sizeof(double) * nd); rv = old; }
     for (int i = 0; i < nc; i++) {
         if (o->iwet[i] != 1) continue;
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int nck = o->c2c[NF * i + k];
             if (o->iwet[nck] == 0 || nck >= nc) {
-                int opp = o->c2c[4 * i + kopp[k]];
-                double vissum = rv[opp] - CVISCON;
-                o->vis[i] = CVISCON + vissum / (1.0 + SMALL);
+                /* average over the faces on the opposite side */
+                const int kkdf = kopp[o->idir[NF * i + k]];
+                double cid = 0.0, vissum = 0.0;
+                for (int kk = 0; kk < o->ncf[i]; kk++)
+                    if (o->idir[NF * i + kk] == kkdf) {
+                        cid += 1.0;
+                        vissum += rv[o->c2c[NF * i + kk]] - CVISCON;
+                    }
+                o->vis[i] = CVISCON + vissum / (cid + SMALL);
                 o->vis[nck] = o->vis[i];
             }
         }
@@ -1300,15 +1651,15 @@ static void flow_eddyvis(cms_oracle *o)
     for (int i = 0; i < nc; i++) {
         if (o->icorner[i] != 1) continue;
         double cid = 0.0, vs = 0.0;
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int nck = o->c2c[NF * i + k];
             cid += o->iwet[nck];
             vs += (rv[nck] - CVISCON) * o->iwet[nck];
         }
         o->vis[i] = CVISCON + vs / (cid + SMALL);
     }
     free(old);
-    interp_cell2face(o, o->vis, 3, o->visk);
+    interp_cell2face(o, o->vis, 3, o->visk, NULL, NULL);
 }
 
 /* check_conv, flow_imp.F90:1576-1747 */
@@ -1351,8 +1702,8 @@ static void velface_eta(cms_oracle *o)
     for (int i = 0; i < o->ncells; i++) {
         if (o->h[i] > o->hmin && o->iwet[i] == 1) {
             double umin = 1.0e6, umax = -1.0e6, vmin = 1.0e6, vmax = -1.0e6;
-            for (int k = 0; k < 4; k++) {
-                int f = 4 * i + k;
+            for (int k = 0; k < o->ncf[i]; k++) {
+                int f = NF * i + k;
                 double vel = o->flux[f] / (o->hk[f] * o->ds[f]);
                 double uf = o->fnx[f] * vel, vf = o->fny[f] * vel;
                 umin = fmin(umin, uf); umax = fmax(umax, uf);
@@ -1390,7 +1741,7 @@ static int flow_recalculate(cms_oracle *o)
         o->pp[i] = 0.0; o->dppx[i] = 0.0; o->dppy[i] = 0.0;
         o->iwet[i] = o->iwet1[i];
     }
-    memcpy(o->flux, o->flux1, sizeof(double) * 4 * nd);
+    memcpy(o->flux, o->flux1, sizeof(double) * NF * nd);
     flow_wetdry(o);
     /* bndflux: flux and cross-shore strings only; none here */
     der_grad_cbgg(o, o->gow, 1);
@@ -1417,6 +1768,8 @@ restart:
             defcor_nvd(o, dsu, dsv);
             for (int i = 0; i < o->ncells; i++) { o->su[i] -= dsu[i]; o->sv[i] -= dsv[i]; }
             free(dsu); free(dsv);
+        } else if (o->njoint > 0 && o->c.ndsch != 0) {
+            defcor_skew(o);
         }
         bound_uv(o);
         o->kry[1] += solve_sys(o, o->su, o->spu, o->rsu, o->u, 2, &o->rmom[1]);
@@ -1450,7 +1803,7 @@ static void flow_update(cms_oracle *o)
         o->u1[i] = o->u[i]; o->v1[i] = o->v[i]; o->p1[i] = o->p[i]; o->h1[i] = o->h[i];
         o->pp[i] = 0.0; o->dppx[i] = 0.0; o->dppy[i] = 0.0;
     }
-    memcpy(o->flux1, o->flux, sizeof(double) * 4 * nd);
+    memcpy(o->flux1, o->flux, sizeof(double) * NF * nd);
     /* maximum water level, :352-355 (maxeta starts at 0, flow.F90:874) */
     for (int i = 0; i < nd; i++) o->maxeta[i] = fmax(o->eta[i], o->maxeta[i]);
 }
@@ -1459,7 +1812,7 @@ static void flow_update(cms_oracle *o)
 static double bnd_inflow(const cms_oracle *o)
 {
     double q = 0.0;
-    for (int j = 0; j < o->nTH; j++) q -= o->flux[4 * o->THc[j] + o->THk[j]];
+    for (int j = 0; j < o->nTH; j++) q -= o->flux[NF * o->THc[j] + o->THk[j]];
     return q * o->dtime;
 }
 
@@ -1468,12 +1821,12 @@ static void flow_convflux(cms_oracle *o)
 {
     for (int i = 0; i < o->ncells; i++)
         for (int j = 0; j < o->nxy[i]; j++) {
-            int k = o->kxy[4 * i + j], f = 4 * i + k, nck = o->c2c[f];
+            int k = o->kxy[NF * i + j], f = NF * i + k, nck = o->c2c[f];
             double uni = o->fnx[f] * o->u[i] + o->fny[f] * o->v[i];
             double unk = o->fnx[f] * o->u[nck] + o->fny[f] * o->v[nck];
             o->flux[f] = o->iwet[i] * o->iwet[nck] * o->ds[f] * o->hk[f]
                          * (o->fintp[f] * unk + (1.0 - o->fintp[f]) * uni);
-            o->flux[4 * nck + o->llec[f]] = -o->flux[f];
+            o->flux[NF * nck + o->llec[f]] = -o->flux[f];
         }
 }
 
@@ -1485,7 +1838,7 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     o->c = *c;
     o->nx = c->nx; o->ny = c->ny;
     build_grid(o);
-    const int nd = o->ncellsD, nc = o->ncells, F = 4 * nd;
+    const int nd = o->ncellsD, nc = o->ncells, F = NF * nd;
 #define A(f) o->f = DALLOC(nd)
     A(u); A(v); A(p); A(h); A(pp); A(dppx); A(dppy); A(dux); A(duy); A(dvx); A(dvy); A(dpx); A(dpy);
     A(vis); A(eta); A(u1); A(v1); A(p1); A(h1);
@@ -1493,13 +1846,16 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     A(apuareap); A(sumu); A(Hu); A(Hv); A(rsu); A(rsv); A(rsp);
     A(coefman); A(cfrict); A(z0); A(uv); A(uelwc); A(bsxy); A(bsvel); A(cmb); A(cbcfuwcap);
     A(wavestrx); A(wavestry); A(fc); A(cbf); A(maxeta);
+    A(dHux); A(dHuy); A(dHvx); A(dHvy); A(dapux); A(dapuy); A(dzbx); A(dzby);
 #undef A
     o->iwet = IALLOC(nd); o->iwet1 = IALLOC(nd); o->icorner = IALLOC(nd);
     o->flux = DALLOC(F); o->flux1 = DALLOC(F); o->hk = DALLOC(F); o->pk = DALLOC(F);
     o->visk = DALLOC(F); o->zbk = DALLOC(F); o->acoef = DALLOC(F);
     o->gow = (gradop *)xcalloc((size_t)nc, sizeof(gradop));
+    o->goa = (gradop *)xcalloc((size_t)nc, sizeof(gradop));
     o->wk = DALLOC(7 * (size_t)nc);
-    o->ia = IALLOC(nc + 1); o->ja = IALLOC(5 * nc); o->diagpos = IALLOC(nc); o->lu = DALLOC(5 * nc);
+    o->ia = IALLOC(nc + 1); o->ja = IALLOC((NF + 1) * nc); o->diagpos = IALLOC(nc);
+    o->lu = DALLOC((NF + 1) * (size_t)nc);
     /* fric_init (friction.F90:315-319): coefman = cfrict = 0, z0 = 1e-5, then
      * the law's input array from the dataset */
     for (int i = 0; i < nd; i++) o->z0[i] = F4(1.0e-5);
@@ -1548,7 +1904,11 @@ cms_oracle *cms_oracle_create(const cms_case *c)
     for (int n = 0; n < 3; n++) { o->rmom0[n] = 1.e10; o->rmom[n] = 0.0; }
     if (o->c.rtol <= 0) o->c.rtol = 1e-10;
     if (o->c.kmax <= 0) o->c.kmax = 2000;
-    interp_cell2face(o, o->zb, 0, o->zbk);       /* zbk: every face of every active cell */
+    /* zbk: every face of every active cell, with the all-cell gradient
+     * operator goa for the joint-cell reconstruction (prestart.f90:309-329) */
+    der_grad_cbgg(o, o->goa, 0);
+    der_grad_eval(o, o->goa, o->zb, o->dzbx, o->dzby);
+    interp_cell2face(o, o->zb, 0, o->zbk, o->dzbx, o->dzby);
     init_state(o);
     return o;
 }
@@ -1570,7 +1930,7 @@ static void init_state(cms_oracle *o)
     der_grad_cbgg(o, o->gow, 1);
     flow_grad_interp(o);
     flow_convflux(o);
-    memcpy(o->flux1, o->flux, sizeof(double) * 4 * nd);
+    memcpy(o->flux1, o->flux, sizeof(double) * NF * nd);
     fric_eval(o);
     flow_eddyvis(o);
 }
@@ -1586,13 +1946,25 @@ void cms_oracle_destroy(cms_oracle *o)
                     o->su, o->sv, o->sp, o->spu, o->spv, o->ssu0, o->ssv0, o->spuv0, o->sppp0, o->sspp0,
                     o->apuareap, o->sumu, o->Hu, o->Hv, o->rsu, o->rsv, o->rsp, o->coefman, o->cfrict, o->z0,
                     o->uv, o->uelwc, o->bsxy, o->bsvel, o->cmb, o->cbcfuwcap, o->wavestrx, o->wavestry, o->fc,
-                    o->cbf, o->maxeta, o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu};
+                    o->cbf, o->maxeta, o->gow, o->wk, o->ia, o->ja, o->diagpos, o->lu,
+                    o->ncf, o->idir, o->nxf, o->kxf, o->nyf, o->kyf, o->isjoint, o->joint, o->rpx, o->rpy,
+                    o->dHux, o->dHuy, o->dHvx, o->dHvy, o->dapux, o->dapuy, o->dzbx, o->dzby, o->goa};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
     free(o);
 }
 
 int cms_oracle_ncells(const cms_oracle *o) { return o->ncells; }
 int cms_oracle_ncellsD(const cms_oracle *o) { return o->ncellsD; }
+int cms_oracle_nf(const cms_oracle *o) { return o->nf; }
+
+int cms_oracle_topology(const cms_oracle *o, int *ncface, int *cell2cell, int *idirface, int *isjoint)
+{
+    if (ncface) memcpy(ncface, o->ncf, sizeof(int) * o->ncells);
+    if (cell2cell) memcpy(cell2cell, o->c2c, sizeof(int) * NF * o->ncellsD);
+    if (idirface) memcpy(idirface, o->idir, sizeof(int) * NF * o->ncellsD);
+    if (isjoint) memcpy(isjoint, o->isjoint, sizeof(int) * o->ncells);
+    return 0;
+}
 double cms_oracle_time(const cms_oracle *o) { return o->timesecs; }
 
 int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info)
@@ -1699,7 +2071,7 @@ int cms_oracle_get_face(const cms_oracle *o, const char *name, double *out)
     else if (!strcmp(name, "visk")) f = o->visk;
     else if (!strcmp(name, "pk")) f = o->pk;
     if (!f) return -1;
-    memcpy(out, f, sizeof(double) * 4 * o->ncells);
+    memcpy(out, f, sizeof(double) * NF * o->ncells);
     return 0;
 }
 
@@ -1718,8 +2090,8 @@ int cms_oracle_grad(cms_oracle *o, const double *var_act, double *gx, double *gy
     double *v = DALLOC(o->ncellsD);
     memcpy(v, var_act, sizeof(double) * o->ncells);
     for (int i = 0; i < o->ncells; i++)
-        for (int k = 0; k < 4; k++) {
-            int nck = o->c2c[4 * i + k];
+        for (int k = 0; k < o->ncf[i]; k++) {
+            int nck = o->c2c[NF * i + k];
             if (nck >= o->ncells) v[nck] = var_act[i];
         }
     der_grad_cbgg(o, o->gow, 1);

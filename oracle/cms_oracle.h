@@ -60,6 +60,19 @@ typedef struct {
     int wse_order;            /* TIME_INTERP_ORDER (0 -> 2)                               */
     int wse_adjust_off;       /* WSE_ADJUSTMENT OFF on the curve string                   */
     double wse_gradx, wse_grady;   /* WSE_GRADIENTS of the curve string [m/m]            */
+    /* Telescoping grid (read_tel, geospatial.F90:403-795).  tel_n > 0 replaces the
+     * rectangle (nx, ny, dx, dy unused): tel_n cells, all active, in the order of the
+     * .tel file; tel_geom[4*i..] = centre x, y and size dx, dy [m]; tel_nb[8*i..] = the
+     * file's neighbour slots N1 N2 E1 E2 S1 S2 W1 W2 (1-based cell numbers, 0 none).
+     * zb, mann, wavestrx/y are per cell in that order.  The water-level string is the
+     * cellstring tel_bnd[tel_nbnd] (0-based cells in string order, th_side >= 0); every
+     * other edge face is a wall. */
+    int tel_n;
+    const double *tel_geom;
+    const int *tel_nb;
+    const int *tel_bnd;
+    int tel_nbnd;
+    int noskew;               /* SKEWNESS_CORRECTION OFF (the default is on, flow.F90:101) */
 } cms_case;
 
 typedef struct {
@@ -73,6 +86,11 @@ typedef struct {
 cms_oracle *cms_oracle_create(const cms_case *c);
 void cms_oracle_destroy(cms_oracle *o);
 int cms_oracle_ncells(const cms_oracle *o);
+/* nmaxfaces: the face stride of get_face (4 Cartesian, 6 telescoping) */
+int cms_oracle_nf(const cms_oracle *o);
+/* Cell-list topology: ncface[ncells], cell2cell[nf*ncellsD] (0-based, -1 none),
+ * idirface[nf*ncellsD] (0..3 = N,E,S,W, -1 none), isjoint[ncells]. */
+int cms_oracle_topology(const cms_oracle *o, int *ncface, int *cell2cell, int *idirface, int *isjoint);
 int cms_oracle_ncellsD(const cms_oracle *o);
 /* Runs nsteps time steps (timing + flow_imp + flow_update).  Returns 0 on
  * success, -1 if the step diverged (iconv==0 with fixed dt). */
@@ -80,8 +98,8 @@ int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info);
 /* Copies an active-cell field (row-major nx*ny) out.  name: "eta","u","v","p",
  * "h","iwet","vis","pp","dpx","dpy","cfrict","uv","bsvel","zb". */
 int cms_oracle_get(const cms_oracle *o, const char *name, double *out);
-/* Unique-face fields: "fluxE","fluxN" [(nx+1)*ny] / [nx*(ny+1)] style is not used;
- * instead the directed face of every active cell: out[4*c+k], k=N,E,S,W. */
+/* Directed face field of every active cell: out[nf*c+k], k in the cell's face
+ * order (N,E,S,W on Cartesian grids). */
 int cms_oracle_get_face(const cms_oracle *o, const char *name, double *out);
 int cms_oracle_set(cms_oracle *o, const char *name, const double *in);
 double cms_oracle_time(const cms_oracle *o);

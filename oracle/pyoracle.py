@@ -30,6 +30,8 @@ class CmsCase(C.Structure):
         ("mturbul", C.c_int), ("mbedfric", C.c_int),
         ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
+        ("tel_n", C.c_int), ("tel_geom", D), ("tel_nb", I), ("tel_bnd", I), ("tel_nbnd", C.c_int),
+        ("noskew", C.c_int),
     ]
 
 
@@ -54,6 +56,8 @@ def lib():
         L.cms_oracle_get_face.argtypes = [C.c_void_p, C.c_char_p, D]
         L.cms_oracle_ncells.argtypes = [C.c_void_p]
         L.cms_oracle_ncellsD.argtypes = [C.c_void_p]
+        L.cms_oracle_nf.argtypes = [C.c_void_p]
+        L.cms_oracle_topology.argtypes = [C.c_void_p, I, I, I, I]
         L.cms_oracle_time.restype = C.c_double
         L.cms_oracle_time.argtypes = [C.c_void_p]
         L.cms_oracle_volume.argtypes = [C.c_void_p, D, D]
@@ -123,9 +127,22 @@ class Oracle:
             c.wse_ntimes, c.wse_order = len(case.wse_times), case.wse_order
             c.wse_adjust_off = int(not getattr(case, "wse_adjust", True))
             c.wse_gradx, c.wse_grady = getattr(case, "wse_grad", (0.0, 0.0))
+        tel = getattr(case, "tel", None)
+        if tel is not None:
+            if case.ndsch >= 5:
+                raise ValueError("NVD schemes are not restated on telescoping grids")
+            geom = np.ascontiguousarray(tel.geom, dtype=np.float64)
+            nb = np.ascontiguousarray(tel.nb, dtype=np.int32)
+            bnd = np.ascontiguousarray(tel.bnd_cells, dtype=np.int32)
+            self._keep += [geom, nb, bnd]
+            c.tel_n, c.tel_geom, c.tel_nb = len(geom), _dp(geom), nb.ctypes.data_as(I)
+            c.tel_bnd, c.tel_nbnd = bnd.ctypes.data_as(I), len(bnd)
+        c.noskew = int(getattr(case, "noskew", 0))
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)
+        self.ncellsD = L.cms_oracle_ncellsD(self.h)
+        self.nf = L.cms_oracle_nf(self.h)
         if getattr(case, "wind_speed", 0.0):
             self.set_wind(*case.wind_vector())
 
@@ -152,10 +169,19 @@ class Oracle:
             raise KeyError(name)
 
     def get_face(self, name):
-        out = np.empty(4 * self.ncells)
+        out = np.empty(self.nf * self.ncells)
         if lib().cms_oracle_get_face(self.h, name.encode(), _dp(out)) != 0:
             raise KeyError(name)
-        return out.reshape(self.ncells, 4)
+        return out.reshape(self.ncells, self.nf)
+
+    def topology(self):
+        """(ncface [n], cell2cell [nD, nf], idirface [nD, nf], isjoint [n])."""
+        n, nd, nf = self.ncells, self.ncellsD, self.nf
+        ncf, c2c = np.empty(n, np.int32), np.empty(nd * nf, np.int32)
+        idr, jt = np.empty(nd * nf, np.int32), np.empty(n, np.int32)
+        lib().cms_oracle_topology(self.h, ncf.ctypes.data_as(I), c2c.ctypes.data_as(I),
+                                  idr.ctypes.data_as(I), jt.ctypes.data_as(I))
+        return ncf, c2c.reshape(nd, nf), idr.reshape(nd, nf), jt
 
     def grad(self, values):
         a = np.ascontiguousarray(values, dtype=np.float64)

@@ -7,8 +7,8 @@
 ! single precision as shipped) into oracle/_ref/ref_run.
 !
 ! Usage: ref_run <case dir/> <case name> <nsteps> <output file|-> [forcing file]
-!   reads <case dir>/<case name>.cmcards (and the files it names) with the
-!   reference's card readers, initialises the model with the calls prestart
+!   reads <case dir>/<case name>.cmcards (and the files it names: a .cart or
+!   a .tel grid) with the reference's card readers, initialises the model with the calls prestart
 !   makes (source/main/prestart.f90:2-329) except those of the output module
 !   (source/output/output.F90 needs Intel's IFPORT module and is not built;
 !   it writes result files only), then runs the time loop of CMS_Flow
@@ -173,7 +173,14 @@ subroutine model_init()
     close(77)
     call assign_proj_names(projfl)
 
-    call read_grid_cart
+    ! the grid file prestart reads (prestart.f90:190-218): a telescoping grid
+    ! (.tel, TELESCOPING card) or a Cartesian one (.cart)
+    if (len_trim(telfile) > 0) then
+      igridtype = 1
+      call read_tel
+    else
+      call read_grid_cart
+    endif
     call geo_init
     call bnd_init
     call struct_init

@@ -169,6 +169,15 @@ def _mk_combo():
 CASES["tidal_combo"] = (_mk_combo, 40, [1, 10, 20, 40])
 
 
+# telescoping grids (read_tel): a shore-parallel band of 10 m cells around a
+# 5 m core across the beach, and an L-shaped patch on the south edge with
+# six-face cells; the tidal physics (wet/dry, radiation stress, M2 tide)
+CASES["tel_band"] = (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1),
+                     30, [1, 10, 20, 30])
+CASES["tel_patch"] = (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2,
+                                              levels=cases.patch_levels(40, 24)), 30, [1, 10, 20, 30])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

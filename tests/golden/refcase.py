@@ -31,11 +31,22 @@ TURB = {0: "CONSTANT", 1: "FALCONER", 2: "PARABOLIC", 3: "SUBGRID-WU", 5: "SUBGR
 FRIC = {1: "BOTTOM_FRICTION_COEF_DATASET", 2: "MANNINGS_N_DATASET", 3: "ROUGHNESS_HEIGHT_DATASET"}
 
 
-def write_case(case, d, name="case", solver="BICGSTAB"):
-    os.makedirs(d, exist_ok=True)
+def write_tel(tel, depth, path):
+    """.tel file read by read_tel (geospatial.F90:403-795): a header line, then
+    angle, origin x, y and the cell count, then one line per cell: id, centre
+    x, y, size dx, dy, the eight neighbour slots and the depth."""
+    with open(path, "w") as f:
+        f.write("CMS-Telescoping\n")
+        f.write(f"0.000000  0.000000  0.000000  {len(tel.geom)}\n")
+        for i, (g, nb) in enumerate(zip(tel.geom, tel.nb)):
+            f.write(f"{i + 1} {g[0]:.9g} {g[1]:.9g} {g[2]:.9g} {g[3]:.9g} "
+                    + " ".join(str(int(v)) for v in nb) + f" {float(depth[i]):.9g} 0\n")
+
+
+def write_cart(case, depth, path):
+    """.cart grid read by read_grid_cart (geospatial.F90:2553-2706)."""
     nx, ny = case.nx, case.ny
-    depth = -np.asarray(case.zb, dtype=np.float32)          # .cart holds depths (zb = -Z)
-    with open(os.path.join(d, name + ".cart"), "w") as f:
+    with open(path, "w") as f:
         f.write("GRID_TYPE REGULAR\n")
         f.write(f"GRID_NUMBER_CELLS_X {nx}\nGRID_NUMBER_CELLS_Y {ny}\n")
         f.write("GRID_RESOLUTION_X_BEGIN\n" + "".join(f"{case.dx:.9g}\n" for _ in range(nx))
@@ -43,12 +54,23 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
         f.write("GRID_RESOLUTION_Y_BEGIN\n" + "".join(f"{case.dy:.9g}\n" for _ in range(ny))
                 + "GRID_RESOLUTION_Y_END\n")
         f.write("GRID_DEPTHS_BEGIN\n" + "".join(f"{v:.9g}\n" for v in depth) + "GRID_DEPTHS_END\n")
+
+
+def write_case(case, d, name="case", solver="BICGSTAB"):
+    os.makedirs(d, exist_ok=True)
+    nx, ny = case.nx, case.ny
+    tel = getattr(case, "tel", None)
+    depth = -np.asarray(case.zb, dtype=np.float32)          # .cart holds depths (zb = -Z)
+    if tel is not None:
+        write_tel(tel, depth, os.path.join(d, name + ".tel"))
+    else:
+        write_cart(case, depth, os.path.join(d, name + ".cart"))
     mbedfric = getattr(case, "mbedfric", 2)
     fric = np.asarray(case.mann, dtype=np.float64)
     if mbedfric == 3:
         fric = fric * 30.0          # the dataset holds roughness heights (z0 = height / 30)
     with open(os.path.join(d, name + "_mann.txt"), "w") as f:
-        f.write(f"{nx * ny} 1\n")
+        f.write(f"{len(fric)} 1\n")
         f.write("".join(f"{float(np.float32(v)):.9g}\n" for v in fric))
     if mbedfric == 4:
         assert np.all(case.mann == case.mann[0]), "the linear law takes one constant"
@@ -56,7 +78,9 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
     else:
         fric_card = FRIC[mbedfric] + f' "{name}_mann.txt" "Friction"'
     side = case.th_side
-    if side == 1:
+    if tel is not None:
+        ids = [int(v) + 1 for v in tel.bnd_cells]
+    elif side == 1:
         ids = [nx + j * nx for j in range(ny)]                  # east column, full-grid ids
     elif side == 3:
         ids = [1 + j * nx for j in range(ny)]
@@ -96,7 +120,7 @@ def write_case(case, d, name="case", solver="BICGSTAB"):
 GRID_ANGLE                          0.0
 GRID_ORIGIN_X                       0.0
 GRID_ORIGIN_Y                       0.0
-GRID_FILE                           "{name}.cart"
+{f'TELESCOPING                         "{name}.tel"' if tel is not None else f'GRID_FILE                           "{name}.cart"'}
 USE_WALL_FRICTION_TERMS             {"ON" if case.wallfric else "OFF"}
 DRYING_DEPTH                        {case.hdry:.9g}
 HYDRO_TIMESTEP                      {case.dt:.9g} 'sec'
@@ -146,7 +170,10 @@ def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600):
         np.concatenate([np.asarray(case.wavestrx, np.float64),
                         np.asarray(case.wavestry, np.float64)]).tofile(force)
         args.append(force)
-    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout)
+    # one thread: three of the reference's OpenMP loops read values their own
+    # iterations write, so their result depends on the thread schedule
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"ref_run failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
     raw = np.fromfile(out, dtype=np.float64)

@@ -175,7 +175,15 @@ def _tidal_combo():
 
 
 CASES["tidal_combo"] = _tidal_combo
-ALL = ["tidal_combo", "tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+# telescoping grids (the .tel layout read_tel reads): a shore-parallel band of
+# 10 m cells around a 5 m core across the beach, and an L-shaped patch on the
+# south edge whose inner corner has a six-face cell
+TEL = {
+    "tel_band": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1),
+    "tel_patch": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)),
+}
+CASES.update(TEL)
+ALL = [*TEL, "tidal_combo", "tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory):
@@ -340,3 +348,34 @@ def test_wse_gradient_is_what_the_reference_uses():
     except AssertionError:
         return                     # wet/dry flags differ already
     assert w["eta"] > 10 * TOL_ETA or w["u"] > 10 * TOL_VEL, w
+
+
+@pytest.mark.parametrize("name", list(TEL))
+def test_skewness_corrections_are_what_the_reference_adds(name):
+    """Without the joint-cell corrections (SKEWNESS_CORRECTION OFF: no
+    reconstructed fluxes, pp and momentum corrections, face values or
+    gradient corrections) the oracle misses the telescoping runs by far more
+    than the bounds."""
+    def noskew(c):
+        c.noskew = 1
+        return Oracle(c, serial=1)
+    try:
+        w = compare_to_gold(name, noskew)
+    except AssertionError:
+        return                     # wet/dry flags differ already
+    assert w["u"] > 10 * TOL_VEL or w["v"] > 10 * TOL_VEL, w
+
+
+def test_uniform_grid_as_telescoping_file_is_the_cartesian_step():
+    """A telescoping cell list with every cell at the base level is the same
+    grid as the rectangle, numbered alike: the oracle's general cell-list path
+    (read_tel layout, no joint cells) gives the Cartesian result bit for bit."""
+    import dataclasses
+    c = cases.tidal(nx=24, ny=16, dx=20.0, seed=1)
+    t = cases.telescope(np.zeros((16, 24), dtype=int), 20.0, side=1)
+    a, b = Oracle(c, serial=1), Oracle(dataclasses.replace(c, tel=t), serial=1)
+    for _ in range(5):
+        a.step(1)
+        b.step(1)
+    for f in ("eta", "u", "v", "p", "iwet"):
+        assert np.array_equal(a.get(f), b.get(f)), f
