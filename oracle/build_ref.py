@@ -9,6 +9,21 @@ TEST INFRASTRUCTURE ONLY (the reference build is a checker for the oracle).
   shipped precision is kept (prec_def ikind=4: single precision).
 * Fortran module dependencies are resolved by repeated passes: each pass
   compiles the files whose modules are now available.
+* With --double (python oracle/build_ref.py --double) the same sources are
+  built in double precision: the tree is copied to a temporary directory
+  outside this repository, prec_def's ikind is set to 8
+  (source/main/definitions.F90:7), and the two calls that pass single-precision
+  latitude/longitude arrays to the real(ikind) argument of writescalTxt
+  (source/boundaries/boundaries.F90:3140,3145, an ASCII converter of the HDF5
+  inputs that the driver never reaches) pass real(lat, ikind) instead, and so
+  does one more call of the same kind in the tools utility
+  (source/utilities/tools.F90:675, whose module also holds string helpers the
+  card readers use).  Two files pass real(ikind) arrays to the
+  single-precision XMDF interface and do not compile at ikind=8: the dredging
+  setup (source/dredge/dredge_setup.F90:749,816,855) and CMS-Wave's HDF5
+  output; neither is on the flow step's path, they stay out of this build, and
+  the driver then sets dredging off itself instead of calling dredge_default.  Outputs:
+  oracle/_ref/libcmsref8.so and oracle/_ref/ref_run8.
 * Outputs (objects, .mod files, the shared library) go to oracle/_ref/ only.
   The library is linked with the driver oracle/ref_driver.f90 (ours).  The
   XMDF/HDF5 C library the reference's I/O wraps is absent from this image; no
@@ -24,15 +39,47 @@ import re
 import subprocess
 import sys
 
+import shutil
+import tempfile
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = os.environ.get("CMS2D_REF", "/root/reference")
 SRC = os.path.join(REF, "source")
 OUT = os.path.join(HERE, "_ref")
-OBJ = os.path.join(OUT, "obj")
-MOD = os.path.join(OUT, "mod")
+DOUBLE = "--double" in sys.argv[1:]
+SUFFIX = "8" if DOUBLE else ""
+# the edits of the double-precision copy: (file, text, replacement, count)
+DOUBLE_EDITS = [
+    ("main/definitions.F90", "integer, parameter :: ikind=4", "integer, parameter :: ikind=8", 1),
+    ("boundaries/boundaries.F90", "trim(dsetname), lat, 1)", "trim(dsetname), real(lat, ikind), 1)", 1),
+    ("boundaries/boundaries.F90", "trim(dsetname), lon, 1)", "trim(dsetname), real(lon, ikind), 1)", 1),
+    ("utilities/tools.F90", "'Maximum_WSE','maxeta',max_array)", "'Maximum_WSE','maxeta',real(max_array, 8))", 1),
+]
+OBJ = os.path.join(OUT, "obj" + SUFFIX)
+MOD = os.path.join(OUT, "mod" + SUFFIX)
 FC = os.environ.get("FC_REF", "/opt/rocm/lib/llvm/bin/flang")
-FFLAGS = ["-cpp", "-fPIC", "-O2", "-fopenmp", "-I", os.path.join(SRC, "main"),
-          "-module-dir", MOD, "-I", MOD, "-w"]
+FFLAGS = []
+
+
+def set_flags():
+    global FFLAGS
+    FFLAGS = ["-cpp", "-fPIC", "-O2", "-fopenmp", "-I", os.path.join(SRC, "main"),
+              "-module-dir", MOD, "-I", MOD, "-w"]
+
+
+def double_copy():
+    """Copies the reference's source tree to a temporary directory and applies
+    DOUBLE_EDITS there; returns the copy's source directory."""
+    tmp = tempfile.mkdtemp(prefix="cmsref8_")
+    dst = os.path.join(tmp, "source")
+    shutil.copytree(SRC, dst)
+    for rel, old, new, count in DOUBLE_EDITS:
+        path = os.path.join(dst, rel)
+        txt = open(path).read()
+        if txt.count(old) != count:
+            raise SystemExit(f"{rel}: expected {count} occurrence(s) of {old!r}")
+        open(path, "w").write(txt.replace(old, new))
+    return dst
 # The program unit is not needed in a library.
 SKIP = {"main/CMS2D_main.f90"}
 
@@ -57,9 +104,13 @@ def compile_one(rel):
 
 
 def main():
+    global SRC
     if not os.path.isdir(SRC):
         print(f"reference sources not found under {SRC}: nothing to build", file=sys.stderr)
         return 0
+    if DOUBLE:
+        SRC = double_copy()
+    set_flags()
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(MOD, exist_ok=True)
     todo = source_list()
@@ -75,22 +126,25 @@ def main():
         if len(left) == len(todo):
             break
         todo = left
-    with open(os.path.join(OUT, "build_ref.log"), "w") as f:
+    with open(os.path.join(OUT, f"build_ref{SUFFIX}.log"), "w") as f:
         for r, e in errs.items():
             f.write(f"=== {r}\n{e}\n")
     drv = os.path.join(HERE, "ref_driver.f90")
-    lib = os.path.join(OUT, "libcmsref.so")
+    lib = os.path.join(OUT, f"libcmsref{SUFFIX}.so")
     objs = [obj_of(r) for r in source_list() if os.path.exists(obj_of(r))]
     subprocess.run([FC, "-shared", "-fopenmp", "-o", lib + ".tmp", *objs], check=True)
     os.replace(lib + ".tmp", lib)
     print(f"linked {lib} from {len(objs)} reference objects", flush=True)
     if os.path.exists(drv):
         dobj = os.path.join(OBJ, "ref_driver.o")
-        exe = os.path.join(OUT, "ref_run")
-        subprocess.run([FC, *FFLAGS, "-c", drv, "-o", dobj], check=True)
-        subprocess.run([FC, "-fopenmp", "-o", exe, dobj, "-L", OUT, "-lcmsref",
+        exe = os.path.join(OUT, f"ref_run{SUFFIX}")
+        dflags = ["-DREF_NO_DREDGE"] if DOUBLE else []
+        subprocess.run([FC, *FFLAGS, *dflags, "-c", drv, "-o", dobj], check=True)
+        subprocess.run([FC, "-fopenmp", "-o", exe, dobj, "-L", OUT, f"-lcmsref{SUFFIX}",
                         "-Wl,-rpath,$ORIGIN", "-Wl,--allow-shlib-undefined"], check=True)
         print(f"linked {exe}", flush=True)
+    if DOUBLE:
+        shutil.rmtree(os.path.dirname(SRC), ignore_errors=True)
     if errs:
         print(f"{len(errs)} reference files did not compile (see {OUT}/build_ref.log): "
               + ", ".join(sorted(errs)), flush=True)

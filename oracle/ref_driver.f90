@@ -123,6 +123,9 @@ subroutine model_init()
     use der_lib,    only: der_grad_eval
     use interp_lib, only: interp_scal_cell2face
     use size_def
+#ifdef REF_NO_DREDGE
+    use dredge_def, only: dredging
+#endif
     implicit none
     integer :: ierr
     character(len=37) :: cardname
@@ -144,7 +147,13 @@ subroutine model_init()
     call der_default
     call diag_default
     call wave_default
+#ifdef REF_NO_DREDGE
+    ! the double-precision build leaves out the dredging module's setup
+    ! (its HDF5 reads do not compile at ikind=8); dredging stays off
+    dredging = .false.
+#else
     call dredge_default
+#endif
 
     open(77, file=ctlfile, status='old')
     do
@@ -166,7 +175,9 @@ subroutine model_init()
       call rol_cards(cardname, foundcard);           if (foundcard) cycle
       call diag_cards(cardname, foundcard);          if (foundcard) cycle
       call wave_cards(cardname, foundcard);          if (foundcard) cycle
+#ifndef REF_NO_DREDGE
       call dredge_cards(cardname, foundcard);        if (foundcard) cycle
+#endif
       call der_cards(cardname, foundcard);           if (foundcard) cycle
       call stat_cards(cardname, foundcard)
     enddo

@@ -3,9 +3,12 @@
 
 Runs oracle/_ref/ref_run (the reference's Fortran sources compiled unmodified
 by oracle/build_ref.py, single precision as shipped, BiCGStab with ILUTP and
-its 1e-3 inner tolerance) on two seeded cases and stores eta, u, v and iwet
-after selected steps in tests/golden/ref_<case>.npz, together with the case
-parameters.  Regenerate with: python tests/golden/make_ref_vectors.py
+its 1e-3 inner tolerance) on seeded cases and stores eta, u, v and iwet after
+selected steps in tests/golden/ref_<case>.npz; with --double the same from
+oracle/_ref/ref_run8 (the reference built in double precision, build_ref.py
+--double) into tests/golden/ref64_<case>.npz.  The reference runs on one
+OpenMP thread (the run's thread count and precision are stored).  Regenerate
+with: python tests/golden/make_ref_vectors.py [--double] [case ...]
 """
 import os
 import sys
@@ -183,20 +186,25 @@ def only(names):
 
 
 def main():
-    if not refcase.available():
-        print("oracle/_ref/ref_run not built (python oracle/build_ref.py)", file=sys.stderr)
+    args = sys.argv[1:]
+    double = "--double" in args
+    args = [a for a in args if a != "--double"]
+    exe = refcase.REF_RUN8 if double else refcase.REF_RUN
+    prefix = "ref64_" if double else "ref_"
+    if not refcase.available(exe):
+        print(f"{exe} not built (python oracle/build_ref.py{' --double' if double else ''})", file=sys.stderr)
         return 1
-    todo = only(sys.argv[1:]) if len(sys.argv) > 1 else CASES
+    todo = only(args) if args else CASES
     for name, (mk, nsteps, keep, *opt) in todo.items():
         c = mk()
         with tempfile.TemporaryDirectory() as d:
-            res = refcase.run(c, d + "/", nsteps, **(opt[0] if opt else {}))
-        out = {"steps": np.array(keep)}
+            res = refcase.run(c, d + "/", nsteps, exe=exe, **(opt[0] if opt else {}))
+        out = {"steps": np.array(keep), "omp_threads": np.array(1), "ikind": np.array(8 if double else 4)}
         for s in keep:
             for f in ("eta", "u", "v", "iwet"):
                 out[f"{f}_{s}"] = res[s - 1][f]
-        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **out)
-        print(f"wrote ref_{name}.npz ({nsteps} steps, kept {keep})")
+        np.savez_compressed(os.path.join(HERE, f"{prefix}{name}.npz"), **out)
+        print(f"wrote {prefix}{name}.npz ({nsteps} steps, kept {keep})")
     return 0
 
 

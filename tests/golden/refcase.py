@@ -16,10 +16,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 REF_RUN = os.path.join(ROOT, "oracle", "_ref", "ref_run")
+REF_RUN8 = os.path.join(ROOT, "oracle", "_ref", "ref_run8")     # double precision (build_ref.py --double)
 
 
-def available():
-    return os.path.exists(REF_RUN)
+def available(exe=REF_RUN):
+    return os.path.exists(exe)
 
 
 # advection scheme names of the ADVECTION_SCHEME card, in ndsch order
@@ -160,11 +161,11 @@ END_PARAMETERS
     return d
 
 
-def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600):
+def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RUN):
     """Returns a list of dicts (eta, u, v, iwet) after each step."""
     write_case(case, d, name, solver)
     out = os.path.join(d, "state.bin")
-    args = [REF_RUN, d.rstrip("/") + "/", name, str(nsteps), out]
+    args = [exe, d.rstrip("/") + "/", name, str(nsteps), out]
     if case.noptset >= 3 and case.wavestrx is not None:
         force = os.path.join(d, "forcing.bin")
         np.concatenate([np.asarray(case.wavestrx, np.float64),
