@@ -77,6 +77,8 @@ class Case:
     tel: object = None        # TelGrid: the cells of a telescoping grid (nx, ny, dx, dy then
                               # describe its base grid; zb, mann, wavestr are per cell)
     noskew: int = 0           # SKEWNESS_CORRECTION OFF (telescoping grids)
+    dtvar: int = 0            # VARIABLE_TIMESTEP (the reference's card; the flow objects take it
+                              # as an argument)
 
     def wind_vector(self):
         """(wndx, wndy) in grid coordinates as windcurve_init makes them

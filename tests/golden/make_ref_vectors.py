@@ -181,6 +181,18 @@ CASES["tel_patch"] = (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2,
                                               levels=cases.patch_levels(40, 24)), 30, [1, 10, 20, 30])
 
 
+# VARIABLE_TIMESTEP ON: the tidal case at dt = 3600 s, whose sixth step ends
+# its outer iterations unconverged (iconv 3), so timing halves the next step
+# and doubles it back two steps later (CMS_Flow.f90:180-205)
+def _mk_dtvar():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1, dt=3600.0)
+    c.dtvar = 1
+    return c
+
+
+CASES["tidal_dtvar"] = (_mk_dtvar, 12, [1, 6, 7, 8, 9, 12])
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 

@@ -127,7 +127,7 @@ DRYING_DEPTH                        {case.hdry:.9g}
 HYDRO_TIMESTEP                      {case.dt:.9g} 'sec'
 STARTING_DATE_TIME                  2001-01-01 00:00:00 UTC
 DURATION_RUN                        {max(1.0, case.dt * case.nsteps / 3600.0):.9g} 'hrs'
-VARIABLE_TIMESTEP                   OFF
+VARIABLE_TIMESTEP                   {"ON" if getattr(case, "dtvar", 0) else "OFF"}
 {f"DURATION_RAMP                       {case.rampdur:.9g} 'hrs'" if case.rampdur else ""}
 SOLUTION_SCHEME                     IMPLICIT
 MATRIX_SOLVER                       {solver}
