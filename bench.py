@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Throughput of the MI355X implicit CMS-Flow step (implicit cell-steps/s).
 
-Workload: the tidal basin case -- 2000 x 1500 Cartesian cells of 20 m (3.0 M
-cells), beach with wetting and drying (hdry 0.05 m), lognormal Manning,
+Workload: the tidal case on a telescoping grid -- 2000 x 1500 base cells of
+20 m with a shore-parallel band of 10 m cells around a 5 m core across the
+beach (3.61 M cells), wetting and drying (hdry 0.05 m), lognormal Manning,
 radiation-stress forcing up to 0.05 N/m^2, M2 tide on the east side,
-dt = 30 s, FP64 SIMPLEC with BiCGStab solves to rtol 1e-10.  A "step" is one
+dt = 30 s, FP64 SIMPLEC with BiCGStab solves to rtol 1e-10.  The same physics
+on the uniform 2000 x 1500 grid is reported beside it.  A "step" is one
 implicit time step (flow_imp + flow_update) over the whole grid, with every
 input resident in HBM.
 
@@ -43,6 +45,16 @@ GROUPS = {
     "spmv_pp": (60, "k_bicg_spmv<StencilOp,1,1,1> (pp system, v = A z, z from the multigrid cycle)"),
     "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
     "mg_up0": (37, "k_mg_up<double> (multigrid level 0: correction + red-black post-smoothing)"),
+}
+# The telescoping step: the pp SpMV on the cell list reads per active cell its
+# face metadata word, per face the neighbour index and coefficient, ap, r0 and
+# z (FP32), and writes v: 8 + 12 F + 8 + 8 + 4 + 8 = 36 + 12 F bytes with F
+# faces (4 on base-level cells: 84); the base-grid multigrid sweeps as above
+# per base cell (3.0 M at 2000 x 1500), counted here per active cell.
+TEL_GROUPS = {
+    "spmv_pp": (84, "k_bicg_spmv<TelOp,1,1,1> (pp system on the cell list, v = A z)"),
+    "mg_down0": (33 * 3.0 / 3.612, "k_mg_down<float> (base-grid multigrid level 0: pre-smoothing + restriction)"),
+    "mg_up0": (37 * 3.0 / 3.612, "k_mg_up<float> (base-grid multigrid level 0: correction + post-smoothing)"),
 }
 # Whole-step byte model (FP64, per active cell): once per step 358 (static
 # sources, wet/dry, velface_eta, level rotation, mass balance); per outer
@@ -197,39 +209,31 @@ def ensemble_form(a):
     return out
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py: --gpus {a.gpus} not supported: this build runs on one GPU "
-              f"(multi-GPU sharding is designed, not built)", file=sys.stderr)
-        sys.exit(2)
-    from cms2d_amd import cases
+def progress(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def measure(a, case, label, groups):
+    """Times a.steps steps of the FP64 form on `case` after a.warmup; the live
+    per-launch times of the kernel groups; the whole-step byte model."""
     from cms2d_amd.flow import FlowStep
-    # CPU baselines first, before this process touches the GPU (the compiled
-    # reference runs as a child process)
-    cpu = port = None
-    if not a.no_cpu_baseline:
-        cpu = cpu_reference(a.ref_sample, a.ref_threads)
-        port = cpu_baseline(a.cpu_sample)
-    case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
     ncells = case.ncells
     g = FlowStep(case)
     mg_tile, graphs = g.tuning()
     for _ in range(a.warmup):
         g.step(1)
-    st0 = {k: g.kernel_stats(k) for k in GROUPS}
+    st0 = {k: g.kernel_stats(k) for k in groups}
     kry = np.zeros(3)
     outer = 0
     t0 = time.perf_counter()
-    kernel_ms = 0.0
     for _ in range(a.steps):
         info = g.step(1)
-        kernel_ms += info["kernel_ms"]
         kry += np.array(info["kry"])
         outer += info["niter"]
     elapsed = time.perf_counter() - t0
+    progress(f"{label}: {a.steps} steps of {ncells} cells in {elapsed:.2f} s")
     kern = {}
-    for k, (bpc, desc) in GROUPS.items():
+    for k, (bpc, desc) in groups.items():
         ms1, n1 = g.kernel_stats(k)
         ms0, n0 = st0[k]
         nl = n1 - n0
@@ -239,33 +243,66 @@ def main():
                    "bytes_per_launch": bpc * ncells,
                    "achieved_GBs": bpc * ncells / (avg * 1e-3) / 1e9 if avg else None}
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    D = kern[dom]
-    achieved = D["achieved_GBs"]
     value = ncells * a.steps / elapsed
     model_b = (STEP_MODEL["per_step"] + STEP_MODEL["per_outer"] * outer / a.steps
                + STEP_MODEL["per_krylov_iter"] * kry.sum() / a.steps)
     model_gbs = model_b * value / 1e9
+    res = {"value": value, "ms_per_step": elapsed * 1e3 / a.steps, "cells": ncells, "label": label,
+           "mg_tile_x": mg_tile, "graph_replay": graphs, "outer_iters_per_step": outer / a.steps,
+           "krylov_iters_per_step": {"pp": kry[0] / a.steps, "u": kry[1] / a.steps, "v": kry[2] / a.steps},
+           "kern": kern, "dominant": dom,
+           "step_model_roofline": {
+               "bytes_per_cell_step": model_b, "achieved": model_gbs, "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": model_gbs / HBM_PEAK_GBS,
+               "model": "whole step, FP64 bytes per active cell: 358 per step + 1156 per outer "
+                        "iteration + 376 per BiCGStab iteration (u, v and pp), with this run's "
+                        "iteration counts, at the measured cell-steps/s"}}
+    return g, res
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: --gpus {a.gpus} not supported: this build runs on one GPU "
+              f"(multi-GPU sharding is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+    from cms2d_amd import cases
+    # CPU baselines first, before this process touches the GPU (the compiled
+    # reference runs as a child process)
+    cpu = port = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_reference(a.ref_sample, a.ref_threads)
+        progress("compiled-reference CPU sample done")
+        port = cpu_baseline(a.cpu_sample)
+        progress("oracle CPU sample done")
+    tel_case = cases.tidal_tel(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
+    progress(f"telescoping grid built: {tel_case.ncells} cells")
+    g, T = measure(a, tel_case, "telescoping", TEL_GROUPS)
+    g.close()
+    D = T["kern"][T["dominant"]]
+    achieved = D["achieved_GBs"]
+    lev = tel_case.tel.level
     out = {
         "metric": "implicit cell-steps/s",
-        "value": value,
+        "value": T["value"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": elapsed * 1e3 / a.steps,
+        "ms_per_step": T["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"tidal basin {a.nx}x{a.ny} uniform Cartesian dx=20m (the 2-level "
-                        f"telescoping band of the reference case is not built yet), wet/dry, "
-                        f"radiation stress, M2 tide, dt=30s, FP64 BiCGStab rtol 1e-10",
-            "cells": ncells, "dt_s": case.dt, "mg_tile_x_autotuned": mg_tile, "graph_replay": graphs,
-            "outer_iters_per_step": outer / a.steps,
-            "krylov_iters_per_step": {"pp": kry[0] / a.steps, "u": kry[1] / a.steps,
-                                      "v": kry[2] / a.steps},
+            "workload": f"tidal case on a telescoping grid: {a.nx}x{a.ny} base cells of 20 m with a "
+                        f"shore-parallel band of 10 m cells around a 5 m core, {T['cells']} cells; "
+                        f"wet/dry, radiation stress, M2 tide, dt=30s, FP64 BiCGStab rtol 1e-10",
+            "cells": T["cells"], "cells_per_level": {"20m": int((lev == 0).sum()), "10m": int((lev == 1).sum()),
+                                                     "5m": int((lev == 2).sum())},
+            "dt_s": tel_case.dt, "outer_iters_per_step": T["outer_iters_per_step"],
+            "krylov_iters_per_step": T["krylov_iters_per_step"],
         },
         "roofline": {
             "kernel": D["kernel"],
@@ -274,23 +311,30 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": TRAFFIC_PER_CELL[dom] * ncells,
+            "traffic": None,
             "avg_launch_ms": D["avg_launch_ms"],
             "launches": D["launches"],
             "bytes_per_launch": D["bytes_per_launch"],
-            "timed_groups": kern,
+            "timed_groups": T["kern"],
         },
-        "step_model_roofline": {
-            "bytes_per_cell_step": model_b, "achieved": model_gbs, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": model_gbs / HBM_PEAK_GBS,
-            "model": "whole step, FP64 bytes per active cell: 358 per step + 1156 per outer "
-                     "iteration + 376 per BiCGStab iteration (u, v and pp), with this run's "
-                     "iteration counts, at the measured cell-steps/s",
-        },
+        "step_model_roofline": T["step_model_roofline"],
         "cpu_baseline": None,
     }
+    case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
+    g, C = measure(a, case, "cartesian", GROUPS)
+    Dc = C["kern"][C["dominant"]]
+    out["cartesian_uniform"] = {
+        "workload": f"the same physics on the uniform {a.nx}x{a.ny} grid of 20 m cells",
+        "value": C["value"], "unit": "cell-steps/s", "ms_per_step": C["ms_per_step"], "cells": C["cells"],
+        "mg_tile_x_autotuned": C["mg_tile_x"], "outer_iters_per_step": C["outer_iters_per_step"],
+        "krylov_iters_per_step": C["krylov_iters_per_step"],
+        "roofline": {"kernel": Dc["kernel"], "achieved": Dc["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                     "frac": Dc["achieved_GBs"] / HBM_PEAK_GBS if Dc["achieved_GBs"] else None,
+                     "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"], "timed_groups": C["kern"]},
+        "step_model_roofline": C["step_model_roofline"]}
     if not a.no_fp32:
-        out["fp32_refine"] = fp32_form(a, case, g, value)
+        out["fp32_refine"] = fp32_form(a, case, g, C["value"])
+        out["fp32_refine"]["workload"] = "the uniform Cartesian grid (the FP32 form is built there)"
     g.close()
     if not a.no_ensemble:
         out["ensemble"] = ensemble_form(a)

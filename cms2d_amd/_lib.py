@@ -36,7 +36,7 @@ EXPORTS = [
     "cms2d_flow_step", "cms2d_flow_get", "cms2d_flow_get_face", "cms2d_flow_set",
     "cms2d_flow_time", "cms2d_flow_kernel_stats", "cms2d_flow_tuning", "cms2d_flow_set_waves",
     "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
-    "cms2d_ensemble_step", "cms2d_solve",
+    "cms2d_ensemble_step", "cms2d_solve", "cms2d_flow_create_tel",
 ]
 
 
@@ -55,6 +55,10 @@ class FlowDesc(C.Structure):
         ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
     ]
+
+
+class TelGrid(C.Structure):
+    _fields_ = [("ncells", C.c_int), ("geom", D), ("nb", I), ("bnd_cells", I), ("nbnd", C.c_int)]
 
 
 class StepInfo(C.Structure):
@@ -84,6 +88,7 @@ def lib():
     L.cms2d_device_count.argtypes = [I]
     L.cms2d_flow_create.argtypes = [C.POINTER(FlowDesc), C.POINTER(C.c_void_p)]
     L.cms2d_flow_destroy.argtypes = [C.c_void_p]
+    L.cms2d_flow_create_tel.argtypes = [C.POINTER(FlowDesc), C.POINTER(TelGrid), C.POINTER(C.c_void_p)]
     L.cms2d_flow_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepInfo)]
     L.cms2d_flow_get.argtypes = [C.c_void_p, C.c_char_p, D]
     L.cms2d_flow_get_face.argtypes = [C.c_void_p, C.c_char_p, D]

@@ -225,52 +225,58 @@ def telescope(levels, dx, side=1) -> TelGrid:
     assert levels.min() >= 0 and levels.max() <= 2
     assert np.abs(np.diff(levels, axis=0)).max(initial=0) <= 1
     assert np.abs(np.diff(levels, axis=1)).max(initial=0) <= 1
-    ids = {}
-    geom, base, lev = [], [], []
-    for j in range(ny):
-        for i in range(nx):
-            L = int(levels[j, i])
-            m = 1 << L
-            h = dx / m
-            for b in range(m):
-                for a in range(m):
-                    I, J = i * m + a, j * m + b
-                    ids[(L, I, J)] = len(geom)
-                    geom.append((I * h + 0.5 * h, J * h + 0.5 * h, h, h))
-                    base.append(j * nx + i)
-                    lev.append(L)
-    n = len(geom)
+    m = 1 << levels.ravel()                        # cells per side of each base cell
+    cnt = m * m
+    first = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    n = int(cnt.sum())
+    base = np.repeat(np.arange(nx * ny), cnt)
+    q = np.arange(n) - np.repeat(first, cnt)       # position inside the base cell
+    mm = np.repeat(m, cnt)
+    a, b = q % mm, q // mm
+    lev = np.repeat(levels.ravel(), cnt)
+    I = (base % nx) * mm + a                       # lattice position at the cell's own level
+    J = (base // nx) * mm + b
+    h = dx / mm
+    geom = f32(np.stack([I * h + 0.5 * h, J * h + 0.5 * h, h, h], axis=1))
+    def ids(L, Ia, Ja):
+        # the number of the level-L cell at lattice position (Ia, Ja) of a base
+        # cell of level L: the base cell's first cell plus its row-major offset
+        bb = (Ja >> L) * nx + (Ia >> L)
+        mk = (1 << L) - 1
+        return first[bb] + (Ja & mk) * (1 << L) + (Ia & mk)
+
     nb = np.zeros((n, 8), dtype=np.int32)
-    # per side: (di, dj), the two finer neighbours' offsets in slot order
-    sides = [((0, 1), [(0, 1), (1, 1)]),     # N: west, east  (fine offsets from 2I, 2J)
-             ((1, 0), [(1, 1), (1, 0)]),     # E: north, south
-             ((0, -1), [(1, 0), (0, 0)]),    # S: east, west
-             ((-1, 0), [(0, 0), (0, 1)])]    # W: south, north
-    for (L, I, J), c in ids.items():
+    # per side: (di, dj) and the offsets of the two finer neighbours in slot order:
+    # north west then east, east north then south, south east then west, west south then north
+    sides = [((0, 1), [(0, 1), (1, 1)]), ((1, 0), [(1, 1), (1, 0)]),
+             ((0, -1), [(1, 0), (0, 0)]), ((-1, 0), [(0, 0), (0, 1)])]
+    for L in range(3):
+        sel = np.nonzero(lev == L)[0]
+        if sel.size == 0:
+            continue
+        Ic, Jc = I[sel], J[sel]
         for s_, ((di, dj), fine) in enumerate(sides):
-            In, Jn = I + di, J + dj
-            m = 1 << L
-            bi, bj = In // m if In >= 0 else -1, Jn // m if Jn >= 0 else -1
-            if not (0 <= In < nx * m and 0 <= Jn < ny * m):
-                continue
-            Lb = int(levels[bj, bi])
-            if Lb == L:
-                nb[c, 2 * s_] = ids[(L, In, Jn)] + 1
-            elif Lb == L - 1:
-                nb[c, 2 * s_] = ids[(L - 1, In // 2, Jn // 2)] + 1
-            else:
-                # the two cells of level L+1 across this side
-                for q, (oa, ob) in enumerate(fine):
+            In, Jn = Ic + di, Jc + dj
+            inside = (In >= 0) & (In < nx << L) & (Jn >= 0) & (Jn < ny << L)
+            c, In, Jn = sel[inside], In[inside], Jn[inside]
+            Lb = levels[Jn >> L, In >> L]
+            same = Lb == L
+            nb[c[same], 2 * s_] = ids(L, In[same], Jn[same]) + 1
+            if L > 0:
+                up = Lb == L - 1
+                nb[c[up], 2 * s_] = ids(L - 1, In[up] >> 1, Jn[up] >> 1) + 1
+            if L < 2:
+                dn = Lb == L + 1
+                for qq, (oa, ob) in enumerate(fine):
                     if di:
-                        fi = 2 * In + (0 if di > 0 else 1)
-                        fj = 2 * J + ob
+                        fi = 2 * In[dn] + (0 if di > 0 else 1)
+                        fj = 2 * (Jn[dn] - dj) + ob
                     else:
-                        fi = 2 * I + oa
-                        fj = 2 * Jn + (0 if dj > 0 else 1)
-                    nb[c, 2 * s_ + q] = ids[(L + 1, fi, fj)] + 1
+                        fi = 2 * (In[dn] - di) + oa
+                        fj = 2 * Jn[dn] + (0 if dj > 0 else 1)
+                    nb[c[dn], 2 * s_ + qq] = ids(L + 1, fi, fj) + 1
     nfaces = np.maximum((nb[:, 0::2] > 0) + (nb[:, 1::2] > 0), 1).sum(axis=1)
     assert nfaces.max() <= 6, "a cell with finer neighbours on more than two sides (nmaxfaces = 6)"
-    geom = f32(np.array(geom))
     # the water-level string: the cells along the edge `side`, in order along it
     g = geom
     Lx, Ly = nx * dx, ny * dx
@@ -286,8 +292,7 @@ def telescope(levels, dx, side=1) -> TelGrid:
     else:
         sel = np.nonzero(np.isclose(g[:, 1] - 0.5 * g[:, 3], 0.0))[0]
         order = sel[np.argsort(g[sel, 0], kind="stable")]
-    return TelGrid(geom, nb, np.array(base, dtype=np.int32), np.array(lev, dtype=np.int32),
-                   order.astype(np.int32))
+    return TelGrid(geom, nb, base.astype(np.int32), lev.astype(np.int32), order.astype(np.int32))
 
 
 def band_levels(nx, ny, c1, c2):

@@ -7,6 +7,7 @@
 #include "flow_kernels.hip"
 #include "krylov_kernels.hip"
 #include "mg_kernels.hip"
+#include "tel_kernels.hip"
 #include "../../include/cms2d_amd.h"
 
 #include <hip/hip_ext.h>
@@ -82,7 +83,12 @@ static void tlaunch(KTimer *tm, int group, Kern kern, dim3 grid, dim3 blk, size_
     else hipLaunchKernelGGL(kern, grid, blk, lds, st, args...);
 }
 
+namespace cms {
+struct TelHost;
+}
+
 struct cms2d_flow {
+    cms::TelHost *tel = nullptr;   // telescoping grid (cell-list step), or null
     int dev = 0;
     hipStream_t st = nullptr;
     Geo g{};
@@ -800,10 +806,17 @@ static int upload_padded(cms2d_flow *F, double *dst, const double *src, bool gho
     return 0;
 }
 
+static void tel_destroy(cms2d_flow *F);
+static int tel_step(cms2d_flow *F, int nsteps, cms2d_step_info *info);
+static int tel_get(cms2d_flow *F, const char *name, double *out);
+static int tel_get_face(cms2d_flow *F, const char *name, double *out);
+static int tel_set(cms2d_flow *F, const char *name, const double *in);
+
 extern "C" int cms2d_flow_destroy(cms2d_flow *F) {
     if (!F) return CMS2D_OK;
     (void)hipSetDevice(F->dev);
     if (F->st) (void)hipStreamSynchronize(F->st);
+    tel_destroy(F);
     for (auto &ge : F->kgraph)
         if (ge) (void)hipGraphExecDestroy(ge);
     for (void *p : F->allocs) (void)hipFree(p);
@@ -840,6 +853,35 @@ static int init_state(cms2d_flow *F) {
     return 0;
 }
 
+// Multigrid hierarchy on an nx x ny grid (level 0 in the padded layout of the
+// flow arrays): 2x2 coarsening until the coarsest fits one workgroup.
+static int mg_alloc(cms2d_flow *F, int nx, int ny) {
+    Mg &M = F->mg;
+    M.nlev = 0;
+    for (;;) {
+        MgLevel &L = M.L[M.nlev];
+        L.nx = nx; L.ny = ny; L.nxp = nx + 2;
+        const size_t n = (size_t)(nx + 2) * (ny + 2);
+        float **arrs[] = {&L.aN, &L.aE, &L.aS, &L.aW, &L.dinv, &L.e};
+        for (float **a : arrs)
+            if (int rc = dalloc(F, a, n)) return rc;
+        if (int rc = dalloc(F, &L.z, n)) return rc;
+        if (M.nlev > 0) {
+            if (int rc = dalloc(F, &L.zo, n)) return rc;
+            if (int rc = dalloc(F, &L.b, n)) return rc;
+        } else {
+            if (int rc = dalloc(F, &L.ap, n)) return rc;
+        }
+        M.nlev++;
+        const bool fits = (long)nx * ny <= MG_COARSE_MAX && (long)(nx + 2) * (ny + 2) <= MG_COARSE_LDS;
+        if (fits && (M.nlev >= MG_MIN_LEVELS || (nx == 1 && ny == 1))) break;
+        if (M.nlev == MG_MAX_LEVELS) return set_err(CMS2D_ERR_ARG, "grid too large for the multigrid hierarchy");
+        nx = (nx + 1) / 2;
+        ny = (ny + 1) / 2;
+    }
+    return 0;
+}
+
 static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (d->nx < 3 || d->ny < 3 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann || d->ntc < 0 ||
         d->ntc > 8 || !(d->dt > 0) || !(std::fabs(d->latitude) <= 90.0) || d->coriolis_plane < 0 ||
@@ -853,9 +895,22 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
         curve = curve || d->side_type[k] == CMS2D_BND_WSE_CURVE;
     }
+    int nopen = 0;
+    for (int k = 0; k < 4; k++) {
+        if (d->side_type[k] == CMS2D_BND_WALL) continue;
+        nopen++;
+        // the set-up adjustment of a water-level side keeps the side in LDS
+        const long len = (k == CMS2D_SIDE_N || k == CMS2D_SIDE_S) ? d->nx : d->ny;
+        if (len * (long)sizeof(double) > 65536)
+            return set_err(CMS2D_ERR_ARG, "water-level side longer than 8192 cells");
+    }
+    if (nopen > 1)
+        return set_err(CMS2D_ERR_ARG, "one water-level side at most (the others are walls)");
     if (curve) {
         if (d->wse_ntimes < 1 || !d->wse_times || !d->wse_values || d->wse_order < 0 || d->wse_order > 3)
             return set_err(CMS2D_ERR_ARG, "water-level curve missing or invalid");
+        if (!std::isfinite(d->wse_gradx) || !std::isfinite(d->wse_grady))
+            return set_err(CMS2D_ERR_ARG, "water-level gradients must be finite");
         for (int i = 0; i < d->wse_ntimes; i++)
             if (!std::isfinite(d->wse_times[i]) || !std::isfinite(d->wse_values[i]) ||
                 (i > 0 && !(d->wse_times[i] > d->wse_times[i - 1])))
@@ -997,29 +1052,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     F->solver = d->solver;
     {
         Mg &M = F->mg;
-        int nx = g.nx, ny = g.ny;
-        M.nlev = 0;
-        for (;;) {
-            MgLevel &L = M.L[M.nlev];
-            L.nx = nx; L.ny = ny; L.nxp = nx + 2;
-            const size_t n = (size_t)(nx + 2) * (ny + 2);
-            float **arrs[] = {&L.aN, &L.aE, &L.aS, &L.aW, &L.dinv, &L.e};
-            for (float **a : arrs)
-                if (int rc = dalloc(F, a, n)) return rc;
-            if (int rc = dalloc(F, &L.z, n)) return rc;
-            if (M.nlev > 0) {
-                if (int rc = dalloc(F, &L.zo, n)) return rc;
-                if (int rc = dalloc(F, &L.b, n)) return rc;
-            } else {
-                if (int rc = dalloc(F, &L.ap, n)) return rc;
-            }
-            M.nlev++;
-            const bool fits = (long)nx * ny <= MG_COARSE_MAX && (long)(nx + 2) * (ny + 2) <= MG_COARSE_LDS;
-            if (fits && (M.nlev >= MG_MIN_LEVELS || (nx == 1 && ny == 1))) break;
-            if (M.nlev == MG_MAX_LEVELS) return set_err(CMS2D_ERR_ARG, "grid too large for the multigrid hierarchy");
-            nx = (nx + 1) / 2;
-            ny = (ny + 1) / 2;
-        }
+        if (int rc = mg_alloc(F, g.nx, g.ny)) return rc;
         if (int rc = dalloc(F, &F->Kpp.zp, N)) return rc;
         if (int rc = dalloc(F, &F->Kpp.zs, N)) return rc;
         KrySys<1, float> &Ki = F->Kin;
@@ -1089,6 +1122,8 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     return capture_graphs(F);
 }
 
+#include "tel.hip"
+
 extern "C" int cms2d_flow_create(const cms2d_flow_desc *d, cms2d_flow **out) {
     if (!d || !out) return set_err(CMS2D_ERR_ARG, "null argument");
     *out = nullptr;
@@ -1105,9 +1140,31 @@ extern "C" int cms2d_flow_create(const cms2d_flow_desc *d, cms2d_flow **out) {
     return CMS2D_OK;
 }
 
+extern "C" int cms2d_flow_create_tel(const cms2d_flow_desc *d, const cms2d_tel_grid *grid, cms2d_flow **out) {
+    if (!d || !grid || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (d->device < 0) return set_err(CMS2D_ERR_ARG, "invalid device");
+    cms2d_flow *F = new (std::nothrow) cms2d_flow();
+    if (!F) return set_err(CMS2D_ERR_ARG, "out of host memory");
+    int rc = tel_check(d, grid);
+    if (rc == 0 && (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= d->device))
+        rc = set_err(CMS2D_ERR_NODEVICE, "no HIP device");
+    if (rc == 0) rc = tel_create(d, grid, F);
+    if (rc != 0) {
+        std::string keep = g_err;
+        cms2d_flow_destroy(F);
+        g_err = keep;
+        return rc;
+    }
+    *out = F;
+    return CMS2D_OK;
+}
+
 extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {
     if (!F || nsteps < 0) return set_err(CMS2D_ERR_ARG, "invalid argument");
     HIPCHK(hipSetDevice(F->dev));
+    if (F->tel) return tel_step(F, nsteps, info);
     HIPCHK(hipEventRecord(F->step_ev[0], F->st));
     for (int n = 0; n < nsteps; n++) {
         // timing, CMS_Flow.f90:109-240; with dtvar the time step is halved
@@ -1292,6 +1349,7 @@ static size_t ck_size(const cms2d_flow *F) {
 
 extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     if (!F || !nbytes) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->tel) return set_err(CMS2D_ERR_ARG, "checkpoints are built for Cartesian grids only");
     const size_t need = ck_size(F);
     if (!buf) { *nbytes = need; return CMS2D_OK; }
     if (*nbytes < need) return set_err(CMS2D_ERR_ARG, "checkpoint buffer too small: " + std::to_string(need) + " bytes needed");
@@ -1325,6 +1383,7 @@ extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
 
 extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes) {
     if (!F || !buf) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->tel) return set_err(CMS2D_ERR_ARG, "checkpoints are built for Cartesian grids only");
     const size_t need = ck_size(F);
     CkHeader h{};
     if (nbytes < sizeof h) return set_err(CMS2D_ERR_ARG, "not a checkpoint");
@@ -1375,6 +1434,7 @@ extern "C" int cms2d_flow_tuning(cms2d_flow *F, int *mg_tile_x, int *graphs) {
 
 extern "C" int cms2d_flow_get(cms2d_flow *F, const char *name, double *out) {
     if (!F || !name || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->tel) return tel_get(F, name, out);
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipStreamSynchronize(F->st));
     const Geo &g = F->g;
@@ -1398,6 +1458,7 @@ extern "C" int cms2d_flow_set(cms2d_flow *F, const char *name, const double *in)
     if (!F || !name || !in) return set_err(CMS2D_ERR_ARG, "null argument");
     if (strcmp(name, "u") && strcmp(name, "v") && strcmp(name, "p"))
         return set_err(CMS2D_ERR_ARG, std::string("field not settable: ") + name);
+    if (F->tel) return tel_set(F, name, in);
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipStreamSynchronize(F->st));
     const Geo &g = F->g;
@@ -1413,6 +1474,7 @@ extern "C" int cms2d_flow_set(cms2d_flow *F, const char *name, const double *in)
 
 extern "C" int cms2d_flow_get_face(cms2d_flow *F, const char *name, double *out) {
     if (!F || !name || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->tel) return tel_get_face(F, name, out);
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipStreamSynchronize(F->st));
     const Geo &g = F->g;
@@ -1452,9 +1514,18 @@ extern "C" int cms2d_flow_kernel_stats(const cms2d_flow *F, const char *name, do
 
 // ----------------------------------------------------------------------------
 // cms2d_solve: solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
-extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
-                           const int *iwet, double *ss, const double *sp, double *res, double *phi, double relax,
-                           double rtol, double atol, int kmax, double *rmom, int *iters) {
+
+// Device buffers of one cms2d_solve call: the staged inputs/outputs and the
+// Krylov scratch (7 vectors of ncells, per-block partials, the scalars).
+struct SolveBufs {
+    double *a, *ss, *sp, *res, *phi, *vec, *partials;
+    int *c2c, *w;
+    KryScal *sc;
+};
+
+static int solve_check_args(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
+                            const int *iwet, const double *ss, const double *sp, const double *res,
+                            const double *phi, double relax) {
     if (ncells <= 0 || ncellsD < ncells || !acoef || !cell2cell || !iwet || !ss || !sp || !res || !phi ||
         !(relax > 0.0))
         return set_err(CMS2D_ERR_ARG, "invalid argument");
@@ -1463,6 +1534,55 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double 
     const long nf = nmaxfaces;
     for (long f = 0; f < nf * ncells; f++)
         if (cell2cell[f] < -1 || cell2cell[f] >= ncellsD) return set_err(CMS2D_ERR_ARG, "cell2cell out of range");
+    return 0;
+}
+
+// Uploads the system, runs the prologue and BiCGStab, downloads ss, res, phi.
+// The scalars are cleared by a kernel (no memset node), so the same sequence
+// serves both entry points.
+static int solve_run(const SolveBufs &B, hipStream_t st, int ncells, int ncellsD, int nmaxfaces,
+                     const double *acoef, const int *cell2cell, const int *iwet, double *ss, const double *sp,
+                     double *res, double *phi, double relax, double rtol, double atol, int kmax, double *rmom,
+                     int *iters) {
+    const size_t nD = ncellsD, n = ncells, nf = nmaxfaces;
+    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
+    HIPCHK(hipMemcpyAsync(B.a, acoef, nf * nD * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.c2c, cell2cell, nf * nD * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.w, iwet, nD * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(B.phi, phi, nD * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (unsigned int *)B.sc,
+                       (int)(sizeof(KryScal) / sizeof(unsigned int)));
+    KryScal h_sc{};
+    KrySys<1> K{};
+    K.x[0] = B.phi;
+    K.r[0] = B.vec; K.r0[0] = B.vec + n; K.p[0] = B.vec + 2 * n; K.v[0] = B.vec + 3 * n; K.t[0] = B.vec + 4 * n;
+    K.ap_w[0] = B.vec + 5 * n;
+    K.dinv_w[0] = B.vec + 6 * n;
+    K.sc = B.sc;
+    K.red.pr[0] = B.partials; K.red.pv[0] = B.partials + 2 * nblk; K.red.pt[0] = B.partials + 3 * nblk;
+    EllOp A{ncells, nmaxfaces, B.c2c, B.a};
+    hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(KBLOCK), 0, st, A, (const int *)B.w, B.ss,
+                       (const double *)B.sp, B.res, (const double *)B.phi, relax, K);
+    int prev = 8;
+    KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,
+             B.sc, &h_sc, &prev, nullptr};
+    if (int rc = run_bicg(X, K, A, 0)) return rc;
+    HIPCHK(hipMemcpyAsync(ss, B.ss, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(res, B.res, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(phi, B.phi, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (rmom) *rmom = h_sc.rmom;
+    if (iters) *iters = h_sc.final_iters;
+    return 0;
+}
+
+extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
+                           const int *iwet, double *ss, const double *sp, double *res, double *phi, double relax,
+                           double rtol, double atol, int kmax, double *rmom, int *iters) {
+    if (int rc = solve_check_args(ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi, relax))
+        return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
     std::vector<void *> mem;
@@ -1473,55 +1593,28 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double 
         mem.push_back(p);
         return p;
     };
-    const size_t nD = ncellsD, n = ncells;
-    int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
-    double *d_a = (double *)alloc(nf * nD * sizeof(double));
-    int *d_c2c = (int *)alloc(nf * nD * sizeof(int));
-    int *d_w = (int *)alloc(nD * sizeof(int));
-    double *d_ss = (double *)alloc(nD * sizeof(double)), *d_sp = (double *)alloc(nD * sizeof(double));
-    double *d_res = (double *)alloc(nD * sizeof(double)), *d_phi = (double *)alloc(nD * sizeof(double));
-    double *vec = (double *)alloc(7 * n * sizeof(double));
-    double *partials = (double *)alloc((size_t)nblk * 5 * sizeof(double));
-    KryScal *d_sc = (KryScal *)alloc(sizeof(KryScal));
-    KryScal h_sc{};
-    if (!d_a || !d_c2c || !d_w || !d_ss || !d_sp || !d_res || !d_phi || !vec || !partials || !d_sc) {
+    const size_t nD = ncellsD, n = ncells, nf = nmaxfaces;
+    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
+    SolveBufs B{};
+    B.a = (double *)alloc(nf * nD * sizeof(double));
+    B.c2c = (int *)alloc(nf * nD * sizeof(int));
+    B.w = (int *)alloc(nD * sizeof(int));
+    B.ss = (double *)alloc(nD * sizeof(double));
+    B.sp = (double *)alloc(nD * sizeof(double));
+    B.res = (double *)alloc(nD * sizeof(double));
+    B.phi = (double *)alloc(nD * sizeof(double));
+    B.vec = (double *)alloc(7 * n * sizeof(double));
+    B.partials = (double *)alloc((size_t)nblk * 5 * sizeof(double));
+    B.sc = (KryScal *)alloc(sizeof(KryScal));
+    if (!B.a || !B.c2c || !B.w || !B.ss || !B.sp || !B.res || !B.phi || !B.vec || !B.partials || !B.sc) {
         cleanup();
         return set_err(CMS2D_ERR_HIP, "device allocation failed");
     }
     hipStream_t st = nullptr;
     int rc = 0;
-    do {
-        if (hipStreamCreate(&st) != hipSuccess) { rc = set_err(CMS2D_ERR_HIP, "stream"); break; }
-        bool ok = hipMemcpyAsync(d_a, acoef, nf * nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_c2c, cell2cell, nf * nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_w, iwet, nD * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemcpyAsync(d_phi, phi, nD * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
-                  hipMemsetAsync(d_sc, 0, sizeof(KryScal), st) == hipSuccess;
-        if (!ok) { rc = set_err(CMS2D_ERR_HIP, "upload failed"); break; }
-        KrySys<1> K{};
-        K.x[0] = d_phi;
-        K.r[0] = vec; K.r0[0] = vec + n; K.p[0] = vec + 2 * n; K.v[0] = vec + 3 * n; K.t[0] = vec + 4 * n;
-        K.ap_w[0] = vec + 5 * n;
-        K.dinv_w[0] = vec + 6 * n;
-        K.sc = d_sc;
-        K.red.pr[0] = partials; K.red.pv[0] = partials + 2 * nblk; K.red.pt[0] = partials + 3 * nblk;
-        EllOp A{ncells, nmaxfaces, d_c2c, d_a};
-        hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(KBLOCK), 0, st, A, (const int *)d_w, d_ss,
-                           (const double *)d_sp, d_res, (const double *)d_phi, relax, K);
-        int prev = 8;
-        KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,
-                 d_sc, &h_sc, &prev, nullptr};
-        if ((rc = run_bicg(X, K, A, 0))) break;
-        ok = hipMemcpyAsync(ss, d_ss, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
-             hipMemcpyAsync(res, d_res, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
-             hipMemcpyAsync(phi, d_phi, n * sizeof(double), hipMemcpyDeviceToHost, st) == hipSuccess &&
-             hipStreamSynchronize(st) == hipSuccess;
-        if (!ok) { rc = set_err(CMS2D_ERR_HIP, "download failed"); break; }
-        if (rmom) *rmom = h_sc.rmom;
-        if (iters) *iters = h_sc.final_iters;
-    } while (0);
+    if (hipStreamCreate(&st) != hipSuccess) rc = set_err(CMS2D_ERR_HIP, "stream");
+    else rc = solve_run(B, st, ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi, relax, rtol,
+                        atol, kmax, rmom, iters);
     if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     cleanup();
     return rc;

@@ -267,6 +267,11 @@ __global__ void __launch_bounds__(KBLOCK) k_bicg_r(Op A, KrySys<NS, T> K, int it
 
 // solve() prologue on the reference's general cell list (solvers.f90:232-240)
 // and the BiCGStab start, for the drop-in cms2d_solve.
+// Clears n 32-bit words with vector stores (used instead of a memset node).
+__global__ void k_zero_words(unsigned int *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 __global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwet, double *ss, const double *sp,
                                                         double *res, const double *phi, double relax,
                                                         KrySys<1> K) {

@@ -1,0 +1,816 @@
+// tel.hip -- host driver of the implicit step on telescoping grids, and the
+// pressure-correction preconditioner that couples the cell list to the
+// multigrid hierarchy of its base grid.  Included by cms2d.hip after the
+// Cartesian driver, whose Krylov, multigrid and control routines it reuses.
+//
+// Preconditioner (z = M^-1 r, FP32): the cells of the refined base cells are
+// smoothed by damped Jacobi, the residual is summed into the base cells (the
+// equations are cell integrals), one V-cycle of the base grid's multigrid
+// solves for the base-cell correction, which is added to every cell of the
+// base cell, and the refined cells are smoothed once more.  The base-grid
+// operator is re-discretised from the cell list: the coefficient across a
+// base-cell face is the sum of the cell-face coefficients it covers, each
+// scaled by its centre distance over the base spacing (a coefficient ~ ds/dn,
+// so the sum keeps the base face's ds/dn), and the row excess ap - sum(a)
+// (storage term, folded boundary coefficients) is summed over the cells.  On
+// unrefined base cells the base operator is the cell operator itself.
+
+namespace cms {
+
+constexpr float TEL_OMEGA = 0.8f;     // damping of the Jacobi smoothing
+
+// Base-grid level 0 of the multigrid hierarchy from the cell-list pp system.
+__global__ void k_tl_mg_setup0(TelG T, const double *apc, const double *ap, const int *bcells, MgLevel L) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= T.nbx * T.nby) return;
+    const int bi = b % T.nbx, bj = b / T.nbx;
+    double a4[4] = {0.0, 0.0, 0.0, 0.0}, e = 0.0;
+    for (int q = T.bstart[b]; q < T.bstart[b + 1]; q++) {
+        const int i = bcells[q];
+        const uint64_t m = T.meta[i];
+        double sa = 0.0;
+        for (int k = 0; k < m_ncf(m); k++) {
+            const int nb = tnb(T, i, k);
+            if (nb >= T.n) continue;
+            const double a = apc[k * T.nd + i];
+            sa += a;
+            if (T.base[nb] == b) continue;
+            const int d = m_dir(m, k);
+            const double dn = (d == KE || d == KW) ? fabs(T.x[nb] - T.x[i]) : fabs(T.y[nb] - T.y[i]);
+            a4[d] += a * dn / ((d == KE || d == KW) ? T.dxb : T.dyb);
+        }
+        e += fmax(ap[i] - sa, 0.0);
+    }
+    const int c = L.idx(bi, bj);
+    const double apb = e + ((a4[KN] + a4[KE]) + (a4[KS] + a4[KW]));
+    L.aN[c] = (float)a4[KN];
+    L.aE[c] = (float)a4[KE];
+    L.aS[c] = (float)a4[KS];
+    L.aW[c] = (float)a4[KW];
+    L.e[c] = (float)e;
+    L.ap[c] = (float)apb;
+    L.dinv[c] = apb > 0.0 ? (float)(1.0 / apb) : 0.0f;
+}
+
+// z1 = omega D^-1 r on the cells of refined base cells, 0 elsewhere
+__global__ void k_tl_mg_pre(TelG T, const double *r, const double *ap, float *z1, const KryScal *sc) {
+    if (sc->done) return;
+    int i;
+    if (!tel_thread(T.n, i)) return;
+    z1[i] = m_smooth(T.meta[i]) ? TEL_OMEGA * (float)(r[i] / ap[i]) : 0.0f;
+}
+// b of the base cell: the summed residual r - A z1 of its cells
+__global__ void k_tl_mg_restrict(TelG T, const double *r, const double *ap, const double *apc, const float *z1,
+                                 const int *bcells, MgLevel L, float *b0, const KryScal *sc) {
+    if (sc->done) return;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= T.nbx * T.nby) return;
+    double acc = 0.0;
+    for (int q = T.bstart[b]; q < T.bstart[b + 1]; q++) {
+        const int i = bcells[q];
+        const uint64_t m = T.meta[i];
+        double ri = r[i];
+        if (m_near(m)) {
+            double az = ap[i] * (double)z1[i];
+            for (int k = 0; k < m_ncf(m); k++) {
+                const int nb = tnb(T, i, k);
+                if (nb < T.n) az -= apc[k * T.nd + i] * (double)z1[nb];
+            }
+            ri -= az;
+        }
+        acc += ri;
+    }
+    b0[L.idx(b % T.nbx, b / T.nbx)] = (float)acc;
+}
+// z = z1 + P zb, then one damped Jacobi sweep on the refined cells
+__global__ void k_tl_mg_post(TelG T, const double *r, const double *ap, const double *apc, const float *z1,
+                             const float *zb, MgLevel L, float *z, const KryScal *sc) {
+    if (sc->done) return;
+    int i;
+    if (!tel_thread(T.n, i)) return;
+    auto z2 = [&](int c) {
+        const int b = T.base[c];
+        return z1[c] + zb[L.idx(b % T.nbx, b / T.nbx)];
+    };
+    const uint64_t m = T.meta[i];
+    const float zi = z2(i);
+    if (!m_smooth(m)) { z[i] = zi; return; }
+    double az = ap[i] * (double)zi;
+    for (int k = 0; k < m_ncf(m); k++) {
+        const int nb = tnb(T, i, k);
+        if (nb < T.n) az -= apc[k * T.nd + i] * (double)z2(nb);
+    }
+    z[i] = zi + TEL_OMEGA * (float)((r[i] - az) / ap[i]);
+}
+
+// Host-side state of a telescoping flow.
+struct TelHost {
+    TelG T{};
+    TelS s{};
+    double *gu[2][2], *gv[2][2], *gpp[2][2], *gHu[2][2], *gHv[2][2], *gap[2][2], *gzb[2][2];
+    int cu = 0, cv = 0, cpp = 0, cHu = 0, cHv = 0, cap = 0;   // current buffer of each pair
+    KrySys<2> Kuv{};
+    KrySys<1> Kpp{};
+    float *z1 = nullptr, *b0 = nullptr;   // smoothing output, base right-hand side
+    int *bcells = nullptr;
+    int nblk_n = 0, nblk_nd = 0, nblk_base = 0;
+    std::vector<int> gown_h;              // owner of each ghost (host copy)
+    int njoint = 0;
+};
+
+}  // namespace cms
+
+// GradIO of a ping-pong pair: read buffer cur, write the other; the caller flips cur.
+static GradIO tel_io(double *(&g)[2][2], int cur) { return GradIO{g[0][cur], g[1][cur], g[0][cur ^ 1], g[1][cur ^ 1]}; }
+
+#define TLAUNCH(kern, nb, ...) hipLaunchKernelGGL(kern, dim3(nb), dim3(BLOCK), 0, F->st, __VA_ARGS__)
+
+static int tel_sync_flags(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    HIPCHK(hipMemcpyAsync(F->h_flags, H.s.flags, 4 * sizeof(int), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+// flow_wetdry(0) (flow.F90:1231-1539)
+static int tel_wetdry(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    const TelG &T = H.T;
+    TelS &s = H.s;
+    TLAUNCH(k_tl_wetdry_h, H.nblk_nd, T, s, F->P);
+    for (int it = 0;; it++) {
+        TLAUNCH(k_set_int, 1, s.flags, 3, 0);
+        TLAUNCH(k_tl_narrow, H.nblk_n, T, s, F->P);
+        if (int rc = tel_sync_flags(F)) return rc;
+        if (F->h_flags[0] == 0) break;
+        if (it > 100000) return set_err(CMS2D_ERR_HIP, "narrow-channel pass did not settle");
+    }
+    TLAUNCH(k_tl_pond_init, H.nblk_nd, T, s);
+    TLAUNCH(k_tl_pond_union, H.nblk_n, T, s);
+    TLAUNCH(k_tl_pond_compress, H.nblk_nd, T, s);
+    TLAUNCH(k_tl_pond_mark, (T.nstr + BLOCK - 1) / BLOCK, T, s);
+    TLAUNCH(k_tl_pond_final, H.nblk_nd, T, s);
+    TLAUNCH(k_set_int, 1, s.flags, 3, 0);
+    TLAUNCH(k_tl_wetdry_post, H.nblk_n, T, s);
+    if (int rc = tel_sync_flags(F)) return rc;
+    if (F->h_flags[2] > 0)
+        for (int sweep = 0; sweep < 5; sweep++) {
+            TLAUNCH(k_tl_d2w_sweep, H.nblk_n, T, s);
+            TLAUNCH(k_tl_d2w_commit, H.nblk_n, T, s);
+        }
+    return 0;
+}
+
+// gradients of u and v (der_gradvec_eval), the joint cells' previous values kept
+static void tel_grad_uv(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    TLAUNCH(k_tl_grad<2>, H.nblk_n, H.T, (const uint8_t *)H.s.iwet, 1, (const double *)H.s.u,
+            (const double *)H.s.v, (const double *)nullptr, tel_io(H.gu, H.cu), tel_io(H.gv, H.cv), GradIO{});
+    H.cu ^= 1;
+    H.cv ^= 1;
+}
+
+// flow_grad_interp (flow_imp.F90:1075-1130), fric_eval and flow_eddyvis
+static int tel_grad_fric_vis(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    const TelG &T = H.T;
+    TelS &s = H.s;
+    tel_grad_uv(F);
+    TLAUNCH(k_tl_pk, H.nblk_n, T, s);
+    TLAUNCH(k_tl_dphk, H.nblk_n, T, s, F->P);
+    TLAUNCH(k_tl_fric_vis, H.nblk_n, T, s, F->P, (const double *)H.gu[0][H.cu], (const double *)H.gu[1][H.cu],
+            (const double *)H.gv[0][H.cv], (const double *)H.gv[1][H.cv]);
+    if (T.ng) TLAUNCH(k_tl_vis_ghosts, (T.ng + BLOCK - 1) / BLOCK, T, s);
+    TLAUNCH(k_tl_vis_fix, H.nblk_nd, T, s);
+    TLAUNCH(k_tl_vis_corner, H.nblk_n, T, s);
+    TLAUNCH(k_tl_visk, H.nblk_n, T, s);
+    TLAUNCH(k_tl_vis_commit, H.nblk_n, T, s);
+    return 0;
+}
+
+// z = M^-1 r of the pp system (r: FP64 Krylov vector)
+static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
+    TelHost &H = *F->tel;
+    const TelG &T = H.T;
+    const KryScal *sc = H.Kpp.sc;
+    const MgLevel &L0 = F->mg.L[0];
+    TLAUNCH(k_tl_mg_pre, H.nblk_n, T, r, (const double *)H.Kpp.ap_w[0], H.z1, sc);
+    TLAUNCH(k_tl_mg_restrict, H.nblk_base, T, r, (const double *)H.Kpp.ap_w[0], (const double *)H.s.apc,
+            (const float *)H.z1, (const int *)H.bcells, L0, H.b0, sc);
+    float *zb = L0.zo;
+    mg_vcycle<float>(F, H.b0, zb, sc);
+    TLAUNCH(k_tl_mg_post, H.nblk_n, T, r, (const double *)H.Kpp.ap_w[0], (const double *)H.s.apc,
+            (const float *)H.z1, (const float *)zb, L0, z, sc);
+}
+struct TelPre {
+    cms2d_flow *F;
+    void operator()(int which) const {
+        TelHost &H = *F->tel;
+        if (which == 0) tel_mg_apply(F, H.Kpp.p[0], H.Kpp.zp);
+        else tel_mg_apply(F, H.Kpp.r[0], H.Kpp.zs);
+    }
+};
+
+static TelOp tel_op(const TelHost &H, const double *a) { return TelOp{H.T.n, H.T.nd, H.T.c2c, H.T.meta, a}; }
+
+static int tel_flow_imp(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    const TelG &T = H.T;
+    TelS &s = H.s;
+    const double timehrs = F->timesecs / 3600.0;
+    double w = 0.0;
+    for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
+    if (!F->wse_t.empty()) w = wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc);
+    if (int rc = tel_wetdry(F)) return rc;
+    F->P.tauwx = F->P.tauwy = 0.0;
+    if (F->windconst) {
+        const double ws = std::sqrt(F->wndx * F->wndx + F->wndy * F->wndy);
+        const double val = F->ramp * wind_drag_hsu(ws) * F4(0.00120) * ws;
+        F->P.tauwx = val * F->wndx;
+        F->P.tauwy = val * F->wndy;
+    }
+    TLAUNCH(k_tl_srcsnk, H.nblk_n, T, s, F->P);
+    const TelOp Am = tel_op(H, s.am), Ap = tel_op(H, s.apc);
+    for (int n = 0; n < 3; n++) F->kry[n] = 0;
+    for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
+        TLAUNCH(k_tl_mom, H.nblk_n, T, s, F->P);
+        if (H.njoint && F->P.ndsch != 0)
+            TLAUNCH(k_tl_defcor_skew, (H.njoint + BLOCK - 1) / BLOCK, T, s, (const double *)H.gu[0][H.cu],
+                    (const double *)H.gu[1][H.cu], (const double *)H.gv[0][H.cv], (const double *)H.gv[1][H.cv]);
+        TLAUNCH(k_tl_bound_uv, H.nblk_n, T, s, F->P);
+        hipLaunchKernelGGL(k_tl_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, T, s, H.Kuv, Am);
+        {
+            KryCtx X = flow_ctx(F, false);
+            if (int rc = run_bicg(X, H.Kuv, Am, 1)) return rc;
+        }
+        TLAUNCH(k_tl_pp_prep, H.nblk_n, T, s);
+        if (T.skewcor) {
+            TLAUNCH(k_tl_grad<3>, H.nblk_n, T, (const uint8_t *)s.iwet, 1, (const double *)s.Hu, (const double *)s.Hv,
+                    (const double *)s.apuareap, tel_io(H.gHu, H.cHu), tel_io(H.gHv, H.cHv), tel_io(H.gap, H.cap));
+            H.cHu ^= 1;
+            H.cHv ^= 1;
+            H.cap ^= 1;
+        }
+        TLAUNCH(k_tl_pp_coef_flux, H.nblk_n, T, s, (const double *)H.gHu[1][H.cHu], (const double *)H.gHv[0][H.cHv],
+                (const double *)H.gap[0][H.cap], (const double *)H.gap[1][H.cap]);
+        if (T.nstr > 0)
+            hipLaunchKernelGGL(k_tl_wse_adjust, dim3(1), dim3(1024), T.nstr * sizeof(double), F->st, T, s, F->P, w,
+                               F->ramp, (int)!(F->wse_adjust_off && !F->wse_t.empty()));
+        hipLaunchKernelGGL(k_tl_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, T, s, F->P, H.Kpp, Ap,
+                           (const double *)H.gpp[0][H.cpp], (const double *)H.gpp[1][H.cpp]);
+        {
+            KryCtx X = flow_ctx(F, true);
+            if (F->precond == CMS2D_PRECOND_MG) {
+                TLAUNCH(k_tl_mg_setup0, H.nblk_base, T, (const double *)s.apc, (const double *)H.Kpp.ap_w[0],
+                        (const int *)H.bcells, F->mg.L[0]);
+                for (int l = 0; l + 1 < F->mg.nlev; l++)
+                    LAUNCH(k_mg_coarsen, mg_nblk(F->mg.L[l + 1]), F->mg.L[l], F->mg.L[l + 1]);
+                F->tm_active = X.tm;
+                int rc = run_bicg<TelOp, 1, 1, double>(X, H.Kpp, Ap, 0, TelPre{F});
+                F->tm_active = nullptr;
+                if (rc) return rc;
+            } else {
+                if (int rc = run_bicg(X, H.Kpp, Ap, 0)) return rc;
+            }
+        }
+        if (T.ng) TLAUNCH(k_tl_ghost_pp, (T.ng + BLOCK - 1) / BLOCK, T, s);
+        TLAUNCH(k_tl_grad<1>, H.nblk_n, T, (const uint8_t *)s.iwet, 1, (const double *)s.pp, (const double *)nullptr,
+                (const double *)nullptr, tel_io(H.gpp, H.cpp), GradIO{}, GradIO{});
+        H.cpp ^= 1;
+        TLAUNCH(k_tl_pvfc1, H.nblk_n, T, s, (const double *)H.gpp[0][H.cpp], (const double *)H.gpp[1][H.cpp]);
+        TLAUNCH(k_tl_pvfc2, H.nblk_nd, T, s, F->P);
+        if (T.ng) TLAUNCH(k_tl_ghost_p, (T.ng + BLOCK - 1) / BLOCK, T, s);
+        if (int rc = tel_grad_fric_vis(F)) return rc;
+        for (int n = 0; n < 3; n++) {
+            F->rmom[n] = F->h_sc[n].rmom;
+            F->kry[n] += F->h_sc[n].final_iters;
+        }
+        check_conv(F);
+        if (F->iconv == 0) return set_err(CMS2D_ERR_DIVERGED, "Model is divergent (check_conv iconv=0)");
+        if (F->iconv == 2) break;
+    }
+    if (F->niter > F->maxit) F->niter = F->maxit;
+    TLAUNCH(k_tl_velface, H.nblk_n, T, s, F->P);
+    return 0;
+}
+
+// Initial state (flow.F90:688-707, prestart.f90:309-329)
+static int tel_init_state(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    TelS &s = H.s;
+    const size_t nd = H.T.nd;
+    if (int rc = tel_wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.u1, s.u, nd * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.v1, s.v, nd * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.p1, s.p, nd * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    HIPCHK(hipMemcpyAsync(s.h1, s.h, nd * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    if (int rc = tel_wetdry(F)) return rc;
+    HIPCHK(hipMemcpyAsync(s.iwet1, s.iwet, nd, hipMemcpyDeviceToDevice, F->st));
+    tel_grad_uv(F);
+    TLAUNCH(k_tl_pk, H.nblk_n, H.T, s);
+    TLAUNCH(k_tl_dphk, H.nblk_n, H.T, s, F->P);
+    TLAUNCH(k_tl_convflux, H.nblk_n, H.T, s);
+    HIPCHK(hipMemcpyAsync(s.flux1, s.flux, TF * nd * sizeof(double), hipMemcpyDeviceToDevice, F->st));
+    // fric_eval and flow_eddyvis without a new gradient
+    TLAUNCH(k_tl_fric_vis, H.nblk_n, H.T, s, F->P, (const double *)H.gu[0][H.cu], (const double *)H.gu[1][H.cu],
+            (const double *)H.gv[0][H.cv], (const double *)H.gv[1][H.cv]);
+    if (H.T.ng) TLAUNCH(k_tl_vis_ghosts, (H.T.ng + BLOCK - 1) / BLOCK, H.T, s);
+    TLAUNCH(k_tl_vis_fix, H.nblk_nd, H.T, s);
+    TLAUNCH(k_tl_vis_corner, H.nblk_n, H.T, s);
+    TLAUNCH(k_tl_visk, H.nblk_n, H.T, s);
+    TLAUNCH(k_tl_vis_commit, H.nblk_n, H.T, s);
+    HIPCHK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+// The topology of read_tel (geospatial.F90:403-795) and the connectivity part
+// of geo_init_cart (:1446-1552) from the .tel arrays, on the host: faces in the
+// order of the file's neighbour slots, one ghost per side without a
+// neighbour (placed one cell size beyond it, the cell's bed), llec2llec, the
+// no-repeat owner of each face, joint cells.
+struct TelTopo {
+    int n = 0, nd = 0;
+    std::vector<int> ncf, c2c, dir, llec, own, joint;
+    std::vector<double> x, y, dx, dy;
+    std::vector<int> gown, gk;
+};
+static int tel_topology(const cms2d_tel_grid *tg, TelTopo &t) {
+    const int n = tg->ncells;
+    t.n = n;
+    int nd = n;
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 8; k += 2) {
+            const int a = tg->nb[8 * i + k], b = tg->nb[8 * i + k + 1];
+            if (a < 0 || a > n || b < 0 || b > n) return set_err(CMS2D_ERR_ARG, "neighbour slot out of range");
+            if (a == 0 && b == 0) nd++;
+        }
+    t.nd = nd;
+    t.ncf.assign(nd, 0);
+    t.c2c.assign((size_t)TF * nd, -1);
+    t.dir.assign((size_t)TF * nd, -1);
+    t.llec.assign((size_t)TF * nd, -1);
+    t.own.assign((size_t)TF * nd, 0);
+    t.x.assign(nd, 0.0); t.y.assign(nd, 0.0); t.dx.assign(nd, 0.0); t.dy.assign(nd, 0.0);
+    for (int i = 0; i < n; i++) {
+        t.x[i] = tg->geom[4 * i]; t.y[i] = tg->geom[4 * i + 1];
+        t.dx[i] = tg->geom[4 * i + 2]; t.dy[i] = tg->geom[4 * i + 3];
+        if (!(t.dx[i] > 0) || !(t.dy[i] > 0)) return set_err(CMS2D_ERR_ARG, "cell size must be positive");
+    }
+    static const int kopp[4] = {KS, KW, KN, KE};
+    int g = n;
+    for (int i = 0; i < n; i++) {
+        int k = 0;
+        for (int j = 0; j < 8; j++) {
+            const int d = j / 2, nck = tg->nb[8 * i + j];
+            const bool open = (j % 2 == 0) && nck == 0 && tg->nb[8 * i + j + 1] == 0;
+            if (!open && nck == 0) continue;
+            if (k == TF) return set_err(CMS2D_ERR_ARG, "a cell with more than six faces");
+            if (open) {
+                t.c2c[(size_t)k * nd + i] = g;
+                t.ncf[g] = 4;
+                t.c2c[(size_t)kopp[d] * nd + g] = i;
+                t.x[g] = t.x[i] + (d == KE ? t.dx[i] : d == KW ? -t.dx[i] : 0.0);
+                t.y[g] = t.y[i] + (d == KN ? t.dy[i] : d == KS ? -t.dy[i] : 0.0);
+                t.dx[g] = t.dx[i]; t.dy[g] = t.dy[i];
+                t.gown.push_back(i);
+                t.gk.push_back(k);
+                g++;
+            } else {
+                t.c2c[(size_t)k * nd + i] = nck - 1;
+            }
+            t.dir[(size_t)k * nd + i] = d;
+            k++;
+        }
+        t.ncf[i] = k;
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < t.ncf[i]; k++) {
+            const int nck = t.c2c[(size_t)k * nd + i];
+            int l = -1;
+            if (nck < n) {
+                for (int kk = 0; kk < t.ncf[nck]; kk++)
+                    if (t.c2c[(size_t)kk * nd + nck] == i) l = kk;
+            } else {
+                l = kopp[t.dir[(size_t)k * nd + i]];
+            }
+            if (l < 0) return set_err(CMS2D_ERR_ARG, "neighbour slots are not reciprocal");
+            t.llec[(size_t)k * nd + i] = l;
+        }
+    std::vector<char> seen((size_t)TF * nd, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < t.ncf[i]; k++) {
+            const size_t f = (size_t)k * nd + i;
+            if (seen[f]) continue;
+            const int nck = t.c2c[f];
+            seen[f] = 1;
+            seen[(size_t)t.llec[f] * nd + nck] = 1;
+            t.own[f] = 1;
+        }
+    t.joint.assign(n, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < t.ncf[i]; k++) {
+            const size_t f = (size_t)k * nd + i;
+            const int nck = t.c2c[f];
+            if (nck >= n) continue;
+            const int d = t.dir[f];
+            const double r = (d == KE || d == KW) ? t.dy[i] / t.dy[nck] : t.dx[i] / t.dx[nck];
+            if (r > F4(1.2) || r < F4(0.8)) { t.joint[i] = 1; t.joint[nck] = 1; }
+        }
+    return 0;
+}
+
+// Device copy of n elements (an empty array still gets one element).
+template <typename T>
+static int tel_upload(cms2d_flow *F, T **dst, const T *src, size_t n) {
+    if (int rc = dalloc(F, dst, std::max<size_t>(n, 1))) return rc;
+    if (n) HIPCHK(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, F->st));
+    return 0;
+}
+
+static void tel_destroy(cms2d_flow *F) {
+    delete F->tel;
+    F->tel = nullptr;
+}
+
+// Checks of the description that need no device.
+static int tel_check(const cms2d_flow_desc *d, const cms2d_tel_grid *tg) {
+    if (!tg->geom || !tg->nb || tg->ncells < 1 || (tg->nbnd > 0 && !tg->bnd_cells))
+        return set_err(CMS2D_ERR_ARG, "invalid telescoping grid");
+    if (d->nx < 1 || d->ny < 1 || !(d->dx > 0) || !(d->dy > 0) || !d->zb || !d->mann)
+        return set_err(CMS2D_ERR_ARG, "invalid flow description");
+    if (d->ndsch >= 5 && d->ndsch <= 9)
+        return set_err(CMS2D_ERR_ARG, "the NVD schemes are built for Cartesian grids only");
+    if (d->precision != CMS2D_PREC_FP64 || d->solver != CMS2D_SOLVER_BICGSTAB || d->dtvar)
+        return set_err(CMS2D_ERR_ARG, "telescoping grids run the FP64 BiCGStab form with a fixed time step");
+    if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
+        return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
+    int nopen = 0, curve = 0;
+    for (int k = 0; k < 4; k++) {
+        if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC &&
+            d->side_type[k] != CMS2D_BND_WSE_CURVE)
+            return set_err(CMS2D_ERR_ARG, "unknown boundary type");
+        nopen += d->side_type[k] != CMS2D_BND_WALL;
+        curve += d->side_type[k] == CMS2D_BND_WSE_CURVE;
+    }
+    if (nopen > 1) return set_err(CMS2D_ERR_ARG, "one water-level side at most (the others are walls)");
+    if ((nopen == 1) != (tg->nbnd > 0)) return set_err(CMS2D_ERR_ARG, "the water-level side needs its cellstring");
+    if (tg->nbnd == 1 || tg->nbnd > 8192) return set_err(CMS2D_ERR_ARG, "the cellstring needs 2..8192 cells");
+    if (curve) {
+        if (d->wse_ntimes < 1 || !d->wse_times || !d->wse_values || d->wse_order < 0 || d->wse_order > 3)
+            return set_err(CMS2D_ERR_ARG, "water-level curve missing or invalid");
+        for (int i = 0; i < d->wse_ntimes; i++)
+            if (!std::isfinite(d->wse_times[i]) || !std::isfinite(d->wse_values[i]) ||
+                (i > 0 && !(d->wse_times[i] > d->wse_times[i - 1])))
+                return set_err(CMS2D_ERR_ARG, "water-level curve times must increase, values be finite");
+        if (d->wse_gradx != 0.0 || d->wse_grady != 0.0)
+            return set_err(CMS2D_ERR_ARG, "water-level gradients are built for Cartesian grids only");
+    }
+    TelTopo t;
+    return tel_topology(tg, t);
+}
+
+static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_flow *F) {
+    int curve = 0;
+    for (int k = 0; k < 4; k++) curve += d->side_type[k] == CMS2D_BND_WSE_CURVE;
+    TelTopo t;
+    if (int rc = tel_topology(tg, t)) return rc;
+    const int n = t.n, nd = t.nd, ng = nd - n;
+    // the water-level string: cellstr_bnd (bnd_lib.f90:215-316)
+    std::vector<int> str_c, str_k;
+    for (int j = 0; j < tg->nbnd; j++) {
+        const int id = tg->bnd_cells[j];
+        if (id < 0 || id >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
+        const int ref = j == 0 ? tg->bnd_cells[1] : j == tg->nbnd - 1 ? tg->bnd_cells[tg->nbnd - 2] : id;
+        if (ref < 0 || ref >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
+        for (int k = 0; k < t.ncf[ref]; k++) {
+            if (t.c2c[(size_t)k * nd + ref] < n) continue;
+            int kf = k;
+            if (ref != id) {
+                kf = -1;
+                for (int kk = 0; kk < t.ncf[id]; kk++)
+                    if (t.dir[(size_t)kk * nd + id] == t.dir[(size_t)k * nd + ref]) { kf = kk; break; }
+            }
+            if (kf < 0 || t.c2c[(size_t)kf * nd + id] < n)
+                return set_err(CMS2D_ERR_ARG, "cellstring cell without an edge face in the string's direction");
+            str_c.push_back(id);
+            str_k.push_back(kf);
+        }
+    }
+    if (str_c.size() > 8192) return set_err(CMS2D_ERR_ARG, "the cellstring has more than 8192 faces");
+    std::vector<uint8_t> gwall(ng, 1), strmask(n, 0);
+    for (size_t j = 0; j < str_c.size(); j++) {
+        const int gc = t.c2c[(size_t)str_k[j] * nd + str_c[j]];
+        if (!gwall[gc - n]) return set_err(CMS2D_ERR_ARG, "a cellstring face listed twice");
+        gwall[gc - n] = 0;
+        strmask[str_c[j]] |= (uint8_t)(1u << str_k[j]);
+    }
+    // base cells
+    std::vector<int> base(n), cnt((size_t)d->nx * d->ny + 1, 0);
+    for (int i = 0; i < n; i++) {
+        const int bi = (int)std::floor(t.x[i] / d->dx), bj = (int)std::floor(t.y[i] / d->dy);
+        if (bi < 0 || bi >= d->nx || bj < 0 || bj >= d->ny) return set_err(CMS2D_ERR_ARG, "cell outside the base grid");
+        base[i] = bj * d->nx + bi;
+        cnt[base[i] + 1]++;
+    }
+    for (size_t b = 0; b < (size_t)d->nx * d->ny; b++) {
+        if (cnt[b + 1] == 0) return set_err(CMS2D_ERR_ARG, "a base cell without cells");
+        cnt[b + 1] += cnt[b];
+    }
+    std::vector<int> bcells(n), fill(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; i++) bcells[fill[base[i]]++] = i;
+    std::vector<uint64_t> meta(n, 0);
+    std::vector<int> jl;
+    for (int i = 0; i < n; i++) {
+        uint64_t m = (uint64_t)t.ncf[i];
+        for (int k = 0; k < t.ncf[i]; k++) {
+            const size_t f = (size_t)k * nd + i;
+            m |= (uint64_t)t.dir[f] << (3 + 2 * k);
+            m |= (uint64_t)t.llec[f] << (15 + 3 * k);
+            m |= (uint64_t)t.own[f] << (33 + k);
+        }
+        if (t.joint[i]) { m |= 1ull << 39; jl.push_back(i); }
+        const bool sm = cnt[base[i] + 1] - cnt[base[i]] > 1;
+        if (sm) m |= 1ull << 40;
+        bool near = sm;
+        for (int k = 0; k < t.ncf[i]; k++) {
+            const int nb = t.c2c[(size_t)k * nd + i];
+            if (nb < n && cnt[base[nb] + 1] - cnt[base[nb]] > 1) near = true;
+        }
+        if (near) m |= 1ull << 41;
+        m |= (uint64_t)strmask[i] << 42;
+        meta[i] = m;
+    }
+
+    F->dev = d->device;
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipStreamCreate(&F->st));
+    F->tel = new (std::nothrow) TelHost();
+    if (!F->tel) return set_err(CMS2D_ERR_ARG, "out of host memory");
+    TelHost &H = *F->tel;
+    TelG &T = H.T;
+    T.n = n; T.nd = nd; T.ng = ng;
+    T.nbx = d->nx; T.nby = d->ny; T.dxb = d->dx; T.dyb = d->dy;
+    T.njoint = (int)jl.size();
+    H.njoint = T.njoint;
+    T.skewcor = T.njoint > 0;
+    T.nstr = (int)str_c.size();
+    {
+        double asum = 0.0;
+        for (int i = 0; i < n; i++) asum += t.dx[i] * t.dy[i];
+        T.areaavg = asum / (double)(float)n;
+    }
+    int *c2c, *gown, *gk, *bas, *bst, *jnt, *sc_, *sk_;
+    uint64_t *met;
+    double *xx, *yy, *ddx, *ddy;
+    uint8_t *gw;
+    if (int rc = tel_upload(F, &c2c, t.c2c.data(), t.c2c.size())) return rc;
+    if (int rc = tel_upload(F, &met, meta.data(), meta.size())) return rc;
+    if (int rc = tel_upload(F, &xx, t.x.data(), (size_t)nd)) return rc;
+    if (int rc = tel_upload(F, &yy, t.y.data(), (size_t)nd)) return rc;
+    if (int rc = tel_upload(F, &ddx, t.dx.data(), (size_t)nd)) return rc;
+    if (int rc = tel_upload(F, &ddy, t.dy.data(), (size_t)nd)) return rc;
+    if (int rc = tel_upload(F, &gown, t.gown.data(), t.gown.size())) return rc;
+    if (int rc = tel_upload(F, &gk, t.gk.data(), t.gk.size())) return rc;
+    if (int rc = tel_upload(F, &gw, gwall.data(), gwall.size())) return rc;
+    if (int rc = tel_upload(F, &bas, base.data(), (size_t)n)) return rc;
+    if (int rc = tel_upload(F, &bst, cnt.data(), cnt.size())) return rc;
+    if (int rc = tel_upload(F, &H.bcells, bcells.data(), (size_t)n)) return rc;
+    if (int rc = tel_upload(F, &jnt, jl.data(), jl.size())) return rc;
+    if (int rc = tel_upload(F, &sc_, str_c.data(), str_c.size())) return rc;
+    if (int rc = tel_upload(F, &sk_, str_k.data(), str_k.size())) return rc;
+    T.c2c = c2c; T.meta = met; T.x = xx; T.y = yy; T.dx = ddx; T.dy = ddy;
+    T.gown = gown; T.gk = gk; T.gwall = gw; T.base = bas; T.bstart = bst; T.joint = jnt;
+    T.str_c = sc_; T.str_k = sk_;
+    H.gown_h = t.gown;
+    H.nblk_n = (n + BLOCK - 1) / BLOCK;
+    H.nblk_nd = (nd + BLOCK - 1) / BLOCK;
+    H.nblk_base = (d->nx * d->ny + BLOCK - 1) / BLOCK;
+    F->nblk_kry = std::min(KRY_BLOCKS, H.nblk_n);
+    F->use_graphs = false;
+
+    Par &P = F->P;
+    P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
+    P.hmin = P.hdry;
+    P.hdry1 = F4(5.0) * P.hdry;
+    P.hdry2 = F4(2.0) * P.hdry;
+    P.dtime = d->dt;
+    P.c2a = F4(0.2) * F4(0.2) * T.areaavg;
+    P.rtol = d->rtol > 0 ? d->rtol : 1e-10;
+    P.atol = d->atol >= 0 ? d->atol : 1e-12;
+    P.kmax = d->kmax > 0 ? d->kmax : 4000;
+    P.noptset = d->noptset;
+    P.wallfric = d->wallfric;
+    P.ndsch = d->ndsch;
+    {
+        static const int turb[5] = {5, 0, 1, 2, 3};
+        P.mturbul = turb[d->turbulence];
+        static const int fric[4] = {2, 1, 3, 4};
+        P.mbedfric = fric[d->friction];
+    }
+    F->maxit = d->maxit > 0 ? d->maxit : 20 + n / 10000;
+    F->rampdur = std::max(d->rampdur, d->dt / 3600.0);
+    F->dtvar = 0;
+    F->dtimebeg = F->deltime = d->dt;
+    F->ntc = d->ntc;
+    for (int k = 0; k < d->ntc; k++) { F->amp[k] = d->amp[k]; F->speed[k] = d->speed[k]; F->phase[k] = d->phase[k]; }
+    for (int q = 0; q < 3; q++) { F->rmom0[q] = 1e10; F->rmom[q] = 0.0; }
+    for (int k = 0; k < 4; k++) F->side_type[k] = d->side_type[k];
+    if (curve) {
+        F->wse_t.assign(d->wse_times, d->wse_times + d->wse_ntimes);
+        F->wse_v.assign(d->wse_values, d->wse_values + d->wse_ntimes);
+        F->wse_order = std::max(std::min(d->wse_order ? d->wse_order : 2, d->wse_ntimes - 1), 1);
+        F->wse_inc = 1;
+        F->wse_adjust_off = d->wse_adjust_off != 0;
+    }
+    F->precond = d->precond;
+    F->precision = d->precision;
+    F->solver = d->solver;
+
+    TelS &s = H.s;
+    double **cf[] = {&s.u, &s.v, &s.p, &s.h, &s.pp, &s.zb, &s.vis, &s.visb, &s.visc, &s.u1, &s.v1, &s.p1, &s.h1,
+                     &s.eta, &s.maxeta, &s.coefman, &s.cfrict, &s.z0, &s.uv, &s.uelwc, &s.bsvel, &s.cbf, &s.fc,
+                     &s.wavestrx, &s.wavestry, &s.su, &s.sv, &s.sp, &s.spu, &s.spv, &s.ssu0, &s.ssv0, &s.spuv0,
+                     &s.sppp0, &s.sspp0, &s.apuareap, &s.sumu, &s.Hu, &s.Hv, &s.resu, &s.resv, &s.resp, &s.dpx,
+                     &s.dpy, &s.tu, &s.tv, &s.tvis};
+    for (double **f : cf)
+        if (int rc = dalloc(F, f, (size_t)nd)) return rc;
+    double **ff[] = {&s.flux, &s.flux1, &s.hk, &s.pk, &s.visk, &s.zbk, &s.am, &s.apc};
+    for (double **f : ff)
+        if (int rc = dalloc(F, f, (size_t)TF * nd)) return rc;
+    if (int rc = dalloc(F, &s.wseadj, std::max<size_t>(ng, 1))) return rc;
+    uint8_t **bf[] = {&s.iwet, &s.iwet1, &s.icorner, &s.ipond};
+    for (uint8_t **f : bf)
+        if (int rc = dalloc(F, f, (size_t)nd)) return rc;
+    if (int rc = dalloc(F, &s.plab, (size_t)nd)) return rc;
+    if (int rc = dalloc(F, &s.flags, 4)) return rc;
+    double *(*pairs[])[2][2] = {&H.gu, &H.gv, &H.gpp, &H.gHu, &H.gHv, &H.gap, &H.gzb};
+    for (auto pr : pairs)
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                if (int rc = dalloc(F, &(*pr)[a][b], (size_t)nd)) return rc;
+    // Krylov systems: partial sums and vectors
+    {
+        const size_t nb = F->nblk_kry;
+        double *red;
+        if (int rc = dalloc(F, &red, 3 * 5 * nb)) return rc;
+        H.Kpp.red.pr[0] = red; H.Kpp.red.pv[0] = red + 2 * nb; H.Kpp.red.pt[0] = red + 3 * nb;
+        for (int q = 0; q < 2; q++) {
+            double *b = red + (q + 1) * 5 * nb;
+            H.Kuv.red.pr[q] = b; H.Kuv.red.pv[q] = b + 2 * nb; H.Kuv.red.pt[q] = b + 3 * nb;
+        }
+        if (int rc = dalloc(F, &F->d_sc, 4)) return rc;
+        double *kv[3][7];
+        for (int q = 0; q < 3; q++)
+            for (int mm = 0; mm < 7; mm++)
+                if (int rc = dalloc(F, &kv[q][mm], (size_t)nd)) return rc;
+        H.Kpp.x[0] = s.pp;
+        H.Kpp.r[0] = kv[0][0]; H.Kpp.r0[0] = kv[0][1]; H.Kpp.p[0] = kv[0][2];
+        H.Kpp.v[0] = kv[0][3]; H.Kpp.t[0] = kv[0][4]; H.Kpp.ap_w[0] = kv[0][5]; H.Kpp.dinv_w[0] = kv[0][6];
+        H.Kpp.sc = F->d_sc;
+        for (int q = 0; q < 2; q++) {
+            H.Kuv.x[q] = q ? s.v : s.u;
+            H.Kuv.r[q] = kv[q + 1][0]; H.Kuv.r0[q] = kv[q + 1][1]; H.Kuv.p[q] = kv[q + 1][2];
+            H.Kuv.v[q] = kv[q + 1][3]; H.Kuv.t[q] = kv[q + 1][4]; H.Kuv.ap_w[q] = kv[q + 1][5];
+            H.Kuv.dinv_w[q] = kv[q + 1][6];
+        }
+        H.Kuv.sc = F->d_sc + 1;
+        if (int rc = dalloc(F, &H.Kpp.zp, (size_t)nd)) return rc;
+        if (int rc = dalloc(F, &H.Kpp.zs, (size_t)nd)) return rc;
+        if (int rc = dalloc(F, &H.z1, (size_t)nd)) return rc;
+    }
+    if (int rc = mg_alloc(F, d->nx, d->ny)) return rc;
+    if (int rc = dalloc(F, &F->mg.L[0].zo, (size_t)(d->nx + 2) * (d->ny + 2))) return rc;
+    if (int rc = dalloc(F, &H.b0, (size_t)(d->nx + 2) * (d->ny + 2))) return rc;
+    F->mg_tile = 128;
+    if (const char *e = getenv("CMS2D_MG_TILE")) {
+        const int tt = atoi(e);
+        if (tt == 64 || tt == 128 || tt == 256) F->mg_tile = tt;
+    }
+    HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
+    memset(F->h_sc, 0, 4 * sizeof(KryScal));
+    F->tm.pool.resize(2 * 256);
+    for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
+    for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
+    // inputs: bed (ghosts take their cell's), friction input, forcing, Coriolis
+    {
+        std::vector<double> zb(nd), z0(nd, (double)1.0e-5f), buf(nd, 0.0);
+        for (int i = 0; i < n; i++) zb[i] = d->zb[i];
+        for (int q = 0; q < ng; q++) zb[n + q] = zb[t.gown[q]];
+        HIPCHK(hipMemcpyAsync(s.zb, zb.data(), nd * sizeof(double), hipMemcpyHostToDevice, F->st));
+        HIPCHK(hipMemcpyAsync(s.z0, z0.data(), nd * sizeof(double), hipMemcpyHostToDevice, F->st));
+        for (int i = 0; i < n; i++) buf[i] = d->mann[i];
+        double *in = P.mbedfric == 1 ? s.cfrict : P.mbedfric == 3 ? s.z0 : P.mbedfric == 4 ? s.cbf : s.coefman;
+        HIPCHK(hipMemcpyAsync(in, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        if (d->wavestrx) HIPCHK(hipMemcpy(s.wavestrx, d->wavestrx, n * sizeof(double), hipMemcpyHostToDevice));
+        if (d->wavestry) HIPCHK(hipMemcpy(s.wavestry, d->wavestry, n * sizeof(double), hipMemcpyHostToDevice));
+        const double pi = std::acos(-1.0), omega = 2.0 * pi / 86400.0, deg2rad = pi / 180.0;
+        const double f0 = 2.0 * omega * std::sin(d->latitude * deg2rad);
+        std::vector<double> fc(nd, f0);
+        if (d->coriolis_plane == 1) {
+            const double beta = 2.0 * omega * std::cos(d->latitude * deg2rad) / 6371000.0;
+            double ysum = 0.0;
+            for (int i = 0; i < n; i++) ysum += t.y[i];
+            const double yavg = ysum / (double)n;
+            for (int i = 0; i < nd; i++) fc[i] = f0 + beta * (t.y[i] - yavg);
+        }
+        HIPCHK(hipMemcpy(s.fc, fc.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // zbk with the all-cell gradient of the bed (its previous value zero)
+    TLAUNCH(k_tl_grad<1>, H.nblk_n, T, (const uint8_t *)s.iwet, 0, (const double *)s.zb, (const double *)nullptr,
+            (const double *)nullptr, tel_io(H.gzb, 0), GradIO{}, GradIO{});
+    TLAUNCH(k_tl_zbk, H.nblk_n, T, s, (const double *)H.gzb[0][1], (const double *)H.gzb[1][1]);
+    return tel_init_state(F);
+}
+
+static int tel_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {
+    TelHost &H = *F->tel;
+    HIPCHK(hipEventRecord(F->step_ev[0], F->st));
+    for (int q = 0; q < nsteps; q++) {
+        F->ntime++;
+        F->timesecs += F->deltime;
+        F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
+        if (int rc = tel_flow_imp(F)) return rc;
+        TLAUNCH(k_tl_update, H.nblk_nd, H.T, H.s);
+        // flow_update clears dppx, dppy: the next pp gradient starts from zero
+        TLAUNCH(k_tl_zero, H.nblk_nd, H.gpp[0][H.cpp], (long)H.T.nd);
+        TLAUNCH(k_tl_zero, H.nblk_nd, H.gpp[1][H.cpp], (long)H.T.nd);
+    }
+    HIPCHK(hipEventRecord(F->step_ev[1], F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    if (info) {
+        info->niter = F->niter;
+        info->iconv = F->iconv;
+        for (int q = 0; q < 3; q++) { info->rmom[q] = F->rmom[q]; info->kry_iters[q] = F->kry[q]; }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, F->step_ev[0], F->step_ev[1]);
+        info->kernel_ms = ms;
+        info->dt = F->deltime;
+        std::vector<uint8_t> w(H.T.n);
+        HIPCHK(hipMemcpyAsync(w.data(), H.s.iwet, H.T.n, hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        int nw = 0;
+        for (uint8_t x : w) nw += x;
+        info->nwet = nw;
+    }
+    return CMS2D_OK;
+}
+
+static double *tel_field(TelHost &H, const char *name) {
+    TelS &s = H.s;
+    struct { const char *n; double *p; } tab[] = {
+        {"eta", s.eta}, {"u", s.u}, {"v", s.v}, {"p", s.p}, {"h", s.h}, {"vis", s.vis}, {"cfrict", s.cfrict},
+        {"coefman", s.coefman}, {"uv", s.uv}, {"bsvel", s.bsvel}, {"pp", s.pp}, {"dpx", s.dpx}, {"dpy", s.dpy},
+        {"z0", s.z0}, {"zb", s.zb}, {"maxeta", s.maxeta}, {"wavestrx", s.wavestrx}, {"wavestry", s.wavestry},
+        {"fc", s.fc}};
+    for (auto &t : tab)
+        if (!strcmp(t.n, name)) return t.p;
+    return nullptr;
+}
+
+static int tel_get(cms2d_flow *F, const char *name, double *out) {
+    TelHost &H = *F->tel;
+    const int n = H.T.n;
+    HIPCHK(hipSetDevice(F->dev));
+    if (!strcmp(name, "iwet")) {
+        std::vector<uint8_t> w(n);
+        HIPCHK(hipMemcpyAsync(w.data(), H.s.iwet, n, hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        for (int i = 0; i < n; i++) out[i] = w[i];
+        return CMS2D_OK;
+    }
+    double *p = tel_field(H, name);
+    if (!p) return set_err(CMS2D_ERR_ARG, std::string("unknown field ") + name);
+    HIPCHK(hipMemcpyAsync(out, p, n * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    return CMS2D_OK;
+}
+
+static int tel_get_face(cms2d_flow *F, const char *name, double *out) {
+    TelHost &H = *F->tel;
+    const int n = H.T.n, nd = H.T.nd;
+    double *p = !strcmp(name, "flux") ? H.s.flux : !strcmp(name, "hk") ? H.s.hk : !strcmp(name, "visk") ? H.s.visk
+                : !strcmp(name, "pk") ? H.s.pk : nullptr;
+    if (!p) return set_err(CMS2D_ERR_ARG, std::string("unknown face field ") + name);
+    HIPCHK(hipSetDevice(F->dev));
+    std::vector<double> buf((size_t)TF * nd);
+    HIPCHK(hipMemcpyAsync(buf.data(), p, buf.size() * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < TF; k++) out[(size_t)TF * i + k] = buf[(size_t)k * nd + i];
+    return CMS2D_OK;
+}
+
+static int tel_set(cms2d_flow *F, const char *name, const double *in) {
+    TelHost &H = *F->tel;
+    double *p = nullptr;
+    if (!strcmp(name, "u")) p = H.s.u;
+    else if (!strcmp(name, "v")) p = H.s.v;
+    else if (!strcmp(name, "p")) p = H.s.p;
+    if (!p) return set_err(CMS2D_ERR_ARG, "set takes u, v or p");
+    HIPCHK(hipSetDevice(F->dev));
+    HIPCHK(hipMemcpyAsync(p, in, H.T.n * sizeof(double), hipMemcpyHostToDevice, F->st));
+    return tel_init_state(F);
+}
+

@@ -22,7 +22,8 @@ def _dp(a):
 
 
 class FlowStep:
-    """Device-resident implicit flow model on a uniform Cartesian grid."""
+    """Device-resident implicit flow model on a uniform Cartesian grid, or on a
+    telescoping grid when the case carries one (case.tel)."""
 
     def __init__(self, case, *, rtol=1e-10, atol=1e-12, kmax=4000, maxit=0,
                  precision=_lib.PREC_FP64, device=0, precond=None, solver=None, dtvar=0):
@@ -66,11 +67,25 @@ class FlowStep:
         d.turbulence = {5: 0, 0: 1, 1: 2, 2: 3, 3: 4}[case.mturbul]   # CMS2D_TURB_* from mturbul
         d.friction = {2: 0, 1: 1, 3: 2, 4: 3}[case.mbedfric]           # CMS2D_FRIC_* from mbedfric
         h = C.c_void_p()
-        check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
+        tel = getattr(case, "tel", None)
+        self.nf = 4
+        if tel is not None:
+            # a telescoping grid: the .tel arrays and the water-level cellstring
+            geom = np.ascontiguousarray(tel.geom, dtype=np.float64)
+            nb = np.ascontiguousarray(tel.nb, dtype=np.int32)
+            bnd = np.ascontiguousarray(tel.bnd_cells, dtype=np.int32)
+            keep += [geom, nb, bnd]
+            tg = _lib.TelGrid()
+            tg.ncells, tg.geom, tg.nb = len(geom), _dp(geom), nb.ctypes.data_as(I)
+            tg.bnd_cells, tg.nbnd = bnd.ctypes.data_as(I), len(bnd)
+            check(L.cms2d_flow_create_tel(C.byref(d), C.byref(tg), C.byref(h)))
+            self.nf = 6
+        else:
+            check(L.cms2d_flow_create(C.byref(d), C.byref(h)))
         self._h = h
         if case.wind_speed:
             self.set_wind(*case.wind_vector())
-        self.ncells = case.nx * case.ny
+        self.ncells = case.ncells
 
     def close(self):
         if getattr(self, "_h", None):
@@ -100,9 +115,9 @@ class FlowStep:
         check(_lib.lib().cms2d_flow_set(self._h, name.encode(), _dp(a)))
 
     def get_face(self, name):
-        out = np.empty(4 * self.ncells)
+        out = np.empty(self.nf * self.ncells)
         check(_lib.lib().cms2d_flow_get_face(self._h, name.encode(), _dp(out)))
-        return out.reshape(self.ncells, 4)
+        return out.reshape(self.ncells, self.nf)
 
     @property
     def time(self):

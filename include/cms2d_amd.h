@@ -14,6 +14,7 @@
  *                       with pbicgstab                 source/solvers/pbicgstab.f90:2-98
  *   cms2d_flow_create <- prestart/flow_init/geo_cart   source/main/prestart.f90:2-329,
  *                       source/flow/flow.F90:592-835, source/geospatial/geospatial.F90:2709-2828
+ *   cms2d_flow_create_tel <- the same with read_tel     source/geospatial/geospatial.F90:403-795
  *   cms2d_flow_get/_get_face/_set <- reads/writes of the flow_def module state
  *                       (eta, u, v, p, h, iwet, flux, hk, visk; source/flow/flow_def.f90:32-106)
  *   cms2d_flow_time  <- timesecs of the time loop      source/main/CMS_Flow.f90:109-240
@@ -88,7 +89,8 @@ typedef struct {
                                      Manning n by default (MANNINGS_N_DATASET)              */
     const double *wavestrx;       /* radiation-stress forcing / rho [nx*ny] or NULL         */
     const double *wavestry;       /* (wave_flowgrid_def wavestrx/wavestry)                  */
-    int side_type[4];             /* CMS2D_BND_* for N,E,S,W                                 */
+    int side_type[4];             /* CMS2D_BND_* for N,E,S,W: one water-level side at most
+                                     (up to 8192 cells long), the others walls               */
     int ntc;                      /* harmonic constituents (<= 8)                            */
     double amp[8];                /* [m]                                                     */
     double speed[8];              /* [rad/hr]                                                */
@@ -140,6 +142,24 @@ typedef struct {
     double dt;                    /* time step of the last step [s] (varies with dtvar)      */
 } cms2d_step_info;
 
+/* Telescoping grid: the content of a .tel file (read_tel,
+ * source/geospatial/geospatial.F90:403-795), every cell active.  Cells in file
+ * order; geom[4*i..] = centre x, y and size dx, dy [m] in grid coordinates
+ * with the origin at the south-west corner of the base grid; nb[8*i..] = the
+ * file's neighbour slots N1 N2 E1 E2 S1 S2 W1 W2 (1-based cell numbers, 0
+ * none): a side with two finer neighbours lists them clockwise (north: west
+ * then east; east: north then south; south: east then west; west: south then
+ * north), at most six faces per cell.  bnd_cells: the water-level cellstring
+ * (0-based, in string order, 2..8192 cells; its faces follow cellstr_bnd,
+ * source/boundaries/bnd_lib.f90:215-316); every other edge face is a wall. */
+typedef struct {
+    int ncells;
+    const double *geom;
+    const int *nb;
+    const int *bnd_cells;
+    int nbnd;
+} cms2d_tel_grid;
+
 typedef struct cms2d_flow cms2d_flow;
 
 const char *cms2d_last_error(void);
@@ -149,6 +169,18 @@ int cms2d_device_count(int *count);
  * (prestart.f90:309-329). */
 int cms2d_flow_create(const cms2d_flow_desc *desc, cms2d_flow **out);
 int cms2d_flow_destroy(cms2d_flow *flow);
+/* A flow on a telescoping grid: read_tel and geo_init_cart
+ * (geospatial.F90:403-795,1400-1611) on the cells of `grid`, the step of
+ * flow_imp with the joint-cell skewness terms (flow_imp_flux.f90:2-169,
+ * flow_imp.F90:718-789,912-962, computational.F90:149-264, der_lib.f90:381-504,
+ * interp_lib.f90:2853-2873).  desc gives the physics and the base grid
+ * (nx, ny, dx, dy: every cell lies in one base cell, which the multigrid
+ * preconditioner coarsens); zb, mann and the forcing are per cell of `grid`;
+ * one side_type names the water-level string of grid->bnd_cells.  FP64
+ * BiCGStab with a fixed time step; the NVD schemes and water-level gradients
+ * are built for Cartesian grids only.  Fields of cms2d_flow_get are per cell;
+ * cms2d_flow_get_face gives out[6*c+k] in the cell's face order. */
+int cms2d_flow_create_tel(const cms2d_flow_desc *desc, const cms2d_tel_grid *grid, cms2d_flow **out);
 /* Advances nsteps time steps: timing, flow_imp and flow_update per step. */
 int cms2d_flow_step(cms2d_flow *flow, int nsteps, cms2d_step_info *info);
 /* Copies an active-cell field to host memory, row-major [nx*ny]:

@@ -67,3 +67,78 @@ def test_gmres_requires_multigrid_fp64():
         with pytest.raises(_lib.Cms2dError) as e:
             FlowStep(cases.lake_at_rest(), solver=_lib.SOLVER_GMRES, **kw)
         assert e.value.code == -1
+
+
+def _desc(case, keep):
+    from cms2d_amd import _lib
+    d = _lib.FlowDesc()
+    d.nx, d.ny, d.dx, d.dy = case.nx, case.ny, case.dx, case.dy
+    for a in (case.zb, case.mann):
+        keep.append(np.ascontiguousarray(a, dtype=np.float64))
+    d.zb, d.mann = keep[-2].ctypes.data_as(_lib.D), keep[-1].ctypes.data_as(_lib.D)
+    d.ntc, d.amp[0], d.speed[0] = 1, 0.5, 0.5
+    d.dt, d.hdry, d.ndsch, d.rtol, d.atol, d.kmax = 30.0, 0.05, 4, 1e-10, 1e-12, 100
+    return d
+
+
+@pytest.mark.parametrize("sides", [(1, 1, 0, 0), (1, 0, 0, 2), (2, 0, 2, 0)])
+def test_more_than_one_water_level_side_refused(sides):
+    """Each water-level string is forced on its own: the side's set-up and the
+    corner cells where two forced sides meet have no pinned counterpart, so a
+    description with two or more of them is refused before any device use."""
+    from cms2d_amd import _lib, cases
+    L, keep = _lib.lib(), []
+    d = _desc(cases.lake_at_rest(), keep)
+    t, v = np.array([0.0, 1.0]), np.array([0.0, 0.1])
+    d.wse_times, d.wse_values = t.ctypes.data_as(_lib.D), v.ctypes.data_as(_lib.D)
+    d.wse_ntimes = 2
+    for k in range(4):
+        d.side_type[k] = sides[k]
+    h = C.c_void_p()
+    assert L.cms2d_flow_create(C.byref(d), C.byref(h)) == -1
+    assert b"one water-level side" in L.cms2d_last_error()
+
+
+def test_long_water_level_side_and_bad_gradient_refused():
+    from cms2d_amd import _lib, cases
+    L, keep = _lib.lib(), []
+    c = cases.lake_at_rest(nx=9000, ny=3)
+    d = _desc(c, keep)
+    d.side_type[0] = _lib.BND_WSE_HARMONIC
+    h = C.c_void_p()
+    assert L.cms2d_flow_create(C.byref(d), C.byref(h)) == -1
+    assert b"8192" in L.cms2d_last_error()
+    d = _desc(cases.lake_at_rest(), keep)
+    t, v = np.array([0.0, 1.0]), np.array([0.0, 0.1])
+    d.wse_times, d.wse_values = t.ctypes.data_as(_lib.D), v.ctypes.data_as(_lib.D)
+    d.wse_ntimes = 2
+    d.side_type[1] = _lib.BND_WSE_CURVE
+    d.wse_grady = float("nan")
+    assert L.cms2d_flow_create(C.byref(d), C.byref(h)) == -1
+    assert b"gradients" in L.cms2d_last_error()
+
+
+def test_telescoping_grid_checked_before_device():
+    """Malformed .tel arrays are refused with CMS2D_ERR_ARG before any device
+    use; a well-formed grid reaches the device probe."""
+    import dataclasses
+
+    from cms2d_amd import _lib, cases
+    from cms2d_amd.flow import FlowStep
+    c = cases.tidal_tel(nx=12, ny=8, dx=20.0, band=((2, 6), (3, 5)), seed=1)
+    bad = dataclasses.replace(c.tel, nb=np.where(c.tel.nb > 0, c.tel.nb + 10_000, 0).astype(np.int32))
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(dataclasses.replace(c, tel=bad))
+    assert e.value.code == -1
+    nb = c.tel.nb.copy()
+    nb[0, 0] = 0                              # one side of a pair without its reciprocal
+    nb[0, 2] = 3
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(dataclasses.replace(c, tel=dataclasses.replace(c.tel, nb=nb)))
+    assert e.value.code == -1
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(dataclasses.replace(c, ndsch=7))
+    assert e.value.code == -1
+    with pytest.raises(_lib.Cms2dError) as e:
+        FlowStep(c)
+    assert e.value.code == -4                 # no device here
