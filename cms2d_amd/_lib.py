@@ -36,7 +36,7 @@ EXPORTS = [
     "cms2d_flow_step", "cms2d_flow_get", "cms2d_flow_get_face", "cms2d_flow_set",
     "cms2d_flow_time", "cms2d_flow_kernel_stats", "cms2d_flow_tuning", "cms2d_flow_set_waves",
     "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
-    "cms2d_ensemble_step", "cms2d_solve", "cms2d_flow_create_tel",
+    "cms2d_ensemble_step", "cms2d_solve", "cms2d_solve_noalloc", "cms2d_flow_create_tel",
 ]
 
 
@@ -105,6 +105,7 @@ def lib():
                                       C.POINTER(StepInfo)]
     L.cms2d_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double, C.c_double,
                               C.c_double, C.c_int, D, I]
+    L.cms2d_solve_noalloc.argtypes = L.cms2d_solve.argtypes
     _LIB = L
     return L
 

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1617,5 +1618,84 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double 
                         atol, kmax, rmom, iters);
     if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     cleanup();
+    return rc;
+}
+
+// ----------------------------------------------------------------------------
+// cms2d_solve_noalloc: cms2d_solve without device allocation.  Its staging
+// and Krylov scratch live in one static device array of CMS2D_SOLVE_SCRATCH_BYTES
+// per device (held until the process ends), carved up per call like the
+// buffers cms2d_solve allocates.  The host-side name of the array is not a
+// device address: every host use goes through the address that
+// hipGetSymbolAddress returns, looked up once per device.  Calls run on the
+// device's null stream; calls that overlap on one device share the array and
+// race.
+static __device__ __align__(256) unsigned char scratch[CMS2D_SOLVE_SCRATCH_BYTES];
+
+static constexpr int SCRATCH_MAX_DEV = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SCRATCH_MAX_DEV];
+
+static int scratch_addr(void **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEV) return set_err(CMS2D_ERR_ARG, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(CMS2D_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Bytes of scratch a call needs (each part 256-byte aligned).
+static size_t solve_scratch_bytes(size_t n, size_t nD, size_t nf, size_t nblk) {
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    return al(nf * nD * sizeof(double)) + al(nf * nD * sizeof(int)) + al(nD * sizeof(int)) +
+           4 * al(nD * sizeof(double)) + al(7 * n * sizeof(double)) + al(nblk * 5 * sizeof(double)) +
+           al(sizeof(KryScal));
+}
+
+extern "C" int cms2d_solve_noalloc(int ncells, int ncellsD, int nmaxfaces, const double *acoef,
+                                   const int *cell2cell, const int *iwet, double *ss, const double *sp,
+                                   double *res, double *phi, double relax, double rtol, double atol, int kmax,
+                                   double *rmom, int *iters) {
+    if (int rc = solve_check_args(ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi, relax))
+        return rc;
+    const size_t nD = ncellsD, n = ncells, nf = nmaxfaces;
+    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
+    const size_t need = solve_scratch_bytes(n, nD, nf, nblk);
+    if (need > (size_t)CMS2D_SOLVE_SCRATCH_BYTES)
+        return set_err(CMS2D_ERR_ARG, "system needs " + std::to_string(need) + " bytes of scratch, beyond the " +
+                                          std::to_string((size_t)CMS2D_SOLVE_SCRATCH_BYTES) +
+                                          " of cms2d_solve_noalloc: use cms2d_solve");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
+    void *base = nullptr;
+    if (int rc = scratch_addr(&base)) return rc;
+    unsigned char *q = (unsigned char *)base;
+    auto take = [&](size_t bytes) {
+        void *p = q;
+        q += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    SolveBufs B{};
+    B.a = (double *)take(nf * nD * sizeof(double));
+    B.c2c = (int *)take(nf * nD * sizeof(int));
+    B.w = (int *)take(nD * sizeof(int));
+    B.ss = (double *)take(nD * sizeof(double));
+    B.sp = (double *)take(nD * sizeof(double));
+    B.res = (double *)take(nD * sizeof(double));
+    B.phi = (double *)take(nD * sizeof(double));
+    B.vec = (double *)take(7 * n * sizeof(double));
+    B.partials = (double *)take((size_t)nblk * 5 * sizeof(double));
+    B.sc = (KryScal *)take(sizeof(KryScal));
+    const int rc = solve_run(B, nullptr, ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi,
+                             relax, rtol, atol, kmax, rmom, iters);
+    (void)hipStreamSynchronize(nullptr);
     return rc;
 }

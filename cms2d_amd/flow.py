@@ -162,13 +162,14 @@ class FlowStep:
 
 
 def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, atol=1e-12,
-          kmax=4000):
+          kmax=4000, noalloc=False):
     """solve(acoef,ss,sp,res,phi,n) on the reference's cell list.
 
     acoef, cell2cell: [ncellsD, nmaxfaces] (4 faces N,E,S,W on Cartesian grids,
     up to 6 in any order on telescoping grids), cell2cell 0-based with -1 for
     no face; iwet, ss, sp, phi: [ncellsD].  ss and phi are updated in place.
-    Returns (res, rmom, iterations)."""
+    noalloc: through cms2d_solve_noalloc (no device allocation; systems up to
+    its static scratch size).  Returns (res, rmom, iterations)."""
     acoef = np.ascontiguousarray(acoef, dtype=np.float64)
     c2c = np.ascontiguousarray(cell2cell, dtype=np.int32)
     iw = np.ascontiguousarray(iwet, dtype=np.int32)
@@ -180,7 +181,8 @@ def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, ato
     res = np.zeros(ncellsD)
     rmom = C.c_double()
     iters = C.c_int()
-    check(_lib.lib().cms2d_solve(ncells, ncellsD, acoef.shape[1], _dp(acoef), c2c.ctypes.data_as(I),
-                                 iw.ctypes.data_as(I), _dp(ss), _dp(spa), _dp(res), _dp(phi),
-                                 relax, rtol, atol, kmax, C.byref(rmom), C.byref(iters)))
+    fn = _lib.lib().cms2d_solve_noalloc if noalloc else _lib.lib().cms2d_solve
+    check(fn(ncells, ncellsD, acoef.shape[1], _dp(acoef), c2c.ctypes.data_as(I),
+             iw.ctypes.data_as(I), _dp(ss), _dp(spa), _dp(res), _dp(phi),
+             relax, rtol, atol, kmax, C.byref(rmom), C.byref(iters)))
     return res, rmom.value, iters.value

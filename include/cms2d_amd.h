@@ -268,6 +268,24 @@ int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, con
                 const int *iwet, double *ss, const double *sp, double *res, double *phi,
                 double relax, double rtol, double atol, int kmax, double *rmom, int *iters);
 
+/* Device scratch of cms2d_solve_noalloc per device [bytes]. */
+#define CMS2D_SOLVE_SCRATCH_BYTES (64u << 20)
+
+/* cms2d_solve without device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, checks and results as
+ * cms2d_solve, which it shares its kernels with; its device buffers (staged
+ * system, Krylov vectors, partial sums, scalars) come from one static device
+ * array of CMS2D_SOLVE_SCRATCH_BYTES per device, held until the process ends.
+ * A system needs about (12 nmaxfaces + 36) ncellsD + 56 ncells bytes of it
+ * (nmaxfaces 4: about 470k cells; 6: about 400k); larger systems are refused
+ * with CMS2D_ERR_ARG.  Runs on the device's null stream and returns after the
+ * solve.  Every call on a device shares that device's array: calls that overlap
+ * on one device race. */
+int cms2d_solve_noalloc(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
+                        const int *iwet, double *ss, const double *sp, double *res, double *phi,
+                        double relax, double rtol, double atol, int kmax, double *rmom, int *iters);
+
 #ifdef __cplusplus
 }
 #endif

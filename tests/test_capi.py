@@ -69,6 +69,24 @@ def test_gmres_requires_multigrid_fp64():
         assert e.value.code == -1
 
 
+def test_noalloc_solve_refuses_systems_beyond_its_scratch():
+    """cms2d_solve_noalloc checks the scratch it needs before any device use."""
+    import ctypes as C
+
+    import numpy as np
+
+    from cms2d_amd.flow import solve
+    n = 600_000
+    nd = n + 4
+    c2c = np.full((nd, 4), -1, dtype=np.int32)
+    acoef = np.zeros((nd, 4))
+    iwet = np.ones(nd, dtype=np.int32)
+    ss, sp, phi = np.zeros(nd), -np.ones(nd), np.zeros(nd)
+    from cms2d_amd import _lib
+    with pytest.raises(_lib.Cms2dError, match="scratch"):
+        solve(acoef, c2c, iwet, ss, sp, phi, 1.0, ncells=n, noalloc=True)
+
+
 def _desc(case, keep):
     from cms2d_amd import _lib
     d = _lib.FlowDesc()
