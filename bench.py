@@ -76,6 +76,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-refinement form")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble")
+    ap.add_argument("--no-cartesian", action="store_true",
+                    help="skip the uniform-grid comparison (and the FP32 form, built there)")
     ap.add_argument("--ens-members", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=3)
     ap.add_argument("--ens-threads", type=int, default=8)
@@ -320,22 +322,23 @@ def main():
         "step_model_roofline": T["step_model_roofline"],
         "cpu_baseline": None,
     }
-    case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
-    g, C = measure(a, case, "cartesian", GROUPS)
-    Dc = C["kern"][C["dominant"]]
-    out["cartesian_uniform"] = {
-        "workload": f"the same physics on the uniform {a.nx}x{a.ny} grid of 20 m cells",
-        "value": C["value"], "unit": "cell-steps/s", "ms_per_step": C["ms_per_step"], "cells": C["cells"],
-        "mg_tile_x_autotuned": C["mg_tile_x"], "outer_iters_per_step": C["outer_iters_per_step"],
-        "krylov_iters_per_step": C["krylov_iters_per_step"],
-        "roofline": {"kernel": Dc["kernel"], "achieved": Dc["achieved_GBs"], "peak": HBM_PEAK_GBS,
-                     "frac": Dc["achieved_GBs"] / HBM_PEAK_GBS if Dc["achieved_GBs"] else None,
-                     "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"], "timed_groups": C["kern"]},
-        "step_model_roofline": C["step_model_roofline"]}
-    if not a.no_fp32:
-        out["fp32_refine"] = fp32_form(a, case, g, C["value"])
-        out["fp32_refine"]["workload"] = "the uniform Cartesian grid (the FP32 form is built there)"
-    g.close()
+    if not a.no_cartesian:
+        case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
+        g, C = measure(a, case, "cartesian", GROUPS)
+        Dc = C["kern"][C["dominant"]]
+        out["cartesian_uniform"] = {
+            "workload": f"the same physics on the uniform {a.nx}x{a.ny} grid of 20 m cells",
+            "value": C["value"], "unit": "cell-steps/s", "ms_per_step": C["ms_per_step"], "cells": C["cells"],
+            "mg_tile_x_autotuned": C["mg_tile_x"], "outer_iters_per_step": C["outer_iters_per_step"],
+            "krylov_iters_per_step": C["krylov_iters_per_step"],
+            "roofline": {"kernel": Dc["kernel"], "achieved": Dc["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                         "frac": Dc["achieved_GBs"] / HBM_PEAK_GBS if Dc["achieved_GBs"] else None,
+                         "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"], "timed_groups": C["kern"]},
+            "step_model_roofline": C["step_model_roofline"]}
+        if not a.no_fp32:
+            out["fp32_refine"] = fp32_form(a, case, g, C["value"])
+            out["fp32_refine"]["workload"] = "the uniform Cartesian grid (the FP32 form is built there)"
+        g.close()
     if not a.no_ensemble:
         out["ensemble"] = ensemble_form(a)
     if not a.no_cpu_baseline:

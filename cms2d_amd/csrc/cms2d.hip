@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 using namespace cms;
@@ -128,6 +129,13 @@ struct cms2d_flow {
     bool use_graphs = false;
     hipGraphExec_t kgraph[4]{};
     int mg_tile = 128;         // level-0 tile width of the multigrid cycle (timed at creation)
+    // the reference's cell order for the three order-dependent loops: the
+    // buffers of consecutive passes (dry-to-wet u, v, vis; fix-up face values
+    // [4*npad]; corner values)
+    int serial_order = 0;
+    double *ser_d2w[2][3]{};
+    double *ser_face[2]{};
+    double *ser_corner[2]{};
     int windconst = 0;         // spatially constant wind (met_def windconst) and its 10-m vector
     double wndx = 0.0, wndy = 0.0;
     int side_type[4]{};        // CMS2D_BND_* per side (the kernels see 1 for every water-level side)
@@ -208,6 +216,31 @@ static int sync_flags(cms2d_flow *F) {
     return 0;
 }
 
+// Passes of one of the serial-order recurrences until a pass changes nothing:
+// pass(first, from, to) evaluates into buffer `to` from buffer `from` (the
+// first pass from the values before the loop); `res` is the buffer that holds
+// the settled values.  A pass fixes at least the next cell of every chain, so
+// the count is bounded by the number of cells.
+template <class Pass>
+static int serial_settle(cms2d_flow *F, Pass pass, int &res) {
+    int *flag = F->s.flags + 3;
+    pass(1, 0, 0);
+    int cur = 0;
+    const long cap = (long)F->g.nx * F->g.ny + 2;
+    for (long q = 0; q < cap; q++) {
+        LAUNCH(k_set_int, 1, flag, 1, 0);
+        pass(0, cur, cur ^ 1);
+        cur ^= 1;
+        HIPCHK(hipMemcpyAsync(F->h_flags + 3, flag, sizeof(int), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        if (!F->h_flags[3]) {
+            res = cur;
+            return 0;
+        }
+    }
+    return set_err(CMS2D_ERR_HIP, "serial-order passes did not settle");
+}
+
 // flow_wetdry(0), flow.F90:1231-1539
 static int wetdry(cms2d_flow *F) {
     Geo &g = F->g;
@@ -234,6 +267,18 @@ static int wetdry(cms2d_flow *F) {
     if (int rc = sync_flags(F)) return rc;
     if (F->h_flags[2] > 0) {
         for (int sweep = 0; sweep < 5; sweep++) {
+            if (F->serial_order) {
+                int res = 0;
+                auto pass = [&](int first, int from, int to) {
+                    double *const *a = F->ser_d2w[from], *const *b = F->ser_d2w[to];
+                    LAUNCH(k_d2w_serial, F->nblk_act, g, s, (const double *)a[0], (const double *)a[1],
+                           (const double *)a[2], b[0], b[1], b[2], first, s.flags + 3);
+                };
+                if (int rc = serial_settle(F, pass, res)) return rc;
+                LAUNCH(k_d2w_serial_commit, F->nblk_act, g, s, (const double *)F->ser_d2w[res][0],
+                       (const double *)F->ser_d2w[res][1], (const double *)F->ser_d2w[res][2]);
+                continue;
+            }
             LAUNCH(k_d2w_sweep, F->nblk_act, g, s);
             LAUNCH(k_d2w_commit, F->nblk_act, g, s);
         }
@@ -654,8 +699,25 @@ static void check_conv(cms2d_flow *F) {
 }
 
 static int eddyvis_tail(cms2d_flow *F) {
-    LAUNCH(k_vis_fix, F->nblk_pad, F->g, F->s);
-    LAUNCH(k_vis_corner, F->nblk_act, F->g, F->s);
+    if (F->serial_order) {
+        int res = 0;
+        auto fix = [&](int first, int from, int to) {
+            LAUNCH(k_vis_fix_serial, F->nblk_act, F->g, F->s, (const double *)F->ser_face[from], F->ser_face[to],
+                   first, F->s.flags + 3);
+        };
+        if (int rc = serial_settle(F, fix, res)) return rc;
+        LAUNCH(k_vis_fix_serial_commit, F->nblk_pad, F->g, F->s, (const double *)F->ser_face[res]);
+        auto corner = [&](int first, int from, int to) {
+            LAUNCH(k_vis_corner_serial, F->nblk_act, F->g, F->s, (const double *)F->ser_corner[from],
+                   F->ser_corner[to], first, F->s.flags + 3);
+        };
+        if (int rc = serial_settle(F, corner, res)) return rc;
+        HIPCHK(hipMemcpyAsync(F->s.visc, F->ser_corner[res], (size_t)F->npad * sizeof(double),
+                              hipMemcpyDeviceToDevice, F->st));
+    } else {
+        LAUNCH(k_vis_fix, F->nblk_pad, F->g, F->s);
+        LAUNCH(k_vis_corner, F->nblk_act, F->g, F->s);
+    }
     LAUNCH(k_visk, F->nblk_act, F->g, F->s);
     return 0;
 }
@@ -1014,6 +1076,15 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     if (int rc = dalloc(F, &s.flags, 4)) return rc;
     if (int rc = dalloc(F, &s.plab, N)) return rc;
     F->state.push_back({s.plab, N * sizeof(int)});
+    F->serial_order = d->serial_order != 0;
+    if (F->serial_order) {
+        for (int b = 0; b < 2; b++) {
+            for (int q = 0; q < 3; q++)
+                if (int rc = dalloc(F, &F->ser_d2w[b][q], N)) return rc;
+            if (int rc = dalloc(F, &F->ser_face[b], 4 * N)) return rc;
+            if (int rc = dalloc(F, &F->ser_corner[b], N)) return rc;
+        }
+    }
     {
         // Krylov partial sums: 5 slots per system (pr: 2, pv: 1, pt: 2)
         double *red;

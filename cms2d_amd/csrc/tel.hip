@@ -3,21 +3,51 @@
 // multigrid hierarchy of its base grid.  Included by cms2d.hip after the
 // Cartesian driver, whose Krylov, multigrid and control routines it reuses.
 //
-// Preconditioner (z = M^-1 r, FP32): the cells of the refined base cells are
-// smoothed by damped Jacobi, the residual is summed into the base cells (the
-// equations are cell integrals), one V-cycle of the base grid's multigrid
-// solves for the base-cell correction, which is added to every cell of the
-// base cell, and the refined cells are smoothed once more.  The base-grid
-// operator is re-discretised from the cell list: the coefficient across a
-// base-cell face is the sum of the cell-face coefficients it covers, each
-// scaled by its centre distance over the base spacing (a coefficient ~ ds/dn,
-// so the sum keeps the base face's ds/dn), and the row excess ap - sum(a)
-// (storage term, folded boundary coefficients) is summed over the cells.  On
-// unrefined base cells the base operator is the cell operator itself.
+// Preconditioner (z = M^-1 r, FP32): a V-cycle over the levels of the
+// quadtree and then the base grid.  Level k (1 <= k < finest) holds squares of
+// 2^-k base cells: the real cells of level k and virtual cells that stand for
+// the finer cells inside them.  Going down, the finest cells take damped Jacobi
+// sweeps, the fine residual r - A z is summed into the level-k squares (the
+// equations are cell integrals), each level takes its sweeps and adds its
+// correction to the cells inside its squares; the residual summed into the
+// base cells is solved by one V-cycle of the base grid's multigrid, whose
+// correction is added to every cell of the base cell; going up, each level
+// and then the finest cells take their sweeps again.  The operators of the
+// coarser levels are re-discretised from the cell list: the coefficient
+// across a square's or base cell's edge is the sum of the cell-face
+// coefficients it covers, each scaled by its centre distance over the coarse
+// one (a coefficient ~ ds/dn, so the sum keeps the coarse face's ds/dn), and
+// the row excess ap - sum(a) (storage term, folded boundary coefficients) is
+// summed over the cells.  On unrefined base cells the base operator is the
+// cell operator itself.
 
 namespace cms {
 
-constexpr float TEL_OMEGA = 0.8f;     // damping of the Jacobi smoothing
+constexpr float TEL_OMEGA = 0.8f;     // damping of the Jacobi sweeps
+constexpr int TEL_NU = 2;             // sweeps per level before and after the coarse correction
+constexpr int TEL_MAX_LEVELS = 5;     // refinement levels below the base grid (16 x 16 cells per base cell)
+static_assert(TEL_NU % 2 == 0, "the up sweeps alternate between two buffers and end in the first");
+
+// Level k of the refined region (1 <= k < finest level): squares of 2^-k base
+// cells.  Lists are ordered by the squares' first cells.
+struct TelLev {
+    int nk;                       // squares
+    const int *kstart, *kcell;    // cells inside each square, ascending: [nk+1], [kstart[nk]]
+    const int *kseg;              // square of each entry of kcell
+    const int *blk;               // entry ranges of the restriction's blocks (whole squares) [nblk+1]
+    int nblk;
+    const int *nbr;               // [4*nk] (side-major) level-k square across N, E, S, W, or -1
+    const int *cstart, *cf;       // faces of the cells crossing each square's edge: flat index k*nd+i
+    const float *cw;              // their weight dn / dn of the square
+    const uint8_t *cs;            // their side 0..3, or 4 toward a coarser cell (diagonal only)
+    const int *ge;                // cells of level >= k [nge], ascending
+    int nge;
+    const int *anc;               // [n] square of each cell of level >= k
+    float *a;                     // [4*nk] (side-major) off-diagonal coefficients
+    float *ap, *dinv;             // [nk]
+    float *r;                     // [nk] summed residual
+    float *z[2];                  // [nk] ping-pong
+};
 
 // Base-grid level 0 of the multigrid hierarchy from the cell-list pp system.
 __global__ void k_tl_mg_setup0(TelG T, const double *apc, const double *ap, const int *bcells, MgLevel L) {
@@ -52,55 +82,160 @@ __global__ void k_tl_mg_setup0(TelG T, const double *apc, const double *ap, cons
     L.dinv[c] = apb > 0.0 ? (float)(1.0 / apb) : 0.0f;
 }
 
-// z1 = omega D^-1 r on the cells of refined base cells, 0 elsewhere
-__global__ void k_tl_mg_pre(TelG T, const double *r, const double *ap, float *z1, const KryScal *sc) {
-    if (sc->done) return;
-    int i;
-    if (!tel_thread(T.n, i)) return;
-    z1[i] = m_smooth(T.meta[i]) ? TEL_OMEGA * (float)(r[i] / ap[i]) : 0.0f;
-}
-// b of the base cell: the summed residual r - A z1 of its cells
-__global__ void k_tl_mg_restrict(TelG T, const double *r, const double *ap, const double *apc, const float *z1,
-                                 const int *bcells, MgLevel L, float *b0, const KryScal *sc) {
-    if (sc->done) return;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= T.nbx * T.nby) return;
-    double acc = 0.0;
-    for (int q = T.bstart[b]; q < T.bstart[b + 1]; q++) {
-        const int i = bcells[q];
-        const uint64_t m = T.meta[i];
-        double ri = r[i];
-        if (m_near(m)) {
-            double az = ap[i] * (double)z1[i];
-            for (int k = 0; k < m_ncf(m); k++) {
-                const int nb = tnb(T, i, k);
-                if (nb < T.n) az -= apc[k * T.nd + i] * (double)z1[nb];
-            }
-            ri -= az;
-        }
-        acc += ri;
+// The residual r - A Z of cell i for a correction Z that is nonzero on
+// refined cells only (so only cells next to them see it); the face loads
+// are issued together (every slot, predicated) and summed in face order.
+__device__ __forceinline__ double tl_res(const TelG &T, const double *r, const double *ap, const double *apc,
+                                         const float *Z, int i, uint64_t m) {
+    double ri = r[i];
+    if (m_near(m)) {
+        const int nf = m_ncf(m);
+        int nbk[TF];
+        float zk[TF];
+#pragma unroll
+        for (int k = 0; k < TF; k++) nbk[k] = k < nf ? tnb(T, i, k) : T.n;
+#pragma unroll
+        for (int k = 0; k < TF; k++) zk[k] = nbk[k] < T.n ? Z[nbk[k]] : 0.0f;
+        double az = ap[i] * (double)Z[i];
+#pragma unroll
+        for (int k = 0; k < TF; k++)
+            if (nbk[k] < T.n) az -= apc[k * T.nd + i] * (double)zk[k];
+        ri -= az;
     }
+    return ri;
+}
+
+// Coefficients of level k from the cell list's pp system (once per outer iteration).
+__global__ void k_tl_cs_setup(TelG T, TelLev V, const double *apc, const double *ap) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V.nk) return;
+    double e = 0.0;
+    for (int q = V.kstart[c]; q < V.kstart[c + 1]; q++) {
+        const int i = V.kcell[q];
+        const uint64_t m = T.meta[i];
+        double sa = 0.0;
+        for (int k = 0; k < m_ncf(m); k++)
+            if (tnb(T, i, k) < T.n) sa += apc[k * T.nd + i];
+        e += fmax(ap[i] - sa, 0.0);
+    }
+    double a4[4] = {0.0, 0.0, 0.0, 0.0}, diag = 0.0;
+    for (int q = V.cstart[c]; q < V.cstart[c + 1]; q++) {
+        const double v = (double)V.cw[q] * apc[V.cf[q]];
+        diag += v;
+        if (V.cs[q] < 4) a4[V.cs[q]] += v;
+    }
+    const double apk = e + diag;
+    for (int d = 0; d < 4; d++) V.a[d * V.nk + c] = (float)a4[d];
+    V.ap[c] = (float)apk;
+    V.dinv[c] = apk > 0.0 ? (float)(1.0 / apk) : 0.0f;
+}
+
+// First sweep from zero on the finest cells, 0 on the other refined cells, into both buffers
+__global__ void k_tl_cs_zero(const int *ref, int nref, const uint8_t *lev, int top, const double *r,
+                             const double *ap, float *Za, float *Zb, const KryScal *sc) {
+    if (sc->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nref) return;
+    const int i = ref[t];
+    const float v = lev[i] == top ? TEL_OMEGA * (float)(r[i] / ap[i]) : 0.0f;
+    Za[i] = v;
+    Zb[i] = v;
+}
+// A damped Jacobi sweep on the finest cells: zout = zin + omega D^-1 (r - A zin)
+__global__ void k_tl_cs_fjac(TelG T, const int *top, int ntop, const double *r, const double *ap, const double *apc,
+                             const float *zin, float *zout, const KryScal *sc) {
+    if (sc->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntop) return;
+    const int i = top[t];
+    const double res = tl_res(T, r, ap, apc, zin, i, T.meta[i]);
+    zout[i] = zin[i] + TEL_OMEGA * (float)(res / ap[i]);
+}
+// Residual r - A Z summed into the level-k squares, and the first sweep from
+// zero there.  One thread per entry of kcell; blocks hold whole squares, and
+// the thread of a square's first entry sums its entries in order from LDS.
+__global__ void __launch_bounds__(BLOCK) k_tl_cs_restrict(TelG T, TelLev V, const double *r, const double *ap,
+                                                          const double *apc, const float *Z, const KryScal *sc) {
+    if (sc->done) return;
+    __shared__ double sr[BLOCK];
+    const int q0 = V.blk[blockIdx.x], q1 = V.blk[blockIdx.x + 1], q = q0 + threadIdx.x;
+    double ri = 0.0;
+    int c = 0;
+    if (q < q1) {
+        const int i = V.kcell[q];
+        ri = tl_res(T, r, ap, apc, Z, i, T.meta[i]);
+        c = V.kseg[q];
+    }
+    sr[threadIdx.x] = ri;
+    __syncthreads();
+    if (q >= q1 || V.kstart[c] != q) return;
+    double acc = 0.0;
+    for (int p = q; p < V.kstart[c + 1]; p++) acc += sr[p - q0];
+    V.r[c] = (float)acc;
+    V.z[0][c] = TEL_OMEGA * (float)acc * V.dinv[c];
+}
+// A damped Jacobi sweep on level k
+__global__ void k_tl_cs_jac(TelLev V, const float *zin, float *zout, const KryScal *sc) {
+    if (sc->done) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V.nk) return;
+    int nb[4];
+    float zn[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) nb[d] = V.nbr[d * V.nk + c];
+#pragma unroll
+    for (int d = 0; d < 4; d++) zn[d] = nb[d] >= 0 ? zin[nb[d]] : 0.0f;
+    float az = V.ap[c] * zin[c];
+#pragma unroll
+    for (int d = 0; d < 4; d++)
+        if (nb[d] >= 0) az -= V.a[d * V.nk + c] * zn[d];
+    zout[c] = zin[c] + TEL_OMEGA * (V.r[c] - az) * V.dinv[c];
+}
+// Adds the level-k correction to the cells inside its squares (Z1, and Z2 if given)
+__global__ void k_tl_cs_prolong(TelLev V, const float *zk, float *Z1, float *Z2, const KryScal *sc) {
+    if (sc->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= V.nge) return;
+    const int i = V.ge[t];
+    const float v = Z1[i] + zk[V.anc[i]];
+    Z1[i] = v;
+    if (Z2) Z2[i] = v;
+}
+// b of every base cell: the summed residual r - A Z of its cells.  One
+// thread per entry of the base-cell-ordered list bcells; blocks hold whole
+// base cells, and the thread of a base cell's first entry sums its entries in
+// order from LDS.
+__global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk, const double *r, const double *ap,
+                                                          const double *apc, const float *Z, const int *bcells,
+                                                          MgLevel L, float *b0, const KryScal *sc) {
+    if (sc->done) return;
+    __shared__ double sr[BLOCK];
+    const int q0 = blk[blockIdx.x], q1 = blk[blockIdx.x + 1], q = q0 + threadIdx.x;
+    int i = 0;
+    double ri = 0.0;
+    if (q < q1) {
+        i = bcells[q];
+        ri = tl_res(T, r, ap, apc, Z, i, T.meta[i]);
+    }
+    sr[threadIdx.x] = ri;
+    __syncthreads();
+    if (q >= q1) return;
+    const int b = T.base[i];
+    if (T.bstart[b] != q) return;
+    double acc = 0.0;
+    for (int p = q; p < T.bstart[b + 1]; p++) acc += sr[p - q0];
     b0[L.idx(b % T.nbx, b / T.nbx)] = (float)acc;
 }
-// z = z1 + P zb, then one damped Jacobi sweep on the refined cells
-__global__ void k_tl_mg_post(TelG T, const double *r, const double *ap, const double *apc, const float *z1,
-                             const float *zb, MgLevel L, float *z, const KryScal *sc) {
+// z = Z + P zb on every cell, into both buffers of the up sweeps
+__global__ void k_tl_cs_post0(TelG T, const float *Z, const float *zb, MgLevel L, float *za, float *zq,
+                              const KryScal *sc) {
     if (sc->done) return;
     int i;
     if (!tel_thread(T.n, i)) return;
-    auto z2 = [&](int c) {
-        const int b = T.base[c];
-        return z1[c] + zb[L.idx(b % T.nbx, b / T.nbx)];
-    };
-    const uint64_t m = T.meta[i];
-    const float zi = z2(i);
-    if (!m_smooth(m)) { z[i] = zi; return; }
-    double az = ap[i] * (double)zi;
-    for (int k = 0; k < m_ncf(m); k++) {
-        const int nb = tnb(T, i, k);
-        if (nb < T.n) az -= apc[k * T.nd + i] * (double)z2(nb);
-    }
-    z[i] = zi + TEL_OMEGA * (float)((r[i] - az) / ap[i]);
+    const int b = T.base[i];
+    const float v = Z[i] + zb[L.idx(b % T.nbx, b / T.nbx)];
+    za[i] = v;
+    zq[i] = v;
 }
 
 // Host-side state of a telescoping flow.
@@ -111,8 +246,16 @@ struct TelHost {
     int cu = 0, cv = 0, cpp = 0, cHu = 0, cHv = 0, cap = 0;   // current buffer of each pair
     KrySys<2> Kuv{};
     KrySys<1> Kpp{};
-    float *z1 = nullptr, *b0 = nullptr;   // smoothing output, base right-hand side
+    float *b0 = nullptr;                  // base right-hand side
+    float *zp[2] = {nullptr, nullptr};    // correction of the down sweeps (ping-pong; 0 on unrefined cells)
+    float *zq = nullptr;                  // second buffer of the up sweeps
     int *bcells = nullptr;
+    int *blkb = nullptr, nblkb = 0;       // blocks of the base restriction (whole base cells)
+    TelLev lv[TEL_MAX_LEVELS]{};          // levels 1 .. top-1
+    int top = 0;                          // finest level
+    int *ref = nullptr, nref = 0;         // refined cells (level >= 1)
+    int *topc = nullptr, ntop = 0;        // cells of the finest level
+    uint8_t *lev = nullptr;               // [n] level of each cell
     int nblk_n = 0, nblk_nd = 0, nblk_base = 0;
     std::vector<int> gown_h;              // owner of each ghost (host copy)
     int njoint = 0;
@@ -188,19 +331,58 @@ static int tel_grad_fric_vis(cms2d_flow *F) {
     return 0;
 }
 
+static unsigned tl_nblk(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+
 // z = M^-1 r of the pp system (r: FP64 Krylov vector)
 static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
     TelHost &H = *F->tel;
     const TelG &T = H.T;
     const KryScal *sc = H.Kpp.sc;
+    const double *ap = H.Kpp.ap_w[0], *apc = H.s.apc;
     const MgLevel &L0 = F->mg.L[0];
-    TLAUNCH(k_tl_mg_pre, H.nblk_n, T, r, (const double *)H.Kpp.ap_w[0], H.z1, sc);
-    TLAUNCH(k_tl_mg_restrict, H.nblk_base, T, r, (const double *)H.Kpp.ap_w[0], (const double *)H.s.apc,
-            (const float *)H.z1, (const int *)H.bcells, L0, H.b0, sc);
+    hipStream_t st = F->st;
+    int cur = 0;
+    if (H.nref) {
+        hipLaunchKernelGGL(k_tl_cs_zero, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, (const int *)H.ref, H.nref,
+                           (const uint8_t *)H.lev, H.top, r, ap, H.zp[0], H.zp[1], sc);
+        for (int q = 1; q < TEL_NU; q++, cur ^= 1)
+            hipLaunchKernelGGL(k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                               H.ntop, r, ap, apc, (const float *)H.zp[cur], H.zp[cur ^ 1], sc);
+    }
+    float *Z = H.zp[cur];
+    for (int k = H.top - 1; k >= 1; k--) {
+        const TelLev &V = H.lv[k];
+        hipLaunchKernelGGL(k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc, (const float *)Z,
+                           sc);
+        int cz = 0;
+        for (int q = 1; q < TEL_NU; q++, cz ^= 1)
+            hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
+                               V.z[cz ^ 1], sc);
+        hipLaunchKernelGGL(k_tl_cs_prolong, dim3(tl_nblk(V.nge)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz], Z,
+                           (float *)nullptr, sc);
+    }
+    hipLaunchKernelGGL(k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb, r, ap, apc,
+                       (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
     float *zb = L0.zo;
     mg_vcycle<float>(F, H.b0, zb, sc);
-    TLAUNCH(k_tl_mg_post, H.nblk_n, T, r, (const double *)H.Kpp.ap_w[0], (const double *)H.s.apc,
-            (const float *)H.z1, (const float *)zb, L0, z, sc);
+    TLAUNCH(k_tl_cs_post0, H.nblk_n, T, (const float *)Z, (const float *)zb, L0, z, H.zq, sc);
+    for (int k = 1; k < H.top; k++) {
+        const TelLev &V = H.lv[k];
+        hipLaunchKernelGGL(k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc, (const float *)z,
+                           sc);
+        int cz = 0;
+        for (int q = 1; q < TEL_NU; q++, cz ^= 1)
+            hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
+                               V.z[cz ^ 1], sc);
+        hipLaunchKernelGGL(k_tl_cs_prolong, dim3(tl_nblk(V.nge)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz], z,
+                           H.zq, sc);
+    }
+    if (H.ntop) {
+        float *zz[2] = {z, H.zq};
+        for (int q = 0; q < TEL_NU; q++)
+            hipLaunchKernelGGL(k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                               H.ntop, r, ap, apc, (const float *)zz[q & 1], zz[(q & 1) ^ 1], sc);
+    }
 }
 struct TelPre {
     cms2d_flow *F;
@@ -263,6 +445,9 @@ static int tel_flow_imp(cms2d_flow *F) {
             if (F->precond == CMS2D_PRECOND_MG) {
                 TLAUNCH(k_tl_mg_setup0, H.nblk_base, T, (const double *)s.apc, (const double *)H.Kpp.ap_w[0],
                         (const int *)H.bcells, F->mg.L[0]);
+                for (int k = 1; k < H.top; k++)
+                    TLAUNCH(k_tl_cs_setup, tl_nblk(H.lv[k].nk), T, H.lv[k], (const double *)s.apc,
+                            (const double *)H.Kpp.ap_w[0]);
                 for (int l = 0; l + 1 < F->mg.nlev; l++)
                     LAUNCH(k_mg_coarsen, mg_nblk(F->mg.L[l + 1]), F->mg.L[l], F->mg.L[l + 1]);
                 F->tm_active = X.tm;
@@ -432,6 +617,118 @@ static void tel_destroy(cms2d_flow *F) {
     F->tel = nullptr;
 }
 
+// Blocks of at most BLOCK entries over consecutive segments [st[s], st[s+1]),
+// each block holding whole segments (no segment is longer than BLOCK).
+static std::vector<int> tel_seg_blocks(const std::vector<int> &st, int nseg) {
+    std::vector<int> blk{0};
+    for (int s = 0; s < nseg; s++)
+        if (st[s + 1] - blk.back() > BLOCK) blk.push_back(st[s]);
+    blk.push_back(st[nseg]);
+    return blk;
+}
+
+// Host copy of a level of the refined region (see TelLev).
+struct TelLevHost {
+    int nk = 0;
+    std::vector<int> kstart, kcell, kseg, blk, nbr, cstart, cf, ge, anc;
+    std::vector<float> cw;
+    std::vector<uint8_t> cs;
+};
+
+// Levels of the cells (base cells split into 2^L x 2^L squares) and the
+// squares of levels 1 .. top-1.
+static int tel_levels(const TelTopo &t, const cms2d_flow_desc *d, std::vector<uint8_t> &lev, int &top,
+                      std::vector<TelLevHost> &lv) {
+    const int n = t.n, nd = t.nd;
+    lev.assign(n, 0);
+    top = 0;
+    for (int i = 0; i < n; i++) {
+        const int L = (int)std::lround(std::log2(d->dx / t.dx[i]));
+        if (L < 0 || L >= TEL_MAX_LEVELS || std::fabs(t.dx[i] * (1 << L) - d->dx) > 1e-6 * d->dx ||
+            std::fabs(t.dy[i] * (1 << L) - d->dy) > 1e-6 * d->dy)
+            return set_err(CMS2D_ERR_ARG, "cell " + std::to_string(i + 1) + " is not a 2^-L part (L < " +
+                                              std::to_string(TEL_MAX_LEVELS) + ") of its base cell");
+        lev[i] = (uint8_t)L;
+        top = std::max(top, L);
+    }
+    lv.assign(std::max(top, 1), TelLevHost{});
+    for (int k = 1; k < top; k++) {
+        TelLevHost &V = lv[k];
+        const double hx = d->dx / (1 << k), hy = d->dy / (1 << k);
+        const long long W = (long long)d->nx << k;
+        std::unordered_map<long long, int> id;
+        std::vector<int> ci, cj;
+        V.anc.assign(n, -1);
+        for (int i = 0; i < n; i++) {
+            if (lev[i] < k) continue;
+            const long long I = (long long)std::floor(t.x[i] / hx), J = (long long)std::floor(t.y[i] / hy);
+            auto it = id.find(J * W + I);
+            if (it == id.end()) {
+                it = id.emplace(J * W + I, V.nk++).first;
+                ci.push_back((int)I);
+                cj.push_back((int)J);
+            }
+            V.anc[i] = it->second;
+        }
+        const int nk = V.nk;
+        V.kstart.assign(nk + 1, 0);
+        for (int i = 0; i < n; i++)
+            if (V.anc[i] >= 0) V.kstart[V.anc[i] + 1]++;
+        for (int c = 0; c < nk; c++) V.kstart[c + 1] += V.kstart[c];
+        V.kcell.assign(V.kstart[nk], 0);
+        V.kseg.assign(V.kstart[nk], 0);
+        {
+            std::vector<int> fill(V.kstart.begin(), V.kstart.end() - 1);
+            for (int i = 0; i < n; i++)
+                if (V.anc[i] >= 0) {
+                    V.kseg[fill[V.anc[i]]] = V.anc[i];
+                    V.kcell[fill[V.anc[i]]++] = i;
+                    V.ge.push_back(i);
+                }
+        }
+        V.blk = tel_seg_blocks(V.kstart, nk);
+        V.nbr.assign((size_t)4 * nk, -1);
+        static const int di[4] = {0, 1, 0, -1}, dj[4] = {1, 0, -1, 0};
+        for (int c = 0; c < nk; c++)
+            for (int dd = 0; dd < 4; dd++) {
+                auto it = id.find((long long)(cj[c] + dj[dd]) * W + (ci[c] + di[dd]));
+                if (ci[c] + di[dd] >= 0 && ci[c] + di[dd] < W && it != id.end()) V.nbr[(size_t)dd * nk + c] = it->second;
+            }
+        V.cstart.assign(nk + 1, 0);
+        for (int c = 0; c < nk; c++) {
+            for (int q = V.kstart[c]; q < V.kstart[c + 1]; q++) {
+                const int i = V.kcell[q];
+                for (int kf = 0; kf < t.ncf[i]; kf++) {
+                    const size_t f = (size_t)kf * nd + i;
+                    const int nb = t.c2c[f];
+                    if (nb >= n || (lev[nb] >= k && V.anc[nb] == c)) continue;
+                    const int dd = t.dir[f];
+                    const bool ew = dd == KE || dd == KW;
+                    const double dnf = ew ? std::fabs(t.x[nb] - t.x[i]) : std::fabs(t.y[nb] - t.y[i]);
+                    double dnk;
+                    uint8_t side;
+                    if (lev[nb] >= k) {
+                        if (V.nbr[(size_t)dd * nk + c] != V.anc[nb])
+                            return set_err(CMS2D_ERR_ARG, "cell " + std::to_string(i + 1) +
+                                                              ": a neighbour outside the adjacent square");
+                        dnk = ew ? hx : hy;
+                        side = (uint8_t)dd;
+                    } else {
+                        const double xc = (ci[c] + 0.5) * hx, yc = (cj[c] + 0.5) * hy;
+                        dnk = ew ? std::fabs(t.x[nb] - xc) : std::fabs(t.y[nb] - yc);
+                        side = 4;
+                    }
+                    V.cf.push_back((int)f);
+                    V.cw.push_back((float)(dnf / dnk));
+                    V.cs.push_back(side);
+                }
+            }
+            V.cstart[c + 1] = (int)V.cf.size();
+        }
+    }
+    return 0;
+}
+
 // Checks of the description that need no device.
 static int tel_check(const cms2d_flow_desc *d, const cms2d_tel_grid *tg) {
     if (!tg->geom || !tg->nb || tg->ncells < 1 || (tg->nbnd > 0 && !tg->bnd_cells))
@@ -444,6 +741,7 @@ static int tel_check(const cms2d_flow_desc *d, const cms2d_tel_grid *tg) {
         return set_err(CMS2D_ERR_ARG, "telescoping grids run the FP64 BiCGStab form with a fixed time step");
     if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
         return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
+    if (d->serial_order) return set_err(CMS2D_ERR_ARG, "serial_order is built for Cartesian grids only");
     int nopen = 0, curve = 0;
     for (int k = 0; k < 4; k++) {
         if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC &&
@@ -518,6 +816,18 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     }
     std::vector<int> bcells(n), fill(cnt.begin(), cnt.end() - 1);
     for (int i = 0; i < n; i++) bcells[fill[base[i]]++] = i;
+    for (size_t b = 0; b < (size_t)d->nx * d->ny; b++)
+        if (cnt[b + 1] - cnt[b] > BLOCK) return set_err(CMS2D_ERR_ARG, "a base cell with more than 256 cells");
+    const std::vector<int> blkb = tel_seg_blocks(cnt, d->nx * d->ny);
+    std::vector<uint8_t> lev;
+    int top = 0;
+    std::vector<TelLevHost> lvh;
+    if (int rc = tel_levels(t, d, lev, top, lvh)) return rc;
+    std::vector<int> refc, topc;
+    for (int i = 0; i < n; i++) {
+        if (lev[i] >= 1) refc.push_back(i);
+        if (top >= 1 && lev[i] == top) topc.push_back(i);
+    }
     std::vector<uint64_t> meta(n, 0);
     std::vector<int> jl;
     for (int i = 0; i < n; i++) {
@@ -575,6 +885,41 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     if (int rc = tel_upload(F, &bas, base.data(), (size_t)n)) return rc;
     if (int rc = tel_upload(F, &bst, cnt.data(), cnt.size())) return rc;
     if (int rc = tel_upload(F, &H.bcells, bcells.data(), (size_t)n)) return rc;
+    if (int rc = tel_upload(F, &H.blkb, blkb.data(), blkb.size())) return rc;
+    H.nblkb = (int)blkb.size() - 1;
+    if (int rc = tel_upload(F, &H.lev, lev.data(), lev.size())) return rc;
+    if (int rc = tel_upload(F, &H.ref, refc.data(), refc.size())) return rc;
+    if (int rc = tel_upload(F, &H.topc, topc.data(), topc.size())) return rc;
+    H.nref = (int)refc.size();
+    H.ntop = (int)topc.size();
+    H.top = top;
+    for (int k = 1; k < top; k++) {
+        const TelLevHost &Vh = lvh[k];
+        TelLev &V = H.lv[k];
+        V.nk = Vh.nk;
+        V.nblk = (int)Vh.blk.size() - 1;
+        V.nge = (int)Vh.ge.size();
+        int *kst, *kce, *ksg, *bl, *nbr, *cst, *cff, *ge, *anc;
+        float *cww;
+        uint8_t *css;
+        if (int rc = tel_upload(F, &kst, Vh.kstart.data(), Vh.kstart.size())) return rc;
+        if (int rc = tel_upload(F, &kce, Vh.kcell.data(), Vh.kcell.size())) return rc;
+        if (int rc = tel_upload(F, &ksg, Vh.kseg.data(), Vh.kseg.size())) return rc;
+        if (int rc = tel_upload(F, &bl, Vh.blk.data(), Vh.blk.size())) return rc;
+        if (int rc = tel_upload(F, &nbr, Vh.nbr.data(), Vh.nbr.size())) return rc;
+        if (int rc = tel_upload(F, &cst, Vh.cstart.data(), Vh.cstart.size())) return rc;
+        if (int rc = tel_upload(F, &cff, Vh.cf.data(), Vh.cf.size())) return rc;
+        if (int rc = tel_upload(F, &cww, Vh.cw.data(), Vh.cw.size())) return rc;
+        if (int rc = tel_upload(F, &css, Vh.cs.data(), Vh.cs.size())) return rc;
+        if (int rc = tel_upload(F, &ge, Vh.ge.data(), Vh.ge.size())) return rc;
+        if (int rc = tel_upload(F, &anc, Vh.anc.data(), Vh.anc.size())) return rc;
+        V.kstart = kst; V.kcell = kce; V.kseg = ksg; V.blk = bl; V.nbr = nbr; V.cstart = cst; V.cf = cff;
+        V.cw = cww; V.cs = css; V.ge = ge; V.anc = anc;
+        if (int rc = dalloc(F, &V.a, (size_t)4 * V.nk)) return rc;
+        float **vf[] = {&V.ap, &V.dinv, &V.r, &V.z[0], &V.z[1]};
+        for (float **f : vf)
+            if (int rc = dalloc(F, f, (size_t)V.nk)) return rc;
+    }
     if (int rc = tel_upload(F, &jnt, jl.data(), jl.size())) return rc;
     if (int rc = tel_upload(F, &sc_, str_c.data(), str_c.size())) return rc;
     if (int rc = tel_upload(F, &sk_, str_k.data(), str_k.size())) return rc;
@@ -676,7 +1021,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         H.Kuv.sc = F->d_sc + 1;
         if (int rc = dalloc(F, &H.Kpp.zp, (size_t)nd)) return rc;
         if (int rc = dalloc(F, &H.Kpp.zs, (size_t)nd)) return rc;
-        if (int rc = dalloc(F, &H.z1, (size_t)nd)) return rc;
+        for (float **f : {&H.zp[0], &H.zp[1], &H.zq})
+            if (int rc = dalloc(F, f, (size_t)nd)) return rc;
     }
     if (int rc = mg_alloc(F, d->nx, d->ny)) return rc;
     if (int rc = dalloc(F, &F->mg.L[0].zo, (size_t)(d->nx + 2) * (d->ny + 2))) return rc;
