@@ -54,6 +54,7 @@ class FlowDesc(C.Structure):
         ("friction", C.c_int),
         ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
+        ("serial_order", C.c_int),
     ]
 
 

@@ -45,8 +45,14 @@ static int set_err(int code, const std::string &m) {
 // events stamped by its own dispatch; after the host synchronises, the
 // launches of iterations before the solve stopped (those that did work) are
 // added up.
-enum { TG_SPMV_PP = 0, TG_MG_DOWN0 = 1, TG_MG_UP0 = 2, TG_N = 3 };
-static const char *const TG_NAME[TG_N] = {"spmv_pp", "mg_down0", "mg_up0"};
+// mg_coarse is the span of a V-cycle's levels below level 0 (events recorded
+// on the stream before the first and after the last of those launches).
+enum {
+    TG_SPMV_PP = 0, TG_MG_DOWN0, TG_MG_UP0, TG_MG_COARSE, TG_TEL_RESTRICT, TG_TEL_FJAC, TG_TEL_CS_RESTRICT,
+    TG_N
+};
+static const char *const TG_NAME[TG_N] = {"spmv_pp", "mg_down0", "mg_up0", "mg_coarse",
+                                          "tel_restrict", "tel_fjac", "tel_cs_restrict"};
 struct KTimer {
     struct Rec {
         int group, it;
@@ -341,7 +347,7 @@ static void bicg_iter(const KryCtx &X, KrySys<NS, T> &K, Op A, Pre &pre, int i) 
     tlaunch(X.tm, TG_SPMV_PP, k_bicg_spmv<Op, NS, 1, PREC, T>, grid, blk, 0, X.st, A, K);
     hipLaunchKernelGGL((k_bicg_s<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, i, X.nblk);
     if (PREC) pre(1);
-    hipLaunchKernelGGL((k_bicg_spmv<Op, NS, 2, PREC, T>), grid, blk, 0, X.st, A, K);
+    tlaunch(X.tm, TG_SPMV_PP, k_bicg_spmv<Op, NS, 2, PREC, T>, grid, blk, 0, X.st, A, K);
     hipLaunchKernelGGL((k_bicg_r<Op, NS, PREC, T>), grid, blk, 0, X.st, A, K, i, X.nblk);
 }
 
@@ -443,11 +449,15 @@ static void mg_vcycle(cms2d_flow *F, const TB *b0, float *z0, const KryScal *sc)
         return;
     }
     mg_level0(F, tm, true, L0, b0, sc);
+    hipEvent_t ea, eb;
+    const bool span = tm && tm->take(TG_MG_COARSE, ea, eb);
+    if (span) (void)hipEventRecord(ea, F->st);
     for (int l = 1; l + 1 < M.nlev; l++)
         LAUNCH(k_mg_down_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
     hipLaunchKernelGGL(k_mg_coarsest<float>, dim3(1), dim3(MG_CT), lds, F->st, Lc, (const float *)Lc.b, sc);
     for (int l = M.nlev - 2; l >= 1; l--)
         LAUNCH(k_mg_up_pc<float>, mg_nblk(M.L[l + 1]), M.L[l], (const float *)M.L[l].b, M.L[l + 1], sc);
+    if (span) (void)hipEventRecord(eb, F->st);
     mg_level0(F, tm, false, L0, b0, sc);
 }
 
@@ -1154,7 +1164,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
     memset(F->h_sc, 0, 4 * sizeof(KryScal));
-    F->tm.pool.resize(2 * 256);
+    F->tm.pool.resize(2 * 2048);
     for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
     // inputs

@@ -1049,6 +1049,160 @@ __global__ void k_vis_corner(Geo g, St s) {
     s.visc[c] = CVISCON + vs / (cid + SMALL);
 }
 
+// ---------------------------------------------- the reference's cell order
+// With serial_order the three loops above follow the reference run on one
+// thread: cells in increasing number (the padded index keeps that order for
+// active cells; ghost cells are numbered after them), faces N, E, S, W, each
+// read seeing every write made before it.  The writes of a loop form a
+// recurrence along chains of cells; a pass evaluates every value from the
+// previous pass's values (the first pass from the values before the loop),
+// and passes repeat until one changes nothing, after which every value
+// follows its whole chain.  *flag is set by a pass that changed a value.
+
+__device__ __forceinline__ bool is_act(const Geo &g, int t) {
+    int i, j;
+    pad_ij(g, t, i, j);
+    return active_ij(g, i, j);
+}
+
+// Dry-to-wet start values (flow.F90:1477-1491), one pass of one sweep: the
+// neighbours that are dry-to-wet cells numbered below c give this sweep's
+// values (pu, pv, pvis of the previous pass), the others the values from
+// before the sweep.
+__global__ void k_d2w_serial(Geo g, St s, const double *pu, const double *pv, const double *pvis, double *nu,
+                             double *nv, double *nvis, int first, int *flag) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (!(s.iwet[c] == 1 && s.iwet1[c] == 0)) return;
+    double us = 0.0, vs = 0.0, viss = 0.0;
+    int ns = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int nb = nbr(g, c, k);
+        const bool now = !first && nb < c && is_act(g, nb) && s.iwet[nb] == 1 && s.iwet1[nb] == 0;
+        us += now ? pu[nb] : s.u[nb];
+        vs += now ? pv[nb] : s.v[nb];
+        viss += now ? pvis[nb] : s.vis[nb];
+        ns += s.iwet[nb];
+    }
+    const double den = (double)(float)ns + SMALL;
+    const double a = us / den / F4(1.25), b = vs / den / F4(1.25), v = viss / den;
+    nu[c] = a;
+    nv[c] = b;
+    nvis[c] = v;
+    if (!first && (a != pu[c] || b != pv[c] || v != pvis[c])) *flag = 1;
+}
+__global__ void k_d2w_serial_commit(Geo g, St s, const double *nu, const double *nv, const double *nvis) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (!(s.iwet[c] == 1 && s.iwet1[c] == 0)) return;
+    s.u[c] = nu[c]; s.v[c] = nv[c]; s.vis[c] = nvis[c];
+}
+
+// Eddy-viscosity fix-up next to dry cells (flow_eddyvis.f90:168-191): the
+// value written at face k of wet cell c (to c and to the dry or ghost cell
+// across it) from the cell o across the opposite face as it stands when the
+// loop reaches (c, k).  pf, nf: [4*npad] face-major values of the previous and
+// this pass.
+__device__ __forceinline__ bool fix_face(const Geo &g, const St &s, int c, int i, int j, int k) {
+    return s.iwet[nbr(g, c, k)] == 0 || nb_ghost(g, i, j, k);
+}
+__device__ __forceinline__ int last_fix_face(const Geo &g, const St &s, int c) {
+    int i, j;
+    pad_ij(g, c, i, j);
+    int kl = -1;
+    for (int k = 0; k < 4; k++)
+        if (fix_face(g, s, c, i, j, k)) kl = k;
+    return kl;
+}
+// vis of cell o when the loop is at (c, k), from the previous pass's face values
+__device__ double vis_serial_read(const Geo &g, const St &s, const double *pf, int o, int c, int k) {
+    const double *vb = s.visb;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    int oi, oj;
+    pad_ij(g, o, oi, oj);
+    if (active_ij(g, oi, oj) && s.iwet[o] == 1) {
+        if (o < c) {
+            const int kl = last_fix_face(g, s, o);
+            if (kl >= 0) return pf[kl * np + o];
+        }
+        return vb[o];
+    }
+    // a dry or ghost cell: the last of its wet neighbours' writes before (c, k)
+    const long long now = 4ll * c + k;
+    long long best = -1;
+    double val = vb[o];
+    for (int d = 0; d < 4; d++) {
+        const int mi = oi + (d == KE) - (d == KW), mj = oj + (d == KN) - (d == KS);
+        if (!active_ij(g, mi, mj)) continue;
+        const int m = (mj + 1) * g.nxp + mi + 1;
+        if (s.iwet[m] != 1) continue;
+        const int km = c_kopp[d];
+        const long long key = 4ll * m + km;
+        if (key < now && key > best) { best = key; val = pf[km * np + m]; }
+    }
+    return val;
+}
+__global__ void k_vis_fix_serial(Geo g, St s, const double *pf, double *nf, int first, int *flag) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (s.iwet[c] != 1) return;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    for (int k = 0; k < 4; k++) {
+        if (!fix_face(g, s, c, i, j, k)) continue;
+        const int o = nbr(g, c, c_kopp[k]);
+        const double r = first ? s.visb[o] : vis_serial_read(g, s, pf, o, c, k);
+        const double v = CVISCON + (r - CVISCON) / (1.0 + SMALL);
+        nf[k * np + c] = v;
+        if (!first && v != pf[k * np + c]) *flag = 1;
+    }
+}
+// vis after the loop from the settled face values: a wet cell keeps its last
+// face's value, a dry or ghost cell its last writer's (the highest-numbered
+// wet neighbour), every other cell its value before the loop.
+__global__ void k_vis_fix_serial_commit(Geo g, St s, const double *pf) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * g.nyp) return;
+    const size_t np = (size_t)g.nxp * g.nyp;
+    int i, j;
+    pad_ij(g, t, i, j);
+    double val = s.visb[t];
+    if (active_ij(g, i, j) && s.iwet[t] == 1) {
+        const int kl = last_fix_face(g, s, t);
+        if (kl >= 0) val = pf[kl * np + t];
+    } else if (active_ij(g, i, j) || ring_side(g, i, j) >= 0) {
+        long long best = -1;
+        for (int d = 0; d < 4; d++) {
+            const int mi = i + (d == KE) - (d == KW), mj = j + (d == KN) - (d == KS);
+            if (!active_ij(g, mi, mj)) continue;
+            const int m = (mj + 1) * g.nxp + mi + 1;
+            if (s.iwet[m] != 1) continue;
+            const int km = c_kopp[d];
+            const long long key = 4ll * m + km;
+            if (key > best) { best = key; val = pf[km * np + m]; }
+        }
+    }
+    s.vis[t] = val;
+}
+// Corner wet nodes (flow_eddyvis.f90:194-206): corner cells numbered below c
+// give their value of this loop (pc of the previous pass).
+__global__ void k_vis_corner_serial(Geo g, St s, const double *pc, double *nc, int first, int *flag) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    if (s.icorner[c] != 1) return;
+    double cid = 0.0, vs = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int nb = nbr(g, c, k);
+        const bool now = !first && nb < c && is_act(g, nb) && s.icorner[nb] == 1;
+        cid += s.iwet[nb];
+        vs += ((now ? pc[nb] : s.vis[nb]) - CVISCON) * s.iwet[nb];
+    }
+    const double v = CVISCON + vs / (cid + SMALL);
+    nc[c] = v;
+    if (!first && v != pc[c]) *flag = 1;
+}
+
 // interp_scal_cell2face(vis, ibc=3) -> visk on owned faces; commits corners.
 __device__ __forceinline__ double vis_final(const Geo &g, const St &s, int x) {
     int j = x / g.nxp - 1, i = x - (j + 1) * g.nxp - 1;

@@ -26,7 +26,8 @@ class FlowStep:
     telescoping grid when the case carries one (case.tel)."""
 
     def __init__(self, case, *, rtol=1e-10, atol=1e-12, kmax=4000, maxit=0,
-                 precision=_lib.PREC_FP64, device=0, precond=None, solver=None, dtvar=0):
+                 precision=_lib.PREC_FP64, device=0, precond=None, solver=None, dtvar=0,
+                 serial_order=0):
         L = _lib.lib()
         self.case = case
         keep = []
@@ -63,6 +64,7 @@ class FlowStep:
         d.precond = _lib.PRECOND_MG if precond is None else precond
         d.solver = _lib.SOLVER_BICGSTAB if solver is None else solver
         d.dtvar = dtvar
+        d.serial_order = serial_order
         d.latitude, d.coriolis_plane = case.latitude, case.coriolis_plane
         d.turbulence = {5: 0, 0: 1, 1: 2, 2: 3, 3: 4}[case.mturbul]   # CMS2D_TURB_* from mturbul
         d.friction = {2: 0, 1: 1, 3: 2, 4: 3}[case.mbedfric]           # CMS2D_FRIC_* from mbedfric

@@ -49,6 +49,7 @@ module cms2d_hip
         integer(c_int) :: wse_ntimes = 0, wse_order = 0
         integer(c_int) :: wse_adjust_off = 0
         real(c_double) :: wse_gradx = 0, wse_grady = 0
+        integer(c_int) :: serial_order = 0
     end type
 
     ! struct cms2d_step_info

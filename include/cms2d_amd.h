@@ -130,6 +130,12 @@ typedef struct {
     double wse_gradx, wse_grady;  /* WSE_GRADIENTS of the curve sides [m/m], grid axes: the level
                                      varies along the side from its middle cell
                                      (boundaries.F90:556-572,1892-1901)                        */
+    int serial_order;             /* 1: the three loops whose iterations read values earlier
+                                     iterations write (dry-to-wet start values, flow.F90:1477-1491;
+                                     eddy-viscosity fix-ups next to dry cells and at corners,
+                                     flow_eddyvis.f90:168-212) give the reference's one-thread
+                                     (cell order) result; 0: every read sees the values from
+                                     before the loop (independent of any order).  Cartesian grids */
 } cms2d_flow_desc;
 
 typedef struct {

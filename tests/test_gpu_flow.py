@@ -15,10 +15,11 @@ from test_reference_vectors import ALL
 # The WSE_GRADIENTS run drives strong flow along the wet/dry edge, where the
 # reference's OpenMP loops that read values other iterations write (the
 # dry-to-wet start values and the eddy-viscosity fix-ups) give an answer that
-# depends on their order: the race-free GPU semantics differ from the
-# reference's serial order by 9e-5 m/s there, above the bounds.  The oracle in
-# the reference's order matches that run; the GPU is checked against the oracle
-# in race-free mode instead (test_wse_gradient_matches_oracle).
+# depends on their order: the race-free default differs from the reference's
+# one-thread order by 9e-5 m/s there, above the bounds.  With serial_order=1
+# the GPU follows that order and matches the case's vectors
+# (test_gpu_serial_order.py); the race-free default is checked against the
+# race-free oracle (test_wse_gradient_matches_oracle).
 REF_CASES = [n for n in ALL if n != "tidal_wse_grad"]
 
 TOL_ETA = 1e-6   # m
