@@ -26,36 +26,95 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Kernel groups timed live (per launch, stream events stamped by the dispatch)
-# with their algorithmic bytes per active cell of one launch:
-#   spmv_pp   k_bicg_spmv MODE 1 (v = A z): read aN,aE,aS,aW,ap,r0 (FP64) and
-#             z (FP32), write v (FP64)                         6*8 + 4 + 8 = 60
-#   mg_down0  k_mg_down on level 0: read b (FP64), aN..aW, 1/ap (FP32), write
-#             z (FP32) and a quarter-size coarse b (FP32)   8 + 5*4 + 4 + 1 = 33
-#   mg_up0    k_mg_up on level 0: read b (FP64), aN..aW, 1/ap, z (FP32) and a
-#             quarter-size coarse z, write z (FP32)      8 + 6*4 + 1 + 4 = 37
+# Kernel groups timed live (per launch, stream events stamped by the dispatch;
+# mg_coarse is the span between events recorded on the stream around a
+# V-cycle's levels below level 0) with their algorithmic bytes per launch:
+#   spmv_pp   k_bicg_spmv MODE 1 (v = A z) and MODE 2 (t = A zs): read
+#             aN,aE,aS,aW,ap,r0 (FP64) and z (FP32), write v (FP64): 60 B per
+#             cell; on the cell list the face metadata word, per face the
+#             neighbour index and coefficient, ap, r0, z, v: 36 + 12 F B
+#   mg_down0  k_mg_down on level 0: read b (FP64, FP32 on telescoping grids),
+#             aN..aW, 1/ap (FP32), write z (FP32) and a quarter-size coarse b:
+#             8 + 5*4 + 4 + 1 = 33 B per base cell (29 with FP32 b)
+#   mg_up0    k_mg_up on level 0: read b, aN..aW, 1/ap, z and a quarter-size
+#             coarse z, write z: 8 + 6*4 + 1 + 4 = 37 B per base cell (33)
+#   mg_coarse the levels below level 0 of one V-cycle: down and up sweeps of
+#             level l read and write 29 + 33 = 62 B per level-l cell
+#   tel_restrict    residual r - A Z summed into the base cells: per cell the
+#             list entry, metadata, r, base cell and its range (28 B), on cells
+#             next to refined cells also ap, Z and per face index, coefficient
+#             and Z (12 + 16 F B), 4 B per base cell written
+#   tel_fjac  damped Jacobi sweep on the finest cells: list entry, metadata,
+#             r, ap, Z in and out, per face index and coefficient: 36 + 12 F B
+#   tel_cs_restrict residual summed into the 10 m squares: per cell of level
+#             >= 1 its entry, square, metadata, r, ap, Z and per face index,
+#             coefficient and Z (36 + 16 F B), per square 16 B
 # HBM bytes per active cell of one launch as measured by rocprofv3 counters
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a copy kernel of known
-# size) on this workload at 2000 x 1500; the commands and the table are in
+# size) on the uniform 2000 x 1500 workload; the commands and the table are in
 # profiles/traffic_tidal_2000x1500.md.
 TRAFFIC_PER_CELL = {"spmv_pp": (166.8 + 24.5) * 1e6 / 3.0e6,
                     "mg_down0": (114.3 + 15.2) * 1e6 / 3.0e6,
                     "mg_up0": (128.1 + 16.9) * 1e6 / 3.0e6}
-GROUPS = {
-    "spmv_pp": (60, "k_bicg_spmv<StencilOp,1,1,1> (pp system, v = A z, z from the multigrid cycle)"),
-    "mg_down0": (33, "k_mg_down<double> (multigrid level 0: red-black pre-smoothing + restriction)"),
-    "mg_up0": (37, "k_mg_up<double> (multigrid level 0: correction + red-black post-smoothing)"),
+# The same per launch on the telescoping workload at 2000 x 1500
+# (profiles/traffic_tel_2000x1500.md).
+TEL_TRAFFIC = {"spmv_pp": 341.6e6, "mg_down0": 111.8e6, "mg_up0": 128.3e6, "tel_restrict": 174.9e6,
+               "tel_cs_restrict": 74.0e6, "tel_fjac": 58.3e6}
+KERNEL_DESC = {
+    "spmv_pp": "k_bicg_spmv MODE 1 and 2 (pp system: v = A z, t = A zs, z from the preconditioner)",
+    "mg_down0": "k_mg_down (multigrid level 0: red-black pre-smoothing + restriction)",
+    "mg_up0": "k_mg_up (multigrid level 0: correction + red-black post-smoothing)",
+    "mg_coarse": "multigrid levels 1.. of a V-cycle (k_mg_down_pc, k_mg_coarsest, k_mg_up_pc; span)",
+    "tel_restrict": "k_tl_mg_restrict (residual summed into the base cells)",
+    "tel_fjac": "k_tl_cs_fjac (damped Jacobi sweep on the 5 m cells)",
+    "tel_cs_restrict": "k_tl_cs_restrict (residual summed into the 10 m squares)",
 }
-# The telescoping step: the pp SpMV on the cell list reads per active cell its
-# face metadata word, per face the neighbour index and coefficient, ap, r0 and
-# z (FP32), and writes v: 8 + 12 F + 8 + 8 + 4 + 8 = 36 + 12 F bytes with F
-# faces (4 on base-level cells: 84); the base-grid multigrid sweeps as above
-# per base cell (3.0 M at 2000 x 1500), counted here per active cell.
-TEL_GROUPS = {
-    "spmv_pp": (84, "k_bicg_spmv<TelOp,1,1,1> (pp system on the cell list, v = A z)"),
-    "mg_down0": (33 * 3.0 / 3.612, "k_mg_down<float> (base-grid multigrid level 0: pre-smoothing + restriction)"),
-    "mg_up0": (37 * 3.0 / 3.612, "k_mg_up<float> (base-grid multigrid level 0: correction + post-smoothing)"),
-}
+
+
+def mg_level_cells(nx, ny):
+    """Cells of the multigrid levels below level 0 (cell-centred 2x2
+    coarsening to at most 4096 cells, at least four levels)."""
+    out = []
+    while True:
+        nx, ny = (nx + 1) // 2, (ny + 1) // 2
+        out.append(nx * ny)
+        if nx * ny <= 4096 and len(out) >= 3:
+            return out
+
+
+def cartesian_groups(case):
+    n = case.nx * case.ny
+    coarse = 62 * sum(mg_level_cells(case.nx, case.ny)[:-1])
+    return {"spmv_pp": 60 * n, "mg_down0": 33 * n, "mg_up0": 37 * n, "mg_coarse": coarse}
+
+
+def tel_groups(case):
+    """Algorithmic bytes per launch of the timed groups on a telescoping grid."""
+    t = case.tel
+    nb = np.asarray(t.nb)
+    lev = np.asarray(t.level)
+    n = len(nb)
+    nbase = case.nx * case.ny
+    faces = (nb > 0).sum(axis=1) + ((nb[:, 0::2] == 0) & (nb[:, 1::2] == 0)).sum(axis=1)
+    refined = lev > 0
+    nbr = np.where(nb > 0, nb - 1, 0)
+    near = refined | (refined[nbr] & (nb > 0)).any(axis=1)
+    top = lev == lev.max() if lev.max() > 0 else np.zeros(n, bool)
+    squares = 0
+    if lev.max() >= 2:
+        g = np.asarray(t.geom)[lev >= 1]
+        h = case.dx / 2
+        squares = len(np.unique(np.floor(g[:, 1] / h) * (2 * case.nx) + np.floor(g[:, 0] / h)))
+    return {
+        "spmv_pp": int((36 + 12 * faces).sum()),
+        "mg_down0": 29 * nbase, "mg_up0": 33 * nbase,
+        "mg_coarse": 62 * sum(mg_level_cells(case.nx, case.ny)[:-1]),
+        "tel_restrict": int(28 * n + 4 * nbase + (12 + 16 * faces[near]).sum()),
+        "tel_fjac": int((36 + 12 * faces[top]).sum()),
+        "tel_cs_restrict": int((36 + 16 * faces[lev >= 1]).sum() + 16 * squares),
+    }
+
+
 # Whole-step byte model (FP64, per active cell): once per step 358 (static
 # sources, wet/dry, velface_eta, level rotation, mass balance); per outer
 # iteration 1156 (momentum assembly 193, three solve prologues 80+80+64, three
@@ -95,7 +154,7 @@ def cpu_baseline(sample):
     from pyoracle import Oracle  # checker library, used here only as the CPU baseline
     from cms2d_amd import cases
     nx, ny, steps = (int(v) for v in sample.split("x"))
-    c = cases.tidal(nx=nx, ny=ny, dx=20.0, seed=1)
+    c = cases.tidal_tel(nx=nx, ny=ny, dx=20.0, seed=1)
     o = Oracle(c, serial=1)
     o.step(1)                     # cold-start step, untimed
     t0 = time.perf_counter()
@@ -103,10 +162,10 @@ def cpu_baseline(sample):
     dt = time.perf_counter() - t0
     if rc != 0:
         return None
-    return {"value": nx * ny * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 oracle (ILU0-BiCGStab, rtol 1e-10, one thread) on the tidal workload "
-                      f"generated at {nx}x{ny} cells (dx 20 m, dt 30 s), {steps} steps after one "
-                      f"untimed cold-start step, {dt:.1f} s"}
+    return {"value": c.ncells * steps / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 oracle (ILU0-BiCGStab, rtol 1e-10, one thread) on the telescoping tidal workload "
+                      f"generated at {nx}x{ny} base cells ({c.ncells} cells, dt 30 s), {steps} steps after "
+                      f"one untimed cold-start step, {dt:.1f} s"}
 
 
 def cpu_reference(sample, threads):
@@ -122,7 +181,7 @@ def cpu_reference(sample, threads):
     if not refcase.available():
         return None
     nx, ny, steps = (int(v) for v in sample.split("x"))
-    c = cases.tidal(nx=nx, ny=ny, dx=20.0, seed=1)
+    c = cases.tidal_tel(nx=nx, ny=ny, dx=20.0, seed=1)
     with tempfile.TemporaryDirectory() as d:
         d = d + "/"
         refcase.write_case(c, d, "tidal")
@@ -142,11 +201,12 @@ def cpu_reference(sample, threads):
     if r.returncode != 0 or len(t) != steps + 1:
         return None
     el = sum(t[1:])
-    return {"value": nx * ny * steps / el, "unit": "cell-steps/s", "cores": threads, "kind": "reference",
+    return {"value": c.ncells * steps / el, "unit": "cell-steps/s", "cores": threads, "kind": "reference",
             "sample": f"the reference's flow_imp/flow_update (its Fortran compiled unmodified with flang -O2 "
-                      f"-fopenmp, single precision and BiCGStab-ILUT settings as shipped) on the tidal workload "
-                      f"generated at {nx}x{ny} cells, {steps} steps after one untimed cold-start step, "
-                      f"{threads} OpenMP threads, {el:.1f} s"}
+                      f"-fopenmp, single precision and BiCGStab-ILUT settings as shipped) on the telescoping "
+                      f"tidal workload generated at {nx}x{ny} base cells ({c.ncells} cells, read from a .tel "
+                      f"file), {steps} steps after one untimed cold-start step, {threads} OpenMP threads, "
+                      f"{el:.1f} s"}
 
 
 def fp32_form(a, case, g64, value64):
@@ -235,16 +295,19 @@ def measure(a, case, label, groups):
     elapsed = time.perf_counter() - t0
     progress(f"{label}: {a.steps} steps of {ncells} cells in {elapsed:.2f} s")
     kern = {}
-    for k, (bpc, desc) in groups.items():
+    for k, nbytes in groups.items():
         ms1, n1 = g.kernel_stats(k)
         ms0, n0 = st0[k]
         nl = n1 - n0
         tot = ms1 * n1 - ms0 * n0
         avg = tot / nl if nl else None
-        kern[k] = {"kernel": desc, "launches": nl, "total_ms": tot, "avg_launch_ms": avg,
-                   "bytes_per_launch": bpc * ncells,
-                   "achieved_GBs": bpc * ncells / (avg * 1e-3) / 1e9 if avg else None}
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+        kern[k] = {"kernel": KERNEL_DESC[k], "launches": nl, "total_ms": tot, "avg_launch_ms": avg,
+                   "share_of_step": tot / (elapsed * 1e3),
+                   "bytes_per_launch": nbytes,
+                   "achieved_GBs": nbytes / (avg * 1e-3) / 1e9 if avg else None}
+    # the roofline kernel: the single kernel with the largest total time
+    # (mg_coarse is a span of nine launches, reported beside it)
+    dom = max((k for k in kern if k != "mg_coarse"), key=lambda k: kern[k]["total_ms"])
     value = ncells * a.steps / elapsed
     model_b = (STEP_MODEL["per_step"] + STEP_MODEL["per_outer"] * outer / a.steps
                + STEP_MODEL["per_krylov_iter"] * kry.sum() / a.steps)
@@ -279,7 +342,7 @@ def main():
         progress("oracle CPU sample done")
     tel_case = cases.tidal_tel(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
     progress(f"telescoping grid built: {tel_case.ncells} cells")
-    g, T = measure(a, tel_case, "telescoping", TEL_GROUPS)
+    g, T = measure(a, tel_case, "telescoping", tel_groups(tel_case))
     g.close()
     D = T["kern"][T["dominant"]]
     achieved = D["achieved_GBs"]
@@ -313,7 +376,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": TEL_TRAFFIC.get(T["dominant"]) if (a.nx, a.ny) == (2000, 1500) else None,
             "avg_launch_ms": D["avg_launch_ms"],
             "launches": D["launches"],
             "bytes_per_launch": D["bytes_per_launch"],
@@ -324,7 +387,7 @@ def main():
     }
     if not a.no_cartesian:
         case = cases.tidal(nx=a.nx, ny=a.ny, dx=20.0, seed=1)
-        g, C = measure(a, case, "cartesian", GROUPS)
+        g, C = measure(a, case, "cartesian", cartesian_groups(case))
         Dc = C["kern"][C["dominant"]]
         out["cartesian_uniform"] = {
             "workload": f"the same physics on the uniform {a.nx}x{a.ny} grid of 20 m cells",
@@ -333,7 +396,8 @@ def main():
             "krylov_iters_per_step": C["krylov_iters_per_step"],
             "roofline": {"kernel": Dc["kernel"], "achieved": Dc["achieved_GBs"], "peak": HBM_PEAK_GBS,
                          "frac": Dc["achieved_GBs"] / HBM_PEAK_GBS if Dc["achieved_GBs"] else None,
-                         "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"], "timed_groups": C["kern"]},
+                         "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"] if C["dominant"] in TRAFFIC_PER_CELL
+                         else None, "timed_groups": C["kern"]},
             "step_model_roofline": C["step_model_roofline"]}
         if not a.no_fp32:
             out["fp32_refine"] = fp32_form(a, case, g, C["value"])

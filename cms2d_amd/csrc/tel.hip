@@ -341,19 +341,20 @@ static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
     const double *ap = H.Kpp.ap_w[0], *apc = H.s.apc;
     const MgLevel &L0 = F->mg.L[0];
     hipStream_t st = F->st;
+    KTimer *tm = F->tm_active;
     int cur = 0;
     if (H.nref) {
         hipLaunchKernelGGL(k_tl_cs_zero, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, (const int *)H.ref, H.nref,
                            (const uint8_t *)H.lev, H.top, r, ap, H.zp[0], H.zp[1], sc);
         for (int q = 1; q < TEL_NU; q++, cur ^= 1)
-            hipLaunchKernelGGL(k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
-                               H.ntop, r, ap, apc, (const float *)H.zp[cur], H.zp[cur ^ 1], sc);
+            tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                    H.ntop, r, ap, apc, (const float *)H.zp[cur], H.zp[cur ^ 1], sc);
     }
     float *Z = H.zp[cur];
     for (int k = H.top - 1; k >= 1; k--) {
         const TelLev &V = H.lv[k];
-        hipLaunchKernelGGL(k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc, (const float *)Z,
-                           sc);
+        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
+                (const float *)Z, sc);
         int cz = 0;
         for (int q = 1; q < TEL_NU; q++, cz ^= 1)
             hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
@@ -361,15 +362,15 @@ static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
         hipLaunchKernelGGL(k_tl_cs_prolong, dim3(tl_nblk(V.nge)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz], Z,
                            (float *)nullptr, sc);
     }
-    hipLaunchKernelGGL(k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb, r, ap, apc,
-                       (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
+    tlaunch(tm, TG_TEL_RESTRICT, k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb, r, ap,
+            apc, (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
     float *zb = L0.zo;
     mg_vcycle<float>(F, H.b0, zb, sc);
     TLAUNCH(k_tl_cs_post0, H.nblk_n, T, (const float *)Z, (const float *)zb, L0, z, H.zq, sc);
     for (int k = 1; k < H.top; k++) {
         const TelLev &V = H.lv[k];
-        hipLaunchKernelGGL(k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc, (const float *)z,
-                           sc);
+        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
+                (const float *)z, sc);
         int cz = 0;
         for (int q = 1; q < TEL_NU; q++, cz ^= 1)
             hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
@@ -380,8 +381,8 @@ static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
     if (H.ntop) {
         float *zz[2] = {z, H.zq};
         for (int q = 0; q < TEL_NU; q++)
-            hipLaunchKernelGGL(k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
-                               H.ntop, r, ap, apc, (const float *)zz[q & 1], zz[(q & 1) ^ 1], sc);
+            tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                    H.ntop, r, ap, apc, (const float *)zz[q & 1], zz[(q & 1) ^ 1], sc);
     }
 }
 struct TelPre {
@@ -1035,7 +1036,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
     memset(F->h_sc, 0, 4 * sizeof(KryScal));
-    F->tm.pool.resize(2 * 256);
+    F->tm.pool.resize(2 * 2048);
     for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));
     // inputs: bed (ghosts take their cell's), friction input, forcing, Coriolis

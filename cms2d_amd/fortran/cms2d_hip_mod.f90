@@ -61,7 +61,23 @@ module cms2d_hip
         real(c_double) :: kernel_ms = 0, dt = 0
     end type
 
+    ! struct cms2d_tel_grid: the content of a .tel file (read_tel)
+    type, bind(C), public :: cms2d_tel_grid
+        integer(c_int) :: ncells = 0
+        type(c_ptr) :: geom = c_null_ptr        ! real(c_double) geom(4, ncells): x, y, dx, dy
+        type(c_ptr) :: nb = c_null_ptr          ! integer(c_int) nb(8, ncells): N1 N2 E1 E2 S1 S2 W1 W2
+        type(c_ptr) :: bnd_cells = c_null_ptr   ! integer(c_int) bnd_cells(nbnd), 0-based
+        integer(c_int) :: nbnd = 0
+    end type
+
     interface
+        function cms2d_flow_create_tel(desc, grid, flow) bind(C, name='cms2d_flow_create_tel') result(rc)
+            import :: c_int, c_ptr, cms2d_flow_desc, cms2d_tel_grid
+            type(cms2d_flow_desc), intent(in) :: desc
+            type(cms2d_tel_grid), intent(in) :: grid
+            type(c_ptr), intent(out) :: flow
+            integer(c_int) :: rc
+        end function
         function cms2d_last_error() bind(C, name='cms2d_last_error') result(p)
             import :: c_ptr
             type(c_ptr) :: p
@@ -142,6 +158,7 @@ module cms2d_hip
     public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
     public :: cms2d_error_message, cms2d_device_count
     public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves, cms2d_flow_set_wind
+    public :: cms2d_flow_create_tel
 
 contains
 
