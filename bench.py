@@ -234,12 +234,25 @@ def fp32_form(a, case, g64, value64):
     area = case.dx * case.dy
     dvol = float(area * (g.get("h").sum() - g64.get("h").sum()))
     nstep = a.warmup + a.steps
+
+    def balance(m):
+        # one more step: volume change minus dt times the boundary inflow (m^3)
+        v0 = m.get("h").sum() * area
+        m.step(1)
+        f = m.get_face("flux").reshape(case.ny, case.nx, 4)
+        q = -(f[-1, :, 0].sum() + f[:, -1, 1].sum() + f[0, :, 2].sum() + f[:, 0, 3].sum())
+        return float(m.get("h").sum() * area - v0 - case.dt * q), float(case.dt * q)
+
+    d32, q32 = balance(g)
+    d64, q64 = balance(g64)
     g.close()
     return {"value": v32, "unit": "cell-steps/s", "ms_per_step": el * 1e3 / a.steps,
             "speedup_vs_fp64": v32 / value64, "pp_inner_krylov_iters_per_step": kry / a.steps,
             "error_vs_fp64_form": err, "wet_cells_differ": wdiff,
             "volume_difference_m3_after_steps": dvol, "steps_compared": nstep,
-            "volume_drift_m_per_step": abs(dvol) / (area * case.ncells) / nstep}
+            "volume_drift_m_per_step": abs(dvol) / (area * case.ncells) / nstep,
+            "mass_balance_one_step": {"fp32_form": {"dV_minus_dtQ_m3": d32, "dtQ_m3": q32},
+                                      "fp64_form": {"dV_minus_dtQ_m3": d64, "dtQ_m3": q64}}}
 
 
 def ensemble_form(a):

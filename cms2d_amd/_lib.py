@@ -37,6 +37,7 @@ EXPORTS = [
     "cms2d_flow_time", "cms2d_flow_kernel_stats", "cms2d_flow_tuning", "cms2d_flow_set_waves",
     "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
     "cms2d_ensemble_step", "cms2d_solve", "cms2d_solve_noalloc", "cms2d_flow_create_tel",
+    "cms2d_solve_adjoint",
 ]
 
 
@@ -107,6 +108,8 @@ def lib():
     L.cms2d_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double, C.c_double,
                               C.c_double, C.c_int, D, I]
     L.cms2d_solve_noalloc.argtypes = L.cms2d_solve.argtypes
+    L.cms2d_solve_adjoint.argtypes = [C.c_int, C.c_int, C.c_int, D, I, D, D, D, C.c_double, D, C.c_double,
+                                      C.c_double, C.c_int, D, D, D, D, I]
     _LIB = L
     return L
 

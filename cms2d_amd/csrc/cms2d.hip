@@ -1004,7 +1004,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
     F->dev = d->device;
     HIPCHK(hipSetDevice(F->dev));
-    HIPCHK(hipStreamCreate(&F->st));
+    HIPCHK(hipStreamCreateWithFlags(&F->st, hipStreamNonBlocking));
     {
         int open[4];
         for (int k = 0; k < 4; k++) {
@@ -1290,7 +1290,8 @@ extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info)
         info->kernel_ms = ms;
         info->dt = F->deltime;
         std::vector<uint8_t> w(F->npad);
-        HIPCHK(hipMemcpy(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
         int nw = 0;
         for (int j = 0; j < F->g.ny; j++)
             for (int i = 0; i < F->g.nx; i++) nw += w[(size_t)(j + 1) * F->g.nxp + i + 1];
@@ -1456,9 +1457,10 @@ extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     memcpy(q, &c, sizeof c);
     q += sizeof c;
     for (auto &a : F->state) {
-        HIPCHK(hipMemcpy(q, a.first, a.second, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(q, a.first, a.second, hipMemcpyDeviceToHost, F->st));
         q += a.second;
     }
+    HIPCHK(hipStreamSynchronize(F->st));
     *nbytes = need;
     return CMS2D_OK;
 }
@@ -1481,9 +1483,10 @@ extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes)
     HIPCHK(hipSetDevice(F->dev));
     HIPCHK(hipStreamSynchronize(F->st));
     for (auto &a : F->state) {
-        HIPCHK(hipMemcpy(a.first, q, a.second, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(a.first, q, a.second, hipMemcpyHostToDevice, F->st));
         q += a.second;
     }
+    HIPCHK(hipStreamSynchronize(F->st));
     F->P = c.P;
     F->timesecs = c.timesecs; F->rampdur = c.rampdur; F->ramp = c.ramp; F->dtimebeg = c.dtimebeg; F->deltime = c.deltime;
     for (int n = 0; n < 3; n++) { F->rmom[n] = c.rmom[n]; F->rmom0[n] = c.rmom0[n]; F->rmom1[n] = c.rmom1[n]; F->kry[n] = c.kry[n]; }
@@ -1523,14 +1526,16 @@ extern "C" int cms2d_flow_get(cms2d_flow *F, const char *name, double *out) {
     std::vector<double> h(F->npad);
     if (!strcmp(name, "iwet")) {
         std::vector<uint8_t> w(F->npad);
-        HIPCHK(hipMemcpy(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(w.data(), F->s.iwet, F->npad, hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
         for (int j = 0; j < g.ny; j++)
             for (int i = 0; i < g.nx; i++) out[(size_t)j * g.nx + i] = w[(size_t)(j + 1) * g.nxp + i + 1];
         return CMS2D_OK;
     }
     double *p = field_ptr(F, name);
     if (!p) return set_err(CMS2D_ERR_ARG, std::string("unknown field ") + name);
-    HIPCHK(hipMemcpy(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
     for (int j = 0; j < g.ny; j++)
         for (int i = 0; i < g.nx; i++) out[(size_t)j * g.nx + i] = h[(size_t)(j + 1) * g.nxp + i + 1];
     return CMS2D_OK;
@@ -1546,10 +1551,12 @@ extern "C" int cms2d_flow_set(cms2d_flow *F, const char *name, const double *in)
     const Geo &g = F->g;
     double *p = field_ptr(F, name);
     std::vector<double> h(F->npad);
-    HIPCHK(hipMemcpy(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(h.data(), p, F->npad * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
     for (int j = 0; j < g.ny; j++)
         for (int i = 0; i < g.nx; i++) h[(size_t)(j + 1) * g.nxp + i + 1] = in[(size_t)j * g.nx + i];
-    HIPCHK(hipMemcpy(p, h.data(), F->npad * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(p, h.data(), F->npad * sizeof(double), hipMemcpyHostToDevice, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
     // an initial condition: rebuild the state as at the start of a run
     return init_state(F);
 }
@@ -1567,8 +1574,9 @@ extern "C" int cms2d_flow_get_face(cms2d_flow *F, const char *name, double *out)
     else if (!strcmp(name, "visk")) { E = F->s.viskE; Nn = F->s.viskN; anti = false; }
     else return set_err(CMS2D_ERR_ARG, std::string("unknown face field ") + name);
     std::vector<double> he(F->npad), hn(F->npad);
-    HIPCHK(hipMemcpy(he.data(), E, F->npad * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hn.data(), Nn, F->npad * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(he.data(), E, F->npad * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipMemcpyAsync(hn.data(), Nn, F->npad * sizeof(double), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
     const double sg = anti ? -1.0 : 1.0;
     for (int j = 0; j < g.ny; j++)
         for (int i = 0; i < g.nx; i++) {
@@ -1619,6 +1627,9 @@ static int solve_check_args(int ncells, int ncellsD, int nmaxfaces, const double
     return 0;
 }
 
+static int solve_device(const SolveBufs &B, hipStream_t st, int ncells, int nmaxfaces, double relax, double rtol,
+                        double atol, int kmax, KryScal &h_sc, double bias = SMALL);
+
 // Uploads the system, runs the prologue and BiCGStab, downloads ss, res, phi.
 // The scalars are cleared by a kernel (no memset node), so the same sequence
 // serves both entry points.
@@ -1627,16 +1638,30 @@ static int solve_run(const SolveBufs &B, hipStream_t st, int ncells, int ncellsD
                      double *res, double *phi, double relax, double rtol, double atol, int kmax, double *rmom,
                      int *iters) {
     const size_t nD = ncellsD, n = ncells, nf = nmaxfaces;
-    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
     HIPCHK(hipMemcpyAsync(B.a, acoef, nf * nD * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(B.c2c, cell2cell, nf * nD * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(B.w, iwet, nD * sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(B.ss, ss, nD * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(B.sp, sp, nD * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(B.phi, phi, nD * sizeof(double), hipMemcpyHostToDevice, st));
+    KryScal h_sc{};
+    if (int rc = solve_device(B, st, ncells, nmaxfaces, relax, rtol, atol, kmax, h_sc)) return rc;
+    HIPCHK(hipMemcpyAsync(ss, B.ss, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(res, B.res, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(phi, B.phi, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (rmom) *rmom = h_sc.rmom;
+    if (iters) *iters = h_sc.final_iters;
+    return 0;
+}
+
+// The prologue and BiCGStab on a system already in B (device memory).
+static int solve_device(const SolveBufs &B, hipStream_t st, int ncells, int nmaxfaces, double relax, double rtol,
+                        double atol, int kmax, KryScal &h_sc, double bias) {
+    const size_t n = ncells;
+    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, (unsigned int *)B.sc,
                        (int)(sizeof(KryScal) / sizeof(unsigned int)));
-    KryScal h_sc{};
     KrySys<1> K{};
     K.x[0] = B.phi;
     K.r[0] = B.vec; K.r0[0] = B.vec + n; K.p[0] = B.vec + 2 * n; K.v[0] = B.vec + 3 * n; K.t[0] = B.vec + 4 * n;
@@ -1646,17 +1671,12 @@ static int solve_run(const SolveBufs &B, hipStream_t st, int ncells, int ncellsD
     K.red.pr[0] = B.partials; K.red.pv[0] = B.partials + 2 * nblk; K.red.pt[0] = B.partials + 3 * nblk;
     EllOp A{ncells, nmaxfaces, B.c2c, B.a};
     hipLaunchKernelGGL(k_ell_prologue, dim3(nblk), dim3(KBLOCK), 0, st, A, (const int *)B.w, B.ss,
-                       (const double *)B.sp, B.res, (const double *)B.phi, relax, K);
+                       (const double *)B.sp, B.res, (const double *)B.phi, relax, K, bias);
     int prev = 8;
     KryCtx X{st, nblk, kmax > 0 ? kmax : 4000, rtol > 0 ? rtol : 1e-10, atol >= 0 ? atol : 1e-12,
              B.sc, &h_sc, &prev, nullptr};
     if (int rc = run_bicg(X, K, A, 0)) return rc;
-    HIPCHK(hipMemcpyAsync(ss, B.ss, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(res, B.res, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(phi, B.phi, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    if (rmom) *rmom = h_sc.rmom;
-    if (iters) *iters = h_sc.final_iters;
     return 0;
 }
 
@@ -1697,6 +1717,173 @@ extern "C" int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double 
     if (hipStreamCreate(&st) != hipSuccess) rc = set_err(CMS2D_ERR_HIP, "stream");
     else rc = solve_run(B, st, ncells, ncellsD, nmaxfaces, acoef, cell2cell, iwet, ss, sp, res, phi, relax, rtol,
                         atol, kmax, rmom, iters);
+    if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
+    cleanup();
+    return rc;
+}
+
+// ----------------------------------------------------------------------------
+// cms2d_solve_adjoint: the reverse of solve.  The solved system on the active
+// cells is A phi = b with A_ii = ap_i = (sum_k a_ik - sp_i) / relax,
+// A_ij = -a_ik for an active neighbour j, and b_i = ss_i + (1 - relax) ap_i
+// phi_old_i + sum over ghost neighbours of a_ik phi_g.  For a loss L(phi) the
+// adjoint lambda = A^-T dL/dphi gives every input's gradient in closed form.
+// A^T has the structure of A (faces are shared by the two cells): its row i
+// holds a_jb of each active neighbour j at the slot of j that points back to
+// i, so the transposed solve runs the same kernels on a system written by
+// k_adj_setup (relax 1, sp chosen so that its diagonal is ap).
+
+__global__ void k_adj_setup(EllOp A, const int *back, const double *sp, double relax, const double *gphi,
+                            double *at, double *spt, double *sst, double *lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.ncells) return;
+    const int nf = A.nf;
+    double asum = 0.0, tsum = 0.0;
+    for (int k = 0; k < nf; k++) {
+        const int j = A.c2c[nf * i + k];
+        if (j < 0) continue;
+        asum += A.a[nf * i + k];
+        const double v = j < A.ncells ? A.a[nf * j + back[nf * i + k]] : 0.0;
+        at[nf * i + k] = v;
+        tsum += v;
+    }
+    const double ap = (asum - sp[i]) / relax;
+    spt[i] = tsum - ap;
+    sst[i] = gphi[i];
+    lam[i] = 0.0;
+}
+// Gradients of the active cells' inputs from lambda.
+__global__ void k_adj_grads(EllOp A, const double *sp, const double *phi_old, const double *phi, double relax,
+                            const double *lam, double *gss, double *gsp, double *ga, double *gpo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.ncells) return;
+    const int nf = A.nf;
+    double asum = 0.0;
+    for (int k = 0; k < nf; k++)
+        if (A.c2c[nf * i + k] >= 0) asum += A.a[nf * i + k];
+    const double ap = (asum - sp[i]) / relax, l = lam[i], rm = 1.0 - relax;
+    gss[i] = l;
+    gpo[i] = l * rm * ap;
+    gsp[i] = l * (phi[i] - rm * phi_old[i]) / relax;
+    const double base = (rm * phi_old[i] - phi[i]) / relax;
+    for (int k = 0; k < nf; k++) {
+        const int j = A.c2c[nf * i + k];
+        ga[nf * i + k] = j < 0 ? 0.0 : l * (phi[j] + base);
+    }
+}
+// Gradient of each ghost value: the sum of lambda_i a_ik over the faces that
+// point to it, in the order of the list gref (flat face indices i*nf+k).
+__global__ void k_adj_ghost(EllOp A, int nghost, const int *gstart, const int *gref, const double *lam,
+                            double *gpo) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nghost) return;
+    double s = 0.0;
+    for (int q = gstart[g]; q < gstart[g + 1]; q++) s += lam[gref[q] / A.nf] * A.a[gref[q]];
+    gpo[A.ncells + g] = s;
+}
+
+extern "C" int cms2d_solve_adjoint(int ncells, int ncellsD, int nmaxfaces, const double *acoef,
+                                   const int *cell2cell, const double *sp, const double *phi_old, const double *phi,
+                                   double relax, const double *gphi, double rtol, double atol, int kmax,
+                                   double *gss, double *gsp, double *gacoef, double *gphi_old, int *iters) {
+    if (ncells <= 0 || ncellsD < ncells || !acoef || !cell2cell || !sp || !phi_old || !phi || !gphi || !gss ||
+        !gsp || !gacoef || !gphi_old || !(relax > 0.0))
+        return set_err(CMS2D_ERR_ARG, "invalid argument");
+    if (nmaxfaces < 1 || nmaxfaces > ELL_MAX_FACES)
+        return set_err(CMS2D_ERR_ARG, "nmaxfaces must be 1.." + std::to_string(ELL_MAX_FACES));
+    const size_t nD = ncellsD, n = ncells, nf = nmaxfaces, ng = nD - n;
+    // the slot of each active neighbour that points back, and the faces into each ghost
+    std::vector<int> back(nf * n, 0), gcnt(ng + 1, 0);
+    for (size_t i = 0; i < n; i++)
+        for (size_t k = 0; k < nf; k++) {
+            const int j = cell2cell[nf * i + k];
+            if (j < -1 || j >= ncellsD) return set_err(CMS2D_ERR_ARG, "cell2cell out of range");
+            if (j < 0) continue;
+            if ((size_t)j >= n) { gcnt[j - n + 1]++; continue; }
+            int b = -1;
+            for (size_t kk = 0; kk < nf; kk++)
+                if (cell2cell[nf * j + kk] == (int)i) { b = (int)kk; break; }
+            if (b < 0)
+                return set_err(CMS2D_ERR_ARG, "cell " + std::to_string(j + 1) + " does not list its neighbour " +
+                                                  std::to_string(i + 1) + " (faces must be shared)");
+            back[nf * i + k] = b;
+        }
+    for (size_t g = 0; g < ng; g++) gcnt[g + 1] += gcnt[g];
+    std::vector<int> gref(std::max<int>(gcnt[ng], 1)), fill(gcnt.begin(), gcnt.end() - 1);
+    for (size_t i = 0; i < n; i++)
+        for (size_t k = 0; k < nf; k++) {
+            const int j = cell2cell[nf * i + k];
+            if (j >= (int)n) gref[fill[j - n]++] = (int)(nf * i + k);
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(CMS2D_ERR_NODEVICE, "no HIP device");
+    std::vector<void *> mem;
+    auto cleanup = [&]() { for (void *p : mem) (void)hipFree(p); };
+    auto alloc = [&](size_t bytes) -> void * {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+        mem.push_back(p);
+        return p;
+    };
+    const int nblk = std::min(KRY_BLOCKS, (ncells + BLOCK - 1) / BLOCK);
+    double *a = (double *)alloc(nf * nD * sizeof(double)), *spd = (double *)alloc(nD * sizeof(double));
+    double *po = (double *)alloc(nD * sizeof(double)), *ph = (double *)alloc(nD * sizeof(double));
+    double *gp = (double *)alloc(nD * sizeof(double));
+    double *o_gss = (double *)alloc(nD * sizeof(double)), *o_gsp = (double *)alloc(nD * sizeof(double));
+    double *o_ga = (double *)alloc(nf * nD * sizeof(double)), *o_gpo = (double *)alloc(nD * sizeof(double));
+    int *c2c = (int *)alloc(nf * nD * sizeof(int)), *bk = (int *)alloc(nf * n * sizeof(int));
+    int *gs = (int *)alloc((ng + 1) * sizeof(int)), *gr = (int *)alloc(gref.size() * sizeof(int));
+    SolveBufs B{};
+    B.a = (double *)alloc(nf * nD * sizeof(double));
+    B.c2c = c2c;
+    B.w = (int *)alloc(nD * sizeof(int));
+    B.ss = (double *)alloc(nD * sizeof(double));
+    B.sp = (double *)alloc(nD * sizeof(double));
+    B.res = (double *)alloc(nD * sizeof(double));
+    B.phi = (double *)alloc(nD * sizeof(double));
+    B.vec = (double *)alloc(7 * n * sizeof(double));
+    B.partials = (double *)alloc((size_t)nblk * 5 * sizeof(double));
+    B.sc = (KryScal *)alloc(sizeof(KryScal));
+    for (void *p : mem)
+        if (!p) { cleanup(); return set_err(CMS2D_ERR_HIP, "device allocation failed"); }
+    hipStream_t st = nullptr;
+    int rc = 0;
+    auto run = [&]() -> int {
+        HIPCHK(hipStreamCreate(&st));
+        HIPCHK(hipMemcpyAsync(a, acoef, nf * nD * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c2c, cell2cell, nf * nD * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(bk, back.data(), nf * n * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(gs, gcnt.data(), (ng + 1) * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(gr, gref.data(), gref.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(spd, sp, nD * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(po, phi_old, nD * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ph, phi, nD * sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(gp, gphi, n * sizeof(double), hipMemcpyHostToDevice, st));
+        const EllOp A{ncells, nmaxfaces, c2c, a};
+        const unsigned nb = (unsigned)((n + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL(k_adj_setup, dim3(nb), dim3(BLOCK), 0, st, A, (const int *)bk, (const double *)spd, relax,
+                           (const double *)gp, B.a, B.sp, B.ss, B.phi);
+        hipLaunchKernelGGL(k_set_int, dim3(nb), dim3(BLOCK), 0, st, B.w, ncells, 1);
+        KryScal h_sc{};
+        if (int r = solve_device(B, st, ncells, nmaxfaces, 1.0, rtol, atol, kmax, h_sc, 0.0)) return r;
+        hipLaunchKernelGGL(k_adj_grads, dim3(nb), dim3(BLOCK), 0, st, A, (const double *)spd, (const double *)po,
+                           (const double *)ph, relax, (const double *)B.phi, o_gss, o_gsp, o_ga, o_gpo);
+        if (ng)
+            hipLaunchKernelGGL(k_adj_ghost, dim3((unsigned)((ng + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, A, (int)ng,
+                               (const int *)gs, (const int *)gr, (const double *)B.phi, o_gpo);
+        HIPCHK(hipMemcpyAsync(gss, o_gss, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(gsp, o_gsp, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(gacoef, o_ga, nf * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(gphi_old, o_gpo, nD * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t i = n; i < nD; i++) {        // ghost rows take no part in the system
+            gss[i] = gsp[i] = 0.0;
+            for (size_t k = 0; k < nf; k++) gacoef[nf * i + k] = 0.0;
+        }
+        if (iters) *iters = h_sc.final_iters;
+        return 0;
+    };
+    rc = run();
     if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
     cleanup();
     return rc;

@@ -272,9 +272,12 @@ __global__ void k_zero_words(unsigned int *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
 
+// bias: the reference's small number added to the starting residual
+// (pbicgstab.f90), 0 for the transposed solve of the adjoint, which must stay
+// linear in its right-hand side.
 __global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwet, double *ss, const double *sp,
                                                         double *res, const double *phi, double relax,
-                                                        KrySys<1> K) {
+                                                        KrySys<1> K, double bias) {
     double red[2] = {0.0, 0.0};
     const Chunk ch = block_chunk(A.ncells);
     const double rinv = 1.0 / relax, relaxm = 1.0 - relax;
@@ -294,7 +297,7 @@ __global__ void __launch_bounds__(KBLOCK) k_ell_prologue(EllOp A, const int *iwe
         ss[c] = ssv;
         double rs = iwet[c] * (anb + ssv - ap * ph) / ap;
         res[c] = rs;
-        double r = anb + ssv - ap * ph + SMALL;
+        double r = anb + ssv - ap * ph + bias;
         K.ap_w[0][c] = ap;
         K.dinv_w[0][c] = 1.0 / ap;
         K.r[0][c] = r; K.r0[0][c] = r;

@@ -49,14 +49,19 @@ struct TelLev {
     float *z[2];                  // [nk] ping-pong
 };
 
-// Base-grid level 0 of the multigrid hierarchy from the cell-list pp system.
-__global__ void k_tl_mg_setup0(TelG T, const double *apc, const double *ap, const int *bcells, MgLevel L) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= T.nbx * T.nby) return;
-    const int bi = b % T.nbx, bj = b / T.nbx;
-    double a4[4] = {0.0, 0.0, 0.0, 0.0}, e = 0.0;
-    for (int q = T.bstart[b]; q < T.bstart[b + 1]; q++) {
+// Base-grid level 0 of the multigrid hierarchy from the cell-list pp system
+// (once per outer iteration).  One thread per entry of the base-cell-ordered
+// list on blocks of whole base cells; the thread of a base cell's first entry
+// sums the contributions of its cells in list order from LDS.
+__global__ void __launch_bounds__(BLOCK) k_tl_mg_setup0(TelG T, const int *blk, const double *apc, const double *ap,
+                                                        const int *bcells, MgLevel L) {
+    __shared__ double sv[5][BLOCK];
+    const int q0 = blk[blockIdx.x], q1 = blk[blockIdx.x + 1], q = q0 + threadIdx.x;
+    double c4[4] = {0.0, 0.0, 0.0, 0.0}, e = 0.0;
+    int b = 0;
+    if (q < q1) {
         const int i = bcells[q];
+        b = T.base[i];
         const uint64_t m = T.meta[i];
         double sa = 0.0;
         for (int k = 0; k < m_ncf(m); k++) {
@@ -67,17 +72,26 @@ __global__ void k_tl_mg_setup0(TelG T, const double *apc, const double *ap, cons
             if (T.base[nb] == b) continue;
             const int d = m_dir(m, k);
             const double dn = (d == KE || d == KW) ? fabs(T.x[nb] - T.x[i]) : fabs(T.y[nb] - T.y[i]);
-            a4[d] += a * dn / ((d == KE || d == KW) ? T.dxb : T.dyb);
+            c4[d] += a * dn / ((d == KE || d == KW) ? T.dxb : T.dyb);
         }
-        e += fmax(ap[i] - sa, 0.0);
+        e = fmax(ap[i] - sa, 0.0);
     }
-    const int c = L.idx(bi, bj);
-    const double apb = e + ((a4[KN] + a4[KE]) + (a4[KS] + a4[KW]));
+    for (int d = 0; d < 4; d++) sv[d][threadIdx.x] = c4[d];
+    sv[4][threadIdx.x] = e;
+    __syncthreads();
+    if (q >= q1 || T.bstart[b] != q) return;
+    double a4[4] = {0.0, 0.0, 0.0, 0.0}, es = 0.0;
+    for (int p = q; p < T.bstart[b + 1]; p++) {
+        for (int d = 0; d < 4; d++) a4[d] += sv[d][p - q0];
+        es += sv[4][p - q0];
+    }
+    const int c = L.idx(b % T.nbx, b / T.nbx);
+    const double apb = es + ((a4[KN] + a4[KE]) + (a4[KS] + a4[KW]));
     L.aN[c] = (float)a4[KN];
     L.aE[c] = (float)a4[KE];
     L.aS[c] = (float)a4[KS];
     L.aW[c] = (float)a4[KW];
-    L.e[c] = (float)e;
+    L.e[c] = (float)es;
     L.ap[c] = (float)apb;
     L.dinv[c] = apb > 0.0 ? (float)(1.0 / apb) : 0.0f;
 }
@@ -256,6 +270,8 @@ struct TelHost {
     int *ref = nullptr, nref = 0;         // refined cells (level >= 1)
     int *topc = nullptr, ntop = 0;        // cells of the finest level
     uint8_t *lev = nullptr;               // [n] level of each cell
+    TelTiles tiles{};                     // ponding tiles
+    int ntile = 0, tile_max = 0;
     int nblk_n = 0, nblk_nd = 0, nblk_base = 0;
     std::vector<int> gown_h;              // owner of each ghost (host copy)
     int njoint = 0;
@@ -288,8 +304,9 @@ static int tel_wetdry(cms2d_flow *F) {
         if (F->h_flags[0] == 0) break;
         if (it > 100000) return set_err(CMS2D_ERR_HIP, "narrow-channel pass did not settle");
     }
-    TLAUNCH(k_tl_pond_init, H.nblk_nd, T, s);
-    TLAUNCH(k_tl_pond_union, H.nblk_n, T, s);
+    hipLaunchKernelGGL(k_tl_pond_local, dim3(H.ntile), dim3(1024), (size_t)H.tile_max * sizeof(int), F->st, T, s,
+                       H.tiles);
+    TLAUNCH(k_tl_pond_merge, H.nblk_n, T, s, H.tiles);
     TLAUNCH(k_tl_pond_compress, H.nblk_nd, T, s);
     TLAUNCH(k_tl_pond_mark, (T.nstr + BLOCK - 1) / BLOCK, T, s);
     TLAUNCH(k_tl_pond_final, H.nblk_nd, T, s);
@@ -444,8 +461,8 @@ static int tel_flow_imp(cms2d_flow *F) {
         {
             KryCtx X = flow_ctx(F, true);
             if (F->precond == CMS2D_PRECOND_MG) {
-                TLAUNCH(k_tl_mg_setup0, H.nblk_base, T, (const double *)s.apc, (const double *)H.Kpp.ap_w[0],
-                        (const int *)H.bcells, F->mg.L[0]);
+                TLAUNCH(k_tl_mg_setup0, H.nblkb, T, (const int *)H.blkb, (const double *)s.apc,
+                        (const double *)H.Kpp.ap_w[0], (const int *)H.bcells, F->mg.L[0]);
                 for (int k = 1; k < H.top; k++)
                     TLAUNCH(k_tl_cs_setup, tl_nblk(H.lv[k].nk), T, H.lv[k], (const double *)s.apc,
                             (const double *)H.Kpp.ap_w[0]);
@@ -854,7 +871,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
 
     F->dev = d->device;
     HIPCHK(hipSetDevice(F->dev));
-    HIPCHK(hipStreamCreate(&F->st));
+    HIPCHK(hipStreamCreateWithFlags(&F->st, hipStreamNonBlocking));
     F->tel = new (std::nothrow) TelHost();
     if (!F->tel) return set_err(CMS2D_ERR_ARG, "out of host memory");
     TelHost &H = *F->tel;
@@ -889,6 +906,40 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     if (int rc = tel_upload(F, &H.blkb, blkb.data(), blkb.size())) return rc;
     H.nblkb = (int)blkb.size() - 1;
     if (int rc = tel_upload(F, &H.lev, lev.data(), lev.size())) return rc;
+    {
+        // ponding tiles: squares of base cells, the side halved until a tile
+        // holds at most 16384 cells (its labels fit in LDS)
+        int side = 32, ntx = 0, nty = 0, mx = 0;
+        std::vector<int> tid(nd), tcnt;
+        for (;; side /= 2) {
+            ntx = (d->nx + side - 1) / side;
+            nty = (d->ny + side - 1) / side;
+            tcnt.assign((size_t)ntx * nty + 1, 0);
+            for (int i = 0; i < nd; i++) {
+                const int b = base[i < n ? i : t.gown[i - n]];
+                tid[i] = (b / d->nx / side) * ntx + (b % d->nx) / side;
+                tcnt[tid[i] + 1]++;
+            }
+            mx = *std::max_element(tcnt.begin(), tcnt.end());
+            if (mx <= 16384 || side == 1) break;
+        }
+        if (mx > 16384) return set_err(CMS2D_ERR_ARG, "a base cell with more than 16384 cells");
+        const int nt = ntx * nty;
+        for (int q = 0; q < nt; q++) tcnt[q + 1] += tcnt[q];
+        std::vector<int> tcell(nd), loc(nd), fill(tcnt.begin(), tcnt.end() - 1);
+        for (int i = 0; i < nd; i++) {
+            loc[i] = fill[tid[i]] - tcnt[tid[i]];
+            tcell[fill[tid[i]]++] = i;
+        }
+        int *a1, *a2, *a3, *a4;
+        if (int rc = tel_upload(F, &a1, tcnt.data(), tcnt.size())) return rc;
+        if (int rc = tel_upload(F, &a2, tcell.data(), tcell.size())) return rc;
+        if (int rc = tel_upload(F, &a3, tid.data(), tid.size())) return rc;
+        if (int rc = tel_upload(F, &a4, loc.data(), loc.size())) return rc;
+        H.tiles = TelTiles{a1, a2, a3, a4};
+        H.ntile = nt;
+        H.tile_max = std::max(mx, 1);
+    }
     if (int rc = tel_upload(F, &H.ref, refc.data(), refc.size())) return rc;
     if (int rc = tel_upload(F, &H.topc, topc.data(), topc.size())) return rc;
     H.nref = (int)refc.size();
@@ -1050,8 +1101,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         double *in = P.mbedfric == 1 ? s.cfrict : P.mbedfric == 3 ? s.z0 : P.mbedfric == 4 ? s.cbf : s.coefman;
         HIPCHK(hipMemcpyAsync(in, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, F->st));
         HIPCHK(hipStreamSynchronize(F->st));
-        if (d->wavestrx) HIPCHK(hipMemcpy(s.wavestrx, d->wavestrx, n * sizeof(double), hipMemcpyHostToDevice));
-        if (d->wavestry) HIPCHK(hipMemcpy(s.wavestry, d->wavestry, n * sizeof(double), hipMemcpyHostToDevice));
+        if (d->wavestrx) HIPCHK(hipMemcpyAsync(s.wavestrx, d->wavestrx, n * sizeof(double), hipMemcpyHostToDevice, F->st));
+        if (d->wavestry) HIPCHK(hipMemcpyAsync(s.wavestry, d->wavestry, n * sizeof(double), hipMemcpyHostToDevice, F->st));
         const double pi = std::acos(-1.0), omega = 2.0 * pi / 86400.0, deg2rad = pi / 180.0;
         const double f0 = 2.0 * omega * std::sin(d->latitude * deg2rad);
         std::vector<double> fc(nd, f0);
@@ -1062,7 +1113,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
             const double yavg = ysum / (double)n;
             for (int i = 0; i < nd; i++) fc[i] = f0 + beta * (t.y[i] - yavg);
         }
-        HIPCHK(hipMemcpy(s.fc, fc.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(s.fc, fc.data(), nd * sizeof(double), hipMemcpyHostToDevice, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
     }
     // zbk with the all-cell gradient of the bed (its previous value zero)
     TLAUNCH(k_tl_grad<1>, H.nblk_n, T, (const uint8_t *)s.iwet, 0, (const double *)s.zb, (const double *)nullptr,

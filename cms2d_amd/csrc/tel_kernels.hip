@@ -295,20 +295,67 @@ __device__ __forceinline__ void tl_union(int *P, int a, int b) {
         a = old;
     }
 }
-__global__ void k_tl_pond_init(TelG T, TelS s) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T.nd) return;
-    s.plab[t] = s.iwet[t] ? t : -1;
-    s.ipond[t] = 0;
+// Tiles of base cells (each cell, ghosts with their owner, in one tile):
+// components inside a tile are labelled in LDS first, so only the faces that
+// cross a tile edge take global unions.  Labels decrease towards the smallest
+// cell number of the component whatever order the unions land in.
+struct TelTiles {
+    const int *tstart, *tcell;   // cells of each tile, ascending [ntile+1], [nd]
+    const int *tile, *loc;       // tile of each cell and its place in the tile's list [nd]
+};
+__device__ __forceinline__ bool tl_pond_link(const TelG &T, const uint8_t *w, int i, int nb) {
+    return w[i] && w[nb] && (nb >= T.n || nb > i);
 }
-__global__ void k_tl_pond_union(TelG T, TelS s) {
+__device__ __forceinline__ int tl_lds_find(int *lab, int x) {
+    for (;;) {
+        int y = __hip_atomic_load(&lab[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (y == x) return x;
+        x = y;
+    }
+}
+__device__ __forceinline__ void tl_lds_union(int *lab, int a, int b) {
+    for (;;) {
+        a = tl_lds_find(lab, a);
+        b = tl_lds_find(lab, b);
+        if (a == b) return;
+        if (a < b) { int t = a; a = b; b = t; }
+        int old = atomicMin(&lab[a], b);
+        if (old == a) return;
+        a = old;
+    }
+}
+// One workgroup per tile (dynamic LDS: one int per cell of the largest tile).
+__global__ void __launch_bounds__(1024) k_tl_pond_local(TelG T, TelS s, TelTiles Q) {
+    extern __shared__ int lab[];
+    const int t = blockIdx.x, q0 = Q.tstart[t], m = Q.tstart[t + 1] - q0;
+    const uint8_t *w = s.iwet;
+    for (int q = threadIdx.x; q < m; q += blockDim.x) lab[q] = w[Q.tcell[q0 + q]] ? q : -1;
+    __syncthreads();
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        const int i = Q.tcell[q0 + q];
+        if (i >= T.n || !w[i]) continue;
+        const uint64_t mt = T.meta[i];
+        for (int k = 0; k < m_ncf(mt); k++) {
+            const int nb = tnb(T, i, k);
+            if (Q.tile[nb] == t && tl_pond_link(T, w, i, nb)) tl_lds_union(lab, q, Q.loc[nb]);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+        const int i = Q.tcell[q0 + q];
+        s.plab[i] = lab[q] >= 0 ? Q.tcell[q0 + tl_lds_find(lab, q)] : -1;
+        s.ipond[i] = 0;
+    }
+}
+// Global unions over the faces that cross a tile edge.
+__global__ void k_tl_pond_merge(TelG T, TelS s, TelTiles Q) {
     int i;
     if (!tel_thread(T.n, i)) return;
     if (!s.iwet[i]) return;
     const uint64_t m = T.meta[i];
     for (int k = 0; k < m_ncf(m); k++) {
         const int nb = tnb(T, i, k);
-        if (s.iwet[nb] && (nb >= T.n || nb > i)) tl_union(s.plab, i, nb);
+        if (Q.tile[nb] != Q.tile[i] && tl_pond_link(T, s.iwet, i, nb)) tl_union(s.plab, i, nb);
     }
 }
 __global__ void k_tl_pond_compress(TelG T, TelS s) {
@@ -542,10 +589,14 @@ struct TelOp {
         x.i = x.j = 0;
         return x;
     }
+    // Passes over the list in order: in pass p the blocks take consecutive
+    // runs of KBLOCK cells from p * gridDim.x * KBLOCK on, consecutive blocks
+    // on one XCD (xcd_block), so the cells a row above and below (a few
+    // thousand entries away) are read by that XCD's blocks in the same pass.
     template <class F>
     __device__ __forceinline__ void each(F &&f) const {
-        const Chunk ch = block_chunk(nc);
-        for (int t = ch.begin + threadIdx.x; t < ch.end; t += KBLOCK) f(at(t));
+        const int b = xcd_block(blockIdx.x, gridDim.x);
+        for (int t = b * KBLOCK + (int)threadIdx.x; t < nc; t += (int)gridDim.x * KBLOCK) f(at(t));
     }
     __device__ __forceinline__ double apply(const double *xv, const double *dinv, const Idx &x) const {
         const int c = x.c, nf = m_ncf(meta[c]);
@@ -828,11 +879,27 @@ __global__ void __launch_bounds__(1024) k_tl_wse_adjust(TelG T, TelS s, Par P, d
     }
     __syncthreads();
     if (threadIdx.x == 0 && adj) {
+        // the running sums in string order, eight LDS reads issued ahead of
+        // their additions
         term[im0] = 0.0;
         double acc = 0.0;
-        for (int im = im0 + 1; im < n; im++) { acc = acc + term[im]; term[im] = acc; }
+        for (int im = im0 + 1; im < n; im += 8) {
+            double t8[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t8[u] = im + u < n ? term[im + u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (im + u < n) { acc = acc + t8[u]; term[im + u] = acc; }
+        }
         acc = 0.0;
-        for (int im = im0 - 1; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
+        for (int im = im0 - 1; im >= 0; im -= 8) {
+            double t8[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t8[u] = im - u >= 0 ? term[im - u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (im - u >= 0) { acc = acc + t8[u]; term[im - u] = acc; }
+        }
     }
     __syncthreads();
     const double base = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);

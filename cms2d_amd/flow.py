@@ -188,3 +188,25 @@ def solve(acoef, cell2cell, iwet, ss, sp, phi, relax, *, ncells, rtol=1e-10, ato
              iw.ctypes.data_as(I), _dp(ss), _dp(spa), _dp(res), _dp(phi),
              relax, rtol, atol, kmax, C.byref(rmom), C.byref(iters)))
     return res, rmom.value, iters.value
+
+
+def solve_adjoint(acoef, cell2cell, sp, phi_old, phi, relax, gphi, *, ncells, rtol=1e-10, atol=1e-12,
+                  kmax=4000):
+    """The reverse of solve (cms2d_solve_adjoint): for phi = solve(acoef, ss,
+    sp, phi_old, relax) and gphi = dL/dphi on the active cells, returns
+    (dL/dss, dL/dsp, dL/dacoef, dL/dphi_old, iterations of the transposed
+    solve), shaped like the inputs."""
+    acoef = np.ascontiguousarray(acoef, dtype=np.float64)
+    c2c = np.ascontiguousarray(cell2cell, dtype=np.int32)
+    ncellsD, nf = acoef.shape
+    assert c2c.shape == acoef.shape
+    spa, po, ph = (np.ascontiguousarray(v, dtype=np.float64) for v in (sp, phi_old, phi))
+    gp = np.ascontiguousarray(gphi, dtype=np.float64)
+    assert spa.shape == po.shape == ph.shape == (ncellsD,) and gp.shape[0] >= ncells
+    gss, gsp, gpo = np.zeros(ncellsD), np.zeros(ncellsD), np.zeros(ncellsD)
+    ga = np.zeros((ncellsD, nf))
+    iters = C.c_int()
+    check(_lib.lib().cms2d_solve_adjoint(ncells, ncellsD, nf, _dp(acoef), c2c.ctypes.data_as(I), _dp(spa),
+                                         _dp(po), _dp(ph), relax, _dp(gp), rtol, atol, kmax, _dp(gss), _dp(gsp),
+                                         _dp(ga), _dp(gpo), C.byref(iters)))
+    return gss, gsp, ga, gpo, iters.value

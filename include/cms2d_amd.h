@@ -274,6 +274,22 @@ int cms2d_solve(int ncells, int ncellsD, int nmaxfaces, const double *acoef, con
                 const int *iwet, double *ss, const double *sp, double *res, double *phi,
                 double relax, double rtol, double atol, int kmax, double *rmom, int *iters);
 
+/* The reverse of cms2d_solve, for gradients through the linear solves (the
+ * transposed Krylov solve of a discrete adjoint).  No counterpart in the
+ * reference, which has no adjoint.  For phi = solve(acoef, ss, sp, phi_old,
+ * relax) on the active cells (ghost values fixed; the system of
+ * solvers.f90:205-304 solved to convergence) and gphi = dL/dphi [ncells], it
+ * solves A^T lambda = gphi (BiCGStab, Jacobi, rtol/atol/kmax as cms2d_solve)
+ * and returns dL/dss, dL/dsp [ncellsD], dL/dacoef [nmaxfaces*ncellsD] and
+ * dL/dphi_old [ncellsD] (active cells through the relaxation term, ghost cells
+ * through their faces); ghost rows of the others are 0.  phi: the solution
+ * cms2d_solve returned (ghosts as given).  Faces must be shared: a cell lists
+ * each active neighbour that lists it. */
+int cms2d_solve_adjoint(int ncells, int ncellsD, int nmaxfaces, const double *acoef, const int *cell2cell,
+                        const double *sp, const double *phi_old, const double *phi, double relax,
+                        const double *gphi, double rtol, double atol, int kmax, double *gss, double *gsp,
+                        double *gacoef, double *gphi_old, int *iters);
+
 /* Device scratch of cms2d_solve_noalloc per device [bytes]. */
 #define CMS2D_SOLVE_SCRATCH_BYTES (64u << 20)
 
