@@ -25,8 +25,8 @@ namespace cms {
 
 constexpr float TEL_OMEGA = 0.8f;     // damping of the Jacobi sweeps
 constexpr int TEL_NU = 2;             // sweeps per level before and after the coarse correction
+static_assert(TEL_NU == 2, "k_tl_cs_pre2 and k_tl_cs_jacp each run the two sweeps of a level");
 constexpr int TEL_MAX_LEVELS = 5;     // refinement levels below the base grid (16 x 16 cells per base cell)
-static_assert(TEL_NU % 2 == 0, "the up sweeps alternate between two buffers and end in the first");
 
 // Level k of the refined region (1 <= k < finest level): squares of 2^-k base
 // cells.  Lists are ordered by the squares' first cells.
@@ -46,7 +46,7 @@ struct TelLev {
     float *a;                     // [4*nk] (side-major) off-diagonal coefficients
     float *ap, *dinv;             // [nk]
     float *r;                     // [nk] summed residual
-    float *z[2];                  // [nk] ping-pong
+    float *z;                     // [nk] the first sweep's values
 };
 
 // Base-grid level 0 of the multigrid hierarchy from the cell-list pp system
@@ -144,16 +144,35 @@ __global__ void k_tl_cs_setup(TelG T, TelLev V, const double *apc, const double 
     V.dinv[c] = apk > 0.0 ? (float)(1.0 / apk) : 0.0f;
 }
 
-// First sweep from zero on the finest cells, 0 on the other refined cells, into both buffers
-__global__ void k_tl_cs_zero(const int *ref, int nref, const uint8_t *lev, int top, const double *r,
-                             const double *ap, float *Za, float *Zb, const KryScal *sc) {
+// The two down sweeps on the finest cells from zero in one pass: the first
+// sweep's values omega D^-1 r are recomputed where the second reads them;
+// the other refined cells get 0.
+__global__ void k_tl_cs_pre2(TelG T, const int *ref, int nref, const uint8_t *lev, int top, const double *r,
+                             const double *ap, const double *apc, float *Z, const KryScal *sc) {
     if (sc->done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nref) return;
     const int i = ref[t];
-    const float v = lev[i] == top ? TEL_OMEGA * (float)(r[i] / ap[i]) : 0.0f;
-    Za[i] = v;
-    Zb[i] = v;
+    if (lev[i] != top) {
+        Z[i] = 0.0f;
+        return;
+    }
+    auto z1 = [&](int x) { return lev[x] == top ? TEL_OMEGA * (float)(r[x] / ap[x]) : 0.0f; };
+    const uint64_t m = T.meta[i];
+    const int nf = m_ncf(m);
+    int nbk[TF];
+    float zk[TF];
+#pragma unroll
+    for (int k = 0; k < TF; k++) nbk[k] = k < nf ? tnb(T, i, k) : T.n;
+#pragma unroll
+    for (int k = 0; k < TF; k++) zk[k] = nbk[k] < T.n ? z1(nbk[k]) : 0.0f;
+    const float zi = z1(i);
+    double az = ap[i] * (double)zi;
+#pragma unroll
+    for (int k = 0; k < TF; k++)
+        if (nbk[k] < T.n) az -= apc[k * T.nd + i] * (double)zk[k];
+    const double res = r[i] - az;
+    Z[i] = zi + TEL_OMEGA * (float)(res / ap[i]);
 }
 // A damped Jacobi sweep on the finest cells: zout = zin + omega D^-1 (r - A zin)
 __global__ void k_tl_cs_fjac(TelG T, const int *top, int ntop, const double *r, const double *ap, const double *apc,
@@ -186,10 +205,11 @@ __global__ void __launch_bounds__(BLOCK) k_tl_cs_restrict(TelG T, TelLev V, cons
     double acc = 0.0;
     for (int p = q; p < V.kstart[c + 1]; p++) acc += sr[p - q0];
     V.r[c] = (float)acc;
-    V.z[0][c] = TEL_OMEGA * (float)acc * V.dinv[c];
+    V.z[c] = TEL_OMEGA * (float)acc * V.dinv[c];
 }
-// A damped Jacobi sweep on level k
-__global__ void k_tl_cs_jac(TelLev V, const float *zin, float *zout, const KryScal *sc) {
+// The second sweep on level k (from the first, z0) and its correction added
+// to the cells inside each square (Z1, and Z2 if given).
+__global__ void k_tl_cs_jacp(TelLev V, const float *z0, float *Z1, float *Z2, const KryScal *sc) {
     if (sc->done) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= V.nk) return;
@@ -198,33 +218,40 @@ __global__ void k_tl_cs_jac(TelLev V, const float *zin, float *zout, const KrySc
 #pragma unroll
     for (int d = 0; d < 4; d++) nb[d] = V.nbr[d * V.nk + c];
 #pragma unroll
-    for (int d = 0; d < 4; d++) zn[d] = nb[d] >= 0 ? zin[nb[d]] : 0.0f;
-    float az = V.ap[c] * zin[c];
+    for (int d = 0; d < 4; d++) zn[d] = nb[d] >= 0 ? z0[nb[d]] : 0.0f;
+    float az = V.ap[c] * z0[c];
 #pragma unroll
     for (int d = 0; d < 4; d++)
         if (nb[d] >= 0) az -= V.a[d * V.nk + c] * zn[d];
-    zout[c] = zin[c] + TEL_OMEGA * (V.r[c] - az) * V.dinv[c];
-}
-// Adds the level-k correction to the cells inside its squares (Z1, and Z2 if given)
-__global__ void k_tl_cs_prolong(TelLev V, const float *zk, float *Z1, float *Z2, const KryScal *sc) {
-    if (sc->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= V.nge) return;
-    const int i = V.ge[t];
-    const float v = Z1[i] + zk[V.anc[i]];
-    Z1[i] = v;
-    if (Z2) Z2[i] = v;
+    const float zc = z0[c] + TEL_OMEGA * (V.r[c] - az) * V.dinv[c];
+    for (int q = V.kstart[c]; q < V.kstart[c + 1]; q++) {
+        const int i = V.kcell[q];
+        const float v = Z1[i] + zc;
+        Z1[i] = v;
+        if (Z2) Z2[i] = v;
+    }
 }
 // b of every base cell: the summed residual r - A Z of its cells.  One
 // thread per entry of the base-cell-ordered list bcells; blocks hold whole
 // base cells, and the thread of a base cell's first entry sums its entries in
-// order from LDS.
-__global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk, const double *r, const double *ap,
-                                                          const double *apc, const float *Z, const int *bcells,
-                                                          MgLevel L, float *b0, const KryScal *sc) {
+// order from LDS.  A block whose entries are all unrefined base cells away
+// from refined ones, numbered like their base cells (plain[block] = its
+// first base cell, else -1), copies r.
+__global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk, const int *plain, const double *r,
+                                                          const double *ap, const double *apc, const float *Z,
+                                                          const int *bcells, MgLevel L, float *b0,
+                                                          const KryScal *sc) {
     if (sc->done) return;
     __shared__ double sr[BLOCK];
     const int q0 = blk[blockIdx.x], q1 = blk[blockIdx.x + 1], q = q0 + threadIdx.x;
+    const int pb = plain[blockIdx.x];
+    if (pb >= 0) {
+        if (q < q1) {
+            const int b = pb + (q - q0);
+            b0[L.idx(b % T.nbx, b / T.nbx)] = (float)r[q];
+        }
+        return;
+    }
     int i = 0;
     double ri = 0.0;
     if (q < q1) {
@@ -240,10 +267,22 @@ __global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk
     for (int p = q; p < T.bstart[b + 1]; p++) acc += sr[p - q0];
     b0[L.idx(b % T.nbx, b / T.nbx)] = (float)acc;
 }
-// z = Z + P zb on every cell, into both buffers of the up sweeps
-__global__ void k_tl_cs_post0(TelG T, const float *Z, const float *zb, MgLevel L, float *za, float *zq,
-                              const KryScal *sc) {
+// z = Z + P zb on every cell, into both buffers of the up sweeps (a block of
+// cells as in k_tl_mg_restrict, plain[block] >= 0, has Z = 0 and is read by
+// no sweep: z = zb only).
+__global__ void k_tl_cs_post0(TelG T, const int *plain, const float *Z, const float *zb, MgLevel L, float *za,
+                              float *zq, const KryScal *sc) {
     if (sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int pb = plain[lb];
+    if (pb >= 0) {
+        const int i = lb * BLOCK + threadIdx.x;
+        if (i < T.n) {
+            const int b = pb + threadIdx.x;
+            za[i] = zb[L.idx(b % T.nbx, b / T.nbx)];
+        }
+        return;
+    }
     int i;
     if (!tel_thread(T.n, i)) return;
     const int b = T.base[i];
@@ -261,10 +300,11 @@ struct TelHost {
     KrySys<2> Kuv{};
     KrySys<1> Kpp{};
     float *b0 = nullptr;                  // base right-hand side
-    float *zp[2] = {nullptr, nullptr};    // correction of the down sweeps (ping-pong; 0 on unrefined cells)
+    float *zp = nullptr;                  // correction of the down sweeps (0 on unrefined cells)
     float *zq = nullptr;                  // second buffer of the up sweeps
     int *bcells = nullptr;
     int *blkb = nullptr, nblkb = 0;       // blocks of the base restriction (whole base cells)
+    int *plainb = nullptr, *plainc = nullptr;   // first base cell of a plain block (see k_tl_mg_restrict), or -1
     TelLev lv[TEL_MAX_LEVELS]{};          // levels 1 .. top-1
     int top = 0;                          // finest level
     int *ref = nullptr, nref = 0;         // refined cells (level >= 1)
@@ -359,47 +399,34 @@ static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
     const MgLevel &L0 = F->mg.L[0];
     hipStream_t st = F->st;
     KTimer *tm = F->tm_active;
-    int cur = 0;
-    if (H.nref) {
-        hipLaunchKernelGGL(k_tl_cs_zero, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, (const int *)H.ref, H.nref,
-                           (const uint8_t *)H.lev, H.top, r, ap, H.zp[0], H.zp[1], sc);
-        for (int q = 1; q < TEL_NU; q++, cur ^= 1)
-            tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
-                    H.ntop, r, ap, apc, (const float *)H.zp[cur], H.zp[cur ^ 1], sc);
-    }
-    float *Z = H.zp[cur];
+    float *Z = H.zp;
+    if (H.nref)
+        hipLaunchKernelGGL(k_tl_cs_pre2, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, T, (const int *)H.ref, H.nref,
+                           (const uint8_t *)H.lev, H.top, r, ap, apc, Z, sc);
     for (int k = H.top - 1; k >= 1; k--) {
         const TelLev &V = H.lv[k];
         tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
                 (const float *)Z, sc);
-        int cz = 0;
-        for (int q = 1; q < TEL_NU; q++, cz ^= 1)
-            hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
-                               V.z[cz ^ 1], sc);
-        hipLaunchKernelGGL(k_tl_cs_prolong, dim3(tl_nblk(V.nge)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz], Z,
+        hipLaunchKernelGGL(k_tl_cs_jacp, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z, Z,
                            (float *)nullptr, sc);
     }
-    tlaunch(tm, TG_TEL_RESTRICT, k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb, r, ap,
-            apc, (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
+    tlaunch(tm, TG_TEL_RESTRICT, k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb,
+            (const int *)H.plainb, r, ap, apc, (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
     float *zb = L0.zo;
     mg_vcycle<float>(F, H.b0, zb, sc);
-    TLAUNCH(k_tl_cs_post0, H.nblk_n, T, (const float *)Z, (const float *)zb, L0, z, H.zq, sc);
+    TLAUNCH(k_tl_cs_post0, H.nblk_n, T, (const int *)H.plainc, (const float *)Z, (const float *)zb, L0, z, H.zq, sc);
     for (int k = 1; k < H.top; k++) {
         const TelLev &V = H.lv[k];
         tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
                 (const float *)z, sc);
-        int cz = 0;
-        for (int q = 1; q < TEL_NU; q++, cz ^= 1)
-            hipLaunchKernelGGL(k_tl_cs_jac, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz],
-                               V.z[cz ^ 1], sc);
-        hipLaunchKernelGGL(k_tl_cs_prolong, dim3(tl_nblk(V.nge)), dim3(BLOCK), 0, st, V, (const float *)V.z[cz], z,
-                           H.zq, sc);
+        hipLaunchKernelGGL(k_tl_cs_jacp, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z, z, H.zq,
+                           sc);
     }
     if (H.ntop) {
-        float *zz[2] = {z, H.zq};
-        for (int q = 0; q < TEL_NU; q++)
-            tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
-                    H.ntop, r, ap, apc, (const float *)zz[q & 1], zz[(q & 1) ^ 1], sc);
+        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                H.ntop, r, ap, apc, (const float *)z, H.zq, sc);
+        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+                H.ntop, r, ap, apc, (const float *)H.zq, z, sc);
     }
 }
 struct TelPre {
@@ -905,6 +932,26 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     if (int rc = tel_upload(F, &H.bcells, bcells.data(), (size_t)n)) return rc;
     if (int rc = tel_upload(F, &H.blkb, blkb.data(), blkb.size())) return rc;
     H.nblkb = (int)blkb.size() - 1;
+    {
+        // plain blocks: every entry an unrefined base cell away from refined
+        // cells, the cells numbered like their base cells
+        auto plain_cell = [&](int q) {
+            const int b = base[q];
+            return bcells[q] == q && cnt[b + 1] - cnt[b] == 1 && !((meta[q] >> 41) & 1u);
+        };
+        auto plain_run = [&](int a, int e) -> int {
+            if (a >= e) return -1;
+            for (int q = a; q < e; q++)
+                if (!plain_cell(q) || base[q] != base[a] + (q - a)) return -1;
+            return base[a];
+        };
+        const int nbc = (n + BLOCK - 1) / BLOCK;
+        std::vector<int> pb(H.nblkb), pc(nbc);
+        for (int k = 0; k < H.nblkb; k++) pb[k] = plain_run(blkb[k], blkb[k + 1]);
+        for (int k = 0; k < nbc; k++) pc[k] = plain_run(k * BLOCK, std::min(n, (k + 1) * BLOCK));
+        if (int rc = tel_upload(F, &H.plainb, pb.data(), pb.size())) return rc;
+        if (int rc = tel_upload(F, &H.plainc, pc.data(), pc.size())) return rc;
+    }
     if (int rc = tel_upload(F, &H.lev, lev.data(), lev.size())) return rc;
     {
         // ponding tiles: squares of base cells, the side halved until a tile
@@ -968,7 +1015,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         V.kstart = kst; V.kcell = kce; V.kseg = ksg; V.blk = bl; V.nbr = nbr; V.cstart = cst; V.cf = cff;
         V.cw = cww; V.cs = css; V.ge = ge; V.anc = anc;
         if (int rc = dalloc(F, &V.a, (size_t)4 * V.nk)) return rc;
-        float **vf[] = {&V.ap, &V.dinv, &V.r, &V.z[0], &V.z[1]};
+        float **vf[] = {&V.ap, &V.dinv, &V.r, &V.z};
         for (float **f : vf)
             if (int rc = dalloc(F, f, (size_t)V.nk)) return rc;
     }
@@ -1073,7 +1120,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         H.Kuv.sc = F->d_sc + 1;
         if (int rc = dalloc(F, &H.Kpp.zp, (size_t)nd)) return rc;
         if (int rc = dalloc(F, &H.Kpp.zs, (size_t)nd)) return rc;
-        for (float **f : {&H.zp[0], &H.zp[1], &H.zq})
+        for (float **f : {&H.zp, &H.zq})
             if (int rc = dalloc(F, f, (size_t)nd)) return rc;
     }
     if (int rc = mg_alloc(F, d->nx, d->ny)) return rc;

@@ -810,6 +810,7 @@ __global__ void __launch_bounds__(KBLOCK) k_tl_pp_src(TelG T, TelS s, Par P, Kry
             su -= aploc * s.p[c];
             sp = 0.0;
         }
+        bool changed = wet == 0;
 #pragma unroll
         for (int k = 0; k < TF; k++) {
             if (k >= nf || !m_str(m, k)) continue;
@@ -819,10 +820,14 @@ __global__ void __launch_bounds__(KBLOCK) k_tl_pp_src(TelG T, TelS s, Par P, Kry
             su += a[k] * ppg;
             sp -= a[k];
             a[k] = 0.0;
+            changed = true;
         }
+        // the coefficients change only on dry cells and cellstring faces
+        if (changed) {
 #pragma unroll
-        for (int k = 0; k < TF; k++)
-            if (k < nf) s.apc[k * T.nd + c] = a[k];
+            for (int k = 0; k < TF; k++)
+                if (k < nf) s.apc[k * T.nd + c] = a[k];
+        }
         s.sp[c] = sp;
         double asum = 0.0;
 #pragma unroll
@@ -879,27 +884,11 @@ __global__ void __launch_bounds__(1024) k_tl_wse_adjust(TelG T, TelS s, Par P, d
     }
     __syncthreads();
     if (threadIdx.x == 0 && adj) {
-        // the running sums in string order, eight LDS reads issued ahead of
-        // their additions
         term[im0] = 0.0;
         double acc = 0.0;
-        for (int im = im0 + 1; im < n; im += 8) {
-            double t8[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) t8[u] = im + u < n ? term[im + u] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (im + u < n) { acc = acc + t8[u]; term[im + u] = acc; }
-        }
+        for (int im = im0 + 1; im < n; im++) { acc = acc + term[im]; term[im] = acc; }
         acc = 0.0;
-        for (int im = im0 - 1; im >= 0; im -= 8) {
-            double t8[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) t8[u] = im - u >= 0 ? term[im - u] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (im - u >= 0) { acc = acc + t8[u]; term[im - u] = acc; }
-        }
+        for (int im = im0 - 1; im >= 0; im--) { acc = acc + term[im]; term[im] = acc; }
     }
     __syncthreads();
     const double base = (1.0 - ramp) * 0.0 + ramp * (w + 0.0);

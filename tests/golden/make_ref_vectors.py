@@ -128,6 +128,16 @@ def _mk_wse_grad():
 CASES["tidal_wse_grad"] = (_mk_wse_grad, 40, [1, 10, 20, 40])
 
 
+def _mk_wse_grad_north():
+    c = _mk_wse(WSE_T, WSE_V, 0)()
+    c.th_side = 0                                   # the curve on the north side,
+    c.wse_grad = (2.0e-5, 0.0)                      # varying along x: WSE_GRADIENTS 2e-5 0
+    return c
+
+
+CASES["tidal_wse_grad_north"] = (_mk_wse_grad_north, 40, [1, 10, 20, 40])
+
+
 # the tidal case without wall friction (USE_WALL_FRICTION_TERMS OFF), and the
 # tidal case run by the reference's default MATRIX_SOLVER GMRES (its restarted
 # GMRES with ILUT; the converged answer does not depend on the Krylov method)

@@ -27,7 +27,7 @@ def test_serial_order_matches_reference_vectors(gpu_available, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["tidal_wse_grad", "tidal", "tidal_combo"])
+@pytest.mark.parametrize("name", ["tidal_wse_grad", "tidal_wse_grad_north", "tidal", "tidal_combo"])
 def test_serial_order_matches_double_precision_reference(gpu_available, name):
     print(f"serial order, double-precision reference {name}", flush=True)
     w = compare_to_gold(name, lambda c: FlowStep(c, serial_order=1), prefix="ref64_")

@@ -147,6 +147,17 @@ WSE["tidal_wse_grad"] = None
 CASES["tidal_wse_grad"] = _tidal_wse_grad
 
 
+def _tidal_wse_grad_north():
+    c = _tidal_wse(WSE_T, WSE_V, 0)()
+    c.th_side = 0                                   # the curve on the north side
+    c.wse_grad = (2.0e-5, 0.0)                      # WSE_GRADIENTS 2e-5 0 (along x)
+    return c
+
+
+WSE["tidal_wse_grad_north"] = None
+CASES["tidal_wse_grad_north"] = _tidal_wse_grad_north
+
+
 def _tidal_nowall():
     c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
     c.wallfric = 0                                  # USE_WALL_FRICTION_TERMS OFF
