@@ -47,22 +47,37 @@ static int set_err(int code, const std::string &m) {
 // added up.
 // mg_coarse is the span of a V-cycle's levels below level 0 (events recorded
 // on the stream before the first and after the last of those launches).
+// Launches inside a captured BiCGStab pair are not timed (an event recorded by
+// a graph node gives no elapsed time here); the capture notes each timed
+// group's launches per pair, so every replayed launch of an iteration that
+// did work is counted, and the average duration comes from the timed direct
+// launches (each solve's first two iterations run direct).
 enum {
     TG_SPMV_PP = 0, TG_MG_DOWN0, TG_MG_UP0, TG_MG_COARSE, TG_TEL_RESTRICT, TG_TEL_FJAC, TG_TEL_CS_RESTRICT,
     TG_N
 };
 static const char *const TG_NAME[TG_N] = {"spmv_pp", "mg_down0", "mg_up0", "mg_coarse",
                                           "tel_restrict", "tel_fjac", "tel_cs_restrict"};
+constexpr int KT_SLOTS = 4;       // graph slots (the Krylov systems of KryScal)
 struct KTimer {
     struct Rec {
         int group, it;
     };
-    std::vector<hipEvent_t> pool;   // event pairs
+    std::vector<hipEvent_t> pool;   // event pairs of direct launches
     std::vector<Rec> rec;           // one per pair in use
     int it = 0;                     // Krylov iteration of the launches being enqueued
-    double ms[TG_N]{};
+    double ms[TG_N]{};              // timed launches: summed duration and count
     long n[TG_N]{};
+    long counted[TG_N]{};           // replayed launches of iterations that did work (not timed)
+    int capture_slot = -1;          // >= 0 while a pair is captured into that graph slot
+    std::vector<Rec> grec[KT_SLOTS];   // the slot's timed-group launches: group, iteration in the pair (0, 1)
+    // A timed launch: events for a direct launch (false if none are left);
+    // while capturing, the launch is noted and runs untimed (false).
     bool take(int group, hipEvent_t &a, hipEvent_t &b) {
+        if (capture_slot >= 0) {
+            grec[capture_slot].push_back({group, it - 2});
+            return false;
+        }
         const size_t k = rec.size();
         if (2 * k + 2 > pool.size()) return false;
         a = pool[2 * k];
@@ -80,6 +95,14 @@ struct KTimer {
         }
         rec.clear();
         return 0;
+    }
+    // After a batch that replayed the pairs starting at iterations first,
+    // first + 2, ..., last of graph slot `slot`: count the launches of
+    // iterations below upto (those that did work).
+    void count_graph(int slot, int first, int last, int upto) {
+        for (int p = first; p <= last; p += 2)
+            for (const Rec &g : grec[slot])
+                if (p + g.it < upto) counted[g.group] += 1;
     }
 };
 
@@ -304,14 +327,12 @@ struct KryCtx {
     hipGraphExec_t *graph = nullptr;   // per-system graph slots (or null: direct launches)
 };
 
-// Grids up to this many cells replay BiCGStab iteration pairs from a graph
-// captured on first use: on them a launch lasts about as long as the host
-// takes to issue it, and a graph issues the ~20 launches of a pair at once.
-// The kernels and their arguments are those of direct launches, so results
-// are bitwise the same either way.
-#ifndef CMS2D_GRAPH_CELLS
-#define CMS2D_GRAPH_CELLS (1 << 20)
-#endif
+// BiCGStab iterations from the third on replay pairs from a graph captured
+// at creation.  On small grids a launch lasts about as long as the host takes
+// to issue it; on large ones the graph still shortens the gaps between
+// dependent launches (the 2000 x 1500 step: 41.2 -> 37.7 ms).  The kernels
+// and their arguments are those of direct launches, so results are bitwise
+// the same either way.  CMS2D_GRAPHS=0 in the environment turns it off.
 
 template <class Body>
 static int capture_graph(hipStream_t st, hipGraphExec_t &ge, Body body) {
@@ -381,9 +402,12 @@ static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre())
     const hipGraphExec_t ge = X.graph ? X.graph[sc0] : nullptr;
     for (bool first = true; first || (!all_done() && it < X.kmax); first = false) {
         const int nb = std::min(batch, X.kmax - it);
+        int rfirst = -1, rlast = -1;     // first and last replayed pair of the batch
         for (int b = 0; b < nb;) {
             if (ge && it >= 2 && (it & 1) == 0 && nb - b >= 2) {
                 HIPCHK(hipGraphLaunch(ge, X.st));
+                if (rfirst < 0) rfirst = it;
+                rlast = it;
                 b += 2;
                 it += 2;
             } else {
@@ -399,7 +423,9 @@ static int run_bicg(KryCtx &X, KrySys<NS, T> &K, Op A, int sc0, Pre pre = Pre())
         if (X.tm) {
             // launches that did work: those of iterations the first system completed
             const KryScal &s0 = X.h_sc[sc0];
-            if (int rc = X.tm->harvest(s0.done ? s0.final_iters : it)) return rc;
+            const int upto = s0.done ? s0.final_iters : it;
+            if (int rc = X.tm->harvest(upto)) return rc;
+            if (rfirst >= 0) X.tm->count_graph(sc0, rfirst, rlast, upto);
         }
         batch = grow;
         if (++nbatch >= 2) grow = std::min(2 * grow, 64);
@@ -632,7 +658,7 @@ static KryCtx flow_ctx(cms2d_flow *F, bool timed) {
     X.st = F->st; X.nblk = F->nblk_kry; X.kmax = F->P.kmax;
     X.rtol = F->P.rtol; X.atol = F->P.atol;
     X.d_sc = F->d_sc; X.h_sc = F->h_sc; X.prev_iters = F->prev_iters;
-    X.tm = (timed && F->stats && !F->use_graphs) ? &F->tm : nullptr;
+    X.tm = (timed && F->stats) ? &F->tm : nullptr;
     X.graph = F->use_graphs ? F->kgraph : nullptr;
     return X;
 }
@@ -653,7 +679,17 @@ static int capture_graphs(cms2d_flow *F) {
     } else if (F->solver == CMS2D_SOLVER_GMRES) {
         // the pp solve is GMRES
     } else if (F->precond == CMS2D_PRECOND_MG) {
-        if (int rc = capture_bicg_pair<StencilOp, 1, 1, double>(X, F->Kpp, A, F->kgraph[0], PpPre{F})) return rc;
+        // the pp pair, its timed-group launches noted for the launch counts
+        KryCtx Xt = X;
+        if (F->stats) {
+            Xt.tm = &F->tm;
+            F->tm.capture_slot = 0;
+            F->tm_active = &F->tm;
+        }
+        const int rc = capture_bicg_pair<StencilOp, 1, 1, double>(Xt, F->Kpp, A, F->kgraph[0], PpPre{F});
+        F->tm.capture_slot = -1;
+        F->tm_active = nullptr;
+        if (rc) return rc;
     } else {
         if (int rc = capture_bicg_pair<StencilOp, 1, 0, double>(X, F->Kpp, A, F->kgraph[0])) return rc;
     }
@@ -1030,7 +1066,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     F->nblk_act = (g.nact + BLOCK - 1) / BLOCK;
     F->nblk_pad = (F->npad + BLOCK - 1) / BLOCK;
     F->nblk_kry = std::min(KRY_BLOCKS, F->nblk_act);
-    F->use_graphs = (long)g.nact <= CMS2D_GRAPH_CELLS;
+    F->use_graphs = true;
     if (const char *e = getenv("CMS2D_GRAPHS")) F->use_graphs = atoi(e) != 0;   // 0: direct launches, 1: graphs
     Par &P = F->P;
     P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
@@ -1595,7 +1631,8 @@ extern "C" int cms2d_flow_kernel_stats(const cms2d_flow *F, const char *name, do
     if (!F || !name || !avg_ms || !count) return set_err(CMS2D_ERR_ARG, "null argument");
     for (int k = 0; k < TG_N; k++)
         if (!strcmp(name, TG_NAME[k])) {
-            *count = F->tm.n[k];
+            // average over the timed launches; count of every launch that did work
+            *count = F->tm.n[k] + F->tm.counted[k];
             *avg_ms = F->tm.n[k] ? F->tm.ms[k] / F->tm.n[k] : 0.0;
             return CMS2D_OK;
         }

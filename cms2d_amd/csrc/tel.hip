@@ -440,6 +440,27 @@ struct TelPre {
 
 static TelOp tel_op(const TelHost &H, const double *a) { return TelOp{H.T.n, H.T.nd, H.T.c2c, H.T.meta, a}; }
 
+// The BiCGStab pairs of the u/v and pp systems, captured once at creation
+// (the pp pair's timed-group launches noted for the launch counts).
+static int tel_capture_graphs(cms2d_flow *F) {
+    if (!F->use_graphs) return 0;
+    TelHost &H = *F->tel;
+    const KryCtx X = flow_ctx(F, false);
+    if (int rc = capture_bicg_pair<TelOp, 2, 0, double>(X, H.Kuv, tel_op(H, H.s.am), F->kgraph[1])) return rc;
+    const TelOp Ap = tel_op(H, H.s.apc);
+    if (F->precond != CMS2D_PRECOND_MG) return capture_bicg_pair<TelOp, 1, 0, double>(X, H.Kpp, Ap, F->kgraph[0]);
+    KryCtx Xt = X;
+    if (F->stats) {
+        Xt.tm = &F->tm;
+        F->tm.capture_slot = 0;
+        F->tm_active = &F->tm;
+    }
+    const int rc = capture_bicg_pair<TelOp, 1, 1, double>(Xt, H.Kpp, Ap, F->kgraph[0], TelPre{F});
+    F->tm.capture_slot = -1;
+    F->tm_active = nullptr;
+    return rc;
+}
+
 static int tel_flow_imp(cms2d_flow *F) {
     TelHost &H = *F->tel;
     const TelG &T = H.T;
@@ -1030,7 +1051,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     H.nblk_nd = (nd + BLOCK - 1) / BLOCK;
     H.nblk_base = (d->nx * d->ny + BLOCK - 1) / BLOCK;
     F->nblk_kry = std::min(KRY_BLOCKS, H.nblk_n);
-    F->use_graphs = false;
+    F->use_graphs = true;
+    if (const char *e = getenv("CMS2D_GRAPHS")) F->use_graphs = atoi(e) != 0;   // 0: direct launches
 
     Par &P = F->P;
     P.hdry = d->hdry > 0 ? d->hdry : F4(0.05);
@@ -1167,7 +1189,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     TLAUNCH(k_tl_grad<1>, H.nblk_n, T, (const uint8_t *)s.iwet, 0, (const double *)s.zb, (const double *)nullptr,
             (const double *)nullptr, tel_io(H.gzb, 0), GradIO{}, GradIO{});
     TLAUNCH(k_tl_zbk, H.nblk_n, T, s, (const double *)H.gzb[0][1], (const double *)H.gzb[1][1]);
-    return tel_init_state(F);
+    if (int rc = tel_init_state(F)) return rc;
+    return tel_capture_graphs(F);
 }
 
 static int tel_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {

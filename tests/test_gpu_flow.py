@@ -197,16 +197,20 @@ def test_gmres_matches_oracle(gpu_available, shape, steps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replay_matches_direct_launches(gpu_available, monkeypatch, precision):
-    """BiCGStab iteration pairs replayed from a captured graph (the launch
-    mode of grids up to 1M cells) give bitwise the state and iteration counts
-    of direct launches."""
-    c = cases.tidal(nx=96, ny=64, dx=20.0, seed=5)
+@pytest.mark.parametrize("precision,kind", [(0, "cartesian"), (1, "cartesian"), (0, "telescoping")])
+def test_graph_replay_matches_direct_launches(gpu_available, monkeypatch, precision, kind):
+    """BiCGStab iteration pairs replayed from a captured graph (the default
+    launch mode) give bitwise the state and iteration counts of direct
+    launches, on a Cartesian and a telescoping grid (whose pp pair carries
+    the quadtree V-cycle)."""
+    if kind == "cartesian":
+        c = cases.tidal(nx=96, ny=64, dx=20.0, seed=5)
+    else:
+        c = cases.tidal_tel(nx=60, ny=40, dx=20.0, band=((2, 10), (4, 8)), seed=5)
     runs = []
     for graphs in ("1", "0"):
         monkeypatch.setenv("CMS2D_GRAPHS", graphs)
-        print(f"graphs {graphs} precision {precision} tidal 96x64, 12 steps", flush=True)
+        print(f"graphs {graphs} precision {precision} {kind}, 12 steps", flush=True)
         g = FlowStep(c, precision=precision)
         kry = [g.step(1)["kry"] for _ in range(12)]
         runs.append((kry, [g.get(n) for n in ("eta", "u", "v", "p")]))
