@@ -137,6 +137,16 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble")
     ap.add_argument("--no-cartesian", action="store_true",
                     help="skip the uniform-grid comparison (and the FP32 form, built there)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the discrete adjoint")
+    ap.add_argument("--adj-n", type=int, default=1000, help="adjoint workload: cells a side")
+    ap.add_argument("--adj-steps", type=int, default=60,
+                    help="adjoint workload: time steps (the full workload is 720)")
+    ap.add_argument("--adj-every", type=int, default=20, help="adjoint: checkpoint interval (steps)")
+    ap.add_argument("--adj-fd-cells", type=int, default=2,
+                    help="adjoint: cells checked by central differences (two forward runs each per field)")
+    ap.add_argument("--adj-rtol", type=float, default=1e-10, help="adjoint workload: Krylov rtol")
+    ap.add_argument("--adj-check-rtol", type=float, default=1e-12,
+                    help="adjoint: Krylov rtol of the runs checked by central differences")
     ap.add_argument("--ens-members", type=int, default=256)
     ap.add_argument("--ens-steps", type=int, default=3)
     ap.add_argument("--ens-threads", type=int, default=8)
@@ -284,6 +294,103 @@ def ensemble_form(a):
     return out
 
 
+def adjoint_form(a):
+    """The discrete adjoint: a 1000 x 1000 grid of 20 m cells (the tidal
+    physics without radiation stress, dt 60 s), 16 gauges at random wet
+    cells, observations from a hidden seeded Manning field (lognormal,
+    sigma_log 0.2, about the case's), loss = sum over steps and gauges of the
+    squared misfit; dL/dn and dL/dzb of every cell from cms2d_flow_adjoint
+    with checkpoints every adj_every steps, timed against the forward run of
+    the same steps, and checked by central differences (h = 1e-6) at random
+    wet cells where the gradient is resolved (|g| >= 1e-3 max|g|).  The
+    derivative of the exponential scheme below its 1e-6 guard is that of the
+    unguarded form (its limit 1 - P/2), so the gradient is the forward run's
+    except on that 1e-6-wide band of each face's Peclet number."""
+    import copy
+    from cms2d_amd import cases
+    from cms2d_amd.flow import FlowStep
+    n, N, K = a.adj_n, a.adj_steps, a.adj_every
+    c = cases.square(nx=n, ny=n, dx=20.0, seed=3, dt=60.0)
+    rng = np.random.default_rng(17)
+    wet = np.nonzero(-c.zb > 1.0)[0]
+    gauges = np.sort(rng.choice(wet, 16, replace=False))
+    kw = dict(rtol=a.adj_rtol, atol=0.0)
+
+    def gauge_run(case):
+        f = FlowStep(case, **kw)
+        eta = np.zeros((N, len(gauges)))
+        for k in range(N):
+            f.step(1)
+            eta[k] = f.get("eta")[gauges]
+        f.close()
+        return eta
+
+    hidden = copy.copy(c)
+    hidden.mann = (c.mann * np.exp(0.2 * rng.standard_normal(c.mann.shape))).astype(np.float32).astype(np.float64)
+    obs = gauge_run(hidden)
+    progress("adjoint: observations from the hidden Manning field done")
+    f = FlowStep(c, **kw)
+    t0 = time.perf_counter()
+    f.step(N)
+    fwd = time.perf_counter() - t0
+    f.close()
+    f = FlowStep(c, **kw)
+    t0 = time.perf_counter()
+    _, loss, gm, gz = f.adjoint(N, gauges, obs=obs, every=K)
+    adj = time.perf_counter() - t0
+    f.close()
+    progress(f"adjoint: {N} steps forward {fwd:.2f} s, adjoint {adj:.2f} s")
+    # the check: forward and adjoint with the solves at adj_check_rtol, whose
+    # stopping error (~rtol relative, changing when a perturbation changes a
+    # solve's iteration count) stays below what a difference at h = 1e-6 resolves
+    kw = dict(rtol=a.adj_check_rtol, atol=0.0)
+    f = FlowStep(c, **kw)
+    _, _, gm, gz = f.adjoint(N, gauges, obs=obs, every=K)
+    f.close()
+    eta0 = gauge_run(c)
+    fd = {"h": 1e-6, "n": [], "zb": []}
+    for field, grad, base in (("n", gm, c.mann), ("zb", gz, c.zb)):
+        resolved = np.nonzero(np.abs(grad) >= 1e-3 * np.abs(grad).max())[0]
+        cand = np.intersect1d(resolved, wet)
+        for k in rng.choice(cand, min(a.adj_fd_cells, len(cand)), replace=False):
+            # central difference at h = 1e-6, else 1e-7, else 1e-5: the first
+            # whose one-sided halves agree to 1 % (a step across a discrete
+            # switch of the forward run makes them disagree); the loss
+            # differences are formed as sum (e - e0)(e + e0 - 2 obs), free of
+            # the cancellation of two nearly equal sums
+            best = None
+            for h in (1e-6, 1e-7, 1e-5):
+                e = []
+                for sgn in (1.0, -1.0):
+                    c2 = copy.copy(c)
+                    arr = base.copy()
+                    arr[k] += sgn * h
+                    setattr(c2, "mann" if field == "n" else "zb", arr)
+                    e.append(gauge_run(c2))
+                d_up = ((e[0] - eta0) * (e[0] + eta0 - 2.0 * obs)).sum() / h
+                d_dn = ((eta0 - e[1]) * (eta0 + e[1] - 2.0 * obs)).sum() / h
+                d = 0.5 * (d_up + d_dn)
+                spread = abs(d_up - d_dn) / abs(d)
+                if best is None or spread < best[0]:
+                    best = (spread, d, h)
+                if spread <= 1e-2:
+                    break
+            spread, d, h = best
+            fd[field].append({"cell": int(k), "adjoint": float(grad[k]), "difference": float(d), "h": h,
+                              "one_sided_spread": float(spread), "rel_err": float(abs(grad[k] - d) / abs(d))})
+        fd[f"resolved_cells_{field}"] = int(len(resolved))
+    progress("adjoint: central differences done")
+    errs = [e["rel_err"] for f_ in ("n", "zb") for e in fd[f_]]
+    return {"workload": f"{n}x{n} cells of 20 m (tidal physics without radiation stress, wet/dry), dt 60 s, "
+                        f"16 gauges, loss = sum of squared gauge misfit against a hidden lognormal Manning field",
+            "cells": n * n, "steps": N, "checkpoint_every": K, "rtol": a.adj_rtol,
+            "check_rtol": a.adj_check_rtol, "loss": loss,
+            "forward_s": fwd, "adjoint_s": adj, "adjoint_over_forward": adj / fwd,
+            "adjoint_cell_steps_per_s": n * n * N / adj,
+            "grad_norm": {"n": float(np.linalg.norm(gm)), "zb": float(np.linalg.norm(gz))},
+            "central_differences": fd, "max_rel_err": max(errs) if errs else None}
+
+
 def progress(msg):
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
@@ -418,6 +525,8 @@ def main():
         g.close()
     if not a.no_ensemble:
         out["ensemble"] = ensemble_form(a)
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_form(a)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu if cpu is not None else port
         out["cpu_baseline_port"] = port

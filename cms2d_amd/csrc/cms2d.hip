@@ -8,6 +8,7 @@
 #include "krylov_kernels.hip"
 #include "mg_kernels.hip"
 #include "tel_kernels.hip"
+#include "adjoint_kernels.hip"
 #include "../../include/cms2d_amd.h"
 
 #include <hip/hip_ext.h>
@@ -118,6 +119,10 @@ namespace cms {
 struct TelHost;
 }
 
+namespace cms {
+struct Tape;
+}
+
 struct cms2d_flow {
     cms::TelHost *tel = nullptr;   // telescoping grid (cell-list step), or null
     int dev = 0;
@@ -173,6 +178,7 @@ struct cms2d_flow {
     int wse_adjust_off = 0;             // WSE_ADJUSTMENT OFF on the curve sides
     double wse_gx = 0.0, wse_gy = 0.0;  // WSE_GRADIENTS of the curve sides
     hipEvent_t step_ev[2]{};
+    cms::Tape *tape = nullptr;   // records the outer iterations for the reverse sweep (or null)
 };
 
 // Lagrange interpolation of the curve (t, v) at time xi with the window search
@@ -438,11 +444,15 @@ static int mg_nblk(const MgLevel &L) { return (L.nx * L.ny + BLOCK - 1) / BLOCK;
 
 // Builds the hierarchy from the assembled pp system (after the solve prologue
 // has written ap).
-static void mg_setup(cms2d_flow *F) {
+static void mg_setup_from(cms2d_flow *F, const double *aN, const double *aE, const double *aS, const double *aW,
+                          const double *ap) {
     Mg &M = F->mg;
-    const St &s = F->s;
-    LAUNCH(k_mg_setup0, mg_nblk(M.L[0]), F->g, s.aN, s.aE, s.aS, s.aW, F->Kpp.ap_w[0], M.L[0]);
+    LAUNCH(k_mg_setup0, mg_nblk(M.L[0]), F->g, aN, aE, aS, aW, ap, M.L[0]);
     for (int l = 0; l + 1 < M.nlev; l++) LAUNCH(k_mg_coarsen, mg_nblk(M.L[l + 1]), M.L[l], M.L[l + 1]);
+}
+static void mg_setup(cms2d_flow *F) {
+    const St &s = F->s;
+    mg_setup_from(F, s.aN, s.aE, s.aS, s.aW, F->Kpp.ap_w[0]);
 }
 
 // The level-0 launch of the cycle (down or up) with tile width TX.
@@ -788,6 +798,18 @@ static int flow_recalculate(cms2d_flow *F) {
 
 static int flow_imp_once(cms2d_flow *F);
 
+// Recording for the reverse sweep (adjoint.hip): the model state at the start
+// of outer iteration `slot` (or at the loop's exit), and the intermediate
+// values of the current outer iteration after stage `what`.
+enum { TAPE_MOM, TAPE_UV, TAPE_PPCOEF, TAPE_PPSRC, TAPE_PPSOL, TAPE_PVFC1 };
+static int tape_snap(cms2d_flow *F, int slot);
+static int tape_mid(cms2d_flow *F, int what);
+#define TAPE(what)                                              \
+    do {                                                        \
+        if (F->tape)                                            \
+            if (int rc_ = tape_mid(F, what)) return rc_;        \
+    } while (0)
+
 // flow_imp, flow_imp.F90:29-312; with dtvar a divergent outer iteration
 // recalculates the step with half the time step (:157-161, 262-272)
 static int flow_imp(cms2d_flow *F) {
@@ -822,15 +844,20 @@ static int flow_imp_once(cms2d_flow *F) {
     StencilOp A{g, s.aN, s.aE, s.aS, s.aW};
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
+        if (F->tape)
+            if (int rc = tape_snap(F, F->niter - 1)) return rc;
         if (F->P.ndsch >= 5 && F->P.ndsch <= 9) LAUNCH(k_defcor, F->nblk_act, g, s, F->P);
         LAUNCH(k_mom, F->nblk_act, g, s, F->P);
+        TAPE(TAPE_MOM);
         hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->Kuv);
         {
             KryCtx X = flow_ctx(F, false);
             if (int rc = run_bicg(X, F->Kuv, A, 1)) return rc;
         }
+        TAPE(TAPE_UV);
         LAUNCH(k_pp_prep, F->nblk_act, g, s);
         LAUNCH(k_pp_coef_flux, F->nblk_act, g, s);
+        TAPE(TAPE_PPCOEF);
         for (int sd = 0; sd < 4; sd++) {
             if (g.side[sd] != 1) continue;
             int n = (sd == KE || sd == KW) ? g.ny : g.nx;
@@ -840,6 +867,7 @@ static int flow_imp_once(cms2d_flow *F) {
                                (int)(curve && F->wse_adjust_off));
         }
         hipLaunchKernelGGL(k_pp_src, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->P, F->Kpp);
+        TAPE(TAPE_PPSRC);
         {
             KryCtx X = flow_ctx(F, true);
             if (F->precision == CMS2D_PREC_FP32_REFINE) {
@@ -858,7 +886,9 @@ static int flow_imp_once(cms2d_flow *F) {
                 if (int rc = run_bicg(X, F->Kpp, A, 0)) return rc;
             }
         }
+        TAPE(TAPE_PPSOL);
         LAUNCH(k_pvfc1, F->nblk_act, g, s);
+        TAPE(TAPE_PVFC1);
         LAUNCH(k_pvfc2, F->nblk_pad, g, s, F->P);
         LAUNCH(k_ghost_p, (2 * g.nx + 2 * g.ny + BLOCK - 1) / BLOCK, g, s);
         LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, F->P);
@@ -872,6 +902,8 @@ static int flow_imp_once(cms2d_flow *F) {
         if (F->iconv == 2) break;
     }
     if (F->niter > F->maxit) F->niter = F->maxit;
+    if (F->tape)
+        if (int rc = tape_snap(F, F->niter)) return rc;
     LAUNCH(k_velface, F->nblk_act, g, s, F->P);
     return 0;
 }
@@ -1279,42 +1311,47 @@ extern "C" int cms2d_flow_create_tel(const cms2d_flow_desc *d, const cms2d_tel_g
     return CMS2D_OK;
 }
 
+// One iteration of the CMS_Flow time loop: timing, CMS_Flow.f90:109-240 (with
+// dtvar the time step is halved after a step that ended with iconv = 3 and
+// doubled back once the pp residual is small and the time falls on the
+// coarser step, :180-205), flow_imp, flow_update.
+static int step_once(cms2d_flow *F) {
+    F->ntime++;
+    if (F->dtvar) {
+        if (F->iconv == 3 && F->dtimebeg / F->deltime <= 10.0) {
+            F->mtime = 0;
+            F->jtime += 1;
+            F->deltime = F->dtimebeg / std::ldexp(1.0, F->jtime);
+        } else if (F->jtime >= 1) {
+            if ((F->mtime >= 2 && F->rmom[0] < F4(1.0e-4) / 100.0) ||
+                (F->mtime >= 4 && F->rmom[0] < F4(1.0e-4) / 10.0)) {
+                const double dtt = F->dtimebeg / std::ldexp(1.0, F->jtime - 1);
+                const double rtime = std::fmod(F->timesecs + dtt, dtt);
+                const double err = std::fabs(rtime) / dtt;
+                if (rtime < F4(0.01) || err < F4(0.01)) {
+                    F->deltime = dtt;
+                    F->mtime = 0;
+                    F->jtime -= 1;
+                }
+            }
+        }
+        F->mtime += 1;
+        F->P.dtime = F->deltime;
+    }
+    F->timesecs += F->deltime;
+    F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
+    if (int rc = flow_imp(F)) return rc;
+    LAUNCH(k_update, F->nblk_pad, F->g, F->s);
+    return 0;
+}
+
 extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {
     if (!F || nsteps < 0) return set_err(CMS2D_ERR_ARG, "invalid argument");
     HIPCHK(hipSetDevice(F->dev));
     if (F->tel) return tel_step(F, nsteps, info);
     HIPCHK(hipEventRecord(F->step_ev[0], F->st));
-    for (int n = 0; n < nsteps; n++) {
-        // timing, CMS_Flow.f90:109-240; with dtvar the time step is halved
-        // after a step that ended with iconv = 3 and doubled back once the pp
-        // residual is small and the time falls on the coarser step (:180-205)
-        F->ntime++;
-        if (F->dtvar) {
-            if (F->iconv == 3 && F->dtimebeg / F->deltime <= 10.0) {
-                F->mtime = 0;
-                F->jtime += 1;
-                F->deltime = F->dtimebeg / std::ldexp(1.0, F->jtime);
-            } else if (F->jtime >= 1) {
-                if ((F->mtime >= 2 && F->rmom[0] < F4(1.0e-4) / 100.0) ||
-                    (F->mtime >= 4 && F->rmom[0] < F4(1.0e-4) / 10.0)) {
-                    const double dtt = F->dtimebeg / std::ldexp(1.0, F->jtime - 1);
-                    const double rtime = std::fmod(F->timesecs + dtt, dtt);
-                    const double err = std::fabs(rtime) / dtt;
-                    if (rtime < F4(0.01) || err < F4(0.01)) {
-                        F->deltime = dtt;
-                        F->mtime = 0;
-                        F->jtime -= 1;
-                    }
-                }
-            }
-            F->mtime += 1;
-            F->P.dtime = F->deltime;
-        }
-        F->timesecs += F->deltime;
-        F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
-        if (int rc = flow_imp(F)) return rc;
-        LAUNCH(k_update, F->nblk_pad, F->g, F->s);
-    }
+    for (int n = 0; n < nsteps; n++)
+        if (int rc = step_once(F)) return rc;
     HIPCHK(hipEventRecord(F->step_ev[1], F->st));
     HIPCHK(hipStreamSynchronize(F->st));
     if (info) {
@@ -1466,6 +1503,29 @@ static size_t ck_size(const cms2d_flow *F) {
     return n;
 }
 
+// The host-side state of the time loop (timing, convergence control, solver
+// batch sizes, boundary-curve window) beside the device arrays.
+static void ck_host_get(const cms2d_flow *F, CkHost &c) {
+    c.P = F->P;
+    c.timesecs = F->timesecs; c.rampdur = F->rampdur; c.ramp = F->ramp; c.dtimebeg = F->dtimebeg; c.deltime = F->deltime;
+    for (int n = 0; n < 3; n++) { c.rmom[n] = F->rmom[n]; c.rmom0[n] = F->rmom0[n]; c.rmom1[n] = F->rmom1[n]; c.kry[n] = F->kry[n]; }
+    c.dtvar = F->dtvar; c.jtime = F->jtime; c.mtime = F->mtime; c.ntime = F->ntime; c.maxit = F->maxit;
+    c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
+    for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
+    c.windconst = F->windconst; c.wndx = F->wndx; c.wndy = F->wndy;
+    c.wse_inc = F->wse_inc;
+}
+static void ck_host_set(cms2d_flow *F, const CkHost &c) {
+    F->P = c.P;
+    F->timesecs = c.timesecs; F->rampdur = c.rampdur; F->ramp = c.ramp; F->dtimebeg = c.dtimebeg; F->deltime = c.deltime;
+    for (int n = 0; n < 3; n++) { F->rmom[n] = c.rmom[n]; F->rmom0[n] = c.rmom0[n]; F->rmom1[n] = c.rmom1[n]; F->kry[n] = c.kry[n]; }
+    F->dtvar = c.dtvar; F->jtime = c.jtime; F->mtime = c.mtime; F->ntime = c.ntime; F->maxit = c.maxit;
+    F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
+    for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
+    F->windconst = c.windconst; F->wndx = c.wndx; F->wndy = c.wndy;
+    F->wse_inc = c.wse_inc;
+}
+
 extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     if (!F || !nbytes) return set_err(CMS2D_ERR_ARG, "null argument");
     if (F->tel) return set_err(CMS2D_ERR_ARG, "checkpoints are built for Cartesian grids only");
@@ -1482,14 +1542,7 @@ extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {
     memcpy(q, &h, sizeof h);
     q += sizeof h;
     CkHost c{};
-    c.P = F->P;
-    c.timesecs = F->timesecs; c.rampdur = F->rampdur; c.ramp = F->ramp; c.dtimebeg = F->dtimebeg; c.deltime = F->deltime;
-    for (int n = 0; n < 3; n++) { c.rmom[n] = F->rmom[n]; c.rmom0[n] = F->rmom0[n]; c.rmom1[n] = F->rmom1[n]; c.kry[n] = F->kry[n]; }
-    c.dtvar = F->dtvar; c.jtime = F->jtime; c.mtime = F->mtime; c.ntime = F->ntime; c.maxit = F->maxit;
-    c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
-    for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
-    c.windconst = F->windconst; c.wndx = F->wndx; c.wndy = F->wndy;
-    c.wse_inc = F->wse_inc;
+    ck_host_get(F, c);
     memcpy(q, &c, sizeof c);
     q += sizeof c;
     for (auto &a : F->state) {
@@ -1523,14 +1576,7 @@ extern "C" int cms2d_flow_restore(cms2d_flow *F, const void *buf, size_t nbytes)
         q += a.second;
     }
     HIPCHK(hipStreamSynchronize(F->st));
-    F->P = c.P;
-    F->timesecs = c.timesecs; F->rampdur = c.rampdur; F->ramp = c.ramp; F->dtimebeg = c.dtimebeg; F->deltime = c.deltime;
-    for (int n = 0; n < 3; n++) { F->rmom[n] = c.rmom[n]; F->rmom0[n] = c.rmom0[n]; F->rmom1[n] = c.rmom1[n]; F->kry[n] = c.kry[n]; }
-    F->dtvar = c.dtvar; F->jtime = c.jtime; F->mtime = c.mtime; F->ntime = c.ntime; F->maxit = c.maxit;
-    F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
-    for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
-    F->windconst = c.windconst; F->wndx = c.wndx; F->wndy = c.wndy;
-    F->wse_inc = c.wse_inc;
+    ck_host_set(F, c);
     return CMS2D_OK;
 }
 
@@ -2004,3 +2050,5 @@ extern "C" int cms2d_solve_noalloc(int ncells, int ncellsD, int nmaxfaces, const
     (void)hipStreamSynchronize(nullptr);
     return rc;
 }
+
+#include "adjoint.hip"

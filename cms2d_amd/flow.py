@@ -157,6 +157,25 @@ class FlowStep:
         check(_lib.lib().cms2d_flow_tuning(self._h, C.byref(t), C.byref(gr)))
         return t.value, bool(gr.value)
 
+    def adjoint(self, nsteps, gauges, *, obs=None, geta=None, every=60):
+        """Discrete adjoint of nsteps steps from the initial state
+        (cms2d_flow_adjoint): gauges are row-major cell indices; with obs
+        [nsteps, ngauge] the loss is sum (eta - obs)^2, else geta [nsteps,
+        ngauge] is dL/deta.  Returns (eta [nsteps, ngauge], loss or None,
+        dL/dn [ncells], dL/dzb [ncells]); the flow is back at its initial
+        state afterwards."""
+        cells = np.ascontiguousarray(gauges, dtype=np.int32)
+        ng = len(cells)
+        eta = np.zeros((nsteps, ng))
+        gm, gz = np.zeros(self.ncells), np.zeros(self.ncells)
+        loss = C.c_double(0.0)
+        o = None if obs is None else np.ascontiguousarray(obs, dtype=np.float64).reshape(nsteps, ng)
+        ge = None if geta is None else np.ascontiguousarray(geta, dtype=np.float64).reshape(nsteps, ng)
+        check(_lib.lib().cms2d_flow_adjoint(self._h, nsteps, every, ng, cells.ctypes.data_as(I),
+                                            D() if o is None else _dp(o), D() if ge is None else _dp(ge),
+                                            _dp(eta), C.cast(C.byref(loss), D), _dp(gm), _dp(gz)))
+        return eta, (loss.value if o is not None else None), gm, gz
+
     def kernel_stats(self, name="spmv_pp"):
         ms, n = C.c_double(), C.c_long()
         check(_lib.lib().cms2d_flow_kernel_stats(self._h, name.encode(), C.byref(ms), C.byref(n)))

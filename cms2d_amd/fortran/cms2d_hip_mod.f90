@@ -153,11 +153,23 @@ module cms2d_hip
             real(c_double), intent(in) :: whgt(*), wper(*), wlen(*), wunitx(*), wunity(*)
             integer(c_int) :: rc
         end function
+        ! discrete adjoint of the time loop: dL/dn and dL/dzb of a gauge water-level loss
+        function cms2d_flow_adjoint(flow, nsteps, every, ngauge, gauge_cells, gauge_obs, geta, eta_gauge, &
+                                    loss, gmann, gzb) bind(C, name='cms2d_flow_adjoint') result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: flow
+            integer(c_int), value :: nsteps, every, ngauge
+            integer(c_int), intent(in) :: gauge_cells(*)
+            type(c_ptr), value :: gauge_obs, geta, eta_gauge, loss
+            real(c_double), intent(out) :: gmann(*), gzb(*)
+            integer(c_int) :: rc
+        end function
     end interface
 
     public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
     public :: cms2d_error_message, cms2d_device_count
     public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves, cms2d_flow_set_wind
+    public :: cms2d_flow_adjoint
     public :: cms2d_flow_create_tel
 
 contains

@@ -290,6 +290,31 @@ int cms2d_solve_adjoint(int ncells, int ncellsD, int nmaxfaces, const double *ac
                         const double *gphi, double rtol, double atol, int kmax, double *gss, double *gsp,
                         double *gacoef, double *gphi_old, int *iters);
 
+/* Discrete adjoint of the implicit time loop.  No counterpart in the
+ * reference, which has no adjoint.  From the flow's initial state (a flow
+ * object on which no step has been taken), runs nsteps steps as
+ * cms2d_flow_step does and returns the gradient of a loss of the water levels
+ * at the gauge cells with respect to the Manning n (gmann) and the bed
+ * elevation (gzb) of every active cell [ncells, row-major], by the reverse of
+ * every kernel of the step (transposed u, v and pp solves included) at the
+ * recorded forward state.  The state is checkpointed every `every` steps and
+ * each interval recomputed in the reverse sweep.
+ *   gauge_cells [ngauge]: row-major active-cell indices;
+ *   gauge_obs [nsteps*ngauge] (step-major): when given, the loss is
+ *     sum (eta - obs)^2 over steps and gauges, returned in *loss;
+ *   geta [nsteps*ngauge]: otherwise dL/deta of each gauge after each step;
+ *   eta_gauge [nsteps*ngauge]: the gauge water levels of the run (or NULL).
+ * Discrete choices (wet/dry flags, outer-iteration counts, clipping branches)
+ * are those of the forward run.  Requires a Cartesian grid, the FP64 form with
+ * BiCGStab and the multigrid preconditioner, fixed dt, Manning friction,
+ * advection schemes 0-4 and no wave or wind forcing (CMS2D_ERR_ARG
+ * otherwise).  Allocates its checkpoints and tape on the device (about
+ * 0.6 KB per cell for each stored state); the flow is back at its initial
+ * state on return. */
+int cms2d_flow_adjoint(cms2d_flow *flow, int nsteps, int every, int ngauge, const int *gauge_cells,
+                       const double *gauge_obs, const double *geta, double *eta_gauge, double *loss,
+                       double *gmann, double *gzb);
+
 /* Device scratch of cms2d_solve_noalloc per device [bytes]. */
 #define CMS2D_SOLVE_SCRATCH_BYTES (64u << 20)
 
