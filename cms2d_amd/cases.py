@@ -213,6 +213,27 @@ class TelGrid:
     level: np.ndarray
     bnd_cells: np.ndarray
 
+    def edge_faces(self):
+        """[n, 6] mask of the domain-edge faces in each cell's face order: the
+        slots N1 N2 E1 E2 S1 S2 W1 W2 in turn give a face per listed neighbour
+        and one to a dummy cell for a side that lists none (read_tel,
+        geospatial.F90:552-690); the latter are the edge faces."""
+        nb = np.asarray(self.nb)
+        even = np.zeros(8, dtype=bool)
+        even[0::2] = True
+        pair_empty = np.repeat((nb[:, 0::2] == 0) & (nb[:, 1::2] == 0), 2, axis=1)
+        edge = even & pair_empty
+        face = edge | (nb > 0)
+        k = np.cumsum(face, axis=1) - 1
+        out = np.zeros((len(nb), 6), dtype=bool)
+        rows, cols = np.nonzero(edge)
+        out[rows, k[rows, cols]] = True
+        return out
+
+    def areas(self):
+        g = np.asarray(self.geom)
+        return g[:, 2] * g[:, 3]
+
 
 def telescope(levels, dx, side=1) -> TelGrid:
     """Quadtree over an [ny, nx] base grid of dx x dx cells: base cell (i, j)

@@ -139,8 +139,10 @@ struct cms2d_flow {
     GmScal *h_gm = nullptr;    // pinned mirror
     int prev_gm = 8;
     Mg mg{};                   // pp preconditioner hierarchy (level 0 z is Kpp.zp / Kpp.zs)
-    KrySys<1, float> Kin{};    // inner FP32 solve of the refinement form
-    KryScal *d_sc = nullptr;   // [4]: pp, u, v, inner FP32 pp
+    KrySys<1, float> Kin{};    // inner FP32 solve of the refinement form (pp)
+    KrySys<2, float> Kin2{};   // inner FP32 solves of the refinement form (u, v)
+    float *a32[4]{};           // their FP32 coefficients
+    KryScal *d_sc = nullptr;   // [6]: pp, u, v, inner FP32 pp, inner FP32 u, v
     KryScal *h_sc = nullptr;   // pinned mirror
     int *h_flags = nullptr;    // pinned
     std::vector<void *> allocs;
@@ -153,7 +155,7 @@ struct cms2d_flow {
     double dtimebeg = 0.0, deltime = 0.0;
     int ntime = 0, maxit = 0, niter = 0, iconv = 1;
     double rmom[3]{}, rmom0[3]{}, rmom1[3]{};
-    int kry[3]{}, prev_iters[4]{4, 2, 2, 4};
+    int kry[3]{}, prev_iters[6]{4, 2, 2, 4, 2, 2};
     // timing of the pp-system SpMV launches (events bracket each launch)
     bool stats = true;
     KTimer tm;                 // live timing of the pp solve's main kernels
@@ -596,15 +598,47 @@ static int pp_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
     InnerPre pre{F};
     int total = 0;
     for (int sw = 0; sw < REF_SWEEPS && !F->h_sc[0].done; sw++) {
-        hipLaunchKernelGGL(k_ref_start, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
+        hipLaunchKernelGGL(k_ref_start<StencilOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
         if (int rc = run_bicg<StencilOp32, 1, 1, float>(Xi, Ki, A32, 3, pre)) return rc;
         total += F->h_sc[3].final_iters;
-        hipLaunchKernelGGL(k_ref_update, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
+        hipLaunchKernelGGL(k_ref_update<StencilOp>, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
         hipLaunchKernelGGL(k_ref_check, dim3(1), dim3(KBLOCK), 0, F->st, K, nb);
         HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
         HIPCHK(hipStreamSynchronize(F->st));
     }
     F->h_sc[0].final_iters = total;
+    return 0;
+}
+
+// The u and v solves of the FP32 form: FP32 Jacobi BiCGStab of both systems
+// (FP32 coefficients and vectors) for each correction, x and the residuals in
+// FP64, each system stopped on its FP64 solve's own threshold.  Krylov
+// iterations are summed over the sweeps.
+static int uv_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
+    KrySys<2> &K = F->Kuv;
+    KrySys<2, float> &Ki = F->Kin2;
+    const int nb = F->nblk_kry;
+    hipLaunchKernelGGL((k_bicg_p<StencilOp, 2, double>), dim3(1), dim3(KBLOCK), 0, F->st, A, K, 0, nb, X.kmax,
+                       X.rtol, X.atol, 1);
+    hipLaunchKernelGGL(k_uv_to32, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, F->a32[0], F->a32[1], F->a32[2],
+                       F->a32[3], Ki);
+    HIPCHK(hipMemcpyAsync(F->h_sc + 1, F->d_sc + 1, 2 * sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const StencilOp32 A32{F->g, F->a32[0], F->a32[1], F->a32[2], F->a32[3]};
+    KryCtx Xi = inner_ctx(X);
+    Xi.graph = nullptr;
+    int total[2] = {0, 0};
+    for (int sw = 0; sw < REF_SWEEPS && !(F->h_sc[1].done && F->h_sc[2].done); sw++) {
+        hipLaunchKernelGGL(k_ref_start2<StencilOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, K);
+        if (int rc = run_bicg<StencilOp32, 2, 0, float>(Xi, Ki, A32, 4)) return rc;
+        for (int q = 0; q < 2; q++) total[q] += F->h_sc[4 + q].final_iters;
+        hipLaunchKernelGGL(k_ref_update2<StencilOp>, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, Ki);
+        hipLaunchKernelGGL(k_ref_check2, dim3(1), dim3(KBLOCK), 0, F->st, K, nb);
+        HIPCHK(hipMemcpyAsync(F->h_sc + 1, F->d_sc + 1, 2 * sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+    }
+    F->h_sc[1].final_iters = total[0];
+    F->h_sc[2].final_iters = total[1];
     return 0;
 }
 
@@ -852,7 +886,11 @@ static int flow_imp_once(cms2d_flow *F) {
         hipLaunchKernelGGL(k_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, g, s, F->Kuv);
         {
             KryCtx X = flow_ctx(F, false);
-            if (int rc = run_bicg(X, F->Kuv, A, 1)) return rc;
+            if (F->precision == CMS2D_PREC_FP32_REFINE) {
+                if (int rc = uv_solve_refine(F, X, A)) return rc;
+            } else if (int rc = run_bicg(X, F->Kuv, A, 1)) {
+                return rc;
+            }
         }
         TAPE(TAPE_UV);
         LAUNCH(k_pp_prep, F->nblk_act, g, s);
@@ -1176,7 +1214,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         double *bi = red + 3 * 5 * nb;
         F->Kin.red.pr[0] = bi; F->Kin.red.pv[0] = bi + 2 * nb; F->Kin.red.pt[0] = bi + 3 * nb;
     }
-    if (int rc = dalloc(F, &F->d_sc, 4)) return rc;
+    if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
     // Krylov vectors
     {
         double *kv[3][7];
@@ -1228,10 +1266,29 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         Ki.zp = F->Kpp.zp;
         Ki.zs = F->Kpp.zs;
         Ki.sc = F->d_sc + 3;
+        if (F->precision == CMS2D_PREC_FP32_REFINE) {
+            KrySys<2, float> &K2 = F->Kin2;
+            for (int q = 0; q < 2; q++) {
+                float **vec[] = {&K2.x[q], &K2.r[q], &K2.r0[q], &K2.p[q], &K2.v[q], &K2.t[q], &K2.ap_w[q], &K2.dinv_w[q]};
+                for (float **v : vec)
+                    if (int rc = dalloc(F, v, N)) return rc;
+            }
+            for (int k = 0; k < 4; k++)
+                if (int rc = dalloc(F, &F->a32[k], N)) return rc;
+            double *ri;
+            const size_t nbk = F->nblk_kry;
+            if (int rc = dalloc(F, &ri, 2 * 5 * nbk)) return rc;
+            for (int q = 0; q < 2; q++) {
+                K2.red.pr[q] = ri + q * 5 * nbk;
+                K2.red.pv[q] = K2.red.pr[q] + 2 * nbk;
+                K2.red.pt[q] = K2.red.pr[q] + 3 * nbk;
+            }
+            K2.sc = F->d_sc + 4;
+        }
     }
-    HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&F->h_sc, 6 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
-    memset(F->h_sc, 0, 4 * sizeof(KryScal));
+    memset(F->h_sc, 0, 6 * sizeof(KryScal));
     F->tm.pool.resize(2 * 2048);
     for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));

@@ -115,14 +115,16 @@ struct StencilOpT {
         x.c = cidx(g, x.i, x.j);
         return x;
     }
-    __device__ __forceinline__ double apply(const double *xv, const double *dinv, const Idx &x) const {
+    // V: the vectors' type (double; float in the inner solves of the FP32 form)
+    template <class V>
+    __device__ __forceinline__ V apply(const V *xv, const V *dinv, const Idx &x) const {
         const int c = x.c;
         const int cN = c + g.nxp, cE = c + 1, cS = c - g.nxp, cW = c - 1;
-        const double zN = x.j == g.ny - 1 ? 0.0 : xv[cN] * dinv[cN];
-        const double zE = x.i == g.nx - 1 ? 0.0 : xv[cE] * dinv[cE];
-        const double zS = x.j == 0 ? 0.0 : xv[cS] * dinv[cS];
-        const double zW = x.i == 0 ? 0.0 : xv[cW] * dinv[cW];
-        const double sum = ((aN[c] * zN + aE[c] * zE) + aS[c] * zS) + aW[c] * zW;
+        const V zN = x.j == g.ny - 1 ? V(0) : xv[cN] * dinv[cN];
+        const V zE = x.i == g.nx - 1 ? V(0) : xv[cE] * dinv[cE];
+        const V zS = x.j == 0 ? V(0) : xv[cS] * dinv[cS];
+        const V zW = x.i == 0 ? V(0) : xv[cW] * dinv[cW];
+        const V sum = (((V)aN[c] * zN + (V)aE[c] * zE) + (V)aS[c] * zS) + (V)aW[c] * zW;
         return xv[c] - sum;
     }
     // y_i = ap_i z_i - sum_k a_k(i) z_nb(k) for an explicit preconditioned

@@ -319,7 +319,8 @@ namespace cms {
 // the FP64 ||r|| meets the FP64 solve's own threshold.
 
 // Inner start from the FP64 residual: r = r0 = (float) r64, x = 0 (p starts from r).
-__global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, float> Ki, const double *r64) {
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_ref_start(Op A, KrySys<1, float> Ki, const double *r64) {
     double red[2] = {0.0, 0.0};
     A.each([&](const Idx &x_) {
         const int c = x_.c;
@@ -334,7 +335,8 @@ __global__ void __launch_bounds__(KBLOCK) k_ref_start(StencilOp32 A, KrySys<1, f
 }
 
 // x += d, r -= A d (FP64 operator on the FP32 correction), partials ||r||^2.
-__global__ void __launch_bounds__(KBLOCK) k_ref_update(StencilOp A, KrySys<1, double> K, const float *d) {
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_ref_update(Op A, KrySys<1, double> K, const float *d) {
     double red[2] = {0.0, 0.0};
     A.each([&](const Idx &x) {
         const int c = x.c;
@@ -345,6 +347,82 @@ __global__ void __launch_bounds__(KBLOCK) k_ref_update(StencilOp A, KrySys<1, do
     });
     double *const dst[2] = {K.red.pr[0], K.red.pr[0] + gridDim.x};
     put_partials<2>(red, dst);
+}
+
+// The u and v solves of the FP32 form (two systems, Jacobi): FP32 copies of
+// the shared coefficients and of each system's diagonal.
+__global__ void __launch_bounds__(KBLOCK) k_uv_to32(StencilOp A, KrySys<2, double> K, float *aN, float *aE, float *aS,
+                                                   float *aW, KrySys<2, float> Ki) {
+    A.each([&](const Idx &x) {
+        const int c = x.c;
+        aN[c] = (float)A.aN[c]; aE[c] = (float)A.aE[c]; aS[c] = (float)A.aS[c]; aW[c] = (float)A.aW[c];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            Ki.ap_w[q][c] = (float)K.ap_w[q][c];
+            Ki.dinv_w[q][c] = (float)K.dinv_w[q][c];
+        }
+    });
+}
+// Start of an inner FP32 sweep of the two systems: r = FP64 residual (0 for
+// a system whose FP64 solve has stopped), x = 0.
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_ref_start2(Op A, KrySys<2, float> Ki, KrySys<2, double> K) {
+    double red[4] = {0.0, 0.0, 0.0, 0.0};
+    bool live[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) live[q] = !K.sc[q].done;
+    A.each([&](const Idx &x_) {
+        const int c = x_.c;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const float r = live[q] ? (float)K.r[q][c] : 0.0f;
+            Ki.r[q][c] = r; Ki.r0[q][c] = r; Ki.x[q][c] = 0.0f;
+            red[2 * q] += (double)r * (double)r;
+        }
+    });
+    red[1] = red[0];
+    red[3] = red[2];
+    double *const dst[4] = {Ki.red.pr[0], Ki.red.pr[0] + gridDim.x, Ki.red.pr[1], Ki.red.pr[1] + gridDim.x};
+    put_partials<4>(red, dst);
+    kry_reset(Ki.sc[0]);
+    kry_reset(Ki.sc[1]);
+}
+// x += d, r -= A d for each running system (FP64 operator on the FP32
+// correction), partials ||r||^2.
+template <class Op>
+__global__ void __launch_bounds__(KBLOCK) k_ref_update2(Op A, KrySys<2, double> K, KrySys<2, float> Ki) {
+    double red[2] = {0.0, 0.0};
+    bool live[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) live[q] = !K.sc[q].done;
+    A.each([&](const Idx &x) {
+        const int c = x.c;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (!live[q]) continue;
+            const float *d = Ki.x[q];
+            K.x[q][c] = K.x[q][c] + (double)d[c];
+            const double r = K.r[q][c] - A.apply_z(d, K.ap_w[q], x);
+            K.r[q][c] = r;
+            red[q] += r * r;
+        }
+    });
+    double *const dst[2] = {K.red.pr[0], K.red.pr[1]};
+    put_partials<2>(red, dst);
+}
+// One block: the FP64 stop tests of the two systems after a sweep.
+__global__ void __launch_bounds__(KBLOCK) k_ref_check2(KrySys<2, double> K, int nb) {
+    double tot[2];
+    const double *src[2] = {K.red.pr[0], K.red.pr[1]};
+    sum_partials<2>(src, nb, tot);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            if (K.sc[q].done) continue;
+            const double nr = sqrt(tot[q]);
+            K.sc[q].rnorm = nr;
+            if (nr <= K.sc[q].thr) K.sc[q].done = 1;
+        }
 }
 
 // One block: the FP64 stop test of the outer sweep.

@@ -99,9 +99,10 @@ __global__ void __launch_bounds__(BLOCK) k_tl_mg_setup0(TelG T, const int *blk, 
 // The residual r - A Z of cell i for a correction Z that is nonzero on
 // refined cells only (so only cells next to them see it); the face loads
 // are issued together (every slot, predicated) and summed in face order.
-__device__ __forceinline__ double tl_res(const TelG &T, const double *r, const double *ap, const double *apc,
+template <class TB>
+__device__ __forceinline__ double tl_res(const TelG &T, const TB *r, const double *ap, const double *apc,
                                          const float *Z, int i, uint64_t m) {
-    double ri = r[i];
+    double ri = (double)r[i];
     if (m_near(m)) {
         const int nf = m_ncf(m);
         int nbk[TF];
@@ -147,7 +148,8 @@ __global__ void k_tl_cs_setup(TelG T, TelLev V, const double *apc, const double 
 // The two down sweeps on the finest cells from zero in one pass: the first
 // sweep's values omega D^-1 r are recomputed where the second reads them;
 // the other refined cells get 0.
-__global__ void k_tl_cs_pre2(TelG T, const int *ref, int nref, const uint8_t *lev, int top, const double *r,
+template <class TB>
+__global__ void k_tl_cs_pre2(TelG T, const int *ref, int nref, const uint8_t *lev, int top, const TB *r,
                              const double *ap, const double *apc, float *Z, const KryScal *sc) {
     if (sc->done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -157,7 +159,7 @@ __global__ void k_tl_cs_pre2(TelG T, const int *ref, int nref, const uint8_t *le
         Z[i] = 0.0f;
         return;
     }
-    auto z1 = [&](int x) { return lev[x] == top ? TEL_OMEGA * (float)(r[x] / ap[x]) : 0.0f; };
+    auto z1 = [&](int x) { return lev[x] == top ? TEL_OMEGA * (float)((double)r[x] / ap[x]) : 0.0f; };
     const uint64_t m = T.meta[i];
     const int nf = m_ncf(m);
     int nbk[TF];
@@ -171,11 +173,12 @@ __global__ void k_tl_cs_pre2(TelG T, const int *ref, int nref, const uint8_t *le
 #pragma unroll
     for (int k = 0; k < TF; k++)
         if (nbk[k] < T.n) az -= apc[k * T.nd + i] * (double)zk[k];
-    const double res = r[i] - az;
+    const double res = (double)r[i] - az;
     Z[i] = zi + TEL_OMEGA * (float)(res / ap[i]);
 }
 // A damped Jacobi sweep on the finest cells: zout = zin + omega D^-1 (r - A zin)
-__global__ void k_tl_cs_fjac(TelG T, const int *top, int ntop, const double *r, const double *ap, const double *apc,
+template <class TB>
+__global__ void k_tl_cs_fjac(TelG T, const int *top, int ntop, const TB *r, const double *ap, const double *apc,
                              const float *zin, float *zout, const KryScal *sc) {
     if (sc->done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,7 +190,8 @@ __global__ void k_tl_cs_fjac(TelG T, const int *top, int ntop, const double *r, 
 // Residual r - A Z summed into the level-k squares, and the first sweep from
 // zero there.  One thread per entry of kcell; blocks hold whole squares, and
 // the thread of a square's first entry sums its entries in order from LDS.
-__global__ void __launch_bounds__(BLOCK) k_tl_cs_restrict(TelG T, TelLev V, const double *r, const double *ap,
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_tl_cs_restrict(TelG T, TelLev V, const TB *r, const double *ap,
                                                           const double *apc, const float *Z, const KryScal *sc) {
     if (sc->done) return;
     __shared__ double sr[BLOCK];
@@ -237,7 +241,8 @@ __global__ void k_tl_cs_jacp(TelLev V, const float *z0, float *Z1, float *Z2, co
 // order from LDS.  A block whose entries are all unrefined base cells away
 // from refined ones, numbered like their base cells (plain[block] = its
 // first base cell, else -1), copies r.
-__global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk, const int *plain, const double *r,
+template <class TB>
+__global__ void __launch_bounds__(BLOCK) k_tl_mg_restrict(TelG T, const int *blk, const int *plain, const TB *r,
                                                           const double *ap, const double *apc, const float *Z,
                                                           const int *bcells, MgLevel L, float *b0,
                                                           const KryScal *sc) {
@@ -302,6 +307,7 @@ struct TelHost {
     float *b0 = nullptr;                  // base right-hand side
     float *zp = nullptr;                  // correction of the down sweeps (0 on unrefined cells)
     float *zq = nullptr;                  // second buffer of the up sweeps
+    float *apc32 = nullptr;               // FP32 coefficients of the FP32 form's current inner solve (u/v, then pp)
     int *bcells = nullptr;
     int *blkb = nullptr, nblkb = 0;       // blocks of the base restriction (whole base cells)
     int *plainb = nullptr, *plainc = nullptr;   // first base cell of a plain block (see k_tl_mg_restrict), or -1
@@ -390,42 +396,43 @@ static int tel_grad_fric_vis(cms2d_flow *F) {
 
 static unsigned tl_nblk(int n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
-// z = M^-1 r of the pp system (r: FP64 Krylov vector)
-static void tel_mg_apply(cms2d_flow *F, const double *r, float *z) {
+// z = M^-1 r of the pp system (r: a Krylov vector of the FP64 solve, or of
+// the inner FP32 solve of the refinement form; sc: that solve's scalars)
+template <class TB>
+static void tel_mg_apply(cms2d_flow *F, const TB *r, float *z, const KryScal *sc) {
     TelHost &H = *F->tel;
     const TelG &T = H.T;
-    const KryScal *sc = H.Kpp.sc;
     const double *ap = H.Kpp.ap_w[0], *apc = H.s.apc;
     const MgLevel &L0 = F->mg.L[0];
     hipStream_t st = F->st;
     KTimer *tm = F->tm_active;
     float *Z = H.zp;
     if (H.nref)
-        hipLaunchKernelGGL(k_tl_cs_pre2, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, T, (const int *)H.ref, H.nref,
+        hipLaunchKernelGGL(k_tl_cs_pre2<TB>, dim3(tl_nblk(H.nref)), dim3(BLOCK), 0, st, T, (const int *)H.ref, H.nref,
                            (const uint8_t *)H.lev, H.top, r, ap, apc, Z, sc);
     for (int k = H.top - 1; k >= 1; k--) {
         const TelLev &V = H.lv[k];
-        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
+        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict<TB>, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
                 (const float *)Z, sc);
         hipLaunchKernelGGL(k_tl_cs_jacp, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z, Z,
                            (float *)nullptr, sc);
     }
-    tlaunch(tm, TG_TEL_RESTRICT, k_tl_mg_restrict, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb,
+    tlaunch(tm, TG_TEL_RESTRICT, k_tl_mg_restrict<TB>, dim3(H.nblkb), dim3(BLOCK), 0, st, T, (const int *)H.blkb,
             (const int *)H.plainb, r, ap, apc, (const float *)Z, (const int *)H.bcells, L0, H.b0, sc);
     float *zb = L0.zo;
     mg_vcycle<float>(F, H.b0, zb, sc);
     TLAUNCH(k_tl_cs_post0, H.nblk_n, T, (const int *)H.plainc, (const float *)Z, (const float *)zb, L0, z, H.zq, sc);
     for (int k = 1; k < H.top; k++) {
         const TelLev &V = H.lv[k];
-        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
+        tlaunch(tm, TG_TEL_CS_RESTRICT, k_tl_cs_restrict<TB>, dim3(V.nblk), dim3(BLOCK), 0, st, T, V, r, ap, apc,
                 (const float *)z, sc);
         hipLaunchKernelGGL(k_tl_cs_jacp, dim3(tl_nblk(V.nk)), dim3(BLOCK), 0, st, V, (const float *)V.z, z, H.zq,
                            sc);
     }
     if (H.ntop) {
-        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac<TB>, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
                 H.ntop, r, ap, apc, (const float *)z, H.zq, sc);
-        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
+        tlaunch(tm, TG_TEL_FJAC, k_tl_cs_fjac<TB>, dim3(tl_nblk(H.ntop)), dim3(BLOCK), 0, st, T, (const int *)H.topc,
                 H.ntop, r, ap, apc, (const float *)H.zq, z, sc);
     }
 }
@@ -433,8 +440,8 @@ struct TelPre {
     cms2d_flow *F;
     void operator()(int which) const {
         TelHost &H = *F->tel;
-        if (which == 0) tel_mg_apply(F, H.Kpp.p[0], H.Kpp.zp);
-        else tel_mg_apply(F, H.Kpp.r[0], H.Kpp.zs);
+        if (which == 0) tel_mg_apply<double>(F, H.Kpp.p[0], H.Kpp.zp, H.Kpp.sc);
+        else tel_mg_apply<double>(F, H.Kpp.r[0], H.Kpp.zs, H.Kpp.sc);
     }
 };
 
@@ -459,6 +466,90 @@ static int tel_capture_graphs(cms2d_flow *F) {
     F->tm.capture_slot = -1;
     F->tm_active = nullptr;
     return rc;
+}
+
+// FP32 copies of the pp coefficients and diagonal for the inner solve
+__global__ void k_tl_to32(TelG T, const double *apc, const double *ap, float *apc32, float *ap32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T.nd) return;
+    for (int k = 0; k < TF; k++) apc32[k * T.nd + i] = (float)apc[k * T.nd + i];
+    ap32[i] = (float)ap[i];
+}
+// FP32 copies of the momentum coefficients and of each system's diagonal
+__global__ void k_tl_uv_to32(TelG T, const double *am, KrySys<2> K, float *a32, KrySys<2, float> Ki) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T.nd) return;
+    for (int k = 0; k < TF; k++) a32[k * T.nd + i] = (float)am[k * T.nd + i];
+    for (int q = 0; q < 2; q++) {
+        Ki.ap_w[q][i] = (float)K.ap_w[q][i];
+        Ki.dinv_w[q][i] = (float)K.dinv_w[q][i];
+    }
+}
+// The u and v solves of the FP32 form on the cell list (as uv_solve_refine).
+static int tel_uv_solve_refine(cms2d_flow *F, KryCtx &X, const TelOp &A) {
+    TelHost &H = *F->tel;
+    KrySys<2> &K = H.Kuv;
+    KrySys<2, float> &Ki = F->Kin2;
+    const int nb = F->nblk_kry;
+    hipLaunchKernelGGL((k_bicg_p<TelOp, 2, double>), dim3(1), dim3(KBLOCK), 0, F->st, A, K, 0, nb, X.kmax, X.rtol,
+                       X.atol, 1);
+    TLAUNCH(k_tl_uv_to32, H.nblk_nd, H.T, (const double *)H.s.am, K, H.apc32, Ki);
+    HIPCHK(hipMemcpyAsync(F->h_sc + 1, F->d_sc + 1, 2 * sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const TelOp32 A32{H.T.n, H.T.nd, H.T.c2c, H.T.meta, H.apc32};
+    KryCtx Xi = inner_ctx(X);
+    Xi.graph = nullptr;
+    int total[2] = {0, 0};
+    for (int sw = 0; sw < REF_SWEEPS && !(F->h_sc[1].done && F->h_sc[2].done); sw++) {
+        hipLaunchKernelGGL(k_ref_start2<TelOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, K);
+        if (int rc = run_bicg<TelOp32, 2, 0, float>(Xi, Ki, A32, 4)) return rc;
+        for (int q = 0; q < 2; q++) total[q] += F->h_sc[4 + q].final_iters;
+        hipLaunchKernelGGL(k_ref_update2<TelOp>, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, Ki);
+        hipLaunchKernelGGL(k_ref_check2, dim3(1), dim3(KBLOCK), 0, F->st, K, nb);
+        HIPCHK(hipMemcpyAsync(F->h_sc + 1, F->d_sc + 1, 2 * sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+    }
+    F->h_sc[1].final_iters = total[0];
+    F->h_sc[2].final_iters = total[1];
+    return 0;
+}
+struct TelInnerPre {
+    cms2d_flow *F;
+    void operator()(int which) const {
+        KrySys<1, float> &Ki = F->Kin;
+        if (which == 0) tel_mg_apply<float>(F, Ki.p[0], Ki.zp, Ki.sc);
+        else tel_mg_apply<float>(F, Ki.r[0], Ki.zs, Ki.sc);
+    }
+};
+// The pp solve of the FP32 form on the cell list: FP32 BiCGStab (FP32
+// coefficients and vectors, the quadtree V-cycle on FP32 right-hand sides) for
+// each correction, x and the residual in FP64, stopped on the FP64 solve's
+// own threshold.  Krylov iterations are summed over the sweeps.
+static int tel_pp_solve_refine(cms2d_flow *F, KryCtx &X, const TelOp &A) {
+    TelHost &H = *F->tel;
+    KrySys<1> &K = H.Kpp;
+    KrySys<1, float> &Ki = F->Kin;
+    const int nb = F->nblk_kry;
+    hipLaunchKernelGGL((k_bicg_p<TelOp, 1, double>), dim3(1), dim3(KBLOCK), 0, F->st, A, K, 0, nb, X.kmax, X.rtol,
+                       X.atol, 1);
+    TLAUNCH(k_tl_to32, H.nblk_nd, H.T, (const double *)H.s.apc, (const double *)K.ap_w[0], H.apc32, Ki.ap_w[0]);
+    HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    const TelOp32 A32{H.T.n, H.T.nd, H.T.c2c, H.T.meta, H.apc32};
+    KryCtx Xi = inner_ctx(X);
+    TelInnerPre pre{F};
+    int total = 0;
+    for (int sw = 0; sw < REF_SWEEPS && !F->h_sc[0].done; sw++) {
+        hipLaunchKernelGGL(k_ref_start<TelOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, (const double *)K.r[0]);
+        if (int rc = run_bicg<TelOp32, 1, 1, float>(Xi, Ki, A32, 3, pre)) return rc;
+        total += F->h_sc[3].final_iters;
+        hipLaunchKernelGGL(k_ref_update<TelOp>, dim3(nb), dim3(KBLOCK), 0, F->st, A, K, (const float *)Ki.x[0]);
+        hipLaunchKernelGGL(k_ref_check, dim3(1), dim3(KBLOCK), 0, F->st, K, nb);
+        HIPCHK(hipMemcpyAsync(F->h_sc, F->d_sc, sizeof(KryScal), hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+    }
+    F->h_sc[0].final_iters = total;
+    return 0;
 }
 
 static int tel_flow_imp(cms2d_flow *F) {
@@ -489,7 +580,11 @@ static int tel_flow_imp(cms2d_flow *F) {
         hipLaunchKernelGGL(k_tl_uv_prologue, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, T, s, H.Kuv, Am);
         {
             KryCtx X = flow_ctx(F, false);
-            if (int rc = run_bicg(X, H.Kuv, Am, 1)) return rc;
+            if (F->precision == CMS2D_PREC_FP32_REFINE) {
+                if (int rc = tel_uv_solve_refine(F, X, Am)) return rc;
+            } else if (int rc = run_bicg(X, H.Kuv, Am, 1)) {
+                return rc;
+            }
         }
         TLAUNCH(k_tl_pp_prep, H.nblk_n, T, s);
         if (T.skewcor) {
@@ -516,10 +611,14 @@ static int tel_flow_imp(cms2d_flow *F) {
                             (const double *)H.Kpp.ap_w[0]);
                 for (int l = 0; l + 1 < F->mg.nlev; l++)
                     LAUNCH(k_mg_coarsen, mg_nblk(F->mg.L[l + 1]), F->mg.L[l], F->mg.L[l + 1]);
-                F->tm_active = X.tm;
-                int rc = run_bicg<TelOp, 1, 1, double>(X, H.Kpp, Ap, 0, TelPre{F});
-                F->tm_active = nullptr;
-                if (rc) return rc;
+                if (F->precision == CMS2D_PREC_FP32_REFINE) {
+                    if (int rc = tel_pp_solve_refine(F, X, Ap)) return rc;
+                } else {
+                    F->tm_active = X.tm;
+                    int rc = run_bicg<TelOp, 1, 1, double>(X, H.Kpp, Ap, 0, TelPre{F});
+                    F->tm_active = nullptr;
+                    if (rc) return rc;
+                }
             } else {
                 if (int rc = run_bicg(X, H.Kpp, Ap, 0)) return rc;
             }
@@ -803,8 +902,11 @@ static int tel_check(const cms2d_flow_desc *d, const cms2d_tel_grid *tg) {
         return set_err(CMS2D_ERR_ARG, "invalid flow description");
     if (d->ndsch >= 5 && d->ndsch <= 9)
         return set_err(CMS2D_ERR_ARG, "the NVD schemes are built for Cartesian grids only");
-    if (d->precision != CMS2D_PREC_FP64 || d->solver != CMS2D_SOLVER_BICGSTAB || d->dtvar)
-        return set_err(CMS2D_ERR_ARG, "telescoping grids run the FP64 BiCGStab form with a fixed time step");
+    if ((d->precision != CMS2D_PREC_FP64 && d->precision != CMS2D_PREC_FP32_REFINE) ||
+        d->solver != CMS2D_SOLVER_BICGSTAB || d->dtvar)
+        return set_err(CMS2D_ERR_ARG, "telescoping grids run BiCGStab (FP64 or the FP32 form) with a fixed time step");
+    if (d->precision == CMS2D_PREC_FP32_REFINE && d->precond != CMS2D_PRECOND_MG)
+        return set_err(CMS2D_ERR_ARG, "the FP32 refinement form uses the multigrid preconditioner");
     if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
         return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
     if (d->serial_order) return set_err(CMS2D_ERR_ARG, "serial_order is built for Cartesian grids only");
@@ -1124,7 +1226,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
             double *b = red + (q + 1) * 5 * nb;
             H.Kuv.red.pr[q] = b; H.Kuv.red.pv[q] = b + 2 * nb; H.Kuv.red.pt[q] = b + 3 * nb;
         }
-        if (int rc = dalloc(F, &F->d_sc, 4)) return rc;
+        if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
         double *kv[3][7];
         for (int q = 0; q < 3; q++)
             for (int mm = 0; mm < 7; mm++)
@@ -1144,6 +1246,34 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         if (int rc = dalloc(F, &H.Kpp.zs, (size_t)nd)) return rc;
         for (float **f : {&H.zp, &H.zq})
             if (int rc = dalloc(F, f, (size_t)nd)) return rc;
+        if (d->precision == CMS2D_PREC_FP32_REFINE) {
+            // the inner FP32 solve: vectors, diagonal, coefficients, partial sums
+            KrySys<1, float> &Ki = F->Kin;
+            float **vec[] = {&Ki.x[0], &Ki.r[0], &Ki.r0[0], &Ki.p[0], &Ki.v[0], &Ki.t[0], &Ki.ap_w[0], &Ki.dinv_w[0]};
+            for (float **v : vec)
+                if (int rc = dalloc(F, v, (size_t)nd)) return rc;
+            if (int rc = dalloc(F, &H.apc32, (size_t)TF * nd)) return rc;
+            double *ri;
+            if (int rc = dalloc(F, &ri, 5 * nb)) return rc;
+            Ki.red.pr[0] = ri; Ki.red.pv[0] = ri + 2 * nb; Ki.red.pt[0] = ri + 3 * nb;
+            Ki.zp = H.Kpp.zp;
+            Ki.zs = H.Kpp.zs;
+            Ki.sc = F->d_sc + 3;
+            KrySys<2, float> &K2 = F->Kin2;
+            for (int q = 0; q < 2; q++) {
+                float **v2[] = {&K2.x[q], &K2.r[q], &K2.r0[q], &K2.p[q], &K2.v[q], &K2.t[q], &K2.ap_w[q], &K2.dinv_w[q]};
+                for (float **v : v2)
+                    if (int rc = dalloc(F, v, (size_t)nd)) return rc;
+            }
+            double *r2;
+            if (int rc = dalloc(F, &r2, 2 * 5 * nb)) return rc;
+            for (int q = 0; q < 2; q++) {
+                K2.red.pr[q] = r2 + q * 5 * nb;
+                K2.red.pv[q] = K2.red.pr[q] + 2 * nb;
+                K2.red.pt[q] = K2.red.pr[q] + 3 * nb;
+            }
+            K2.sc = F->d_sc + 4;
+        }
     }
     if (int rc = mg_alloc(F, d->nx, d->ny)) return rc;
     if (int rc = dalloc(F, &F->mg.L[0].zo, (size_t)(d->nx + 2) * (d->ny + 2))) return rc;
@@ -1153,9 +1283,9 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         const int tt = atoi(e);
         if (tt == 64 || tt == 128 || tt == 256) F->mg_tile = tt;
     }
-    HIPCHK(hipHostMalloc((void **)&F->h_sc, 4 * sizeof(KryScal), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void **)&F->h_sc, 6 * sizeof(KryScal), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void **)&F->h_flags, 4 * sizeof(int), hipHostMallocDefault));
-    memset(F->h_sc, 0, 4 * sizeof(KryScal));
+    memset(F->h_sc, 0, 6 * sizeof(KryScal));
     F->tm.pool.resize(2 * 2048);
     for (auto &e : F->tm.pool) HIPCHK(hipEventCreate(&e));
     for (auto &e : F->step_ev) HIPCHK(hipEventCreate(&e));

@@ -577,11 +577,12 @@ __global__ void k_tl_bound_uv(TelG T, TelS s, Par P) {
 
 // Cell-list operator for the Krylov kernels: y_i = ap_i z_i - sum_k a_k(i) z_nb
 // over active neighbours (z = 0 on ghost cells, pbicgstab.f90:55,76).
-struct TelOp {
+template <class CA>
+struct TelOpT {
     int nc, nd;
     const int *c2c;
     const uint64_t *meta;
-    const double *a;             // [TF*nd]
+    const CA *a;                 // [TF*nd]
     __device__ __forceinline__ int n() const { return nc; }
     __device__ __forceinline__ Idx at(int t) const {
         Idx x;
@@ -598,12 +599,14 @@ struct TelOp {
         const int b = xcd_block(blockIdx.x, gridDim.x);
         for (int t = b * KBLOCK + (int)threadIdx.x; t < nc; t += (int)gridDim.x * KBLOCK) f(at(t));
     }
-    __device__ __forceinline__ double apply(const double *xv, const double *dinv, const Idx &x) const {
+    // V: the vectors' type (double; float in the inner solves of the FP32 form)
+    template <class V>
+    __device__ __forceinline__ V apply(const V *xv, const V *dinv, const Idx &x) const {
         const int c = x.c, nf = m_ncf(meta[c]);
-        double sum = 0.0;
+        V sum = V(0);
         for (int k = 0; k < nf; k++) {
             const int nb = c2c[k * nd + c];
-            if (nb < nc) sum += a[k * nd + c] * (xv[nb] * dinv[nb]);
+            if (nb < nc) sum += (V)a[k * nd + c] * (xv[nb] * dinv[nb]);
         }
         return xv[c] - sum;
     }
@@ -618,6 +621,8 @@ struct TelOp {
         return ap[c] * (C)z[c] - sum;
     }
 };
+using TelOp = TelOpT<double>;
+using TelOp32 = TelOpT<float>;   // the inner solve of the FP32 form
 
 // solve prologue for u (n=2) and v (n=3), relax 0.8 (solvers.f90:232-240) and
 // the BiCGStab start (pbicgstab.f90:21-31).

@@ -26,29 +26,11 @@ def boundary_inflow(f, c):
     return -(f[-1, :, 0].sum() + f[:, -1, 1].sum() + f[0, :, 2].sum() + f[:, 0, 3].sum())
 
 
-def tel_boundary_faces(tel):
-    """[ncells, 6] mask of the domain-edge faces of a telescoping grid in each
-    cell's face order: the neighbour slots N1 N2 E1 E2 S1 S2 W1 W2 in turn, a
-    face per listed neighbour and one to a dummy cell for a side that lists
-    none (read_tel, source/geospatial/geospatial.F90:552-690)."""
-    nb = np.asarray(tel.nb)
-    even = np.zeros(8, dtype=bool)
-    even[0::2] = True
-    pair_empty = np.repeat((nb[:, 0::2] == 0) & (nb[:, 1::2] == 0), 2, axis=1)
-    edge = even & pair_empty
-    face = edge | (nb > 0)
-    k = np.cumsum(face, axis=1) - 1
-    out = np.zeros((len(nb), 6), dtype=bool)
-    rows, cols = np.nonzero(edge)
-    out[rows, k[rows, cols]] = True
-    return out
-
-
 def check_balance(model, c, spin, steps):
     tel = getattr(c, "tel", None)
     if tel is not None:
-        area = np.asarray(tel.geom)[:, 2] * np.asarray(tel.geom)[:, 3]
-        edge = tel_boundary_faces(tel)
+        area = tel.areas()
+        edge = tel.edge_faces()
         inflow = lambda f: -f[edge].sum()  # noqa: E731
     else:
         area = c.dx * c.dy
@@ -82,12 +64,14 @@ def test_oracle_mass_balance_telescoping():
 
 
 @pytest.mark.gpu
-def test_gpu_mass_balance_telescoping_full_size(gpu_available):
-    """The 3.61 M-cell telescoping tidal case of the benchmark."""
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gpu_mass_balance_telescoping_full_size(gpu_available, precision):
+    """The 3.61 M-cell telescoping tidal case of the benchmark (FP64 form and
+    the FP32 refinement form)."""
     from cms2d_amd.flow import FlowStep
     c = cases.tidal_tel()
-    print(f"mass balance telescoping tidal, {c.ncells} cells", flush=True)
-    g = FlowStep(c)
+    print(f"mass balance telescoping tidal, {c.ncells} cells, precision {precision}", flush=True)
+    g = FlowStep(c, precision=precision)
     worst = check_balance(g, c, spin=30, steps=5)
     print(f"worst relative imbalance {worst:.2e}", flush=True)
     g.close()
