@@ -143,7 +143,7 @@ def parse():
                     help="adjoint workload: time steps (the full workload is 720)")
     ap.add_argument("--adj-every", type=int, default=20, help="adjoint: checkpoint interval (steps)")
     ap.add_argument("--adj-fd-cells", type=int, default=2,
-                    help="adjoint: cells checked by central differences (two forward runs each per field)")
+                    help="adjoint: cells checked by central differences (four to eight forward runs each per field)")
     ap.add_argument("--adj-rtol", type=float, default=1e-10, help="adjoint workload: Krylov rtol")
     ap.add_argument("--adj-check-rtol", type=float, default=1e-12,
                     help="adjoint: Krylov rtol of the runs checked by central differences")
@@ -306,8 +306,9 @@ def adjoint_form(a):
     sigma_log 0.2, about the case's), loss = sum over steps and gauges of the
     squared misfit; dL/dn and dL/dzb of every cell from cms2d_flow_adjoint
     with checkpoints every adj_every steps, timed against the forward run of
-    the same steps, and checked by central differences (h = 1e-6) at random
-    wet cells where the gradient is resolved (|g| >= 1e-3 max|g|).  The
+    the same steps, and checked by central differences (h = 1e-6 and 5e-7,
+    else 1e-5 and 5e-6) at random wet cells where the gradient is resolved
+    (|g| >= 1e-3 max|g|).  The
     derivative of the exponential scheme below its 1e-6 guard is that of the
     unguarded form (its limit 1 - P/2), so the gradient is the forward run's
     except on that 1e-6-wide band of each face's Peclet number."""
@@ -353,40 +354,39 @@ def adjoint_form(a):
     _, _, gm, gz = f.adjoint(N, gauges, obs=obs, every=K)
     f.close()
     progress(f"adjoint: gradient at rtol {a.adj_check_rtol:g} for the check done")
-    eta0 = gauge_run(c)
-    progress("adjoint: base run for the differences done")
-    fd = {"h": 1e-6, "n": [], "zb": []}
+    fd = {"h": "1e-6 and 5e-7, else 1e-5 and 5e-6", "n": [], "zb": []}
     for field, grad, base in (("n", gm, c.mann), ("zb", gz, c.zb)):
         resolved = np.nonzero(np.abs(grad) >= 1e-3 * np.abs(grad).max())[0]
         cand = np.intersect1d(resolved, wet)
         for k in rng.choice(cand, min(a.adj_fd_cells, len(cand)), replace=False):
-            # central difference at h = 1e-6, else 1e-7, else 1e-5: the first
-            # whose one-sided halves agree to 1 % (a step across a discrete
-            # switch of the forward run makes them disagree); the loss
-            # differences are formed as sum (e - e0)(e + e0 - 2 obs), free of
-            # the cancellation of two nearly equal sums
+            # central differences at h and h/2 for h = 1e-6, then 1e-5: the
+            # first pair that agrees to 1e-3 (a step across a discrete switch
+            # of the forward run, or one too small for the loss's rounding,
+            # makes the two disagree); the loss differences are formed as
+            # sum (e+ - e-)(e+ + e- - 2 obs), free of the cancellation of two
+            # nearly equal sums
             best = None
-            for h in (1e-6, 1e-7, 1e-5):
-                e = []
-                for sgn in (1.0, -1.0):
-                    c2 = copy.copy(c)
-                    arr = base.copy()
-                    arr[k] += sgn * h
-                    setattr(c2, "mann" if field == "n" else "zb", arr)
-                    e.append(gauge_run(c2))
-                    progress(f"adjoint: run with {field}[{int(k)}] {sgn * h:+g} done")
-                d_up = ((e[0] - eta0) * (e[0] + eta0 - 2.0 * obs)).sum() / h
-                d_dn = ((eta0 - e[1]) * (eta0 + e[1] - 2.0 * obs)).sum() / h
-                d = 0.5 * (d_up + d_dn)
-                spread = abs(d_up - d_dn) / abs(d)
-                if best is None or spread < best[0]:
-                    best = (spread, d, h)
-                if spread <= 1e-2:
+            for h in (1e-6, 1e-5):
+                cen = []
+                for hh in (h, h / 2):
+                    e = []
+                    for sgn in (1.0, -1.0):
+                        c2 = copy.copy(c)
+                        arr = base.copy()
+                        arr[k] += sgn * hh
+                        setattr(c2, "mann" if field == "n" else "zb", arr)
+                        e.append(gauge_run(c2))
+                        progress(f"adjoint: run with {field}[{int(k)}] {sgn * hh:+g} done")
+                    cen.append(((e[0] - e[1]) * (e[0] + e[1] - 2.0 * obs)).sum() / (2.0 * hh))
+                agree = abs(cen[0] - cen[1]) / abs(cen[1])
+                if best is None or agree < best[0]:
+                    best = (agree, cen[1], h / 2)
+                if agree <= 1e-3:
                     break
             spread, d, h = best
             progress(f"adjoint: difference at cell {int(k)} of {field} done")
             fd[field].append({"cell": int(k), "adjoint": float(grad[k]), "difference": float(d), "h": h,
-                              "one_sided_spread": float(spread), "rel_err": float(abs(grad[k] - d) / abs(d))})
+                              "h_vs_2h_agreement": float(spread), "rel_err": float(abs(grad[k] - d) / abs(d))})
         fd[f"resolved_cells_{field}"] = int(len(resolved))
     progress("adjoint: central differences done")
     errs = [e["rel_err"] for f_ in ("n", "zb") for e in fd[f_]]

@@ -163,7 +163,7 @@ struct cms2d_flow {
     // launch-bound grids: BiCGStab iteration pairs replayed from captured
     // graphs, one per Krylov system (indexed like d_sc)
     bool use_graphs = false;
-    hipGraphExec_t kgraph[4]{};
+    hipGraphExec_t kgraph[6]{};
     int mg_tile = 128;         // level-0 tile width of the multigrid cycle (timed at creation)
     // the reference's cell order for the three order-dependent loops: the
     // buffers of consecutive passes (dry-to-wet u, v, vis; fix-up face values
@@ -549,6 +549,10 @@ static int tune_mg_tile(cms2d_flow *F) {
 static constexpr double REF_RTOL = CMS2D_REF_RTOL;
 static constexpr int REF_KMAX = 200;
 static constexpr int REF_SWEEPS = 40;
+// The u and v systems (Jacobi, diagonally dominant) converge at a steady rate
+// to FP32 BiCGStab's floor near 1e-6: a 1e-5 inner stop needs two sweeps for
+// the FP64 1e-10 where 1e-2 needs five, each with its own host round trip.
+static constexpr double REF_RTOL_UV = 1e-5;
 
 // Multigrid preconditioners of the pp solve (FP64 Krylov vectors) and of the
 // inner FP32 solve.
@@ -626,7 +630,7 @@ static int uv_solve_refine(cms2d_flow *F, KryCtx &X, StencilOp A) {
     HIPCHK(hipStreamSynchronize(F->st));
     const StencilOp32 A32{F->g, F->a32[0], F->a32[1], F->a32[2], F->a32[3]};
     KryCtx Xi = inner_ctx(X);
-    Xi.graph = nullptr;
+    Xi.rtol = REF_RTOL_UV;
     int total[2] = {0, 0};
     for (int sw = 0; sw < REF_SWEEPS && !(F->h_sc[1].done && F->h_sc[2].done); sw++) {
         hipLaunchKernelGGL(k_ref_start2<StencilOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, K);
@@ -715,12 +719,17 @@ static int capture_graphs(cms2d_flow *F) {
     const St &s = F->s;
     const StencilOp A{F->g, s.aN, s.aE, s.aS, s.aW};
     const KryCtx X = flow_ctx(F, false);
-    if (int rc = capture_bicg_pair<StencilOp, 2, 0, double>(X, F->Kuv, A, F->kgraph[1])) return rc;
     if (F->precision == CMS2D_PREC_FP32_REFINE) {
+        // the inner FP32 pairs of the refinement form (u and v; pp)
+        const StencilOp32 A32{F->g, F->a32[0], F->a32[1], F->a32[2], F->a32[3]};
+        if (int rc = capture_bicg_pair<StencilOp32, 2, 0, float>(inner_ctx(X), F->Kin2, A32, F->kgraph[4])) return rc;
         if (int rc = capture_bicg_pair<StencilOp32, 1, 1, float>(inner_ctx(X), F->Kin, inner_op(F), F->kgraph[3],
                                                                  InnerPre{F}))
             return rc;
-    } else if (F->solver == CMS2D_SOLVER_GMRES) {
+        return 0;
+    }
+    if (int rc = capture_bicg_pair<StencilOp, 2, 0, double>(X, F->Kuv, A, F->kgraph[1])) return rc;
+    if (F->solver == CMS2D_SOLVER_GMRES) {
         // the pp solve is GMRES
     } else if (F->precond == CMS2D_PRECOND_MG) {
         // the pp pair, its timed-group launches noted for the launch counts

@@ -445,6 +445,15 @@ struct TelPre {
     }
 };
 
+struct TelInnerPre {
+    cms2d_flow *F;
+    void operator()(int which) const {
+        KrySys<1, float> &Ki = F->Kin;
+        if (which == 0) tel_mg_apply<float>(F, Ki.p[0], Ki.zp, Ki.sc);
+        else tel_mg_apply<float>(F, Ki.r[0], Ki.zs, Ki.sc);
+    }
+};
+
 static TelOp tel_op(const TelHost &H, const double *a) { return TelOp{H.T.n, H.T.nd, H.T.c2c, H.T.meta, a}; }
 
 // The BiCGStab pairs of the u/v and pp systems, captured once at creation
@@ -453,6 +462,13 @@ static int tel_capture_graphs(cms2d_flow *F) {
     if (!F->use_graphs) return 0;
     TelHost &H = *F->tel;
     const KryCtx X = flow_ctx(F, false);
+    if (F->precision == CMS2D_PREC_FP32_REFINE) {
+        // the inner FP32 pairs of the refinement form (u and v; pp), on the
+        // FP32 coefficients converted before each of their solves
+        const TelOp32 A32{H.T.n, H.T.nd, H.T.c2c, H.T.meta, H.apc32};
+        if (int rc = capture_bicg_pair<TelOp32, 2, 0, float>(inner_ctx(X), F->Kin2, A32, F->kgraph[4])) return rc;
+        return capture_bicg_pair<TelOp32, 1, 1, float>(inner_ctx(X), F->Kin, A32, F->kgraph[3], TelInnerPre{F});
+    }
     if (int rc = capture_bicg_pair<TelOp, 2, 0, double>(X, H.Kuv, tel_op(H, H.s.am), F->kgraph[1])) return rc;
     const TelOp Ap = tel_op(H, H.s.apc);
     if (F->precond != CMS2D_PRECOND_MG) return capture_bicg_pair<TelOp, 1, 0, double>(X, H.Kpp, Ap, F->kgraph[0]);
@@ -498,7 +514,7 @@ static int tel_uv_solve_refine(cms2d_flow *F, KryCtx &X, const TelOp &A) {
     HIPCHK(hipStreamSynchronize(F->st));
     const TelOp32 A32{H.T.n, H.T.nd, H.T.c2c, H.T.meta, H.apc32};
     KryCtx Xi = inner_ctx(X);
-    Xi.graph = nullptr;
+    Xi.rtol = REF_RTOL_UV;
     int total[2] = {0, 0};
     for (int sw = 0; sw < REF_SWEEPS && !(F->h_sc[1].done && F->h_sc[2].done); sw++) {
         hipLaunchKernelGGL(k_ref_start2<TelOp32>, dim3(nb), dim3(KBLOCK), 0, F->st, A32, Ki, K);
@@ -513,14 +529,6 @@ static int tel_uv_solve_refine(cms2d_flow *F, KryCtx &X, const TelOp &A) {
     F->h_sc[2].final_iters = total[1];
     return 0;
 }
-struct TelInnerPre {
-    cms2d_flow *F;
-    void operator()(int which) const {
-        KrySys<1, float> &Ki = F->Kin;
-        if (which == 0) tel_mg_apply<float>(F, Ki.p[0], Ki.zp, Ki.sc);
-        else tel_mg_apply<float>(F, Ki.r[0], Ki.zs, Ki.sc);
-    }
-};
 // The pp solve of the FP32 form on the cell list: FP32 BiCGStab (FP32
 // coefficients and vectors, the quadtree V-cycle on FP32 right-hand sides) for
 // each correction, x and the residual in FP64, stopped on the FP64 solve's

@@ -25,8 +25,10 @@
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
  * cms2d_flow_kernel_stats (live kernel timing for measurement),
- * cms2d_flow_tuning (the launch configuration picked for a flow) and
- * cms2d_ensemble_step (independent domains side by side).
+ * cms2d_flow_tuning (the launch configuration picked for a flow),
+ * cms2d_ensemble_step (independent domains side by side), cms2d_solve_adjoint
+ * and cms2d_flow_adjoint (the discrete adjoint of a solve and of the time
+ * loop) and cms2d_solve_noalloc (for callers that cannot allocate).
  */
 #ifndef CMS2D_AMD_H
 #define CMS2D_AMD_H
@@ -182,9 +184,9 @@ int cms2d_flow_destroy(cms2d_flow *flow);
  * interp_lib.f90:2853-2873).  desc gives the physics and the base grid
  * (nx, ny, dx, dy: every cell lies in one base cell, which the multigrid
  * preconditioner coarsens); zb, mann and the forcing are per cell of `grid`;
- * one side_type names the water-level string of grid->bnd_cells.  FP64
- * BiCGStab with a fixed time step; the NVD schemes and water-level gradients
- * are built for Cartesian grids only.  Fields of cms2d_flow_get are per cell;
+ * one side_type names the water-level string of grid->bnd_cells.  BiCGStab
+ * (the FP64 form or CMS2D_PREC_FP32_REFINE) with a fixed time step; the NVD
+ * schemes and water-level gradients are built for Cartesian grids only.  Fields of cms2d_flow_get are per cell;
  * cms2d_flow_get_face gives out[6*c+k] in the cell's face order. */
 int cms2d_flow_create_tel(const cms2d_flow_desc *desc, const cms2d_tel_grid *grid, cms2d_flow **out);
 /* Advances nsteps time steps: timing, flow_imp and flow_update per step. */
