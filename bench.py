@@ -437,6 +437,13 @@ def measure(a, case, label, groups):
     # the roofline kernel: the single kernel with the largest total time
     # (mg_coarse is a span of nine launches, reported beside it)
     dom = max((k for k in kern if k != "mg_coarse"), key=lambda k: kern[k]["total_ms"])
+    # flow_balance and flow_step_stat as the device keeps them (cms2d_flow_balance):
+    # balance = storage + efflux = dV - dt Q_in of a step [m^3]
+    b = g.balance()
+    vbal = {"last_step": {k: b[k] for k in ("storage", "efflux", "balance", "error_pct")},
+            "cumulative": {k: b["cum_" + k] for k in ("storage", "efflux", "balance", "error_pct")},
+            "steps": b["steps"], "mean_balance_m3_per_step": b["cum_balance"] / max(b["steps"], 1),
+            "extremes_last_step": {k: (b[k], b[k + "_cell"]) for k in ("wse", "u", "v", "courant")}}
     value = ncells * a.steps / elapsed
     model_b = (STEP_MODEL["per_step"] + STEP_MODEL["per_outer"] * outer / a.steps
                + STEP_MODEL["per_krylov_iter"] * kry.sum() / a.steps)
@@ -444,7 +451,7 @@ def measure(a, case, label, groups):
     res = {"value": value, "ms_per_step": elapsed * 1e3 / a.steps, "cells": ncells, "label": label,
            "mg_tile_x": mg_tile, "graph_replay": graphs, "outer_iters_per_step": outer / a.steps,
            "krylov_iters_per_step": {"pp": kry[0] / a.steps, "u": kry[1] / a.steps, "v": kry[2] / a.steps},
-           "kern": kern, "dominant": dom,
+           "kern": kern, "dominant": dom, "volume_balance": vbal,
            "step_model_roofline": {
                "bytes_per_cell_step": model_b, "achieved": model_gbs, "peak": HBM_PEAK_GBS,
                "unit": "GB/s", "frac": model_gbs / HBM_PEAK_GBS,
@@ -517,6 +524,7 @@ def main():
             "timed_groups": T["kern"],
         },
         "step_model_roofline": T["step_model_roofline"],
+        "volume_balance": T["volume_balance"],
         "cpu_baseline": None,
     }
     if fp32_tel is not None:
@@ -534,7 +542,7 @@ def main():
                          "frac": Dc["achieved_GBs"] / HBM_PEAK_GBS if Dc["achieved_GBs"] else None,
                          "traffic": TRAFFIC_PER_CELL[C["dominant"]] * C["cells"] if C["dominant"] in TRAFFIC_PER_CELL
                          else None, "timed_groups": C["kern"]},
-            "step_model_roofline": C["step_model_roofline"]}
+            "step_model_roofline": C["step_model_roofline"], "volume_balance": C["volume_balance"]}
         if not a.no_fp32:
             out["cartesian_uniform"]["fp32_refine"] = fp32_form(a, case, g, C["value"])
         g.close()

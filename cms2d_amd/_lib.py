@@ -38,7 +38,7 @@ EXPORTS = [
     "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
     "cms2d_ensemble_step", "cms2d_solve", "cms2d_solve_noalloc", "cms2d_flow_create_tel",
     "cms2d_solve_adjoint",
-    "cms2d_flow_adjoint",
+    "cms2d_flow_adjoint", "cms2d_flow_balance",
 ]
 
 
@@ -68,6 +68,14 @@ class StepInfo(C.Structure):
     _fields_ = [("niter", C.c_int), ("iconv", C.c_int), ("rmom", C.c_double * 3),
                 ("kry_iters", C.c_int * 3), ("nwet", C.c_int), ("kernel_ms", C.c_double),
                 ("dt", C.c_double)]
+
+
+class Balance(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "storage", "efflux", "flux", "balance", "net", "error_pct",
+        "cum_storage", "cum_efflux", "cum_flux", "cum_balance", "cum_net", "cum_error_pct",
+        "wse", "u", "v", "courant")] + [
+        (n, C.c_int) for n in ("wse_cell", "u_cell", "v_cell", "courant_cell", "steps")]
 
 
 _LIB = None
@@ -109,6 +117,7 @@ def lib():
     L.cms2d_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, D, C.c_double, C.c_double,
                               C.c_double, C.c_int, D, I]
     L.cms2d_solve_noalloc.argtypes = L.cms2d_solve.argtypes
+    L.cms2d_flow_balance.argtypes = [C.c_void_p, C.POINTER(Balance)]
     L.cms2d_flow_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, I, D, D, D, D, D, D]
     L.cms2d_solve_adjoint.argtypes = [C.c_int, C.c_int, C.c_int, D, I, D, D, D, C.c_double, D, C.c_double,
                                       C.c_double, C.c_int, D, D, D, D, I]

@@ -8,6 +8,7 @@
 #include "krylov_kernels.hip"
 #include "mg_kernels.hip"
 #include "tel_kernels.hip"
+#include "balance_kernels.hip"
 #include "adjoint_kernels.hip"
 #include "../../include/cms2d_amd.h"
 
@@ -181,6 +182,10 @@ struct cms2d_flow {
     double wse_gx = 0.0, wse_gy = 0.0;  // WSE_GRADIENTS of the curve sides
     hipEvent_t step_ev[2]{};
     cms::Tape *tape = nullptr;   // records the outer iterations for the reverse sweep (or null)
+    // volume balance and extremes of the last step (flow_balance, flow_step_stat):
+    // per-block partials on the fixed Krylov grid and the running totals
+    double *bal_part = nullptr;
+    cms::BalAcc *bal = nullptr;
 };
 
 // Lagrange interpolation of the curve (t, v) at time xi with the window search
@@ -1224,6 +1229,9 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         F->Kin.red.pr[0] = bi; F->Kin.red.pv[0] = bi + 2 * nb; F->Kin.red.pt[0] = bi + 3 * nb;
     }
     if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
+    if (int rc = dalloc(F, &F->bal_part, BAL_PART * (size_t)F->nblk_kry)) return rc;
+    if (int rc = dalloc(F, &F->bal, 1)) return rc;
+    F->state.push_back({F->bal, sizeof(BalAcc)});
     // Krylov vectors
     {
         double *kv[3][7];
@@ -1407,6 +1415,9 @@ static int step_once(cms2d_flow *F) {
     F->timesecs += F->deltime;
     F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
     if (int rc = flow_imp(F)) return rc;
+    hipLaunchKernelGGL(k_bal_part, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, F->g, F->s, F->P, F->bal_part);
+    hipLaunchKernelGGL(k_bal_fin, dim3(1), dim3(KBLOCK), 0, F->st, F->g, F->s, F->P, (const double *)F->bal_part,
+                       F->nblk_kry, F->bal);
     LAUNCH(k_update, F->nblk_pad, F->g, F->s);
     return 0;
 }
@@ -1436,6 +1447,26 @@ extern "C" int cms2d_flow_step(cms2d_flow *F, int nsteps, cms2d_step_info *info)
             for (int i = 0; i < F->g.nx; i++) nw += w[(size_t)(j + 1) * F->g.nxp + i + 1];
         info->nwet = nw;
     }
+    return CMS2D_OK;
+}
+
+// flow_balance (flow_imp.F90:2260-2372) and flow_step_stat (flow.F90:1912-2008)
+// of the last step, from the totals the device keeps (k_bal_part, k_bal_fin).
+extern "C" int cms2d_flow_balance(cms2d_flow *F, cms2d_balance *out) {
+    if (!F || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(F->dev));
+    BalAcc a{};
+    HIPCHK(hipMemcpyAsync(&a, F->bal, sizeof a, hipMemcpyDeviceToHost, F->st));
+    HIPCHK(hipStreamSynchronize(F->st));
+    out->storage = a.stg; out->efflux = a.bnd; out->flux = a.flux;
+    out->balance = a.bal; out->net = a.net; out->error_pct = a.err;
+    out->cum_storage = a.cstg; out->cum_efflux = a.cbnd; out->cum_flux = a.cflux;
+    out->cum_balance = a.cbal; out->cum_net = a.cnet; out->cum_error_pct = a.cerr;
+    out->wse = a.xv[0] * GRAVINV;   // pxtrm*gravinv (flow.F90:2001-2005)
+    out->u = a.xv[1]; out->v = a.xv[2]; out->courant = a.xv[3];
+    out->wse_cell = (int)a.xc[0]; out->u_cell = (int)a.xc[1]; out->v_cell = (int)a.xc[2];
+    out->courant_cell = (int)a.xc[3];
+    out->steps = (int)a.steps;
     return CMS2D_OK;
 }
 

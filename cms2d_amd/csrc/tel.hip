@@ -1235,6 +1235,8 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
             H.Kuv.red.pr[q] = b; H.Kuv.red.pv[q] = b + 2 * nb; H.Kuv.red.pt[q] = b + 3 * nb;
         }
         if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
+        if (int rc = dalloc(F, &F->bal_part, BAL_PART * nb)) return rc;
+        if (int rc = dalloc(F, &F->bal, 1)) return rc;
         double *kv[3][7];
         for (int q = 0; q < 3; q++)
             for (int mm = 0; mm < 7; mm++)
@@ -1339,6 +1341,9 @@ static int tel_step(cms2d_flow *F, int nsteps, cms2d_step_info *info) {
         F->timesecs += F->deltime;
         F->ramp = ramp_func(F->timesecs / 3600.0, F->rampdur);
         if (int rc = tel_flow_imp(F)) return rc;
+        hipLaunchKernelGGL(k_tl_bal_part, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, H.T, H.s, F->P, F->bal_part);
+        hipLaunchKernelGGL(k_tl_bal_fin, dim3(1), dim3(KBLOCK), 0, F->st, H.T, H.s, F->P,
+                           (const double *)F->bal_part, F->nblk_kry, F->bal);
         TLAUNCH(k_tl_update, H.nblk_nd, H.T, H.s);
         // flow_update clears dppx, dppy: the next pp gradient starts from zero
         TLAUNCH(k_tl_zero, H.nblk_nd, H.gpp[0][H.cpp], (long)H.T.nd);

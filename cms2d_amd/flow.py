@@ -107,6 +107,16 @@ class FlowStep:
                 "kry": list(info.kry_iters), "nwet": info.nwet, "kernel_ms": info.kernel_ms,
                 "dt": info.dt}
 
+    def balance(self):
+        """flow_balance and flow_step_stat of the last step (cms2d_flow_balance):
+        the step's and the cumulative water volumes [m^3] (storage, efflux
+        through the water-level string, flux through every cell face, balance,
+        net, percentage errors) and the extreme water level, u, v and Courant
+        number over the wet cells with their cells."""
+        b = _lib.Balance()
+        check(_lib.lib().cms2d_flow_balance(self._h, C.byref(b)))
+        return {name: getattr(b, name) for name, _ in _lib.Balance._fields_}
+
     def get(self, name):
         out = np.empty(self.ncells)
         check(_lib.lib().cms2d_flow_get(self._h, name.encode(), _dp(out)))

@@ -61,6 +61,16 @@ module cms2d_hip
         real(c_double) :: kernel_ms = 0, dt = 0
     end type
 
+    ! struct cms2d_balance: flow_balance and flow_step_stat of the last step
+    type, bind(C), public :: cms2d_balance
+        real(c_double) :: storage = 0, efflux = 0, flux = 0, balance = 0, net = 0, error_pct = 0
+        real(c_double) :: cum_storage = 0, cum_efflux = 0, cum_flux = 0, cum_balance = 0, cum_net = 0, &
+                          cum_error_pct = 0
+        real(c_double) :: wse = 0, u = 0, v = 0, courant = 0
+        integer(c_int) :: wse_cell = 0, u_cell = 0, v_cell = 0, courant_cell = 0   ! 0-based
+        integer(c_int) :: steps = 0
+    end type
+
     ! struct cms2d_tel_grid: the content of a .tel file (read_tel)
     type, bind(C), public :: cms2d_tel_grid
         integer(c_int) :: ncells = 0
@@ -153,6 +163,13 @@ module cms2d_hip
             real(c_double), intent(in) :: whgt(*), wper(*), wlen(*), wunitx(*), wunity(*)
             integer(c_int) :: rc
         end function
+        ! flow_balance (flow_imp.F90:2260-2372) and flow_step_stat (flow.F90:1912-2008)
+        function cms2d_flow_balance(flow, bal) bind(C, name='cms2d_flow_balance') result(rc)
+            import :: c_int, c_ptr, cms2d_balance
+            type(c_ptr), value :: flow
+            type(cms2d_balance), intent(out) :: bal
+            integer(c_int) :: rc
+        end function
         ! discrete adjoint of the time loop: dL/dn and dL/dzb of a gauge water-level loss
         function cms2d_flow_adjoint(flow, nsteps, every, ngauge, gauge_cells, gauge_obs, geta, eta_gauge, &
                                     loss, gmann, gzb) bind(C, name='cms2d_flow_adjoint') result(rc)
@@ -169,7 +186,7 @@ module cms2d_hip
     public :: cms2d_solve_ref, cms2d_flow_create, cms2d_flow_step, cms2d_flow_get, cms2d_flow_destroy
     public :: cms2d_error_message, cms2d_device_count
     public :: cms2d_flow_checkpoint, cms2d_flow_restore, cms2d_flow_set_waves, cms2d_flow_set_wind
-    public :: cms2d_flow_adjoint
+    public :: cms2d_flow_adjoint, cms2d_flow_balance
     public :: cms2d_flow_create_tel
 
 contains

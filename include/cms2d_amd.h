@@ -22,6 +22,8 @@
  *   cms2d_flow_set_wind  <- windcurve_eval (constant wind) source/meteorological/meteorological.F90:995-1033
  *   cms2d_flow_checkpoint/_restore <- hot start write/read (exact-resume variant)
  *                       source/hot_start/hot_start.F90:693-1122,1125-1344,1512-1560
+ *   cms2d_flow_balance <- flow_balance and flow_step_stat source/flow/flow_imp.F90:2260-2372,
+ *                       source/flow/flow.F90:1912-2008
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
  * cms2d_flow_kernel_stats (live kernel timing for measurement),
@@ -191,6 +193,29 @@ int cms2d_flow_destroy(cms2d_flow *flow);
 int cms2d_flow_create_tel(const cms2d_flow_desc *desc, const cms2d_tel_grid *grid, cms2d_flow **out);
 /* Advances nsteps time steps: timing, flow_imp and flow_update per step. */
 int cms2d_flow_step(cms2d_flow *flow, int nsteps, cms2d_step_info *info);
+
+/* Water volume balance of the last step and since creation (flow_balance,
+ * source/flow/flow_imp.F90:2260-2372, two-level time scheme, no rain or
+ * evaporation) and the step's extremes over the wet cells (flow_step_stat,
+ * source/flow/flow.F90:1912-2008), computed on the device after every step.
+ * Volumes in m^3: storage = sum area (h - h1); efflux = dt times the outward
+ * flux through the water-level string's faces; flux = dt times the sum of
+ * every cell's outward face fluxes; balance = storage + efflux; net =
+ * storage + flux; error_pct = 100 balance / max(|storage| + |efflux|, 1e-10);
+ * cum_error_pct = 100 cum_balance / max(the largest |cum_storage| so far,
+ * 1e-10).  Extremes: the signed water level, u and v of largest magnitude and
+ * the largest Courant number dt max((|u| + c)/dx, (|v| + c)/dy), c = sqrt(g h),
+ * each with its cell (0-based: row-major on a Cartesian grid, list order on a
+ * telescoping one); the first such cell in cell order, cell 0 and 0 when no
+ * wet cell exceeds 0.  The running sums are part of the checkpointed state. */
+typedef struct {
+    double storage, efflux, flux, balance, net, error_pct;
+    double cum_storage, cum_efflux, cum_flux, cum_balance, cum_net, cum_error_pct;
+    double wse, u, v, courant;
+    int wse_cell, u_cell, v_cell, courant_cell;
+    int steps;                    /* steps counted since creation */
+} cms2d_balance;
+int cms2d_flow_balance(cms2d_flow *flow, cms2d_balance *out);
 /* Copies an active-cell field to host memory, row-major [nx*ny]:
  * "eta","u","v","p","h","iwet","vis","cfrict","coefman","z0","uv","bsvel","pp","dpx","dpy",
  * "zb","fc","wavestrx","wavestry", and "maxeta" (running maximum of eta over the steps,

@@ -213,3 +213,76 @@ class Oracle:
     @property
     def time(self):
         return lib().cms_oracle_time(self.h)
+
+
+GRAV = 9.81
+GRAVINV = 1.0 / 9.81      # flow_def gravinv
+
+
+class VolumeBalance:
+    """flow_balance (source/flow/flow_imp.F90:2260-2372; ntsch 1, no rain or
+    evaporation) and flow_step_stat (source/flow/flow.F90:1912-2008) restated
+    over the exported fields of a model (this oracle or the HIP flow: get and
+    get_face with the same names).  Call before() ahead of a step and after()
+    behind it.  Volumes in m^3; extremes over the wet cells, cells 0-based;
+    flow_step_stat keeps the first cell whose key is strictly larger than the
+    best so far, from key 0 at the first cell."""
+
+    def __init__(self, case):
+        tel = getattr(case, "tel", None)
+        self.dt = case.dt
+        if tel is not None:
+            g = np.asarray(tel.geom)
+            self.area, self.dx, self.dy = g[:, 2] * g[:, 3], g[:, 2], g[:, 3]
+            # the string's faces: the domain-edge faces of its cells (the
+            # others among them are walls with zero flux)
+            cells = np.asarray(tel.bnd_cells)
+            edge = tel.edge_faces()
+            self.str_faces = [(c, k) for c in cells for k in np.nonzero(edge[c])[0]]
+        else:
+            n = case.nx * case.ny
+            self.area = np.full(n, case.dx * case.dy)
+            self.dx, self.dy = np.full(n, case.dx), np.full(n, case.dy)
+            side = case.th_side          # N, E, S, W; -1: no water-level side
+            self.str_faces = []
+            if side >= 0:
+                j, i = np.mgrid[0:case.ny, 0:case.nx]
+                on = [j == case.ny - 1, i == case.nx - 1, j == 0, i == 0][side]
+                self.str_faces = [(c, side) for c in np.nonzero(on.ravel())[0]]
+        self.cum = dict(storage=0.0, efflux=0.0, flux=0.0, norm=0.0)
+        self.h1 = None
+
+    def before(self, model):
+        self.h1 = model.get("h")
+
+    def after(self, model, dt=None):
+        """The step's values and the running totals; dt: the step's time step
+        when it varies (VARIABLE_TIMESTEP)."""
+        dt = self.dt if dt is None else dt
+        h, f = model.get("h"), model.get_face("flux")
+        stg = float((self.area * (h - self.h1)).sum())
+        flux = float((dt * f.sum(axis=1)).sum())
+        bnd = float(sum(dt * f[c, k] for c, k in self.str_faces))
+        c = self.cum
+        c["storage"] += stg
+        c["flux"] += flux
+        c["efflux"] += bnd
+        c["norm"] = max(c["norm"], abs(c["storage"]))
+        out = dict(dt=dt, storage=stg, efflux=bnd, flux=flux, balance=stg + bnd, net=stg + flux,
+                   error_pct=100.0 * (stg + bnd) / max(abs(stg) + abs(bnd), 1e-10),
+                   cum_storage=c["storage"], cum_efflux=c["efflux"], cum_flux=c["flux"],
+                   cum_balance=c["storage"] + c["efflux"], cum_net=c["storage"] + c["flux"],
+                   cum_error_pct=100.0 * (c["storage"] + c["efflux"]) / max(c["norm"], 1e-10))
+        wet = model.get("iwet") == 1
+        p, u, v = model.get("p"), model.get("u"), model.get("v")
+        cel = np.sqrt(GRAV * np.where(wet, h, 0.0))
+        cr = dt * np.maximum((np.abs(u) + cel) / self.dx, (np.abs(v) + cel) / self.dy)
+        for name, key, val in (("wse", np.abs(p), p * GRAVINV), ("u", np.abs(u), u), ("v", np.abs(v), v),
+                               ("courant", cr, cr)):
+            k = np.where(wet, key, -1.0)
+            i = int(np.argmax(k))
+            if k[i] > 0.0:
+                out[name], out[name + "_cell"] = float(val[i]), i
+            else:
+                out[name], out[name + "_cell"] = 0.0, 0
+        return out

@@ -20,6 +20,13 @@
 !   with zero wave height (source/waves/wave.F90:506-540).  With output file
 !   "-" nothing is written.  Prints the wall time of every step (timing,
 !   flow_imp, flow_update; output excluded) as "step_seconds <n> <t>".
+!   With the environment variable CMS2D_REF_BALANCE naming a file (and the
+!   card WATER_VOLUME_BALANCE ON, so that flow_imp calls flow_balance,
+!   source/flow/flow_imp.F90:305) every step also appends, as float64 stream
+!   data, the cumulative volumes of flow_balance (volH2Ocumstg, volH2Ocumbnd,
+!   volH2Ocumflux, volH2Ocumnorm; flow_imp.F90:2260-2372) and the extremes of
+!   flow_step_stat (pxtrm, idp, uxtrm, idu, vxtrm, idv, crmax, idcr, cell
+!   numbers 1-based; source/flow/flow.F90:1912-2008).
 !=============================================================================
 program ref_run
     use comvarbl,  only: ctlfile, flowpath, casename
@@ -27,11 +34,12 @@ program ref_run
     use diag_def,  only: dgfile, dgunit, interactive
     use geo_def,   only: telfile
     use size_def,  only: ncells
-    use flow_def,  only: eta, u, v, iwet
+    use flow_def,  only: eta, u, v, iwet, volH2Ocumstg, volH2Ocumbnd, volH2Ocumflux, volH2Ocumnorm
+    use comvarbl,  only: pxtrm, idp, uxtrm, idu, vxtrm, idv, crmax, idcr
     use wave_flowgrid_def, only: wavestrx, wavestry, constant_waves
     implicit none
-    character(len=400) :: dir, name, arg, outfile, forcefile
-    integer :: nsteps, n, i, ou, fu
+    character(len=400) :: dir, name, arg, outfile, forcefile, balfile
+    integer :: nsteps, n, i, ou, fu, bu, blen
     integer(8) :: t0, t1, rate
     logical :: dump
     real(8), allocatable :: buf(:)
@@ -76,6 +84,8 @@ program ref_run
       deallocate(buf)
     endif
 
+    call get_environment_variable('CMS2D_REF_BALANCE', balfile, blen)
+    if (blen > 0) open(newunit=bu, file=trim(balfile), access='stream', form='unformatted', status='replace')
     dump = trim(outfile) /= '-'
     if (dump) open(newunit=ou, file=trim(outfile), access='stream', form='unformatted', status='replace')
     allocate(buf(ncells))
@@ -87,6 +97,9 @@ program ref_run
       call flow_update
       call system_clock(t1)
       write(*, '(a,i0,1x,es14.6)') 'step_seconds ', n, real(t1 - t0, 8) / real(rate, 8)
+      if (blen > 0) write(bu) real(volH2Ocumstg, 8), real(volH2Ocumbnd, 8), real(volH2Ocumflux, 8), &
+          real(volH2Ocumnorm, 8), real(pxtrm, 8), real(idp, 8), real(uxtrm, 8), real(idu, 8), &
+          real(vxtrm, 8), real(idv, 8), real(crmax, 8), real(idcr, 8)
       if (.not. dump) cycle
       write(ou) real(ncells, 8)
       do i = 1, ncells
@@ -107,6 +120,7 @@ program ref_run
       write(ou) buf
     enddo
     if (dump) close(ou)
+    if (blen > 0) close(bu)
 end program ref_run
 
 !-----------------------------------------------------------------------------

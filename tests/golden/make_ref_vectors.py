@@ -8,7 +8,9 @@ selected steps in tests/golden/ref_<case>.npz; with --double the same from
 oracle/_ref/ref_run8 (the reference built in double precision, build_ref.py
 --double) into tests/golden/ref64_<case>.npz.  The reference runs on one
 OpenMP thread (the run's thread count and precision are stored).  Regenerate
-with: python tests/golden/make_ref_vectors.py [--double] [case ...]
+with: python tests/golden/make_ref_vectors.py [--double] [case ...]; with
+--balance, the per-step volume balance and extremes of the double-precision
+build (ref64_balance_<case>.npz).
 """
 import os
 import sys
@@ -203,6 +205,33 @@ def _mk_dtvar():
 CASES["tidal_dtvar"] = (_mk_dtvar, 12, [1, 6, 7, 8, 9, 12])
 
 
+# flow_balance and flow_step_stat of every step (WATER_VOLUME_BALANCE ON):
+# ref64_balance_<case>.npz holds "bal" [nsteps, 12] (cumulative storage,
+# efflux, flux and norm of flow_balance; pxtrm, idp, uxtrm, idu, vxtrm, idv,
+# crmax, idcr of flow_step_stat, cells 1-based); written with --balance
+BALANCE = {
+    "tidal": (lambda: cases.tidal(nx=48, ny=32, dx=20.0, seed=1), 40),
+    "basin": (lambda: cases.basin(nx=40, ny=20, dx=10.0, seed=0), 40),
+    "square": (lambda: cases.square(nx=40, ny=24, dx=20.0, seed=3), 40),
+    "tel_band": (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1), 40),
+    "tel_patch": (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)), 40),
+    "tidal_dtvar": (_mk_dtvar, 12),
+}
+
+
+def write_balance(names):
+    exe = refcase.REF_RUN8
+    for name, (mk, nsteps) in BALANCE.items():
+        if names and name not in names:
+            continue
+        with tempfile.TemporaryDirectory() as d:
+            res = refcase.run(mk(), d + "/", nsteps, exe=exe, balance=True)
+        bal = np.stack([r["bal"] for r in res])
+        np.savez_compressed(os.path.join(HERE, f"ref64_balance_{name}.npz"), bal=bal, omp_threads=np.array(1),
+                            ikind=np.array(8))
+        print(f"wrote ref64_balance_{name}.npz ({nsteps} steps)")
+
+
 def only(names):
     return {k: v for k, v in CASES.items() if k in names}
 
@@ -211,6 +240,12 @@ def main():
     args = sys.argv[1:]
     double = "--double" in args
     args = [a for a in args if a != "--double"]
+    if "--balance" in args:
+        if not refcase.available(refcase.REF_RUN8):
+            print(f"{refcase.REF_RUN8} not built (python oracle/build_ref.py --double)", file=sys.stderr)
+            return 1
+        write_balance([a for a in args if a != "--balance"])
+        return 0
     exe = refcase.REF_RUN8 if double else refcase.REF_RUN
     prefix = "ref64_" if double else "ref_"
     if not refcase.available(exe):

@@ -57,7 +57,7 @@ def write_cart(case, depth, path):
         f.write("GRID_DEPTHS_BEGIN\n" + "".join(f"{v:.9g}\n" for v in depth) + "GRID_DEPTHS_END\n")
 
 
-def write_case(case, d, name="case", solver="BICGSTAB"):
+def write_case(case, d, name="case", solver="BICGSTAB", balance=False):
     os.makedirs(d, exist_ok=True)
     nx, ny = case.nx, case.ny
     tel = getattr(case, "tel", None)
@@ -139,6 +139,7 @@ MATRIX_SOLVER                       {solver}
 {f'WIND_DIRECTION_CURVE                "{name}_wdir.xys"' if case.wind_speed else ""}
 {fric_card}
 {"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
+{"WATER_VOLUME_BALANCE                ON" if balance else ""}
 OUTPUT_FILE_TYPE                    ASCII
 HORIZONTAL_PROJECTION_BEGIN
   DATUM                             None
@@ -161,9 +162,12 @@ END_PARAMETERS
     return d
 
 
-def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RUN):
-    """Returns a list of dicts (eta, u, v, iwet) after each step."""
-    write_case(case, d, name, solver)
+def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RUN, balance=False):
+    """Returns a list of dicts (eta, u, v, iwet) after each step; with
+    balance=True each also holds "bal": flow_balance's cumulative storage,
+    efflux, flux and norm, then flow_step_stat's pxtrm, idp, uxtrm, idu,
+    vxtrm, idv, crmax, idcr (cells 1-based)."""
+    write_case(case, d, name, solver, balance=balance)
     out = os.path.join(d, "state.bin")
     args = [exe, d.rstrip("/") + "/", name, str(nsteps), out]
     if case.noptset >= 3 and case.wavestrx is not None:
@@ -174,6 +178,9 @@ def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RU
     # one thread: three of the reference's OpenMP loops read values their own
     # iterations write, so their result depends on the thread schedule
     env = dict(os.environ, OMP_NUM_THREADS="1")
+    balfile = os.path.join(d, "balance.bin")
+    if balance:
+        env["CMS2D_REF_BALANCE"] = balfile
     r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"ref_run failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
@@ -186,4 +193,8 @@ def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RU
         blk = raw[pos:pos + 4 * n].reshape(4, n)
         pos += 4 * n
         res.append({"eta": blk[0], "u": blk[1], "v": blk[2], "iwet": blk[3]})
+    if balance:
+        bal = np.fromfile(balfile, dtype=np.float64).reshape(-1, 12)
+        for r, b in zip(res, bal):
+            r["bal"] = b
     return res
