@@ -79,6 +79,12 @@ class Case:
     noskew: int = 0           # SKEWNESS_CORRECTION OFF (telescoping grids)
     dtvar: int = 0            # VARIABLE_TIMESTEP (the reference's card; the flow objects take it
                               # as an argument)
+    q_side: int = -1          # side of a flux (river) string, 0=N,1=E,2=S,3=W, -1 none:
+    q_const: float = 0.0      # its total inflow [m^3/s] (FLUX_VALUE ... 'm^3/s'), or
+    q_times: object = None    # a curve: times [hr] and inflows [m^3/s] (FLUX_CURVE),
+    q_values: object = None   # Lagrange order q_order (0: the reference's default 1)
+    q_order: int = 0
+    q_cmvel: float = 0.0      # CONVEYANCE coefficient (0: the reference's default 0.667)
 
     def wind_vector(self):
         """(wndx, wndy) in grid coordinates as windcurve_init makes them
@@ -152,6 +158,29 @@ def tidal(nx=2000, ny=1500, dx=20.0, seed=1, nsteps=2880, dt=30.0, radstress=Tru
         noptset = 3
     return Case("tidal", nx, ny, dx, dx, zb, mann, wx, wy, 1, [0.5], [M2_SPEED_DEG_HR], [90.0],
                 dt, nsteps, noptset=noptset)
+
+
+def river(nx=48, ny=32, dx=20.0, seed=4, nsteps=40, dt=30.0, q_side=3, q=40.0, q_curve=None, q_order=0,
+          q_cmvel=0.0, shoal=False) -> Case:
+    """An estuary: the M2 tide on the east side and a river inflow (flux
+    string) on side q_side (west by default), depth 2 m at the west edge to
+    12 m at the east with bumps; q_curve = (hours, m^3/s) replaces the
+    constant q.  shoal=True lifts the west edge so that its southern half is
+    dry land: the river then enters through the wet part of its string only."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    depth = 2.0 + 10.0 * x[None, :] * np.ones((ny, 1))
+    if shoal:
+        depth = depth - 2.4 * np.exp(-x[None, :] * 8.0) * (1.0 - y[:, None])
+    depth = depth + _bumps(rng, nx, ny, 20, 0.15, max(2, nx / 40), max(3, nx / 12))
+    zb = f32(-depth).ravel()
+    mann = f32(0.025 * np.exp(0.2 * rng.standard_normal(nx * ny)))
+    c = Case("river", nx, ny, dx, dx, zb, mann, None, None, 1, [0.5], [M2_SPEED_DEG_HR], [90.0], dt, nsteps)
+    c.q_side, c.q_const, c.q_order, c.q_cmvel = q_side, q, q_order, q_cmvel
+    if q_curve is not None:
+        c.q_times, c.q_values = list(q_curve[0]), list(q_curve[1])
+    return c
 
 
 def square(nx=1000, ny=1000, dx=20.0, seed=3, nsteps=720, dt=60.0) -> Case:

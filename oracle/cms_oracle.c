@@ -91,6 +91,9 @@ struct cms_oracle {
     /* boundary strings (bnd_def) */
     int nW, *Wc, *Wk;
     int nTH, *THc, *THk;
+    int nQ, *Qc, *Qk;     /* flux (river) string Q_str, cells and faces in string order */
+    double qflux, cmvel, qcos, qsin;   /* its total inflow this step, conveyance, inflow direction */
+    int q_inc;            /* window start of the flux curve (Q_str%inc) */
     double *wsebnd, *wseadj;
     int wseadjust;
     int wse_inc;          /* window start of the water-level curve (H_str%inc) */
@@ -442,27 +445,28 @@ static void build_grid(cms_oracle *o)
  * by cellstr_bnd (bnd_lib.f90:215-316: the edge faces of every inner cell of
  * the string; the end cells take the direction of their neighbour in the
  * string), walls on every other edge face (W_str, boundaries.F90:2380-2435). */
-static void build_bnd(cms_oracle *o)
+/* The cells of a Cartesian side in string order (the order the string's
+ * cellstring file lists them: along x or y from the south-west). */
+static int side_cells(const cms_oracle *o, int side, int **cells)
 {
-    const int nx = o->nx, ny = o->ny, nc = o->ncells;
-    int side = o->c.th_side;
-    int ns = 0, *cells = NULL;
-    if (side >= 0 && o->c.tel_n > 0) {
-        ns = o->c.tel_nbnd;
-        cells = IALLOC(ns);
-        for (int j = 0; j < ns; j++) cells[j] = o->c.tel_bnd[j];
-    } else if (side >= 0) {
-        ns = (side == KE || side == KW) ? ny : nx;
-        cells = IALLOC(ns);
-        for (int j = 0; j < ns; j++) {
-            if (side == KE) cells[j] = j * nx + (nx - 1);
-            else if (side == KW) cells[j] = j * nx;
-            else if (side == KN) cells[j] = (ny - 1) * nx + j;
-            else cells[j] = j;
-        }
+    const int nx = o->nx, ny = o->ny;
+    const int ns = (side == KE || side == KW) ? ny : nx;
+    *cells = IALLOC(ns);
+    for (int j = 0; j < ns; j++) {
+        if (side == KE) (*cells)[j] = j * nx + (nx - 1);
+        else if (side == KW) (*cells)[j] = j * nx;
+        else if (side == KN) (*cells)[j] = (ny - 1) * nx + j;
+        else (*cells)[j] = j;
     }
+    return ns;
+}
+
+/* cellstr_bnd: the string's (cell, face) pairs */
+static int string_faces(const cms_oracle *o, const int *cells, int ns, int **sc, int **sk)
+{
+    const int nc = o->ncells;
     int cap = 4 * ns + 1, n = 0;
-    o->THc = IALLOC(cap); o->THk = IALLOC(cap);
+    *sc = IALLOC(cap); *sk = IALLOC(cap);
     for (int j = 0; j < ns && ns >= 2; j++) {
         const int id = cells[j];
         const int ref = j == 0 ? cells[1] : j == ns - 1 ? cells[ns - 2] : id;
@@ -474,11 +478,42 @@ static void build_bnd(cms_oracle *o)
                 for (int kk = 0; kk < o->ncf[id]; kk++)
                     if (o->idir[NF * id + kk] == o->idir[NF * ref + k]) { kf = kk; break; }
             }
-            o->THc[n] = id; o->THk[n] = kf; n++;
+            (*sc)[n] = id; (*sk)[n] = kf; n++;
         }
     }
-    o->nTH = n;
+    return n;
+}
+
+static void build_bnd(cms_oracle *o)
+{
+    const int nc = o->ncells;
+    int side = o->c.th_side;
+    int ns = 0, *cells = NULL;
+    if (side >= 0 && o->c.tel_n > 0) {
+        ns = o->c.tel_nbnd;
+        cells = IALLOC(ns);
+        for (int j = 0; j < ns; j++) cells[j] = o->c.tel_bnd[j];
+    } else if (side >= 0) {
+        ns = side_cells(o, side, &cells);
+    }
+    o->nTH = string_faces(o, cells, ns, &o->THc, &o->THk);
     free(cells);
+    /* the flux string (Cartesian sides): inflow along the inward normal,
+     * local_angle = (4 - idirface) 90 degrees (boundaries.F90:1750-1762) */
+    o->nQ = 0;
+    if (o->c.q_side >= 0 && o->c.tel_n == 0) {
+        ns = side_cells(o, o->c.q_side, &cells);
+        o->nQ = string_faces(o, cells, ns, &o->Qc, &o->Qk);
+        free(cells);
+        const double pi = acos(-1.0), deg2rad = pi / 180.0;
+        const double angle = (double)(float)((float)(4 - (o->c.q_side + 1)) * 90.0f);
+        o->qcos = cos(angle * deg2rad);
+        o->qsin = sin(angle * deg2rad);
+        o->cmvel = o->c.q_cmvel > 0.0 ? fmax(fmin(o->c.q_cmvel, 1.0), 0.5) : F4(0.667);
+        o->q_inc = 1;
+    } else {
+        o->Qc = IALLOC(1); o->Qk = IALLOC(1);
+    }
     o->wsebnd = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadj = DALLOC(o->nTH > 0 ? o->nTH : 1);
     o->wseadjust = o->nTH >= 3;          /* boundaries.F90:1370,1808 */
@@ -486,7 +521,8 @@ static void build_bnd(cms_oracle *o)
     o->wse_inc = 1;
     char *claimed = (char *)xcalloc((size_t)o->ncellsD, 1);
     for (int j = 0; j < o->nTH; j++) claimed[o->c2c[NF * o->THc[j] + o->THk[j]]] = 1;
-    n = 0;
+    for (int j = 0; j < o->nQ; j++) claimed[o->c2c[NF * o->Qc[j] + o->Qk[j]]] = 1;
+    int n = 0;
     for (int i = 0; i < nc; i++)
         for (int k = 0; k < o->ncf[i]; k++) {
             int nck = o->c2c[NF * i + k];
@@ -663,6 +699,11 @@ static void flow_pond(cms_oracle *o)
 {
     const int nc = o->ncells, nd = o->ncellsD;
     int *ipond = IALLOC(nd);
+    for (int j = 0; j < o->nQ; j++) {
+        int i = o->Qc[j], nck = o->c2c[NF * i + o->Qk[j]];
+        ipond[i] = o->iwet[i];
+        ipond[nck] = o->iwet[nck];
+    }
     for (int j = 0; j < o->nTH; j++) {
         int i = o->THc[j], nck = o->c2c[NF * i + o->THk[j]];
         ipond[i] = o->iwet[i];
@@ -702,6 +743,9 @@ static void flow_wetdry(cms_oracle *o)
         else o->iwet[i] = 1;
     }
     for (int j = 0; j < o->nW; j++) o->iwet[o->c2c[NF * o->Wc[j] + o->Wk[j]]] = 0;
+    /* forcing strings: the ghost takes its cell's flag (:1606-1613) */
+    for (int j = 0; j < o->nQ; j++) o->iwet[o->c2c[NF * o->Qc[j] + o->Qk[j]]] = o->iwet[o->Qc[j]];
+    for (int j = 0; j < o->nTH; j++) o->iwet[o->c2c[NF * o->THc[j] + o->THk[j]]] = o->iwet[o->THc[j]];
     /* narrow-channel removal (Cartesian and telescoping branch, :1342-1455),
      * sequential in place; wdir(i, d) sums the wet flags of the neighbours
      * of i in direction d, wdir2 those two steps away through an active cell
@@ -828,6 +872,18 @@ double cms_oracle_plagr(int m, const double *x, const double *y, int ni, double 
  * ------------------------------------------------------------------------- */
 static void bnd_eval(cms_oracle *o)
 {
+    /* flux string (:4078-4094): constant or curve (order nti limited to
+     * ntimes - 1, :1783), times the ramp */
+    if (o->nQ > 0) {
+        double q = o->c.q_const;
+        if (o->c.q_ntimes > 0) {
+            int ni = o->c.q_order ? o->c.q_order : 1;
+            if (ni > o->c.q_ntimes - 1) ni = o->c.q_ntimes - 1;
+            if (ni < 1) ni = 1;
+            q = cms_oracle_plagr(o->c.q_ntimes, o->c.q_times, o->c.q_values, ni, o->timehrs, &o->q_inc);
+        }
+        o->qflux = o->ramp * q;
+    }
     if (o->c.wse_ntimes > 0) {
         int ni = o->c.wse_order ? o->c.wse_order : 2;
         if (ni > o->c.wse_ntimes - 1) ni = o->c.wse_ntimes - 1;
@@ -1028,8 +1084,55 @@ int cms_oracle_defcor(cms_oracle *o, double *dsu, double *dsv)
 }
 
 /* bound_uv, boundaries.F90:3310-3490, with bnd_vel_openwse (bnd_vel.f90:35-95) */
+/* bndriver_vel (boundaries.F90:4259-4356): the flux string's inflow split
+ * over its faces in proportion to iwet(i) iwet(ghost) hk^cmvel / max(n, 1e-5)
+ * times hk (the conveyance h^(1+cmvel)/n), along the inflow direction; the
+ * ghost cells take the face velocities, the faces their fluxes. */
+static void bndriver_vel(cms_oracle *o)
+{
+    if (o->nQ == 0) return;
+    double sumw = 0.0;
+    for (int j = 0; j < o->nQ; j++) {
+        const int i = o->Qc[j], f = NF * i + o->Qk[j], nck = o->c2c[f];
+        const double val = o->iwet[i] * o->iwet[nck] * pow(o->hk[f], o->cmvel) / fmax(o->coefman[i], F4(1.0e-5));
+        o->u[nck] = val * o->qcos;
+        o->v[nck] = val * o->qsin;
+        sumw = sumw + o->ds[f] * o->hk[f] * (o->u[nck] * o->fnx[f] + o->v[nck] * o->fny[f]);
+    }
+    const double qnorm = o->qflux / fmax(fabs(sumw), F4(1.0e-6));
+    for (int j = 0; j < o->nQ; j++) {
+        const int i = o->Qc[j], f = NF * i + o->Qk[j], nck = o->c2c[f];
+        o->u[nck] = o->u[nck] * qnorm;
+        o->v[nck] = o->v[nck] * qnorm;
+        o->flux[f] = o->iwet[i] * o->iwet[nck] * o->ds[f] * o->hk[f] * (o->u[nck] * o->fnx[f] + o->v[nck] * o->fny[f]);
+        o->flux[NF * nck + o->llec[f]] = -o->flux[f];
+    }
+}
+
+/* bndflux / fluxbnd (boundaries.F90:4359-4414,4740-4764): the flux string's
+ * face fluxes from its ghost velocities */
+static void bndflux(cms_oracle *o)
+{
+    for (int j = 0; j < o->nQ; j++) {
+        const int i = o->Qc[j], f = NF * i + o->Qk[j], nck = o->c2c[f];
+        o->flux[f] = o->iwet[i] * o->ds[f] * o->hk[f] * (o->fnx[f] * o->u[nck] + o->fny[f] * o->v[nck]);
+        o->flux[NF * nck + o->llec[f]] = -o->flux[f];
+    }
+}
+
 static void bound_uv(cms_oracle *o)
 {
+    /* bnd_vel_flux (bnd_vel.f90:2-32): the flux faces' upwind coefficient
+     * times the ghost velocity moves into the sources */
+    for (int j = 0; j < o->nQ; j++) {
+        const int i = o->Qc[j], f = NF * i + o->Qk[j], nck = o->c2c[f];
+        const double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
+        o->acoef[f] = cms_oracle_schmcoef(1, ddk, o->flux[f]);
+        o->su[i] += o->acoef[f] * o->u[nck];
+        o->sv[i] += o->acoef[f] * o->v[nck];
+        o->sp[i] -= o->acoef[f];
+        o->acoef[f] = 0.0;
+    }
     for (int j = 0; j < o->nTH; j++) {
         int i = o->THc[j], k = o->THk[j], f = NF * i + k, nck = o->c2c[f];
         double ddk = o->visk[f] * o->hk[f] * o->dsxy[f];
@@ -1348,6 +1451,7 @@ static void coefsourcesink_pp(cms_oracle *o)
             o->flux[f] = fl;
             o->flux[NF * nck + o->llec[f]] = -fl;
         }
+    bndflux(o);
     for (int i = 0; i < nc; i++) {
         o->sp[i] = o->sppp0[i];
         double fs = 0.0;
@@ -1412,6 +1516,7 @@ static void bound_pp(cms_oracle *o)
         o->su[i] -= aploc * o->p[i];
         o->sp[i] = 0.0;
     }
+    for (int j = 0; j < o->nQ; j++) o->acoef[NF * o->Qc[j] + o->Qk[j]] = 0.0;
     if (o->nTH > 0) {
         const double *wse = o->wsebnd;
         if (o->wseadjust) {
@@ -1467,6 +1572,10 @@ static void pvfc(cms_oracle *o)
         int i = o->Wc[j];
         o->pp[o->c2c[NF * i + o->Wk[j]]] = o->pp[i];
     }
+    for (int j = 0; j < o->nQ; j++) {
+        int i = o->Qc[j];
+        o->pp[o->c2c[NF * i + o->Qk[j]]] = o->pp[i];
+    }
     der_grad_eval(o, o->gow, o->pp, o->dppx, o->dppy);
     for (int i = 0; i < nc; i++) {
         if (o->iwet[i] == 0) continue;
@@ -1508,6 +1617,11 @@ static void pvfc(cms_oracle *o)
     for (int j = 0; j < o->nW; j++) {
         int i = o->Wc[j];
         o->p[o->c2c[NF * i + o->Wk[j]]] = o->p[i];
+    }
+    for (int j = 0; j < o->nQ; j++) {
+        int i = o->Qc[j], nck = o->c2c[NF * i + o->Qk[j]];
+        o->p[nck] = o->p[i];
+        o->h[nck] = fmax(o->hmin, o->p[nck] * (1.0 / GRAV) - o->zb[nck]);
     }
 }
 
@@ -1620,6 +1734,10 @@ static void flow_eddyvis(cms_oracle *o)
         }
     }
     /* bndcopy2ghost: forcing strings */
+    for (int j = 0; j < o->nQ; j++) {
+        int i = o->Qc[j];
+        o->vis[o->c2c[NF * i + o->Qk[j]]] = o->vis[i];
+    }
     for (int j = 0; j < o->nTH; j++) {
         int i = o->THc[j];
         o->vis[o->c2c[NF * i + o->THk[j]]] = o->vis[i];
@@ -1743,7 +1861,7 @@ static int flow_recalculate(cms_oracle *o)
     }
     memcpy(o->flux, o->flux1, sizeof(double) * NF * nd);
     flow_wetdry(o);
-    /* bndflux: flux and cross-shore strings only; none here */
+    bndflux(o);
     der_grad_cbgg(o, o->gow, 1);
     flow_grad_interp(o);
     fric_eval(o);
@@ -1762,6 +1880,7 @@ restart:
     flow_srcsnk_static(o);
     o->kry[0] = o->kry[1] = o->kry[2] = 0;
     for (o->niter = 1; o->niter <= o->maxit; o->niter++) {
+        bndriver_vel(o);
         coeffsourcesink_uv(o);
         if (o->c.ndsch >= 5 && o->c.ndsch <= 9) {   /* deferred corrections, flow_imp.F90:140-147 */
             double *dsu = DALLOC(o->ncells), *dsv = DALLOC(o->ncells);
@@ -1779,6 +1898,8 @@ restart:
         o->kry[0] += solve_sys(o, o->su, o->sp, o->rsp, o->pp, 1, &o->rmom[0]);
         pvfc(o);
         flow_grad_interp(o);
+        bndriver_vel(o);
+        bndflux(o);
         fric_eval(o);
         flow_eddyvis(o);
         check_conv(o);
@@ -1812,6 +1933,7 @@ static void flow_update(cms_oracle *o)
 static double bnd_inflow(const cms_oracle *o)
 {
     double q = 0.0;
+    for (int j = 0; j < o->nQ; j++) q -= o->flux[NF * o->Qc[j] + o->Qk[j]];
     for (int j = 0; j < o->nTH; j++) q -= o->flux[NF * o->THc[j] + o->THk[j]];
     return q * o->dtime;
 }
@@ -1828,6 +1950,7 @@ static void flow_convflux(cms_oracle *o)
                          * (o->fintp[f] * unk + (1.0 - o->fintp[f]) * uni);
             o->flux[NF * nck + o->llec[f]] = -o->flux[f];
         }
+    bndflux(o);
 }
 
 static void init_state(cms_oracle *o);
@@ -1869,6 +1992,14 @@ cms_oracle *cms_oracle_create(const cms_case *c)
         o->cmb[i] = 1.0;
         o->wavestrx[i] = c->wavestrx ? c->wavestrx[i] : 0.0;
         o->wavestry[i] = c->wavestry ? c->wavestry[i] : 0.0;
+    }
+    /* fric_init's copy of the friction to the dummy cells (friction.F90:397-407)
+     * loops `do i=1,ncells+1,ncellsD`: one pass, i = 1, so the first active
+     * cell takes cfrict, coefman and z0 of its first active neighbour in face
+     * order, and the dummy cells keep theirs. */
+    for (int k = 0; k < o->ncf[0]; k++) {
+        const int nck = o->c2c[NF * 0 + k];
+        if (nck < nc) { o->cfrict[0] = o->cfrict[nck]; o->coefman[0] = o->coefman[nck]; o->z0[0] = o->z0[nck]; break; }
     }
     /* Coriolis, flow.F90:720-736: f-plane fc = 2 omega sin(lat), beta-plane
      * fc + 2 omega cos(lat)/R (y - mean y); omega = 2 pi/86400, R = 6371 km
@@ -1940,7 +2071,7 @@ void cms_oracle_destroy(cms_oracle *o)
     if (!o) return;
     void *ptrs[] = {o->c2c, o->llec, o->nxy, o->kxy, o->dx, o->dy, o->x, o->y, o->area, o->ds, o->dn,
                     o->fnx, o->fny, o->rx, o->ry, o->dsx, o->dsy, o->dsxy, o->fintp, o->Wc, o->Wk, o->THc,
-                    o->THk, o->wsebnd, o->wseadj, o->u, o->v, o->p, o->h, o->pp, o->dppx, o->dppy, o->dux,
+                    o->THk, o->Qc, o->Qk, o->wsebnd, o->wseadj, o->u, o->v, o->p, o->h, o->pp, o->dppx, o->dppy, o->dux,
                     o->duy, o->dvx, o->dvy, o->dpx, o->dpy, o->vis, o->zb, o->eta, o->u1, o->v1, o->p1, o->h1,
                     o->iwet, o->iwet1, o->icorner, o->flux, o->flux1, o->hk, o->pk, o->visk, o->zbk, o->acoef,
                     o->su, o->sv, o->sp, o->spu, o->spv, o->ssu0, o->ssv0, o->spuv0, o->sppp0, o->sspp0,

@@ -73,6 +73,20 @@ typedef struct {
     const int *tel_bnd;
     int tel_nbnd;
     int noskew;               /* SKEWNESS_CORRECTION OFF (the default is on, flow.F90:101) */
+    /* Flux (river) string Q_str on a Cartesian side (bnd_block.f90:201-208,429-444;
+     * boundaries.F90:1742-1790,4078-4094; bndriver_vel :4259-4356): q_side >= 0 names
+     * the side whose cells take the total inflow [m^3/s per boundary, positive into
+     * the domain]: q_ntimes = 0 the constant q_const, else the curve q_times [hr] /
+     * q_values, Lagrange order q_order (0 -> 1), both times the ramp; distributed over
+     * the string in proportion to hk^(1+q_cmvel)/n (q_cmvel <= 0 -> 0.667, the
+     * default-real literal) along the inward normal. */
+    int q_side;
+    double q_const;
+    const double *q_times;
+    const double *q_values;
+    int q_ntimes;
+    int q_order;
+    double q_cmvel;
 } cms_case;
 
 typedef struct {

@@ -32,6 +32,8 @@ class CmsCase(C.Structure):
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
         ("tel_n", C.c_int), ("tel_geom", D), ("tel_nb", I), ("tel_bnd", I), ("tel_nbnd", C.c_int),
         ("noskew", C.c_int),
+        ("q_side", C.c_int), ("q_const", C.c_double), ("q_times", D), ("q_values", D),
+        ("q_ntimes", C.c_int), ("q_order", C.c_int), ("q_cmvel", C.c_double),
     ]
 
 
@@ -138,6 +140,15 @@ class Oracle:
             c.tel_n, c.tel_geom, c.tel_nb = len(geom), _dp(geom), nb.ctypes.data_as(I)
             c.tel_bnd, c.tel_nbnd = bnd.ctypes.data_as(I), len(bnd)
         c.noskew = int(getattr(case, "noskew", 0))
+        c.q_side = getattr(case, "q_side", -1)
+        if c.q_side >= 0:
+            if tel is not None:
+                raise ValueError("flux strings are restated on Cartesian grids only")
+            c.q_const = case.q_const
+            if case.q_times is not None:
+                c.q_times, c.q_values = arr(case.q_times), arr(case.q_values)
+                c.q_ntimes, c.q_order = len(case.q_times), case.q_order
+            c.q_cmvel = case.q_cmvel
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)
@@ -243,12 +254,13 @@ class VolumeBalance:
             n = case.nx * case.ny
             self.area = np.full(n, case.dx * case.dy)
             self.dx, self.dy = np.full(n, case.dx), np.full(n, case.dy)
-            side = case.th_side          # N, E, S, W; -1: no water-level side
+            # every forcing string: the flux side and the water-level side
             self.str_faces = []
-            if side >= 0:
-                j, i = np.mgrid[0:case.ny, 0:case.nx]
-                on = [j == case.ny - 1, i == case.nx - 1, j == 0, i == 0][side]
-                self.str_faces = [(c, side) for c in np.nonzero(on.ravel())[0]]
+            j, i = np.mgrid[0:case.ny, 0:case.nx]
+            for side in (getattr(case, "q_side", -1), case.th_side):     # N, E, S, W; -1: none
+                if side >= 0:
+                    on = [j == case.ny - 1, i == case.nx - 1, j == 0, i == 0][side]
+                    self.str_faces += [(c, side) for c in np.nonzero(on.ravel())[0]]
         self.cum = dict(storage=0.0, efflux=0.0, flux=0.0, norm=0.0)
         self.h1 = None
 

@@ -204,6 +204,19 @@ def _mk_dtvar():
 
 CASES["tidal_dtvar"] = (_mk_dtvar, 12, [1, 6, 7, 8, 9, 12])
 
+# flux (river) strings (FLUX_BEGIN blocks): a constant inflow on the west side
+# with the tide on the east, on the north side, a Lagrange curve, a shoal that
+# dries part of the string, and the CONVEYANCE card on the south side
+RIVER = {
+    "river": dict(),
+    "river_north": dict(q_side=0),
+    "river_curve": dict(q_curve=([0.0, 0.005, 0.01, 0.3], [0.0, 30.0, 60.0, 45.0]), q_order=2),
+    "river_shoal": dict(shoal=True, q=25.0),
+    "river_conveyance": dict(q_cmvel=0.8, q_side=2),
+}
+for _nm, _kw in RIVER.items():
+    CASES[_nm] = ((lambda kw=_kw: cases.river(**kw)), 40, [1, 10, 20, 40])
+
 
 # flow_balance and flow_step_stat of every step (WATER_VOLUME_BALANCE ON):
 # ref64_balance_<case>.npz holds "bal" [nsteps, 12] (cumulative storage,

@@ -57,6 +57,17 @@ def write_cart(case, depth, path):
         f.write("GRID_DEPTHS_BEGIN\n" + "".join(f"{v:.9g}\n" for v in depth) + "GRID_DEPTHS_END\n")
 
 
+def side_ids(nx, ny, side):
+    """Full-grid ids (1-based) of the cells along a side, in string order."""
+    if side == 1:
+        return [nx + j * nx for j in range(ny)]                  # east column
+    if side == 3:
+        return [1 + j * nx for j in range(ny)]
+    if side == 0:
+        return [i + 1 + (ny - 1) * nx for i in range(nx)]
+    return [i + 1 for i in range(nx)]
+
+
 def write_case(case, d, name="case", solver="BICGSTAB", balance=False):
     os.makedirs(d, exist_ok=True)
     nx, ny = case.nx, case.ny
@@ -81,18 +92,33 @@ def write_case(case, d, name="case", solver="BICGSTAB", balance=False):
     side = case.th_side
     if tel is not None:
         ids = [int(v) + 1 for v in tel.bnd_cells]
-    elif side == 1:
-        ids = [nx + j * nx for j in range(ny)]                  # east column, full-grid ids
-    elif side == 3:
-        ids = [1 + j * nx for j in range(ny)]
-    elif side == 0:
-        ids = [i + 1 + (ny - 1) * nx for i in range(nx)]
     else:
-        ids = [i + 1 for i in range(nx)]
+        ids = side_ids(nx, ny, side)
     with open(os.path.join(d, name + ".bid"), "w") as f:
         f.write("1\n")
         f.write(f"{len(ids)}\n")
         f.write(" ".join(str(v) for v in ids) + "\n")
+    river = ""
+    qs = getattr(case, "q_side", -1)
+    if qs >= 0:
+        # the flux string's cells (full-grid ids, string order) and its FLUX block
+        # (flux_block, boundaries.F90:295-423): the total inflow per boundary
+        qids = side_ids(nx, ny, qs)
+        with open(os.path.join(d, name + "_q.bid"), "w") as f:
+            f.write(f"1\n{len(qids)}\n" + " ".join(str(v) for v in qids) + "\n")
+        if case.q_times is not None:
+            with open(os.path.join(d, f"{name}_q.xys"), "w") as f:
+                f.write(f"XYS 1 {len(case.q_times)}\n")
+                f.write("".join(f"{t:.9g} {v:.9g}\n" for t, v in zip(case.q_times, case.q_values)))
+            qcard = f'    FLUX_CURVE "{name}_q.xys"\n    FLUX_UNITS \'m^3/s/boundary\'\n'
+            if case.q_order:
+                qcard += f"    TEMPORAL_INTERPOLATION_ORDER {case.q_order}\n"
+        else:
+            qcard = f"    FLUX_VALUE {case.q_const:.9g} 'm^3/s'\n"
+        if case.q_cmvel:
+            qcard += f"    CONVEYANCE {case.q_cmvel:.9g}\n"
+        river = (f'BOUNDARY_BEGIN\n  NAME "river"\n  CELLSTRING "{name}_q.bid" "1"\n  FLUX_BEGIN\n'
+                 f"{qcard}  FLUX_END\nBOUNDARY_END\n")
     if getattr(case, "wse_times", None) is not None:
         # single water level from an XYS curve (wse_block, boundaries.F90:482-485), hours / m
         with open(os.path.join(d, f"{name}_wse.xys"), "w") as f:
@@ -155,7 +181,7 @@ BOUNDARY_BEGIN
   NAME "tide"
   CELLSTRING "{name}.bid" "1"
 {forcing}BOUNDARY_END
-END_PARAMETERS
+{river}END_PARAMETERS
 """
     with open(os.path.join(d, name + ".cmcards"), "w") as f:
         f.write(cards)

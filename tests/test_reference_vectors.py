@@ -195,7 +195,21 @@ TEL = {
     "tel_patch": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)),
 }
 CASES.update(TEL)
-ALL = [*TEL, "tidal_combo", "tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
+# flux (river) strings: bndriver_vel, bnd_vel_flux, bndflux and the flux
+# string's ghosts (boundaries.F90:4259-4414,4740-4764, bnd_vel.f90:2-32,
+# flow_imp.F90:866-874,984-995); the wet corner cell also exercises fric_init's
+# friction copy to the first cell (friction.F90:397-407)
+RIVER = {
+    "river": dict(),
+    "river_north": dict(q_side=0),
+    "river_curve": dict(q_curve=([0.0, 0.005, 0.01, 0.3], [0.0, 30.0, 60.0, 45.0]), q_order=2),
+    "river_shoal": dict(shoal=True, q=25.0),
+    "river_conveyance": dict(q_cmvel=0.8, q_side=2),
+}
+for _name, _kw in RIVER.items():
+    CASES[_name] = (lambda kw=_kw: cases.river(**kw))
+
+ALL = [*TEL, *RIVER, "tidal_combo", "tidal_ramp", "basin_200x100", "basin", "tidal", *NVD, *SCHEMES, *CORIOLIS, "tidal_wind", *TURB, *FRIC, *WSE, "tidal_nowallfric", "tidal_refgmres"]
 
 
 def compare_to_gold(name, model_factory, prefix="ref_"):
