@@ -19,6 +19,7 @@ I = C.POINTER(C.c_int)
 BND_WALL = 0
 BND_WSE_HARMONIC = 1
 BND_WSE_CURVE = 2
+BND_FLUX = 3
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 PRECOND_MG = 0
@@ -57,6 +58,8 @@ class FlowDesc(C.Structure):
         ("wse_times", D), ("wse_values", D), ("wse_ntimes", C.c_int), ("wse_order", C.c_int),
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
         ("serial_order", C.c_int),
+        ("flux_q", C.c_double), ("flux_times", D), ("flux_values", D), ("flux_ntimes", C.c_int),
+        ("flux_order", C.c_int), ("flux_conveyance", C.c_double),
     ]
 
 

@@ -284,6 +284,7 @@ static int adjoint_check(const cms2d_flow *F) {
         return set_err(CMS2D_ERR_ARG, "the adjoint needs the FP64 form, BiCGStab and the multigrid preconditioner");
     if (F->dtvar || F->serial_order) return set_err(CMS2D_ERR_ARG, "the adjoint needs a fixed time step and the default cell order");
     if (F->P.mbedfric != 2) return set_err(CMS2D_ERR_ARG, "the adjoint is built for Manning friction");
+    if (F->qside >= 0) return set_err(CMS2D_ERR_ARG, "the adjoint is built without flux sides");
     if (F->P.ndsch > 4) return set_err(CMS2D_ERR_ARG, "the adjoint is built for the schemes without deferred corrections (ndsch 0-4)");
     if (F->P.noptset >= 3 || F->windconst)
         return set_err(CMS2D_ERR_ARG, "the adjoint is built without wave and wind forcing");

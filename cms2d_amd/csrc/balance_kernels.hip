@@ -155,12 +155,12 @@ __device__ __forceinline__ void bal_finish(const double *part, int nb, double bn
     A.steps += 1.0;
 }
 
-// Cartesian: the string is every active cell along a water-level side, its
-// face the side's ghost face.
+// Cartesian: the strings are the active cells along the water-level and flux
+// sides, their faces the sides' ghost faces.
 __global__ void __launch_bounds__(KBLOCK) k_bal_fin(Geo g, St s, Par P, const double *part, int nb, BalAcc *acc) {
     double bnd = 0.0;
     for (int k = 0; k < 4; k++) {
-        if (g.side[k] != 1) continue;
+        if (g.side[k] == 0) continue;      // water-level and flux strings
         const int n = (k == KN || k == KS) ? g.nx : g.ny;
         for (int t = threadIdx.x; t < n; t += KBLOCK) {
             const int i = k == KE ? g.nx - 1 : k == KW ? 0 : t;

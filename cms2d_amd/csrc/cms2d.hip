@@ -175,7 +175,16 @@ struct cms2d_flow {
     double *ser_corner[2]{};
     int windconst = 0;         // spatially constant wind (met_def windconst) and its 10-m vector
     double wndx = 0.0, wndy = 0.0;
-    int side_type[4]{};        // CMS2D_BND_* per side (the kernels see 1 for every water-level side)
+    int side_type[4]{};        // CMS2D_BND_* per side (the kernels see 1 for every water-level side,
+                               // 2 for the flux side)
+    // the flux (river) side, Q_str: its side (-1 none), total inflow or curve, interpolation
+    // order and window start (Q_str%nti, %inc), conveyance, inflow direction, and this
+    // step's inflow (bnd_eval)
+    int qside = -1;
+    double q_const = 0.0;
+    std::vector<double> q_t, q_v;
+    int q_order = 1, q_inc = 1;
+    double cmvel = 0.0, qcos = 0.0, qsin = 0.0, qflux = 0.0;
     std::vector<double> wse_t, wse_v;   // water-level curve of CMS2D_BND_WSE_CURVE sides
     int wse_order = 1, wse_inc = 1;     // interpolation order, window start (H_str%inc, 1-based)
     int wse_adjust_off = 0;             // WSE_ADJUSTMENT OFF on the curve sides
@@ -828,6 +837,14 @@ static int eddyvis_tail(cms2d_flow *F) {
 
 // flow_recalculate, flow_imp.F90:1133-1242: back to the previous time level
 // with half the time step (the ramp keeps the value timing gave it).
+// bndriver_vel (boundaries.F90:4259-4356) on the flux side, if any
+static void river_vel(cms2d_flow *F) {
+    if (F->qside < 0) return;
+    const int n = (F->qside == KN || F->qside == KS) ? F->g.nx : F->g.ny;
+    hipLaunchKernelGGL(k_river, dim3(1), dim3(1024), (size_t)n * sizeof(double), F->st, F->g, F->s, F->qside,
+                       F->qflux, F->cmvel, F->qcos, F->qsin);
+}
+
 static int flow_recalculate(cms2d_flow *F) {
     F->mtime = 1;
     F->jtime += 1;
@@ -839,7 +856,7 @@ static int flow_recalculate(cms2d_flow *F) {
         return set_err(CMS2D_ERR_DIVERGED, "Model is divergent (time step reduced below 1/256 of its start value)");
     LAUNCH(k_recalc, F->nblk_pad, F->g, F->s);
     if (int rc = wetdry(F)) return rc;
-    // bndflux: flux and cross-shore strings only, none on these grids
+    if (F->qside >= 0) LAUNCH(k_bndflux, F->nblk_act, F->g, F->s);   // bndflux
     LAUNCH(k_grad_fric_vis, F->nblk_act, F->g, F->s, F->P);
     return eddyvis_tail(F);
 }
@@ -879,6 +896,11 @@ static int flow_imp_once(cms2d_flow *F) {
     // ramp and gradient terms are applied per cell in k_wse_adjust
     double wsecurve = 0.0;
     if (!F->wse_t.empty()) wsecurve = wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc);
+    // the flux string's inflow (:4078-4094): constant or the curve, times the ramp
+    if (F->qside >= 0) {
+        const double q = F->q_t.empty() ? F->q_const : wse_curve_eval(F->q_t, F->q_v, F->q_order, timehrs, F->q_inc);
+        F->qflux = F->ramp * q;
+    }
     if (int rc = wetdry(F)) return rc;
     F->P.tauwx = F->P.tauwy = 0.0;
     if (F->windconst) {
@@ -894,6 +916,7 @@ static int flow_imp_once(cms2d_flow *F) {
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
         if (F->tape)
             if (int rc = tape_snap(F, F->niter - 1)) return rc;
+        river_vel(F);
         if (F->P.ndsch >= 5 && F->P.ndsch <= 9) LAUNCH(k_defcor, F->nblk_act, g, s, F->P);
         LAUNCH(k_mom, F->nblk_act, g, s, F->P);
         TAPE(TAPE_MOM);
@@ -942,8 +965,11 @@ static int flow_imp_once(cms2d_flow *F) {
         LAUNCH(k_pvfc1, F->nblk_act, g, s);
         TAPE(TAPE_PVFC1);
         LAUNCH(k_pvfc2, F->nblk_pad, g, s, F->P);
-        LAUNCH(k_ghost_p, (2 * g.nx + 2 * g.ny + BLOCK - 1) / BLOCK, g, s);
+        LAUNCH(k_ghost_p, (2 * g.nx + 2 * g.ny + BLOCK - 1) / BLOCK, g, s, F->P);
         LAUNCH(k_grad_fric_vis, F->nblk_act, g, s, F->P);
+        // bndriver_vel and bndflux after flow_grad_interp (flow_imp.F90:236-237): the
+        // weights take the new hk; bndflux then writes the fluxes bndriver_vel wrote
+        river_vel(F);
         if (int rc = eddyvis_tail(F)) return rc;
         for (int n = 0; n < 3; n++) {
             F->rmom[n] = F->h_sc[n].rmom;
@@ -1084,14 +1110,15 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     bool curve = false;
     for (int k = 0; k < 4; k++) {
         if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC &&
-            d->side_type[k] != CMS2D_BND_WSE_CURVE)
+            d->side_type[k] != CMS2D_BND_WSE_CURVE && d->side_type[k] != CMS2D_BND_FLUX)
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
         curve = curve || d->side_type[k] == CMS2D_BND_WSE_CURVE;
     }
-    int nopen = 0;
+    int nopen = 0, nflux = 0;
     for (int k = 0; k < 4; k++) {
         if (d->side_type[k] == CMS2D_BND_WALL) continue;
-        nopen++;
+        if (d->side_type[k] == CMS2D_BND_FLUX) nflux++;
+        else nopen++;
         // the set-up adjustment of a water-level side keeps the side in LDS
         const long len = (k == CMS2D_SIDE_N || k == CMS2D_SIDE_S) ? d->nx : d->ny;
         if (len * (long)sizeof(double) > 65536)
@@ -1099,6 +1126,23 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
     }
     if (nopen > 1)
         return set_err(CMS2D_ERR_ARG, "one water-level side at most (the others are walls)");
+    if (nflux > 1) return set_err(CMS2D_ERR_ARG, "one flux side at most");
+    if (nflux) {
+        // bndriver_vel reads the Manning n of the previous friction evaluation; the laws
+        // that rewrite n each evaluation are not built with a flux side
+        if (d->friction != CMS2D_FRIC_MANNING) return set_err(CMS2D_ERR_ARG, "a flux side needs Manning friction");
+        if (d->flux_ntimes == 1 || d->flux_ntimes < 0 || d->flux_order < 0 || d->flux_order > 3 ||
+            !(d->flux_conveyance >= 0.0) || !std::isfinite(d->flux_conveyance))
+            return set_err(CMS2D_ERR_ARG, "flux curve or conveyance invalid");
+        if (d->flux_ntimes == 0 && !std::isfinite(d->flux_q)) return set_err(CMS2D_ERR_ARG, "flux not finite");
+        if (d->flux_ntimes > 0) {
+            if (!d->flux_times || !d->flux_values) return set_err(CMS2D_ERR_ARG, "flux curve missing");
+            for (int i = 0; i < d->flux_ntimes; i++)
+                if (!std::isfinite(d->flux_times[i]) || !std::isfinite(d->flux_values[i]) ||
+                    (i > 0 && !(d->flux_times[i] > d->flux_times[i - 1])))
+                    return set_err(CMS2D_ERR_ARG, "flux curve times must increase, values be finite");
+        }
+    }
     if (curve) {
         if (d->wse_ntimes < 1 || !d->wse_times || !d->wse_values || d->wse_order < 0 || d->wse_order > 3)
             return set_err(CMS2D_ERR_ARG, "water-level curve missing or invalid");
@@ -1129,7 +1173,25 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         int open[4];
         for (int k = 0; k < 4; k++) {
             F->side_type[k] = d->side_type[k];
-            open[k] = d->side_type[k] == CMS2D_BND_WALL ? 0 : 1;
+            open[k] = d->side_type[k] == CMS2D_BND_WALL ? 0 : d->side_type[k] == CMS2D_BND_FLUX ? 2 : 1;
+            if (d->side_type[k] == CMS2D_BND_FLUX) {
+                // Q_str: inflow along the inward normal, local_angle = (4 - idirface) 90
+                // degrees (boundaries.F90:1750-1762); conveyance 0.667 (a default-real
+                // literal, bnd_block.f90:201) or the card's value limited to [0.5, 1]
+                F->qside = k;
+                F->q_const = d->flux_q;
+                if (d->flux_ntimes > 0) {
+                    F->q_t.assign(d->flux_times, d->flux_times + d->flux_ntimes);
+                    F->q_v.assign(d->flux_values, d->flux_values + d->flux_ntimes);
+                }
+                F->q_order = std::max(std::min(d->flux_order ? d->flux_order : 1, std::max(d->flux_ntimes - 1, 1)), 1);
+                F->q_inc = 1;
+                F->cmvel = d->flux_conveyance > 0.0 ? std::max(std::min(d->flux_conveyance, 1.0), 0.5) : F4(0.667);
+                const double pi = std::acos(-1.0), deg2rad = pi / 180.0;
+                const double angle = (double)((float)(4 - (k + 1)) * 90.0f);
+                F->qcos = std::cos(angle * deg2rad);
+                F->qsin = std::sin(angle * deg2rad);
+            }
         }
         fill_geo(F->g, d->nx, d->ny, d->dx, d->dy, open);
         bool curve = false;
@@ -1318,7 +1380,13 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         HIPCHK(hipMemcpyAsync(s.z0, z0.data(), N * sizeof(double), hipMemcpyHostToDevice, F->st));
         HIPCHK(hipStreamSynchronize(F->st));
         double *in = P.mbedfric == 1 ? s.cfrict : P.mbedfric == 3 ? s.z0 : P.mbedfric == 4 ? s.cbf : s.coefman;
-        if (int rc = upload_padded(F, in, d->mann, false)) return rc;
+        // fric_init's copy to the dummy cells (friction.F90:397-407) loops
+        // `do i=1,ncells+1,ncellsD`, one pass for i = 1: the first cell takes
+        // cfrict, coefman and z0 of its first active neighbour in face order
+        // (north); the other arrays hold their defaults there
+        std::vector<double> fr(d->mann, d->mann + g.nact);
+        if (P.mbedfric != 4) fr[0] = fr[g.nx];
+        if (int rc = upload_padded(F, in, fr.data(), false)) return rc;
     }
     if (d->wavestrx && int(upload_padded(F, s.wavestrx, d->wavestrx, false))) return CMS2D_ERR_HIP;
     if (d->wavestry && int(upload_padded(F, s.wavestry, d->wavestry, false))) return CMS2D_ERR_HIP;
@@ -1586,10 +1654,10 @@ struct CkHost {
     double timesecs, rampdur, ramp, dtimebeg, deltime;
     double rmom[3], rmom0[3], rmom1[3];
     int dtvar, jtime, mtime, ntime, maxit, niter, iconv;
-    int kry[3], prev_iters[4], prev_gm;
+    int kry[3], prev_iters[6], prev_gm;
     int windconst;
     double wndx, wndy;
-    int wse_inc;
+    int wse_inc, q_inc;
 };
 const char CK_MAGIC[8] = {'C', 'M', 'S', '2', 'D', 'C', 'K', '1'};
 }  // namespace
@@ -1608,9 +1676,10 @@ static void ck_host_get(const cms2d_flow *F, CkHost &c) {
     for (int n = 0; n < 3; n++) { c.rmom[n] = F->rmom[n]; c.rmom0[n] = F->rmom0[n]; c.rmom1[n] = F->rmom1[n]; c.kry[n] = F->kry[n]; }
     c.dtvar = F->dtvar; c.jtime = F->jtime; c.mtime = F->mtime; c.ntime = F->ntime; c.maxit = F->maxit;
     c.niter = F->niter; c.iconv = F->iconv; c.prev_gm = F->prev_gm;
-    for (int n = 0; n < 4; n++) c.prev_iters[n] = F->prev_iters[n];
+    for (int n = 0; n < 6; n++) c.prev_iters[n] = F->prev_iters[n];
     c.windconst = F->windconst; c.wndx = F->wndx; c.wndy = F->wndy;
     c.wse_inc = F->wse_inc;
+    c.q_inc = F->q_inc;
 }
 static void ck_host_set(cms2d_flow *F, const CkHost &c) {
     F->P = c.P;
@@ -1618,9 +1687,10 @@ static void ck_host_set(cms2d_flow *F, const CkHost &c) {
     for (int n = 0; n < 3; n++) { F->rmom[n] = c.rmom[n]; F->rmom0[n] = c.rmom0[n]; F->rmom1[n] = c.rmom1[n]; F->kry[n] = c.kry[n]; }
     F->dtvar = c.dtvar; F->jtime = c.jtime; F->mtime = c.mtime; F->ntime = c.ntime; F->maxit = c.maxit;
     F->niter = c.niter; F->iconv = c.iconv; F->prev_gm = c.prev_gm;
-    for (int n = 0; n < 4; n++) F->prev_iters[n] = c.prev_iters[n];
+    for (int n = 0; n < 6; n++) F->prev_iters[n] = c.prev_iters[n];
     F->windconst = c.windconst; F->wndx = c.wndx; F->wndy = c.wndy;
     F->wse_inc = c.wse_inc;
+    F->q_inc = c.q_inc;
 }
 
 extern "C" int cms2d_flow_checkpoint(cms2d_flow *F, void *buf, size_t *nbytes) {

@@ -193,7 +193,13 @@ __global__ void k_wetdry_h(Geo g, St s, Par P) {
     if (h < P.hdry) { h = fmax(h, P.hmin); w = 0; }
     if (!active_ij(g, i, j)) {
         int sd = ring_side(g, i, j);
-        if (sd < 0 || g.side[sd] == 0) w = 0;     // corner or wall ghost
+        if (sd < 0 || g.side[sd] == 0) {
+            w = 0;                                  // corner or wall ghost
+        } else {
+            // forcing string: the ghost takes its cell's flag (flow.F90:1606-1613)
+            const int c = sd == KN ? t - g.nxp : sd == KE ? t - 1 : sd == KS ? t + g.nxp : t + 1;
+            w = s.p[c] * GRAVINV - s.zb[c] < P.hdry ? 0 : 1;
+        }
     }
     s.h[t] = h;
     s.iwet[t] = w;
@@ -542,6 +548,18 @@ __global__ void k_mom(Geo g, St s, Par P) {
         a[k] = schmcoef(P.ndsch, ddk, fl[k]);
     }
     const int wet = s.iwet[c];
+    // flux sides first (bound_uv's order): bnd_vel_flux (bnd_vel.f90:2-32), the
+    // upwind coefficient times the ghost velocity moves into the sources
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!(nb_ghost(g, i, j, k) && g.side[k] == 2)) continue;
+        const int gc = nbr(g, c, k);
+        const double ab = schmcoef(1, vk[k] * hk[k] * g.dsxy[k], fl[k]);
+        su += ab * s.u[gc];
+        sv += ab * s.v[gc];
+        sp -= ab;
+        a[k] = 0.0;
+    }
     // harmonic water-level sides: fold the boundary face into su/sv/sp
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -620,9 +638,18 @@ __device__ __forceinline__ double pp_coef(const Geo &g, const St &s, int own, in
     double suf = fk * s.sumu[nck] + fp * s.sumu[own];
     return hk * hk * g.ds[k] * apf / (1.0 - suf) / g.dn[k];
 }
-// Rhie-Chow face flux of `own` in direction k (flow_imp_flux.f90:382-406)
+// fluxbnd (boundaries.F90:4740-4764): the outward flux of a flux-side face
+// from its ghost's velocity
+__device__ __forceinline__ double bnd_flux(const Geo &g, const St &s, int c, int k) {
+    const int gc = nbr(g, c, k);
+    const double hk = fsym(s.hkE, s.hkN, g, c, k);
+    return (double)s.iwet[c] * g.ds[k] * hk * (g.fnx[k] * s.u[gc] + g.fny[k] * s.v[gc]);
+}
+// Rhie-Chow face flux of `own` in direction k (flow_imp_flux.f90:382-406);
+// on flux sides the bndflux that follows (flow_imp.F90:697) overrides it
 __device__ __forceinline__ double rc_flux(const Geo &g, const St &s, int own, int oi, int oj, int k) {
     int nck = nbr(g, own, k);
+    if (nb_ghost(g, oi, oj, k) && g.side[k] == 2) return bnd_flux(g, s, own, k);
     if (s.iwet[own] * s.iwet[nck] == 0) return 0.0;
     double hk = fsym(s.hkE, s.hkN, g, own, k);
     if (nb_ghost(g, oi, oj, k))
@@ -761,6 +788,9 @@ __global__ void __launch_bounds__(KBLOCK) k_pp_src(Geo g, St s, Par P, KrySys<1>
             sp = 0.0;
         }
 #pragma unroll
+        for (int k = 0; k < 4; k++)      // flux sides: closed faces (boundaries.F90:3544-3551)
+            if (nb_ghost(g, i, j, k) && g.side[k] == 2) a[k] = 0.0;
+#pragma unroll
         for (int k = 0; k < 4; k++) {
             if (!(nb_ghost(g, i, j, k) && g.side[k] == 1)) continue;
             int gc = nbr(g, c, k);
@@ -804,7 +834,7 @@ __global__ void k_pvfc1(Geo g, St s) {
     if (!act_thread(g, i, j, c)) return;
 #pragma unroll
     for (int k = 0; k < 4; k++)
-        if (nb_ghost(g, i, j, k) && g.side[k] == 0) s.pp[nbr(g, c, k)] = s.pp[c];
+        if (nb_ghost(g, i, j, k) && g.side[k] != 1) s.pp[nbr(g, c, k)] = s.pp[c];   // walls, flux sides
     const double *vars[1] = {s.pp};
     double gx, gy;
     cbgg_grad(g, s.iwet, c, 1, vars, &gx, &gy);
@@ -840,8 +870,9 @@ __global__ void k_pvfc2(Geo g, St s, Par P) {
     s.pp[t] = FACPP * pp;
 }
 
-// pvfc (flow_imp.F90:972-980): wall ghosts take the pressure of their cell
-__global__ void k_ghost_p(Geo g, St s) {
+// pvfc (flow_imp.F90:972-995): wall ghosts take the pressure of their cell,
+// flux-side ghosts also the depth that follows from it
+__global__ void k_ghost_p(Geo g, St s, Par P) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     int nr = 2 * g.nx + 2 * g.ny;
     if (t >= nr) return;
@@ -850,9 +881,10 @@ __global__ void k_ghost_p(Geo g, St s) {
     else if (t < 2 * g.nx) { i = t - g.nx; j = 0; k = KS; }
     else if (t < 2 * g.nx + g.ny) { i = g.nx - 1; j = t - 2 * g.nx; k = KE; }
     else { i = 0; j = t - 2 * g.nx - g.ny; k = KW; }
-    if (g.side[k] != 0) return;
-    int c = cidx(g, i, j);
-    s.p[nbr(g, c, k)] = s.p[c];
+    if (g.side[k] == 1) return;
+    int c = cidx(g, i, j), gc = nbr(g, c, k);
+    s.p[gc] = s.p[c];
+    if (g.side[k] == 2) s.h[gc] = fmax(P.hmin, s.p[gc] * GRAVINV - s.zb[gc]);
 }
 
 // flow_grad_interp (flow_imp.F90:1106-1127) + fric_eval (friction.F90:615-797,
@@ -941,7 +973,7 @@ __global__ void k_grad_fric_vis(Geo g, St s, Par P) {
     for (int k = 0; k < 4; k++) {
         if (!nb_ghost(g, i, j, k)) continue;
         int gc = nbr(g, c, k);
-        s.visb[gc] = g.side[k] == 1 ? vis : s.vis[gc];
+        s.visb[gc] = g.side[k] != 0 ? vis : s.vis[gc];
     }
 }
 
@@ -1288,6 +1320,13 @@ __device__ __forceinline__ double conv_face(const Geo &g, const St &s, int c, in
     const double unk = g.fnx[k] * s.u[nb] + g.fny[k] * s.v[nb];
     return (double)(s.iwet[c] * s.iwet[nb]) * g.ds[k] * hk * (g.fintp[k] * unk + (1.0 - g.fintp[k]) * uni);
 }
+// bndflux (boundaries.F90:4359-4414) on the faces of a flux side
+__device__ __forceinline__ void bnd_fluxes(const Geo &g, const St &s, int i, int j, int c) {
+    if (j == g.ny - 1 && g.side[KN] == 2) s.fluxN[c] = bnd_flux(g, s, c, KN);
+    if (i == g.nx - 1 && g.side[KE] == 2) s.fluxE[c] = bnd_flux(g, s, c, KE);
+    if (j == 0 && g.side[KS] == 2) s.fluxN[c - g.nxp] = -bnd_flux(g, s, c, KS);
+    if (i == 0 && g.side[KW] == 2) s.fluxE[c - 1] = -bnd_flux(g, s, c, KW);
+}
 __global__ void k_convflux(Geo g, St s) {
     int i, j, c;
     if (!act_thread(g, i, j, c)) return;
@@ -1295,6 +1334,69 @@ __global__ void k_convflux(Geo g, St s) {
     s.fluxE[c] = conv_face(g, s, c, c + 1, KE, s.hkE[c]);
     if (j == 0) s.fluxN[c - g.nxp] = -conv_face(g, s, c, c - g.nxp, KS, s.hkN[c - g.nxp]);
     if (i == 0) s.fluxE[c - 1] = -conv_face(g, s, c, c - 1, KW, s.hkE[c - 1]);
+    bnd_fluxes(g, s, i, j, c);
+}
+__global__ void k_bndflux(Geo g, St s) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    bnd_fluxes(g, s, i, j, c);
+}
+
+// bndriver_vel (boundaries.F90:4259-4356) on flux side k: the weights
+// iwet(i) iwet(ghost) hk^cmvel / max(n, 1e-5) along (cos, sin) on the ghosts,
+// their flux sum sumw = sum ds hk (u fnx + v fny) in string order (one
+// thread, as the reference rounds it), then the ghosts' velocities and the
+// faces' fluxes scaled to the string's inflow qflux.  One workgroup; the
+// string's terms in LDS (n doubles).
+__global__ void __launch_bounds__(1024) k_river(Geo g, St s, int k, double qflux, double cmvel, double qcos,
+                                                double qsin) {
+    extern __shared__ double term[];
+    __shared__ double qnorm;
+    const int n = (k == KN || k == KS) ? g.nx : g.ny;
+    auto cell = [&](int m) {
+        const int i = k == KE ? g.nx - 1 : k == KW ? 0 : m;
+        const int j = k == KN ? g.ny - 1 : k == KS ? 0 : m;
+        return cidx(g, i, j);
+    };
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const int c = cell(m), gc = nbr(g, c, k);
+        const double hk = fsym(s.hkE, s.hkN, g, c, k);
+        const double val = (double)(s.iwet[c] * s.iwet[gc]) * pow(hk, cmvel) / fmax(s.coefman[c], F4(1.0e-5));
+        const double ug = val * qcos, vg = val * qsin;
+        s.u[gc] = ug;
+        s.v[gc] = vg;
+        term[m] = g.ds[k] * hk * (ug * g.fnx[k] + vg * g.fny[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // sequential sum in string order; terms read 16 at a time so the LDS loads overlap
+        constexpr int U = 16;
+        double sumw = 0.0;
+        int m = 0;
+        for (; m + U <= n; m += U) {
+            double t[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) t[q] = term[m + q];
+#pragma unroll
+            for (int q = 0; q < U; q++) sumw = sumw + t[q];
+        }
+        for (; m < n; m++) sumw = sumw + term[m];
+        qnorm = qflux / fmax(fabs(sumw), F4(1.0e-6));
+    }
+    __syncthreads();
+    const double qn = qnorm;
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        const int c = cell(m), gc = nbr(g, c, k);
+        const double hk = fsym(s.hkE, s.hkN, g, c, k);
+        const double ug = s.u[gc] * qn, vg = s.v[gc] * qn;
+        s.u[gc] = ug;
+        s.v[gc] = vg;
+        const double f = (double)(s.iwet[c] * s.iwet[gc]) * g.ds[k] * hk * (ug * g.fnx[k] + vg * g.fny[k]);
+        if (k == KN) s.fluxN[c] = f;
+        else if (k == KE) s.fluxE[c] = f;
+        else if (k == KS) s.fluxN[c - g.nxp] = -f;
+        else s.fluxE[c - 1] = -f;
+    }
 }
 
 // interp_scal_cell2face(zb, ibc=0) on owned faces (prestart.f90:320)

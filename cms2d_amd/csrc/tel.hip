@@ -1307,6 +1307,13 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         HIPCHK(hipMemcpyAsync(s.zb, zb.data(), nd * sizeof(double), hipMemcpyHostToDevice, F->st));
         HIPCHK(hipMemcpyAsync(s.z0, z0.data(), nd * sizeof(double), hipMemcpyHostToDevice, F->st));
         for (int i = 0; i < n; i++) buf[i] = d->mann[i];
+        // fric_init's copy to the dummy cells (friction.F90:397-407) runs for the
+        // first cell only (its loop stride is ncellsD): it takes the friction of its
+        // first active neighbour in face order
+        for (int k = 0; k < t.ncf[0] && P.mbedfric != 4; k++) {
+            const int nb = t.c2c[(size_t)k * nd + 0];
+            if (nb >= 0 && nb < n) { buf[0] = buf[nb]; break; }
+        }
         double *in = P.mbedfric == 1 ? s.cfrict : P.mbedfric == 3 ? s.z0 : P.mbedfric == 4 ? s.cbf : s.coefman;
         HIPCHK(hipMemcpyAsync(in, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, F->st));
         HIPCHK(hipStreamSynchronize(F->st));

@@ -55,6 +55,13 @@ class FlowStep:
                 d.wse_gradx, d.wse_grady = case.wse_grad
             else:
                 d.side_type[case.th_side] = _lib.BND_WSE_HARMONIC
+        if getattr(case, "q_side", -1) >= 0:
+            # the flux (river) string: total inflow or its curve, conveyance
+            d.side_type[case.q_side] = _lib.BND_FLUX
+            d.flux_q, d.flux_order, d.flux_conveyance = case.q_const, case.q_order, case.q_cmvel
+            if case.q_times is not None:
+                d.flux_times, d.flux_values = arr(case.q_times), arr(case.q_values)
+                d.flux_ntimes = len(case.q_times)
         d.ntc = len(case.amp)
         for k in range(d.ntc):
             d.amp[k], d.speed[k], d.phase[k] = case.amp[k], case.speed_rad_hr[k], case.phase_rad[k]

@@ -19,7 +19,8 @@ module cms2d_hip
 
     integer(c_int), parameter, public :: CMS2D_OK = 0, CMS2D_ERR_ARG = -1, CMS2D_ERR_HIP = -2, &
                                          CMS2D_ERR_DIVERGED = -3, CMS2D_ERR_NODEVICE = -4
-    integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1, CMS2D_BND_WSE_CURVE = 2
+    integer(c_int), parameter, public :: CMS2D_BND_WALL = 0, CMS2D_BND_WSE_HARMONIC = 1, CMS2D_BND_WSE_CURVE = 2, &
+                                         CMS2D_BND_FLUX = 3
     integer(c_int), parameter, public :: CMS2D_PREC_FP64 = 0
     integer(c_int), parameter, public :: CMS2D_PRECOND_MG = 0, CMS2D_PRECOND_JACOBI = 1
     integer(c_int), parameter, public :: CMS2D_SOLVER_BICGSTAB = 0, CMS2D_SOLVER_GMRES = 1
@@ -50,6 +51,10 @@ module cms2d_hip
         integer(c_int) :: wse_adjust_off = 0
         real(c_double) :: wse_gradx = 0, wse_grady = 0
         integer(c_int) :: serial_order = 0
+        real(c_double) :: flux_q = 0                                   ! CMS2D_BND_FLUX side: inflow [m^3/s]
+        type(c_ptr) :: flux_times = c_null_ptr, flux_values = c_null_ptr   ! or its curve [hr], [m^3/s]
+        integer(c_int) :: flux_ntimes = 0, flux_order = 0
+        real(c_double) :: flux_conveyance = 0
     end type
 
     ! struct cms2d_step_info

@@ -53,6 +53,12 @@ extern "C" {
 /* Boundary types of a side. */
 #define CMS2D_BND_WALL 0          /* closed wall (W_str, boundaries.F90:2413-2435)            */
 #define CMS2D_BND_WSE_HARMONIC 1  /* harmonic water level (TH_str, boundaries.F90:4098-4124) */
+#define CMS2D_BND_FLUX 3          /* flux (river) string, FLUX_VALUE / FLUX_CURVE (Q_str,
+                                     boundaries.F90:265-445,4078-4094; bndriver_vel :4259-4356;
+                                     bnd_vel_flux, bnd_vel.f90:2-32; bndflux :4359-4414): the total
+                                     inflow flux_q, or the curve flux_times/flux_values, split over
+                                     the side's faces by the conveyance hk^(1+c)/n along the inward
+                                     normal; Manning friction only                              */
 #define CMS2D_BND_WSE_CURVE 2     /* single water level from a time series, WSE_CURVE / WSE_CONSTANT
                                      (H_str, boundaries.F90:4127-4150): the curve wse_times/
                                      wse_values, Lagrange interpolation of order wse_order
@@ -93,8 +99,8 @@ typedef struct {
                                      Manning n by default (MANNINGS_N_DATASET)              */
     const double *wavestrx;       /* radiation-stress forcing / rho [nx*ny] or NULL         */
     const double *wavestry;       /* (wave_flowgrid_def wavestrx/wavestry)                  */
-    int side_type[4];             /* CMS2D_BND_* for N,E,S,W: one water-level side at most
-                                     (up to 8192 cells long), the others walls               */
+    int side_type[4];             /* CMS2D_BND_* for N,E,S,W: one water-level side and one flux
+                                     side at most (up to 8192 cells long), the others walls  */
     int ntc;                      /* harmonic constituents (<= 8)                            */
     double amp[8];                /* [m]                                                     */
     double speed[8];              /* [rad/hr]                                                */
@@ -140,6 +146,14 @@ typedef struct {
                                      flow_eddyvis.f90:168-212) give the reference's one-thread
                                      (cell order) result; 0: every read sees the values from
                                      before the loop (independent of any order).  Cartesian grids */
+    double flux_q;                /* CMS2D_BND_FLUX side: total inflow [m^3/s], positive into the
+                                     domain (FLUX_VALUE ... 'm^3/s'), when flux_ntimes = 0   */
+    const double *flux_times;     /* or its curve: times [hr], increasing, and inflows [m^3/s], */
+    const double *flux_values;    /* flux_ntimes >= 2 of each (FLUX_CURVE, m^3/s/boundary)     */
+    int flux_ntimes;
+    int flux_order;               /* TEMPORAL_INTERPOLATION_ORDER 1-3 (0 -> 1, the reference's
+                                     default for fluxes), limited to flux_ntimes - 1          */
+    double flux_conveyance;       /* CONVEYANCE coefficient, limited to [0.5, 1]; 0 -> 0.667   */
 } cms2d_flow_desc;
 
 typedef struct {

@@ -229,6 +229,8 @@ BALANCE = {
     "tel_band": (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1), 40),
     "tel_patch": (lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)), 40),
     "tidal_dtvar": (_mk_dtvar, 12),
+    "river": (lambda: cases.river(), 40),
+    "river_shoal": (lambda: cases.river(shoal=True, q=25.0), 40),
 }
 
 

@@ -25,6 +25,8 @@ BAL_CASES = {
     "square": lambda: cases.square(nx=40, ny=24, dx=20.0, seed=3),
     "tel_band": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1),
     "tel_patch": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)),
+    "river": lambda: cases.river(),                          # efflux through both strings
+    "river_shoal": lambda: cases.river(shoal=True, q=25.0),
 }
 
 

@@ -21,7 +21,12 @@ from test_reference_vectors import ALL
 # the GPU follows that order and matches the case's vectors
 # (test_gpu_serial_order.py); the race-free default is checked against the
 # race-free oracle (test_wse_gradient_matches_oracle).
-REF_CASES = [n for n in ALL if n not in ("tidal_wse_grad", "tidal_wse_grad_north") and not n.startswith("river")]
+# With a flux string on the north or south side the same order dependence
+# moves velocities by 9e-6 and 4e-6 m/s (the oracle in race-free mode differs
+# from the reference by the same; in the reference's order both match the
+# double-precision runs to 5e-9 m/s, test_gpu_serial_order.py).
+REF_CASES = [n for n in ALL if n not in ("tidal_wse_grad", "tidal_wse_grad_north", "river_north",
+                                         "river_conveyance")]
 
 TOL_ETA = 1e-6   # m
 TOL_VEL = 1e-6   # m/s

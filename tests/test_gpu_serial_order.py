@@ -4,14 +4,15 @@ values, flow.F90:1477-1491; the eddy-viscosity fix-ups next to dry cells and
 at corners, flow_eddyvis.f90:168-212) give the result of the reference run on
 one thread, which is how the golden vectors were made.  The race-free default
 differs from that order where flow runs strongly along the wet/dry edge (the
-WSE_GRADIENTS case, 9e-5 m/s), so that case is checked against the reference
-here."""
+WSE_GRADIENTS case, 9e-5 m/s) and with a flux string on the north or south
+side (9e-6 and 4e-6 m/s; the race-free oracle differs from the reference by the
+same), so those cases are checked against the reference here."""
 import numpy as np
 import pytest
 
 from cms2d_amd.flow import FlowStep
 from pyoracle import Oracle
-from test_reference_vectors import (ALL, CASES, TEL, TOL64_ETA, TOL64_VEL, TOL_ETA, TOL_VEL,
+from test_reference_vectors import (ALL, CASES, RIVER, TEL, TOL64_ETA, TOL64_VEL, TOL_ETA, TOL_VEL,
                                     compare_to_gold)
 
 SERIAL = [n for n in ALL if n not in TEL]     # serial_order is built for Cartesian grids
@@ -27,7 +28,7 @@ def test_serial_order_matches_reference_vectors(gpu_available, name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["tidal_wse_grad", "tidal_wse_grad_north", "tidal", "tidal_combo"])
+@pytest.mark.parametrize("name", ["tidal_wse_grad", "tidal_wse_grad_north", "tidal", "tidal_combo", *RIVER])
 def test_serial_order_matches_double_precision_reference(gpu_available, name):
     print(f"serial order, double-precision reference {name}", flush=True)
     w = compare_to_gold(name, lambda c: FlowStep(c, serial_order=1), prefix="ref64_")
