@@ -391,5 +391,11 @@ extern "C" int cms2d_flow_adjoint(cms2d_flow *F, int nsteps, int every, int ngau
             if (gmann) gmann[a] = gm[c];
             if (gzb) gzb[a] = gb[c];
         }
+    // the first cell's n is its north neighbour's (fric_init's copy at creation):
+    // its adjoint belongs to that input
+    if (gmann) {
+        gmann[g.nx] += gmann[0];
+        gmann[0] = 0.0;
+    }
     return CMS2D_OK;
 }

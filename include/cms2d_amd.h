@@ -336,7 +336,9 @@ int cms2d_solve_adjoint(int ncells, int ncellsD, int nmaxfaces, const double *ac
  * object on which no step has been taken), runs nsteps steps as
  * cms2d_flow_step does and returns the gradient of a loss of the water levels
  * at the gauge cells with respect to the Manning n (gmann) and the bed
- * elevation (gzb) of every active cell [ncells, row-major], by the reverse of
+ * elevation (gzb) of every active cell [ncells, row-major; gmann with respect
+ * to the input n of cms2d_flow_desc, so the first cell's entry is 0: the step
+ * uses its north neighbour's n there, as the reference's fric_init does], by the reverse of
  * every kernel of the step (transposed u, v and pp solves included) at the
  * recorded forward state.  The state is checkpointed every `every` steps and
  * each interval recomputed in the reverse sweep.
@@ -348,7 +350,7 @@ int cms2d_solve_adjoint(int ncells, int ncellsD, int nmaxfaces, const double *ac
  * Discrete choices (wet/dry flags, outer-iteration counts, clipping branches)
  * are those of the forward run.  Requires a Cartesian grid, the FP64 form with
  * BiCGStab and the multigrid preconditioner, fixed dt, Manning friction,
- * advection schemes 0-4 and no wave or wind forcing (CMS2D_ERR_ARG
+ * advection schemes 0-4, no flux side and no wave or wind forcing (CMS2D_ERR_ARG
  * otherwise).  Allocates its checkpoints and tape on the device (about
  * 0.6 KB per cell for each stored state); the flow is back at its initial
  * state on return. */
