@@ -39,7 +39,7 @@ EXPORTS = [
     "cms2d_flow_checkpoint", "cms2d_flow_restore", "cms2d_flow_set_wind",
     "cms2d_ensemble_step", "cms2d_solve", "cms2d_solve_noalloc", "cms2d_flow_create_tel",
     "cms2d_solve_adjoint",
-    "cms2d_flow_adjoint", "cms2d_flow_balance",
+    "cms2d_flow_adjoint", "cms2d_flow_balance", "cms2d_flow_statistics", "cms2d_flow_stat",
 ]
 
 
@@ -121,6 +121,8 @@ def lib():
                               C.c_double, C.c_int, D, I]
     L.cms2d_solve_noalloc.argtypes = L.cms2d_solve.argtypes
     L.cms2d_flow_balance.argtypes = [C.c_void_p, C.POINTER(Balance)]
+    L.cms2d_flow_statistics.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    L.cms2d_flow_stat.argtypes = [C.c_void_p, C.c_char_p, D, C.POINTER(C.c_int)]
     L.cms2d_flow_adjoint.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, I, D, D, D, D, D, D]
     L.cms2d_solve_adjoint.argtypes = [C.c_int, C.c_int, C.c_int, D, I, D, D, D, C.c_double, D, C.c_double,
                                       C.c_double, C.c_int, D, D, D, D, I]

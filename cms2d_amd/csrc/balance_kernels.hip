@@ -180,3 +180,80 @@ __global__ void __launch_bounds__(KBLOCK) k_tl_bal_fin(TelG T, TelS s, Par P, co
 }
 
 }  // namespace cms
+
+namespace cms {
+
+// Flow statistics (stat_update_flow, source/statistics/statistics.F90:476-720):
+// per-cell accumulators over the steps of a window, in the order of stat_def.
+enum { S_ETAMAX, S_TETAMAX, S_ETAMIN, S_TETAMIN, S_UVMAX, S_UMAX, S_VMAX, S_TUVMAX, S_ETAMEAN, S_UR, S_VR,
+       S_UVCURV, S_HYDPER, S_RSUAVG, S_RSVAVG, S_RSPAVG, S_RSUMAX, S_RSVMAX, S_RSPMAX, S_ETAMAXGRAD, S_UVMAXGRAD,
+       S_FRXMAX, S_FRYMAX, S_UMAXCURVX, S_UMAXCURVY, S_VMAXCURVX, S_VMAXCURVY, S_CFLXMAX, S_CFLYMAX,
+       S_QCFLXMAX, S_QCFLYMAX, S_QFRXMAX, S_QFRYMAX, NSTAT };
+
+// One step of every active cell: the wet-cell extremes, sums and gradient
+// terms (the second derivatives with the wet-cell gradient operator, d2xy2d),
+// then the Courant and quality terms of every cell.  A new minimum anywhere
+// sets *flag: the reference assigns the time to the whole tetamin array.
+__global__ void k_stat(Geo g, St s, Par P, double *acc, size_t npad, double timehrs, int *flag) {
+    int i, j, c;
+    if (!act_thread(g, i, j, c)) return;
+    auto S = [&](int q) -> double & { return acc[(size_t)q * npad + c]; };
+    const double u = s.u[c], v = s.v[c], h = s.h[c];
+    if (s.iwet[c] != 0) {
+        const double eta = s.eta[c], uv = s.uv[c];
+        if (eta > S(S_ETAMAX)) { S(S_ETAMAX) = eta; S(S_TETAMAX) = timehrs; }
+        if (eta < S(S_ETAMIN)) { S(S_ETAMIN) = eta; *flag = 1; }
+        if (uv > S(S_UVMAX)) { S(S_UVMAX) = uv; S(S_UMAX) = u; S(S_VMAX) = v; S(S_TUVMAX) = timehrs; }
+        S(S_ETAMEAN) += eta;
+        S(S_UR) += u;
+        S(S_VR) += v;
+        // streamwise_curvature (flow_lib.F90:256-276)
+        S(S_UVCURV) += (u * u * s.dvx[c] - v * v * s.duy[c] + u * v * (s.dvy[c] - s.dux[c])) /
+                       pow(fmax(uv, F4(1.e-6)), 1.5);
+        S(S_HYDPER) += (double)s.iwet[c];
+        S(S_RSUAVG) += s.resu[c];
+        S(S_RSVAVG) += s.resv[c];
+        S(S_RSPAVG) += s.resp[c];
+        S(S_RSPMAX) = fmax(S(S_RSPMAX), s.resp[c]);
+        S(S_RSUMAX) = fmax(S(S_RSUMAX), s.resu[c]);
+        S(S_RSVMAX) = fmax(S(S_RSVMAX), s.resv[c]);
+        double val = sqrt(s.dpx[c] * s.dpx[c] + s.dpy[c] * s.dpy[c]) * GRAVINV;
+        S(S_ETAMAXGRAD) = fmax(S(S_ETAMAXGRAD), val);
+        val = sqrt(s.dux[c] * s.dux[c] + s.duy[c] * s.duy[c]);
+        S(S_UVMAXGRAD) = fmax(S(S_UVMAXGRAD), val);
+        val = sqrt(s.dvx[c] * s.dvx[c] + s.dvy[c] * s.dvy[c]);
+        S(S_UVMAXGRAD) = fmax(S(S_UVMAXGRAD), val);
+        const double cel = sqrt(GRAV * h);
+        S(S_FRXMAX) = fmax(S(S_FRXMAX), u / cel);
+        S(S_FRYMAX) = fmax(S(S_FRYMAX), v / cel);
+        const double *vars[4] = {s.dux, s.duy, s.dvx, s.dvy};
+        double gx[4], gy[4];
+        cbgg_grad(g, s.iwet, c, 4, vars, gx, gy);
+        S(S_UMAXCURVX) = fmax(S(S_UMAXCURVX), gx[0]);
+        S(S_UMAXCURVY) = fmax(S(S_UMAXCURVY), gy[1]);
+        S(S_VMAXCURVX) = fmax(S(S_VMAXCURVX), gx[2]);
+        S(S_VMAXCURVY) = fmax(S(S_VMAXCURVY), gy[3]);
+    }
+    const double cel = sqrt(GRAV * h);
+    S(S_CFLXMAX) = fmax(S(S_CFLXMAX), (fabs(u) + cel) * P.dtime / g.dx);
+    S(S_CFLYMAX) = fmax(S(S_CFLYMAX), (fabs(v) + cel) * P.dtime / g.dy);
+    S(S_QCFLXMAX) = fmax(S(S_QCFLXMAX), g.dx / P.dtime / (fabs(u) + sqrt(GRAV * h)));
+    S(S_QCFLYMAX) = fmax(S(S_QCFLYMAX), g.dy / P.dtime / (fabs(v) + sqrt(GRAV * h)));
+    S(S_QFRXMAX) = fmax(S(S_QFRXMAX), g.dx / P.dtime * fabs(u) / sqrt(GRAV * h));
+    S(S_QFRYMAX) = fmax(S(S_QFRYMAX), g.dy / P.dtime * fabs(v) / sqrt(GRAV * h));
+}
+
+// tetamin = timehrs on every cell after a step that set a new minimum; the
+// window's reset (all accumulators 0, etamin 1e20, the times kept) when close.
+__global__ void k_stat_fix(Geo g, double *acc, size_t npad, double timehrs, const int *flag, int close) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npad) return;
+    if (*flag) acc[(size_t)S_TETAMIN * npad + t] = timehrs;
+    if (!close) return;
+    for (int q = 0; q < NSTAT; q++) {
+        if (q == S_TETAMAX || q == S_TETAMIN || q == S_TUVMAX) continue;
+        acc[(size_t)q * npad + t] = q == S_ETAMIN ? F4(1.0e20) : 0.0;
+    }
+}
+
+}  // namespace cms

@@ -195,6 +195,12 @@ struct cms2d_flow {
     // per-block partials on the fixed Krylov grid and the running totals
     double *bal_part = nullptr;
     cms::BalAcc *bal = nullptr;
+    // flow statistics (FLOW_STATISTICS): window [stat_t1, stat_t2] s, steps counted,
+    // accumulators [NSTAT][npad], the new-minimum flag
+    int stat_on = 0, stat_n = 0;
+    double stat_t1 = 0.0, stat_t2 = 0.0;
+    double *stat_acc = nullptr;
+    int *stat_flag = nullptr;
 };
 
 // Lagrange interpolation of the curve (t, v) at time xi with the window search
@@ -1486,6 +1492,17 @@ static int step_once(cms2d_flow *F) {
     hipLaunchKernelGGL(k_bal_part, dim3(F->nblk_kry), dim3(KBLOCK), 0, F->st, F->g, F->s, F->P, F->bal_part);
     hipLaunchKernelGGL(k_bal_fin, dim3(1), dim3(KBLOCK), 0, F->st, F->g, F->s, F->P, (const double *)F->bal_part,
                        F->nblk_kry, F->bal);
+    // stat_update (CMS_Flow.f90:78-81, before flow_update): from the window's start,
+    // the window closing every stat_t2 - stat_t1 after it (statistics.F90:503,632)
+    if (F->stat_on && F->timesecs + F4(1.0e-4) >= F->stat_t1) {
+        F->stat_n += 1;
+        const double th = F->timesecs / 3600.0;
+        const int close = std::fmod(F->timesecs - F->stat_t1, F->stat_t2 - F->stat_t1) <= F4(1.0e-4);
+        LAUNCH(k_set_int, 1, F->stat_flag, 1, 0);
+        LAUNCH(k_stat, F->nblk_act, F->g, F->s, F->P, F->stat_acc, (size_t)F->npad, th, F->stat_flag);
+        LAUNCH(k_stat_fix, F->nblk_pad, F->g, F->stat_acc, (size_t)F->npad, th, (const int *)F->stat_flag, close);
+        if (close) F->stat_n = 0;
+    }
     LAUNCH(k_update, F->nblk_pad, F->g, F->s);
     return 0;
 }
@@ -1536,6 +1553,50 @@ extern "C" int cms2d_flow_balance(cms2d_flow *F, cms2d_balance *out) {
     out->courant_cell = (int)a.xc[3];
     out->steps = (int)a.steps;
     return CMS2D_OK;
+}
+
+// FLOW_STATISTICS (stat_init_flow, statistics.F90:172-320): accumulators zero.
+extern "C" int cms2d_flow_statistics(cms2d_flow *F, double t_start, double t_end) {
+    if (!F) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (F->tel) return set_err(CMS2D_ERR_ARG, "flow statistics are built for Cartesian grids");
+    if (!(t_end > t_start) || !std::isfinite(t_start) || !std::isfinite(t_end))
+        return set_err(CMS2D_ERR_ARG, "statistics window must have t_end > t_start");
+    HIPCHK(hipSetDevice(F->dev));
+    if (!F->stat_acc) {
+        if (int rc = dalloc(F, &F->stat_acc, (size_t)NSTAT * F->npad)) return rc;
+        if (int rc = dalloc(F, &F->stat_flag, 1)) return rc;
+    } else {
+        HIPCHK(hipMemsetAsync(F->stat_acc, 0, (size_t)NSTAT * F->npad * sizeof(double), F->st));
+    }
+    F->stat_on = 1;
+    F->stat_n = 0;
+    F->stat_t1 = t_start;
+    F->stat_t2 = t_end;
+    return CMS2D_OK;
+}
+
+extern "C" int cms2d_flow_stat(cms2d_flow *F, const char *name, double *out, int *count) {
+    static const char *names[NSTAT] = {
+        "etamax", "tetamax", "etamin", "tetamin", "uvmax", "umax", "vmax", "tuvmax", "etamean", "ur", "vr", "uvcurv",
+        "hydper", "rsuavg", "rsvavg", "rspavg", "rsumax", "rsvmax", "rspmax", "etamaxgrad", "uvmaxgrad", "frxmax",
+        "frymax", "umaxcurvx", "umaxcurvy", "vmaxcurvx", "vmaxcurvy", "cflxmax", "cflymax", "qcflxmax", "qcflymax",
+        "qfrxmax", "qfrymax"};
+    if (!F || !name || !out) return set_err(CMS2D_ERR_ARG, "null argument");
+    if (!F->stat_on) return set_err(CMS2D_ERR_ARG, "flow statistics not enabled (cms2d_flow_statistics)");
+    for (int q = 0; q < NSTAT; q++) {
+        if (strcmp(name, names[q])) continue;
+        const Geo &g = F->g;
+        std::vector<double> buf(F->npad);
+        HIPCHK(hipSetDevice(F->dev));
+        HIPCHK(hipMemcpyAsync(buf.data(), F->stat_acc + (size_t)q * F->npad, F->npad * sizeof(double),
+                              hipMemcpyDeviceToHost, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+        for (int j = 0; j < g.ny; j++)
+            for (int i = 0; i < g.nx; i++) out[(size_t)j * g.nx + i] = buf[(size_t)(j + 1) * g.nxp + i + 1];
+        if (count) *count = F->stat_n;
+        return CMS2D_OK;
+    }
+    return set_err(CMS2D_ERR_ARG, std::string("unknown statistic ") + name);
 }
 
 extern "C" int cms2d_ensemble_step(cms2d_flow **flows, int nflows, int nsteps, int nthreads,

@@ -124,6 +124,17 @@ class FlowStep:
         check(_lib.lib().cms2d_flow_balance(self._h, C.byref(b)))
         return {name: getattr(b, name) for name, _ in _lib.Balance._fields_}
 
+    def statistics(self, t_start, t_end):
+        """Flow statistics over windows [t_start, t_end] in seconds
+        (cms2d_flow_statistics, the FLOW_STATISTICS card)."""
+        check(_lib.lib().cms2d_flow_statistics(self._h, float(t_start), float(t_end)))
+
+    def stat(self, name):
+        """(accumulator of `name` per cell, steps counted in the open window)."""
+        out, n = np.empty(self.ncells), C.c_int()
+        check(_lib.lib().cms2d_flow_stat(self._h, name.encode(), _dp(out), C.byref(n)))
+        return out, n.value
+
     def get(self, name):
         out = np.empty(self.ncells)
         check(_lib.lib().cms2d_flow_get(self._h, name.encode(), _dp(out)))

@@ -24,6 +24,8 @@
  *                       source/hot_start/hot_start.F90:693-1122,1125-1344,1512-1560
  *   cms2d_flow_balance <- flow_balance and flow_step_stat source/flow/flow_imp.F90:2260-2372,
  *                       source/flow/flow.F90:1912-2008
+ *   cms2d_flow_statistics/_stat <- stat_init_flow, stat_update_flow
+ *                       source/statistics/statistics.F90:172-320,476-720
  *   cms2d_last_error <- the diag_print_error messages  source/utilities/diag_lib.f90:177-220
  * Without a counterpart in the reference: cms2d_device_count (device probe),
  * cms2d_flow_kernel_stats (live kernel timing for measurement),
@@ -230,6 +232,25 @@ typedef struct {
     int steps;                    /* steps counted since creation */
 } cms2d_balance;
 int cms2d_flow_balance(cms2d_flow *flow, cms2d_balance *out);
+
+/* Flow statistics over time windows (the FLOW_STATISTICS card; stat_init_flow
+ * and stat_update_flow, source/statistics/statistics.F90:172-320,476-720):
+ * from t_start [s] every step adds to per-cell accumulators, and the window
+ * closes every t_end - t_start after t_start, where the reference writes its
+ * statistics and resets them (extremes and sums to 0, etamin to 1e20, the
+ * times of the extremes kept).  Cartesian grids. */
+int cms2d_flow_statistics(cms2d_flow *flow, double t_start, double t_end);
+/* One accumulator per active cell (row-major) and the steps counted in the
+ * open window: "etamax", "tetamax" [hr], "etamin", "tetamin" (the reference
+ * sets every cell's to the time of the latest new minimum anywhere), "uvmax",
+ * "umax", "vmax", "tuvmax", the sums "etamean", "ur", "vr", "uvcurv"
+ * (streamwise curvature), "hydper" (wet steps), "rsuavg", "rsvavg", "rspavg",
+ * the maxima "rsumax", "rsvmax", "rspmax" (normalized residuals),
+ * "etamaxgrad", "uvmaxgrad", "frxmax", "frymax" (u/c, v/c), "umaxcurvx",
+ * "umaxcurvy", "vmaxcurvx", "vmaxcurvy" (second derivatives), "cflxmax",
+ * "cflymax", "qcflxmax", "qcflymax", "qfrxmax", "qfrymax".  The reference's
+ * window results are the sums divided by the count. */
+int cms2d_flow_stat(cms2d_flow *flow, const char *name, double *out, int *count);
 /* Copies an active-cell field to host memory, row-major [nx*ny]:
  * "eta","u","v","p","h","iwet","vis","cfrict","coefman","z0","uv","bsvel","pp","dpx","dpy",
  * "zb","fc","wavestrx","wavestry", and "maxeta" (running maximum of eta over the steps,

@@ -92,6 +92,11 @@ struct cms_oracle {
     int nW, *Wc, *Wk;
     int nTH, *THc, *THk;
     int nQ, *Qc, *Qk;     /* flux (river) string Q_str, cells and faces in string order */
+    /* flow statistics (stat_def): on, window start/end [s], steps counted, the
+     * per-cell accumulators in the order of cms_oracle_stat_names */
+    int stat_on, stat_n;
+    double stat_t1, stat_t2;
+    double *st[33];
     double qflux, cmvel, qcos, qsin;   /* its total inflow this step, conveyance, inflow direction */
     int q_inc;            /* window start of the flux curve (Q_str%inc) */
     double *wsebnd, *wseadj;
@@ -1954,6 +1959,7 @@ static void flow_convflux(cms_oracle *o)
 }
 
 static void init_state(cms_oracle *o);
+static void stat_update_flow(cms_oracle *o);
 
 cms_oracle *cms_oracle_create(const cms_case *c)
 {
@@ -2081,6 +2087,7 @@ void cms_oracle_destroy(cms_oracle *o)
                     o->ncf, o->idir, o->nxf, o->kxf, o->nyf, o->kyf, o->isjoint, o->joint, o->rpx, o->rpy,
                     o->dHux, o->dHuy, o->dHvx, o->dHvy, o->dapux, o->dapuy, o->dzbx, o->dzby, o->goa};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); k++) free(ptrs[k]);
+    for (int q = 0; q < 33; q++) free(o->st[q]);
     free(o);
 }
 
@@ -2142,9 +2149,117 @@ int cms_oracle_step(cms_oracle *o, int nsteps, cms_step_info *info)
         }
         if (rc != 0) return rc;
         o->bndvol += bnd_inflow(o);
+        if (o->stat_on) stat_update_flow(o);       /* CMS_Flow.f90:78-81, before flow_update */
         flow_update(o);
     }
     return 0;
+}
+
+/* Flow statistics (stat_update_flow, source/statistics/statistics.F90:476-720):
+ * accumulators over the steps of a window, reset when the window closes. */
+enum { S_ETAMAX, S_TETAMAX, S_ETAMIN, S_TETAMIN, S_UVMAX, S_UMAX, S_VMAX, S_TUVMAX, S_ETAMEAN, S_UR, S_VR,
+       S_UVCURV, S_HYDPER, S_RSUAVG, S_RSVAVG, S_RSPAVG, S_RSUMAX, S_RSVMAX, S_RSPMAX, S_ETAMAXGRAD, S_UVMAXGRAD,
+       S_FRXMAX, S_FRYMAX, S_UMAXCURVX, S_UMAXCURVY, S_VMAXCURVX, S_VMAXCURVY, S_CFLXMAX, S_CFLYMAX,
+       S_QCFLXMAX, S_QCFLYMAX, S_QFRXMAX, S_QFRYMAX, NSTAT };
+static const char *stat_names[NSTAT] = {
+    "etamax", "tetamax", "etamin", "tetamin", "uvmax", "umax", "vmax", "tuvmax", "etamean", "ur", "vr", "uvcurv",
+    "hydper", "rsuavg", "rsvavg", "rspavg", "rsumax", "rsvmax", "rspmax", "etamaxgrad", "uvmaxgrad", "frxmax",
+    "frymax", "umaxcurvx", "umaxcurvy", "vmaxcurvx", "vmaxcurvy", "cflxmax", "cflymax", "qcflxmax", "qcflymax",
+    "qfrxmax", "qfrymax"};
+
+/* dx2d / dy2d (der_lib.f90): one cell's gradient component with the wet-cell operator */
+static double grad_x(const cms_oracle *o, int i, const double *var)
+{
+    const gradop *G = &o->gow[i];
+    double sx = 0.0;
+    for (int k = 0; k < G->ncx; k++) sx += G->wcx[k] * var[G->icx[k]];
+    return sx;
+}
+static double grad_y(const cms_oracle *o, int i, const double *var)
+{
+    const gradop *G = &o->gow[i];
+    double sy = 0.0;
+    for (int k = 0; k < G->ncy; k++) sy += G->wcy[k] * var[G->icy[k]];
+    return sy;
+}
+
+static void stat_update_flow(cms_oracle *o)
+{
+    if (o->ctime + F4(1.0e-4) < o->stat_t1) return;
+    o->stat_n += 1;
+    double **S = o->st;
+    const int nc = o->ncells;
+    for (int i = 0; i < nc; i++) {
+        if (o->iwet[i] == 0) continue;
+        if (o->eta[i] > S[S_ETAMAX][i]) { S[S_ETAMAX][i] = o->eta[i]; S[S_TETAMAX][i] = o->timehrs; }
+        if (o->eta[i] < S[S_ETAMIN][i]) {
+            S[S_ETAMIN][i] = o->eta[i];
+            for (int q = 0; q < o->ncellsD; q++) S[S_TETAMIN][q] = o->timehrs;   /* tetamin = timehrs: the array */
+        }
+        if (o->uv[i] > S[S_UVMAX][i]) {
+            S[S_UVMAX][i] = o->uv[i]; S[S_UMAX][i] = o->u[i]; S[S_VMAX][i] = o->v[i]; S[S_TUVMAX][i] = o->timehrs;
+        }
+        S[S_ETAMEAN][i] += o->eta[i];
+        S[S_UR][i] += o->u[i];
+        S[S_VR][i] += o->v[i];
+        /* streamwise_curvature (flow_lib.F90:256-276) */
+        const double u = o->u[i], v = o->v[i];
+        S[S_UVCURV][i] += (u * u * o->dvx[i] - v * v * o->duy[i] + u * v * (o->dvy[i] - o->dux[i]))
+                          / pow(fmax(o->uv[i], F4(1.e-6)), 1.5);
+        S[S_HYDPER][i] += o->iwet[i];
+        S[S_RSUAVG][i] += o->rsu[i]; S[S_RSVAVG][i] += o->rsv[i]; S[S_RSPAVG][i] += o->rsp[i];
+        S[S_RSPMAX][i] = fmax(S[S_RSPMAX][i], o->rsp[i]);
+        S[S_RSUMAX][i] = fmax(S[S_RSUMAX][i], o->rsu[i]);
+        S[S_RSVMAX][i] = fmax(S[S_RSVMAX][i], o->rsv[i]);
+        double val = sqrt(o->dpx[i] * o->dpx[i] + o->dpy[i] * o->dpy[i]) * (1.0 / GRAV);
+        S[S_ETAMAXGRAD][i] = fmax(S[S_ETAMAXGRAD][i], val);
+        val = sqrt(o->dux[i] * o->dux[i] + o->duy[i] * o->duy[i]);
+        S[S_UVMAXGRAD][i] = fmax(S[S_UVMAXGRAD][i], val);
+        val = sqrt(o->dvx[i] * o->dvx[i] + o->dvy[i] * o->dvy[i]);
+        S[S_UVMAXGRAD][i] = fmax(S[S_UVMAXGRAD][i], val);
+        const double c = sqrt(GRAV * o->h[i]);
+        S[S_FRXMAX][i] = fmax(S[S_FRXMAX][i], u / c);
+        S[S_FRYMAX][i] = fmax(S[S_FRYMAX][i], v / c);
+        S[S_UMAXCURVX][i] = fmax(S[S_UMAXCURVX][i], grad_x(o, i, o->dux));
+        S[S_UMAXCURVY][i] = fmax(S[S_UMAXCURVY][i], grad_y(o, i, o->duy));
+        S[S_VMAXCURVX][i] = fmax(S[S_VMAXCURVX][i], grad_x(o, i, o->dvx));
+        S[S_VMAXCURVY][i] = fmax(S[S_VMAXCURVY][i], grad_y(o, i, o->dvy));
+    }
+    for (int i = 0; i < nc; i++) {
+        const double c = sqrt(GRAV * o->h[i]);
+        S[S_CFLXMAX][i] = fmax(S[S_CFLXMAX][i], (fabs(o->u[i]) + c) * o->dtime / o->dx[i]);
+        S[S_CFLYMAX][i] = fmax(S[S_CFLYMAX][i], (fabs(o->v[i]) + c) * o->dtime / o->dy[i]);
+    }
+    for (int i = 0; i < nc; i++) {
+        S[S_QCFLXMAX][i] = fmax(S[S_QCFLXMAX][i], o->dx[i] / o->dtime / (fabs(o->u[i]) + sqrt(GRAV * o->h[i])));
+        S[S_QCFLYMAX][i] = fmax(S[S_QCFLYMAX][i], o->dy[i] / o->dtime / (fabs(o->v[i]) + sqrt(GRAV * o->h[i])));
+        S[S_QFRXMAX][i] = fmax(S[S_QFRXMAX][i], o->dx[i] / o->dtime * fabs(o->u[i]) / sqrt(GRAV * o->h[i]));
+        S[S_QFRYMAX][i] = fmax(S[S_QFRYMAX][i], o->dy[i] / o->dtime * fabs(o->v[i]) / sqrt(GRAV * o->h[i]));
+    }
+    if (fmod(o->timesecs - o->stat_t1, o->stat_t2 - o->stat_t1) > F4(1.0e-4)) return;
+    /* the window closes: results written (not restated), accumulators reset */
+    o->stat_n = 0;
+    for (int q = 0; q < NSTAT; q++) {
+        if (q == S_TETAMAX || q == S_TETAMIN || q == S_TUVMAX) continue;
+        for (int i = 0; i < nc; i++) S[q][i] = q == S_ETAMIN ? F4(1.0e20) : 0.0;
+    }
+}
+
+/* FLOW_STATISTICS start end [s]: accumulators zero (stat_init_flow, :172-320) */
+int cms_oracle_stats(cms_oracle *o, double t1, double t2)
+{
+    if (!(t2 > t1)) return -1;
+    o->stat_on = 1; o->stat_n = 0; o->stat_t1 = t1; o->stat_t2 = t2;
+    for (int q = 0; q < NSTAT; q++) { free(o->st[q]); o->st[q] = DALLOC(o->ncellsD); }
+    return 0;
+}
+int cms_oracle_stat_get(const cms_oracle *o, const char *name, double *out, int *count)
+{
+    if (!o->stat_on) return -1;
+    *count = o->stat_n;
+    for (int q = 0; q < NSTAT; q++)
+        if (!strcmp(name, stat_names[q])) { memcpy(out, o->st[q], sizeof(double) * o->ncells); return 0; }
+    return -1;
 }
 
 static const double *field(const cms_oracle *o, const char *name)

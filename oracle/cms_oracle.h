@@ -164,6 +164,12 @@ int cms_oracle_solve(int ncells, int ncellsD, int nf, const double *acoef, const
                      double *phi, double relax, double rtol, double atol, int kmax,
                      double *rmom);
 
+/* Flow statistics over windows [t1, t2] [s] (FLOW_STATISTICS, stat_update_flow,
+ * source/statistics/statistics.F90:476-720): the accumulators of one field
+ * ("etamax", "etamean", ... in the order of stat_def) and the steps counted. */
+int cms_oracle_stats(cms_oracle *o, double t1, double t2);
+int cms_oracle_stat_get(const cms_oracle *o, const char *name, double *out, int *count);
+
 #ifdef __cplusplus
 }
 #endif

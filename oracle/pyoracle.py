@@ -73,6 +73,8 @@ def lib():
         L.cms_oracle_plagr.restype = C.c_double
         L.cms_oracle_plagr.argtypes = [C.c_int, D, D, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.cms_oracle_grad.argtypes = [C.c_void_p, D, D, D]
+        L.cms_oracle_stats.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.cms_oracle_stat_get.argtypes = [C.c_void_p, C.c_char_p, D, C.POINTER(C.c_int)]
         L.cms_oracle_set_waves.argtypes = [C.c_void_p, D, D, D, D, D]
         L.cms_oracle_set_wind.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.cms_oracle_wind_drag_hsu.restype = C.c_double
@@ -184,6 +186,18 @@ class Oracle:
         if lib().cms_oracle_get_face(self.h, name.encode(), _dp(out)) != 0:
             raise KeyError(name)
         return out.reshape(self.ncells, self.nf)
+
+    def stats(self, t1, t2):
+        """Flow statistics over windows [t1, t2] in seconds (FLOW_STATISTICS)."""
+        if lib().cms_oracle_stats(self.h, float(t1), float(t2)) != 0:
+            raise ValueError("statistics window")
+
+    def stat(self, name):
+        """(accumulator of `name` per cell, steps counted in the open window)."""
+        out, n = np.empty(self.ncells), C.c_int()
+        if lib().cms_oracle_stat_get(self.h, name.encode(), _dp(out), C.byref(n)) != 0:
+            raise KeyError(name)
+        return out, n.value
 
     def topology(self):
         """(ncface [n], cell2cell [nD, nf], idirface [nD, nf], isjoint [n])."""

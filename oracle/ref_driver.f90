@@ -27,6 +27,11 @@
 !   volH2Ocumflux, volH2Ocumnorm; flow_imp.F90:2260-2372) and the extremes of
 !   flow_step_stat (pxtrm, idp, uxtrm, idu, vxtrm, idv, crmax, idcr, cell
 !   numbers 1-based; source/flow/flow.F90:1912-2008).
+!   With CMS2D_REF_STATS naming a file (and a FLOW_STATISTICS card) the loop
+!   also calls stat_update where CMS_Flow does (after flow_imp, before
+!   flow_update, source/main/CMS_Flow.f90:78-81) and appends after each step
+!   nflowstatcount and the flow statistics of every cell (stat_def, in the
+!   order of STATS below).
 !=============================================================================
 program ref_run
     use comvarbl,  only: ctlfile, flowpath, casename
@@ -36,10 +41,14 @@ program ref_run
     use size_def,  only: ncells
     use flow_def,  only: eta, u, v, iwet, volH2Ocumstg, volH2Ocumbnd, volH2Ocumflux, volH2Ocumnorm
     use comvarbl,  only: pxtrm, idp, uxtrm, idu, vxtrm, idv, crmax, idcr
+    use stat_def,  only: nflowstatcount, etamax, tetamax, etamin, tetamin, uvmax, umax, vmax, tuvmax, &
+        etamean, ur, vr, uvcurv, hydper, rsuavg, rsvavg, rspavg, rsumax, rsvmax, rspmax, etamaxgrad, &
+        uvmaxgrad, frxmax, frymax, umaxcurvx, umaxcurvy, vmaxcurvx, vmaxcurvy, cflxmax, cflymax, &
+        qcflxmax, qcflymax, qFrxmax, qFrymax
     use wave_flowgrid_def, only: wavestrx, wavestry, constant_waves
     implicit none
-    character(len=400) :: dir, name, arg, outfile, forcefile, balfile
-    integer :: nsteps, n, i, ou, fu, bu, blen
+    character(len=400) :: dir, name, arg, outfile, forcefile, balfile, statfile
+    integer :: nsteps, n, i, ou, fu, bu, blen, su, slen
     integer(8) :: t0, t1, rate
     logical :: dump
     real(8), allocatable :: buf(:)
@@ -84,6 +93,8 @@ program ref_run
       deallocate(buf)
     endif
 
+    call get_environment_variable('CMS2D_REF_STATS', statfile, slen)
+    if (slen > 0) open(newunit=su, file=trim(statfile), access='stream', form='unformatted', status='replace')
     call get_environment_variable('CMS2D_REF_BALANCE', balfile, blen)
     if (blen > 0) open(newunit=bu, file=trim(balfile), access='stream', form='unformatted', status='replace')
     dump = trim(outfile) /= '-'
@@ -94,6 +105,27 @@ program ref_run
       call timing
       if (constant_waves) call wave_const_update
       call flow_imp
+      if (slen > 0) call stat_update
+      if (slen > 0) then
+        write(su) real(nflowstatcount, 8)
+        write(su) (real(etamax(i), 8), i = 1, ncells), (real(tetamax(i), 8), i = 1, ncells)
+        write(su) (real(etamin(i), 8), i = 1, ncells), (real(tetamin(i), 8), i = 1, ncells)
+        write(su) (real(uvmax(i), 8), i = 1, ncells), (real(umax(i), 8), i = 1, ncells)
+        write(su) (real(vmax(i), 8), i = 1, ncells), (real(tuvmax(i), 8), i = 1, ncells)
+        write(su) (real(etamean(i), 8), i = 1, ncells), (real(ur(i), 8), i = 1, ncells)
+        write(su) (real(vr(i), 8), i = 1, ncells), (real(uvcurv(i), 8), i = 1, ncells)
+        write(su) (real(hydper(i), 8), i = 1, ncells), (real(rsuavg(i), 8), i = 1, ncells)
+        write(su) (real(rsvavg(i), 8), i = 1, ncells), (real(rspavg(i), 8), i = 1, ncells)
+        write(su) (real(rsumax(i), 8), i = 1, ncells), (real(rsvmax(i), 8), i = 1, ncells)
+        write(su) (real(rspmax(i), 8), i = 1, ncells), (real(etamaxgrad(i), 8), i = 1, ncells)
+        write(su) (real(uvmaxgrad(i), 8), i = 1, ncells), (real(frxmax(i), 8), i = 1, ncells)
+        write(su) (real(frymax(i), 8), i = 1, ncells), (real(umaxcurvx(i), 8), i = 1, ncells)
+        write(su) (real(umaxcurvy(i), 8), i = 1, ncells), (real(vmaxcurvx(i), 8), i = 1, ncells)
+        write(su) (real(vmaxcurvy(i), 8), i = 1, ncells), (real(cflxmax(i), 8), i = 1, ncells)
+        write(su) (real(cflymax(i), 8), i = 1, ncells), (real(qcflxmax(i), 8), i = 1, ncells)
+        write(su) (real(qcflymax(i), 8), i = 1, ncells), (real(qFrxmax(i), 8), i = 1, ncells)
+        write(su) (real(qFrymax(i), 8), i = 1, ncells)
+      endif
       call flow_update
       call system_clock(t1)
       write(*, '(a,i0,1x,es14.6)') 'step_seconds ', n, real(t1 - t0, 8) / real(rate, 8)
@@ -121,6 +153,7 @@ program ref_run
     enddo
     if (dump) close(ou)
     if (blen > 0) close(bu)
+    if (slen > 0) close(su)
 end program ref_run
 
 !-----------------------------------------------------------------------------

@@ -234,6 +234,23 @@ BALANCE = {
 }
 
 
+# flow statistics (FLOW_STATISTICS 0 0.05 0 hrs: windows of 6 steps of 30 s)
+# of the tidal case: ref64_stats_tidal.npz holds every stat_def field after
+# step 5 (open window) and step 11 (the second window, after a reset); written
+# with --stats
+def write_stats():
+    c = cases.tidal(nx=48, ny=32, dx=20.0, seed=1)
+    with tempfile.TemporaryDirectory() as d:
+        res = refcase.run(c, d + "/", 11, exe=refcase.REF_RUN8, stats=(0.0, 0.05, 0.0))
+    out = {"steps": np.array([5, 11]), "window_s": np.array([0.0, 180.0])}
+    for s in (5, 11):
+        out[f"count_{s}"] = np.array(res[s - 1]["statcount"])
+        for k, v in res[s - 1]["stats"].items():
+            out[f"{k}_{s}"] = v
+    np.savez_compressed(os.path.join(HERE, "ref64_stats_tidal.npz"), **out)
+    print("wrote ref64_stats_tidal.npz")
+
+
 def write_balance(names):
     exe = refcase.REF_RUN8
     for name, (mk, nsteps) in BALANCE.items():
@@ -255,6 +272,9 @@ def main():
     args = sys.argv[1:]
     double = "--double" in args
     args = [a for a in args if a != "--double"]
+    if "--stats" in args:
+        write_stats()
+        return 0
     if "--balance" in args:
         if not refcase.available(refcase.REF_RUN8):
             print(f"{refcase.REF_RUN8} not built (python oracle/build_ref.py --double)", file=sys.stderr)

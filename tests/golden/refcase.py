@@ -68,7 +68,13 @@ def side_ids(nx, ny, side):
     return [i + 1 for i in range(nx)]
 
 
-def write_case(case, d, name="case", solver="BICGSTAB", balance=False):
+STATS = ("etamax", "tetamax", "etamin", "tetamin", "uvmax", "umax", "vmax", "tuvmax", "etamean", "ur", "vr",
+         "uvcurv", "hydper", "rsuavg", "rsvavg", "rspavg", "rsumax", "rsvmax", "rspmax", "etamaxgrad", "uvmaxgrad",
+         "frxmax", "frymax", "umaxcurvx", "umaxcurvy", "vmaxcurvx", "vmaxcurvy", "cflxmax", "cflymax", "qcflxmax",
+         "qcflymax", "qfrxmax", "qfrymax")
+
+
+def write_case(case, d, name="case", solver="BICGSTAB", balance=False, stats=None):
     os.makedirs(d, exist_ok=True)
     nx, ny = case.nx, case.ny
     tel = getattr(case, "tel", None)
@@ -166,6 +172,7 @@ MATRIX_SOLVER                       {solver}
 {fric_card}
 {"CONSTANT_WAVE_HEIGHT                0.0" if case.noptset >= 3 else ""}
 {"WATER_VOLUME_BALANCE                ON" if balance else ""}
+{f"FLOW_STATISTICS                     {stats[0]:.9g} {stats[1]:.9g} {stats[2]:.9g} 'hrs'" if stats else ""}
 OUTPUT_FILE_TYPE                    ASCII
 HORIZONTAL_PROJECTION_BEGIN
   DATUM                             None
@@ -188,12 +195,12 @@ BOUNDARY_BEGIN
     return d
 
 
-def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RUN, balance=False):
+def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RUN, balance=False, stats=None):
     """Returns a list of dicts (eta, u, v, iwet) after each step; with
     balance=True each also holds "bal": flow_balance's cumulative storage,
     efflux, flux and norm, then flow_step_stat's pxtrm, idp, uxtrm, idu,
     vxtrm, idv, crmax, idcr (cells 1-based)."""
-    write_case(case, d, name, solver, balance=balance)
+    write_case(case, d, name, solver, balance=balance, stats=stats)
     out = os.path.join(d, "state.bin")
     args = [exe, d.rstrip("/") + "/", name, str(nsteps), out]
     if case.noptset >= 3 and case.wavestrx is not None:
@@ -207,6 +214,9 @@ def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RU
     balfile = os.path.join(d, "balance.bin")
     if balance:
         env["CMS2D_REF_BALANCE"] = balfile
+    statfile = os.path.join(d, "stats.bin")
+    if stats:
+        env["CMS2D_REF_STATS"] = statfile
     r = subprocess.run(args, cwd=d, capture_output=True, text=True, timeout=timeout, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"ref_run failed ({r.returncode}):\n{r.stdout[-3000:]}\n{r.stderr[-3000:]}")
@@ -219,6 +229,13 @@ def run(case, d, nsteps, name="case", solver="BICGSTAB", timeout=600, exe=REF_RU
         blk = raw[pos:pos + 4 * n].reshape(4, n)
         pos += 4 * n
         res.append({"eta": blk[0], "u": blk[1], "v": blk[2], "iwet": blk[3]})
+    if stats:
+        raw = np.fromfile(statfile, dtype=np.float64)
+        n = case.ncells
+        blk = raw.reshape(-1, 1 + len(STATS) * n)
+        for r, b in zip(res, blk):
+            r["statcount"] = int(b[0])
+            r["stats"] = {k: b[1 + q * n:1 + (q + 1) * n] for q, k in enumerate(STATS)}
     if balance:
         bal = np.fromfile(balfile, dtype=np.float64).reshape(-1, 12)
         for r, b in zip(res, bal):
