@@ -59,7 +59,8 @@ class FlowDesc(C.Structure):
         ("wse_adjust_off", C.c_int), ("wse_gradx", C.c_double), ("wse_grady", C.c_double),
         ("serial_order", C.c_int),
         ("flux_q", C.c_double), ("flux_times", D), ("flux_values", D), ("flux_ntimes", C.c_int),
-        ("flux_order", C.c_int), ("flux_conveyance", C.c_double),
+        ("flux_order", C.c_int), ("flux_conveyance", C.c_double), ("flux_angle_on", C.c_int),
+        ("flux_angle", C.c_double),
     ]
 
 

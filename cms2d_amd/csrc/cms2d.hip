@@ -1141,6 +1141,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             !(d->flux_conveyance >= 0.0) || !std::isfinite(d->flux_conveyance))
             return set_err(CMS2D_ERR_ARG, "flux curve or conveyance invalid");
         if (d->flux_ntimes == 0 && !std::isfinite(d->flux_q)) return set_err(CMS2D_ERR_ARG, "flux not finite");
+        if (d->flux_angle_on && !std::isfinite(d->flux_angle)) return set_err(CMS2D_ERR_ARG, "inflow angle not finite");
         if (d->flux_ntimes > 0) {
             if (!d->flux_times || !d->flux_values) return set_err(CMS2D_ERR_ARG, "flux curve missing");
             for (int i = 0; i < d->flux_ntimes; i++)
@@ -1194,7 +1195,14 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                 F->q_inc = 1;
                 F->cmvel = d->flux_conveyance > 0.0 ? std::max(std::min(d->flux_conveyance, 1.0), 0.5) : F4(0.667);
                 const double pi = std::acos(-1.0), deg2rad = pi / 180.0;
-                const double angle = (double)((float)(4 - (k + 1)) * 90.0f);
+                double angle = (double)((float)(4 - (k + 1)) * 90.0f);
+                if (d->flux_angle_on) {
+                    // INFLOW_ANGLE (boundaries.F90:362-370): 90 - angle - grid azimuth (0),
+                    // within [0, 360]
+                    angle = 90.0 - d->flux_angle - 0.0;
+                    if (angle < 0.0) angle += 360.0;
+                    else if (angle > 360.0) angle -= 360.0;
+                }
                 F->qcos = std::cos(angle * deg2rad);
                 F->qsin = std::sin(angle * deg2rad);
             }

@@ -55,6 +55,8 @@ module cms2d_hip
         type(c_ptr) :: flux_times = c_null_ptr, flux_values = c_null_ptr   ! or its curve [hr], [m^3/s]
         integer(c_int) :: flux_ntimes = 0, flux_order = 0
         real(c_double) :: flux_conveyance = 0
+        integer(c_int) :: flux_angle_on = 0                            ! INFLOW_ANGLE given
+        real(c_double) :: flux_angle = 0                               ! [deg clockwise from north]
     end type
 
     ! struct cms2d_step_info

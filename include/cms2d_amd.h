@@ -60,7 +60,7 @@ extern "C" {
                                      bnd_vel_flux, bnd_vel.f90:2-32; bndflux :4359-4414): the total
                                      inflow flux_q, or the curve flux_times/flux_values, split over
                                      the side's faces by the conveyance hk^(1+c)/n along the inward
-                                     normal; Manning friction only                              */
+                                     normal or flux_angle; Manning friction only                */
 #define CMS2D_BND_WSE_CURVE 2     /* single water level from a time series, WSE_CURVE / WSE_CONSTANT
                                      (H_str, boundaries.F90:4127-4150): the curve wse_times/
                                      wse_values, Lagrange interpolation of order wse_order
@@ -156,6 +156,9 @@ typedef struct {
     int flux_order;               /* TEMPORAL_INTERPOLATION_ORDER 1-3 (0 -> 1, the reference's
                                      default for fluxes), limited to flux_ntimes - 1          */
     double flux_conveyance;       /* CONVEYANCE coefficient, limited to [0.5, 1]; 0 -> 0.667   */
+    int flux_angle_on;            /* 1: INFLOW_ANGLE flux_angle [deg clockwise from north, grid
+                                     angle 0] instead of the inward normal                    */
+    double flux_angle;
 } cms2d_flow_desc;
 
 typedef struct {

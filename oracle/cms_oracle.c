@@ -511,7 +511,12 @@ static void build_bnd(cms_oracle *o)
         o->nQ = string_faces(o, cells, ns, &o->Qc, &o->Qk);
         free(cells);
         const double pi = acos(-1.0), deg2rad = pi / 180.0;
-        const double angle = (double)(float)((float)(4 - (o->c.q_side + 1)) * 90.0f);
+        double angle = (double)(float)((float)(4 - (o->c.q_side + 1)) * 90.0f);
+        if (o->c.q_angle_on) {   /* INFLOW_ANGLE (boundaries.F90:362-370), grid azimuth 0 */
+            angle = 90.0 - o->c.q_angle - 0.0;
+            if (angle < 0.0) angle += 360.0;
+            else if (angle > 360.0) angle -= 360.0;
+        }
         o->qcos = cos(angle * deg2rad);
         o->qsin = sin(angle * deg2rad);
         o->cmvel = o->c.q_cmvel > 0.0 ? fmax(fmin(o->c.q_cmvel, 1.0), 0.5) : F4(0.667);

@@ -87,6 +87,8 @@ typedef struct {
     int q_ntimes;
     int q_order;
     double q_cmvel;
+    int q_angle_on;           /* INFLOW_ANGLE q_angle [deg clockwise from north] instead of the normal */
+    double q_angle;
 } cms_case;
 
 typedef struct {

@@ -34,6 +34,7 @@ class CmsCase(C.Structure):
         ("noskew", C.c_int),
         ("q_side", C.c_int), ("q_const", C.c_double), ("q_times", D), ("q_values", D),
         ("q_ntimes", C.c_int), ("q_order", C.c_int), ("q_cmvel", C.c_double),
+        ("q_angle_on", C.c_int), ("q_angle", C.c_double),
     ]
 
 
@@ -151,6 +152,8 @@ class Oracle:
                 c.q_times, c.q_values = arr(case.q_times), arr(case.q_values)
                 c.q_ntimes, c.q_order = len(case.q_times), case.q_order
             c.q_cmvel = case.q_cmvel
+            if getattr(case, "q_angle", None) is not None:
+                c.q_angle_on, c.q_angle = 1, case.q_angle
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

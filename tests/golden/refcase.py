@@ -123,6 +123,8 @@ def write_case(case, d, name="case", solver="BICGSTAB", balance=False, stats=Non
             qcard = f"    FLUX_VALUE {case.q_const:.9g} 'm^3/s'\n"
         if case.q_cmvel:
             qcard += f"    CONVEYANCE {case.q_cmvel:.9g}\n"
+        if getattr(case, "q_angle", None) is not None:
+            qcard += f"    INFLOW_ANGLE {case.q_angle:.9g}\n"
         river = (f'BOUNDARY_BEGIN\n  NAME "river"\n  CELLSTRING "{name}_q.bid" "1"\n  FLUX_BEGIN\n'
                  f"{qcard}  FLUX_END\nBOUNDARY_END\n")
     if getattr(case, "wse_times", None) is not None:

@@ -205,6 +205,7 @@ RIVER = {
     "river_curve": dict(q_curve=([0.0, 0.005, 0.01, 0.3], [0.0, 30.0, 60.0, 45.0]), q_order=2),
     "river_shoal": dict(shoal=True, q=25.0),
     "river_conveyance": dict(q_cmvel=0.8, q_side=2),
+    "river_angle": dict(q_angle=75.0),           # INFLOW_ANGLE: 15 degrees off the normal
 }
 for _name, _kw in RIVER.items():
     CASES[_name] = (lambda kw=_kw: cases.river(**kw))
