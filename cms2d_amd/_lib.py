@@ -60,7 +60,7 @@ class FlowDesc(C.Structure):
         ("serial_order", C.c_int),
         ("flux_q", C.c_double), ("flux_times", D), ("flux_values", D), ("flux_ntimes", C.c_int),
         ("flux_order", C.c_int), ("flux_conveyance", C.c_double), ("flux_angle_on", C.c_int),
-        ("flux_angle", C.c_double),
+        ("flux_angle", C.c_double), ("flux_rating", C.c_int),
     ]
 
 

@@ -86,6 +86,7 @@ class Case:
     q_order: int = 0
     q_cmvel: float = 0.0      # CONVEYANCE coefficient (0: the reference's default 0.667)
     q_angle: object = None    # INFLOW_ANGLE [deg clockwise from north] (None: the inward normal)
+    q_rating: bool = False    # q_times/q_values are a RATING_CURVE: mean depth [m] -> inflow
 
     def wind_vector(self):
         """(wndx, wndy) in grid coordinates as windcurve_init makes them
@@ -162,7 +163,7 @@ def tidal(nx=2000, ny=1500, dx=20.0, seed=1, nsteps=2880, dt=30.0, radstress=Tru
 
 
 def river(nx=48, ny=32, dx=20.0, seed=4, nsteps=40, dt=30.0, q_side=3, q=40.0, q_curve=None, q_order=0,
-          q_cmvel=0.0, shoal=False, q_angle=None) -> Case:
+          q_cmvel=0.0, shoal=False, q_angle=None, q_rating=None) -> Case:
     """An estuary: the M2 tide on the east side and a river inflow (flux
     string) on side q_side (west by default), depth 2 m at the west edge to
     12 m at the east with bumps; q_curve = (hours, m^3/s) replaces the
@@ -181,6 +182,8 @@ def river(nx=48, ny=32, dx=20.0, seed=4, nsteps=40, dt=30.0, q_side=3, q=40.0, q
     c.q_side, c.q_const, c.q_order, c.q_cmvel, c.q_angle = q_side, q, q_order, q_cmvel, q_angle
     if q_curve is not None:
         c.q_times, c.q_values = list(q_curve[0]), list(q_curve[1])
+    if q_rating is not None:                 # (stages [m], inflows [m^3/s])
+        c.q_times, c.q_values, c.q_rating = list(q_rating[0]), list(q_rating[1]), True
     return c
 
 

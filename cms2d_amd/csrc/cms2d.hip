@@ -185,6 +185,8 @@ struct cms2d_flow {
     std::vector<double> q_t, q_v;
     int q_order = 1, q_inc = 1;
     double cmvel = 0.0, qcos = 0.0, qsin = 0.0, qflux = 0.0;
+    int q_rating = 0;          // RATING_CURVE: the table and the step's inflow on the device
+    double *q_dev = nullptr;   // [1 + 2m]: inflow, stages, flows
     std::vector<double> wse_t, wse_v;   // water-level curve of CMS2D_BND_WSE_CURVE sides
     int wse_order = 1, wse_inc = 1;     // interpolation order, window start (H_str%inc, 1-based)
     int wse_adjust_off = 0;             // WSE_ADJUSTMENT OFF on the curve sides
@@ -848,7 +850,7 @@ static void river_vel(cms2d_flow *F) {
     if (F->qside < 0) return;
     const int n = (F->qside == KN || F->qside == KS) ? F->g.nx : F->g.ny;
     hipLaunchKernelGGL(k_river, dim3(1), dim3(1024), (size_t)n * sizeof(double), F->st, F->g, F->s, F->qside,
-                       F->qflux, F->cmvel, F->qcos, F->qsin);
+                       F->qflux, F->cmvel, F->qcos, F->qsin, (const double *)(F->q_rating ? F->q_dev : nullptr));
 }
 
 static int flow_recalculate(cms2d_flow *F) {
@@ -903,7 +905,10 @@ static int flow_imp_once(cms2d_flow *F) {
     double wsecurve = 0.0;
     if (!F->wse_t.empty()) wsecurve = wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc);
     // the flux string's inflow (:4078-4094): constant or the curve, times the ramp
-    if (F->qside >= 0) {
+    if (F->qside >= 0 && F->q_rating) {
+        // the stage is the string's mean depth at the step's start: evaluated on the device
+        LAUNCH(k_river_rating, 1, g, s, F->qside, (int)F->q_t.size(), F->ramp, F->q_dev);
+    } else if (F->qside >= 0) {
         const double q = F->q_t.empty() ? F->q_const : wse_curve_eval(F->q_t, F->q_v, F->q_order, timehrs, F->q_inc);
         F->qflux = F->ramp * q;
     }
@@ -1142,6 +1147,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
             return set_err(CMS2D_ERR_ARG, "flux curve or conveyance invalid");
         if (d->flux_ntimes == 0 && !std::isfinite(d->flux_q)) return set_err(CMS2D_ERR_ARG, "flux not finite");
         if (d->flux_angle_on && !std::isfinite(d->flux_angle)) return set_err(CMS2D_ERR_ARG, "inflow angle not finite");
+        if (d->flux_rating && d->flux_ntimes < 2) return set_err(CMS2D_ERR_ARG, "a rating curve needs 2 or more stages");
         if (d->flux_ntimes > 0) {
             if (!d->flux_times || !d->flux_values) return set_err(CMS2D_ERR_ARG, "flux curve missing");
             for (int i = 0; i < d->flux_ntimes; i++)
@@ -1192,6 +1198,7 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
                     F->q_v.assign(d->flux_values, d->flux_values + d->flux_ntimes);
                 }
                 F->q_order = std::max(std::min(d->flux_order ? d->flux_order : 1, std::max(d->flux_ntimes - 1, 1)), 1);
+                F->q_rating = d->flux_rating != 0;
                 F->q_inc = 1;
                 F->cmvel = d->flux_conveyance > 0.0 ? std::max(std::min(d->flux_conveyance, 1.0), 0.5) : F4(0.667);
                 const double pi = std::acos(-1.0), deg2rad = pi / 180.0;
@@ -1305,6 +1312,15 @@ static int create_impl(const cms2d_flow_desc *d, cms2d_flow *F) {
         F->Kin.red.pr[0] = bi; F->Kin.red.pv[0] = bi + 2 * nb; F->Kin.red.pt[0] = bi + 3 * nb;
     }
     if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
+    if (F->q_rating) {
+        // the rating table beside the step's inflow
+        const int m = (int)F->q_t.size();
+        if (int rc = dalloc(F, &F->q_dev, 1 + 2 * (size_t)m)) return rc;
+        std::vector<double> tab(1 + 2 * (size_t)m, 0.0);
+        for (int i = 0; i < m; i++) { tab[1 + i] = F->q_t[i]; tab[1 + m + i] = F->q_v[i]; }
+        HIPCHK(hipMemcpyAsync(F->q_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, F->st));
+        HIPCHK(hipStreamSynchronize(F->st));
+    }
     if (int rc = dalloc(F, &F->bal_part, BAL_PART * (size_t)F->nblk_kry)) return rc;
     if (int rc = dalloc(F, &F->bal, 1)) return rc;
     F->state.push_back({F->bal, sizeof(BalAcc)});

@@ -1349,7 +1349,7 @@ __global__ void k_bndflux(Geo g, St s) {
 // faces' fluxes scaled to the string's inflow qflux.  One workgroup; the
 // string's terms in LDS (n doubles).
 __global__ void __launch_bounds__(1024) k_river(Geo g, St s, int k, double qflux, double cmvel, double qcos,
-                                                double qsin) {
+                                                double qsin, const double *qdev) {
     extern __shared__ double term[];
     __shared__ double qnorm;
     const int n = (k == KN || k == KS) ? g.nx : g.ny;
@@ -1381,7 +1381,8 @@ __global__ void __launch_bounds__(1024) k_river(Geo g, St s, int k, double qflux
             for (int q = 0; q < U; q++) sumw = sumw + t[q];
         }
         for (; m < n; m++) sumw = sumw + term[m];
-        qnorm = qflux / fmax(fabs(sumw), F4(1.0e-6));
+        const double q = qdev ? qdev[0] : qflux;      // a rating curve's inflow of this step
+        qnorm = q / fmax(fabs(sumw), F4(1.0e-6));
     }
     __syncthreads();
     const double qn = qnorm;
@@ -1407,6 +1408,43 @@ __global__ void k_zbk(Geo g, St s) {
     s.zbkE[c] = g.fintp[KE] * s.zb[c + 1] + (1.0 - g.fintp[KE]) * s.zb[c];
     if (j == 0) s.zbkN[c - g.nxp] = g.fintp[KS] * s.zb[c - g.nxp] + (1.0 - g.fintp[KS]) * s.zb[c];
     if (i == 0) s.zbkE[c - 1] = g.fintp[KW] * s.zb[c - 1] + (1.0 - g.fintp[KW]) * s.zb[c];
+}
+
+// Stage-flow (rating) curve of flux side k (bnd_eval, boundaries.F90:4084-4094):
+// the mean depth of the string's cells (summed in string order), then
+// plagr_fit of order 1 from a bisection search over the m stages (constant
+// beyond the table), times the ramp; q[0] <- the step's inflow, the table in
+// q[1..m] (stages) and q[1+m..2m] (flows).
+__global__ void k_river_rating(Geo g, St s, int k, int m, double ramp, double *q) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n = (k == KN || k == KS) ? g.nx : g.ny;
+    double stage = 0.0;
+    for (int t = 0; t < n; t++) {
+        const int i = k == KE ? g.nx - 1 : k == KW ? 0 : t;
+        const int j = k == KN ? g.ny - 1 : k == KS ? 0 : t;
+        stage = stage + s.h[cidx(g, i, j)];
+    }
+    stage = stage / (double)n;
+    const double *x = q + 1, *v = q + 1 + m;
+    double w;
+    if (stage < x[0]) w = v[0];
+    else if (stage > x[m - 1]) w = v[m - 1];
+    else {
+        int i = 1, jj = m;
+        while (jj > i + 1) {
+            const int kk = (i + jj) / 2;
+            if (stage < x[kk - 1]) jj = kk;
+            else i = kk;
+        }
+        double lb0 = 1.0, lb1 = 1.0;
+        lb0 = lb0 * (stage - x[i]) / (x[i - 1] - x[i]);
+        lb1 = lb1 * (stage - x[i - 1]) / (x[i] - x[i - 1]);
+        const double sum = (0.0 + lb0) + lb1;
+        w = 0.0;
+        w += lb0 / sum * v[i - 1];
+        w += lb1 / sum * v[i];
+    }
+    q[0] = ramp * w;
 }
 
 __global__ void k_set_int(int *p, int n, int v) {

@@ -61,6 +61,7 @@ class FlowStep:
             d.flux_q, d.flux_order, d.flux_conveyance = case.q_const, case.q_order, case.q_cmvel
             if getattr(case, "q_angle", None) is not None:
                 d.flux_angle_on, d.flux_angle = 1, case.q_angle
+            d.flux_rating = int(getattr(case, "q_rating", False))
             if case.q_times is not None:
                 d.flux_times, d.flux_values = arr(case.q_times), arr(case.q_values)
                 d.flux_ntimes = len(case.q_times)

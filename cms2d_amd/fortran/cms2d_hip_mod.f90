@@ -57,6 +57,7 @@ module cms2d_hip
         real(c_double) :: flux_conveyance = 0
         integer(c_int) :: flux_angle_on = 0                            ! INFLOW_ANGLE given
         real(c_double) :: flux_angle = 0                               ! [deg clockwise from north]
+        integer(c_int) :: flux_rating = 0                              ! RATING_CURVE: stage [m] -> flow
     end type
 
     ! struct cms2d_step_info

@@ -159,6 +159,9 @@ typedef struct {
     int flux_angle_on;            /* 1: INFLOW_ANGLE flux_angle [deg clockwise from north, grid
                                      angle 0] instead of the inward normal                    */
     double flux_angle;
+    int flux_rating;              /* 1: flux_times/flux_values are a RATING_CURVE: the inflow follows
+                                     the mean depth of the string's cells [m] (stage) by linear
+                                     interpolation, constant beyond the table (:4084-4093)    */
 } cms2d_flow_desc;
 
 typedef struct {

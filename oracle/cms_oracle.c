@@ -886,7 +886,15 @@ static void bnd_eval(cms_oracle *o)
      * ntimes - 1, :1783), times the ramp */
     if (o->nQ > 0) {
         double q = o->c.q_const;
-        if (o->c.q_ntimes > 0) {
+        if (o->c.q_rating) {
+            /* stage-flow curve (:4084-4093): the mean depth of the string's cells,
+             * first-order interpolation from a bisection search */
+            double stage = 0.0;
+            for (int j = 0; j < o->nQ; j++) stage = stage + o->h[o->Qc[j]];
+            stage = stage / (double)o->nQ;
+            int k = 0;
+            q = cms_oracle_plagr(o->c.q_ntimes, o->c.q_times, o->c.q_values, 1, stage, &k);
+        } else if (o->c.q_ntimes > 0) {
             int ni = o->c.q_order ? o->c.q_order : 1;
             if (ni > o->c.q_ntimes - 1) ni = o->c.q_ntimes - 1;
             if (ni < 1) ni = 1;

@@ -89,6 +89,7 @@ typedef struct {
     double q_cmvel;
     int q_angle_on;           /* INFLOW_ANGLE q_angle [deg clockwise from north] instead of the normal */
     double q_angle;
+    int q_rating;             /* q_times/q_values are a stage-flow curve (mean depth of the string) */
 } cms_case;
 
 typedef struct {

@@ -34,7 +34,7 @@ class CmsCase(C.Structure):
         ("noskew", C.c_int),
         ("q_side", C.c_int), ("q_const", C.c_double), ("q_times", D), ("q_values", D),
         ("q_ntimes", C.c_int), ("q_order", C.c_int), ("q_cmvel", C.c_double),
-        ("q_angle_on", C.c_int), ("q_angle", C.c_double),
+        ("q_angle_on", C.c_int), ("q_angle", C.c_double), ("q_rating", C.c_int),
     ]
 
 
@@ -154,6 +154,7 @@ class Oracle:
             c.q_cmvel = case.q_cmvel
             if getattr(case, "q_angle", None) is not None:
                 c.q_angle_on, c.q_angle = 1, case.q_angle
+            c.q_rating = int(getattr(case, "q_rating", False))
         self._c = c
         self.h = L.cms_oracle_create(C.byref(c))
         self.ncells = L.cms_oracle_ncells(self.h)

@@ -116,7 +116,8 @@ def write_case(case, d, name="case", solver="BICGSTAB", balance=False, stats=Non
             with open(os.path.join(d, f"{name}_q.xys"), "w") as f:
                 f.write(f"XYS 1 {len(case.q_times)}\n")
                 f.write("".join(f"{t:.9g} {v:.9g}\n" for t, v in zip(case.q_times, case.q_values)))
-            qcard = f'    FLUX_CURVE "{name}_q.xys"\n    FLUX_UNITS \'m^3/s/boundary\'\n'
+            kind = "RATING_CURVE" if getattr(case, "q_rating", False) else "FLUX_CURVE"
+            qcard = f'    {kind} "{name}_q.xys"\n    FLUX_UNITS \'m^3/s/boundary\'\n'
             if case.q_order:
                 qcard += f"    TEMPORAL_INTERPOLATION_ORDER {case.q_order}\n"
         else:

@@ -206,6 +206,7 @@ RIVER = {
     "river_shoal": dict(shoal=True, q=25.0),
     "river_conveyance": dict(q_cmvel=0.8, q_side=2),
     "river_angle": dict(q_angle=75.0),           # INFLOW_ANGLE: 15 degrees off the normal
+    "river_rating": dict(q_rating=([1.0, 2.0, 2.5, 4.0], [10.0, 30.0, 45.0, 80.0])),   # RATING_CURVE
 }
 for _name, _kw in RIVER.items():
     CASES[_name] = (lambda kw=_kw: cases.river(**kw))
