@@ -221,7 +221,8 @@ __global__ void k_tl_grad(TelG T, const uint8_t *iwet, int ibc, const double *v0
 }
 
 // ------------------------------------------------------------ wet and dry
-// flow_wetdry(0) first part (flow.F90:1250-1275) on every cell; wall ghosts dry
+// flow_wetdry(0) first part (flow.F90:1250-1275) on every cell; wall ghosts dry,
+// the water-level string's ghosts take their cell's flag (:1606-1613)
 __global__ void k_tl_wetdry_h(TelG T, TelS s, Par P) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T.nd) return;
@@ -229,7 +230,10 @@ __global__ void k_tl_wetdry_h(TelG T, TelS s, Par P) {
     uint8_t w;
     if (h < P.hdry) { h = fmax(h, P.hmin); w = 0; }
     else w = 1;
-    if (t >= T.n && T.gwall[t - T.n]) w = 0;
+    if (t >= T.n) {
+        const int c = T.gown[t - T.n];
+        w = T.gwall[t - T.n] ? 0 : (s.p[c] * GRAVINV - s.zb[c] < P.hdry ? 0 : 1);
+    }
     s.h[t] = h;
     s.iwet[t] = w;
 }
