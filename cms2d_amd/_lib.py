@@ -65,7 +65,8 @@ class FlowDesc(C.Structure):
 
 
 class TelGrid(C.Structure):
-    _fields_ = [("ncells", C.c_int), ("geom", D), ("nb", I), ("bnd_cells", I), ("nbnd", C.c_int)]
+    _fields_ = [("ncells", C.c_int), ("geom", D), ("nb", I), ("bnd_cells", I), ("nbnd", C.c_int),
+                ("flux_cells", I), ("nflux", C.c_int)]
 
 
 class StepInfo(C.Structure):

@@ -187,6 +187,40 @@ def river(nx=48, ny=32, dx=20.0, seed=4, nsteps=40, dt=30.0, q_side=3, q=40.0, q
     return c
 
 
+def river_tel(nx=40, ny=24, dx=20.0, seed=4, q=40.0, **kw) -> Case:
+    """The estuary of ``river`` on a telescoping grid: a band of 10 m cells with
+    a 5 m core near the river mouth (base columns 2-10, 4-8), the tide's
+    string on the east edge, the flux string on the west edge (its cells
+    south to north); kw are river's options of the inflow (q_curve, q_rating,
+    q_angle, q_cmvel)."""
+    levels = band_levels(nx, ny, (2, 10), (4, 8))
+    tel = telescope(levels, dx, side=1)
+    rng = np.random.default_rng(seed)
+    x, y = tel.geom[:, 0], tel.geom[:, 1]
+    xn = x / (nx * dx)
+    depth = 2.0 + 10.0 * xn
+    xi, yj = x / dx, y / dx
+    for _ in range(20):
+        ci, cj = rng.uniform(0, nx), rng.uniform(0, ny)
+        r = rng.uniform(max(2, nx / 40), max(3, nx / 12))
+        depth = depth + rng.normal(0.0, 0.15) * np.exp(-((xi - ci) ** 2 + (yj - cj) ** 2) / (2 * r * r))
+    zb = f32(-depth)
+    mann = f32(0.025 * np.exp(0.2 * rng.standard_normal(len(x))))
+    c = Case("river_tel", nx, ny, dx, dx, zb, mann, None, None, 1, [0.5], [M2_SPEED_DEG_HR], [90.0], 30.0, 40)
+    c.tel = tel
+    west = np.nonzero(np.isclose(x - tel.geom[:, 2] / 2, 0.0))[0]
+    tel.flux_cells = west[np.argsort(y[west])]
+    c.q_side, c.q_const = 3, q
+    for k, v in kw.items():
+        if k == "q_curve":
+            c.q_times, c.q_values = list(v[0]), list(v[1])
+        elif k == "q_rating":
+            c.q_times, c.q_values, c.q_rating = list(v[0]), list(v[1]), True
+        else:
+            setattr(c, k, v)
+    return c
+
+
 def square(nx=1000, ny=1000, dx=20.0, seed=3, nsteps=720, dt=60.0) -> Case:
     c = tidal(nx, ny, dx, seed=seed, nsteps=nsteps, dt=dt, radstress=False)
     c.name = "square"

@@ -176,6 +176,8 @@ __global__ void __launch_bounds__(KBLOCK) k_tl_bal_fin(TelG T, TelS s, Par P, co
     double bnd = 0.0;
     for (int j = threadIdx.x; j < T.nstr; j += KBLOCK)
         bnd += P.dtime * s.flux[(size_t)T.str_k[j] * T.nd + T.str_c[j]];
+    for (int j = threadIdx.x; j < T.nq; j += KBLOCK)
+        bnd += P.dtime * s.flux[(size_t)T.q_k[j] * T.nd + T.q_c[j]];
     bal_finish(part, nb, bnd, acc);
 }
 

@@ -354,7 +354,8 @@ static int tel_wetdry(cms2d_flow *F) {
                        H.tiles);
     TLAUNCH(k_tl_pond_merge, H.nblk_n, T, s, H.tiles);
     TLAUNCH(k_tl_pond_compress, H.nblk_nd, T, s);
-    TLAUNCH(k_tl_pond_mark, (T.nstr + BLOCK - 1) / BLOCK, T, s);
+    TLAUNCH(k_tl_pond_mark, (T.nstr + BLOCK - 1) / BLOCK, T, s, T.str_c, T.str_k, T.nstr);
+    if (T.nq) TLAUNCH(k_tl_pond_mark, (T.nq + BLOCK - 1) / BLOCK, T, s, T.q_c, T.q_k, T.nq);
     TLAUNCH(k_tl_pond_final, H.nblk_nd, T, s);
     TLAUNCH(k_set_int, 1, s.flags, 3, 0);
     TLAUNCH(k_tl_wetdry_post, H.nblk_n, T, s);
@@ -560,6 +561,14 @@ static int tel_pp_solve_refine(cms2d_flow *F, KryCtx &X, const TelOp &A) {
     return 0;
 }
 
+// bndriver_vel on the flux string, if any
+static void tel_river(cms2d_flow *F) {
+    TelHost &H = *F->tel;
+    if (F->qside < 0 || H.T.nq == 0) return;
+    hipLaunchKernelGGL(k_tl_river, dim3(1), dim3(1024), (size_t)H.T.nq * sizeof(double), F->st, H.T, H.s, F->qflux,
+                       F->cmvel, F->qcos, F->qsin, (const double *)(F->q_rating ? F->q_dev : nullptr));
+}
+
 static int tel_flow_imp(cms2d_flow *F) {
     TelHost &H = *F->tel;
     const TelG &T = H.T;
@@ -568,6 +577,13 @@ static int tel_flow_imp(cms2d_flow *F) {
     double w = 0.0;
     for (int k = 0; k < F->ntc; k++) w += F->amp[k] * std::cos(F->speed[k] * timehrs - F->phase[k]);
     if (!F->wse_t.empty()) w = wse_curve_eval(F->wse_t, F->wse_v, F->wse_order, timehrs, F->wse_inc);
+    // the flux string's inflow (bnd_eval, boundaries.F90:4078-4094), before flow_wetdry
+    if (F->qside >= 0 && F->q_rating) {
+        TLAUNCH(k_tl_river_rating, 1, T, s, (int)F->q_t.size(), F->ramp, F->q_dev);
+    } else if (F->qside >= 0) {
+        const double q = F->q_t.empty() ? F->q_const : wse_curve_eval(F->q_t, F->q_v, F->q_order, timehrs, F->q_inc);
+        F->qflux = F->ramp * q;
+    }
     if (int rc = tel_wetdry(F)) return rc;
     F->P.tauwx = F->P.tauwy = 0.0;
     if (F->windconst) {
@@ -580,6 +596,7 @@ static int tel_flow_imp(cms2d_flow *F) {
     const TelOp Am = tel_op(H, s.am), Ap = tel_op(H, s.apc);
     for (int n = 0; n < 3; n++) F->kry[n] = 0;
     for (F->niter = 1; F->niter <= F->maxit; F->niter++) {
+        tel_river(F);
         TLAUNCH(k_tl_mom, H.nblk_n, T, s, F->P);
         if (H.njoint && F->P.ndsch != 0)
             TLAUNCH(k_tl_defcor_skew, (H.njoint + BLOCK - 1) / BLOCK, T, s, (const double *)H.gu[0][H.cu],
@@ -637,8 +654,9 @@ static int tel_flow_imp(cms2d_flow *F) {
         H.cpp ^= 1;
         TLAUNCH(k_tl_pvfc1, H.nblk_n, T, s, (const double *)H.gpp[0][H.cpp], (const double *)H.gpp[1][H.cpp]);
         TLAUNCH(k_tl_pvfc2, H.nblk_nd, T, s, F->P);
-        if (T.ng) TLAUNCH(k_tl_ghost_p, (T.ng + BLOCK - 1) / BLOCK, T, s);
+        if (T.ng) TLAUNCH(k_tl_ghost_p, (T.ng + BLOCK - 1) / BLOCK, T, s, F->P);
         if (int rc = tel_grad_fric_vis(F)) return rc;
+        tel_river(F);       // bndriver_vel after flow_grad_interp (flow_imp.F90:236-237)
         for (int n = 0; n < 3; n++) {
             F->rmom[n] = F->h_sc[n].rmom;
             F->kry[n] += F->h_sc[n].final_iters;
@@ -918,13 +936,29 @@ static int tel_check(const cms2d_flow_desc *d, const cms2d_tel_grid *tg) {
     if (d->precond != CMS2D_PRECOND_MG && d->precond != CMS2D_PRECOND_JACOBI)
         return set_err(CMS2D_ERR_ARG, "unknown preconditioner");
     if (d->serial_order) return set_err(CMS2D_ERR_ARG, "serial_order is built for Cartesian grids only");
-    int nopen = 0, curve = 0;
+    int nopen = 0, curve = 0, nflux = 0;
     for (int k = 0; k < 4; k++) {
         if (d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_WSE_HARMONIC &&
-            d->side_type[k] != CMS2D_BND_WSE_CURVE)
+            d->side_type[k] != CMS2D_BND_WSE_CURVE && d->side_type[k] != CMS2D_BND_FLUX)
             return set_err(CMS2D_ERR_ARG, "unknown boundary type");
-        nopen += d->side_type[k] != CMS2D_BND_WALL;
+        nflux += d->side_type[k] == CMS2D_BND_FLUX;
+        nopen += d->side_type[k] != CMS2D_BND_WALL && d->side_type[k] != CMS2D_BND_FLUX;
         curve += d->side_type[k] == CMS2D_BND_WSE_CURVE;
+    }
+    if (nflux > 1) return set_err(CMS2D_ERR_ARG, "one flux side at most");
+    if ((nflux == 1) != (tg->nflux > 0) || (tg->nflux > 0 && !tg->flux_cells))
+        return set_err(CMS2D_ERR_ARG, "the flux side needs its cellstring");
+    if (tg->nflux == 1 || tg->nflux > 8192) return set_err(CMS2D_ERR_ARG, "the flux cellstring needs 2..8192 cells");
+    if (nflux) {
+        if (d->friction != CMS2D_FRIC_MANNING) return set_err(CMS2D_ERR_ARG, "a flux side needs Manning friction");
+        if (d->flux_ntimes == 1 || d->flux_ntimes < 0 || d->flux_order < 0 || d->flux_order > 3 ||
+            !(d->flux_conveyance >= 0.0) || !std::isfinite(d->flux_conveyance) ||
+            (d->flux_ntimes == 0 && !std::isfinite(d->flux_q)) || (d->flux_ntimes > 0 && (!d->flux_times || !d->flux_values)))
+            return set_err(CMS2D_ERR_ARG, "flux curve or conveyance invalid");
+        for (int i = 0; i < d->flux_ntimes; i++)
+            if (!std::isfinite(d->flux_times[i]) || !std::isfinite(d->flux_values[i]) ||
+                (i > 0 && !(d->flux_times[i] > d->flux_times[i - 1])))
+                return set_err(CMS2D_ERR_ARG, "flux curve times must increase, values be finite");
     }
     if (nopen > 1) return set_err(CMS2D_ERR_ARG, "one water-level side at most (the others are walls)");
     if ((nopen == 1) != (tg->nbnd > 0)) return set_err(CMS2D_ERR_ARG, "the water-level side needs its cellstring");
@@ -949,34 +983,69 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     TelTopo t;
     if (int rc = tel_topology(tg, t)) return rc;
     const int n = t.n, nd = t.nd, ng = nd - n;
-    // the water-level string: cellstr_bnd (bnd_lib.f90:215-316)
-    std::vector<int> str_c, str_k;
-    for (int j = 0; j < tg->nbnd; j++) {
-        const int id = tg->bnd_cells[j];
-        if (id < 0 || id >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
-        const int ref = j == 0 ? tg->bnd_cells[1] : j == tg->nbnd - 1 ? tg->bnd_cells[tg->nbnd - 2] : id;
-        if (ref < 0 || ref >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
-        for (int k = 0; k < t.ncf[ref]; k++) {
-            if (t.c2c[(size_t)k * nd + ref] < n) continue;
-            int kf = k;
-            if (ref != id) {
-                kf = -1;
-                for (int kk = 0; kk < t.ncf[id]; kk++)
-                    if (t.dir[(size_t)kk * nd + id] == t.dir[(size_t)k * nd + ref]) { kf = kk; break; }
+    // the water-level and flux strings: cellstr_bnd (bnd_lib.f90:215-316)
+    auto string_faces = [&](const int *cells, int ncs, std::vector<int> &sc, std::vector<int> &sk) -> int {
+        for (int j = 0; j < ncs; j++) {
+            const int id = cells[j];
+            if (id < 0 || id >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
+            const int ref = j == 0 ? cells[1] : j == ncs - 1 ? cells[ncs - 2] : id;
+            if (ref < 0 || ref >= n) return set_err(CMS2D_ERR_ARG, "cellstring cell out of range");
+            for (int k = 0; k < t.ncf[ref]; k++) {
+                if (t.c2c[(size_t)k * nd + ref] < n) continue;
+                int kf = k;
+                if (ref != id) {
+                    kf = -1;
+                    for (int kk = 0; kk < t.ncf[id]; kk++)
+                        if (t.dir[(size_t)kk * nd + id] == t.dir[(size_t)k * nd + ref]) { kf = kk; break; }
+                }
+                if (kf < 0 || t.c2c[(size_t)kf * nd + id] < n)
+                    return set_err(CMS2D_ERR_ARG, "cellstring cell without an edge face in the string's direction");
+                sc.push_back(id);
+                sk.push_back(kf);
             }
-            if (kf < 0 || t.c2c[(size_t)kf * nd + id] < n)
-                return set_err(CMS2D_ERR_ARG, "cellstring cell without an edge face in the string's direction");
-            str_c.push_back(id);
-            str_k.push_back(kf);
         }
-    }
-    if (str_c.size() > 8192) return set_err(CMS2D_ERR_ARG, "the cellstring has more than 8192 faces");
-    std::vector<uint8_t> gwall(ng, 1), strmask(n, 0);
+        if (sc.size() > 8192) return set_err(CMS2D_ERR_ARG, "the cellstring has more than 8192 faces");
+        return 0;
+    };
+    std::vector<int> str_c, str_k, q_c, q_k;
+    if (int rc = string_faces(tg->bnd_cells, tg->nbnd, str_c, str_k)) return rc;
+    if (int rc = string_faces(tg->flux_cells, tg->nflux, q_c, q_k)) return rc;
+    std::vector<uint8_t> gwall(ng, 1), strmask(n, 0), qmask(n, 0);
     for (size_t j = 0; j < str_c.size(); j++) {
         const int gc = t.c2c[(size_t)str_k[j] * nd + str_c[j]];
         if (!gwall[gc - n]) return set_err(CMS2D_ERR_ARG, "a cellstring face listed twice");
         gwall[gc - n] = 0;
         strmask[str_c[j]] |= (uint8_t)(1u << str_k[j]);
+    }
+    for (size_t j = 0; j < q_c.size(); j++) {
+        const int gc = t.c2c[(size_t)q_k[j] * nd + q_c[j]];
+        if (gwall[gc - n] != 1) return set_err(CMS2D_ERR_ARG, "a cellstring face listed twice");
+        gwall[gc - n] = 2;
+        qmask[q_c[j]] |= (uint8_t)(1u << q_k[j]);
+    }
+    for (int k = 0; k < 4; k++) {
+        if (d->side_type[k] != CMS2D_BND_FLUX) continue;
+        // Q_str as on Cartesian grids: inflow along the side's inward normal or
+        // INFLOW_ANGLE, conveyance, curve or rating table
+        F->qside = k;
+        F->q_const = d->flux_q;
+        if (d->flux_ntimes > 0) {
+            F->q_t.assign(d->flux_times, d->flux_times + d->flux_ntimes);
+            F->q_v.assign(d->flux_values, d->flux_values + d->flux_ntimes);
+        }
+        F->q_order = std::max(std::min(d->flux_order ? d->flux_order : 1, std::max(d->flux_ntimes - 1, 1)), 1);
+        F->q_rating = d->flux_rating != 0;
+        F->q_inc = 1;
+        F->cmvel = d->flux_conveyance > 0.0 ? std::max(std::min(d->flux_conveyance, 1.0), 0.5) : F4(0.667);
+        const double pi = std::acos(-1.0), deg2rad = pi / 180.0;
+        double angle = (double)((float)(4 - (k + 1)) * 90.0f);
+        if (d->flux_angle_on) {
+            angle = 90.0 - d->flux_angle - 0.0;
+            if (angle < 0.0) angle += 360.0;
+            else if (angle > 360.0) angle -= 360.0;
+        }
+        F->qcos = std::cos(angle * deg2rad);
+        F->qsin = std::sin(angle * deg2rad);
     }
     // base cells
     std::vector<int> base(n), cnt((size_t)d->nx * d->ny + 1, 0);
@@ -1024,6 +1093,7 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         }
         if (near) m |= 1ull << 41;
         m |= (uint64_t)strmask[i] << 42;
+        m |= (uint64_t)qmask[i] << 48;
         meta[i] = m;
     }
 
@@ -1040,12 +1110,13 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     H.njoint = T.njoint;
     T.skewcor = T.njoint > 0;
     T.nstr = (int)str_c.size();
+    T.nq = (int)q_c.size();
     {
         double asum = 0.0;
         for (int i = 0; i < n; i++) asum += t.dx[i] * t.dy[i];
         T.areaavg = asum / (double)(float)n;
     }
-    int *c2c, *gown, *gk, *bas, *bst, *jnt, *sc_, *sk_;
+    int *c2c, *gown, *gk, *bas, *bst, *jnt, *sc_, *sk_, *qc_ = nullptr, *qk_ = nullptr;
     uint64_t *met;
     double *xx, *yy, *ddx, *ddy;
     uint8_t *gw;
@@ -1156,6 +1227,11 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
     T.c2c = c2c; T.meta = met; T.x = xx; T.y = yy; T.dx = ddx; T.dy = ddy;
     T.gown = gown; T.gk = gk; T.gwall = gw; T.base = bas; T.bstart = bst; T.joint = jnt;
     T.str_c = sc_; T.str_k = sk_;
+    if (!q_c.empty()) {
+        if (int rc = tel_upload(F, &qc_, q_c.data(), q_c.size())) return rc;
+        if (int rc = tel_upload(F, &qk_, q_k.data(), q_k.size())) return rc;
+    }
+    T.q_c = qc_; T.q_k = qk_;
     H.gown_h = t.gown;
     H.nblk_n = (n + BLOCK - 1) / BLOCK;
     H.nblk_nd = (nd + BLOCK - 1) / BLOCK;
@@ -1236,6 +1312,14 @@ static int tel_create(const cms2d_flow_desc *d, const cms2d_tel_grid *tg, cms2d_
         }
         if (int rc = dalloc(F, &F->d_sc, 6)) return rc;
         if (int rc = dalloc(F, &F->bal_part, BAL_PART * nb)) return rc;
+        if (F->q_rating) {
+            const int m = (int)F->q_t.size();
+            if (int rc = dalloc(F, &F->q_dev, 1 + 2 * (size_t)m)) return rc;
+            std::vector<double> tab(1 + 2 * (size_t)m, 0.0);
+            for (int i = 0; i < m; i++) { tab[1 + i] = F->q_t[i]; tab[1 + m + i] = F->q_v[i]; }
+            HIPCHK(hipMemcpyAsync(F->q_dev, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, F->st));
+            HIPCHK(hipStreamSynchronize(F->st));
+        }
         if (int rc = dalloc(F, &F->bal, 1)) return rc;
         double *kv[3][7];
         for (int q = 0; q < 3; q++)

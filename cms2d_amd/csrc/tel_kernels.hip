@@ -31,6 +31,7 @@ __device__ __forceinline__ bool m_joint(uint64_t m) { return (m >> 39) & 1u; }
 __device__ __forceinline__ bool m_smooth(uint64_t m) { return (m >> 40) & 1u; }
 __device__ __forceinline__ bool m_near(uint64_t m) { return (m >> 41) & 1u; }
 __device__ __forceinline__ bool m_str(uint64_t m, int k) { return (m >> (42 + k)) & 1u; }
+__device__ __forceinline__ bool m_qstr(uint64_t m, int k) { return (m >> (48 + k)) & 1u; }   // flux string
 
 // Topology and geometry (device pointers).
 struct TelG {
@@ -39,7 +40,7 @@ struct TelG {
     const uint64_t *meta;        // [n]
     const double *x, *y, *dx, *dy;   // [nd]
     const int *gown, *gk;        // [ng]: owner cell and its face of ghost n+g
-    const uint8_t *gwall;        // [ng]: 1 wall ghost, 0 water-level ghost
+    const uint8_t *gwall;        // [ng]: 1 wall ghost, 0 water-level ghost, 2 flux-string ghost
     const int *base;             // [n]: base cell j*nbx+i
     const int *bstart;           // [nbx*nby+1]: cells of base cell b
     int nbx, nby;
@@ -49,6 +50,8 @@ struct TelG {
     int skewcor;                 // skewness corrections (and the gradient corrections) on
     const int *str_c, *str_k;    // water-level string, string order
     int nstr;
+    const int *q_c, *q_k;        // flux string, string order
+    int nq;
 };
 __device__ __forceinline__ int tnb(const TelG &T, int i, int k) { return T.c2c[k * T.nd + i]; }
 
@@ -232,7 +235,7 @@ __global__ void k_tl_wetdry_h(TelG T, TelS s, Par P) {
     else w = 1;
     if (t >= T.n) {
         const int c = T.gown[t - T.n];
-        w = T.gwall[t - T.n] ? 0 : (s.p[c] * GRAVINV - s.zb[c] < P.hdry ? 0 : 1);
+        w = T.gwall[t - T.n] == 1 ? 0 : (s.p[c] * GRAVINV - s.zb[c] < P.hdry ? 0 : 1);
     }
     s.h[t] = h;
     s.iwet[t] = w;
@@ -367,10 +370,10 @@ __global__ void k_tl_pond_compress(TelG T, TelS s) {
     if (t >= T.nd) return;
     if (s.plab[t] >= 0) s.plab[t] = tl_find(s.plab, t);
 }
-__global__ void k_tl_pond_mark(TelG T, TelS s) {
+__global__ void k_tl_pond_mark(TelG T, TelS s, const int *sc, const int *sk, int ns) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= T.nstr) return;
-    const int i = T.str_c[j], nb = tnb(T, i, T.str_k[j]);
+    if (j >= ns) return;
+    const int i = sc[j], nb = tnb(T, i, sk[j]);
     if (s.iwet[i]) s.ipond[s.plab[i]] = 1;
     if (s.iwet[nb]) s.ipond[s.plab[nb]] = 1;
 }
@@ -531,6 +534,17 @@ __global__ void k_tl_bound_uv(TelG T, TelS s, Par P) {
     const uint64_t m = T.meta[i];
     const int nf = m_ncf(m);
     double sp = s.sp[i];
+    // flux string first (bound_uv's order): bnd_vel_flux (bnd_vel.f90:2-32)
+    for (int k = 0; k < nf; k++) {
+        if (!m_qstr(m, k)) continue;
+        const int f = k * T.nd + i, nb = tnb(T, i, k);
+        const FGeo g = fgeo(T, i, nb, m_dir(m, k));
+        const double a = schmcoef(1, s.visk[f] * s.hk[f] * g.dsxy, s.flux[f]);
+        s.su[i] = s.su[i] + a * s.u[nb];
+        s.sv[i] = s.sv[i] + a * s.v[nb];
+        sp = sp - a;
+        s.am[f] = 0.0;
+    }
     for (int k = 0; k < nf; k++) {
         if (!m_str(m, k)) continue;
         const int f = k * T.nd + i, nb = tnb(T, i, k);
@@ -740,6 +754,8 @@ __global__ void k_tl_pp_coef_flux(TelG T, TelS s, const double *dHuy, const doub
                 fl = hk * g.ds * ((g.fnx * Huf + g.fny * Hvf) - apf0 * hk * (s.p[nb] - s.p[i]) / g.dn);
             }
         }
+        if (m_qstr(m, k))      // bndflux (flow_imp.F90:697): the flux string's faces from their ghosts
+            fl = (double)s.iwet[i] * g.ds * hk * (g.fnx * s.u[nb] + g.fny * s.v[nb]);
         s.apc[f] = a;
         s.flux[f] = fl;
         if (b >= 0) { s.apc[b] = a; s.flux[b] = -fl; }
@@ -820,6 +836,9 @@ __global__ void __launch_bounds__(KBLOCK) k_tl_pp_src(TelG T, TelS s, Par P, Kry
             sp = 0.0;
         }
         bool changed = wet == 0;
+#pragma unroll
+        for (int k = 0; k < TF; k++)      // flux string: closed faces (boundaries.F90:3544-3551)
+            if (k < nf && m_qstr(m, k)) { a[k] = 0.0; changed = true; }
 #pragma unroll
         for (int k = 0; k < TF; k++) {
             if (k >= nf || !m_str(m, k)) continue;
@@ -955,10 +974,11 @@ __global__ void k_tl_pvfc2(TelG T, TelS s, Par P) {
     s.pp[t] = FACPP * pp;
 }
 // wall ghosts take the pressure of their cell (flow_imp.F90:972-980)
-__global__ void k_tl_ghost_p(TelG T, TelS s) {
+__global__ void k_tl_ghost_p(TelG T, TelS s, Par P) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= T.ng) return;
     if (T.gwall[g]) s.p[T.n + g] = s.p[T.gown[g]];
+    if (T.gwall[g] == 2) s.h[T.n + g] = fmax(P.hmin, s.p[T.n + g] * GRAVINV - s.zb[T.n + g]);
 }
 
 // -------------------------------------------------- face values, gradients
@@ -1105,7 +1125,7 @@ __global__ void k_tl_fric_vis(TelG T, TelS s, Par P, const double *dux, const do
 __global__ void k_tl_vis_ghosts(TelG T, TelS s) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= T.ng) return;
-    s.visb[T.n + g] = T.gwall[g] ? s.vis[T.n + g] : s.visb[T.gown[g]];
+    s.visb[T.n + g] = T.gwall[g] == 1 ? s.vis[T.n + g] : s.visb[T.gown[g]];
 }
 
 // Eddy-viscosity fix-ups next to dry cells and ghosts (flow_eddyvis.f90:166-186),
@@ -1252,10 +1272,76 @@ __global__ void k_tl_convflux(TelG T, TelS s) {
         const FGeo g = fgeo(T, i, nb, m_dir(m, k));
         const double uni = g.fnx * s.u[i] + g.fny * s.v[i];
         const double unk = g.fnx * s.u[nb] + g.fny * s.v[nb];
-        const double fl = (double)(s.iwet[i] * s.iwet[nb]) * g.ds * s.hk[f] * (g.fintp * unk + (1.0 - g.fintp) * uni);
+        double fl = (double)(s.iwet[i] * s.iwet[nb]) * g.ds * s.hk[f] * (g.fintp * unk + (1.0 - g.fintp) * uni);
+        if (m_qstr(m, k)) fl = (double)s.iwet[i] * g.ds * s.hk[f] * (g.fnx * s.u[nb] + g.fny * s.v[nb]);   // bndflux
         s.flux[f] = fl;
         s.flux[m_llec(m, k) * T.nd + nb] = -fl;
     }
+}
+
+// bndriver_vel (boundaries.F90:4259-4356) on the flux string of a cell list;
+// as k_river: weights on the ghosts, their flux sum in string order in one
+// thread, then scaled to the inflow (qdev[0] for a rating curve).
+__global__ void __launch_bounds__(1024) k_tl_river(TelG T, TelS s, double qflux, double cmvel, double qcos,
+                                                   double qsin, const double *qdev) {
+    extern __shared__ double term[];
+    __shared__ double qnorm;
+    const int n = T.nq;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int c = T.q_c[j], k = T.q_k[j], f = k * T.nd + c, gc = tnb(T, c, k);
+        const FGeo g = fgeo(T, c, gc, m_dir(T.meta[c], k));
+        const double hk = s.hk[f];
+        const double val = (double)(s.iwet[c] * s.iwet[gc]) * pow(hk, cmvel) / fmax(s.coefman[c], F4(1.0e-5));
+        const double ug = val * qcos, vg = val * qsin;
+        s.u[gc] = ug;
+        s.v[gc] = vg;
+        term[j] = g.ds * hk * (ug * g.fnx + vg * g.fny);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sumw = 0.0;
+        for (int j = 0; j < n; j++) sumw = sumw + term[j];
+        qnorm = (qdev ? qdev[0] : qflux) / fmax(fabs(sumw), F4(1.0e-6));
+    }
+    __syncthreads();
+    const double qn = qnorm;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int c = T.q_c[j], k = T.q_k[j], f = k * T.nd + c, gc = tnb(T, c, k);
+        const FGeo g = fgeo(T, c, gc, m_dir(T.meta[c], k));
+        const double ug = s.u[gc] * qn, vg = s.v[gc] * qn;
+        s.u[gc] = ug;
+        s.v[gc] = vg;
+        const double fl = (double)(s.iwet[c] * s.iwet[gc]) * g.ds * s.hk[f] * (ug * g.fnx + vg * g.fny);
+        s.flux[f] = fl;
+        s.flux[m_llec(T.meta[c], k) * T.nd + gc] = -fl;
+    }
+}
+// the rating curve's inflow from the flux string's mean depth (as k_river_rating)
+__global__ void k_tl_river_rating(TelG T, TelS s, int m, double ramp, double *q) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double stage = 0.0;
+    for (int j = 0; j < T.nq; j++) stage = stage + s.h[T.q_c[j]];
+    stage = stage / (double)T.nq;
+    const double *x = q + 1, *v = q + 1 + m;
+    double w;
+    if (stage < x[0]) w = v[0];
+    else if (stage > x[m - 1]) w = v[m - 1];
+    else {
+        int i = 1, jj = m;
+        while (jj > i + 1) {
+            const int kk = (i + jj) / 2;
+            if (stage < x[kk - 1]) jj = kk;
+            else i = kk;
+        }
+        double lb0 = 1.0, lb1 = 1.0;
+        lb0 = lb0 * (stage - x[i]) / (x[i - 1] - x[i]);
+        lb1 = lb1 * (stage - x[i - 1]) / (x[i] - x[i - 1]);
+        const double sum = (0.0 + lb0) + lb1;
+        w = 0.0;
+        w += lb0 / sum * v[i - 1];
+        w += lb1 / sum * v[i];
+    }
+    q[0] = ramp * w;
 }
 
 __global__ void k_tl_zero(double *p, long n) {

@@ -90,6 +90,10 @@ class FlowStep:
             tg = _lib.TelGrid()
             tg.ncells, tg.geom, tg.nb = len(geom), _dp(geom), nb.ctypes.data_as(I)
             tg.bnd_cells, tg.nbnd = bnd.ctypes.data_as(I), len(bnd)
+            if getattr(case, "q_side", -1) >= 0:
+                qc = np.ascontiguousarray(tel.flux_cells, dtype=np.int32)
+                keep.append(qc)
+                tg.flux_cells, tg.nflux = qc.ctypes.data_as(I), len(qc)
             check(L.cms2d_flow_create_tel(C.byref(d), C.byref(tg), C.byref(h)))
             self.nf = 6
         else:

@@ -86,6 +86,8 @@ module cms2d_hip
         type(c_ptr) :: nb = c_null_ptr          ! integer(c_int) nb(8, ncells): N1 N2 E1 E2 S1 S2 W1 W2
         type(c_ptr) :: bnd_cells = c_null_ptr   ! integer(c_int) bnd_cells(nbnd), 0-based
         integer(c_int) :: nbnd = 0
+        type(c_ptr) :: flux_cells = c_null_ptr  ! integer(c_int) flux_cells(nflux), 0-based
+        integer(c_int) :: nflux = 0
     end type
 
     interface

@@ -190,6 +190,8 @@ typedef struct {
     const int *nb;
     const int *bnd_cells;
     int nbnd;
+    const int *flux_cells;        /* the flux string (CMS2D_BND_FLUX side), 0-based, string order */
+    int nflux;
 } cms2d_tel_grid;
 
 typedef struct cms2d_flow cms2d_flow;
@@ -208,7 +210,8 @@ int cms2d_flow_destroy(cms2d_flow *flow);
  * interp_lib.f90:2853-2873).  desc gives the physics and the base grid
  * (nx, ny, dx, dy: every cell lies in one base cell, which the multigrid
  * preconditioner coarsens); zb, mann and the forcing are per cell of `grid`;
- * one side_type names the water-level string of grid->bnd_cells.  BiCGStab
+ * one side_type names the water-level string of grid->bnd_cells, and a
+ * CMS2D_BND_FLUX side the flux string of grid->flux_cells.  BiCGStab
  * (the FP64 form or CMS2D_PREC_FP32_REFINE) with a fixed time step; the NVD
  * schemes and water-level gradients are built for Cartesian grids only.  Fields of cms2d_flow_get are per cell;
  * cms2d_flow_get_face gives out[6*c+k] in the cell's face order. */

@@ -506,8 +506,14 @@ static void build_bnd(cms_oracle *o)
     /* the flux string (Cartesian sides): inflow along the inward normal,
      * local_angle = (4 - idirface) 90 degrees (boundaries.F90:1750-1762) */
     o->nQ = 0;
-    if (o->c.q_side >= 0 && o->c.tel_n == 0) {
-        ns = side_cells(o, o->c.q_side, &cells);
+    if (o->c.q_side >= 0 && (o->c.tel_n == 0 || o->c.tel_nq > 0)) {
+        if (o->c.tel_n > 0) {
+            ns = o->c.tel_nq;
+            cells = IALLOC(ns);
+            for (int j = 0; j < ns; j++) cells[j] = o->c.tel_qcells[j];
+        } else {
+            ns = side_cells(o, o->c.q_side, &cells);
+        }
         o->nQ = string_faces(o, cells, ns, &o->Qc, &o->Qk);
         free(cells);
         const double pi = acos(-1.0), deg2rad = pi / 180.0;

@@ -90,6 +90,8 @@ typedef struct {
     int q_angle_on;           /* INFLOW_ANGLE q_angle [deg clockwise from north] instead of the normal */
     double q_angle;
     int q_rating;             /* q_times/q_values are a stage-flow curve (mean depth of the string) */
+    const int *tel_qcells;    /* telescoping grid: the flux string's cells (0-based, string order) */
+    int tel_nq;
 } cms_case;
 
 typedef struct {

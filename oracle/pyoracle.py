@@ -35,6 +35,7 @@ class CmsCase(C.Structure):
         ("q_side", C.c_int), ("q_const", C.c_double), ("q_times", D), ("q_values", D),
         ("q_ntimes", C.c_int), ("q_order", C.c_int), ("q_cmvel", C.c_double),
         ("q_angle_on", C.c_int), ("q_angle", C.c_double), ("q_rating", C.c_int),
+        ("tel_qcells", I), ("tel_nq", C.c_int),
     ]
 
 
@@ -146,7 +147,9 @@ class Oracle:
         c.q_side = getattr(case, "q_side", -1)
         if c.q_side >= 0:
             if tel is not None:
-                raise ValueError("flux strings are restated on Cartesian grids only")
+                qc = np.ascontiguousarray(tel.flux_cells, dtype=np.int32)
+                self._keep.append(qc)
+                c.tel_qcells, c.tel_nq = qc.ctypes.data_as(I), len(qc)
             c.q_const = case.q_const
             if case.q_times is not None:
                 c.q_times, c.q_values = arr(case.q_times), arr(case.q_values)
@@ -263,9 +266,11 @@ class VolumeBalance:
         if tel is not None:
             g = np.asarray(tel.geom)
             self.area, self.dx, self.dy = g[:, 2] * g[:, 3], g[:, 2], g[:, 3]
-            # the string's faces: the domain-edge faces of its cells (the
+            # the strings' faces: the domain-edge faces of their cells (the
             # others among them are walls with zero flux)
-            cells = np.asarray(tel.bnd_cells)
+            cells = list(np.asarray(tel.bnd_cells))
+            if getattr(case, "q_side", -1) >= 0:
+                cells += [c for c in np.asarray(tel.flux_cells) if c not in set(cells)]
             edge = tel.edge_faces()
             self.str_faces = [(c, k) for c in cells for k in np.nonzero(edge[c])[0]]
         else:

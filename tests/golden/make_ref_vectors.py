@@ -218,6 +218,10 @@ RIVER = {
 }
 for _nm, _kw in RIVER.items():
     CASES[_nm] = ((lambda kw=_kw: cases.river(**kw)), 40, [1, 10, 20, 40])
+# the flux string on a telescoping grid (constant inflow; a rating curve)
+CASES["tel_river"] = (lambda: cases.river_tel(), 40, [1, 10, 20, 40])
+CASES["tel_river_rating"] = (lambda: cases.river_tel(q_rating=([1.0, 2.0, 2.5, 4.0], [10.0, 30.0, 45.0, 80.0])),
+                             40, [1, 10, 20, 40])
 
 
 # flow_balance and flow_step_stat of every step (WATER_VOLUME_BALANCE ON):

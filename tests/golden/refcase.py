@@ -109,7 +109,7 @@ def write_case(case, d, name="case", solver="BICGSTAB", balance=False, stats=Non
     if qs >= 0:
         # the flux string's cells (full-grid ids, string order) and its FLUX block
         # (flux_block, boundaries.F90:295-423): the total inflow per boundary
-        qids = side_ids(nx, ny, qs)
+        qids = [int(v) + 1 for v in tel.flux_cells] if tel is not None else side_ids(nx, ny, qs)
         with open(os.path.join(d, name + "_q.bid"), "w") as f:
             f.write(f"1\n{len(qids)}\n" + " ".join(str(v) for v in qids) + "\n")
         if case.q_times is not None:

@@ -11,6 +11,7 @@ import pytest
 
 from cms2d_amd import _lib, cases
 from cms2d_amd.flow import FlowStep
+from pyoracle import Oracle
 from test_gpu_flow import compare, run_pair
 from test_reference_vectors import RIVER
 
@@ -89,9 +90,9 @@ def test_river_rejects():
     c.mbedfric = 1                     # drag law: n changes every friction evaluation
     with pytest.raises(RuntimeError, match="Manning"):
         FlowStep(c)
-    t = cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1)
-    t.q_side, t.q_const = 3, 10.0
-    with pytest.raises(RuntimeError, match="boundary type"):
+    t = cases.river_tel()
+    t.tel.flux_cells = t.tel.flux_cells[:1]          # a one-cell string
+    with pytest.raises(RuntimeError, match="2..8192"):
         FlowStep(t)
 
 
@@ -101,3 +102,23 @@ def test_river_adjoint_rejected(gpu_available):
     with pytest.raises(RuntimeError, match="flux sides"):
         f.adjoint(2, [5], obs=np.zeros((2, 1)))
     f.close()
+
+
+@pytest.mark.parametrize("name", ["tel_river", "tel_river_rating"])
+def test_river_telescoping_matches_oracle(gpu_available, name):
+    """The flux string on a telescoping grid (the band near the mouth), constant
+    and by a rating curve, against the oracle on the same cell list."""
+    from test_reference_vectors import CASES
+    c = CASES[name]()
+    print(f"{name}: {c.ncells} cells", flush=True)
+    o = Oracle(c, serial=0)
+    g = FlowStep(c)
+    for _ in range(20):
+        assert o.step(1)[0] == 0
+        g.step(1)
+    w = o.get("iwet")
+    assert np.array_equal(w, g.get("iwet"))
+    err = {f: np.abs(o.get(f) - g.get(f))[w > 0].max() for f in ("eta", "u", "v")}
+    print(name, err, flush=True)
+    assert max(err.values()) <= 1e-9, err
+    g.close()

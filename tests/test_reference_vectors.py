@@ -193,6 +193,9 @@ CASES["tidal_combo"] = _tidal_combo
 TEL = {
     "tel_band": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, band=((2, 10), (4, 8)), seed=1),
     "tel_patch": lambda: cases.tidal_tel(nx=40, ny=24, dx=20.0, seed=2, levels=cases.patch_levels(40, 24)),
+    # a flux string on the west edge of a telescoping grid, constant and by a rating curve
+    "tel_river": lambda: cases.river_tel(),
+    "tel_river_rating": lambda: cases.river_tel(q_rating=([1.0, 2.0, 2.5, 4.0], [10.0, 30.0, 45.0, 80.0])),
 }
 CASES.update(TEL)
 # flux (river) strings: bndriver_vel, bnd_vel_flux, bndflux and the flux
