@@ -53,6 +53,22 @@ def l05_case():
     return c
 
 
+def l05_tel_case():
+    """The same case on its own .tel grid (the reference reads it with
+    read_tel): the file's neighbour slots N1 N2 E1 E2 S1 S2 W1 W2 (columns 6-13)
+    and the two cellstrings as cell lists; every cell is its own base cell."""
+    c = l05_case()
+    rows = [ln.split() for ln in open(os.path.join(DIR, NAME + ".tel")).read().splitlines()[2:] if ln.strip()]
+    geom = np.array([[float(r[1]), float(r[2]), float(r[3]), float(r[4])] for r in rows])
+    nb = np.array([[int(v) for v in r[5:13]] for r in rows], dtype=np.int32)
+    n = len(rows)
+    tel = cases.TelGrid(geom, nb, np.arange(n, dtype=np.int32), np.zeros(n, dtype=np.int32),
+                        np.array([219, 439, 659], dtype=np.int32))
+    tel.flux_cells = np.array([0, 220, 440], dtype=np.int32)
+    c.tel = tel
+    return c
+
+
 def l05_baseline(field="wse"):
     """(times [hr], values [ntimes, 660]) of the reference's run: water level,
     or the velocity components."""

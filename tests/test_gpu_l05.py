@@ -42,3 +42,21 @@ def test_l05_matches_oracle_and_reference_run(gpu_available):
     print("against the oracle", worst, "against the reference run at 0.5 h: eta", de, "u", du, "v", dv, flush=True)
     assert de <= TOL_ETA_L05 and du <= TOL_VEL_L05 and dv <= TOL_VEL_L05
     g.close()
+
+
+def test_l05_on_its_telescoping_grid(gpu_available):
+    """The case on its .tel grid (flux and water-level cellstrings as the card
+    file names them): the same flow as the Cartesian 220 x 3 grid."""
+    from l05_case import l05_tel_case
+    ct, cc = l05_tel_case(), l05_case()
+    print("L05_Implicit_Test01 on its .tel grid, 30 steps", flush=True)
+    gt = FlowStep(ct, rtol=1e-13, atol=0.0)
+    gc = FlowStep(cc, rtol=1e-13, atol=0.0)
+    for _ in range(30):
+        gt.step(1)
+        gc.step(1)
+    d = max(np.abs(gt.get(f) - gc.get(f)).max() for f in ("eta", "u", "v"))
+    print("telescoping against Cartesian after 30 steps", d, flush=True)
+    assert d <= 1e-9, d
+    gt.close()
+    gc.close()
