@@ -224,7 +224,7 @@ int cms2d_flow_step(cms2d_flow *flow, int nsteps, cms2d_step_info *info);
  * evaporation) and the step's extremes over the wet cells (flow_step_stat,
  * source/flow/flow.F90:1912-2008), computed on the device after every step.
  * Volumes in m^3: storage = sum area (h - h1); efflux = dt times the outward
- * flux through the water-level string's faces; flux = dt times the sum of
+ * flux through the forcing strings' faces (water level and flux); flux = dt times the sum of
  * every cell's outward face fluxes; balance = storage + efflux; net =
  * storage + flux; error_pct = 100 balance / max(|storage| + |efflux|, 1e-10);
  * cum_error_pct = 100 cum_balance / max(the largest |cum_storage| so far,
